@@ -1,0 +1,182 @@
+"""Seeded synthetic scenes for the rigid-contact workloads.
+
+Every scene is built through :class:`ModelBuilder` from a ``numpy.random.Generator(PCG64(seed))``.
+
+* ``box_pile``      1 world, N boxes (edges U[0.1, 0.5] m per axis, density 1000 kg/m^3) at uniform
+                    positions in [-1, 1]^2 x [0.5, 2.5] m with uniform random orientations (Shoemake),
+                    friction 0.5 on every shape including the infinite ground plane.
+
+Positions are rejection-sampled so that no two bounding spheres overlap at t = 0 (a dropped pile
+starts separated; position-based contact resolves any initial overlap d by injecting a velocity of
+about d/dt, so overlapping starts are not a drop test).  When 2000 draws find no free spot the
+sampling box grows 0.25 m upward.
+* ``mixed_worlds``  W worlds sharing one set of 32 primitive sizes (11 spheres r U[0.05, 0.2], 11 boxes
+                    as above, 10 capsules r U[0.05, 0.15], full cylinder length h U[0.1, 0.4] so
+                    half_height = h/2), each world with its own seeded poses in the same volume, one
+                    global ground plane, friction 0.5.
+* ``uneven_worlds`` W worlds with log-uniform body counts in [lo, hi] (same primitive mix and
+                    distributions), optionally a revolute chain of capsule links per world.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._types import quat_from_axis_angle, quat_mul, transform
+from .builder import ModelBuilder
+
+
+def shoemake_quat(rng: np.random.Generator) -> np.ndarray:
+    """Uniform random unit quaternion (Shoemake 1992), (x, y, z, w)."""
+    u1, u2, u3 = rng.random(3)
+    a = math.sqrt(1.0 - u1)
+    b = math.sqrt(u1)
+    return np.array(
+        [a * math.sin(2 * math.pi * u2), a * math.cos(2 * math.pi * u2), b * math.sin(2 * math.pi * u3),
+         b * math.cos(2 * math.pi * u3)]
+    )
+
+
+class _Placer:
+    """Sequential rejection sampling of non-overlapping bounding spheres in [-1,1]^2 x [0.5, z_hi]."""
+
+    def __init__(self, rng: np.random.Generator, z_hi: float = 2.5):
+        self.rng = rng
+        self.z_hi = z_hi
+        self.pts: list[np.ndarray] = []
+        self.rad: list[float] = []
+
+    def place(self, radius: float) -> np.ndarray:
+        while True:
+            for _ in range(2000):
+                p = np.array([self.rng.uniform(-1, 1), self.rng.uniform(-1, 1), self.rng.uniform(0.5, self.z_hi)])
+                if all(np.linalg.norm(p - q) > radius + r for q, r in zip(self.pts, self.rad)):
+                    self.pts.append(p)
+                    self.rad.append(radius)
+                    return p
+            self.z_hi += 0.25
+
+
+def _bound_radius(kind, dims) -> float:
+    if kind == "sphere":
+        return float(dims[0])
+    if kind == "box":
+        return float(np.linalg.norm(dims))
+    return float(dims[0] + dims[1])
+
+
+def _cfg(mu: float) -> ModelBuilder.ShapeConfig:
+    c = ModelBuilder.ShapeConfig()
+    c.mu = mu
+    return c
+
+
+def box_pile(n_boxes: int = 64, seed: int = 0, mu: float = 0.5, device=None, world: bool = True):
+    """One world of ``n_boxes`` seeded boxes above an infinite ground plane."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = ModelBuilder()
+    b.add_ground_plane(cfg=_cfg(mu))
+    if world:
+        b.begin_world()
+    cfg = _cfg(mu)
+    placer = _Placer(rng)
+    for _ in range(n_boxes):
+        he = 0.5 * rng.uniform(0.1, 0.5, size=3)
+        p = placer.place(float(np.linalg.norm(he)))
+        q = shoemake_quat(rng)
+        body = b.add_body(xform=transform(p, q))
+        b.add_shape_box(body, hx=he[0], hy=he[1], hz=he[2], cfg=cfg)
+    if world:
+        b.end_world()
+    return b.finalize(device)
+
+
+def _primitive_sizes(rng, n_sphere, n_box, n_capsule):
+    sizes = []
+    for _ in range(n_sphere):
+        sizes.append(("sphere", (rng.uniform(0.05, 0.2),)))
+    for _ in range(n_box):
+        sizes.append(("box", tuple(0.5 * rng.uniform(0.1, 0.5, size=3))))
+    for _ in range(n_capsule):
+        sizes.append(("capsule", (rng.uniform(0.05, 0.15), 0.5 * rng.uniform(0.1, 0.4))))
+    return sizes
+
+
+def _add_primitive(b: ModelBuilder, kind, dims, p, q, cfg):
+    body = b.add_body(xform=transform(p, q))
+    if kind == "sphere":
+        b.add_shape_sphere(body, radius=dims[0], cfg=cfg)
+    elif kind == "box":
+        b.add_shape_box(body, hx=dims[0], hy=dims[1], hz=dims[2], cfg=cfg)
+    else:
+        b.add_shape_capsule(body, radius=dims[0], half_height=dims[1], cfg=cfg)
+    return body
+
+
+def mixed_worlds(world_count: int = 4096, seed: int = 0, mu: float = 0.5, device=None,
+                 n_sphere: int = 11, n_box: int = 11, n_capsule: int = 10):
+    """``world_count`` worlds of the same 32 primitives with per-world seeded poses."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    sizes = _primitive_sizes(rng, n_sphere, n_box, n_capsule)
+    b = ModelBuilder()
+    cfg = _cfg(mu)
+    b.add_ground_plane(cfg=_cfg(mu))
+    for _w in range(world_count):
+        b.begin_world()
+        placer = _Placer(rng)
+        for kind, dims in sizes:
+            p = placer.place(_bound_radius(kind, dims))
+            _add_primitive(b, kind, dims, p, shoemake_quat(rng), cfg)
+        b.end_world()
+    return b.finalize(device)
+
+
+def uneven_worlds(world_count: int = 2048, seed: int = 0, mu: float = 0.5, device=None, lo: int = 8,
+                  hi: int = 256, chain_links: int = 6, chain_limit: float = 0.5 * math.pi):
+    """Worlds with log-uniform body counts in [lo, hi] plus an optional revolute capsule chain."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = ModelBuilder()
+    cfg = _cfg(mu)
+    b.add_ground_plane(cfg=_cfg(mu))
+    for _w in range(world_count):
+        n = int(math.floor(math.exp(rng.uniform(math.log(lo), math.log(hi)))))
+        n = max(lo, min(hi, n))
+        ns = n // 3 + (1 if n % 3 > 0 else 0)
+        nb = n // 3 + (1 if n % 3 > 1 else 0)
+        nc = n - ns - nb
+        b.begin_world()
+        placer = _Placer(rng)
+        for kind, dims in _primitive_sizes(rng, ns, nb, nc):
+            p = placer.place(_bound_radius(kind, dims))
+            _add_primitive(b, kind, dims, p, shoemake_quat(rng), cfg)
+        if chain_links > 0:
+            _add_chain(b, rng, chain_links, chain_limit, cfg)
+        b.end_world()
+    return b.finalize(device)
+
+
+def _add_chain(b: ModelBuilder, rng, links: int, limit: float, cfg):
+    """A revolute chain of capsule links hanging from a world anchor above the pile."""
+    r, hh = 0.04, 0.1
+    base = np.array([rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), 3.0])
+    # links lie along +X; each capsule's local +Z axis is rotated onto X
+    q_link = quat_from_axis_angle((0.0, 1.0, 0.0), 0.5 * math.pi)
+    prev = -1
+    seg = 2.0 * (hh + r)
+    for i in range(links):
+        c = base + np.array([(i + 0.5) * seg, 0.0, 0.0])
+        body = b.add_link(xform=transform(c, (0.0, 0.0, 0.0, 1.0)))
+        b.add_shape_capsule(body, xform=transform((0.0, 0.0, 0.0), q_link), radius=r, half_height=hh, cfg=cfg)
+        parent_xform = transform(base if prev < 0 else (0.5 * seg, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0))
+        j = b.add_joint_revolute(prev, body, parent_xform=parent_xform,
+                                 child_xform=transform((-0.5 * seg, 0.0, 0.0), (0.0, 0.0, 0.0, 1.0)),
+                                 axis=(0.0, 1.0, 0.0), limit_lower=-limit, limit_upper=limit)
+        if i == 0:
+            b.add_articulation([j])
+        prev = body
+    return prev
+
+
+__all__ = ["box_pile", "mixed_worlds", "uneven_worlds", "shoemake_quat", "quat_mul"]

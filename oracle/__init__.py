@@ -1,0 +1,213 @@
+"""CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of ``liboracle.so``, the scalar C++ restatement of Newton's rigid-contact step
+(see ``oracle.h`` for the per-function reference citations).  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` import this module; the product
+package never does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+_i32p = ctypes.POINTER(ctypes.c_int)
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class OracleModel(ctypes.Structure):
+    _fields_ = [
+        ("body_count", ctypes.c_int), ("shape_count", ctypes.c_int), ("joint_count", ctypes.c_int),
+        ("world_count", ctypes.c_int),
+        ("body_com", _f32p), ("body_inertia", _f32p), ("body_inv_mass", _f32p), ("body_inv_inertia", _f32p),
+        ("body_flags", _i32p), ("body_world", _i32p),
+        ("shape_transform", _f32p), ("shape_body", _i32p), ("shape_type", _i32p), ("shape_scale", _f32p),
+        ("shape_margin", _f32p), ("shape_gap", _f32p), ("shape_radius", _f32p), ("shape_flags", _i32p),
+        ("shape_group", _i32p), ("shape_world", _i32p), ("shape_mu", _f32p), ("shape_mu_torsional", _f32p),
+        ("shape_mu_rolling", _f32p),
+        ("segment_count", ctypes.c_int), ("segment_start", _i32p), ("segment_shape", _i32p),
+        ("filter_count", ctypes.c_int), ("filter_pairs", _i32p),
+        ("gravity_count", ctypes.c_int), ("gravity", _f32p),
+        ("joint_type", _i32p), ("joint_enabled", _i32p), ("joint_parent", _i32p), ("joint_child", _i32p),
+        ("joint_X_p", _f32p), ("joint_X_c", _f32p), ("joint_qd_start", _i32p), ("joint_dof_dim", _i32p),
+        ("joint_axis", _f32p), ("joint_limit_lower", _f32p), ("joint_limit_upper", _f32p),
+        ("joint_target_ke", _f32p), ("joint_target_kd", _f32p),
+    ]
+
+
+class OracleXpbdParams(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int), ("rigid_contact_relaxation", ctypes.c_float),
+        ("rigid_contact_con_weighting", ctypes.c_int), ("angular_damping", ctypes.c_float),
+        ("joint_linear_relaxation", ctypes.c_float), ("joint_angular_relaxation", ctypes.c_float),
+        ("joint_linear_compliance", ctypes.c_float), ("joint_angular_compliance", ctypes.c_float),
+    ]
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so with the recipe in oracle/Makefile."""
+    so = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = build()
+        L = ctypes.CDLL(so)
+        L.oracle_broadphase_f32.restype = ctypes.c_int
+        L.oracle_collide_f32.restype = ctypes.c_int
+        L.oracle_collide_f64.restype = ctypes.c_int
+        L.oracle_xpbd_step_f32.restype = None
+        L.oracle_xpbd_step_f64.restype = None
+        _LIB = L
+    return _LIB
+
+
+def _p(a: np.ndarray, t):
+    return a.ctypes.data_as(t)
+
+
+class Oracle:
+    """Holds a model's host arrays (kept alive) and runs the restatement on them."""
+
+    def __init__(self, model):
+        h = model.host
+        self.model = model
+        keep = {}
+
+        def f32(name):
+            a = np.ascontiguousarray(h[name], dtype=np.float32)
+            keep[name] = a
+            return _p(a, _f32p)
+
+        def i32(name):
+            a = np.ascontiguousarray(h[name], dtype=np.int32)
+            keep[name] = a
+            return _p(a, _i32p)
+
+        m = OracleModel()
+        m.body_count = model.body_count
+        m.shape_count = model.shape_count
+        m.joint_count = model.joint_count
+        m.world_count = model.world_count
+        m.body_com, m.body_inertia = f32("body_com"), f32("body_inertia")
+        m.body_inv_mass, m.body_inv_inertia = f32("body_inv_mass"), f32("body_inv_inertia")
+        m.body_flags, m.body_world = i32("body_flags"), i32("body_world")
+        m.shape_transform, m.shape_body, m.shape_type = f32("shape_transform"), i32("shape_body"), i32("shape_type")
+        m.shape_scale, m.shape_margin, m.shape_gap = f32("shape_scale"), f32("shape_margin"), f32("shape_gap")
+        m.shape_radius, m.shape_flags = f32("shape_collision_radius"), i32("shape_flags")
+        m.shape_group, m.shape_world = i32("shape_collision_group"), i32("shape_world")
+        m.shape_mu, m.shape_mu_torsional = f32("shape_material_mu"), f32("shape_material_mu_torsional")
+        m.shape_mu_rolling = f32("shape_material_mu_rolling")
+        m.segment_count = len(h["segment_start"]) - 1
+        m.segment_start, m.segment_shape = i32("segment_start"), i32("segment_shape")
+        m.filter_count = len(h["shape_filter_pairs"])
+        m.filter_pairs = i32("shape_filter_pairs")
+        m.gravity_count = len(h["gravity"])
+        m.gravity = f32("gravity")
+        m.joint_type, m.joint_enabled = i32("joint_type"), i32("joint_enabled")
+        m.joint_parent, m.joint_child = i32("joint_parent"), i32("joint_child")
+        m.joint_X_p, m.joint_X_c = f32("joint_X_p"), f32("joint_X_c")
+        m.joint_qd_start, m.joint_dof_dim = i32("joint_qd_start"), i32("joint_dof_dim")
+        m.joint_axis = f32("joint_axis")
+        m.joint_limit_lower, m.joint_limit_upper = f32("joint_limit_lower"), f32("joint_limit_upper")
+        m.joint_target_ke, m.joint_target_kd = f32("joint_target_ke"), f32("joint_target_kd")
+        self._keep = keep
+        self.m = m
+        self.lib = lib()
+
+    def broadphase(self, body_q: np.ndarray, capacity: int | None = None) -> np.ndarray:
+        q = np.ascontiguousarray(body_q, dtype=np.float32)
+        cap = capacity if capacity is not None else max(1, self.model.shape_count * 64)
+        out = np.zeros((cap, 2), dtype=np.int32)
+        n = self.lib.oracle_broadphase_f32(ctypes.byref(self.m), _p(q, _f32p), cap, _p(out, _i32p))
+        return out[: min(n, cap)]
+
+    def collide(self, body_q: np.ndarray, capacity: int | None = None, precision: str = "f32") -> dict:
+        dt = np.float32 if precision == "f32" else np.float64
+        tp = _f32p if precision == "f32" else _f64p
+        q = np.ascontiguousarray(body_q, dtype=dt)
+        cap = capacity if capacity is not None else max(1000, self.model.shape_count * 40)
+        out = {
+            "shape0": np.zeros(cap, np.int32), "shape1": np.zeros(cap, np.int32),
+            "point0": np.zeros((cap, 3), dt), "point1": np.zeros((cap, 3), dt), "offset0": np.zeros((cap, 3), dt),
+            "offset1": np.zeros((cap, 3), dt), "normal": np.zeros((cap, 3), dt), "margin0": np.zeros(cap, dt),
+            "margin1": np.zeros(cap, dt), "sort_key": np.zeros(cap, np.int64),
+        }
+        fn = self.lib.oracle_collide_f32 if precision == "f32" else self.lib.oracle_collide_f64
+        n = fn(ctypes.byref(self.m), _p(q, tp), cap, _p(out["shape0"], _i32p), _p(out["shape1"], _i32p),
+               _p(out["point0"], tp), _p(out["point1"], tp), _p(out["offset0"], tp), _p(out["offset1"], tp),
+               _p(out["normal"], tp), _p(out["margin0"], tp), _p(out["margin1"], tp), _p(out["sort_key"], _i64p))
+        k = min(n, cap)
+        res = {key: v[:k] for key, v in out.items()}
+        res["count"] = n
+        return res
+
+    def step(self, body_q, body_qd, body_f, contacts: dict | None, dt: float, iterations: int = 2,
+             rigid_contact_relaxation: float = 0.8, rigid_contact_con_weighting: bool = True,
+             angular_damping: float = 0.0, joint_linear_relaxation: float = 0.7,
+             joint_angular_relaxation: float = 0.4, joint_linear_compliance: float = 0.0,
+             joint_angular_compliance: float = 0.0, joint_f=None, joint_target_pos=None, joint_target_vel=None,
+             precision: str = "f32"):
+        """One SolverXPBD.step; returns (body_q_out, body_qd_out)."""
+        h = self.model.host
+        dtp = np.float32 if precision == "f32" else np.float64
+        tp = _f32p if precision == "f32" else _f64p
+        p = OracleXpbdParams(iterations, rigid_contact_relaxation, int(rigid_contact_con_weighting), angular_damping,
+                             joint_linear_relaxation, joint_angular_relaxation, joint_linear_compliance,
+                             joint_angular_compliance)
+        q = np.ascontiguousarray(body_q, dtype=dtp)
+        qd = np.ascontiguousarray(body_qd, dtype=dtp)
+        f = np.ascontiguousarray(body_f, dtype=dtp)
+        jf = np.ascontiguousarray(h["joint_f"] if joint_f is None else joint_f, dtype=np.float32)
+        tpos = np.ascontiguousarray(h["joint_target_pos"] if joint_target_pos is None else joint_target_pos,
+                                    dtype=np.float32)
+        tvel = np.ascontiguousarray(h["joint_target_vel"] if joint_target_vel is None else joint_target_vel,
+                                    dtype=np.float32)
+        if jf.size == 0:
+            jf = np.zeros(1, np.float32)
+        if tpos.size == 0:
+            tpos = np.zeros(1, np.float32)
+        if tvel.size == 0:
+            tvel = np.zeros(1, np.float32)
+        if contacts is None:
+            nc = -1
+            c = {k: np.zeros((1, 3), dtp) for k in ("point0", "point1", "offset0", "offset1", "normal")}
+            c.update({"shape0": np.zeros(1, np.int32), "shape1": np.zeros(1, np.int32),
+                      "margin0": np.zeros(1, dtp), "margin1": np.zeros(1, dtp)})
+        else:
+            nc = len(contacts["shape0"])
+            c = {}
+            for k in ("point0", "point1", "offset0", "offset1", "normal"):
+                c[k] = np.ascontiguousarray(contacts[k], dtype=dtp).reshape(-1, 3)
+                if c[k].size == 0:
+                    c[k] = np.zeros((1, 3), dtp)
+            for k in ("margin0", "margin1"):
+                c[k] = np.ascontiguousarray(contacts[k], dtype=dtp)
+                if c[k].size == 0:
+                    c[k] = np.zeros(1, dtp)
+            for k in ("shape0", "shape1"):
+                c[k] = np.ascontiguousarray(contacts[k], dtype=np.int32)
+                if c[k].size == 0:
+                    c[k] = np.zeros(1, np.int32)
+        qo = np.zeros_like(q)
+        qdo = np.zeros_like(qd)
+        fn = self.lib.oracle_xpbd_step_f32 if precision == "f32" else self.lib.oracle_xpbd_step_f64
+        dtc = ctypes.c_float(dt) if precision == "f32" else ctypes.c_double(dt)
+        fn(ctypes.byref(self.m), ctypes.byref(p), _p(q, tp), _p(qd, tp), _p(f, tp), _p(jf, _f32p), _p(tpos, _f32p),
+           _p(tvel, _f32p), nc, _p(c["shape0"], _i32p), _p(c["shape1"], _i32p), _p(c["point0"], tp),
+           _p(c["point1"], tp), _p(c["offset0"], tp), _p(c["offset1"], tp), _p(c["normal"], tp), _p(c["margin0"], tp),
+           _p(c["margin1"], tp), dtc, _p(qo, tp), _p(qdo, tp))
+        return qo, qdo

@@ -1,0 +1,121 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY (tests/, __graft_entry__.smoke(), bench.py cpu_baseline).
+// extern "C" entry points of the CPU restatement declared in oracle.h.
+#include "oracle.h"
+
+#include <vector>
+
+#include "oracle_collide.h"
+#include "oracle_xpbd.h"
+
+using namespace oracle;
+
+template <class T, class S>
+static int collide_impl(const OracleModel* m, const S* body_q, int capacity, int* shape0, int* shape1, S* point0,
+                        S* point1, S* offset0, S* offset1, S* normal, S* margin0, S* margin1, int64_t* sort_key) {
+  std::vector<Contact<T>> out;
+  int n = collide<T, S>(m, body_q, out);
+  int w = n < capacity ? n : capacity;
+  for (int i = 0; i < w; ++i) {
+    const Contact<T>& c = out[i];
+    shape0[i] = c.shape0;
+    shape1[i] = c.shape1;
+    const V3<T>* vs[5] = {&c.point0, &c.point1, &c.offset0, &c.offset1, &c.normal};
+    S* dst[5] = {point0, point1, offset0, offset1, normal};
+    for (int k = 0; k < 5; ++k) {
+      dst[k][3 * i] = S(vs[k]->x);
+      dst[k][3 * i + 1] = S(vs[k]->y);
+      dst[k][3 * i + 2] = S(vs[k]->z);
+    }
+    margin0[i] = S(c.margin0);
+    margin1[i] = S(c.margin1);
+    if (sort_key) sort_key[i] = c.key;
+  }
+  return n;
+}
+
+extern "C" int oracle_broadphase_f32(const OracleModel* m, const float* body_q, int capacity, int* pairs_out) {
+  std::vector<ShapeWorld<float>> sw(m->shape_count);
+  for (int s = 0; s < m->shape_count; ++s) sw[s] = compute_shape_world<float, float>(m, body_q, s);
+  std::vector<std::pair<int, int>> pairs;
+  sap_pairs(m, sw, pairs);
+  int n = int(pairs.size());
+  for (int i = 0; i < n && i < capacity; ++i) {
+    pairs_out[2 * i] = pairs[i].first;
+    pairs_out[2 * i + 1] = pairs[i].second;
+  }
+  return n;
+}
+
+extern "C" int oracle_collide_f32(const OracleModel* m, const float* body_q, int capacity, int* shape0, int* shape1,
+                                  float* point0, float* point1, float* offset0, float* offset1, float* normal,
+                                  float* margin0, float* margin1, int64_t* sort_key) {
+  return collide_impl<float, float>(m, body_q, capacity, shape0, shape1, point0, point1, offset0, offset1, normal,
+                                    margin0, margin1, sort_key);
+}
+
+extern "C" int oracle_collide_f64(const OracleModel* m, const double* body_q, int capacity, int* shape0, int* shape1,
+                                  double* point0, double* point1, double* offset0, double* offset1, double* normal,
+                                  double* margin0, double* margin1, int64_t* sort_key) {
+  return collide_impl<double, double>(m, body_q, capacity, shape0, shape1, point0, point1, offset0, offset1, normal,
+                                      margin0, margin1, sort_key);
+}
+
+template <class T, class S>
+static void step_impl(const OracleModel* m, const OracleXpbdParams* p, const S* q_in, const S* qd_in, const S* f_in,
+                      const float* joint_f, const float* tpos, const float* tvel, int nc, const int* shape0,
+                      const int* shape1, const S* point0, const S* point1, const S* offset0, const S* offset1,
+                      const S* normal, const S* margin0, const S* margin1, S dt, S* q_out, S* qd_out) {
+  int B = m->body_count;
+  std::vector<Body<T>> state(B);
+  std::vector<Spatial<T>> f(B);
+  for (int b = 0; b < B; ++b) {
+    state[b].q = load_xf_t<T, S>(q_in + 7 * b);
+    state[b].v = V3<T>{T(qd_in[6 * b]), T(qd_in[6 * b + 1]), T(qd_in[6 * b + 2])};
+    state[b].w = V3<T>{T(qd_in[6 * b + 3]), T(qd_in[6 * b + 4]), T(qd_in[6 * b + 5])};
+    f[b].lin = V3<T>{T(f_in[6 * b]), T(f_in[6 * b + 1]), T(f_in[6 * b + 2])};
+    f[b].ang = V3<T>{T(f_in[6 * b + 3]), T(f_in[6 * b + 4]), T(f_in[6 * b + 5])};
+  }
+  std::vector<ContactIn<T>> cs(nc > 0 ? nc : 0);
+  auto ld = [](const S* a, int i) { return V3<T>{T(a[3 * i]), T(a[3 * i + 1]), T(a[3 * i + 2])}; };
+  for (int i = 0; i < nc; ++i) {
+    cs[i].shape0 = shape0[i];
+    cs[i].shape1 = shape1[i];
+    cs[i].p0 = ld(point0, i);
+    cs[i].p1 = ld(point1, i);
+    cs[i].o0 = ld(offset0, i);
+    cs[i].o1 = ld(offset1, i);
+    cs[i].n = ld(normal, i);
+    cs[i].m0 = T(margin0[i]);
+    cs[i].m1 = T(margin1[i]);
+  }
+  xpbd_step<T>(m, p, state, f, joint_f, tpos, tvel, cs, nc >= 0, T(dt));
+  for (int b = 0; b < B; ++b) {
+    const Body<T>& s = state[b];
+    S* q = q_out + 7 * b;
+    q[0] = S(s.q.p.x); q[1] = S(s.q.p.y); q[2] = S(s.q.p.z);
+    q[3] = S(s.q.q.x); q[4] = S(s.q.q.y); q[5] = S(s.q.q.z); q[6] = S(s.q.q.w);
+    S* qd = qd_out + 6 * b;
+    qd[0] = S(s.v.x); qd[1] = S(s.v.y); qd[2] = S(s.v.z);
+    qd[3] = S(s.w.x); qd[4] = S(s.w.y); qd[5] = S(s.w.z);
+  }
+}
+
+extern "C" void oracle_xpbd_step_f32(const OracleModel* m, const OracleXpbdParams* p, const float* q_in,
+                                     const float* qd_in, const float* f_in, const float* joint_f, const float* tpos,
+                                     const float* tvel, int nc, const int* shape0, const int* shape1,
+                                     const float* point0, const float* point1, const float* offset0,
+                                     const float* offset1, const float* normal, const float* margin0,
+                                     const float* margin1, float dt, float* q_out, float* qd_out) {
+  step_impl<float, float>(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, nc, shape0, shape1, point0, point1, offset0,
+                          offset1, normal, margin0, margin1, dt, q_out, qd_out);
+}
+
+extern "C" void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const double* q_in,
+                                     const double* qd_in, const double* f_in, const float* joint_f, const float* tpos,
+                                     const float* tvel, int nc, const int* shape0, const int* shape1,
+                                     const double* point0, const double* point1, const double* offset0,
+                                     const double* offset1, const double* normal, const double* margin0,
+                                     const double* margin1, double dt, double* q_out, double* qd_out) {
+  step_impl<double, double>(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, nc, shape0, shape1, point0, point1,
+                            offset0, offset1, normal, margin0, margin1, dt, q_out, qd_out);
+}

@@ -1,0 +1,108 @@
+/* CPU oracle -- TEST INFRASTRUCTURE ONLY (tests/, __graft_entry__.smoke(), bench.py cpu_baseline).
+ *
+ * C interface of the scalar CPU restatement of Newton's rigid-contact step
+ * (newton/_src/sim/collide.py, geometry/{broad_phase_sap,narrow_phase,collision_*,mpr,simplex_solver,
+ * multicontact}.py, solvers/solver.py, solvers/xpbd/{kernels,solver_xpbd}.py).
+ * Built by oracle/Makefile into oracle/liboracle.so; the product library never links it.
+ */
+#ifndef NMC_ORACLE_H
+#define NMC_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OracleModel {
+  int body_count, shape_count, joint_count, world_count;
+  /* bodies */
+  const float* body_com;         /* B x 3 */
+  const float* body_inertia;     /* B x 9 */
+  const float* body_inv_mass;    /* B */
+  const float* body_inv_inertia; /* B x 9 */
+  const int* body_flags;         /* B */
+  const int* body_world;         /* B */
+  /* shapes */
+  const float* shape_transform; /* S x 7 */
+  const int* shape_body;
+  const int* shape_type;
+  const float* shape_scale; /* S x 3 */
+  const float* shape_margin;
+  const float* shape_gap;
+  const float* shape_radius;
+  const int* shape_flags;
+  const int* shape_group;
+  const int* shape_world;
+  const float* shape_mu;
+  const float* shape_mu_torsional;
+  const float* shape_mu_rolling;
+  /* broad-phase segments (world shapes + global shapes; last segment = global only) */
+  int segment_count;
+  const int* segment_start; /* segment_count + 1 */
+  const int* segment_shape;
+  /* sorted canonical excluded pairs */
+  int filter_count;
+  const int* filter_pairs; /* filter_count x 2 */
+  /* gravity per world */
+  int gravity_count;
+  const float* gravity; /* gravity_count x 3 */
+  /* joints */
+  const int* joint_type;
+  const int* joint_enabled;
+  const int* joint_parent;
+  const int* joint_child;
+  const float* joint_X_p; /* J x 7 */
+  const float* joint_X_c; /* J x 7 */
+  const int* joint_qd_start;
+  const int* joint_dof_dim; /* J x 2 */
+  const float* joint_axis;  /* D x 3 */
+  const float* joint_limit_lower;
+  const float* joint_limit_upper;
+  const float* joint_target_ke;
+  const float* joint_target_kd;
+} OracleModel;
+
+typedef struct OracleXpbdParams {
+  int iterations;
+  float rigid_contact_relaxation;
+  int rigid_contact_con_weighting;
+  float angular_damping;
+  float joint_linear_relaxation;
+  float joint_angular_relaxation;
+  float joint_linear_compliance;
+  float joint_angular_compliance;
+} OracleXpbdParams;
+
+/* Broad phase only: writes canonical (min,max) pairs in discovery order, returns the pair count
+ * (may exceed capacity; pairs past capacity are dropped).  body_q is B x 7. */
+int oracle_broadphase_f32(const OracleModel* m, const float* body_q, int capacity, int* pairs_out);
+
+/* Full collide: contacts sorted by the 64-bit key (shape_a<<43 | shape_b<<23 | sub_key).
+ * Outputs per contact: shape0, shape1 (int), point0, point1, offset0, offset1, normal (x3),
+ * margin0, margin1, sort_key (int64).  Returns the contact count (may exceed capacity). */
+int oracle_collide_f32(const OracleModel* m, const float* body_q, int capacity, int* shape0, int* shape1,
+                       float* point0, float* point1, float* offset0, float* offset1, float* normal, float* margin0,
+                       float* margin1, int64_t* sort_key);
+int oracle_collide_f64(const OracleModel* m, const double* body_q, int capacity, int* shape0, int* shape1,
+                       double* point0, double* point1, double* offset0, double* offset1, double* normal,
+                       double* margin0, double* margin1, int64_t* sort_key);
+
+/* One SolverXPBD.step(state_in, state_out, control, contacts, dt) over contacts given in order. */
+void oracle_xpbd_step_f32(const OracleModel* m, const OracleXpbdParams* p, const float* body_q_in,
+                          const float* body_qd_in, const float* body_f_in, const float* joint_f,
+                          const float* joint_target_pos, const float* joint_target_vel, int contact_count,
+                          const int* shape0, const int* shape1, const float* point0, const float* point1,
+                          const float* offset0, const float* offset1, const float* normal, const float* margin0,
+                          const float* margin1, float dt, float* body_q_out, float* body_qd_out);
+void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const double* body_q_in,
+                          const double* body_qd_in, const double* body_f_in, const float* joint_f,
+                          const float* joint_target_pos, const float* joint_target_vel, int contact_count,
+                          const int* shape0, const int* shape1, const double* point0, const double* point1,
+                          const double* offset0, const double* offset1, const double* normal,
+                          const double* margin0, const double* margin1, double dt, double* body_q_out,
+                          double* body_qd_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
