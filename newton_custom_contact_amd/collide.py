@@ -1,0 +1,158 @@
+"""``CollisionPipeline``: AABBs -> SAP broad phase -> narrow phase -> Contacts, on the GPU.
+
+Mirrors newton/_src/sim/collide.py (class ``CollisionPipeline``, line 464): constructor arguments
+(``collide.py:483-504``), ``contacts()`` (``:858-885``), ``collide(state, contacts)`` (``:902``) and the
+capacity estimators (``_estimate_rigid_contact_max`` ``:278-376``, ``_compute_per_world_shape_pairs_max``
+``:379-418``, ``_resolve_shape_pairs_max`` ``:421-439``).
+
+``collide`` launches three kernels from libnmc.so on the current torch stream and never synchronises the
+host, so it can be captured in a graph.  The broad phase always runs per-world sweep-and-prune; the
+``broad_phase`` argument accepts the reference's names and ``"nxn"``/``"explicit"`` produce the same pair
+set (every one of them reports the AABB-overlapping, filter-passing pairs).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from ._native import NativeModel, check, load, ptr, stream_handle
+from ._types import GeoType, ShapeFlags
+from .contacts import Contacts
+
+_SUPPORTED = {int(GeoType.PLANE), int(GeoType.SPHERE), int(GeoType.CAPSULE), int(GeoType.BOX)}
+
+
+def estimate_rigid_contact_max(model) -> int:
+    """collide.py:278-376 (neighbour-budget estimate, per-world plane term)."""
+    h = model.host
+    st = h["shape_type"]
+    PRIMITIVE_CPP, MESH_CPP, MAX_NEIGHBORS = 5, 40, 20
+    mesh_mask = (st == int(GeoType.MESH)) | (st == int(GeoType.HFIELD))
+    plane_mask = st == int(GeoType.PLANE)
+    non_plane = ~plane_mask
+    n_mesh = int(np.count_nonzero(mesh_mask))
+    n_np = int(np.count_nonzero(non_plane))
+    n_prim = n_np - n_mesh
+    n_planes = int(np.count_nonzero(plane_mask))
+    non_plane_contacts = (n_prim * MAX_NEIGHBORS * PRIMITIVE_CPP + n_mesh * MAX_NEIGHBORS * MESH_CPP) // 2
+    avg_cpp = (n_prim * PRIMITIVE_CPP + n_mesh * MESH_CPP) // max(n_np, 1) if n_np > 0 else 0
+    plane_contacts = 0
+    if n_planes > 0 and n_np > 0:
+        sw = h["shape_world"]
+        W = model.world_count
+        if W > 0:
+            gmask = sw == -1
+            lmask = ~gmask
+            g_planes = int(np.count_nonzero(gmask & plane_mask))
+            g_np = int(np.count_nonzero(gmask & non_plane))
+            lp = np.bincount(sw[lmask & plane_mask], minlength=W)[:W]
+            lnp = np.bincount(sw[lmask & non_plane], minlength=W)[:W]
+            ppw = lp + g_planes
+            npw = lnp + g_np
+            ppc = int(np.sum(ppw.astype(np.int64) * npw.astype(np.int64)))
+            if W > 1:
+                ppc -= (W - 1) * g_planes * g_np
+            plane_contacts = ppc * avg_cpp
+        else:
+            plane_contacts = n_planes * (n_prim * PRIMITIVE_CPP + n_mesh * MESH_CPP)
+    return max(1000, non_plane_contacts + plane_contacts)
+
+
+def per_world_shape_pairs_max(model) -> int:
+    """collide.py:379-418: sum of n(n-1)/2 over the broad-phase segments."""
+    seg = np.diff(model.host["segment_start"]).astype(np.int64)
+    return int(np.sum(seg * (seg - 1) // 2))
+
+
+class CollisionPipeline:
+    """GPU collision pipeline with the reference's constructor and ``collide`` API."""
+
+    def __init__(self, model, *, reduce_contacts: bool = True, rigid_contact_max: int | None = None,
+                 max_triangle_pairs: int = 1000000, shape_pairs_filtered=None, soft_contact_max: int | None = None,
+                 soft_contact_margin: float = 0.01, requires_grad: bool | None = None, broad_phase=None,
+                 narrow_phase=None, sdf_hydroelastic_config=None, shape_pairs_max: int | None = None,
+                 deterministic: bool = False, contact_matching: str = "disabled",
+                 contact_matching_pos_threshold: float = 0.0005, contact_matching_normal_dot_threshold: float = 0.995,
+                 contact_report: bool = False, verify_buffers: bool = True):
+        if contact_matching not in ("disabled", "latest", "sticky"):
+            raise ValueError(f"contact_matching must be one of 'disabled', 'latest', 'sticky', got {contact_matching!r}")
+        if contact_matching != "disabled":
+            raise NotImplementedError("contact matching is not part of this build yet")
+        if contact_report:
+            raise ValueError('contact_report=True requires contact_matching != "disabled"')
+        mode = "explicit" if broad_phase is None else str(broad_phase).lower()
+        if mode not in ("nxn", "sap", "explicit"):
+            raise ValueError(f"Unsupported broad phase mode: {broad_phase!r}")
+        if narrow_phase is not None:
+            raise ValueError("expert narrow_phase objects are not supported")
+        types = set(np.unique(model.host["shape_type"]).tolist())
+        bad = types - _SUPPORTED
+        if bad:
+            names = ", ".join(GeoType(t).name for t in sorted(bad))
+            raise NotImplementedError(f"shape types not supported by the GPU narrow phase: {names}")
+        self.model = model
+        self.broad_phase_mode = mode
+        self.device = model.device
+        self.deterministic = bool(deterministic)
+        self.verify_buffers = verify_buffers
+        self.reduce_contacts = reduce_contacts
+        self.requires_grad = bool(model.requires_grad if requires_grad is None else requires_grad)
+        self.soft_contact_margin = soft_contact_margin
+        if rigid_contact_max is None:
+            rigid_contact_max = int(getattr(model, "rigid_contact_max", 0) or 0) or estimate_rigid_contact_max(model)
+        self._rigid_contact_max = int(rigid_contact_max)
+        model.rigid_contact_max = self._rigid_contact_max
+        if shape_pairs_max is None:
+            shape_pairs_max = per_world_shape_pairs_max(model)
+        elif shape_pairs_max <= 0:
+            raise ValueError(f"shape_pairs_max must be a positive integer or None, got {shape_pairs_max}")
+        self.shape_pairs_max = int(shape_pairs_max)
+        self._soft_contact_max = int(soft_contact_max) if soft_contact_max is not None else 0
+        colliding = (model.host["shape_flags"] & int(ShapeFlags.COLLIDE_SHAPES)) != 0
+        self.shape_count = model.shape_count
+        self._colliding = int(np.count_nonzero(colliding))
+        self.native = NativeModel(model, self._rigid_contact_max, self.shape_pairs_max)
+        self._lib = load()
+
+    @property
+    def rigid_contact_max(self) -> int:
+        return self._rigid_contact_max
+
+    @property
+    def soft_contact_max(self) -> int:
+        return self._soft_contact_max
+
+    def contacts(self) -> Contacts:
+        """collide.py:858-885."""
+        return Contacts(self.rigid_contact_max, self.soft_contact_max, requires_grad=self.requires_grad,
+                        device=self.device, segment_count=len(self.model.host["segment_start"]) - 1,
+                        deterministic=self.deterministic,
+                        requested_attributes=self.model.get_requested_contact_attributes())
+
+    def collide(self, state, contacts: Contacts, *, soft_contact_margin: float | None = None):
+        """collide.py:902-1230 (rigid part): writes ``contacts`` from ``state.body_q``."""
+        if contacts.rigid_contact_max != self.rigid_contact_max:
+            raise ValueError(
+                f"Contacts buffer capacity ({contacts.rigid_contact_max}) does not match the pipeline "
+                f"({self.rigid_contact_max}); use CollisionPipeline.contacts()"
+            )
+        if self.deterministic and contacts.rigid_contact_sort_key is None:
+            raise ValueError("deterministic pipeline needs a Contacts buffer with sort keys; use pipeline.contacts()")
+        q = state.body_q
+        if q is None or q.device != self.device or not q.is_contiguous():
+            raise ValueError("state.body_q must be a contiguous tensor on the model device")
+        rc = self._lib.nmc_collide(self.native.handle, ctypes.c_void_p(ptr(q)), ctypes.byref(contacts.native_view()),
+                                   ctypes.c_void_p(stream_handle(self.device)))
+        check(rc, "nmc_collide")
+        contacts.contact_generation += 1
+
+    def overflow_counters(self) -> dict:
+        """Overflow counters of every collide so far (verify_narrow_phase_buffers, narrow_phase.py:1334)."""
+        c = self.native.counters()
+        return {"pairs_dropped": int(c[0]), "staged_contacts_dropped": int(c[1]), "contacts_over_capacity": int(c[2]),
+                "solver_ordering_overflow": int(c[3])}
+
+
+__all__ = ["CollisionPipeline", "estimate_rigid_contact_max", "per_world_shape_pairs_max"]

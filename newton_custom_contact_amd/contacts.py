@@ -1,0 +1,100 @@
+"""``Contacts``: rigid contact buffers (newton/_src/sim/contacts.py:43-358, rigid part).
+
+Field names, dtypes and shapes follow the reference.  Contacts are stored in canonical order:
+segment-major (world 0, world 1, ..., then the global-only segment) and ascending by the 64-bit sort key
+``shape0 << 43 | shape1 << 23 | sub_key`` inside each segment.  Two extra per-segment arrays,
+``world_contact_start`` / ``world_contact_count``, index that layout for the per-world solver.
+``rigid_contact_tids`` is zero, as the reference's writer always stores (collide.py:134).
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._native import NmcContacts, ptr
+
+
+class Contacts:
+    EXTENDED_ATTRIBUTES: frozenset[str] = frozenset(("force",))
+
+    def __init__(self, rigid_contact_max: int, soft_contact_max: int = 0, requires_grad: bool = False,
+                 device=None, segment_count: int = 1, deterministic: bool = False,
+                 requested_attributes: set[str] | None = None):
+        dev = torch.device(device) if device is not None else torch.device("cuda:0")
+        C = int(rigid_contact_max)
+        self.device = dev
+        self.rigid_contact_max = C
+        self.soft_contact_max = int(soft_contact_max)
+        self.requires_grad = requires_grad
+        self.clear_buffers = False
+        f = dict(device=dev, dtype=torch.float32)
+        i = dict(device=dev, dtype=torch.int32)
+        self.contact_counters = torch.zeros(2, **i)
+        self.rigid_contact_count = self.contact_counters[0:1]
+        self.soft_contact_count = self.contact_counters[1:2]
+        self.contact_generation = torch.zeros(1, **i)
+        self.rigid_contact_shape0 = torch.full((C,), -1, **i)
+        self.rigid_contact_shape1 = torch.full((C,), -1, **i)
+        self.rigid_contact_point0 = torch.zeros((C, 3), **f)
+        self.rigid_contact_point1 = torch.zeros((C, 3), **f)
+        self.rigid_contact_offset0 = torch.zeros((C, 3), **f)
+        self.rigid_contact_offset1 = torch.zeros((C, 3), **f)
+        self.rigid_contact_normal = torch.zeros((C, 3), **f)
+        self.rigid_contact_margin0 = torch.zeros((C,), **f)
+        self.rigid_contact_margin1 = torch.zeros((C,), **f)
+        self.rigid_contact_tids = torch.zeros((C,), **i)
+        self.rigid_contact_sort_key = torch.zeros((C,), device=dev, dtype=torch.int64) if deterministic else None
+        self.rigid_contact_match_index = None
+        self.world_contact_start = torch.zeros((segment_count,), **i)
+        self.world_contact_count = torch.zeros((segment_count,), **i)
+        attrs = requested_attributes or set()
+        self.force = torch.zeros((C, 6), **f) if "force" in attrs else None
+        self._view = None
+
+    def native_view(self) -> NmcContacts:
+        if self._view is None:
+            v = NmcContacts()
+            v.capacity = self.rigid_contact_max
+            v.count = ptr(self.rigid_contact_count)
+            v.shape0, v.shape1 = ptr(self.rigid_contact_shape0), ptr(self.rigid_contact_shape1)
+            v.point0, v.point1 = ptr(self.rigid_contact_point0), ptr(self.rigid_contact_point1)
+            v.offset0, v.offset1 = ptr(self.rigid_contact_offset0), ptr(self.rigid_contact_offset1)
+            v.normal = ptr(self.rigid_contact_normal)
+            v.margin0, v.margin1 = ptr(self.rigid_contact_margin0), ptr(self.rigid_contact_margin1)
+            v.sort_key = ptr(self.rigid_contact_sort_key)
+            v.world_start, v.world_count = ptr(self.world_contact_start), ptr(self.world_contact_count)
+            self._view = v
+        return self._view
+
+    def clear(self, bump_generation: bool = True):
+        """contacts.py: reset the counters (buffers are overwritten by the next collide)."""
+        self.contact_counters.zero_()
+        self.world_contact_count.zero_()
+        if bump_generation:
+            self.contact_generation += 1
+
+    def numpy(self) -> dict:
+        """Host copy of the stored rigid contacts (``min(count, capacity)`` rows)."""
+        n = int(self.rigid_contact_count.item())
+        n = min(n, self.rigid_contact_max)
+        out = {
+            "count": int(self.rigid_contact_count.item()),
+            "shape0": self.rigid_contact_shape0[:n].cpu().numpy(),
+            "shape1": self.rigid_contact_shape1[:n].cpu().numpy(),
+            "point0": self.rigid_contact_point0[:n].cpu().numpy(),
+            "point1": self.rigid_contact_point1[:n].cpu().numpy(),
+            "offset0": self.rigid_contact_offset0[:n].cpu().numpy(),
+            "offset1": self.rigid_contact_offset1[:n].cpu().numpy(),
+            "normal": self.rigid_contact_normal[:n].cpu().numpy(),
+            "margin0": self.rigid_contact_margin0[:n].cpu().numpy(),
+            "margin1": self.rigid_contact_margin1[:n].cpu().numpy(),
+        }
+        if self.rigid_contact_sort_key is not None:
+            out["sort_key"] = self.rigid_contact_sort_key[:n].cpu().numpy()
+        return out
+
+
+__all__ = ["Contacts"]
+_ = ctypes

@@ -1,0 +1,372 @@
+// Collision pipeline kernels for gfx950.
+//
+// One workgroup owns one broad-phase segment (a world's shapes plus the global shapes, reference
+// precompute_world_map, newton/_src/geometry/broad_phase_common.py:183-300) and runs, entirely out of
+// LDS: shape world transforms + AABBs (collide.py:150-275), sweep-and-prune on the reference's fixed
+// axis (broad_phase_sap.py:43-394: project, bitonic sort of (lower, index) keys, sweep with the
+// world / group / exclusion / AABB tests), a bitonic sort of the surviving pairs by contact sort key,
+// and the narrow phase, one pair per lane.  Each pair owns five staging slots; a per-segment scan and a
+// grid-wide scan then place every contact at its final index, so the Contacts buffer comes out in
+// canonical order (segment-major, sort key inside a segment) with no float atomics and no global sort.
+//
+// Launch sequence of nmc_collide (all on the caller's stream, graph-capturable):
+//   collide_segment_kernel  <<<segments, 256>>>   AABB + SAP + narrow phase -> staging
+//   contact_scan_kernel     <<<1, 1024>>>         segment offsets, total count
+//   contact_compact_kernel  <<<segments, 256>>>   staging -> Contacts arrays
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "narrow.h"
+#include "nmc_internal.h"
+
+namespace nmc {
+
+constexpr int CBLOCK = 256;
+
+__device__ __forceinline__ unsigned sortable_float(float f) {
+  if (f == 0.0f) f = 0.0f;  // -0 and +0 compare equal: give them one key
+  unsigned u = __float_as_uint(f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+__device__ __forceinline__ bool group_pair(int a, int b) {
+  if (a == 0 || b == 0) return false;
+  if (a > 0) return a == b || b < 0;
+  return a != b;
+}
+
+__device__ __forceinline__ bool excluded(int a, int b, const int* __restrict__ fp, int n) {
+  if (n <= 0) return false;
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    int mid = (lo + hi) >> 1;
+    int ma = fp[2 * mid], mb = fp[2 * mid + 1];
+    if (a == ma && b == mb) return true;
+    bool less = a < ma || (a == ma && b < mb);
+    if (less) hi = mid - 1; else lo = mid + 1;
+  }
+  return false;
+}
+
+// In-LDS bitonic sort of n2 (power of two) keys, ascending.
+template <class K>
+__device__ void bitonic_sort(K* keys, int n2) {
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          K a = keys[i], b = keys[ixj];
+          bool up = (i & k) == 0;
+          if ((a > b) == up) {
+            keys[i] = b;
+            keys[ixj] = a;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+struct SegLds {
+  float* X;       // 7 per shape
+  float* gs;      // 3
+  float* margin;  // 1
+  float* lo;      // 3
+  float* hi;      // 3
+  float* plo;     // 1
+  float* phi;     // 1
+  int* type;
+  int* gid;
+  unsigned long long* skeys;  // sort keys (n2)
+  unsigned* pairs;            // pair keys (pair_cap)
+};
+
+__device__ __forceinline__ ShapeW lds_shape(const SegLds& L, int l) {
+  ShapeW s;
+  s.X = xform{{L.X[7 * l], L.X[7 * l + 1], L.X[7 * l + 2]},
+              {L.X[7 * l + 3], L.X[7 * l + 4], L.X[7 * l + 5], L.X[7 * l + 6]}};
+  s.gscale = f3{L.gs[3 * l], L.gs[3 * l + 1], L.gs[3 * l + 2]};
+  s.margin = L.margin[l];
+  s.lo = f3{L.lo[3 * l], L.lo[3 * l + 1], L.lo[3 * l + 2]};
+  s.hi = f3{L.hi[3 * l], L.hi[3 * l + 1], L.hi[3 * l + 2]};
+  return s;
+}
+
+__global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d, const float* __restrict__ body_q,
+                                                                  StageBuffers st, int* __restrict__ seg_pair_count,
+                                                                  int* __restrict__ seg_pair_base,
+                                                                  int* __restrict__ seg_contact_count,
+                                                                  int* __restrict__ stage_alloc,
+                                                                  int* __restrict__ counters, int pair_cap,
+                                                                  int max_n) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_npairs;
+  __shared__ int s_base;
+  __shared__ int s_red[CBLOCK / 64];
+  const int seg = blockIdx.x;
+  const int s0 = d.segment_start[seg];
+  const int n = d.segment_start[seg + 1] - s0;
+  const bool regular = seg < d.segment_count - 1;
+  const int n2max = next_pow2(max_n > 1 ? max_n : 1);
+
+  SegLds L;
+  unsigned char* p = smem;
+  L.skeys = reinterpret_cast<unsigned long long*>(p); p += sizeof(unsigned long long) * n2max;
+  L.X = reinterpret_cast<float*>(p); p += sizeof(float) * 7 * max_n;
+  L.gs = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
+  L.margin = reinterpret_cast<float*>(p); p += sizeof(float) * max_n;
+  L.lo = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
+  L.hi = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
+  L.plo = reinterpret_cast<float*>(p); p += sizeof(float) * max_n;
+  L.phi = reinterpret_cast<float*>(p); p += sizeof(float) * max_n;
+  L.type = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
+  L.gid = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
+  L.pairs = reinterpret_cast<unsigned*>(p);
+
+  if (threadIdx.x == 0) s_npairs = 0;
+
+  // ---- shape world transforms, AABBs, SAP projections (collide.py:150-275, broad_phase_sap.py:43-63)
+  const f3 dir = normalize(f3{0.5935f, 0.7790f, 0.1235f});
+  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+    int s = d.segment_shape[s0 + l];
+    ShapeW w = shape_world(s, body_q, d);
+    store_xform(L.X + 7 * l, w.X);
+    store3(L.gs + 3 * l, w.gscale);
+    L.margin[l] = w.margin;
+    store3(L.lo + 3 * l, w.lo);
+    store3(L.hi + 3 * l, w.hi);
+    L.type[l] = d.shape_type[s];
+    L.gid[l] = s;
+    f3 half = 0.5f * (w.hi - w.lo);
+    half = f3{half.x + 0.0f, half.y + 0.0f, half.z + 0.0f};
+    float radius = dot(dir, half);
+    float center = dot(dir, 0.5f * (w.lo + w.hi));
+    float plo = center - radius;
+    L.plo[l] = plo;
+    L.phi[l] = center + radius;
+    L.skeys[l] = (static_cast<unsigned long long>(sortable_float(plo)) << 32) | static_cast<unsigned>(l);
+  }
+  const int n2 = next_pow2(n > 1 ? n : 1);
+  for (int l = n + threadIdx.x; l < n2; l += blockDim.x) L.skeys[l] = ~0ull;
+  __syncthreads();
+  bitonic_sort(L.skeys, n2);
+
+  // ---- sweep (broad_phase_sap.py:189-394): element i sweeps forward while lower_j < upper_i
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    int li = static_cast<int>(L.skeys[i] & 0xffffffffu);
+    float upper = L.phi[li];
+    for (int j = i + 1; j < n; ++j) {
+      int lj = static_cast<int>(L.skeys[j] & 0xffffffffu);
+      if (!(L.plo[lj] < upper)) break;
+      int t1 = L.gid[li], t2 = L.gid[lj];
+      if (t1 == t2) continue;
+      int la = t1 < t2 ? li : lj, lb = t1 < t2 ? lj : li;
+      int a = L.gid[la], b = L.gid[lb];
+      int w1 = d.shape_world[a], w2 = d.shape_world[b];
+      if (w1 == -1 && w2 == -1 && regular) continue;
+      if (w1 != -1 && w2 != -1 && w1 != w2) continue;
+      if (!group_pair(d.shape_group[a], d.shape_group[b])) continue;
+      if (excluded(a, b, d.filter_pairs, d.filter_count)) continue;
+      const float c = 0.0f + 0.0f;
+      bool ov = L.lo[3 * la] <= L.hi[3 * lb] + c && L.hi[3 * la] >= L.lo[3 * lb] - c &&
+                L.lo[3 * la + 1] <= L.hi[3 * lb + 1] + c && L.hi[3 * la + 1] >= L.lo[3 * lb + 1] - c &&
+                L.lo[3 * la + 2] <= L.hi[3 * lb + 2] + c && L.hi[3 * la + 2] >= L.lo[3 * lb + 2] - c;
+      if (!ov) continue;
+      // narrow-phase order: lower GeoType first (narrow_phase.py:212-215); ranks order like global ids
+      int ra = la, rb = lb;
+      if (L.type[la] > L.type[lb]) { ra = lb; rb = la; }
+      int slot = atomicAdd(&s_npairs, 1);
+      if (slot < pair_cap) L.pairs[slot] = (static_cast<unsigned>(ra) << 16) | static_cast<unsigned>(rb);
+    }
+  }
+  __syncthreads();
+  int np = s_npairs;
+  if (np > pair_cap) {
+    if (threadIdx.x == 0) atomicAdd(&counters[0], np - pair_cap);
+    np = pair_cap;
+  }
+  const int pn2 = next_pow2(np > 1 ? np : 1);
+  for (int l = np + threadIdx.x; l < pn2; l += blockDim.x) L.pairs[l] = 0xffffffffu;
+  __syncthreads();
+  bitonic_sort(L.pairs, pn2);
+
+  // ---- staging allocation (5 slots per pair)
+  if (threadIdx.x == 0) {
+    int base = atomicAdd(stage_alloc, np);
+    long long cap_pairs = st.slot_capacity / 5;
+    int keep = np;
+    if (static_cast<long long>(base) + np > cap_pairs) {
+      keep = static_cast<int>(cap_pairs - base);
+      if (keep < 0) keep = 0;
+      atomicAdd(&counters[1], np - keep);
+    }
+    s_base = base;
+    seg_pair_base[seg] = base;
+    seg_pair_count[seg] = keep;
+    s_npairs = keep;
+  }
+  __syncthreads();
+  np = s_npairs;
+  const int base = s_base;
+
+  // ---- narrow phase, one pair per lane
+  int local = 0;
+  PairOut out;
+  WriteCtx wctx{body_q, d.shape_body, d.shape_gap, &out};
+  for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
+    unsigned key = L.pairs[pi];
+    int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
+    int sa = L.gid[ra], sb = L.gid[rb];
+    ShapeW A = lds_shape(L, ra), B = lds_shape(L, rb);
+    float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
+    narrow_pair(sa, sb, L.type[ra], L.type[rb], A, B, gap_sum, wctx);
+    long long slot0 = 5ll * (base + pi);
+    for (int k = 0; k < out.n; ++k) {
+      long long s = slot0 + k;
+      st.shape0[s] = sa;
+      st.shape1[s] = sb;
+      store3(st.point0 + 3 * s, out.point0[k]);
+      store3(st.point1 + 3 * s, out.point1[k]);
+      store3(st.offset0 + 3 * s, out.offset0[k]);
+      store3(st.offset1 + 3 * s, out.offset1[k]);
+      store3(st.normal + 3 * s, out.normal[k]);
+      st.margin0[s] = out.margin0[k];
+      st.margin1[s] = out.margin1[k];
+      st.sub_key[s] = out.sub_key[k];
+    }
+    st.mask[base + pi] = static_cast<unsigned char>(out.n);
+    local += out.n;
+  }
+  // block reduction of the segment's contact count
+  for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = local;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int t = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += s_red[i];
+    seg_contact_count[seg] = t;
+  }
+}
+
+// Exclusive scan of per-segment contact counts -> Contacts.world_start / world_count / count.
+__global__ void __launch_bounds__(1024) contact_scan_kernel(const int* __restrict__ seg_contact_count, int segments,
+                                                             NmcContacts c, int* __restrict__ stage_alloc,
+                                                             int* __restrict__ counters) {
+  __shared__ int s_part[1024];
+  __shared__ int s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < segments; base += blockDim.x) {
+    int i = base + threadIdx.x;
+    int v = i < segments ? seg_contact_count[i] : 0;
+    s_part[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int t = threadIdx.x >= (unsigned)off ? s_part[threadIdx.x - off] : 0;
+      __syncthreads();
+      s_part[threadIdx.x] += t;
+      __syncthreads();
+    }
+    int incl = s_part[threadIdx.x] + s_carry;
+    int excl = incl - v;
+    if (i < segments) {
+      c.world_start[i] = excl;
+      int room = c.capacity - excl;
+      int stored = v < room ? v : room;
+      c.world_count[i] = stored > 0 ? stored : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) s_carry = incl;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    c.count[0] = s_carry;
+    if (s_carry > c.capacity) atomicAdd(&counters[2], s_carry - c.capacity);
+    stage_alloc[0] = 0;
+  }
+}
+
+// Staging -> Contacts arrays at canonical indices.
+__global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st, const int* __restrict__ seg_pair_count,
+                                                                  const int* __restrict__ seg_pair_base, NmcContacts c) {
+  __shared__ int s_scan[CBLOCK];
+  __shared__ int s_carry;
+  const int seg = blockIdx.x;
+  const int np = seg_pair_count[seg];
+  const int base = seg_pair_base[seg];
+  const int start = c.world_start[seg];
+  const int stored = c.world_count[seg];
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int b0 = 0; b0 < np; b0 += blockDim.x) {
+    int pi = b0 + threadIdx.x;
+    int cnt = pi < np ? static_cast<int>(st.mask[base + pi]) : 0;
+    s_scan[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int t = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+      __syncthreads();
+      s_scan[threadIdx.x] += t;
+      __syncthreads();
+    }
+    int excl = s_scan[threadIdx.x] - cnt + s_carry;
+    for (int k = 0; k < cnt; ++k) {
+      int local = excl + k;
+      if (local >= stored) break;
+      long long s = 5ll * (base + pi) + k;
+      int o = start + local;
+      int s0 = st.shape0[s], s1 = st.shape1[s];
+      c.shape0[o] = s0;
+      c.shape1[o] = s1;
+      for (int q = 0; q < 3; ++q) {
+        c.point0[3 * o + q] = st.point0[3 * s + q];
+        c.point1[3 * o + q] = st.point1[3 * s + q];
+        c.offset0[3 * o + q] = st.offset0[3 * s + q];
+        c.offset1[3 * o + q] = st.offset1[3 * s + q];
+        c.normal[3 * o + q] = st.normal[3 * s + q];
+      }
+      c.margin0[o] = st.margin0[s];
+      c.margin1[o] = st.margin1[s];
+      if (c.sort_key) {
+        c.sort_key[o] = ((static_cast<long long>(s0) & 0xFFFFF) << 43) | ((static_cast<long long>(s1) & 0xFFFFF) << 23) |
+                        (static_cast<long long>(st.sub_key[s]) & 0x7FFFFF);
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) s_carry = excl + cnt;
+    __syncthreads();
+  }
+}
+
+size_t collide_lds_bytes(int max_n, int pair_cap) {
+  int n2 = 1;
+  while (n2 < max_n) n2 <<= 1;
+  size_t b = sizeof(unsigned long long) * n2 + sizeof(float) * 21 * (size_t)max_n + sizeof(int) * 2 * (size_t)max_n;
+  b += sizeof(unsigned) * (size_t)pair_cap;
+  return (b + 15) & ~size_t(15);
+}
+
+hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContacts* c, hipStream_t stream) {
+  const NmcModelDesc& d = m->d;
+  int segs = d.segment_count;
+  if (segs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(collide_segment_kernel, dim3(segs), dim3(CBLOCK), m->collide_lds, stream, d, body_q, m->stage,
+                     m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc, m->counters,
+                     m->pair_cap, d.max_segment_shapes);
+  hipLaunchKernelGGL(contact_scan_kernel, dim3(1), dim3(1024), 0, stream, m->seg_contact_count, segs, *c,
+                     m->stage_alloc, m->counters);
+  hipLaunchKernelGGL(contact_compact_kernel, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_count,
+                     m->seg_pair_base, *c);
+  return hipGetLastError();
+}
+
+}  // namespace nmc

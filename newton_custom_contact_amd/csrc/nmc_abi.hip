@@ -1,0 +1,143 @@
+// extern "C" entry points of libnmc.so (declared in include/nmc.h).
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "nmc_internal.h"
+
+namespace nmc {
+
+size_t collide_lds_bytes(int max_n, int pair_cap);
+size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap);
+hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContacts* c, hipStream_t stream);
+hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
+                       const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
+                       const NmcContacts* c, float dt, float* q_out, float* qd_out, hipStream_t stream);
+
+static thread_local std::string g_error;
+void set_error(const std::string& msg) { g_error = msg; }
+
+static int fail(const std::string& msg) {
+  set_error(msg);
+  return -1;
+}
+
+static int hip_fail(hipError_t e, const char* what) {
+  return fail(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+static hipError_t dmalloc(T** p, size_t count) {
+  size_t bytes = sizeof(T) * (count > 0 ? count : 1);
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(p), bytes);
+  if (e == hipSuccess) e = hipMemset(*p, 0, bytes);
+  return e;
+}
+
+static int next_pow2(int n) {
+  int p = 1;
+  while (p < n) p <<= 1;
+  return p;
+}
+
+}  // namespace nmc
+
+using namespace nmc;
+
+extern "C" const char* nmc_last_error(void) { return g_error.c_str(); }
+
+extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
+  if (!desc || !out) return fail("nmc_model_create: null argument");
+  const NmcModelDesc& d = *desc;
+  if (d.segment_count != d.world_count + 1) return fail("segment_count must equal world_count + 1");
+  if (d.max_world_bodies > 512) return fail("worlds with more than 512 bodies are not supported by the XPBD kernel");
+  if (d.max_segment_shapes > 65535) return fail("a broad-phase segment may hold at most 65535 shapes");
+  NmcModel* m = new NmcModel();
+  m->d = d;
+  hipError_t e = hipGetDevice(&m->device);
+  if (e != hipSuccess) { delete m; return hip_fail(e, "hipGetDevice"); }
+  int segs = d.segment_count;
+  // per-segment pair buffer: the worst case n(n-1)/2 of the largest segment, capped at 16384 entries
+  long long maxn = d.max_segment_shapes;
+  long long worst = maxn * (maxn - 1) / 2;
+  int pair_cap = static_cast<int>(worst < 16384 ? worst : 16384);
+  if (pair_cap < 1) pair_cap = 1;
+  m->pair_cap = next_pow2(pair_cap);
+  m->collide_lds = collide_lds_bytes(d.max_segment_shapes > 0 ? d.max_segment_shapes : 1, m->pair_cap);
+  // staging: 5 slots per broad-phase pair; the pair bound is the reference's shape_pairs_max, and the
+  // slot count is capped by 2x the contact capacity (a pair yields 1-5 contacts, most yield 1-2)
+  long long pairs = d.shape_pairs_max > 0 ? d.shape_pairs_max : 1;
+  long long slots = 5 * pairs;
+  long long cap2 = 2ll * (d.rigid_contact_max > 0 ? d.rigid_contact_max : 1) + 5ll * m->pair_cap;
+  if (slots > cap2) slots = cap2;
+  m->stage.slot_capacity = slots;
+  // solver ordering buffer: 2 entries per contact of the densest world
+  m->xpbd_contact_cap = 8192;
+  m->xpbd_lds = xpbd_lds_bytes(d.max_world_bodies, d.max_world_joints, m->xpbd_contact_cap);
+  if (m->collide_lds > 160 * 1024 || m->xpbd_lds > 160 * 1024) {
+    delete m;
+    return fail("per-world LDS requirement exceeds 160 KiB");
+  }
+  StageBuffers& s = m->stage;
+  if ((e = dmalloc(&s.shape0, slots)) != hipSuccess || (e = dmalloc(&s.shape1, slots)) != hipSuccess ||
+      (e = dmalloc(&s.point0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.point1, 3 * slots)) != hipSuccess ||
+      (e = dmalloc(&s.offset0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.offset1, 3 * slots)) != hipSuccess ||
+      (e = dmalloc(&s.normal, 3 * slots)) != hipSuccess || (e = dmalloc(&s.margin0, slots)) != hipSuccess ||
+      (e = dmalloc(&s.margin1, slots)) != hipSuccess || (e = dmalloc(&s.sub_key, slots)) != hipSuccess ||
+      (e = dmalloc(&s.mask, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&m->seg_pair_count, segs)) != hipSuccess ||
+      (e = dmalloc(&m->seg_pair_base, segs)) != hipSuccess || (e = dmalloc(&m->seg_contact_count, segs)) != hipSuccess ||
+      (e = dmalloc(&m->stage_alloc, 1)) != hipSuccess || (e = dmalloc(&m->counters, 4)) != hipSuccess) {
+    nmc_model_destroy(m);
+    return hip_fail(e, "nmc_model_create: device allocation");
+  }
+  *out = m;
+  return 0;
+}
+
+extern "C" void nmc_model_destroy(NmcModel* m) {
+  if (!m) return;
+  StageBuffers& s = m->stage;
+  void* ptrs[] = {s.shape0, s.shape1, s.point0, s.point1, s.offset0, s.offset1, s.normal, s.margin0, s.margin1,
+                  s.sub_key, s.mask, m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc,
+                  m->counters};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  delete m;
+}
+
+extern "C" int nmc_collide(NmcModel* m, const float* body_q, const NmcContacts* c, nmc_stream_t stream) {
+  if (!m || !c) return fail("nmc_collide: null argument");
+  if (m->d.body_count > 0 && !body_q) return fail("nmc_collide: body_q is null");
+  if (!c->count || !c->shape0 || !c->shape1 || !c->point0 || !c->point1 || !c->offset0 || !c->offset1 ||
+      !c->normal || !c->margin0 || !c->margin1 || !c->world_start || !c->world_count)
+    return fail("nmc_collide: contacts buffer has a null array");
+  hipError_t e = launch_collide(m, body_q, c, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "nmc_collide launch");
+  return 0;
+}
+
+extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q_in, const float* qd_in,
+                             const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
+                             const NmcContacts* c, float dt, float* q_out, float* qd_out, nmc_stream_t stream) {
+  if (!m || !p) return fail("nmc_xpbd_step: null argument");
+  if (m->d.body_count > 0 && (!q_in || !qd_in || !f_in || !q_out || !qd_out))
+    return fail("nmc_xpbd_step: body state array is null");
+  if (m->d.joint_count > 0 && (!joint_f || !tpos || !tvel)) return fail("nmc_xpbd_step: joint array is null");
+  if (p->has_contacts && !c) return fail("nmc_xpbd_step: has_contacts set but contacts is null");
+  if (q_in == q_out || qd_in == qd_out) return fail("nmc_xpbd_step: state_in and state_out must not alias");
+  hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
+                             qd_out, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step launch");
+  return 0;
+}
+
+extern "C" int nmc_get_counters(NmcModel* m, int* out, nmc_stream_t stream) {
+  if (!m || !out) return fail("nmc_get_counters: null argument");
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  hipError_t e = hipMemcpyAsync(out, m->counters, 4 * sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "nmc_get_counters");
+  return 0;
+}

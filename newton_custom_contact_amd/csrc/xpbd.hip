@@ -1,0 +1,741 @@
+// XPBD step kernel for gfx950: one workgroup advances one world through a whole
+// SolverXPBD.step (newton/_src/solvers/xpbd/solver_xpbd.py:259-764):
+//   joint forces -> integrate -> iterations x { contact solve + apply, joint solve + apply } -> store
+// Body state stays in LDS for all iterations; HBM is touched once to load the step's inputs and once to
+// store its outputs, plus the world's contact records (L2-resident across iterations).
+//
+// The reference accumulates per-body deltas with float atomics (kernels.py:2292-2297), which makes the
+// sum order scheduling dependent.  Here every per-body sum runs in canonical order: contributions of
+// contact c (both sides) land in LDS slots, each body owns the sorted list of its (contact, side) entries,
+// and one lane per body adds them in ascending contact order (joints: ascending joint order, from the
+// static body->joint table).  The result is deterministic and equals a sequential sweep over contacts.
+#include <hip/hip_runtime.h>
+
+#include "nmc_internal.h"
+#include "nmc_math.h"
+
+namespace nmc {
+
+struct Spatial { f3 lin, ang; };
+struct BodyState { xform q; f3 v, w; };
+
+// integrate_rigid_body (newton/_src/solvers/solver.py:50-95)
+__device__ __forceinline__ BodyState integrate_body(const BodyState& s, const Spatial& f, f3 com, const m33& I,
+                                                   float inv_mass, const m33& inv_I, f3 g, float ang_damp, float dt) {
+  f3 x0 = s.q.p;
+  qf r0 = s.q.q;
+  f3 x_com = x0 + rot(r0, com);
+  float nz = inv_mass != 0.0f ? 1.0f : 0.0f;
+  f3 v1 = s.v + (f.lin * inv_mass + g * nz) * dt;
+  f3 x1 = x_com + v1 * dt;
+  f3 wb = rot_inv(r0, s.w);
+  f3 tb = rot_inv(r0, f.ang) - cross(wb, mv(I, wb));
+  f3 w1 = rot(r0, wb + mv(inv_I, tb) * dt);
+  qf wq{w1.x, w1.y, w1.z, 0.0f};
+  qf r1 = qnormalize(qadd(r0, qscale(qscale(qmul(wq, r0), 0.5f), dt)));
+  w1 = w1 * (1.0f - ang_damp * dt);
+  BodyState o;
+  o.q = xform{x1 - rot(r1, com), r1};
+  o.v = v1;
+  o.w = w1;
+  return o;
+}
+
+// apply_body_deltas (newton/_src/solvers/xpbd/kernels.py:809-879)
+__device__ __forceinline__ BodyState apply_deltas(const BodyState& s, f3 com, const m33& I, float inv_m,
+                                                 const m33& inv_I, const Spatial& delta, bool has_weights,
+                                                 float inv_weight, float dt) {
+  if (inv_m == 0.0f) return s;
+  qf q0 = s.q.q;
+  float weight = 1.0f;
+  if (has_weights && inv_weight > 0.0f) weight = 1.0f / inv_weight;
+  f3 dp = delta.lin * (inv_m * weight);
+  f3 dq = delta.ang * weight;
+  f3 wb = rot_inv(q0, s.w);
+  f3 dwb = mv(inv_I, rot_inv(q0, dq));
+  f3 tb = cross(dwb, mv(I, wb + dwb)) + cross(wb, mv(I, dwb));
+  f3 dw1 = rot(q0, dwb - mv(mscale(dt, inv_I), tb));
+  f3 dwdt = dw1 * dt;
+  qf dq4{dwdt.x, dwdt.y, dwdt.z, 0.0f};
+  qf q1 = qadd(q0, qmul(qf{0.5f * dq4.x, 0.5f * dq4.y, 0.5f * dq4.z, 0.5f * dq4.w}, q0));
+  q1 = qnormalize(q1);
+  f3 x_com = s.q.p + rot(q0, com);
+  f3 p1 = x_com + dp * dt;
+  p1 = p1 - rot(q1, com);
+  BodyState o;
+  o.q = xform{p1, q1};
+  f3 v1 = s.v + dp;
+  f3 w1 = s.w + dw1;
+  if (len(v1) < 1e-4f) v1 = f3{0.0f, 0.0f, 0.0f};
+  if (len(w1) < 1e-4f) w1 = f3{0.0f, 0.0f, 0.0f};
+  o.v = v1;
+  o.w = w1;
+  return o;
+}
+
+// compute_contact_constraint_delta (kernels.py:1945-1980)
+__device__ __forceinline__ float contact_delta(float err, qf qa, qf qb, float mia, float mib, const m33& Ia,
+                                               const m33& Ib, f3 la, f3 lb, f3 aa, f3 ab, float relax, float dt) {
+  float denom = 0.0f;
+  denom += len_sq(la) * mia;
+  denom += len_sq(lb) * mib;
+  f3 ra = rot_inv(qa, aa);
+  f3 rb = rot_inv(qb, ab);
+  denom += dot(ra, mv(Ia, ra));
+  denom += dot(rb, mv(Ib, rb));
+  float dl = -err;
+  if (denom > 0.0f) dl /= dt * denom;
+  return dl * relax;
+}
+
+__device__ __forceinline__ float positional_correction(float err, float derr, qf qa, qf qb, float mia, float mib,
+                                                       const m33& Ia, const m33& Ib, f3 la, f3 lb, f3 aa, f3 ab,
+                                                       float compliance, float damping, float dt) {
+  float denom = 0.0f;
+  denom += len_sq(la) * mia;
+  denom += len_sq(lb) * mib;
+  f3 ra = rot_inv(qa, aa);
+  f3 rb = rot_inv(qb, ab);
+  denom += dot(ra, mv(Ia, ra));
+  denom += dot(rb, mv(Ib, rb));
+  float alpha = compliance;
+  float gamma = compliance * damping;
+  float dl = -(err + alpha * 0.0f + gamma * derr);
+  if (denom + alpha > 0.0f) dl /= (dt + gamma) * denom + alpha / dt;
+  return dl;
+}
+
+__device__ __forceinline__ float angular_correction(float err, float derr, qf qa, qf qb, const m33& Ia, const m33& Ib,
+                                                    f3 aa, f3 ab, float compliance, float damping, float dt) {
+  float denom = 0.0f;
+  f3 ra = rot_inv(qa, aa);
+  f3 rb = rot_inv(qb, ab);
+  denom += dot(ra, mv(Ia, ra));
+  denom += dot(rb, mv(Ib, rb));
+  float alpha = compliance;
+  float gamma = compliance * damping;
+  float dl = -(err + alpha * 0.0f + gamma * derr);
+  if (denom + alpha > 0.0f) dl /= (dt + gamma) * denom + alpha / dt;
+  return dl;
+}
+
+// Per-world LDS views of body data (local body index).
+struct WorldLds {
+  float* q;      // 7
+  float* qd;     // 6
+  float* com;    // 3
+  float* inv_m;  // 1 (effective)
+  float* inv_I;  // 9 (effective)
+  int* flags;
+};
+
+__device__ __forceinline__ xform lds_q(const WorldLds& L, int b) {
+  const float* a = L.q + 7 * b;
+  return xform{{a[0], a[1], a[2]}, {a[3], a[4], a[5], a[6]}};
+}
+__device__ __forceinline__ m33 lds_m33(const float* a) {
+  m33 r;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.m[i][j] = a[3 * i + j];
+  return r;
+}
+
+// solve_body_contact_positions (kernels.py:2062-2299) for one contact; bodies are local indices or -1.
+__device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f3 o0, f3 o1, f3 n, float m0, float m1,
+                              float mu, float mut, float mur, float relax, float dt, Spatial& da, Spatial& db) {
+  const m33 Z = {{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}};
+  xform Xa = la >= 0 ? lds_q(L, la) : xident();
+  xform Xb = lb >= 0 ? lds_q(L, lb) : xident();
+  f3 bx_a = xpoint(Xa, p0);
+  f3 bx_b = xpoint(Xb, p1);
+  float thickness = m0 + m1;
+  float d = dot(n, bx_b - bx_a) - thickness;
+  if (d >= 0.0f) return false;
+  f3 zero{0.0f, 0.0f, 0.0f};
+  float mia = 0.0f, mib = 0.0f;
+  m33 Ia = Z, Ib = Z;
+  f3 com_a = zero, com_b = zero, om_a = zero, om_b = zero;
+  if (la >= 0) {
+    com_a = load3(L.com + 3 * la); mia = L.inv_m[la]; Ia = lds_m33(L.inv_I + 9 * la); om_a = load3(L.qd + 6 * la + 3);
+  }
+  if (lb >= 0) {
+    com_b = load3(L.com + 3 * lb); mib = L.inv_m[lb]; Ib = lds_m33(L.inv_I + 9 * lb); om_b = load3(L.qd + 6 * lb + 3);
+  }
+  f3 r_a = bx_a - xpoint(Xa, com_a);
+  f3 r_b = bx_b - xpoint(Xb, com_b);
+  f3 ang_a = -cross(r_a, n);
+  f3 ang_b = cross(r_b, n);
+  float lambda_n = contact_delta(d, Xa.q, Xb.q, mia, mib, Ia, Ib, -n, n, ang_a, ang_b, relax, dt);
+  f3 lda = (-n) * lambda_n, ldb = n * lambda_n;
+  f3 ada = ang_a * lambda_n, adb = ang_b * lambda_n;
+  if (mu > 0.0f) {
+    bx_a = bx_a + xvector(Xa, o0);
+    bx_b = bx_b + xvector(Xb, o1);
+    f3 delta = bx_b - bx_a;
+    f3 fd = delta - dot(n, delta) * n;
+    r_a = bx_a - xpoint(Xa, com_a);
+    r_b = bx_b - xpoint(Xb, com_b);
+    f3 rel_v = zero;
+    if (la >= 0 && (L.flags[la] & BODY_KINEMATIC)) {
+      f3 va = load3(L.qd + 6 * la) + cross(load3(L.qd + 6 * la + 3), r_a);
+      rel_v = rel_v - (va - dot(n, va) * n);
+    }
+    if (lb >= 0 && (L.flags[lb] & BODY_KINEMATIC)) {
+      f3 vb = load3(L.qd + 6 * lb) + cross(load3(L.qd + 6 * lb + 3), r_b);
+      rel_v = rel_v + (vb - dot(n, vb) * n);
+    }
+    fd = fd + rel_v * dt;
+    f3 perp = normalize(fd);
+    ang_a = -cross(r_a, perp);
+    ang_b = cross(r_b, perp);
+    float err = len(fd);
+    if (err > 0.0f) {
+      float lf = contact_delta(err, Xa.q, Xb.q, mia, mib, Ia, Ib, -perp, perp, ang_a, ang_b, relax, dt);
+      lf = fmax2(lf, -lambda_n * mu);
+      lda = lda - perp * lf;
+      ldb = ldb + perp * lf;
+      ada = ada + ang_a * lf;
+      adb = adb + ang_b * lf;
+    }
+  }
+  f3 dom = om_b - om_a;
+  if (mut > 0.0f) {
+    float err = dot(dom, n) * dt;
+    if (fabsf(err) > 0.0f) {
+      float lt = contact_delta(err, Xa.q, Xb.q, mia, mib, Ia, Ib, zero, zero, -n, n, relax, dt);
+      lt = clampf(lt, -lambda_n * mut, lambda_n * mut);
+      ada = ada - n * lt;
+      adb = adb + n * lt;
+    }
+  }
+  if (mur > 0.0f) {
+    dom = dom - dot(n, dom) * n;
+    float err = len(dom) * dt;
+    if (err > 0.0f) {
+      f3 rn = normalize(dom);
+      float lr = contact_delta(err, Xa.q, Xb.q, mia, mib, Ia, Ib, zero, zero, -rn, rn, relax, dt);
+      lr = fmax2(lr, -lambda_n * mur);
+      ada = ada - rn * lr;
+      adb = adb + rn * lr;
+    }
+  }
+  da = Spatial{lda, ada};
+  db = Spatial{ldb, adb};
+  return true;
+}
+
+__device__ __forceinline__ void axis_limits_update(f3 axis, float lo_l, float up_l, f3& lo, f3& up) {
+  f3 lt = axis * lo_l, ut = axis * up_l;
+  f3 l = vmin(lt, ut), u = vmax(lt, ut);
+  lo = vmin(lo, l);
+  up = vmax(up, u);
+}
+__device__ __forceinline__ void weighted_target(f3 axis, float target, float weight, f3& targets, f3& weights) {
+  f3 wa = axis * weight;
+  targets = targets + wa * target;
+  weights = weights + f3{fabsf(wa.x), fabsf(wa.y), fabsf(wa.z)};
+}
+
+struct JointAxes { f3 lo, up, tpos, kew, tvel, kdw; };
+
+__device__ void joint_axes(const NmcModelDesc& d, int start, int count, const float* tposa, const float* tvela,
+                           JointAxes& A) {
+  f3 zero{0.0f, 0.0f, 0.0f};
+  A = JointAxes{zero, zero, zero, zero, zero, zero};
+  for (int k = 0; k < count; ++k) {
+    int ai = start + k;
+    f3 axis = load3(d.joint_axis + 3 * ai);
+    if (k == 0) {
+      f3 lt = axis * d.joint_limit_lower[ai], ut = axis * d.joint_limit_upper[ai];
+      A.lo = vmin(lt, ut);
+      A.up = vmax(lt, ut);
+    } else {
+      axis_limits_update(axis, d.joint_limit_lower[ai], d.joint_limit_upper[ai], A.lo, A.up);
+    }
+    float ke = d.joint_target_ke[ai], kd = d.joint_target_kd[ai];
+    if (ke > 0.0f) weighted_target(axis, tposa[ai], ke, A.tpos, A.kew);
+    if (kd > 0.0f) weighted_target(axis, tvela[ai], kd, A.tvel, A.kdw);
+  }
+  for (int i = 0; i < 3; ++i)
+    if (get(A.kew, i) > 0.0f) put(A.tpos, i, get(A.tpos, i) / get(A.kew, i));
+  for (int i = 0; i < 3; ++i)
+    if (get(A.kdw, i) > 0.0f) put(A.tvel, i, get(A.tvel, i) / get(A.kdw, i));
+}
+
+// solve_body_joints (kernels.py:1441-1943) for one joint; parent/child given as local body indices
+// (-1 = world).  Returns false when the joint writes nothing.
+__device__ bool joint_solve(const NmcModelDesc& d, const WorldLds& L, int j, int lp, int lc, const float* tposa,
+                            const float* tvela, float lin_compl, float ang_compl, float ang_relax, float lin_relax,
+                            float dt, Spatial& dpo, Spatial& dco) {
+  int type = d.joint_type[j];
+  if (!d.joint_enabled[j]) return false;
+  if (type == JT_FREE || type == JT_DISTANCE) return false;
+  const m33 Z = {{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}};
+  f3 zero{0.0f, 0.0f, 0.0f};
+  xform X_pj = load_xform(d.joint_X_p + 7 * j);
+  xform X_cj = load_xform(d.joint_X_c + 7 * j);
+  xform X_wp = X_pj, pose_p = X_pj;
+  float m_inv_p = 0.0f;
+  m33 I_inv_p = Z;
+  f3 com_p = zero, vel_p = zero, omega_p = zero;
+  if (lp >= 0) {
+    pose_p = lds_q(L, lp);
+    X_wp = xmul(pose_p, X_wp);
+    com_p = load3(L.com + 3 * lp);
+    m_inv_p = L.inv_m[lp];
+    I_inv_p = lds_m33(L.inv_I + 9 * lp);
+    vel_p = load3(L.qd + 6 * lp);
+    omega_p = load3(L.qd + 6 * lp + 3);
+  }
+  xform pose_c = lds_q(L, lc);
+  xform X_wc = xmul(pose_c, X_cj);
+  f3 com_c = load3(L.com + 3 * lc);
+  float m_inv_c = L.inv_m[lc];
+  m33 I_inv_c = lds_m33(L.inv_I + 9 * lc);
+  f3 vel_c = load3(L.qd + 6 * lc), omega_c = load3(L.qd + 6 * lc + 3);
+  if (m_inv_p == 0.0f && m_inv_c == 0.0f) return false;
+  f3 ldp = zero, adp = zero, ldc = zero, adc = zero;
+  xform rel_pose = xmul(xinverse(X_wp), X_wc);
+  f3 rel_p = rel_pose.p;
+  f3 x_c = X_wc.p;
+  int axis_start = d.joint_qd_start[j];
+  int lin_n = d.joint_dof_dim[2 * j], ang_n = d.joint_dof_dim[2 * j + 1];
+  f3 world_com_p = xpoint(pose_p, com_p);
+  {
+    JointAxes A;
+    joint_axes(d, axis_start, lin_n, tposa, tvela, A);
+    m33 frame_p = qmat(X_wp.q);
+    f3 r_p = x_c - world_com_p;
+    f3 r_c = x_c - xpoint(pose_c, com_c);
+    for (int dim = 0; dim < 3; ++dim) {
+      float e = get(rel_p, dim);
+      f3 lcv{frame_p.m[0][dim], frame_p.m[1][dim], frame_p.m[2][dim]};
+      f3 lpv = -lcv;
+      f3 ap = -cross(r_p, lcv);
+      f3 ac = cross(r_c, lcv);
+      float derr = dot(lpv, vel_p) + dot(lcv, vel_c) + dot(ap, omega_p) + dot(ac, omega_c);
+      float err = 0.0f, compliance = lin_compl, damping = 0.0f;
+      float derr_rel = derr - get(A.tvel, dim);
+      float lower = get(A.lo, dim), upper = get(A.up, dim);
+      if (e < lower) err = e - lower;
+      else if (e > upper) err = e - upper;
+      else {
+        float target = clampf(get(A.tpos, dim), lower, upper);
+        if (get(A.kew, dim) > 0.0f) { err = e - target; compliance = 1.0f / get(A.kew, dim); damping = get(A.kdw, dim); }
+        else if (get(A.kdw, dim) > 0.0f) { compliance = 1.0f / get(A.kdw, dim); damping = get(A.kdw, dim); }
+      }
+      if (fabsf(err) > 1e-9f || fabsf(derr_rel) > 1e-9f) {
+        float dl = positional_correction(err, derr_rel, pose_p.q, pose_c.q, m_inv_p, m_inv_c, I_inv_p, I_inv_c, lpv,
+                                         lcv, ap, ac, compliance, damping, dt);
+        ldp = ldp + lpv * (dl * lin_relax);
+        adp = adp + ap * (dl * ang_relax);
+        ldc = ldc + lcv * (dl * lin_relax);
+        adc = adc + ac * (dl * ang_relax);
+      }
+    }
+  }
+  if (type == JT_FIXED || type == JT_PRISMATIC || type == JT_REVOLUTE || type == JT_D6) {
+    qf q_p = X_wp.q, q_c = X_wc.q;
+    if (q_p.x * q_c.x + q_p.y * q_c.y + q_p.z * q_c.z + q_p.w * q_c.w < 0.0f) q_c = qscale(q_c, -1.0f);
+    qf rel_q = qmul(qinv(q_p), q_c);
+    qf qtwist = qnormalize(qf{rel_q.x, 0.0f, 0.0f, rel_q.w});
+    qf qswing = qmul(rel_q, qinv(qtwist));
+    float s = sqrtf(rel_q.x * rel_q.x + rel_q.w * rel_q.w);
+    float invs = 1.0f / s;
+    float invscube = invs * invs * invs;
+    float errs[3];
+    errs[0] = 2.0f * asinf(clampf(qtwist.x, -1.0f, 1.0f));
+    errs[1] = qswing.y;
+    errs[2] = qswing.z;
+    qf g[3];
+    g[0] = qf{invs - rel_q.x * rel_q.x * invscube, 0.0f, 0.0f, -(rel_q.w * rel_q.x) * invscube};
+    g[1] = qf{-rel_q.w * (rel_q.w * rel_q.z + rel_q.x * rel_q.y) * invscube, rel_q.w * invs, -rel_q.x * invs,
+              rel_q.x * (rel_q.w * rel_q.z + rel_q.x * rel_q.y) * invscube};
+    g[2] = qf{rel_q.w * (rel_q.w * rel_q.y - rel_q.x * rel_q.z) * invscube, rel_q.x * invs, rel_q.w * invs,
+              rel_q.x * (rel_q.z * rel_q.x - rel_q.w * rel_q.y) * invscube};
+    g[0] = qscale(g[0], 2.0f / fabsf(qtwist.w));
+    float swing_sq = qswing.w * qswing.w;
+    if (swing_sq + 1.0e-4f < 1.0f) {
+      float dd = sqrtf(1.0f - qswing.w * qswing.w);
+      float theta = 2.0f * acosf(clampf(qswing.w, -1.0f, 1.0f));
+      float scale = theta / dd;
+      errs[1] *= scale;
+      errs[2] *= scale;
+      g[1] = qscale(g[1], scale);
+      g[2] = qscale(g[2], scale);
+    }
+    JointAxes A;
+    joint_axes(d, axis_start + lin_n, ang_n, tposa, tvela, A);
+    for (int dim = 0; dim < 3; ++dim) {
+      float e = errs[dim];
+      qf qc = qmul(qmul(qf{0.5f * q_p.x, 0.5f * q_p.y, 0.5f * q_p.z, 0.5f * q_p.w}, g[dim]), qinv(q_c));
+      f3 ac{qc.x, qc.y, qc.z};
+      f3 ap = -ac;
+      float derr = dot(ap, omega_p) + dot(ac, omega_c);
+      float err = 0.0f, compliance = ang_compl, damping = 0.0f;
+      float ac_len = len(ac);
+      float derr_rel = derr - get(A.tvel, dim) * ac_len;
+      float lower = get(A.lo, dim), upper = get(A.up, dim);
+      if (e < lower) err = e - lower;
+      else if (e > upper) err = e - upper;
+      else {
+        float target = clampf(get(A.tpos, dim), lower, upper);
+        if (get(A.kew, dim) > 0.0f) { err = e - target; compliance = 1.0f / get(A.kew, dim); damping = get(A.kdw, dim); }
+        else if (get(A.kdw, dim) > 0.0f) { damping = get(A.kdw, dim); compliance = 1.0f / get(A.kdw, dim); }
+      }
+      float dl = angular_correction(err, derr_rel, pose_p.q, pose_c.q, I_inv_p, I_inv_c, ap, ac, compliance, damping,
+                                    dt) * ang_relax;
+      adp = adp + ap * dl;
+      adc = adc + ac * dl;
+    }
+  }
+  dpo = Spatial{ldp, adp};
+  dco = Spatial{ldc, adc};
+  return true;
+}
+
+constexpr int XB = 256;  // lanes per world workgroup
+constexpr int BPL = 2;   // bodies per lane (worlds up to 512 bodies)
+
+struct XpbdArgs {
+  NmcModelDesc d;
+  NmcXpbdParams p;
+  const float* q_in;
+  const float* qd_in;
+  const float* f_in;
+  const float* joint_f;
+  const float* tpos;
+  const float* tvel;
+  NmcContacts c;
+  float dt;
+  float* q_out;
+  float* qd_out;
+  int key_cap;  // (contact, side) entries the LDS ordering buffer holds (power of two)
+  int* counters;
+};
+
+__device__ __forceinline__ void sort_keys(unsigned* keys, int n2) {
+  for (int k = 2; k <= n2; k <<= 1) {
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        int ixj = i ^ j;
+        if (ixj > i) {
+          unsigned x = keys[i], y = keys[ixj];
+          bool up = (i & k) == 0;
+          if ((x > y) == up) { keys[i] = y; keys[ixj] = x; }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+__device__ __forceinline__ BodyState lds_state(const WorldLds& L, int lb) {
+  return BodyState{lds_q(L, lb), load3(L.qd + 6 * lb), load3(L.qd + 6 * lb + 3)};
+}
+__device__ __forceinline__ void lds_store_state(const WorldLds& L, int lb, const BodyState& s) {
+  store_xform(L.q + 7 * lb, s.q);
+  store3(L.qd + 6 * lb, s.v);
+  store3(L.qd + 6 * lb + 3, s.w);
+}
+
+__global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const NmcModelDesc& d = a.d;
+  const int w = blockIdx.x;  // world index; w == world_count owns the global bodies
+  const int b0 = d.world_body_first[w];
+  const int nb = d.world_body_count[w];
+  const int j0 = d.world_joint_first[w];
+  const int nj = d.world_joint_count[w];
+  const int mb = d.max_world_bodies > 0 ? d.max_world_bodies : 1;
+  const int mj = d.max_world_joints > 0 ? d.max_world_joints : 1;
+  const float dt = a.dt;
+
+  WorldLds L;
+  unsigned char* p = smem;
+  L.q = reinterpret_cast<float*>(p); p += sizeof(float) * 7 * mb;
+  L.qd = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * mb;
+  float* fl = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * mb;
+  L.com = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * mb;
+  L.inv_m = reinterpret_cast<float*>(p); p += sizeof(float) * mb;
+  L.inv_I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
+  L.flags = reinterpret_cast<int*>(p); p += sizeof(int) * mb;
+  int* seg = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
+  float* contrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * XB;
+  int* active = reinterpret_cast<int*>(p); p += sizeof(int) * XB;
+  float* jcontrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * mj;
+  int* jactive = reinterpret_cast<int*>(p); p += sizeof(int) * mj;
+  unsigned* keys = reinterpret_cast<unsigned*>(p);
+
+  const bool has_c = a.p.has_contacts != 0 && w < d.segment_count;
+  const int cstart = has_c ? a.c.world_start[w] : 0;
+  int nc = has_c ? a.c.world_count[w] : 0;
+  if (2 * nc > a.key_cap) {
+    if (threadIdx.x == 0) atomicAdd(&a.counters[3], 1);
+    nc = a.key_cap / 2;
+  }
+
+  // ---- load (solver.py:160-175 effective inverse mass/inertia of kinematic bodies is zero)
+  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+    int b = b0 + lb;
+    bool kin = (d.body_flags[b] & BODY_KINEMATIC) != 0;
+    L.flags[lb] = d.body_flags[b];
+    L.inv_m[lb] = kin ? 0.0f : d.body_inv_mass[b];
+    for (int k = 0; k < 9; ++k) L.inv_I[9 * lb + k] = kin ? 0.0f : d.body_inv_inertia[9 * b + k];
+    for (int k = 0; k < 3; ++k) L.com[3 * lb + k] = d.body_com[3 * b + k];
+    for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = a.q_in[7 * b + k];
+    for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = a.qd_in[6 * b + k];
+  }
+  __syncthreads();
+
+  // ---- body_f + joint forces in joint order (kernels.py:890-1004)
+  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+    int b = b0 + lb;
+    Spatial f{load3(a.f_in + 6 * b), load3(a.f_in + 6 * b + 3)};
+    if (d.joint_count) {
+      for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
+        int ent = d.body_joint_entry[e];
+        int j = ent >> 1, side = ent & 1;
+        int type = d.joint_type[j];
+        if (!d.joint_enabled[j] || type == JT_FIXED || type == JT_CABLE) continue;
+        int id_p = d.joint_parent[j], id_c = d.joint_child[j];
+        int qs = d.joint_qd_start[j];
+        if (type == JT_FREE || type == JT_DISTANCE) {
+          f3 ft{a.joint_f[qs], a.joint_f[qs + 1], a.joint_f[qs + 2]};
+          f3 tt{a.joint_f[qs + 3], a.joint_f[qs + 4], a.joint_f[qs + 5]};
+          if (side == 1) { f.lin = f.lin + ft; f.ang = f.ang + tt; }
+          else if (id_p >= 0) { f.lin = f.lin - ft; f.ang = f.ang - tt; }
+          continue;
+        }
+        xform X_wp = load_xform(d.joint_X_p + 7 * j), pose_p = X_wp;
+        f3 com_p{0.0f, 0.0f, 0.0f};
+        if (id_p >= 0) {
+          pose_p = lds_q(L, id_p - b0);
+          X_wp = xmul(pose_p, X_wp);
+          com_p = load3(L.com + 3 * (id_p - b0));
+        }
+        f3 r_p = X_wp.p - xpoint(pose_p, com_p);
+        xform pose_c = lds_q(L, id_c - b0);
+        xform X_wc = xmul(pose_c, load_xform(d.joint_X_c + 7 * j));
+        f3 r_c = X_wc.p - xpoint(pose_c, load3(L.com + 3 * (id_c - b0)));
+        int lin_n = d.joint_dof_dim[2 * j], ang_n = d.joint_dof_dim[2 * j + 1];
+        f3 ft{0.0f, 0.0f, 0.0f}, tt{0.0f, 0.0f, 0.0f};
+        if (type == JT_BALL) {
+          tt = f3{a.joint_f[qs], a.joint_f[qs + 1], a.joint_f[qs + 2]};
+        } else {
+          for (int k = 0; k < lin_n && k < 3; ++k)
+            ft = ft + a.joint_f[qs + k] * xvector(X_wp, load3(d.joint_axis + 3 * (qs + k)));
+          for (int k = 0; k < ang_n && k < 3; ++k)
+            tt = tt + a.joint_f[qs + lin_n + k] * xvector(X_wp, load3(d.joint_axis + 3 * (qs + lin_n + k)));
+        }
+        if (side == 0) { f.lin = f.lin - ft; f.ang = f.ang - (tt + cross(r_p, ft)); }
+        else { f.lin = f.lin + ft; f.ang = f.ang + (tt + cross(r_c, ft)); }
+      }
+    }
+    store3(fl + 6 * lb, f.lin);
+    store3(fl + 6 * lb + 3, f.ang);
+  }
+  __syncthreads();
+
+  // ---- integrate in place (solver.py:98-157)
+  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+    int b = b0 + lb;
+    if (L.flags[lb] & BODY_KINEMATIC) continue;
+    int wi = d.body_world[b];
+    int gi = wi > 0 ? wi : 0;
+    Spatial f{load3(fl + 6 * lb), load3(fl + 6 * lb + 3)};
+    BodyState o = integrate_body(lds_state(L, lb), f, load3(L.com + 3 * lb), load_m33(d.body_inertia + 9 * b),
+                                 d.body_inv_mass[b], load_m33(d.body_inv_inertia + 9 * b), load3(d.gravity + 3 * gi),
+                                 a.p.angular_damping, dt);
+    lds_store_state(L, lb, o);
+  }
+
+  // ---- (contact, side) entries sorted by (body, contact): each body's contributions in contact order
+  if (has_c && nc > 0) {
+    int ne = 2 * nc;
+    int n2 = 1;
+    while (n2 < ne) n2 <<= 1;
+    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
+      unsigned key = 0xffffffffu;
+      if (e < ne) {
+        int c = e >> 1;
+        int s = (e & 1) ? a.c.shape1[cstart + c] : a.c.shape0[cstart + c];
+        int b = s >= 0 ? d.shape_body[s] : -1;
+        int lb = b >= 0 ? b - b0 : -1;
+        if (lb >= 0 && lb < nb) key = (static_cast<unsigned>(lb) << 20) | static_cast<unsigned>(e);
+      }
+      keys[e] = key;
+    }
+    __syncthreads();
+    sort_keys(keys, n2);
+    for (int lb = threadIdx.x; lb <= nb; lb += blockDim.x) {
+      unsigned target = static_cast<unsigned>(lb) << 20;
+      int lo = 0, hi = n2;
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (keys[mid] < target) lo = mid + 1; else hi = mid;
+      }
+      seg[lb] = lo;
+    }
+  }
+  __syncthreads();
+
+  const float relax = a.p.rigid_contact_relaxation;
+  for (int it = 0; it < a.p.iterations; ++it) {
+    // ---------------- contacts (kernels.py:2062-2299) + apply (kernels.py:809-879)
+    if (has_c) {
+      Spatial acc[BPL];
+      float cnt[BPL];
+      int ep[BPL], ee[BPL];
+#pragma unroll
+      for (int r = 0; r < BPL; ++r) {
+        int lb = threadIdx.x + r * XB;
+        acc[r] = Spatial{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        cnt[r] = 0.0f;
+        ep[r] = (nc > 0 && lb < nb) ? seg[lb] : 0;
+        ee[r] = (nc > 0 && lb < nb) ? seg[lb + 1] : 0;
+      }
+      for (int cb = 0; cb < nc; cb += XB) {
+        int c = cb + threadIdx.x;
+        int act = 0;
+        if (c < nc) {
+          int gc = cstart + c;
+          int s0 = a.c.shape0[gc], s1 = a.c.shape1[gc];
+          int ba = s0 >= 0 ? d.shape_body[s0] : -1;
+          int bb = s1 >= 0 ? d.shape_body[s1] : -1;
+          if (s0 != s1 && ba != bb) {
+            int la = ba >= 0 ? ba - b0 : -1;
+            int lbb = bb >= 0 ? bb - b0 : -1;
+            if (la >= nb) la = -1;  // bodies outside this world never appear in its segment
+            if (lbb >= nb) lbb = -1;
+            int nz = 0;
+            float mu = 0.0f, mut = 0.0f, mur = 0.0f;
+            if (s0 >= 0) { nz += 1; mu += d.shape_mu[s0]; mut += d.shape_mu_torsional[s0]; mur += d.shape_mu_rolling[s0]; }
+            if (s1 >= 0) { nz += 1; mu += d.shape_mu[s1]; mut += d.shape_mu_torsional[s1]; mur += d.shape_mu_rolling[s1]; }
+            if (nz > 0) { mu /= (float)nz; mut /= (float)nz; mur /= (float)nz; }
+            Spatial da, db;
+            if (contact_solve(L, la, lbb, load3(a.c.point0 + 3 * gc), load3(a.c.point1 + 3 * gc),
+                              load3(a.c.offset0 + 3 * gc), load3(a.c.offset1 + 3 * gc), load3(a.c.normal + 3 * gc),
+                              a.c.margin0[gc], a.c.margin1[gc], mu, mut, mur, relax, dt, da, db)) {
+              act = 1;
+              float* o = contrib + 12 * threadIdx.x;
+              store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
+            }
+          }
+        }
+        active[threadIdx.x] = act;
+        __syncthreads();
+        const unsigned cend = static_cast<unsigned>(cb + XB);
+#pragma unroll
+        for (int r = 0; r < BPL; ++r) {
+          while (ep[r] < ee[r]) {
+            unsigned ent = keys[ep[r]] & 0xfffffu;
+            unsigned cc = ent >> 1;
+            if (cc >= cend) break;
+            int slot = static_cast<int>(cc) - cb;
+            if (active[slot]) {
+              const float* o = contrib + 12 * slot + 6 * (ent & 1u);
+              acc[r].lin = acc[r].lin + load3(o);
+              acc[r].ang = acc[r].ang + load3(o + 3);
+              cnt[r] += 1.0f;
+            }
+            ep[r] += 1;
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < BPL; ++r) {
+        int lb = threadIdx.x + r * XB;
+        if (lb < nb) {
+          int b = b0 + lb;
+          BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), load_m33(d.body_inertia + 9 * b),
+                                     L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc[r],
+                                     a.p.rigid_contact_con_weighting != 0, cnt[r], dt);
+          lds_store_state(L, lb, o);
+        }
+      }
+      __syncthreads();
+    }
+    // ---------------- joints (kernels.py:1441-1943) + apply without weights
+    if (d.joint_count) {
+      for (int jl = threadIdx.x; jl < nj; jl += blockDim.x) {
+        int j = j0 + jl;
+        int id_p = d.joint_parent[j], id_c = d.joint_child[j];
+        Spatial dp, dc;
+        bool ok = joint_solve(d, L, j, id_p >= 0 ? id_p - b0 : -1, id_c - b0, a.tpos, a.tvel,
+                              a.p.joint_linear_compliance, a.p.joint_angular_compliance, a.p.joint_angular_relaxation,
+                              a.p.joint_linear_relaxation, dt, dp, dc);
+        jactive[jl] = ok ? 1 : 0;
+        if (ok) {
+          float* o = jcontrib + 12 * jl;
+          store3(o, dp.lin); store3(o + 3, dp.ang); store3(o + 6, dc.lin); store3(o + 9, dc.ang);
+        }
+      }
+      __syncthreads();
+      for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+        int b = b0 + lb;
+        Spatial acc{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+        for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
+          int ent = d.body_joint_entry[e];
+          int jl = (ent >> 1) - j0;
+          if (!jactive[jl]) continue;
+          const float* o = jcontrib + 12 * jl + 6 * (ent & 1);
+          acc.lin = acc.lin + load3(o);
+          acc.ang = acc.ang + load3(o + 3);
+        }
+        BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), load_m33(d.body_inertia + 9 * b),
+                                   L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt);
+        lds_store_state(L, lb, o);
+      }
+      __syncthreads();
+    }
+  }
+
+  // ---- store (copy_kinematic_body_state_kernel, kernels.py:18-31: kinematic bodies pass through)
+  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+    int b = b0 + lb;
+    if (L.flags[lb] & BODY_KINEMATIC) {
+      for (int k = 0; k < 7; ++k) a.q_out[7 * b + k] = a.q_in[7 * b + k];
+      for (int k = 0; k < 6; ++k) a.qd_out[6 * b + k] = a.qd_in[6 * b + k];
+    } else {
+      for (int k = 0; k < 7; ++k) a.q_out[7 * b + k] = L.q[7 * lb + k];
+      for (int k = 0; k < 6; ++k) a.qd_out[6 * b + k] = L.qd[6 * lb + k];
+    }
+  }
+}
+
+size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap) {
+  size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 1;
+  size_t b = sizeof(float) * (7 + 6 + 6 + 3 + 1 + 9) * mb + sizeof(int) * mb + sizeof(int) * (mb + 1);
+  b += sizeof(float) * 12 * XB + sizeof(int) * XB + sizeof(float) * 12 * mj + sizeof(int) * mj;
+  b += sizeof(unsigned) * (size_t)key_cap;
+  return (b + 15) & ~size_t(15);
+}
+
+hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
+                       const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
+                       const NmcContacts* c, float dt, float* q_out, float* qd_out, hipStream_t stream) {
+  XpbdArgs a;
+  a.d = m->d;
+  a.p = *params;
+  a.q_in = q_in;
+  a.qd_in = qd_in;
+  a.f_in = f_in;
+  a.joint_f = joint_f;
+  a.tpos = tpos;
+  a.tvel = tvel;
+  if (c) a.c = *c; else { a.c = NmcContacts{}; a.p.has_contacts = 0; }
+  a.dt = dt;
+  a.q_out = q_out;
+  a.qd_out = qd_out;
+  a.key_cap = m->xpbd_contact_cap;
+  a.counters = m->counters;
+  int worlds = m->d.world_count + 1;
+  hipLaunchKernelGGL(xpbd_world_kernel, dim3(worlds), dim3(XB), m->xpbd_lds, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace nmc

@@ -1,0 +1,105 @@
+"""``solvers.SolverXPBD``: the XPBD rigid-body step on the GPU.
+
+Mirrors newton/_src/solvers/xpbd/solver_xpbd.py (``SolverXPBD``, line 33): constructor defaults
+(``:95-109``), ``step(state_in, state_out, control, contacts, dt)`` (``:259-764``) and
+``notify_model_changed`` (``:141``).  One ``step`` is one launch of the per-world kernel in libnmc.so:
+joint forces, integration, ``iterations`` x (contact solve + apply, joint solve + apply) and the store.
+
+Differences in observable behaviour, both deliberate:
+* ``state_in`` is read-only here; the reference uses ``state_in.body_q/qd`` as ping-pong scratch
+  (solver_xpbd.py:218-230) so its contents after ``step`` are unspecified there.
+* per-body sums over contacts and joints run in canonical (contact / joint index) order instead of
+  float atomics, so results do not depend on thread scheduling.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from ._native import NativeModel, NmcXpbdParams, check, load, ptr, stream_handle
+
+
+class SolverNotifyFlags:
+    """newton/_src/solvers/flags.py:10-56."""
+
+    JOINT_PROPERTIES = 1 << 0
+    JOINT_DOF_PROPERTIES = 1 << 1
+    BODY_PROPERTIES = 1 << 2
+    BODY_INERTIAL_PROPERTIES = 1 << 3
+    SHAPE_PROPERTIES = 1 << 4
+    MODEL_PROPERTIES = 1 << 5
+
+
+class SolverBase:
+    def __init__(self, model):
+        self.model = model
+
+    @property
+    def device(self):
+        return self.model.device
+
+
+class SolverXPBD(SolverBase):
+    def __init__(self, model, iterations: int = 2, soft_body_relaxation: float = 0.9,
+                 soft_contact_relaxation: float = 0.9, joint_linear_relaxation: float = 0.7,
+                 joint_angular_relaxation: float = 0.4, joint_linear_compliance: float = 0.0,
+                 joint_angular_compliance: float = 0.0, rigid_contact_relaxation: float = 0.8,
+                 rigid_contact_con_weighting: bool = True, angular_damping: float = 0.0,
+                 enable_restitution: bool = False):
+        super().__init__(model)
+        if enable_restitution:
+            raise NotImplementedError("restitution is not part of this build")
+        self.iterations = int(iterations)
+        self.soft_body_relaxation = soft_body_relaxation
+        self.soft_contact_relaxation = soft_contact_relaxation
+        self.joint_linear_relaxation = joint_linear_relaxation
+        self.joint_angular_relaxation = joint_angular_relaxation
+        self.joint_linear_compliance = joint_linear_compliance
+        self.joint_angular_compliance = joint_angular_compliance
+        self.rigid_contact_relaxation = rigid_contact_relaxation
+        self.rigid_contact_con_weighting = rigid_contact_con_weighting
+        self.angular_damping = angular_damping
+        self.enable_restitution = enable_restitution
+        self.native = NativeModel(model, rigid_contact_max=1, shape_pairs_max=1)
+        self._lib = load()
+        self._control = None
+
+    def notify_model_changed(self, flags: int) -> None:
+        """Kinematic flags and inverse masses are read on every step; nothing is cached."""
+
+    def params(self, has_contacts: bool) -> NmcXpbdParams:
+        return NmcXpbdParams(self.iterations, self.rigid_contact_relaxation, int(self.rigid_contact_con_weighting),
+                             self.angular_damping, self.joint_linear_relaxation, self.joint_angular_relaxation,
+                             self.joint_linear_compliance, self.joint_angular_compliance, int(has_contacts))
+
+    def step(self, state_in, state_out, control, contacts, dt: float) -> None:
+        """solver_xpbd.py:259-764."""
+        model = self.model
+        if control is None:
+            if self._control is None:
+                self._control = model.control(clone_variables=False)
+            control = self._control
+        if state_in.body_q is state_out.body_q:
+            raise ValueError("state_in and state_out must be different State objects")
+        p = self.params(contacts is not None)
+        cv = ctypes.byref(contacts.native_view()) if contacts is not None else None
+        for t in (state_in.body_q, state_in.body_qd, state_in.body_f, state_out.body_q, state_out.body_qd):
+            if t.device != model.device or not t.is_contiguous():
+                raise ValueError("state arrays must be contiguous tensors on the model device")
+        rc = self._lib.nmc_xpbd_step(
+            self.native.handle, ctypes.byref(p), ctypes.c_void_p(ptr(state_in.body_q)),
+            ctypes.c_void_p(ptr(state_in.body_qd)), ctypes.c_void_p(ptr(state_in.body_f)),
+            ctypes.c_void_p(ptr(control.joint_f)), ctypes.c_void_p(ptr(control.joint_target_pos)),
+            ctypes.c_void_p(ptr(control.joint_target_vel)), cv, ctypes.c_float(dt),
+            ctypes.c_void_p(ptr(state_out.body_q)), ctypes.c_void_p(ptr(state_out.body_qd)),
+            ctypes.c_void_p(stream_handle(model.device)))
+        check(rc, "nmc_xpbd_step")
+
+    def update_contacts(self, contacts, state=None) -> None:
+        raise NotImplementedError("contact force reporting (update_contacts) is not part of this build yet")
+
+
+__all__ = ["SolverBase", "SolverNotifyFlags", "SolverXPBD"]
+_ = torch

@@ -767,10 +767,14 @@ template <class T> V2<T> intersection_point(V2<T> s, V2<T> e, V2<T> a, V2<T> b) 
   return (T(1) - interp) * a + interp * b;
 }
 
+// The clip buffer holds 12 entries (two convex polygons of <= 5 vertices meet in <= 10); the index
+// guards never fire on well-formed input.
+static const int kClipMax = 12;
+
 template <class T> void insert_v2(V2<T>* arr, int count, int index, V2<T> el) {
   int i = count;
   while (i > index) { arr[i] = arr[i - 1]; i -= 1; }
-  arr[index] = el;
+  if (index < kClipMax) arr[index] = el;
 }
 
 template <class T> int trim_in_place(V2<T> s, V2<T> e, V2<T>* loop, int loop_count) {
@@ -795,25 +799,25 @@ template <class T> int trim_in_place(V2<T> s, V2<T> e, V2<T>* loop, int loop_cou
     new_loop_count = loop_count;
     int i = 0;
     while (i < loop_count) {
-      if (keep) { li += 1; loop[li] = loop[i]; }
+      if (keep) { li += 1; if (li < kClipMax && i < kClipMax) loop[li] = loop[i]; }
       if (i == change_a || i == change_b) {
         V2<T> pt = i == change_a ? ia : ib;
         if (li == i && !keep) {
           li += 1;
-          insert_v2(loop, new_loop_count, li, pt);
+          insert_v2(loop, new_loop_count < kClipMax - 1 ? new_loop_count : kClipMax - 1, li, pt);
           new_loop_count += 1;
           i += 1;
           change_b += 1;
           loop_count += 1;
         } else {
           li += 1;
-          loop[li] = pt;
+          if (li < kClipMax) loop[li] = pt;
         }
         keep = !keep;
       }
       i += 1;
     }
-    new_loop_count = li + 1;
+    new_loop_count = li + 1 < kClipMax ? li + 1 : kClipMax;
   } else if (prev_outside) {
     new_loop_count = 0;
   } else {
