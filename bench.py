@@ -1,0 +1,236 @@
+"""World-steps/s of the rigid-contact step on one MI355X.
+
+Workload: 4096 identical-structure worlds x 32 mixed primitives (11 spheres, 11 boxes, 10 capsules) dropped
+onto one global ground plane, per-world seeded poses, fp32, dt = 1/60 s as 4 substeps, 10 XPBD
+iterations; each substep is CollisionPipeline.collide + SolverXPBD.step, and one step (4 substeps,
+16 kernel launches) is captured in a hipGraph and replayed.  Synthetic scene, no external data.
+
+One JSON line is printed.  ``roofline`` prices the dominant kernel against HBM bandwidth with the
+algorithmic bytes of the API boundary (each substep's collide/step read their inputs and write their
+outputs once):
+    B_collide = 28 N_b + 64 N_s + 76 N_c + 4          bytes per world-substep
+    B_step    = 76 N_b + 96 N_b + 16 N_s + 76 N_c + 52 N_b + 120 N_j
+N_c is the measured mean contact count per world per substep in the timed region.
+``cpu_baseline`` times the C++ oracle (scalar restatement of the reference algorithm, 1 thread) on a
+bounded sample: a snapshot of the first worlds of the timed state, stepped on the host.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=240)
+    p.add_argument("--warmup", type=int, default=120)
+    p.add_argument("--worlds", type=int, default=4096)
+    p.add_argument("--iterations", type=int, default=10)
+    p.add_argument("--substeps", type=int, default=4)
+    p.add_argument("--seed", type=int, default=0)
+    p.add_argument("--no-graph", action="store_true")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--kernel-reps", type=int, default=20)
+    return p.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (worlds shard across "
+              "GPUs with no per-step exchange, so the multi-GPU figure is N x the one-GPU figure by construction)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from newton_custom_contact_amd import CollisionPipeline, scenes
+    from newton_custom_contact_amd.solvers import SolverXPBD
+
+    dev = "cuda:0"
+    t0 = time.time()
+    model = scenes.mixed_worlds(args.worlds, seed=args.seed, device=dev)
+    print(f"[bench] scene: {args.worlds} worlds, {model.body_count} bodies, {model.shape_count} shapes "
+          f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    pipe = CollisionPipeline(model, broad_phase="sap")
+    solver = SolverXPBD(model, iterations=args.iterations)
+    contacts = pipe.contacts()
+    states = [model.state(), model.state()]
+    dt = 1.0 / 60.0
+    sdt = dt / args.substeps
+    nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step():
+        for s in range(args.substeps):
+            a, b = states[s % 2], states[(s + 1) % 2]
+            pipe.collide(a, contacts)
+            solver.step(a, b, None, contacts, sdt)
+            nc_acc.add_(contacts.rigid_contact_count)
+        if args.substeps % 2:
+            states[0].body_q.copy_(states[1].body_q)
+            states[0].body_qd.copy_(states[1].body_qd)
+
+    torch.cuda.synchronize()
+    graph = None
+    if not args.no_graph:
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            step()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step()
+        torch.cuda.synchronize()
+
+    def run(n):
+        for _ in range(n):
+            if graph is not None:
+                graph.replay()
+            else:
+                step()
+
+    run(args.warmup)
+    torch.cuda.synchronize()
+    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
+    nc_acc.zero_()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    e0.record()
+    run(args.steps)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = e0.elapsed_time(e1)
+    ms_per_step = ms / args.steps
+    ws_per_s = args.worlds * args.steps / (ms / 1e3)
+    n_c = float(nc_acc.item()) / (args.steps * args.substeps * args.worlds)
+    print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.1f} ms), N_c={n_c:.1f}",
+          file=sys.stderr, flush=True)
+
+    # ---- per-kernel timing of the two phases on the launch stream (HIP events)
+    stream = torch.cuda.current_stream()
+    reps = args.kernel_reps
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4 * reps)]
+    a, b = states[0], states[1]
+    t_col, t_xp = [], []
+    for r in range(reps):
+        ev[4 * r].record(stream)
+        pipe.collide(a, contacts)
+        ev[4 * r + 1].record(stream)
+        ev[4 * r + 2].record(stream)
+        solver.step(a, b, None, contacts, sdt)
+        ev[4 * r + 3].record(stream)
+        a, b = b, a
+    torch.cuda.synchronize()
+    for r in range(reps):
+        t_col.append(ev[4 * r].elapsed_time(ev[4 * r + 1]))
+        t_xp.append(ev[4 * r + 2].elapsed_time(ev[4 * r + 3]))
+    t_col_ms = float(np.mean(t_col))
+    t_xp_ms = float(np.mean(t_xp))
+
+    N_b, N_s, N_j = 32, 33, 0
+    b_collide = 28 * N_b + 64 * N_s + 76 * n_c + 4
+    b_step = 76 * N_b + 96 * N_b + 16 * N_s + 76 * n_c + 52 * N_b + 120 * N_j
+    b_ws = args.substeps * (b_collide + b_step)
+    roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
+    if t_xp_ms >= t_col_ms:
+        dom, dom_ms, dom_bytes = "xpbd_world_kernel", t_xp_ms, args.worlds * b_step
+    else:
+        dom, dom_ms, dom_bytes = "collide (segment + scan + compact kernels)", t_col_ms, args.worlds * b_collide
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(model, states[0], args, sdt)
+
+    line = {
+        "metric": "world-steps/s @4096 worlds; iters to 1e-4 m pen residual, learned vs plain",
+        "value": round(ws_per_s, 1),
+        "unit": "world-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"mixed primitives: {args.worlds} worlds x 32 (11 spheres, 11 boxes, 10 capsules) on a "
+                        f"ground plane, dt=1/60 s, {args.substeps} substeps, {args.iterations} XPBD iterations, "
+                        f"{'hipGraph' if graph is not None else 'eager'} step",
+            "worlds": args.worlds,
+            "bodies_per_world": N_b,
+            "mean_contacts_per_world_substep": round(n_c, 2),
+            "bytes_per_world_step": round(b_ws, 1),
+            "hbm_roofline_world_steps_per_s": round(roof_ws, 1),
+            "fraction_of_step_roofline": round(ws_per_s / roof_ws, 5),
+            "collide_ms_per_substep": round(t_col_ms, 4),
+            "xpbd_ms_per_substep": round(t_xp_ms, 4),
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": round(achieved, 2),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5),
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def cpu_baseline(model, state, args, sdt):
+    """Oracle (C++ scalar restatement, 1 thread) on a snapshot of the first worlds of the timed state."""
+    import torch
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from newton_custom_contact_amd.model import Model
+    from oracle import Oracle
+
+    n_w = 32
+    nb = 32
+    q = state.body_q[: n_w * nb].cpu().numpy().reshape(n_w, nb, 7)
+    qd = state.body_qd[: n_w * nb].cpu().numpy().reshape(n_w * nb, 6)
+    sub = Model._replicate(model.template, n_w, q, "cpu")
+    o = Oracle(sub)
+    qq = sub.host["body_q"].copy()
+    f = np.zeros_like(qd)
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        for _ in range(args.substeps):
+            c = o.collide(qq)
+            qq, qd = o.step(qq, qd, f, c, sdt, iterations=args.iterations)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= args.cpu_seconds:
+            break
+    _ = torch
+    return {
+        "value": round(n_w * steps / el, 2),
+        "unit": "world-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{n_w} worlds x {steps} steps ({el:.1f} s) from the GPU run's timed state, C++ oracle, 1 thread",
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -165,11 +165,132 @@ class Model:
         m.shape_collision_filter_pairs = set(b.shape_collision_filter_pairs)
         fp = sorted(m.shape_collision_filter_pairs)
         h["shape_filter_pairs"] = arr(fp, _I32, (len(fp), 2))
+        m._finish()
+        return m
 
-        m._build_world_tables()
-        for name, a in h.items():
-            setattr(m, name, torch.from_numpy(np.ascontiguousarray(a)).to(m.device))
-        m.shape_contact_pair_count = 0
+    def _finish(self):
+        """Derive the per-world tables and upload every host array to the device."""
+        self._build_world_tables()
+        for name, a in self.host.items():
+            setattr(self, name, torch.from_numpy(np.ascontiguousarray(a)).to(self.device))
+        self.shape_contact_pair_count = 0
+
+    @classmethod
+    def _replicate(cls, template: Model, world_count: int, body_q: np.ndarray, device) -> Model:
+        """Tile a one-world template ``world_count`` times (ModelBuilder.replicate without per-entity
+        Python work).
+
+        ``template`` holds global entities (world -1, ordered first) and exactly one world (world 0);
+        ``body_q`` is ``(world_count, bodies_per_world, 7)``.  The result is identical to building the
+        same worlds one by one with ``begin_world``/``add_body``/``add_shape_*``/``end_world``.
+        """
+        th = template.host
+        W = int(world_count)
+        if template.world_count != 1:
+            raise ValueError("template must contain exactly one world")
+        bw, sw, jw = th["body_world"], th["shape_world"], th["joint_world"]
+        gb, gs, gj = np.nonzero(bw == -1)[0], np.nonzero(sw == -1)[0], np.nonzero(jw == -1)[0]
+        wb, ws, wj = np.nonzero(bw == 0)[0], np.nonzero(sw == 0)[0], np.nonzero(jw == 0)[0]
+        for g, w in ((gb, wb), (gs, ws), (gj, wj)):
+            if g.size and w.size and g.max() > w.min():
+                raise ValueError("template: global entities must precede the world's entities")
+        nb, ns, nj = wb.size, ws.size, wj.size
+        if body_q.shape != (W, nb, 7):
+            raise ValueError(f"body_q must be ({W}, {nb}, 7)")
+        m = cls(device)
+        m.world_count = W
+        m.up_axis = template.up_axis
+        m.body_count = gb.size + W * nb
+        m.shape_count = gs.size + W * ns
+        m.joint_count = gj.size + W * nj
+        D = len(th["joint_qd"])
+        C = len(th["joint_q"])
+        ndof_g = int(th["joint_qd_start"][wj[0]]) if nj else D
+        ndof_w = D - ndof_g
+        ncoord_g = int(th["joint_q_start"][wj[0]]) if nj else C
+        ncoord_w = C - ncoord_g
+        m.joint_dof_count = ndof_g + W * ndof_w
+        m.joint_coord_count = ncoord_g + W * ncoord_w
+        h = m.host
+        wid_b = np.repeat(np.arange(W, dtype=_I32), nb)
+        wid_s = np.repeat(np.arange(W, dtype=_I32), ns)
+        wid_j = np.repeat(np.arange(W, dtype=_I32), nj)
+        b_off = gb.size + np.arange(W, dtype=np.int64) * nb  # first body of each world
+
+        def tile(name, sel_g, sel_w):
+            a = th[name]
+            return np.concatenate([a[sel_g], np.tile(a[sel_w], (W,) + (1,) * (a.ndim - 1))])
+
+        for name in ("body_qd", "body_com", "body_mass", "body_inv_mass", "body_inertia", "body_inv_inertia",
+                     "body_flags"):
+            h[name] = tile(name, gb, wb)
+        h["body_q"] = np.concatenate([th["body_q"][gb], body_q.reshape(W * nb, 7).astype(_F32)])
+        h["body_world"] = np.concatenate([bw[gb], wid_b]).astype(_I32)
+        for name in ("shape_transform", "shape_type", "shape_scale", "shape_flags", "shape_margin", "shape_gap",
+                     "shape_collision_radius", "shape_collision_group", "shape_material_mu",
+                     "shape_material_mu_torsional", "shape_material_mu_rolling", "shape_material_restitution"):
+            h[name] = tile(name, gs, ws)
+        sb = th["shape_body"]
+        tb = sb[ws].astype(np.int64)
+        shape_body_w = np.where(tb >= 0, (tb - wb[0] if nb else 0)[None, :] + b_off[:, None], -1) if ns else \
+            np.zeros((W, 0), np.int64)
+        h["shape_body"] = np.concatenate([sb[gs], shape_body_w.reshape(-1)]).astype(_I32)
+        h["shape_world"] = np.concatenate([sw[gs], wid_s]).astype(_I32)
+        s_off = gs.size + np.arange(W, dtype=np.int64) * ns
+        # joints
+        for name in ("joint_type", "joint_enabled", "joint_X_p", "joint_X_c", "joint_dof_dim"):
+            h[name] = tile(name, gj, wj)
+        jp, jc = th["joint_parent"], th["joint_child"]
+        par_w = jp[wj].astype(np.int64)
+        chi_w = jc[wj].astype(np.int64)
+        h["joint_parent"] = np.concatenate(
+            [jp[gj], np.where(par_w >= 0, (par_w - (wb[0] if nb else 0))[None, :] + b_off[:, None], -1).reshape(-1)]
+        ).astype(_I32)
+        h["joint_child"] = np.concatenate(
+            [jc[gj], ((chi_w - (wb[0] if nb else 0))[None, :] + b_off[:, None]).reshape(-1)]).astype(_I32)
+        h["joint_world"] = np.concatenate([jw[gj], wid_j]).astype(_I32)
+        qds = th["joint_qd_start"].astype(np.int64)
+        qs = th["joint_q_start"].astype(np.int64)
+        dof_off = ndof_g + np.arange(W, dtype=np.int64) * ndof_w
+        crd_off = ncoord_g + np.arange(W, dtype=np.int64) * ncoord_w
+        h["joint_qd_start"] = np.concatenate(
+            [qds[gj], ((qds[wj] - ndof_g)[None, :] + dof_off[:, None]).reshape(-1)]).astype(_I32)
+        h["joint_q_start"] = np.concatenate(
+            [qs[gj], ((qs[wj] - ncoord_g)[None, :] + crd_off[:, None]).reshape(-1)]).astype(_I32)
+        for name in ("joint_axis", "joint_limit_lower", "joint_limit_upper", "joint_target_ke", "joint_target_kd",
+                     "joint_target_pos", "joint_target_vel", "joint_qd", "joint_f"):
+            a = th[name]
+            h[name] = np.concatenate([a[:ndof_g], np.tile(a[ndof_g:], (W,) + (1,) * (a.ndim - 1))])
+        jq = np.concatenate([th["joint_q"][:ncoord_g], np.tile(th["joint_q"][ncoord_g:], W)]).astype(_F32)
+        # FREE joints carry the child body pose as coordinates
+        free = np.nonzero(h["joint_type"] == int(JointType.FREE))[0]
+        if free.size:
+            st = h["joint_q_start"][free].astype(np.int64)
+            for k in range(7):
+                jq[st + k] = h["body_q"][h["joint_child"][free], k]
+        h["joint_q"] = jq
+        h["gravity"] = np.tile(th["gravity"][0:1], (W, 1)).astype(_F32)
+        fp = th["shape_filter_pairs"].astype(np.int64)
+        if fp.size:
+            is_w = sw[fp] >= 0
+            g_pairs = fp[~is_w.any(axis=1)]
+            w_pairs = fp[is_w.all(axis=1)] - ws[0]
+            mixed = fp[is_w.any(axis=1) & ~is_w.all(axis=1)]
+            parts = [g_pairs]
+            if w_pairs.size:
+                parts.append((w_pairs[None, :, :] + s_off[:, None, None]).reshape(-1, 2))
+            if mixed.size:
+                mw = np.where(sw[mixed] >= 0, mixed - ws[0], -1)
+                rep = np.where(mw[None] >= 0, mw[None] + s_off[:, None, None], mixed[None])
+                parts.append(np.sort(rep, axis=2).reshape(-1, 2))
+            allp = np.concatenate(parts).astype(_I32)
+            allp = np.unique(allp, axis=0)
+            h["shape_filter_pairs"] = allp
+        else:
+            h["shape_filter_pairs"] = np.zeros((0, 2), _I32)
+        m.shape_collision_filter_pairs = {tuple(map(int, p)) for p in h["shape_filter_pairs"]} \
+            if len(h["shape_filter_pairs"]) < 100000 else set()
+        m._finish()
         return m
 
     def _build_world_tables(self):
@@ -186,54 +307,63 @@ class Model:
         sw = h["shape_world"]
         jw = h["joint_world"]
 
-        def csr(ids, n_world, what):
-            start = np.zeros(n_world + 1, dtype=_I32)
-            local = ids[ids >= 0]
-            counts = np.bincount(local, minlength=n_world)[:n_world] if local.size else np.zeros(n_world, int)
-            start[1:] = np.cumsum(counts)
-            # entities of world w must be contiguous: [start[w], start[w+1])
+        def ranges(ids, what):
+            """first index and count of each world's entities (entities of a world are contiguous)."""
+            first = np.zeros(W + 1, dtype=_I32)
+            count = np.zeros(W + 1, dtype=_I32)
             idx = np.nonzero(ids >= 0)[0]
-            if idx.size and not np.all(np.diff(ids[idx]) >= 0):
-                raise ValueError(f"{what} of each world must be contiguous and in world order")
-            return start, idx
+            if idx.size:
+                wid = ids[idx]
+                if np.any(np.diff(wid) < 0):
+                    raise ValueError(f"{what} of each world must be contiguous and in world order")
+                cnt = np.bincount(wid, minlength=W)[:W]
+                starts = np.searchsorted(wid, np.arange(W))
+                first[:W] = np.where(cnt > 0, idx[np.minimum(starts, idx.size - 1)], 0)
+                count[:W] = cnt
+                # contiguity inside a world: last - first + 1 == count
+                ends = np.where(cnt > 0, idx[np.minimum(starts + cnt - 1, idx.size - 1)], -1)
+                if np.any((cnt > 0) & (ends - first[:W] + 1 != cnt)):
+                    raise ValueError(f"{what} of a world must be contiguous")
+            return first, count
 
-        self.world_body_start, body_idx = csr(bw, W, "bodies")
-        self.world_shape_start, _ = csr(sw, W, "shapes")
-        self.world_joint_start, _ = csr(jw, W, "joints")
-        # bodies of world w are contiguous; their first global index
-        first_body = np.full(W + 1, 0, dtype=_I32)
-        for w in range(W):
-            sel = np.nonzero(bw == w)[0]
-            first_body[w] = sel[0] if sel.size else 0
-        h["world_body_first"] = first_body
-        sel = np.nonzero(sw >= 0)[0]
-        first_shape = np.zeros(W + 1, dtype=_I32)
-        for w in range(W):
-            s = np.nonzero(sw == w)[0]
-            first_shape[w] = s[0] if s.size else 0
-        h["world_shape_first"] = first_shape
-        first_joint = np.zeros(W + 1, dtype=_I32)
-        for w in range(W):
-            s = np.nonzero(jw == w)[0]
-            first_joint[w] = s[0] if s.size else 0
-        h["world_joint_first"] = first_joint
-        h["world_body_count"] = np.diff(self.world_body_start).astype(_I32) if W else np.zeros(0, _I32)
-        h["world_shape_count"] = np.diff(self.world_shape_start).astype(_I32) if W else np.zeros(0, _I32)
-        h["world_joint_count"] = np.diff(self.world_joint_start).astype(_I32) if W else np.zeros(0, _I32)
+        h["world_body_first"], h["world_body_count"] = ranges(bw, "bodies")
+        h["world_shape_first"], h["world_shape_count"] = ranges(sw, "shapes")
+        h["world_joint_first"], h["world_joint_count"] = ranges(jw, "joints")
 
         colliding = (h["shape_flags"] & int(ShapeFlags.COLLIDE_SHAPES)) != 0
         glob = np.nonzero(colliding & (sw == -1))[0].astype(_I32)
-        seg_start = [0]
-        seg_shape = []
-        for w in range(W):
-            own = np.nonzero(colliding & (sw == w))[0].astype(_I32)
-            seg = np.sort(np.concatenate([own, glob]))
-            seg_shape.append(seg)
-            seg_start.append(seg_start[-1] + len(seg))
-        seg_shape.append(glob)
-        seg_start.append(seg_start[-1] + len(glob))
-        h["segment_start"] = np.asarray(seg_start, dtype=_I32)
-        h["segment_shape"] = (np.concatenate(seg_shape) if seg_shape else np.zeros(0)).astype(_I32)
+        own_idx = np.nonzero(colliding & (sw >= 0))[0].astype(_I32)
+        own_w = sw[own_idx]
+        # segment w = sorted(world w colliding shapes + global colliding shapes); last segment = globals
+        order = np.lexsort((own_idx, own_w))
+        own_idx, own_w = own_idx[order], own_w[order]
+        own_cnt = np.bincount(own_w, minlength=W)[:W] if W else np.zeros(0, np.int64)
+        seg_len = np.concatenate([own_cnt + len(glob), [len(glob)]]).astype(np.int64)
+        seg_start = np.zeros(W + 2, dtype=np.int64)
+        seg_start[1:] = np.cumsum(seg_len)
+        seg_shape = np.empty(int(seg_start[-1]), dtype=_I32)
+        if W:
+            own_start = np.zeros(W + 1, np.int64)
+            own_start[1:] = np.cumsum(own_cnt)
+            # merged order inside a segment: position of every own shape and every global shape
+            rank_own = np.arange(own_idx.size) - own_start[own_w]
+            g_before_own = np.searchsorted(glob, own_idx)  # globals with smaller index
+            seg_shape[seg_start[own_w] + rank_own + g_before_own] = own_idx
+            if len(glob):
+                for_w = np.repeat(np.arange(W), len(glob))
+                gl = np.tile(glob, W)
+                gr = np.tile(np.arange(len(glob)), W)
+                # own colliding shapes of world w with a smaller index than global shape g: world shapes
+                # occupy one contiguous index block, so a prefix count over colliding world shapes answers it
+                pref = np.zeros(len(sw) + 1, np.int64)
+                pref[1:] = np.cumsum(colliding & (sw >= 0))
+                first = h["world_shape_first"][:W].astype(np.int64)
+                own_before = pref[gl] - pref[first[for_w]]
+                own_before = np.clip(own_before, 0, own_cnt[for_w])
+                seg_shape[seg_start[for_w] + gr + own_before] = gl
+        seg_shape[seg_start[W]:] = glob
+        h["segment_start"] = seg_start.astype(_I32)
+        h["segment_shape"] = seg_shape
         h["global_shape_count"] = np.asarray([len(glob)], dtype=_I32)
         self.global_dynamic_bodies = int(np.count_nonzero((bw == -1) & (h["body_inv_mass"] > 0.0)))
 

@@ -116,10 +116,35 @@ def _add_primitive(b: ModelBuilder, kind, dims, p, q, cfg):
 
 
 def mixed_worlds(world_count: int = 4096, seed: int = 0, mu: float = 0.5, device=None,
-                 n_sphere: int = 11, n_box: int = 11, n_capsule: int = 10):
-    """``world_count`` worlds of the same 32 primitives with per-world seeded poses."""
+                 n_sphere: int = 11, n_box: int = 11, n_capsule: int = 10, fast: bool = True):
+    """``world_count`` worlds of the same 32 primitives with per-world seeded poses.
+
+    ``fast`` builds one template world through the builder and tiles it (Model._replicate) with the
+    poses drawn in exactly the order the per-world builder loop draws them, so both paths give the
+    same model.
+    """
     rng = np.random.Generator(np.random.PCG64(seed))
     sizes = _primitive_sizes(rng, n_sphere, n_box, n_capsule)
+    if fast:
+        from .model import Model  # noqa: PLC0415
+
+        poses = np.zeros((world_count, len(sizes), 7))
+        for w in range(world_count):
+            placer = _Placer(rng)
+            for i, (kind, dims) in enumerate(sizes):
+                poses[w, i, :3] = placer.place(_bound_radius(kind, dims))
+                poses[w, i, 3:] = shoemake_quat(rng)
+        t = ModelBuilder()
+        cfg = _cfg(mu)
+        t.add_ground_plane(cfg=_cfg(mu))
+        t.begin_world()
+        for kind, dims in sizes:
+            _add_primitive(t, kind, dims, np.zeros(3), np.array([0.0, 0.0, 0.0, 1.0]), cfg)
+        t.end_world()
+        template = t.finalize("cpu")
+        model = Model._replicate(template, world_count, poses, device)
+        model.template = template
+        return model
     b = ModelBuilder()
     cfg = _cfg(mu)
     b.add_ground_plane(cfg=_cfg(mu))
