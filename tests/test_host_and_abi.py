@@ -1,0 +1,131 @@
+"""Host logic and the C-ABI library surface (CPU only: no kernel launches)."""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from newton_custom_contact_amd import _native, scenes
+from newton_custom_contact_amd.builder import ModelBuilder
+from newton_custom_contact_amd.collide import estimate_rigid_contact_max, per_world_shape_pairs_max
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    header = open(os.path.join(ROOT, "include", "nmc.h")).read()
+    declared = set(re.findall(r"\b(nmc_[a-z_]+)\s*\(", header))
+    assert declared == set(_native.EXPORTS)
+    lib = ctypes.CDLL(_native.lib_path())
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert _native.load().nmc_last_error() == b""
+
+
+def test_structs_match_header_field_order():
+    header = open(os.path.join(ROOT, "include", "nmc.h")).read()
+
+    def fields(struct):
+        body = re.search(r"typedef struct " + struct + r" \{(.*?)\} " + struct + ";", header, re.S).group(1)
+        body = re.sub(r"/\*.*?\*/", "", body, flags=re.S)
+        names = []
+        for decl in body.split(";"):
+            decl = decl.strip()
+            if not decl:
+                continue
+            parts = decl.split(",")
+            first = re.findall(r"[A-Za-z_][A-Za-z0-9_]*", parts[0])[-1]
+            names.append(first)
+            names += [re.findall(r"[A-Za-z_][A-Za-z0-9_]*", p)[-1] for p in parts[1:]]
+        return names
+
+    for cname, pyc in (("NmcModelDesc", _native.NmcModelDesc), ("NmcContacts", _native.NmcContacts),
+                       ("NmcXpbdParams", _native.NmcXpbdParams)):
+        assert fields(cname) == [f[0] for f in pyc._fields_], cname
+
+
+def test_fast_replication_equals_builder_loop():
+    slow = scenes.mixed_worlds(7, seed=11, device="cpu", fast=False)
+    fast = scenes.mixed_worlds(7, seed=11, device="cpu", fast=True)
+    assert set(slow.host) == set(fast.host)
+    for k in slow.host:
+        assert np.array_equal(slow.host[k], fast.host[k]), k
+
+
+def test_world_tables_and_segments():
+    m = scenes.uneven_worlds(6, seed=2, device="cpu", chain_links=3)
+    h = m.host
+    sw = h["shape_world"]
+    glob = np.nonzero(sw == -1)[0]
+    seg = [np.sort(np.concatenate([np.nonzero(sw == w)[0], glob])) for w in range(m.world_count)] + [glob]
+    assert np.array_equal(h["segment_shape"], np.concatenate(seg))
+    assert np.array_equal(np.diff(h["segment_start"]), [len(s) for s in seg])
+    for w in range(m.world_count):
+        bodies = np.nonzero(h["body_world"] == w)[0]
+        assert h["world_body_first"][w] == bodies[0] and h["world_body_count"][w] == len(bodies)
+        joints = np.nonzero(h["joint_world"] == w)[0]
+        assert h["world_joint_first"][w] == joints[0] and h["world_joint_count"][w] == len(joints)
+    # revolute chain links filter their parent's shapes (builder.py add_joint collision_filter_parent)
+    jt = h["joint_type"]
+    rev = np.nonzero(jt == 1)[0]
+    assert len(rev) == 3 * m.world_count
+    fp = {tuple(p) for p in h["shape_filter_pairs"].tolist()}
+    for j in rev:
+        p, c = h["joint_parent"][j], h["joint_child"][j]
+        if p < 0:
+            continue
+        sp = np.nonzero(h["shape_body"] == p)[0][0]
+        sc = np.nonzero(h["shape_body"] == c)[0][0]
+        assert (min(sp, sc), max(sp, sc)) in fp
+
+
+def test_capacity_estimates_follow_reference_formulas():
+    m = scenes.mixed_worlds(16, seed=0, device="cpu")
+    # collide.py:278-376: 32 primitives/world, 20 neighbours x 5 cpp / 2, plus plane pairs x avg cpp
+    n_prim = 32 * 16
+    expected = (n_prim * 20 * 5) // 2 + (1 * 32 * 16) * 5
+    assert estimate_rigid_contact_max(m) == max(1000, expected)
+    # collide.py:379-418: each segment (32 + plane) contributes n(n-1)/2, plus the global segment
+    assert per_world_shape_pairs_max(m) == 16 * (33 * 32 // 2)
+
+
+def test_model_state_and_control_factories():
+    m = scenes.box_pile(4, device="cpu")
+    s = m.state()
+    assert s.body_q.shape == (4, 7) and s.body_qd.shape == (4, 6) and s.body_f.shape == (4, 6)
+    s.body_f += 1
+    s.clear_forces()
+    assert float(s.body_f.abs().sum()) == 0.0
+    s2 = m.state()
+    s2.body_q += 1
+    s2.assign(s)
+    assert np.array_equal(s2.body_q.numpy(), s.body_q.numpy())
+    c = m.control()
+    assert c.joint_f.shape == (m.joint_dof_count,)
+
+
+def test_gpu_only_product_path_fails_loudly_on_cpu():
+    from newton_custom_contact_amd import CollisionPipeline
+    from newton_custom_contact_amd.solvers import SolverXPBD
+
+    m = scenes.box_pile(4, device="cpu")
+    with pytest.raises(RuntimeError, match="GPU only"):
+        CollisionPipeline(m, broad_phase="sap")
+    with pytest.raises(RuntimeError, match="GPU only"):
+        SolverXPBD(m)
+
+
+def test_builder_rejects_cross_world_joint():
+    b = ModelBuilder()
+    b.begin_world()
+    a = b.add_body()
+    b.end_world()
+    b.begin_world()
+    c = b.add_body()
+    with pytest.raises(ValueError):
+        b.add_joint_revolute(a, c)
+    b.end_world()
