@@ -13,7 +13,7 @@ import numpy as np
 import torch
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "libnmc.so")
+_LIB_PATH = os.environ.get("NMC_LIBRARY", os.path.join(_HERE, "libnmc.so"))
 _lib = None
 
 _vp = ctypes.c_void_p

@@ -73,8 +73,10 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   long long cap2 = 2ll * (d.rigid_contact_max > 0 ? d.rigid_contact_max : 1) + 5ll * m->pair_cap;
   if (slots > cap2) slots = cap2;
   m->stage.slot_capacity = slots;
-  // solver ordering buffer: 2 entries per contact of the densest world
-  m->xpbd_contact_cap = 8192;
+  // solver ordering buffer: 2 entries per contact; worlds with up to 512 contacts order them in LDS,
+  // denser worlds in a global buffer of 2 entries per contact slot
+  m->xpbd_contact_cap = 1024;
+  m->xpbd_keys_cap = 2ll * (d.rigid_contact_max > 0 ? d.rigid_contact_max : 1);
   m->xpbd_lds = xpbd_lds_bytes(d.max_world_bodies, d.max_world_joints, m->xpbd_contact_cap);
   if (m->collide_lds > 160 * 1024 || m->xpbd_lds > 160 * 1024) {
     delete m;
@@ -88,7 +90,8 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       (e = dmalloc(&s.margin1, slots)) != hipSuccess || (e = dmalloc(&s.sub_key, slots)) != hipSuccess ||
       (e = dmalloc(&s.mask, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&m->seg_pair_count, segs)) != hipSuccess ||
       (e = dmalloc(&m->seg_pair_base, segs)) != hipSuccess || (e = dmalloc(&m->seg_contact_count, segs)) != hipSuccess ||
-      (e = dmalloc(&m->stage_alloc, 1)) != hipSuccess || (e = dmalloc(&m->counters, 4)) != hipSuccess) {
+      (e = dmalloc(&m->stage_alloc, 1)) != hipSuccess || (e = dmalloc(&m->counters, 4)) != hipSuccess ||
+      (e = dmalloc(&m->xpbd_keys, m->xpbd_keys_cap)) != hipSuccess) {
     nmc_model_destroy(m);
     return hip_fail(e, "nmc_model_create: device allocation");
   }
@@ -101,7 +104,7 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
   StageBuffers& s = m->stage;
   void* ptrs[] = {s.shape0, s.shape1, s.point0, s.point1, s.offset0, s.offset1, s.normal, s.margin0, s.margin1,
                   s.sub_key, s.mask, m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc,
-                  m->counters};
+                  m->counters, m->xpbd_keys};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;

@@ -47,7 +47,9 @@ struct NmcModelImpl {
   int pair_cap;          // per-segment pair buffer (LDS) capacity
   size_t collide_lds;
   size_t xpbd_lds;
-  int xpbd_contact_cap;  // per-world contacts the solver orders in LDS
+  int xpbd_contact_cap;  // (contact, side) entries a world orders in LDS
+  unsigned* xpbd_keys;   // [xpbd_keys_cap] ordering entries of denser worlds, at 2 x their contact offset
+  long long xpbd_keys_cap;
 };
 
 void set_error(const std::string& msg);

@@ -126,6 +126,7 @@ struct WorldLds {
   float* com;    // 3
   float* inv_m;  // 1 (effective)
   float* inv_I;  // 9 (effective)
+  float* I;      // 9
   int* flags;
 };
 
@@ -396,8 +397,21 @@ __device__ bool joint_solve(const NmcModelDesc& d, const WorldLds& L, int j, int
   return true;
 }
 
-constexpr int XB = 256;  // lanes per world workgroup
-constexpr int BPL = 2;   // bodies per lane (worlds up to 512 bodies)
+// apply_body_deltas with an all-zero delta and no weights (the pass that follows the joint solve for
+// bodies whose only joint is FREE, solver_xpbd.py:590-630).  With delta = 0 every correction term in
+// apply_deltas is a signed zero, so the update reduces to renormalising q, re-deriving the origin from
+// the unchanged COM, and the velocity clamp; all non-zero results are bit-identical to apply_deltas.
+__device__ __forceinline__ BodyState apply_zero_delta(const BodyState& s, f3 com, float inv_m) {
+  if (inv_m == 0.0f) return s;
+  qf q1 = qnormalize(s.q.q);
+  f3 x_com = s.q.p + rot(s.q.q, com);
+  BodyState o;
+  o.q = xform{x_com - rot(q1, com), q1};
+  o.v = len(s.v) < 1e-4f ? f3{0.0f, 0.0f, 0.0f} : s.v;
+  o.w = len(s.w) < 1e-4f ? f3{0.0f, 0.0f, 0.0f} : s.w;
+  return o;
+}
+
 
 struct XpbdArgs {
   NmcModelDesc d;
@@ -412,24 +426,45 @@ struct XpbdArgs {
   float dt;
   float* q_out;
   float* qd_out;
-  int key_cap;  // (contact, side) entries the LDS ordering buffer holds (power of two)
+  int key_cap;  // (contact, side) entries the LDS ordering buffer holds
+  unsigned* keys_global;  // ordering buffer of worlds whose entries exceed key_cap (2 per contact slot)
+  long long keys_global_cap;
   int* counters;
 };
 
-__device__ __forceinline__ void sort_keys(unsigned* keys, int n2) {
-  for (int k = 2; k <= n2; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          unsigned x = keys[i], y = keys[ixj];
-          bool up = (i & k) == 0;
-          if ((x > y) == up) { keys[i] = y; keys[ixj] = x; }
-        }
-      }
-      __syncthreads();
-    }
-  }
+// Per-contact data that is constant for the whole step (solve_body_contact_positions inputs).
+struct CReg {
+  f3 p0, p1, o0, o1, n;
+  float m0, m1, mu, mut, mur;
+  int la, lb;  // local body indices, -1 = static
+  int valid;
+};
+
+__device__ __forceinline__ CReg load_contact(const XpbdArgs& a, int gc, int b0, int nb) {
+  const NmcModelDesc& d = a.d;
+  CReg r;
+  int s0 = a.c.shape0[gc], s1 = a.c.shape1[gc];
+  int ba = s0 >= 0 ? d.shape_body[s0] : -1;
+  int bb = s1 >= 0 ? d.shape_body[s1] : -1;
+  r.valid = (s0 != s1 && ba != bb) ? 1 : 0;
+  r.la = ba >= 0 ? ba - b0 : -1;
+  r.lb = bb >= 0 ? bb - b0 : -1;
+  if (r.la >= nb) r.la = -1;  // bodies outside this world never appear in its segment
+  if (r.lb >= nb) r.lb = -1;
+  int nz = 0;
+  float mu = 0.0f, mut = 0.0f, mur = 0.0f;
+  if (s0 >= 0) { nz += 1; mu += d.shape_mu[s0]; mut += d.shape_mu_torsional[s0]; mur += d.shape_mu_rolling[s0]; }
+  if (s1 >= 0) { nz += 1; mu += d.shape_mu[s1]; mut += d.shape_mu_torsional[s1]; mur += d.shape_mu_rolling[s1]; }
+  if (nz > 0) { mu /= (float)nz; mut /= (float)nz; mur /= (float)nz; }
+  r.mu = mu; r.mut = mut; r.mur = mur;
+  r.p0 = load3(a.c.point0 + 3 * gc);
+  r.p1 = load3(a.c.point1 + 3 * gc);
+  r.o0 = load3(a.c.offset0 + 3 * gc);
+  r.o1 = load3(a.c.offset1 + 3 * gc);
+  r.n = load3(a.c.normal + 3 * gc);
+  r.m0 = a.c.margin0[gc];
+  r.m1 = a.c.margin1[gc];
+  return r;
 }
 
 __device__ __forceinline__ BodyState lds_state(const WorldLds& L, int lb) {
@@ -441,8 +476,36 @@ __device__ __forceinline__ void lds_store_state(const WorldLds& L, int lb, const
   store3(L.qd + 6 * lb + 3, s.w);
 }
 
-__global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
+// Exclusive scan of cnt[0..n) into out[0..n] (out[n] = total) by the whole block.
+__device__ void block_exclusive_scan(const int* cnt, int* out, int n, int* tmp) {
+  __shared__ int s_carry;
+  if (threadIdx.x == 0) s_carry = 0;
+  __syncthreads();
+  for (int base = 0; base < n; base += blockDim.x) {
+    int i = base + threadIdx.x;
+    int v = i < n ? cnt[i] : 0;
+    tmp[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+      int t = threadIdx.x >= (unsigned)off ? tmp[threadIdx.x - off] : 0;
+      __syncthreads();
+      tmp[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (i < n) out[i] = tmp[threadIdx.x] - v + s_carry;
+    __syncthreads();
+    if (threadIdx.x == blockDim.x - 1) s_carry += tmp[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[n] = s_carry;
+  __syncthreads();
+}
+
+// BS lanes per world; each lane keeps CPL contacts in registers for the whole step and owns BPL bodies.
+template <int BS, int CPL, int BPL>
+__global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int s_articulated;
   const NmcModelDesc& d = a.d;
   const int w = blockIdx.x;  // world index; w == world_count owns the global bodies
   const int b0 = d.world_body_first[w];
@@ -451,7 +514,9 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
   const int nj = d.world_joint_count[w];
   const int mb = d.max_world_bodies > 0 ? d.max_world_bodies : 1;
   const int mj = d.max_world_joints > 0 ? d.max_world_joints : 1;
+  constexpr int CH = BS * CPL;  // contacts per chunk
   const float dt = a.dt;
+  const int tid = threadIdx.x;
 
   WorldLds L;
   unsigned char* p = smem;
@@ -461,10 +526,13 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
   L.com = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * mb;
   L.inv_m = reinterpret_cast<float*>(p); p += sizeof(float) * mb;
   L.inv_I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
+  L.I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
   L.flags = reinterpret_cast<int*>(p); p += sizeof(int) * mb;
   int* seg = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
-  float* contrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * XB;
-  int* active = reinterpret_cast<int*>(p); p += sizeof(int) * XB;
+  int* cur = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
+  int* scan_tmp = reinterpret_cast<int*>(p); p += sizeof(int) * BS;
+  float* contrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * CH;
+  int* active = reinterpret_cast<int*>(p); p += sizeof(int) * CH;
   float* jcontrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * mj;
   int* jactive = reinterpret_cast<int*>(p); p += sizeof(int) * mj;
   unsigned* keys = reinterpret_cast<unsigned*>(p);
@@ -473,12 +541,18 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
   const int cstart = has_c ? a.c.world_start[w] : 0;
   int nc = has_c ? a.c.world_count[w] : 0;
   if (2 * nc > a.key_cap) {
-    if (threadIdx.x == 0) atomicAdd(&a.counters[3], 1);
-    nc = a.key_cap / 2;
+    // dense world: order its entries in the global buffer at its contact offset
+    long long room = a.keys_global_cap - 2ll * cstart;
+    if (2ll * nc > room) {
+      if (tid == 0) atomicAdd(&a.counters[3], 1);
+      nc = room > 0 ? static_cast<int>(room / 2) : 0;
+    }
+    keys = a.keys_global + 2ll * cstart;
   }
+  if (tid == 0) s_articulated = 0;
 
-  // ---- load (solver.py:160-175 effective inverse mass/inertia of kinematic bodies is zero)
-  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+  // ---- load (solver.py:160-175: kinematic bodies get zero effective inverse mass/inertia)
+  for (int lb = tid; lb < nb; lb += BS) {
     int b = b0 + lb;
     bool kin = (d.body_flags[b] & BODY_KINEMATIC) != 0;
     L.flags[lb] = d.body_flags[b];
@@ -487,11 +561,25 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
     for (int k = 0; k < 3; ++k) L.com[3 * lb + k] = d.body_com[3 * b + k];
     for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = a.q_in[7 * b + k];
     for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = a.qd_in[6 * b + k];
+    cur[lb] = 0;
+  }
+  for (int jl = tid; jl < nj; jl += BS) {
+    int j = j0 + jl;
+    int t = d.joint_type[j];
+    if (d.joint_enabled[j] && t != JT_FREE && t != JT_DISTANCE) s_articulated = 1;
+  }
+  // contacts held in registers for the whole step (chunk 0)
+  CReg creg[CPL];
+#pragma unroll
+  for (int jj = 0; jj < CPL; ++jj) {
+    int c = jj * BS + tid;
+    creg[jj].valid = 0;
+    if (c < nc) creg[jj] = load_contact(a, cstart + c, b0, nb);
   }
   __syncthreads();
 
   // ---- body_f + joint forces in joint order (kernels.py:890-1004)
-  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+  for (int lb = tid; lb < nb; lb += BS) {
     int b = b0 + lb;
     Spatial f{load3(a.f_in + 6 * b), load3(a.f_in + 6 * b + 3)};
     if (d.joint_count) {
@@ -537,52 +625,77 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
     store3(fl + 6 * lb, f.lin);
     store3(fl + 6 * lb + 3, f.ang);
   }
+  // per-body (contact, side) counts for the ordering lists
+  for (int c = tid; c < nc; c += BS) {
+    CReg r;
+    if (c < CH) {
+      r = creg[0];
+#pragma unroll
+      for (int jj = 1; jj < CPL; ++jj)
+        if (c >= jj * BS) r = creg[jj];
+    } else {
+      r = load_contact(a, cstart + c, b0, nb);
+    }
+    if (r.valid) {
+      if (r.la >= 0) atomicAdd(&cur[r.la], 1);
+      if (r.lb >= 0) atomicAdd(&cur[r.lb], 1);
+    }
+  }
   __syncthreads();
 
-  // ---- integrate in place (solver.py:98-157)
-  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
-    int b = b0 + lb;
-    if (L.flags[lb] & BODY_KINEMATIC) continue;
-    int wi = d.body_world[b];
-    int gi = wi > 0 ? wi : 0;
-    Spatial f{load3(fl + 6 * lb), load3(fl + 6 * lb + 3)};
-    BodyState o = integrate_body(lds_state(L, lb), f, load3(L.com + 3 * lb), load_m33(d.body_inertia + 9 * b),
-                                 d.body_inv_mass[b], load_m33(d.body_inv_inertia + 9 * b), load3(d.gravity + 3 * gi),
-                                 a.p.angular_damping, dt);
-    lds_store_state(L, lb, o);
-  }
-
-  // ---- (contact, side) entries sorted by (body, contact): each body's contributions in contact order
-  if (has_c && nc > 0) {
-    int ne = 2 * nc;
-    int n2 = 1;
-    while (n2 < ne) n2 <<= 1;
-    for (int e = threadIdx.x; e < n2; e += blockDim.x) {
-      unsigned key = 0xffffffffu;
-      if (e < ne) {
-        int c = e >> 1;
-        int s = (e & 1) ? a.c.shape1[cstart + c] : a.c.shape0[cstart + c];
-        int b = s >= 0 ? d.shape_body[s] : -1;
-        int lb = b >= 0 ? b - b0 : -1;
-        if (lb >= 0 && lb < nb) key = (static_cast<unsigned>(lb) << 20) | static_cast<unsigned>(e);
+  // ---- integrate in place (solver.py:98-157); static body data of the lane's bodies stays in registers
+#pragma unroll
+  for (int r = 0; r < BPL; ++r) {
+    int lb = tid + r * BS;
+    if (lb < nb) {
+      int b = b0 + lb;
+      for (int k = 0; k < 9; ++k) L.I[9 * lb + k] = d.body_inertia[9 * b + k];
+      if (!(L.flags[lb] & BODY_KINEMATIC)) {
+        int wi = d.body_world[b];
+        int gi = wi > 0 ? wi : 0;
+        Spatial f{load3(fl + 6 * lb), load3(fl + 6 * lb + 3)};
+        BodyState o = integrate_body(lds_state(L, lb), f, load3(L.com + 3 * lb), load_m33(d.body_inertia + 9 * b),
+                                     d.body_inv_mass[b],
+                                     load_m33(d.body_inv_inertia + 9 * b), load3(d.gravity + 3 * gi),
+                                     a.p.angular_damping, dt);
+        lds_store_state(L, lb, o);
       }
-      keys[e] = key;
     }
-    __syncthreads();
-    sort_keys(keys, n2);
-    for (int lb = threadIdx.x; lb <= nb; lb += blockDim.x) {
-      unsigned target = static_cast<unsigned>(lb) << 20;
-      int lo = 0, hi = n2;
-      while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (keys[mid] < target) lo = mid + 1; else hi = mid;
-      }
-      seg[lb] = lo;
+  }
+  // ---- ordering lists: counting sort by body, then each body sorts its (short) list by contact
+  block_exclusive_scan(cur, seg, nb, scan_tmp);
+  for (int lb = tid; lb < nb; lb += BS) cur[lb] = seg[lb];
+  __syncthreads();
+  for (int c = tid; c < nc; c += BS) {
+    CReg r;
+    if (c < CH) {
+      r = creg[0];
+#pragma unroll
+      for (int jj = 1; jj < CPL; ++jj)
+        if (c >= jj * BS) r = creg[jj];
+    } else {
+      r = load_contact(a, cstart + c, b0, nb);
+    }
+    if (r.valid) {
+      if (r.la >= 0) keys[atomicAdd(&cur[r.la], 1)] = static_cast<unsigned>(2 * c);
+      if (r.lb >= 0) keys[atomicAdd(&cur[r.lb], 1)] = static_cast<unsigned>(2 * c + 1);
+    }
+  }
+  __syncthreads();
+  for (int lb = tid; lb < nb; lb += BS) {
+    int s0 = seg[lb], s1 = seg[lb + 1];
+    for (int i = s0 + 1; i < s1; ++i) {
+      unsigned k = keys[i];
+      int j = i - 1;
+      while (j >= s0 && keys[j] > k) { keys[j + 1] = keys[j]; --j; }
+      keys[j + 1] = k;
     }
   }
   __syncthreads();
 
   const float relax = a.p.rigid_contact_relaxation;
+  const bool weighted = a.p.rigid_contact_con_weighting != 0;
+  const bool articulated = s_articulated != 0;
   for (int it = 0; it < a.p.iterations; ++it) {
     // ---------------- contacts (kernels.py:2062-2299) + apply (kernels.py:809-879)
     if (has_c) {
@@ -591,47 +704,36 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
       int ep[BPL], ee[BPL];
 #pragma unroll
       for (int r = 0; r < BPL; ++r) {
-        int lb = threadIdx.x + r * XB;
+        int lb = tid + r * BS;
         acc[r] = Spatial{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
         cnt[r] = 0.0f;
-        ep[r] = (nc > 0 && lb < nb) ? seg[lb] : 0;
-        ee[r] = (nc > 0 && lb < nb) ? seg[lb + 1] : 0;
+        ep[r] = lb < nb ? seg[lb] : 0;
+        ee[r] = lb < nb ? seg[lb + 1] : 0;
       }
-      for (int cb = 0; cb < nc; cb += XB) {
-        int c = cb + threadIdx.x;
-        int act = 0;
-        if (c < nc) {
-          int gc = cstart + c;
-          int s0 = a.c.shape0[gc], s1 = a.c.shape1[gc];
-          int ba = s0 >= 0 ? d.shape_body[s0] : -1;
-          int bb = s1 >= 0 ? d.shape_body[s1] : -1;
-          if (s0 != s1 && ba != bb) {
-            int la = ba >= 0 ? ba - b0 : -1;
-            int lbb = bb >= 0 ? bb - b0 : -1;
-            if (la >= nb) la = -1;  // bodies outside this world never appear in its segment
-            if (lbb >= nb) lbb = -1;
-            int nz = 0;
-            float mu = 0.0f, mut = 0.0f, mur = 0.0f;
-            if (s0 >= 0) { nz += 1; mu += d.shape_mu[s0]; mut += d.shape_mu_torsional[s0]; mur += d.shape_mu_rolling[s0]; }
-            if (s1 >= 0) { nz += 1; mu += d.shape_mu[s1]; mut += d.shape_mu_torsional[s1]; mur += d.shape_mu_rolling[s1]; }
-            if (nz > 0) { mu /= (float)nz; mut /= (float)nz; mur /= (float)nz; }
+      for (int cb = 0; cb < nc; cb += CH) {
+#pragma unroll
+        for (int jj = 0; jj < CPL; ++jj) {
+          int c = cb + jj * BS + tid;
+          int slot = jj * BS + tid;
+          int act = 0;
+          if (c < nc) {
+            CReg r = cb == 0 ? creg[jj] : load_contact(a, cstart + c, b0, nb);
             Spatial da, db;
-            if (contact_solve(L, la, lbb, load3(a.c.point0 + 3 * gc), load3(a.c.point1 + 3 * gc),
-                              load3(a.c.offset0 + 3 * gc), load3(a.c.offset1 + 3 * gc), load3(a.c.normal + 3 * gc),
-                              a.c.margin0[gc], a.c.margin1[gc], mu, mut, mur, relax, dt, da, db)) {
+            if (r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu, r.mut, r.mur,
+                                         relax, dt, da, db)) {
               act = 1;
-              float* o = contrib + 12 * threadIdx.x;
+              float* o = contrib + 12 * slot;
               store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
             }
           }
+          active[slot] = act;
         }
-        active[threadIdx.x] = act;
         __syncthreads();
-        const unsigned cend = static_cast<unsigned>(cb + XB);
+        const unsigned cend = static_cast<unsigned>(cb + CH);
 #pragma unroll
         for (int r = 0; r < BPL; ++r) {
           while (ep[r] < ee[r]) {
-            unsigned ent = keys[ep[r]] & 0xfffffu;
+            unsigned ent = keys[ep[r]];
             unsigned cc = ent >> 1;
             if (cc >= cend) break;
             int slot = static_cast<int>(cc) - cb;
@@ -644,58 +746,66 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
             ep[r] += 1;
           }
         }
+        if (cb + CH < nc) __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < BPL; ++r) {
+        int lb = tid + r * BS;
+        if (lb < nb) {
+          BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
+                                     lds_m33(L.inv_I + 9 * lb), acc[r], weighted, cnt[r], dt);
+          lds_store_state(L, lb, o);
+        }
+      }
+    }
+    // ---------------- joints (kernels.py:1441-1943) + apply without weights
+    if (d.joint_count) {
+      if (articulated) {
+        __syncthreads();
+        for (int jl = tid; jl < nj; jl += BS) {
+          int j = j0 + jl;
+          int id_p = d.joint_parent[j], id_c = d.joint_child[j];
+          Spatial dp, dc;
+          bool ok = joint_solve(d, L, j, id_p >= 0 ? id_p - b0 : -1, id_c - b0, a.tpos, a.tvel,
+                                a.p.joint_linear_compliance, a.p.joint_angular_compliance,
+                                a.p.joint_angular_relaxation, a.p.joint_linear_relaxation, dt, dp, dc);
+          jactive[jl] = ok ? 1 : 0;
+          if (ok) {
+            float* o = jcontrib + 12 * jl;
+            store3(o, dp.lin); store3(o + 3, dp.ang); store3(o + 6, dc.lin); store3(o + 9, dc.ang);
+          }
+        }
         __syncthreads();
       }
 #pragma unroll
       for (int r = 0; r < BPL; ++r) {
-        int lb = threadIdx.x + r * XB;
-        if (lb < nb) {
-          int b = b0 + lb;
-          BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), load_m33(d.body_inertia + 9 * b),
-                                     L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc[r],
-                                     a.p.rigid_contact_con_weighting != 0, cnt[r], dt);
-          lds_store_state(L, lb, o);
-        }
-      }
-      __syncthreads();
-    }
-    // ---------------- joints (kernels.py:1441-1943) + apply without weights
-    if (d.joint_count) {
-      for (int jl = threadIdx.x; jl < nj; jl += blockDim.x) {
-        int j = j0 + jl;
-        int id_p = d.joint_parent[j], id_c = d.joint_child[j];
-        Spatial dp, dc;
-        bool ok = joint_solve(d, L, j, id_p >= 0 ? id_p - b0 : -1, id_c - b0, a.tpos, a.tvel,
-                              a.p.joint_linear_compliance, a.p.joint_angular_compliance, a.p.joint_angular_relaxation,
-                              a.p.joint_linear_relaxation, dt, dp, dc);
-        jactive[jl] = ok ? 1 : 0;
-        if (ok) {
-          float* o = jcontrib + 12 * jl;
-          store3(o, dp.lin); store3(o + 3, dp.ang); store3(o + 6, dc.lin); store3(o + 9, dc.ang);
-        }
-      }
-      __syncthreads();
-      for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+        int lb = tid + r * BS;
+        if (lb >= nb) continue;
         int b = b0 + lb;
         Spatial acc{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-        for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
-          int ent = d.body_joint_entry[e];
-          int jl = (ent >> 1) - j0;
-          if (!jactive[jl]) continue;
-          const float* o = jcontrib + 12 * jl + 6 * (ent & 1);
-          acc.lin = acc.lin + load3(o);
-          acc.ang = acc.ang + load3(o + 3);
+        bool any = false;
+        if (articulated) {
+          for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
+            int ent = d.body_joint_entry[e];
+            int jl = (ent >> 1) - j0;
+            if (!jactive[jl]) continue;
+            const float* o = jcontrib + 12 * jl + 6 * (ent & 1);
+            acc.lin = acc.lin + load3(o);
+            acc.ang = acc.ang + load3(o + 3);
+            any = true;
+          }
         }
-        BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), load_m33(d.body_inertia + 9 * b),
-                                   L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt);
+        BodyState o = any ? apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
+                                         lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt)
+                          : apply_zero_delta(lds_state(L, lb), load3(L.com + 3 * lb), L.inv_m[lb]);
         lds_store_state(L, lb, o);
       }
-      __syncthreads();
     }
+    __syncthreads();
   }
 
   // ---- store (copy_kinematic_body_state_kernel, kernels.py:18-31: kinematic bodies pass through)
-  for (int lb = threadIdx.x; lb < nb; lb += blockDim.x) {
+  for (int lb = tid; lb < nb; lb += BS) {
     int b = b0 + lb;
     if (L.flags[lb] & BODY_KINEMATIC) {
       for (int k = 0; k < 7; ++k) a.q_out[7 * b + k] = a.q_in[7 * b + k];
@@ -707,10 +817,26 @@ __global__ void __launch_bounds__(XB) xpbd_world_kernel(XpbdArgs a) {
   }
 }
 
+// Launch shapes: <lanes, contacts per lane, bodies per lane>.  Worlds up to 64 bodies run on one
+// wave (64 lanes, 128 contacts held in registers, barriers cost one wave's issue); worlds up to 256
+// bodies run one body per lane on 256 lanes; larger worlds (up to 512 bodies) give each lane two.
+// Measured on 4096 worlds x 32 bodies: 64 lanes 0.377 ms per substep, 128 lanes 0.499 ms.
+struct XpbdShape { int bs, cpl, bpl; };
+static XpbdShape xpbd_shape(int max_bodies) {
+  if (max_bodies <= 64) return {64, 2, 1};
+  if (max_bodies <= 256) return {256, 1, 1};
+  return {256, 1, 2};
+}
+
+int xpbd_block_size(int max_bodies) { return xpbd_shape(max_bodies).bs; }
+
 size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap) {
+  XpbdShape sh = xpbd_shape(max_bodies);
+  int bs = sh.bs;
   size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 1;
-  size_t b = sizeof(float) * (7 + 6 + 6 + 3 + 1 + 9) * mb + sizeof(int) * mb + sizeof(int) * (mb + 1);
-  b += sizeof(float) * 12 * XB + sizeof(int) * XB + sizeof(float) * 12 * mj + sizeof(int) * mj;
+  size_t ch = (size_t)bs * sh.cpl;
+  size_t b = sizeof(float) * (7 + 6 + 6 + 3 + 1 + 9 + 9) * mb + sizeof(int) * mb + 2 * sizeof(int) * (mb + 1);
+  b += sizeof(int) * bs + sizeof(float) * 12 * ch + sizeof(int) * ch + sizeof(float) * 12 * mj + sizeof(int) * mj;
   b += sizeof(unsigned) * (size_t)key_cap;
   return (b + 15) & ~size_t(15);
 }
@@ -732,9 +858,17 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   a.q_out = q_out;
   a.qd_out = qd_out;
   a.key_cap = m->xpbd_contact_cap;
+  a.keys_global = m->xpbd_keys;
+  a.keys_global_cap = m->xpbd_keys_cap;
   a.counters = m->counters;
   int worlds = m->d.world_count + 1;
-  hipLaunchKernelGGL(xpbd_world_kernel, dim3(worlds), dim3(XB), m->xpbd_lds, stream, a);
+  XpbdShape sh = xpbd_shape(m->d.max_world_bodies);
+  if (sh.bs == 64)
+    hipLaunchKernelGGL((xpbd_world_kernel<64, 2, 1>), dim3(worlds), dim3(64), m->xpbd_lds, stream, a);
+  else if (sh.bpl == 1)
+    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 1>), dim3(worlds), dim3(256), m->xpbd_lds, stream, a);
+  else
+    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 2>), dim3(worlds), dim3(256), m->xpbd_lds, stream, a);
   return hipGetLastError();
 }
 
