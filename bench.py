@@ -41,7 +41,23 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--kernel-reps", type=int, default=20)
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="only run a device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return p.parse_args()
+
+
+def copy_probe(mib: int) -> None:
+    """A plain copy of known size: FETCH_SIZE / WRITE_SIZE read on it give the counters' scale."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
 def main():
@@ -51,6 +67,9 @@ def main():
               "GPUs with no per-step exchange, so the multi-GPU figure is N x the one-GPU figure by construction)",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
     import torch
 
     from newton_custom_contact_amd import CollisionPipeline, scenes
@@ -120,26 +139,26 @@ def main():
     print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.1f} ms), N_c={n_c:.1f}",
           file=sys.stderr, flush=True)
 
-    # ---- per-kernel timing of the two phases on the launch stream (HIP events)
+    # ---- per-phase launch durations on the launch stream (HIP events), at the end state of the timed
+    # region: `reps` back-to-back launches of one phase between two events, so the GPU never waits on
+    # the host and the average is the kernels' own duration (collide: 4 kernels, step: 1 kernel)
     stream = torch.cuda.current_stream()
     reps = args.kernel_reps
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4 * reps)]
     a, b = states[0], states[1]
-    t_col, t_xp = [], []
-    for r in range(reps):
-        ev[4 * r].record(stream)
-        pipe.collide(a, contacts)
-        ev[4 * r + 1].record(stream)
-        ev[4 * r + 2].record(stream)
-        solver.step(a, b, None, contacts, sdt)
-        ev[4 * r + 3].record(stream)
-        a, b = b, a
+    pipe.collide(a, contacts)
     torch.cuda.synchronize()
-    for r in range(reps):
-        t_col.append(ev[4 * r].elapsed_time(ev[4 * r + 1]))
-        t_xp.append(ev[4 * r + 2].elapsed_time(ev[4 * r + 3]))
-    t_col_ms = float(np.mean(t_col))
-    t_xp_ms = float(np.mean(t_xp))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        pipe.collide(a, contacts)
+    ev[1].record(stream)
+    ev[2].record(stream)
+    for _ in range(reps):
+        solver.step(a, b, None, contacts, sdt)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    t_col_ms = ev[0].elapsed_time(ev[1]) / reps
+    t_xp_ms = ev[2].elapsed_time(ev[3]) / reps
 
     N_b, N_s, N_j = 32, 33, 0
     b_collide = 28 * N_b + 64 * N_s + 76 * n_c + 4
@@ -149,7 +168,8 @@ def main():
     if t_xp_ms >= t_col_ms:
         dom, dom_ms, dom_bytes = "xpbd_world_kernel", t_xp_ms, args.worlds * b_step
     else:
-        dom, dom_ms, dom_bytes = "collide (segment + scan + compact kernels)", t_col_ms, args.worlds * b_collide
+        dom, dom_ms, dom_bytes = ("collide phase: collide_segment_kernel + narrow_kernel + contact_scan_kernel + "
+                                  "contact_compact_kernel", t_col_ms, args.worlds * b_collide)
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
 
     cpu = None
@@ -184,6 +204,7 @@ def main():
         },
         "roofline": {
             "kernel": dom,
+            "launch_ms": round(dom_ms, 4),
             "bound": "hbm",
             "achieved": round(achieved, 2),
             "peak": HBM_PEAK_GBS,

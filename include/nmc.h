@@ -87,7 +87,8 @@ typedef struct NmcModelDesc {
   const float* joint_limit_upper;
   const float* joint_target_ke;
   const float* joint_target_kd;
-  /* capacities */
+  /* capacities: nmc_collide stages up to 5 records per broad-phase pair (shape_pairs_max pairs);
+   * nmc_xpbd_step accepts Contacts buffers of at most rigid_contact_max slots */
   int rigid_contact_max;
   int shape_pairs_max;
 } NmcModelDesc;
@@ -132,7 +133,8 @@ void nmc_model_destroy(NmcModel* model);
  * canonical order: segment-major, and inside a segment ascending by the 64-bit sort key. */
 int nmc_collide(NmcModel* model, const float* body_q, const NmcContacts* contacts, nmc_stream_t stream);
 
-/* One XPBD step: integrate, `iterations` x (contact solve + apply, joint solve + apply). */
+/* One XPBD step: integrate, `iterations` x (contact solve + apply, joint solve + apply).  Fails when
+ * contacts->capacity exceeds the model's rigid_contact_max or when state in/out arrays alias. */
 int nmc_xpbd_step(NmcModel* model, const NmcXpbdParams* params, const float* body_q_in, const float* body_qd_in,
                   const float* body_f_in, const float* joint_f, const float* joint_target_pos,
                   const float* joint_target_vel, const NmcContacts* contacts, float dt, float* body_q_out,

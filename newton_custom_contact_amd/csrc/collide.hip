@@ -1,16 +1,19 @@
 // Collision pipeline kernels for gfx950.
 //
 // One workgroup owns one broad-phase segment (a world's shapes plus the global shapes, reference
-// precompute_world_map, newton/_src/geometry/broad_phase_common.py:183-300) and runs, entirely out of
-// LDS: shape world transforms + AABBs (collide.py:150-275), sweep-and-prune on the reference's fixed
+// precompute_world_map, newton/_src/geometry/broad_phase_common.py:183-300) and runs, out of LDS:
+// shape world transforms + AABBs (collide.py:150-275) and sweep-and-prune on the reference's fixed
 // axis (broad_phase_sap.py:43-394: project, bitonic sort of (lower, index) keys, sweep with the
-// world / group / exclusion / AABB tests), a bitonic sort of the surviving pairs by contact sort key,
-// and the narrow phase, one pair per lane.  Each pair owns five staging slots; a per-segment scan and a
-// grid-wide scan then place every contact at its final index, so the Contacts buffer comes out in
-// canonical order (segment-major, sort key inside a segment) with no float atomics and no global sort.
+// world / group / exclusion / AABB tests), then a bitonic sort of the surviving pairs by contact sort
+// key.  The narrow phase then runs over all pairs of all segments at once, one pair per lane, with the
+// MPR/GJK pairs grouped apart from the analytic ones.  Each pair owns five staging slots; a
+// per-segment scan and a grid-wide scan place every contact at its final index, so the Contacts
+// buffer comes out in canonical order (segment-major, sort key inside a segment) with no float
+// atomics and no global sort.
 //
 // Launch sequence of nmc_collide (all on the caller's stream, graph-capturable):
-//   collide_segment_kernel  <<<segments, 256>>>   AABB + SAP + narrow phase -> staging
+//   collide_segment_kernel  <<<segments, 256>>>   AABB + SAP + pair sort -> staged pairs, routes
+//   narrow_kernel           <<<<=2048, 256>>>     narrow phase -> staging slots, per-segment counts
 //   contact_scan_kernel     <<<1, 1024>>>         segment offsets, total count
 //   contact_compact_kernel  <<<segments, 256>>>   staging -> Contacts arrays
 #include <hip/hip_runtime.h>
@@ -23,6 +26,7 @@
 namespace nmc {
 
 constexpr int CBLOCK = 256;
+constexpr int NARROW_MAX_BLOCKS = 2048;  // 8 workgroups per CU; the narrow kernel strides over the rest
 
 __device__ __forceinline__ unsigned sortable_float(float f) {
   if (f == 0.0f) f = 0.0f;  // -0 and +0 compare equal: give them one key
@@ -36,15 +40,22 @@ __device__ __forceinline__ bool group_pair(int a, int b) {
   return a != b;
 }
 
-__device__ __forceinline__ bool excluded(int a, int b, const int* __restrict__ fp, int n) {
-  if (n <= 0) return false;
-  int lo = 0, hi = n - 1;
-  while (lo <= hi) {
+// First filter-pair entry whose first shape is >= a (entries are sorted (a, b) pairs with a < b).
+__device__ __forceinline__ int filter_lower_bound(const int* __restrict__ fp, int n, int a) {
+  int lo = 0, hi = n;
+  while (lo < hi) {
     int mid = (lo + hi) >> 1;
-    int ma = fp[2 * mid], mb = fp[2 * mid + 1];
-    if (a == ma && b == mb) return true;
-    bool less = a < ma || (a == ma && b < mb);
-    if (less) hi = mid - 1; else lo = mid + 1;
+    if (fp[2 * mid] < a) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+// is_pair_excluded (broad_phase_common.py:82-113) over the entries [e0, e1) whose first shape is a.
+__device__ __forceinline__ bool excluded(int b, const int* __restrict__ fp, int e0, int e1) {
+  for (int e = e0; e < e1; ++e) {
+    int mb = fp[2 * e + 1];
+    if (mb == b) return true;
+    if (mb > b) break;
   }
   return false;
 }
@@ -77,30 +88,40 @@ __device__ __forceinline__ int next_pow2(int n) {
 }
 
 struct SegLds {
-  float* X;       // 7 per shape
-  float* gs;      // 3
-  float* margin;  // 1
-  float* lo;      // 3
-  float* hi;      // 3
-  float* plo;     // 1
-  float* phi;     // 1
+  float* lo;   // 3 per shape, AABB expanded by margin + gap
+  float* hi;   // 3
+  float* plo;  // 1, SAP projection interval
+  float* phi;  // 1
   int* type;
   int* gid;
+  int* world;
+  int* group;
+  int* fbeg;   // filter-pair entries with this shape first: [fbeg, fend)
+  int* fend;
   unsigned long long* skeys;  // sort keys (n2)
   unsigned* pairs;            // pair keys (pair_cap)
 };
 
-__device__ __forceinline__ ShapeW lds_shape(const SegLds& L, int l) {
+__device__ __forceinline__ void store_wc(float* __restrict__ o, const ShapeW& w) {
+  store_xform(o, w.X);
+  store3(o + 7, w.gscale);
+  o[10] = w.margin;
+  store3(o + 11, w.lo);
+  store3(o + 14, w.hi);
+}
+
+__device__ __forceinline__ ShapeW load_wc(const float* __restrict__ o) {
   ShapeW s;
-  s.X = xform{{L.X[7 * l], L.X[7 * l + 1], L.X[7 * l + 2]},
-              {L.X[7 * l + 3], L.X[7 * l + 4], L.X[7 * l + 5], L.X[7 * l + 6]}};
-  s.gscale = f3{L.gs[3 * l], L.gs[3 * l + 1], L.gs[3 * l + 2]};
-  s.margin = L.margin[l];
-  s.lo = f3{L.lo[3 * l], L.lo[3 * l + 1], L.lo[3 * l + 2]};
-  s.hi = f3{L.hi[3 * l], L.hi[3 * l + 1], L.hi[3 * l + 2]};
+  s.X = load_xform(o);
+  s.gscale = load3(o + 7);
+  s.margin = o[10];
+  s.lo = load3(o + 11);
+  s.hi = load3(o + 14);
   return s;
 }
 
+// Broad phase of one segment: shape world data (kept in shape_wc for the narrow phase), SAP, the
+// segment's pairs in key order at staging pairs [base, base + np), and their route entries.
 __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d, const float* __restrict__ body_q,
                                                                   StageBuffers st, int* __restrict__ seg_pair_count,
                                                                   int* __restrict__ seg_pair_base,
@@ -111,7 +132,8 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_npairs;
   __shared__ int s_base;
-  __shared__ int s_red[CBLOCK / 64];
+  __shared__ int s_cls[2];
+  __shared__ int s_rbase[2];
   const int seg = blockIdx.x;
   const int s0 = d.segment_start[seg];
   const int n = d.segment_start[seg + 1] - s0;
@@ -121,31 +143,45 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
   SegLds L;
   unsigned char* p = smem;
   L.skeys = reinterpret_cast<unsigned long long*>(p); p += sizeof(unsigned long long) * n2max;
-  L.X = reinterpret_cast<float*>(p); p += sizeof(float) * 7 * max_n;
-  L.gs = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
-  L.margin = reinterpret_cast<float*>(p); p += sizeof(float) * max_n;
   L.lo = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
   L.hi = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
   L.plo = reinterpret_cast<float*>(p); p += sizeof(float) * max_n;
   L.phi = reinterpret_cast<float*>(p); p += sizeof(float) * max_n;
   L.type = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
   L.gid = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
+  L.world = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
+  L.group = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
+  L.fbeg = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
+  L.fend = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
   L.pairs = reinterpret_cast<unsigned*>(p);
 
-  if (threadIdx.x == 0) s_npairs = 0;
+  if (threadIdx.x == 0) {
+    s_npairs = 0;
+    s_cls[0] = 0;
+    s_cls[1] = 0;
+    seg_contact_count[seg] = 0;
+  }
 
   // ---- shape world transforms, AABBs, SAP projections (collide.py:150-275, broad_phase_sap.py:43-63)
   const f3 dir = normalize(f3{0.5935f, 0.7790f, 0.1235f});
   for (int l = threadIdx.x; l < n; l += blockDim.x) {
     int s = d.segment_shape[s0 + l];
     ShapeW w = shape_world(s, body_q, d);
-    store_xform(L.X + 7 * l, w.X);
-    store3(L.gs + 3 * l, w.gscale);
-    L.margin[l] = w.margin;
+    // a world's shapes are written by its own segment, global shapes by the dedicated last segment
+    if (d.shape_world[s] != -1 || !regular) store_wc(st.shape_wc + 20ll * s, w);
     store3(L.lo + 3 * l, w.lo);
     store3(L.hi + 3 * l, w.hi);
     L.type[l] = d.shape_type[s];
     L.gid[l] = s;
+    L.world[l] = d.shape_world[s];
+    L.group[l] = d.shape_group[s];
+    int fb = 0, fe = 0;
+    if (d.filter_count > 0) {
+      fb = filter_lower_bound(d.filter_pairs, d.filter_count, s);
+      fe = filter_lower_bound(d.filter_pairs, d.filter_count, s + 1);
+    }
+    L.fbeg[l] = fb;
+    L.fend[l] = fe;
     f3 half = 0.5f * (w.hi - w.lo);
     half = f3{half.x + 0.0f, half.y + 0.0f, half.z + 0.0f};
     float radius = dot(dir, half);
@@ -170,17 +206,17 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
       int t1 = L.gid[li], t2 = L.gid[lj];
       if (t1 == t2) continue;
       int la = t1 < t2 ? li : lj, lb = t1 < t2 ? lj : li;
-      int a = L.gid[la], b = L.gid[lb];
-      int w1 = d.shape_world[a], w2 = d.shape_world[b];
+      // the reference's tests form one conjunction; the LDS-only ones go first
+      int w1 = L.world[la], w2 = L.world[lb];
       if (w1 == -1 && w2 == -1 && regular) continue;
       if (w1 != -1 && w2 != -1 && w1 != w2) continue;
-      if (!group_pair(d.shape_group[a], d.shape_group[b])) continue;
-      if (excluded(a, b, d.filter_pairs, d.filter_count)) continue;
+      if (!group_pair(L.group[la], L.group[lb])) continue;
       const float c = 0.0f + 0.0f;
       bool ov = L.lo[3 * la] <= L.hi[3 * lb] + c && L.hi[3 * la] >= L.lo[3 * lb] - c &&
                 L.lo[3 * la + 1] <= L.hi[3 * lb + 1] + c && L.hi[3 * la + 1] >= L.lo[3 * lb + 1] - c &&
                 L.lo[3 * la + 2] <= L.hi[3 * lb + 2] + c && L.hi[3 * la + 2] >= L.lo[3 * lb + 2] - c;
       if (!ov) continue;
+      if (excluded(L.gid[lb], d.filter_pairs, L.fbeg[la], L.fend[la])) continue;
       // narrow-phase order: lower GeoType first (narrow_phase.py:212-215); ranks order like global ids
       int ra = la, rb = lb;
       if (L.type[la] > L.type[lb]) { ra = lb; rb = la; }
@@ -199,10 +235,10 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
   __syncthreads();
   bitonic_sort(L.pairs, pn2);
 
-  // ---- staging allocation (5 slots per pair)
+  // ---- staging allocation (one pair entry + 5 contact slots per pair)
+  const long long cap_pairs = st.slot_capacity / 5;
   if (threadIdx.x == 0) {
     int base = atomicAdd(stage_alloc, np);
-    long long cap_pairs = st.slot_capacity / 5;
     int keep = np;
     if (static_cast<long long>(base) + np > cap_pairs) {
       keep = static_cast<int>(cap_pairs - base);
@@ -218,48 +254,55 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
   np = s_npairs;
   const int base = s_base;
 
-  // ---- narrow phase, one pair per lane
-  int local = 0;
-  PairOut out;
-  WriteCtx wctx{body_q, d.shape_body, d.shape_gap, &out};
+  // ---- pairs out, counted per route (analytic / MPR-GJK)
   for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
     unsigned key = L.pairs[pi];
     int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
-    int sa = L.gid[ra], sb = L.gid[rb];
-    ShapeW A = lds_shape(L, ra), B = lds_shape(L, rb);
-    float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
-    narrow_pair(sa, sb, L.type[ra], L.type[rb], A, B, gap_sum, wctx);
-    long long slot0 = 5ll * (base + pi);
-    for (int k = 0; k < out.n; ++k) {
-      long long s = slot0 + k;
-      st.shape0[s] = sa;
-      st.shape1[s] = sb;
-      store3(st.point0 + 3 * s, out.point0[k]);
-      store3(st.point1 + 3 * s, out.point1[k]);
-      store3(st.offset0 + 3 * s, out.offset0[k]);
-      store3(st.offset1 + 3 * s, out.offset1[k]);
-      store3(st.normal + 3 * s, out.normal[k]);
-      st.margin0[s] = out.margin0[k];
-      st.margin1[s] = out.margin1[k];
-      st.sub_key[s] = out.sub_key[k];
-    }
-    st.mask[base + pi] = static_cast<unsigned char>(out.n);
-    local += out.n;
+    st.pair_a[base + pi] = L.gid[ra];
+    st.pair_b[base + pi] = L.gid[rb];
+    st.pair_seg[base + pi] = seg;
+    atomicAdd(&s_cls[convex_route(L.type[ra], L.type[rb]) ? 1 : 0], 1);
   }
-  // block reduction of the segment's contact count
-  for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
-  if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = local;
   __syncthreads();
   if (threadIdx.x == 0) {
-    int t = 0;
-    for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += s_red[i];
-    seg_contact_count[seg] = t;
+    s_rbase[0] = atomicAdd(&st.route_count[0], s_cls[0]);
+    s_rbase[1] = atomicAdd(&st.route_count[1], s_cls[1]);
+    s_cls[0] = 0;
+    s_cls[1] = 0;
+  }
+  __syncthreads();
+  for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
+    unsigned key = L.pairs[pi];
+    int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
+    int cls = convex_route(L.type[ra], L.type[rb]) ? 1 : 0;
+    int r = s_rbase[cls] + atomicAdd(&s_cls[cls], 1);
+    st.route[cls ? cap_pairs - 1 - r : r] = base + pi;
+  }
+}
+
+// Narrow phase over every staged pair, one pair per lane: MPR/GJK pairs first (the long ones), then
+// the analytic pairs, so that a wave's lanes mostly run the same collider.
+__global__ void __launch_bounds__(CBLOCK) narrow_kernel(NmcModelDesc d, const float* __restrict__ body_q,
+                                                        StageBuffers st, int* __restrict__ seg_contact_count) {
+  const long long cap_pairs = st.slot_capacity / 5;
+  const int n_conv = st.route_count[1];
+  const int total = st.route_count[0] + n_conv;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    int g = i < n_conv ? st.route[cap_pairs - 1 - i] : st.route[i - n_conv];
+    int sa = st.pair_a[g], sb = st.pair_b[g];
+    ShapeW A = load_wc(st.shape_wc + 20ll * sa), B = load_wc(st.shape_wc + 20ll * sb);
+    float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
+    WriteCtx w{body_q, d.shape_body, d.shape_gap, st, 5ll * g, 0};
+    narrow_pair(sa, sb, d.shape_type[sa], d.shape_type[sb], A, B, gap_sum, w);
+    st.mask[g] = static_cast<unsigned char>(w.n);
+    if (w.n) atomicAdd(&seg_contact_count[st.pair_seg[g]], w.n);
   }
 }
 
 // Exclusive scan of per-segment contact counts -> Contacts.world_start / world_count / count.
 __global__ void __launch_bounds__(1024) contact_scan_kernel(const int* __restrict__ seg_contact_count, int segments,
                                                              NmcContacts c, int* __restrict__ stage_alloc,
+                                                             int* __restrict__ route_count,
                                                              int* __restrict__ counters) {
   __shared__ int s_part[1024];
   __shared__ int s_carry;
@@ -292,6 +335,8 @@ __global__ void __launch_bounds__(1024) contact_scan_kernel(const int* __restric
     c.count[0] = s_carry;
     if (s_carry > c.capacity) atomicAdd(&counters[2], s_carry - c.capacity);
     stage_alloc[0] = 0;
+    route_count[0] = 0;
+    route_count[1] = 0;
   }
 }
 
@@ -350,7 +395,7 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
 size_t collide_lds_bytes(int max_n, int pair_cap) {
   int n2 = 1;
   while (n2 < max_n) n2 <<= 1;
-  size_t b = sizeof(unsigned long long) * n2 + sizeof(float) * 21 * (size_t)max_n + sizeof(int) * 2 * (size_t)max_n;
+  size_t b = sizeof(unsigned long long) * n2 + sizeof(float) * 8 * (size_t)max_n + sizeof(int) * 6 * (size_t)max_n;
   b += sizeof(unsigned) * (size_t)pair_cap;
   return (b + 15) & ~size_t(15);
 }
@@ -362,8 +407,13 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
   hipLaunchKernelGGL(collide_segment_kernel, dim3(segs), dim3(CBLOCK), m->collide_lds, stream, d, body_q, m->stage,
                      m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc, m->counters,
                      m->pair_cap, d.max_segment_shapes);
+  long long pairs = m->stage.slot_capacity / 5;
+  long long nblk = (pairs + CBLOCK - 1) / CBLOCK;
+  int narrow_blocks = static_cast<int>(nblk < NARROW_MAX_BLOCKS ? nblk : NARROW_MAX_BLOCKS);
+  hipLaunchKernelGGL(narrow_kernel, dim3(narrow_blocks), dim3(CBLOCK), 0, stream, d, body_q, m->stage,
+                     m->seg_contact_count);
   hipLaunchKernelGGL(contact_scan_kernel, dim3(1), dim3(1024), 0, stream, m->seg_contact_count, segs, *c,
-                     m->stage_alloc, m->counters);
+                     m->stage_alloc, m->stage.route_count, m->counters);
   hipLaunchKernelGGL(contact_compact_kernel, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_count,
                      m->seg_pair_base, *c);
   return hipGetLastError();

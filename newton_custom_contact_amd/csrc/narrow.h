@@ -41,18 +41,14 @@ struct CData {  // ContactData (contact_data.py:17-56)
 };
 
 // Output of one pair: up to 5 records in sub-key order.
-struct PairOut {
-  int n;
-  int sub_key[5];
-  f3 point0[5], point1[5], offset0[5], offset1[5], normal[5];
-  float margin0[5], margin1[5];
-};
-
+// Destination of one pair's contacts: five consecutive staging slots starting at slot0.
 struct WriteCtx {
   const float* __restrict__ body_q;
   const int* __restrict__ shape_body;
   const float* __restrict__ shape_gap;
-  PairOut* out;
+  StageBuffers st;
+  long long slot0;
+  int n;  // records written so far
 };
 
 __device__ __forceinline__ f3 support(const GShape& g, f3 d) {
@@ -152,22 +148,24 @@ __device__ __forceinline__ void emit(const CData& c, WriteCtx& w, bool check_gap
   float d = contact_d(c, n, aw, bw);
   float gap = w.shape_gap[c.shape_a] + w.shape_gap[c.shape_b];
   if (check_gap && d > gap) return;
-  PairOut& o = *w.out;
-  int k = o.n;
-  if (k >= 5) return;
+  if (w.n >= 5) return;
+  const long long s = w.slot0 + w.n;
   float off_a = c.r_a + c.m_a, off_b = c.r_b + c.m_b;
   int b0 = w.shape_body[c.shape_a], b1 = w.shape_body[c.shape_b];
   xform Xa = b0 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b0));
   xform Xb = b1 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b1));
-  o.sub_key[k] = c.sub_key;
-  o.point0[k] = xpoint(Xa, aw);
-  o.point1[k] = xpoint(Xb, bw);
-  o.offset0[k] = xvector(Xa, off_a * n);
-  o.offset1[k] = xvector(Xb, (-off_b) * n);
-  o.normal[k] = n;
-  o.margin0[k] = off_a;
-  o.margin1[k] = off_b;
-  o.n = k + 1;
+  const StageBuffers& st = w.st;
+  st.shape0[s] = c.shape_a;
+  st.shape1[s] = c.shape_b;
+  st.sub_key[s] = c.sub_key;
+  store3(st.point0 + 3 * s, xpoint(Xa, aw));
+  store3(st.point1 + 3 * s, xpoint(Xb, bw));
+  store3(st.offset0 + 3 * s, xvector(Xa, off_a * n));
+  store3(st.offset1 + 3 * s, xvector(Xb, (-off_b) * n));
+  store3(st.normal + 3 * s, n);
+  st.margin0[s] = off_a;
+  st.margin1[s] = off_b;
+  w.n += 1;
 }
 
 __device__ __forceinline__ CData post_process(CData c, int type_a, int type_b) {
@@ -1028,10 +1026,16 @@ __device__ void solve_convex(const PairCtx& P, float threshold, bool skip_multi,
   manifold(P, rel_ob, rel_pb, point_a, point_b, normal, w);
 }
 
-// One broad-phase pair (sa, sb already ordered by GeoType) -> up to 5 records in w.out.
+// Pairs that go through MPR/GJK + the manifold (narrow_phase.py:136-605 routes every other supported
+// type pair to an analytic collider).
+__device__ __forceinline__ bool convex_route(int ta, int tb) {
+  return tb == GEO_BOX && (ta == GEO_BOX || ta == GEO_CAPSULE);
+}
+
+// One broad-phase pair (sa, sb already ordered by GeoType) -> up to 5 records at w.slot0.
 __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, const ShapeW& B, float gap_sum,
                             WriteCtx& w) {
-  w.out->n = 0;
+  w.n = 0;
   f3 scale_a = A.gscale, scale_b = B.gscale;
   f3 pos_a = A.X.p, pos_b = B.X.p;
   qf qa = A.X.q, qb = B.X.q;

@@ -88,7 +88,11 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       (e = dmalloc(&s.offset0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.offset1, 3 * slots)) != hipSuccess ||
       (e = dmalloc(&s.normal, 3 * slots)) != hipSuccess || (e = dmalloc(&s.margin0, slots)) != hipSuccess ||
       (e = dmalloc(&s.margin1, slots)) != hipSuccess || (e = dmalloc(&s.sub_key, slots)) != hipSuccess ||
-      (e = dmalloc(&s.mask, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&m->seg_pair_count, segs)) != hipSuccess ||
+      (e = dmalloc(&s.mask, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&s.pair_a, slots / 5 + 1)) != hipSuccess ||
+      (e = dmalloc(&s.pair_b, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&s.pair_seg, slots / 5 + 1)) != hipSuccess ||
+      (e = dmalloc(&s.route, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&s.route_count, 2)) != hipSuccess ||
+      (e = dmalloc(&s.shape_wc, 20 * (size_t)(d.shape_count > 0 ? d.shape_count : 1))) != hipSuccess ||
+      (e = dmalloc(&m->seg_pair_count, segs)) != hipSuccess ||
       (e = dmalloc(&m->seg_pair_base, segs)) != hipSuccess || (e = dmalloc(&m->seg_contact_count, segs)) != hipSuccess ||
       (e = dmalloc(&m->stage_alloc, 1)) != hipSuccess || (e = dmalloc(&m->counters, 4)) != hipSuccess ||
       (e = dmalloc(&m->xpbd_keys, m->xpbd_keys_cap)) != hipSuccess) {
@@ -103,7 +107,8 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
   if (!m) return;
   StageBuffers& s = m->stage;
   void* ptrs[] = {s.shape0, s.shape1, s.point0, s.point1, s.offset0, s.offset1, s.normal, s.margin0, s.margin1,
-                  s.sub_key, s.mask, m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc,
+                  s.sub_key, s.mask, s.pair_a, s.pair_b, s.pair_seg, s.route, s.route_count, s.shape_wc,
+                  m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc,
                   m->counters, m->xpbd_keys};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -130,6 +135,8 @@ extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q
   if (m->d.joint_count > 0 && (!joint_f || !tpos || !tvel)) return fail("nmc_xpbd_step: joint array is null");
   if (p->has_contacts && !c) return fail("nmc_xpbd_step: has_contacts set but contacts is null");
   if (q_in == q_out || qd_in == qd_out) return fail("nmc_xpbd_step: state_in and state_out must not alias");
+  if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
+    return fail("nmc_xpbd_step: contacts capacity exceeds the rigid_contact_max the model was created with");
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
                              qd_out, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step launch");

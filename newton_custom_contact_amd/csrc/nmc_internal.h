@@ -30,8 +30,15 @@ struct StageBuffers {
   float* margin0;
   float* margin1;
   int* sub_key;
-  unsigned char* mask;  // per pair: bit k set when slot k holds a contact
+  unsigned char* mask;  // per pair: number of records in its slots (0-5)
   long long slot_capacity;
+  // broad-phase output, one entry per pair (slot_capacity / 5 entries)
+  int* pair_a;      // first shape (lower GeoType), global id
+  int* pair_b;
+  int* pair_seg;    // segment the pair belongs to
+  int* route;       // pair indices: analytic pairs from the front, MPR/GJK pairs from the back
+  int* route_count; // [2] analytic, MPR/GJK
+  float* shape_wc;  // [shape_count x 20] world transform 7, scale 3, margin 1, AABB lo 3 / hi 3, pad 3
 };
 
 struct NmcModelImpl {

@@ -71,6 +71,32 @@ __device__ __forceinline__ qf qnormalize(qf q) {
   if (l > 0.0f) return {q.x / l, q.y / l, q.z / l, q.w / l};
   return {0.0f, 0.0f, 0.0f, 0.0f};
 }
+// asin / acos by the Cephes single-precision method (S. L. Moshier, Cephes Math Library, asinf.c and
+// acosf.c): odd polynomial in x^2 on |x| <= 0.5, half-angle reduction through sqrt above; about 2 ulp.
+// Written out operation by operation (no contraction) so every IEEE-conforming FP32 unit computes the
+// same bits.
+__device__ __forceinline__ float asin_c(float x) {
+  float a = fabsf(x);
+  float z, r;
+  bool big = a > 0.5f;
+  if (big) {
+    z = 0.5f * (1.0f - a);
+    r = sqrtf(z);
+  } else {
+    z = a * a;
+    r = a;
+  }
+  float poly = (((4.2163199048e-2f * z + 2.4181311049e-2f) * z + 4.5470025998e-2f) * z + 7.4953002686e-2f) * z +
+               1.6666752422e-1f;
+  float y = poly * z * r + r;
+  if (big) y = 1.5707963267948966f - (y + y);
+  return x < 0.0f ? -y : y;
+}
+__device__ __forceinline__ float acos_c(float x) {
+  if (x < -0.5f) return 3.14159265358979323846f - 2.0f * asin_c(sqrtf(0.5f * (1.0f + x)));
+  if (x > 0.5f) return 2.0f * asin_c(sqrtf(0.5f * (1.0f - x)));
+  return 1.5707963267948966f - asin_c(x);
+}
 __device__ __forceinline__ f3 rot(qf q, f3 x) {
   float c = 2.0f * q.w * q.w - 1.0f;
   float d = 2.0f * (q.x * x.x + q.y * x.y + q.z * x.z);

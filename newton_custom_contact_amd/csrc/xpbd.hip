@@ -347,7 +347,7 @@ __device__ bool joint_solve(const NmcModelDesc& d, const WorldLds& L, int j, int
     float invs = 1.0f / s;
     float invscube = invs * invs * invs;
     float errs[3];
-    errs[0] = 2.0f * asinf(clampf(qtwist.x, -1.0f, 1.0f));
+    errs[0] = 2.0f * asin_c(clampf(qtwist.x, -1.0f, 1.0f));
     errs[1] = qswing.y;
     errs[2] = qswing.z;
     qf g[3];
@@ -360,7 +360,7 @@ __device__ bool joint_solve(const NmcModelDesc& d, const WorldLds& L, int j, int
     float swing_sq = qswing.w * qswing.w;
     if (swing_sq + 1.0e-4f < 1.0f) {
       float dd = sqrtf(1.0f - qswing.w * qswing.w);
-      float theta = 2.0f * acosf(clampf(qswing.w, -1.0f, 1.0f));
+      float theta = 2.0f * acos_c(clampf(qswing.w, -1.0f, 1.0f));
       float scale = theta / dd;
       errs[1] *= scale;
       errs[2] *= scale;

@@ -45,16 +45,16 @@ class _Placer:
     def __init__(self, rng: np.random.Generator, z_hi: float = 2.5):
         self.rng = rng
         self.z_hi = z_hi
-        self.pts: list[np.ndarray] = []
-        self.rad: list[float] = []
+        self.pts = np.zeros((0, 3))
+        self.rad = np.zeros(0)
 
     def place(self, radius: float) -> np.ndarray:
         while True:
             for _ in range(2000):
                 p = np.array([self.rng.uniform(-1, 1), self.rng.uniform(-1, 1), self.rng.uniform(0.5, self.z_hi)])
-                if all(np.linalg.norm(p - q) > radius + r for q, r in zip(self.pts, self.rad)):
-                    self.pts.append(p)
-                    self.rad.append(radius)
+                if np.all(np.linalg.norm(self.pts - p, axis=1) > radius + self.rad):
+                    self.pts = np.vstack([self.pts, p])
+                    self.rad = np.append(self.rad, radius)
                     return p
             self.z_hi += 0.25
 

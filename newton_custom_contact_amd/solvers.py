@@ -62,7 +62,12 @@ class SolverXPBD(SolverBase):
         self.rigid_contact_con_weighting = rigid_contact_con_weighting
         self.angular_damping = angular_damping
         self.enable_restitution = enable_restitution
-        self.native = NativeModel(model, rigid_contact_max=1, shape_pairs_max=1)
+        # the solver orders each dense world's (contact, body) entries in a buffer of 2 per contact slot,
+        # sized for the largest Contacts buffer seen (the pipeline's default capacity to begin with)
+        from .collide import estimate_rigid_contact_max  # noqa: PLC0415
+
+        self._contact_capacity = max(1, estimate_rigid_contact_max(model))
+        self.native = NativeModel(model, rigid_contact_max=self._contact_capacity, shape_pairs_max=1)
         self._lib = load()
         self._control = None
 
@@ -84,6 +89,11 @@ class SolverXPBD(SolverBase):
         if state_in.body_q is state_out.body_q:
             raise ValueError("state_in and state_out must be different State objects")
         p = self.params(contacts is not None)
+        if contacts is not None and contacts.rigid_contact_max > self._contact_capacity:
+            # a larger Contacts buffer than any before: regrow the ordering buffer (allocates; do it
+            # before capturing the step in a graph)
+            self._contact_capacity = contacts.rigid_contact_max
+            self.native = NativeModel(model, rigid_contact_max=self._contact_capacity, shape_pairs_max=1)
         cv = ctypes.byref(contacts.native_view()) if contacts is not None else None
         for t in (state_in.body_q, state_in.body_qd, state_in.body_f, state_out.body_q, state_out.body_qd):
             if t.device != model.device or not t.is_contiguous():

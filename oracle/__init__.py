@@ -71,8 +71,26 @@ def lib():
         L.oracle_collide_f64.restype = ctypes.c_int
         L.oracle_xpbd_step_f32.restype = None
         L.oracle_xpbd_step_f64.restype = None
+        L.oracle_asin_f32.restype = None
+        L.oracle_acos_f32.restype = None
         _LIB = L
     return _LIB
+
+
+def asin_f32(x: np.ndarray) -> np.ndarray:
+    """The restatement's FP32 asin (revolute twist angle, kernels.py:1441-1943)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().oracle_asin_f32(_p(x, _f32p), ctypes.c_int(x.size), _p(out, _f32p))
+    return out
+
+
+def acos_f32(x: np.ndarray) -> np.ndarray:
+    """The restatement's FP32 acos (revolute swing angle)."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().oracle_acos_f32(_p(x, _f32p), ctypes.c_int(x.size), _p(out, _f32p))
+    return out
 
 
 def _p(a: np.ndarray, t):

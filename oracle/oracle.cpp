@@ -119,3 +119,11 @@ extern "C" void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParam
   step_impl<double, double>(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, nc, shape0, shape1, point0, point1,
                             offset0, offset1, normal, margin0, margin1, dt, q_out, qd_out);
 }
+
+extern "C" void oracle_asin_f32(const float* x, int n, float* out) {
+  for (int i = 0; i < n; ++i) out[i] = tasin(x[i]);
+}
+
+extern "C" void oracle_acos_f32(const float* x, int n, float* out) {
+  for (int i = 0; i < n; ++i) out[i] = tacos(x[i]);
+}

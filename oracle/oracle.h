@@ -102,6 +102,10 @@ void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const
                           const double* margin0, const double* margin1, double dt, double* body_q_out,
                           double* body_qd_out);
 
+// Elementwise FP32 asin / acos of the restatement (the revolute joint's twist and swing angles).
+void oracle_asin_f32(const float* x, int n, float* out);
+void oracle_acos_f32(const float* x, int n, float* out);
+
 #ifdef __cplusplus
 }
 #endif

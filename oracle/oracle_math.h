@@ -35,6 +35,32 @@ template <class T> inline T tclamp(T x, T lo, T hi) { return tmin(tmax(lo, x), h
 template <class T> inline T tabs(T a) { return std::fabs(a); }
 template <class T> inline T tsqrt(T a) { return std::sqrt(a); }
 
+// asin / acos.  FP64: libm.  FP32: the Cephes single-precision method (S. L. Moshier, Cephes Math
+// Library, asinf.c / acosf.c), evaluated operation by operation so that it is reproducible on any
+// IEEE FP32 unit; within 2 ulp of the exact value (pinned against float64 arcsin/arccos in the tests).
+inline double tasin(double x) { return std::asin(x); }
+inline double tacos(double x) { return std::acos(x); }
+inline float tasin(float x) {
+  float a = std::fabs(x);
+  bool big = a > 0.5f;
+  float z = big ? 0.5f * (1.0f - a) : a * a;
+  float r = big ? std::sqrt(z) : a;
+  float poly = 4.2163199048e-2f;
+  poly = poly * z + 2.4181311049e-2f;
+  poly = poly * z + 4.5470025998e-2f;
+  poly = poly * z + 7.4953002686e-2f;
+  poly = poly * z + 1.6666752422e-1f;
+  float y = poly * z * r + r;
+  if (big) y = static_cast<float>(1.5707963267948966) - (y + y);
+  return x < 0.0f ? -y : y;
+}
+inline float tacos(float x) {
+  const float pi = static_cast<float>(3.14159265358979323846), half_pi = static_cast<float>(1.5707963267948966);
+  if (x < -0.5f) return pi - 2.0f * tasin(std::sqrt(0.5f * (1.0f + x)));
+  if (x > 0.5f) return 2.0f * tasin(std::sqrt(0.5f * (1.0f - x)));
+  return half_pi - tasin(x);
+}
+
 template <class T> inline V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
 template <class T> inline V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
 template <class T> inline V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }

@@ -392,7 +392,7 @@ template <class T> bool joint_solve(const XModel<T>& X, const std::vector<Body<T
     T s = tsqrt(rel_q.x * rel_q.x + rel_q.w * rel_q.w);
     T invs = T(1) / s;
     T invscube = invs * invs * invs;
-    T err_0 = T(2) * std::asin(tclamp(qtwist.x, T(-1), T(1)));
+    T err_0 = T(2) * tasin(tclamp(qtwist.x, T(-1), T(1)));
     T err_1 = qswing.y, err_2 = qswing.z;
     Q<T> g0{invs - rel_q.x * rel_q.x * invscube, T(0), T(0), -(rel_q.w * rel_q.x) * invscube};
     Q<T> g1{-rel_q.w * (rel_q.w * rel_q.z + rel_q.x * rel_q.y) * invscube, rel_q.w * invs, -rel_q.x * invs,
@@ -404,7 +404,7 @@ template <class T> bool joint_solve(const XModel<T>& X, const std::vector<Body<T
     const T angularEps = T(1.0e-4);
     if (swing_sq + angularEps < T(1)) {
       T d = tsqrt(T(1) - qswing.w * qswing.w);
-      T theta = T(2) * std::acos(tclamp(qswing.w, T(-1), T(1)));
+      T theta = T(2) * tacos(tclamp(qswing.w, T(-1), T(1)));
       T scale = theta / d;
       err_1 *= scale;
       err_2 *= scale;

@@ -248,3 +248,26 @@ def test_fp32_vs_fp64_restatement():
     # 1e-7 m in a resolved penetration becomes ~1e-5 m/s per correction at dt = 1/240 s, and ten
     # iterations of coupled contacts amplify it; 1e-2 m/s bounds it on this scene
     np.testing.assert_allclose(qd32, qd64, atol=1e-2)
+
+
+def test_fp32_asin_acos_error_bound():
+    """The revolute twist/swing angles use an FP32 asin/acos (Cephes method) that the HIP kernels
+    share bit for bit; it is pinned here against float64 arcsin/arccos: asin within 3 ulp, acos
+    within 2 ulp, exact at the special points, odd/even symmetry."""
+    import oracle
+
+    x = np.linspace(-1, 1, 2_000_001, dtype=np.float32)
+    extra = np.float32([-1, -0.5, 0.5, 1, 0, 1e-8, -1e-8, np.nextafter(np.float32(0.5), np.float32(1)),
+                        np.nextafter(np.float32(-0.5), np.float32(-1))])
+    x = np.unique(np.concatenate([x, extra]))
+    for f, ref, bound in ((oracle.asin_f32, np.arcsin, 3.0), (oracle.acos_f32, np.arccos, 2.0)):
+        y = f(x).astype(np.float64)
+        r = ref(x.astype(np.float64))
+        ulp = np.spacing(np.abs(r).astype(np.float32)).astype(np.float64)
+        assert np.max(np.abs(y - r) / ulp) <= bound
+    a = oracle.asin_f32(np.float32([0.0, 1.0, -1.0, 0.5]))
+    assert a[0] == 0.0 and a[1] == np.float32(np.pi / 2) and a[2] == -np.float32(np.pi / 2)
+    assert abs(float(a[3]) - np.pi / 6) < 1e-7
+    assert np.array_equal(oracle.asin_f32(-x), -oracle.asin_f32(x))
+    c = oracle.acos_f32(np.float32([1.0, -1.0, 0.0]))
+    assert c[0] == 0.0 and c[1] == np.float32(np.pi) and c[2] == np.float32(np.pi / 2)
