@@ -87,8 +87,10 @@ typedef struct NmcModelDesc {
   const float* joint_limit_upper;
   const float* joint_target_ke;
   const float* joint_target_kd;
-  /* capacities: nmc_collide stages up to 5 records per broad-phase pair (shape_pairs_max pairs);
-   * nmc_xpbd_step accepts Contacts buffers of at most rigid_contact_max slots */
+  /* capacities: nmc_xpbd_step accepts Contacts buffers of at most rigid_contact_max slots.
+   * shape_pairs_max > 0 creates the collision buffers (each segment gets a fixed region for its
+   * worst case n(n-1)/2 pairs, 5 contact slots per pair); 0 creates a solver-only model on which
+   * nmc_collide fails. */
   int rigid_contact_max;
   int shape_pairs_max;
 } NmcModelDesc;
@@ -140,10 +142,10 @@ int nmc_xpbd_step(NmcModel* model, const NmcXpbdParams* params, const float* bod
                   const float* joint_target_vel, const NmcContacts* contacts, float dt, float* body_q_out,
                   float* body_qd_out, nmc_stream_t stream);
 
-/* Copies the overflow counters to host (synchronises the stream):
- * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = staged contacts dropped,
- * out[2] = contacts beyond capacity, out[3] = segments whose contacts exceed the solver's per-world
- * ordering buffer (solved in the slower global-memory path). */
+/* Copies the counters accumulated since model creation to host (synchronises the stream):
+ * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = reserved (0),
+ * out[2] = contacts beyond the Contacts capacity, out[3] = worlds whose contacts did not fit the
+ * solver's ordering buffer (cannot happen when contacts->capacity <= rigid_contact_max). */
 int nmc_get_counters(NmcModel* model, int* out, nmc_stream_t stream);
 
 const char* nmc_last_error(void);

@@ -151,7 +151,7 @@ class CollisionPipeline:
     def overflow_counters(self) -> dict:
         """Overflow counters of every collide so far (verify_narrow_phase_buffers, narrow_phase.py:1334)."""
         c = self.native.counters()
-        return {"pairs_dropped": int(c[0]), "staged_contacts_dropped": int(c[1]), "contacts_over_capacity": int(c[2]),
+        return {"pairs_dropped": int(c[0]), "contacts_over_capacity": int(c[2]),
                 "solver_ordering_overflow": int(c[3])}
 
 

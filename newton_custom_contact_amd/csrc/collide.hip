@@ -6,16 +6,19 @@
 // axis (broad_phase_sap.py:43-394: project, bitonic sort of (lower, index) keys, sweep with the
 // world / group / exclusion / AABB tests), then a bitonic sort of the surviving pairs by contact sort
 // key.  The narrow phase then runs over all pairs of all segments at once, one pair per lane, with the
-// MPR/GJK pairs grouped apart from the analytic ones.  Each pair owns five staging slots; a
-// per-segment scan and a grid-wide scan place every contact at its final index, so the Contacts
-// buffer comes out in canonical order (segment-major, sort key inside a segment) with no float
-// atomics and no global sort.
+// MPR/GJK pairs grouped apart from the analytic ones.  Each segment owns a fixed region of pair
+// entries and each pair five contact slots, so no kernel uses a global atomic counter (same-address
+// atomics serialise across the chip's eight L2 slices); the two cross-segment prefix sums are done by
+// one-workgroup scan kernels.  The Contacts buffer comes out in canonical order (segment-major, sort
+// key inside a segment) with no float atomics and no global sort.
 //
 // Launch sequence of nmc_collide (all on the caller's stream, graph-capturable):
-//   collide_segment_kernel  <<<segments, 256>>>   AABB + SAP + pair sort -> staged pairs, routes
-//   narrow_kernel           <<<<=2048, 256>>>     narrow phase -> staging slots, per-segment counts
-//   contact_scan_kernel     <<<1, 1024>>>         segment offsets, total count
-//   contact_compact_kernel  <<<segments, 256>>>   staging -> Contacts arrays
+//   collide_segment_kernel  <<<segments, 256>>>   AABB + SAP + pair sort -> pair region, route list
+//   route_scan_kernel       <<<1, 1024>>>         route offsets over segments
+//   narrow_kernel           <<<<=2048, 256>>>     narrow phase -> contact slots, per-pair counts
+//   contact_count_kernel    <<<segments, 64>>>    contacts per segment
+//   contact_scan_kernel     <<<1, 1024>>>         Contacts.world_start / world_count / count
+//   contact_compact_kernel  <<<segments, 256>>>   slots -> Contacts arrays
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -87,6 +90,55 @@ __device__ __forceinline__ int next_pow2(int n) {
   return p;
 }
 
+// Inclusive scan of one value per lane over a CBLOCK-lane workgroup (wave shuffles + 4 wave totals).
+__device__ __forceinline__ int block_inclusive_scan(int v, int* s_wave) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int off = 1; off < 64; off <<= 1) {
+    int t = __shfl_up(v, off, 64);
+    if (lane >= off) v += t;
+  }
+  if (lane == 63) s_wave[w] = v;
+  __syncthreads();
+  int add = 0;
+  for (int i = 0; i < w; ++i) add += s_wave[i];
+  __syncthreads();
+  return v + add;
+}
+
+// Exclusive scans of two per-segment count arrays by one 1024-lane workgroup; returns the totals.
+constexpr int SCAN_BLOCK = 1024;
+__device__ void segment_scan2(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ a_out,
+                              int* __restrict__ b_out, int n, int& a_total, int& b_total) {
+  __shared__ int s_wa[SCAN_BLOCK / 64], s_wb[SCAN_BLOCK / 64];
+  __shared__ int s_carry[2];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) { s_carry[0] = 0; s_carry[1] = 0; }
+  __syncthreads();
+  for (int base = 0; base < n; base += SCAN_BLOCK) {
+    const int i = base + threadIdx.x;
+    const int va = i < n ? a[i] : 0;
+    const int vb = (b && i < n) ? b[i] : 0;
+    int ia = va, ib = vb;
+    for (int off = 1; off < 64; off <<= 1) {
+      int ta = __shfl_up(ia, off, 64), tb = __shfl_up(ib, off, 64);
+      if (lane >= off) { ia += ta; ib += tb; }
+    }
+    if (lane == 63) { s_wa[w] = ia; s_wb[w] = ib; }
+    __syncthreads();
+    int ca = s_carry[0], cb = s_carry[1];
+    for (int k = 0; k < w; ++k) { ca += s_wa[k]; cb += s_wb[k]; }
+    if (i < n) {
+      a_out[i] = ca + ia - va;
+      if (b_out) b_out[i] = cb + ib - vb;
+    }
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) { s_carry[0] = ca + ia; s_carry[1] = cb + ib; }
+    __syncthreads();
+  }
+  a_total = s_carry[0];
+  b_total = s_carry[1];
+}
+
 struct SegLds {
   float* lo;   // 3 per shape, AABB expanded by margin + gap
   float* hi;   // 3
@@ -123,15 +175,15 @@ __device__ __forceinline__ ShapeW load_wc(const float* __restrict__ o) {
 // Broad phase of one segment: shape world data (kept in shape_wc for the narrow phase), SAP, the
 // segment's pairs in key order at staging pairs [base, base + np), and their route entries.
 __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d, const float* __restrict__ body_q,
-                                                                  StageBuffers st, int* __restrict__ seg_pair_count,
-                                                                  int* __restrict__ seg_pair_base,
-                                                                  int* __restrict__ seg_contact_count,
-                                                                  int* __restrict__ stage_alloc,
+                                                                  StageBuffers st,
+                                                                  const long long* __restrict__ seg_pair_off,
+                                                                  const int* __restrict__ seg_pair_cap,
+                                                                  int* __restrict__ seg_pair_count,
+                                                                  int* __restrict__ seg_conv_count,
                                                                   int* __restrict__ counters, int pair_cap,
                                                                   int max_n) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_npairs;
-  __shared__ int s_base;
   __shared__ int s_cls[2];
   __shared__ int s_rbase[2];
   const int seg = blockIdx.x;
@@ -159,7 +211,6 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
     s_npairs = 0;
     s_cls[0] = 0;
     s_cls[1] = 0;
-    seg_contact_count[seg] = 0;
   }
 
   // ---- shape world transforms, AABBs, SAP projections (collide.py:150-275, broad_phase_sap.py:43-63)
@@ -235,141 +286,145 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
   __syncthreads();
   bitonic_sort(L.pairs, pn2);
 
-  // ---- staging allocation (one pair entry + 5 contact slots per pair)
-  const long long cap_pairs = st.slot_capacity / 5;
-  if (threadIdx.x == 0) {
-    int base = atomicAdd(stage_alloc, np);
-    int keep = np;
-    if (static_cast<long long>(base) + np > cap_pairs) {
-      keep = static_cast<int>(cap_pairs - base);
-      if (keep < 0) keep = 0;
-      atomicAdd(&counters[1], np - keep);
-    }
-    s_base = base;
-    seg_pair_base[seg] = base;
-    seg_pair_count[seg] = keep;
-    s_npairs = keep;
-  }
-  __syncthreads();
-  np = s_npairs;
-  const int base = s_base;
-
-  // ---- pairs out, counted per route (analytic / MPR-GJK)
+  // ---- pairs out into the segment's region (key order) + its route list (MPR/GJK first)
+  const long long poff = seg_pair_off[seg];
+  const int pcap = seg_pair_cap[seg];
+  if (np > pcap) np = pcap;  // cannot happen: np <= n(n-1)/2 and np <= pair_cap
   for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
     unsigned key = L.pairs[pi];
     int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
-    st.pair_a[base + pi] = L.gid[ra];
-    st.pair_b[base + pi] = L.gid[rb];
-    st.pair_seg[base + pi] = seg;
+    st.pair_a[poff + pi] = L.gid[ra];
+    st.pair_b[poff + pi] = L.gid[rb];
     atomicAdd(&s_cls[convex_route(L.type[ra], L.type[rb]) ? 1 : 0], 1);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    s_rbase[0] = atomicAdd(&st.route_count[0], s_cls[0]);
-    s_rbase[1] = atomicAdd(&st.route_count[1], s_cls[1]);
+    seg_pair_count[seg] = np;
+    seg_conv_count[seg] = s_cls[1];
+    s_rbase[1] = 0;
+    s_rbase[0] = s_cls[1];
     s_cls[0] = 0;
     s_cls[1] = 0;
   }
   __syncthreads();
   for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
     unsigned key = L.pairs[pi];
-    int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
-    int cls = convex_route(L.type[ra], L.type[rb]) ? 1 : 0;
-    int r = s_rbase[cls] + atomicAdd(&s_cls[cls], 1);
-    st.route[cls ? cap_pairs - 1 - r : r] = base + pi;
+    int cls = convex_route(L.type[key >> 16], L.type[key & 0xffffu]) ? 1 : 0;
+    st.route[poff + s_rbase[cls] + atomicAdd(&s_cls[cls], 1)] = pi;
+  }
+}
+
+// Route offsets: exclusive prefix of MPR/GJK and analytic pair counts over segments.
+__global__ void __launch_bounds__(SCAN_BLOCK) route_scan_kernel(const int* __restrict__ seg_pair_count,
+                                                               const int* __restrict__ seg_conv_count,
+                                                               int* __restrict__ seg_conv_off,
+                                                               int* __restrict__ seg_ana_off,
+                                                               int* __restrict__ route_count, int segments) {
+  int conv_total, pair_total;
+  // pair offsets minus conv offsets = analytic offsets
+  segment_scan2(seg_conv_count, seg_pair_count, seg_conv_off, seg_ana_off, segments, conv_total, pair_total);
+  for (int i = threadIdx.x; i < segments; i += blockDim.x) seg_ana_off[i] -= seg_conv_off[i];
+  if (threadIdx.x == 0) {
+    route_count[0] = pair_total - conv_total;
+    route_count[1] = conv_total;
   }
 }
 
 // Narrow phase over every staged pair, one pair per lane: MPR/GJK pairs first (the long ones), then
-// the analytic pairs, so that a wave's lanes mostly run the same collider.
+// the analytic pairs, so that a wave's lanes mostly run the same collider.  Lane i finds its segment
+// by binary search over the route offsets.
 __global__ void __launch_bounds__(CBLOCK) narrow_kernel(NmcModelDesc d, const float* __restrict__ body_q,
-                                                        StageBuffers st, int* __restrict__ seg_contact_count) {
-  const long long cap_pairs = st.slot_capacity / 5;
+                                                        StageBuffers st, const long long* __restrict__ seg_pair_off,
+                                                        const int* __restrict__ seg_conv_count,
+                                                        const int* __restrict__ seg_conv_off,
+                                                        const int* __restrict__ seg_ana_off, int segments) {
   const int n_conv = st.route_count[1];
   const int total = st.route_count[0] + n_conv;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
-    int g = i < n_conv ? st.route[cap_pairs - 1 - i] : st.route[i - n_conv];
+    const bool conv = i < n_conv;
+    const int key = conv ? i : i - n_conv;
+    const int* __restrict__ offs = conv ? seg_conv_off : seg_ana_off;
+    int lo = 0, hi = segments - 1;  // last segment whose offset is <= key
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (offs[mid] <= key) lo = mid; else hi = mid - 1;
+    }
+    const long long poff = seg_pair_off[lo];
+    const int r = key - offs[lo] + (conv ? 0 : seg_conv_count[lo]);
+    const long long g = poff + st.route[poff + r];
     int sa = st.pair_a[g], sb = st.pair_b[g];
     ShapeW A = load_wc(st.shape_wc + 20ll * sa), B = load_wc(st.shape_wc + 20ll * sb);
     float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
     WriteCtx w{body_q, d.shape_body, d.shape_gap, st, 5ll * g, 0};
     narrow_pair(sa, sb, d.shape_type[sa], d.shape_type[sb], A, B, gap_sum, w);
     st.mask[g] = static_cast<unsigned char>(w.n);
-    if (w.n) atomicAdd(&seg_contact_count[st.pair_seg[g]], w.n);
   }
 }
 
-// Exclusive scan of per-segment contact counts -> Contacts.world_start / world_count / count.
-__global__ void __launch_bounds__(1024) contact_scan_kernel(const int* __restrict__ seg_contact_count, int segments,
-                                                             NmcContacts c, int* __restrict__ stage_alloc,
-                                                             int* __restrict__ route_count,
-                                                             int* __restrict__ counters) {
-  __shared__ int s_part[1024];
-  __shared__ int s_carry;
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (int base = 0; base < segments; base += blockDim.x) {
-    int i = base + threadIdx.x;
-    int v = i < segments ? seg_contact_count[i] : 0;
-    s_part[threadIdx.x] = v;
-    __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-      int t = threadIdx.x >= (unsigned)off ? s_part[threadIdx.x - off] : 0;
-      __syncthreads();
-      s_part[threadIdx.x] += t;
-      __syncthreads();
-    }
-    int incl = s_part[threadIdx.x] + s_carry;
-    int excl = incl - v;
-    if (i < segments) {
-      c.world_start[i] = excl;
-      int room = c.capacity - excl;
-      int stored = v < room ? v : room;
-      c.world_count[i] = stored > 0 ? stored : 0;
-    }
-    __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) s_carry = incl;
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    c.count[0] = s_carry;
-    if (s_carry > c.capacity) atomicAdd(&counters[2], s_carry - c.capacity);
-    stage_alloc[0] = 0;
-    route_count[0] = 0;
-    route_count[1] = 0;
-  }
-}
-
-// Staging -> Contacts arrays at canonical indices.
-__global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st, const int* __restrict__ seg_pair_count,
-                                                                  const int* __restrict__ seg_pair_base, NmcContacts c) {
-  __shared__ int s_scan[CBLOCK];
-  __shared__ int s_carry;
+// Contacts per segment (sum of its pairs' record counts).
+__global__ void __launch_bounds__(64) contact_count_kernel(StageBuffers st, const long long* __restrict__ seg_pair_off,
+                                                           const int* __restrict__ seg_pair_count,
+                                                           int* __restrict__ seg_contact_count) {
   const int seg = blockIdx.x;
   const int np = seg_pair_count[seg];
-  const int base = seg_pair_base[seg];
-  const int start = c.world_start[seg];
-  const int stored = c.world_count[seg];
-  if (threadIdx.x == 0) s_carry = 0;
-  __syncthreads();
-  for (int b0 = 0; b0 < np; b0 += blockDim.x) {
-    int pi = b0 + threadIdx.x;
-    int cnt = pi < np ? static_cast<int>(st.mask[base + pi]) : 0;
-    s_scan[threadIdx.x] = cnt;
+  const long long poff = seg_pair_off[seg];
+  int local = 0;
+  for (int pi = threadIdx.x; pi < np; pi += 64) local += st.mask[poff + pi];
+  for (int off = 32; off > 0; off >>= 1) local += __shfl_down(local, off, 64);
+  if (threadIdx.x == 0) seg_contact_count[seg] = local;
+}
+
+// Contact offsets: exclusive prefix over segments -> Contacts.world_start / world_count / count.
+__global__ void __launch_bounds__(SCAN_BLOCK) contact_scan_kernel(const int* __restrict__ seg_contact_count,
+                                                                 NmcContacts c, int segments,
+                                                                 int* __restrict__ counters) {
+  int total, unused;
+  segment_scan2(seg_contact_count, nullptr, c.world_start, nullptr, segments, total, unused);
+  for (int i = threadIdx.x; i < segments; i += blockDim.x) {
+    int start = c.world_start[i];
+    int room = c.capacity - start;
+    int v = seg_contact_count[i];
+    int stored = v < room ? v : room;
+    c.world_count[i] = stored > 0 ? stored : 0;
+  }
+  if (threadIdx.x == 0) {
+    c.count[0] = total;
+    if (total > c.capacity) atomicAdd(&counters[2], total - c.capacity);
+  }
+}
+
+// Staging -> Contacts at canonical indices, one lane per output contact (coalesced stores).
+__global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st,
+                                                                  const long long* __restrict__ seg_pair_off,
+                                                                  const int* __restrict__ seg_pair_count,
+                                                                  NmcContacts c) {
+  __shared__ int s_scan[CBLOCK];
+  __shared__ int s_wave[CBLOCK / 64];
+  const int seg = blockIdx.x;
+  const int np = seg_pair_count[seg];
+  const long long poff = seg_pair_off[seg];
+  const int start = c.world_start[seg], stored = c.world_count[seg];
+  int run = 0;  // contacts of the earlier chunks
+  for (int b0 = 0; b0 < np && run < stored; b0 += CBLOCK) {
+    const int pi = b0 + threadIdx.x;
+    const int cnt = pi < np ? static_cast<int>(st.mask[poff + pi]) : 0;
+    const int incl = block_inclusive_scan(cnt, s_wave);
+    s_scan[threadIdx.x] = incl;
     __syncthreads();
-    for (int off = 1; off < (int)blockDim.x; off <<= 1) {
-      int t = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
-      __syncthreads();
-      s_scan[threadIdx.x] += t;
-      __syncthreads();
-    }
-    int excl = s_scan[threadIdx.x] - cnt + s_carry;
-    for (int k = 0; k < cnt; ++k) {
-      int local = excl + k;
-      if (local >= stored) break;
-      long long s = 5ll * (base + pi) + k;
-      int o = start + local;
-      int s0 = st.shape0[s], s1 = st.shape1[s];
+    const int chunk = s_scan[CBLOCK - 1];
+    for (int t = threadIdx.x; t < chunk; t += CBLOCK) {
+      const int local_idx = run + t;
+      if (local_idx >= stored) break;
+      int lo = 0, hi = CBLOCK - 1;  // first pair of the chunk whose inclusive count exceeds t
+      while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (s_scan[mid] > t) hi = mid; else lo = mid + 1;
+      }
+      const int k = t - (lo > 0 ? s_scan[lo - 1] : 0);
+      const long long g = poff + b0 + lo;
+      const long long s = 5 * g + k;
+      const int o = start + local_idx;
+      const int s0 = st.pair_a[g], s1 = st.pair_b[g];
       c.shape0[o] = s0;
       c.shape1[o] = s1;
       for (int q = 0; q < 3; ++q) {
@@ -386,8 +441,7 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
                         (static_cast<long long>(st.sub_key[s]) & 0x7FFFFF);
       }
     }
-    __syncthreads();
-    if (threadIdx.x == blockDim.x - 1) s_carry = excl + cnt;
+    run += chunk;
     __syncthreads();
   }
 }
@@ -405,17 +459,21 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
   int segs = d.segment_count;
   if (segs <= 0) return hipSuccess;
   hipLaunchKernelGGL(collide_segment_kernel, dim3(segs), dim3(CBLOCK), m->collide_lds, stream, d, body_q, m->stage,
-                     m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc, m->counters,
+                     m->seg_pair_off, m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count, m->counters,
                      m->pair_cap, d.max_segment_shapes);
-  long long pairs = m->stage.slot_capacity / 5;
+  hipLaunchKernelGGL(route_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, m->seg_pair_count, m->seg_conv_count,
+                     m->seg_conv_off, m->seg_ana_off, m->stage.route_count, segs);
+  long long pairs = m->stage.pair_capacity;
   long long nblk = (pairs + CBLOCK - 1) / CBLOCK;
   int narrow_blocks = static_cast<int>(nblk < NARROW_MAX_BLOCKS ? nblk : NARROW_MAX_BLOCKS);
   hipLaunchKernelGGL(narrow_kernel, dim3(narrow_blocks), dim3(CBLOCK), 0, stream, d, body_q, m->stage,
-                     m->seg_contact_count);
-  hipLaunchKernelGGL(contact_scan_kernel, dim3(1), dim3(1024), 0, stream, m->seg_contact_count, segs, *c,
-                     m->stage_alloc, m->stage.route_count, m->counters);
-  hipLaunchKernelGGL(contact_compact_kernel, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_count,
-                     m->seg_pair_base, *c);
+                     m->seg_pair_off, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, segs);
+  hipLaunchKernelGGL(contact_count_kernel, dim3(segs), dim3(64), 0, stream, m->stage, m->seg_pair_off,
+                     m->seg_pair_count, m->seg_contact_count);
+  hipLaunchKernelGGL(contact_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, m->seg_contact_count, *c, segs,
+                     m->counters);
+  hipLaunchKernelGGL(contact_compact_kernel, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_off,
+                     m->seg_pair_count, *c);
   return hipGetLastError();
 }
 

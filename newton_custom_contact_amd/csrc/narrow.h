@@ -155,8 +155,6 @@ __device__ __forceinline__ void emit(const CData& c, WriteCtx& w, bool check_gap
   xform Xa = b0 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b0));
   xform Xb = b1 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b1));
   const StageBuffers& st = w.st;
-  st.shape0[s] = c.shape_a;
-  st.shape1[s] = c.shape_b;
   st.sub_key[s] = c.sub_key;
   store3(st.point0 + 3 * s, xpoint(Xa, aw));
   store3(st.point1 + 3 * s, xpoint(Xb, bw));

@@ -4,6 +4,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "nmc_internal.h"
 
@@ -66,13 +67,6 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   if (pair_cap < 1) pair_cap = 1;
   m->pair_cap = next_pow2(pair_cap);
   m->collide_lds = collide_lds_bytes(d.max_segment_shapes > 0 ? d.max_segment_shapes : 1, m->pair_cap);
-  // staging: 5 slots per broad-phase pair; the pair bound is the reference's shape_pairs_max, and the
-  // slot count is capped by 2x the contact capacity (a pair yields 1-5 contacts, most yield 1-2)
-  long long pairs = d.shape_pairs_max > 0 ? d.shape_pairs_max : 1;
-  long long slots = 5 * pairs;
-  long long cap2 = 2ll * (d.rigid_contact_max > 0 ? d.rigid_contact_max : 1) + 5ll * m->pair_cap;
-  if (slots > cap2) slots = cap2;
-  m->stage.slot_capacity = slots;
   // solver ordering buffer: 2 entries per contact; worlds with up to 512 contacts order them in LDS,
   // denser worlds in a global buffer of 2 entries per contact slot
   m->xpbd_contact_cap = 1024;
@@ -82,22 +76,47 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     delete m;
     return fail("per-world LDS requirement exceeds 160 KiB");
   }
-  StageBuffers& s = m->stage;
-  if ((e = dmalloc(&s.shape0, slots)) != hipSuccess || (e = dmalloc(&s.shape1, slots)) != hipSuccess ||
-      (e = dmalloc(&s.point0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.point1, 3 * slots)) != hipSuccess ||
-      (e = dmalloc(&s.offset0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.offset1, 3 * slots)) != hipSuccess ||
-      (e = dmalloc(&s.normal, 3 * slots)) != hipSuccess || (e = dmalloc(&s.margin0, slots)) != hipSuccess ||
-      (e = dmalloc(&s.margin1, slots)) != hipSuccess || (e = dmalloc(&s.sub_key, slots)) != hipSuccess ||
-      (e = dmalloc(&s.mask, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&s.pair_a, slots / 5 + 1)) != hipSuccess ||
-      (e = dmalloc(&s.pair_b, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&s.pair_seg, slots / 5 + 1)) != hipSuccess ||
-      (e = dmalloc(&s.route, slots / 5 + 1)) != hipSuccess || (e = dmalloc(&s.route_count, 2)) != hipSuccess ||
-      (e = dmalloc(&s.shape_wc, 20 * (size_t)(d.shape_count > 0 ? d.shape_count : 1))) != hipSuccess ||
-      (e = dmalloc(&m->seg_pair_count, segs)) != hipSuccess ||
-      (e = dmalloc(&m->seg_pair_base, segs)) != hipSuccess || (e = dmalloc(&m->seg_contact_count, segs)) != hipSuccess ||
-      (e = dmalloc(&m->stage_alloc, 1)) != hipSuccess || (e = dmalloc(&m->counters, 4)) != hipSuccess ||
-      (e = dmalloc(&m->xpbd_keys, m->xpbd_keys_cap)) != hipSuccess) {
+  if ((e = dmalloc(&m->counters, 4)) != hipSuccess || (e = dmalloc(&m->xpbd_keys, m->xpbd_keys_cap)) != hipSuccess) {
     nmc_model_destroy(m);
     return hip_fail(e, "nmc_model_create: device allocation");
+  }
+  if (d.shape_pairs_max > 0) {
+    // pair regions: segment s holds up to min(n_s (n_s - 1) / 2, LDS pair buffer) pairs
+    std::vector<int> starts(segs + 1);
+    e = hipMemcpy(starts.data(), d.segment_start, sizeof(int) * (segs + 1), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { nmc_model_destroy(m); return hip_fail(e, "nmc_model_create: segment_start"); }
+    std::vector<long long> off(segs);
+    std::vector<int> cap(segs);
+    long long total = 0;
+    for (int sg = 0; sg < segs; ++sg) {
+      long long n = starts[sg + 1] - starts[sg];
+      long long w = n * (n - 1) / 2;
+      cap[sg] = static_cast<int>(w < m->pair_cap ? w : m->pair_cap);
+      off[sg] = total;
+      total += cap[sg];
+    }
+    if (total >= (1ll << 31)) { nmc_model_destroy(m); return fail("broad-phase pair regions exceed 2^31 entries"); }
+    long long pairs = total > 0 ? total : 1;
+    long long slots = 5 * pairs;
+    StageBuffers& s = m->stage;
+    s.pair_capacity = pairs;
+    if ((e = dmalloc(&s.point0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.point1, 3 * slots)) != hipSuccess ||
+        (e = dmalloc(&s.offset0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.offset1, 3 * slots)) != hipSuccess ||
+        (e = dmalloc(&s.normal, 3 * slots)) != hipSuccess || (e = dmalloc(&s.margin0, slots)) != hipSuccess ||
+        (e = dmalloc(&s.margin1, slots)) != hipSuccess || (e = dmalloc(&s.sub_key, slots)) != hipSuccess ||
+        (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
+        (e = dmalloc(&s.pair_b, pairs)) != hipSuccess || (e = dmalloc(&s.route, pairs)) != hipSuccess ||
+        (e = dmalloc(&s.route_count, 2)) != hipSuccess ||
+        (e = dmalloc(&s.shape_wc, 20 * (size_t)(d.shape_count > 0 ? d.shape_count : 1))) != hipSuccess ||
+        (e = dmalloc(&m->seg_pair_off, segs)) != hipSuccess || (e = dmalloc(&m->seg_pair_cap, segs)) != hipSuccess ||
+        (e = dmalloc(&m->seg_pair_count, segs)) != hipSuccess || (e = dmalloc(&m->seg_conv_count, segs)) != hipSuccess ||
+        (e = dmalloc(&m->seg_conv_off, segs)) != hipSuccess || (e = dmalloc(&m->seg_ana_off, segs)) != hipSuccess ||
+        (e = dmalloc(&m->seg_contact_count, segs)) != hipSuccess ||
+        (e = hipMemcpy(m->seg_pair_off, off.data(), sizeof(long long) * segs, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(m->seg_pair_cap, cap.data(), sizeof(int) * segs, hipMemcpyHostToDevice)) != hipSuccess) {
+      nmc_model_destroy(m);
+      return hip_fail(e, "nmc_model_create: device allocation");
+    }
   }
   *out = m;
   return 0;
@@ -106,9 +125,9 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
 extern "C" void nmc_model_destroy(NmcModel* m) {
   if (!m) return;
   StageBuffers& s = m->stage;
-  void* ptrs[] = {s.shape0, s.shape1, s.point0, s.point1, s.offset0, s.offset1, s.normal, s.margin0, s.margin1,
-                  s.sub_key, s.mask, s.pair_a, s.pair_b, s.pair_seg, s.route, s.route_count, s.shape_wc,
-                  m->seg_pair_count, m->seg_pair_base, m->seg_contact_count, m->stage_alloc,
+  void* ptrs[] = {s.point0, s.point1, s.offset0, s.offset1, s.normal, s.margin0, s.margin1, s.sub_key, s.mask,
+                  s.pair_a, s.pair_b, s.route, s.route_count, s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
+                  m->seg_pair_count, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, m->seg_contact_count,
                   m->counters, m->xpbd_keys};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -117,6 +136,7 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
 
 extern "C" int nmc_collide(NmcModel* m, const float* body_q, const NmcContacts* c, nmc_stream_t stream) {
   if (!m || !c) return fail("nmc_collide: null argument");
+  if (!m->stage.pair_a) return fail("nmc_collide: the model was created with shape_pairs_max = 0 (solver only)");
   if (m->d.body_count > 0 && !body_q) return fail("nmc_collide: body_q is null");
   if (!c->count || !c->shape0 || !c->shape1 || !c->point0 || !c->point1 || !c->offset0 || !c->offset1 ||
       !c->normal || !c->margin0 || !c->margin1 || !c->world_start || !c->world_count)

@@ -18,10 +18,10 @@ enum { JT_PRISMATIC = 0, JT_REVOLUTE = 1, JT_BALL = 2, JT_FIXED = 3, JT_FREE = 4
 
 constexpr float MAXVAL = 1.0e10f;  // newton/_src/core/types.py:72
 
-// Staged contact records produced by the narrow phase (5 slots per broad-phase pair).
+// Broad-phase pairs and their staged contact records.  Every segment owns a fixed region of pair
+// entries (its worst case n(n-1)/2, capped by the LDS pair buffer) at seg_pair_off[seg]; pair g owns
+// contact slots [5g, 5g + 5).  Fixed regions need no global allocation counter.
 struct StageBuffers {
-  int* shape0;
-  int* shape1;
   float* point0;
   float* point1;
   float* offset0;
@@ -30,15 +30,13 @@ struct StageBuffers {
   float* margin0;
   float* margin1;
   int* sub_key;
-  unsigned char* mask;  // per pair: number of records in its slots (0-5)
-  long long slot_capacity;
-  // broad-phase output, one entry per pair (slot_capacity / 5 entries)
-  int* pair_a;      // first shape (lower GeoType), global id
+  unsigned char* mask;   // per pair: number of records in its slots (0-5)
+  int* pair_a;           // first shape (lower GeoType), global id
   int* pair_b;
-  int* pair_seg;    // segment the pair belongs to
-  int* route;       // pair indices: analytic pairs from the front, MPR/GJK pairs from the back
-  int* route_count; // [2] analytic, MPR/GJK
-  float* shape_wc;  // [shape_count x 20] world transform 7, scale 3, margin 1, AABB lo 3 / hi 3, pad 3
+  long long pair_capacity;  // total pair entries (sum of the segment regions)
+  int* route;            // per segment region: local pair indices, MPR/GJK pairs first, then analytic
+  int* route_count;      // [2] analytic, MPR/GJK pairs over all segments
+  float* shape_wc;       // [shape_count x 20] world transform 7, scale 3, margin 1, AABB lo 3 / hi 3, pad 3
 };
 
 struct NmcModelImpl {
@@ -46,10 +44,13 @@ struct NmcModelImpl {
   int device;
   // collide scratch
   StageBuffers stage;
-  int* seg_pair_count;   // [segments] pairs found (may exceed the per-segment buffer)
-  int* seg_pair_base;    // [segments] first staging pair of the segment
-  int* seg_contact_count;  // [segments] valid staged contacts
-  int* stage_alloc;      // [1] staging pairs handed out during the current collide
+  long long* seg_pair_off;  // [segments] first pair entry of each segment's region
+  int* seg_pair_cap;        // [segments] size of the region
+  int* seg_pair_count;      // [segments] pairs found this call (after the LDS cap)
+  int* seg_conv_count;      // [segments] of which MPR/GJK pairs
+  int* seg_conv_off;        // [segments] exclusive prefix of MPR/GJK pairs over segments
+  int* seg_ana_off;         // [segments] exclusive prefix of analytic pairs over segments
+  int* seg_contact_count;   // [segments] contacts produced this call
   int* counters;         // [4] overflow counters (see nmc_get_counters)
   int pair_cap;          // per-segment pair buffer (LDS) capacity
   size_t collide_lds;

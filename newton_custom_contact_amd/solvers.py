@@ -67,7 +67,7 @@ class SolverXPBD(SolverBase):
         from .collide import estimate_rigid_contact_max  # noqa: PLC0415
 
         self._contact_capacity = max(1, estimate_rigid_contact_max(model))
-        self.native = NativeModel(model, rigid_contact_max=self._contact_capacity, shape_pairs_max=1)
+        self.native = NativeModel(model, rigid_contact_max=self._contact_capacity, shape_pairs_max=0)
         self._lib = load()
         self._control = None
 
@@ -93,7 +93,7 @@ class SolverXPBD(SolverBase):
             # a larger Contacts buffer than any before: regrow the ordering buffer (allocates; do it
             # before capturing the step in a graph)
             self._contact_capacity = contacts.rigid_contact_max
-            self.native = NativeModel(model, rigid_contact_max=self._contact_capacity, shape_pairs_max=1)
+            self.native = NativeModel(model, rigid_contact_max=self._contact_capacity, shape_pairs_max=0)
         cv = ctypes.byref(contacts.native_view()) if contacts is not None else None
         for t in (state_in.body_q, state_in.body_qd, state_in.body_f, state_out.body_q, state_out.body_qd):
             if t.device != model.device or not t.is_contiguous():

@@ -54,7 +54,8 @@ def test_collide_parity(cuda_device, name, make):
         reports.append(compare_contacts(got, ref))
     print(name, reports, flush=True)
     assert sum(r["count"] for r in reports) > 0
-    assert pipe.overflow_counters()["pairs_dropped"] == 0
+    oc = pipe.overflow_counters()
+    assert oc["pairs_dropped"] == 0 and oc["contacts_over_capacity"] == 0
 
 
 @pytest.mark.parametrize("name,make", SCENES, ids=[s[0] for s in SCENES])
