@@ -425,20 +425,20 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
       const long long s = 5 * g + k;
       const int o = start + local_idx;
       const int s0 = st.pair_a[g], s1 = st.pair_b[g];
+      const float4* r = st.rec + 5 * s;
+      const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
       c.shape0[o] = s0;
       c.shape1[o] = s1;
-      for (int q = 0; q < 3; ++q) {
-        c.point0[3 * o + q] = st.point0[3 * s + q];
-        c.point1[3 * o + q] = st.point1[3 * s + q];
-        c.offset0[3 * o + q] = st.offset0[3 * s + q];
-        c.offset1[3 * o + q] = st.offset1[3 * s + q];
-        c.normal[3 * o + q] = st.normal[3 * s + q];
-      }
-      c.margin0[o] = st.margin0[s];
-      c.margin1[o] = st.margin1[s];
+      c.point0[3 * o] = r0.x; c.point0[3 * o + 1] = r0.y; c.point0[3 * o + 2] = r0.z;
+      c.point1[3 * o] = r0.w; c.point1[3 * o + 1] = r1.x; c.point1[3 * o + 2] = r1.y;
+      c.offset0[3 * o] = r1.z; c.offset0[3 * o + 1] = r1.w; c.offset0[3 * o + 2] = r2.x;
+      c.offset1[3 * o] = r2.y; c.offset1[3 * o + 1] = r2.z; c.offset1[3 * o + 2] = r2.w;
+      c.normal[3 * o] = r3.x; c.normal[3 * o + 1] = r3.y; c.normal[3 * o + 2] = r3.z;
+      c.margin0[o] = r3.w;
+      c.margin1[o] = r4.x;
       if (c.sort_key) {
         c.sort_key[o] = ((static_cast<long long>(s0) & 0xFFFFF) << 43) | ((static_cast<long long>(s1) & 0xFFFFF) << 23) |
-                        (static_cast<long long>(st.sub_key[s]) & 0x7FFFFF);
+                        (static_cast<long long>(__float_as_int(r4.y)) & 0x7FFFFF);
       }
     }
     run += chunk;

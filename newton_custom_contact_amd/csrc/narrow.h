@@ -154,15 +154,14 @@ __device__ __forceinline__ void emit(const CData& c, WriteCtx& w, bool check_gap
   int b0 = w.shape_body[c.shape_a], b1 = w.shape_body[c.shape_b];
   xform Xa = b0 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b0));
   xform Xb = b1 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b1));
-  const StageBuffers& st = w.st;
-  st.sub_key[s] = c.sub_key;
-  store3(st.point0 + 3 * s, xpoint(Xa, aw));
-  store3(st.point1 + 3 * s, xpoint(Xb, bw));
-  store3(st.offset0 + 3 * s, xvector(Xa, off_a * n));
-  store3(st.offset1 + 3 * s, xvector(Xb, (-off_b) * n));
-  store3(st.normal + 3 * s, n);
-  st.margin0[s] = off_a;
-  st.margin1[s] = off_b;
+  f3 p0 = xpoint(Xa, aw), p1 = xpoint(Xb, bw);
+  f3 o0 = xvector(Xa, off_a * n), o1 = xvector(Xb, (-off_b) * n);
+  float4* r = w.st.rec + 5 * s;
+  r[0] = make_float4(p0.x, p0.y, p0.z, p1.x);
+  r[1] = make_float4(p1.y, p1.z, o0.x, o0.y);
+  r[2] = make_float4(o0.z, o1.x, o1.y, o1.z);
+  r[3] = make_float4(n.x, n.y, n.z, off_a);
+  r[4] = make_float4(off_b, __int_as_float(c.sub_key), 0.0f, 0.0f);
   w.n += 1;
 }
 

@@ -100,11 +100,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     long long slots = 5 * pairs;
     StageBuffers& s = m->stage;
     s.pair_capacity = pairs;
-    if ((e = dmalloc(&s.point0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.point1, 3 * slots)) != hipSuccess ||
-        (e = dmalloc(&s.offset0, 3 * slots)) != hipSuccess || (e = dmalloc(&s.offset1, 3 * slots)) != hipSuccess ||
-        (e = dmalloc(&s.normal, 3 * slots)) != hipSuccess || (e = dmalloc(&s.margin0, slots)) != hipSuccess ||
-        (e = dmalloc(&s.margin1, slots)) != hipSuccess || (e = dmalloc(&s.sub_key, slots)) != hipSuccess ||
-        (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
+    if ((e = dmalloc(&s.rec, 5 * slots)) != hipSuccess || (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
         (e = dmalloc(&s.pair_b, pairs)) != hipSuccess || (e = dmalloc(&s.route, pairs)) != hipSuccess ||
         (e = dmalloc(&s.route_count, 2)) != hipSuccess ||
         (e = dmalloc(&s.shape_wc, 20 * (size_t)(d.shape_count > 0 ? d.shape_count : 1))) != hipSuccess ||
@@ -125,7 +121,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
 extern "C" void nmc_model_destroy(NmcModel* m) {
   if (!m) return;
   StageBuffers& s = m->stage;
-  void* ptrs[] = {s.point0, s.point1, s.offset0, s.offset1, s.normal, s.margin0, s.margin1, s.sub_key, s.mask,
+  void* ptrs[] = {s.rec, s.mask,
                   s.pair_a, s.pair_b, s.route, s.route_count, s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
                   m->seg_pair_count, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, m->seg_contact_count,
                   m->counters, m->xpbd_keys};

@@ -22,14 +22,9 @@ constexpr float MAXVAL = 1.0e10f;  // newton/_src/core/types.py:72
 // entries (its worst case n(n-1)/2, capped by the LDS pair buffer) at seg_pair_off[seg]; pair g owns
 // contact slots [5g, 5g + 5).  Fixed regions need no global allocation counter.
 struct StageBuffers {
-  float* point0;
-  float* point1;
-  float* offset0;
-  float* offset1;
-  float* normal;
-  float* margin0;
-  float* margin1;
-  int* sub_key;
+  // one 80-byte record per contact slot: point0 3, point1 3, offset0 3, offset1 3, normal 3, margin0,
+  // margin1, sub_key (int bits), pad -- written and read as five 16-byte vectors
+  float4* rec;
   unsigned char* mask;   // per pair: number of records in its slots (0-5)
   int* pair_a;           // first shape (lower GeoType), global id
   int* pair_b;
