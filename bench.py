@@ -165,11 +165,14 @@ def main():
     b_step = 76 * N_b + 96 * N_b + 16 * N_s + 76 * n_c + 52 * N_b + 120 * N_j
     b_ws = args.substeps * (b_collide + b_step)
     roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
+    collide_kernels = ("collide_segment_kernel", "route_scan_kernel", "narrow_kernel", "contact_count_kernel",
+                       "contact_scan_kernel", "contact_compact_kernel")
     if t_xp_ms >= t_col_ms:
         dom, dom_ms, dom_bytes = "xpbd_world_kernel", t_xp_ms, args.worlds * b_step
+        dom_kernels = ("xpbd_world_kernel",)
     else:
-        dom, dom_ms, dom_bytes = ("collide phase: collide_segment_kernel + narrow_kernel + contact_scan_kernel + "
-                                  "contact_compact_kernel", t_col_ms, args.worlds * b_collide)
+        dom, dom_ms, dom_bytes = "collide phase: " + " + ".join(collide_kernels), t_col_ms, args.worlds * b_collide
+        dom_kernels = collide_kernels
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
 
     cpu = None
@@ -210,11 +213,28 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": None,
+            "traffic": measured_traffic(args, dom_kernels),
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+
+
+def measured_traffic(args, kernels):
+    """HBM bytes per launch of the dominant kernel(s) from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes (copy-calibrated), valid for the default workload only; None otherwise."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "mixed4096_hbm_traffic.json")
+    if not (args.worlds == 4096 and args.iterations == 10 and args.substeps == 4 and os.path.exists(path)):
+        return None
+    with open(path) as f:
+        prof = json.load(f)["kernels"]
+    total = 0
+    for k in kernels:
+        hit = [v for name, v in prof.items() if name.startswith(k)]
+        if not hit:
+            return None
+        total += hit[0]["traffic_bytes"]
+    return total
 
 
 def cpu_baseline(model, state, args, sdt):
