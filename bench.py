@@ -13,6 +13,9 @@ outputs once):
 N_c is the measured mean contact count per world per substep in the timed region.
 ``cpu_baseline`` times the C++ oracle (scalar restatement of the reference algorithm, 1 thread) on a
 bounded sample: a snapshot of the first worlds of the timed state, stepped on the host.
+``learned`` times the same workload with the learned warm start (SolverXPBDWarmStart: prep launch,
+network, warm launch with ``--warm-iterations`` iterations and the per-world residual check) and
+reports its penetration residual beside the plain solver's at the same state.
 """
 
 from __future__ import annotations
@@ -41,6 +44,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--kernel-reps", type=int, default=20)
+    p.add_argument("--no-learned", action="store_true", help="skip the learned warm-start form")
+    p.add_argument("--warm-iterations", type=int, default=3)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="only run a device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return p.parse_args()
@@ -175,6 +180,10 @@ def main():
         dom_kernels = collide_kernels
     achieved = dom_bytes / (dom_ms / 1e3) / 1e9
 
+    learned = None
+    if not args.no_learned:
+        learned = learned_form(args, model, dev, ws_per_s)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(model, states[0], args, sdt)
@@ -216,8 +225,97 @@ def main():
             "traffic": measured_traffic(args, dom_kernels),
         },
         "cpu_baseline": cpu,
+        "learned": learned,
     }
     print(json.dumps(line), flush=True)
+
+
+def learned_form(args, model, dev, plain_ws_per_s):
+    """The same workload with the learned warm start (prep launch, network, warm launch with
+    ``warm_iterations`` iterations and the per-world residual check), hipGraph-captured like the plain step;
+    plus the residual comparison: mean over worlds of the largest penetration after each step."""
+    import torch
+
+    from newton_custom_contact_amd import CollisionPipeline
+    from newton_custom_contact_amd.learned import WEIGHTS_PATH, SolverXPBDWarmStart, WarmStartMLP
+
+    if not os.path.exists(WEIGHTS_PATH):
+        return None
+    mlp = WarmStartMLP.load(WEIGHTS_PATH, device=dev)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    solver = SolverXPBDWarmStart(model, mlp, iterations=args.iterations, warm_iterations=args.warm_iterations)
+    contacts = pipe.contacts()
+    states = [model.state(), model.state()]
+    sdt = (1.0 / 60.0) / args.substeps
+    W = model.world_count
+    res_acc = torch.zeros(1, dtype=torch.float64, device=dev)
+    fb_acc = torch.zeros(1, dtype=torch.float64, device=dev)
+
+    def step():
+        for s in range(args.substeps):
+            a, b = states[s % 2], states[(s + 1) % 2]
+            pipe.collide(a, contacts)
+            solver.step(a, b, None, contacts, sdt)
+            buf = solver.buffers
+            res_acc.add_(buf["residual"][:W].mean())
+            fb_acc.add_(buf["fallback"][:W].sum())
+        if args.substeps % 2:
+            states[0].body_q.copy_(states[1].body_q)
+            states[0].body_qd.copy_(states[1].body_qd)
+
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = None
+    if not args.no_graph:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step()
+        torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        graph.replay() if graph is not None else step()
+    torch.cuda.synchronize()
+    res_acc.zero_()
+    fb_acc.zero_()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        graph.replay() if graph is not None else step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ws = args.worlds * args.steps / (ms / 1e3)
+    n_sub = args.steps * args.substeps
+    warm_res = float(res_acc.item()) / n_sub
+    fb_frac = float(fb_acc.item()) / (n_sub * W)
+    # plain residual at the learned run's end state: the plain step with 10 and with warm_iterations iterations
+    a, tmp = states[0], states[1]
+    pipe.collide(a, contacts)
+    solver.log_step(a, tmp, None, contacts, sdt, remember=False)
+    plain10 = float(solver.buffers["residual"][:W].mean())
+    solver.log_step(a, tmp, None, contacts, sdt, iterations=args.warm_iterations, remember=False)
+    plain_k = float(solver.buffers["residual"][:W].mean())
+    solver.step(a, tmp, None, contacts, sdt, remember=False)
+    warm_here = float(solver.buffers["residual"][:W].mean())
+    print(f"[bench] learned: {ws:.1f} world-steps/s, residual {warm_res:.3e}, fallback {fb_frac:.3f}",
+          file=sys.stderr, flush=True)
+    return {
+        "value": round(ws, 1),
+        "unit": "world-steps/s",
+        "ms_per_step": round(ms / args.steps, 4),
+        "speedup_vs_plain": round(ws / plain_ws_per_s, 3),
+        "warm_iterations": args.warm_iterations,
+        "fallback_iterations": args.iterations,
+        "fallback_world_fraction": round(fb_frac, 4),
+        "mean_world_residual_m": warm_res,
+        "end_state_residual_m": {"learned": warm_here, f"plain_{args.warm_iterations}": plain_k,
+                                 f"plain_{args.iterations}": plain10},
+        "network": "16-128-128-1 SiLU, BF16 MFMA / FP32 accumulate, newton_custom_contact_amd/data/warmstart_mlp.npz",
+    }
 
 
 def measured_traffic(args, kernels):

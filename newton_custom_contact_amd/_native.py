@@ -62,8 +62,34 @@ class NmcXpbdParams(ctypes.Structure):
     ]
 
 
-EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_step", "nmc_get_counters",
-           "nmc_last_error")
+class NmcLearnIO(ctypes.Structure):
+    _fields_ = [
+        ("mode", ctypes.c_int), ("warm_iterations", ctypes.c_int), ("residual_tol", ctypes.c_float),
+        ("feat", _vp), ("lambda_sum", _vp), ("lambda0", _vp), ("residual", _vp), ("fallback", _vp),
+        ("q_mid", _vp), ("qd_mid", _vp), ("prev_key", _vp), ("prev_start", _vp), ("prev_count", _vp),
+        ("prev_lambda", _vp),
+    ]
+
+
+class NmcMlp(ctypes.Structure):
+    _fields_ = [
+        ("w1_bf16", _vp), ("b1", _vp), ("w2_bf16", _vp), ("b2", _vp), ("w3", _vp), ("b3", _vp),
+        ("x_mean", _vp), ("x_inv_std", _vp), ("out_scale", ctypes.c_float),
+    ]
+
+
+class NmcMlpTrainBuffers(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("y", "z1", "z2", "xb", "h1", "dz2b", "dz1b", "dz2f", "dz1f", "g3", "dy")]
+
+
+class NmcMlpGrads(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("w1", "b1", "w2", "b2", "w3", "b3")]
+
+
+NMC_STEP_PLAIN, NMC_STEP_LOG, NMC_STEP_PREP, NMC_STEP_WARM = 0, 1, 2, 3
+
+EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_step", "nmc_xpbd_step_learn",
+           "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_get_counters", "nmc_last_error")
 
 
 def lib_path() -> str:
@@ -89,6 +115,15 @@ def load() -> ctypes.CDLL:
         L.nmc_xpbd_step.argtypes = [_vp, ctypes.POINTER(NmcXpbdParams), _vp, _vp, _vp, _vp, _vp, _vp,
                                     ctypes.POINTER(NmcContacts), ctypes.c_float, _vp, _vp, _vp]
         L.nmc_xpbd_step.restype = ctypes.c_int
+        L.nmc_xpbd_step_learn.argtypes = [_vp, ctypes.POINTER(NmcXpbdParams), _vp, _vp, _vp, _vp, _vp, _vp,
+                                          ctypes.POINTER(NmcContacts), ctypes.c_float, _vp, _vp,
+                                          ctypes.POINTER(NmcLearnIO), _vp]
+        L.nmc_xpbd_step_learn.restype = ctypes.c_int
+        L.nmc_mlp_forward.argtypes = [ctypes.POINTER(NmcMlp), _vp, _vp, ctypes.c_int, _vp, _vp]
+        L.nmc_mlp_forward.restype = ctypes.c_int
+        L.nmc_mlp_train_grad.argtypes = [ctypes.POINTER(NmcMlp), _vp, _vp, ctypes.c_int,
+                                         ctypes.POINTER(NmcMlpTrainBuffers), ctypes.POINTER(NmcMlpGrads), _vp]
+        L.nmc_mlp_train_grad.restype = ctypes.c_int
         L.nmc_get_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.nmc_get_counters.restype = ctypes.c_int
         L.nmc_last_error.argtypes = []

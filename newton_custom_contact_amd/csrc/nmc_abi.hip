@@ -15,17 +15,18 @@ size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap);
 hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContacts* c, hipStream_t stream);
 hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
                        const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
-                       const NmcContacts* c, float dt, float* q_out, float* qd_out, hipStream_t stream);
+                       const NmcContacts* c, float dt, float* q_out, float* qd_out, const NmcLearnIO* lrn,
+                       hipStream_t stream);
 
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
 
-static int fail(const std::string& msg) {
+int fail(const std::string& msg) {
   set_error(msg);
   return -1;
 }
 
-static int hip_fail(hipError_t e, const char* what) {
+int hip_fail(hipError_t e, const char* what) {
   return fail(std::string(what) + ": " + hipGetErrorString(e));
 }
 
@@ -154,8 +155,41 @@ extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q
   if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
     return fail("nmc_xpbd_step: contacts capacity exceeds the rigid_contact_max the model was created with");
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
-                             qd_out, reinterpret_cast<hipStream_t>(stream));
+                             qd_out, nullptr, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step launch");
+  return 0;
+}
+
+extern "C" int nmc_xpbd_step_learn(NmcModel* m, const NmcXpbdParams* p, const float* q_in, const float* qd_in,
+                                   const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
+                                   const NmcContacts* c, float dt, float* q_out, float* qd_out, const NmcLearnIO* lrn,
+                                   nmc_stream_t stream) {
+  if (!m || !p || !lrn) return fail("nmc_xpbd_step_learn: null argument");
+  if (m->d.body_count > 0 && (!q_in || !qd_in || !f_in))
+    return fail("nmc_xpbd_step_learn: body state array is null");
+  if (m->d.joint_count > 0 && (!joint_f || !tpos || !tvel)) return fail("nmc_xpbd_step_learn: joint array is null");
+  if (p->has_contacts && !c) return fail("nmc_xpbd_step_learn: has_contacts set but contacts is null");
+  if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
+    return fail("nmc_xpbd_step_learn: contacts capacity exceeds the rigid_contact_max the model was created with");
+  const int mode = lrn->mode;
+  if (mode < NMC_STEP_PLAIN || mode > NMC_STEP_WARM) return fail("nmc_xpbd_step_learn: unknown mode");
+  if (mode != NMC_STEP_PREP) {
+    if (m->d.body_count > 0 && (!q_out || !qd_out)) return fail("nmc_xpbd_step_learn: output state is null");
+    if (q_in == q_out || qd_in == qd_out) return fail("nmc_xpbd_step_learn: state_in and state_out must not alias");
+  }
+  if ((mode == NMC_STEP_LOG || mode == NMC_STEP_PREP) && !lrn->feat) return fail("nmc_xpbd_step_learn: feat is null");
+  if ((mode == NMC_STEP_LOG || mode == NMC_STEP_WARM) && !lrn->lambda_sum)
+    return fail("nmc_xpbd_step_learn: lambda_sum is null");
+  if ((mode == NMC_STEP_PREP || mode == NMC_STEP_WARM) && (!lrn->q_mid || !lrn->qd_mid))
+    return fail("nmc_xpbd_step_learn: q_mid / qd_mid is null");
+  if (mode == NMC_STEP_WARM && !lrn->lambda0) return fail("nmc_xpbd_step_learn: lambda0 is null");
+  if (mode == NMC_STEP_WARM && (lrn->warm_iterations < 0 || lrn->warm_iterations > p->iterations))
+    return fail("nmc_xpbd_step_learn: warm_iterations must be in [0, iterations]");
+  if (lrn->prev_key && (!lrn->prev_start || !lrn->prev_count || !lrn->prev_lambda))
+    return fail("nmc_xpbd_step_learn: prev_key given without prev_start / prev_count / prev_lambda");
+  hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
+                             qd_out, lrn, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step_learn launch");
   return 0;
 }
 

@@ -56,6 +56,8 @@ struct NmcModelImpl {
 };
 
 void set_error(const std::string& msg);
+int fail(const std::string& msg);               // sets the error string, returns -1
+int hip_fail(hipError_t e, const char* what);  // same, with the HIP error text
 
 }  // namespace nmc
 

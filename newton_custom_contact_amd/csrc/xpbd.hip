@@ -145,7 +145,8 @@ __device__ __forceinline__ m33 lds_m33(const float* a) {
 
 // solve_body_contact_positions (kernels.py:2062-2299) for one contact; bodies are local indices or -1.
 __device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f3 o0, f3 o1, f3 n, float m0, float m1,
-                              float mu, float mut, float mur, float relax, float dt, Spatial& da, Spatial& db) {
+                              float mu, float mut, float mur, float relax, float dt, Spatial& da, Spatial& db,
+                              float& lam_out) {
   const m33 Z = {{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}};
   xform Xa = la >= 0 ? lds_q(L, la) : xident();
   xform Xb = lb >= 0 ? lds_q(L, lb) : xident();
@@ -169,6 +170,7 @@ __device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f
   f3 ang_a = -cross(r_a, n);
   f3 ang_b = cross(r_b, n);
   float lambda_n = contact_delta(d, Xa.q, Xb.q, mia, mib, Ia, Ib, -n, n, ang_a, ang_b, relax, dt);
+  lam_out = lambda_n;
   f3 lda = (-n) * lambda_n, ldb = n * lambda_n;
   f3 ada = ang_a * lambda_n, adb = ang_b * lambda_n;
   if (mu > 0.0f) {
@@ -430,7 +432,11 @@ struct XpbdArgs {
   unsigned* keys_global;  // ordering buffer of worlds whose entries exceed key_cap (2 per contact slot)
   long long keys_global_cap;
   int* counters;
+  NmcLearnIO lrn;  // learned warm start I/O (all null / 0 for the plain step)
 };
+
+// Solver modes (template parameter of the world kernel).
+enum { XM_PLAIN = 0, XM_LOG = 1, XM_PREP = 2, XM_WARM = 3 };
 
 // Per-contact data that is constant for the whole step (solve_body_contact_positions inputs).
 struct CReg {
@@ -501,12 +507,109 @@ __device__ void block_exclusive_scan(const int* cnt, int* out, int n, int* tmp) 
   __syncthreads();
 }
 
+// Per-contact inputs of the warm-start network at the integrated state (index order of the restatement):
+// d, normal effective inverse mass w, -d / (dt w) (0 unless d < 0), previous substep's multiplier of the
+// matched contact, relative normal velocity, relative tangential speed, both inverse masses, both
+// bodies' contact counts, normal z, friction, both static flags, both lever arms |r x n|.
+__device__ void contact_features(const WorldLds& L, const CReg& r, int cnt_a, int cnt_b, float lam_prev, float dt,
+                                 float* __restrict__ out) {
+  const m33 Z = {{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}};
+  float f[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) f[k] = 0.0f;
+  if (r.valid) {
+    const f3 zero{0.0f, 0.0f, 0.0f};
+    const int la = r.la, lb = r.lb;
+    xform Xa = la >= 0 ? lds_q(L, la) : xident();
+    xform Xb = lb >= 0 ? lds_q(L, lb) : xident();
+    f3 bx_a = xpoint(Xa, r.p0);
+    f3 bx_b = xpoint(Xb, r.p1);
+    float dd = dot(r.n, bx_b - bx_a) - (r.m0 + r.m1);
+    f3 com_a = la >= 0 ? load3(L.com + 3 * la) : zero;
+    f3 com_b = lb >= 0 ? load3(L.com + 3 * lb) : zero;
+    f3 r_a = bx_a - xpoint(Xa, com_a);
+    f3 r_b = bx_b - xpoint(Xb, com_b);
+    f3 ang_a = -cross(r_a, r.n);
+    f3 ang_b = cross(r_b, r.n);
+    float mia = la >= 0 ? L.inv_m[la] : 0.0f;
+    float mib = lb >= 0 ? L.inv_m[lb] : 0.0f;
+    m33 Ia = la >= 0 ? lds_m33(L.inv_I + 9 * la) : Z;
+    m33 Ib = lb >= 0 ? lds_m33(L.inv_I + 9 * lb) : Z;
+    float w = 0.0f;
+    w += len_sq(-r.n) * mia;
+    w += len_sq(r.n) * mib;
+    f3 ra = rot_inv(Xa.q, ang_a);
+    f3 rb = rot_inv(Xb.q, ang_b);
+    w += dot(ra, mv(Ia, ra));
+    w += dot(rb, mv(Ib, rb));
+    f3 va = la >= 0 ? load3(L.qd + 6 * la) + cross(load3(L.qd + 6 * la + 3), r_a) : zero;
+    f3 vb = lb >= 0 ? load3(L.qd + 6 * lb) + cross(load3(L.qd + 6 * lb + 3), r_b) : zero;
+    f3 rv = vb - va;
+    float vn = dot(r.n, rv);
+    f3 vt = rv - r.n * vn;
+    f[0] = dd;
+    f[1] = w;
+    f[2] = (dd < 0.0f && w > 0.0f) ? -dd / (dt * w) : 0.0f;
+    f[3] = lam_prev;
+    f[4] = vn;
+    f[5] = len(vt);
+    f[6] = mia;
+    f[7] = mib;
+    f[8] = la >= 0 ? (float)cnt_a : 0.0f;
+    f[9] = lb >= 0 ? (float)cnt_b : 0.0f;
+    f[10] = r.n.z;
+    f[11] = r.mu;
+    f[12] = mia == 0.0f ? 1.0f : 0.0f;
+    f[13] = mib == 0.0f ? 1.0f : 0.0f;
+    f[14] = len(ang_a);
+    f[15] = len(ang_b);
+  }
+  float4* o = reinterpret_cast<float4*>(out);
+  o[0] = make_float4(f[0], f[1], f[2], f[3]);
+  o[1] = make_float4(f[4], f[5], f[6], f[7]);
+  o[2] = make_float4(f[8], f[9], f[10], f[11]);
+  o[3] = make_float4(f[12], f[13], f[14], f[15]);
+}
+
+// Penetration of one contact at the current state: max(0, -d) (0 for invalid contacts).
+__device__ __forceinline__ float contact_penetration(const WorldLds& L, const CReg& r) {
+  if (!r.valid) return 0.0f;
+  xform Xa = r.la >= 0 ? lds_q(L, r.la) : xident();
+  xform Xb = r.lb >= 0 ? lds_q(L, r.lb) : xident();
+  float dd = dot(r.n, xpoint(Xb, r.p1) - xpoint(Xa, r.p0)) - (r.m0 + r.m1);
+  return -dd > 0.0f ? -dd : 0.0f;
+}
+
+// Warm-start deltas of one contact: its predicted normal multiplier applied along the normal.
+__device__ __forceinline__ bool warm_contrib(const WorldLds& L, const CReg& r, float lam, Spatial& da, Spatial& db) {
+  if (!(r.valid && lam > 0.0f)) return false;
+  const f3 zero{0.0f, 0.0f, 0.0f};
+  xform Xa = r.la >= 0 ? lds_q(L, r.la) : xident();
+  xform Xb = r.lb >= 0 ? lds_q(L, r.lb) : xident();
+  f3 bx_a = xpoint(Xa, r.p0);
+  f3 bx_b = xpoint(Xb, r.p1);
+  f3 com_a = r.la >= 0 ? load3(L.com + 3 * r.la) : zero;
+  f3 com_b = r.lb >= 0 ? load3(L.com + 3 * r.lb) : zero;
+  f3 ang_a = -cross(bx_a - xpoint(Xa, com_a), r.n);
+  f3 ang_b = cross(bx_b - xpoint(Xb, com_b), r.n);
+  da = Spatial{(-r.n) * lam, ang_a * lam};
+  db = Spatial{r.n * lam, ang_b * lam};
+  return true;
+}
+
 // BS lanes per world; each lane keeps CPL contacts in registers for the whole step and owns BPL bodies.
-template <int BS, int CPL, int BPL>
+// MODE: XM_PLAIN the reference step; XM_LOG the same step, also writing features and per-contact normal
+// multiplier sums; XM_PREP joint forces + integration + features, integrated state out; XM_WARM from the
+// integrated state: warm-start pass, warm_iterations iterations, per-world residual check and cold
+// fallback (integrated state, full iteration count).
+template <int BS, int CPL, int BPL, int MODE>
 __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_articulated;
+  __shared__ float s_red[BS / 64];
+  __shared__ int s_fallback;
   const NmcModelDesc& d = a.d;
+  const NmcLearnIO& lrn = a.lrn;
   const int w = blockIdx.x;  // world index; w == world_count owns the global bodies
   const int b0 = d.world_body_first[w];
   const int nb = d.world_body_count[w];
@@ -549,9 +652,14 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
     }
     keys = a.keys_global + 2ll * cstart;
   }
-  if (tid == 0) s_articulated = 0;
+  if (tid == 0) {
+    s_articulated = 0;
+    s_fallback = 0;
+  }
 
   // ---- load (solver.py:160-175: kinematic bodies get zero effective inverse mass/inertia)
+  const float* __restrict__ q_src = MODE == XM_WARM ? lrn.q_mid : a.q_in;
+  const float* __restrict__ qd_src = MODE == XM_WARM ? lrn.qd_mid : a.qd_in;
   for (int lb = tid; lb < nb; lb += BS) {
     int b = b0 + lb;
     bool kin = (d.body_flags[b] & BODY_KINEMATIC) != 0;
@@ -559,8 +667,8 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
     L.inv_m[lb] = kin ? 0.0f : d.body_inv_mass[b];
     for (int k = 0; k < 9; ++k) L.inv_I[9 * lb + k] = kin ? 0.0f : d.body_inv_inertia[9 * b + k];
     for (int k = 0; k < 3; ++k) L.com[3 * lb + k] = d.body_com[3 * b + k];
-    for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = a.q_in[7 * b + k];
-    for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = a.qd_in[6 * b + k];
+    for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = q_src[7 * b + k];
+    for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = qd_src[6 * b + k];
     cur[lb] = 0;
   }
   for (int jl = tid; jl < nj; jl += BS) {
@@ -579,51 +687,53 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   __syncthreads();
 
   // ---- body_f + joint forces in joint order (kernels.py:890-1004)
-  for (int lb = tid; lb < nb; lb += BS) {
-    int b = b0 + lb;
-    Spatial f{load3(a.f_in + 6 * b), load3(a.f_in + 6 * b + 3)};
-    if (d.joint_count) {
-      for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
-        int ent = d.body_joint_entry[e];
-        int j = ent >> 1, side = ent & 1;
-        int type = d.joint_type[j];
-        if (!d.joint_enabled[j] || type == JT_FIXED || type == JT_CABLE) continue;
-        int id_p = d.joint_parent[j], id_c = d.joint_child[j];
-        int qs = d.joint_qd_start[j];
-        if (type == JT_FREE || type == JT_DISTANCE) {
-          f3 ft{a.joint_f[qs], a.joint_f[qs + 1], a.joint_f[qs + 2]};
-          f3 tt{a.joint_f[qs + 3], a.joint_f[qs + 4], a.joint_f[qs + 5]};
-          if (side == 1) { f.lin = f.lin + ft; f.ang = f.ang + tt; }
-          else if (id_p >= 0) { f.lin = f.lin - ft; f.ang = f.ang - tt; }
-          continue;
+  if (MODE != XM_WARM) {
+    for (int lb = tid; lb < nb; lb += BS) {
+      int b = b0 + lb;
+      Spatial f{load3(a.f_in + 6 * b), load3(a.f_in + 6 * b + 3)};
+      if (d.joint_count) {
+        for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
+          int ent = d.body_joint_entry[e];
+          int j = ent >> 1, side = ent & 1;
+          int type = d.joint_type[j];
+          if (!d.joint_enabled[j] || type == JT_FIXED || type == JT_CABLE) continue;
+          int id_p = d.joint_parent[j], id_c = d.joint_child[j];
+          int qs = d.joint_qd_start[j];
+          if (type == JT_FREE || type == JT_DISTANCE) {
+            f3 ft{a.joint_f[qs], a.joint_f[qs + 1], a.joint_f[qs + 2]};
+            f3 tt{a.joint_f[qs + 3], a.joint_f[qs + 4], a.joint_f[qs + 5]};
+            if (side == 1) { f.lin = f.lin + ft; f.ang = f.ang + tt; }
+            else if (id_p >= 0) { f.lin = f.lin - ft; f.ang = f.ang - tt; }
+            continue;
+          }
+          xform X_wp = load_xform(d.joint_X_p + 7 * j), pose_p = X_wp;
+          f3 com_p{0.0f, 0.0f, 0.0f};
+          if (id_p >= 0) {
+            pose_p = lds_q(L, id_p - b0);
+            X_wp = xmul(pose_p, X_wp);
+            com_p = load3(L.com + 3 * (id_p - b0));
+          }
+          f3 r_p = X_wp.p - xpoint(pose_p, com_p);
+          xform pose_c = lds_q(L, id_c - b0);
+          xform X_wc = xmul(pose_c, load_xform(d.joint_X_c + 7 * j));
+          f3 r_c = X_wc.p - xpoint(pose_c, load3(L.com + 3 * (id_c - b0)));
+          int lin_n = d.joint_dof_dim[2 * j], ang_n = d.joint_dof_dim[2 * j + 1];
+          f3 ft{0.0f, 0.0f, 0.0f}, tt{0.0f, 0.0f, 0.0f};
+          if (type == JT_BALL) {
+            tt = f3{a.joint_f[qs], a.joint_f[qs + 1], a.joint_f[qs + 2]};
+          } else {
+            for (int k = 0; k < lin_n && k < 3; ++k)
+              ft = ft + a.joint_f[qs + k] * xvector(X_wp, load3(d.joint_axis + 3 * (qs + k)));
+            for (int k = 0; k < ang_n && k < 3; ++k)
+              tt = tt + a.joint_f[qs + lin_n + k] * xvector(X_wp, load3(d.joint_axis + 3 * (qs + lin_n + k)));
+          }
+          if (side == 0) { f.lin = f.lin - ft; f.ang = f.ang - (tt + cross(r_p, ft)); }
+          else { f.lin = f.lin + ft; f.ang = f.ang + (tt + cross(r_c, ft)); }
         }
-        xform X_wp = load_xform(d.joint_X_p + 7 * j), pose_p = X_wp;
-        f3 com_p{0.0f, 0.0f, 0.0f};
-        if (id_p >= 0) {
-          pose_p = lds_q(L, id_p - b0);
-          X_wp = xmul(pose_p, X_wp);
-          com_p = load3(L.com + 3 * (id_p - b0));
-        }
-        f3 r_p = X_wp.p - xpoint(pose_p, com_p);
-        xform pose_c = lds_q(L, id_c - b0);
-        xform X_wc = xmul(pose_c, load_xform(d.joint_X_c + 7 * j));
-        f3 r_c = X_wc.p - xpoint(pose_c, load3(L.com + 3 * (id_c - b0)));
-        int lin_n = d.joint_dof_dim[2 * j], ang_n = d.joint_dof_dim[2 * j + 1];
-        f3 ft{0.0f, 0.0f, 0.0f}, tt{0.0f, 0.0f, 0.0f};
-        if (type == JT_BALL) {
-          tt = f3{a.joint_f[qs], a.joint_f[qs + 1], a.joint_f[qs + 2]};
-        } else {
-          for (int k = 0; k < lin_n && k < 3; ++k)
-            ft = ft + a.joint_f[qs + k] * xvector(X_wp, load3(d.joint_axis + 3 * (qs + k)));
-          for (int k = 0; k < ang_n && k < 3; ++k)
-            tt = tt + a.joint_f[qs + lin_n + k] * xvector(X_wp, load3(d.joint_axis + 3 * (qs + lin_n + k)));
-        }
-        if (side == 0) { f.lin = f.lin - ft; f.ang = f.ang - (tt + cross(r_p, ft)); }
-        else { f.lin = f.lin + ft; f.ang = f.ang + (tt + cross(r_c, ft)); }
       }
+      store3(fl + 6 * lb, f.lin);
+      store3(fl + 6 * lb + 3, f.ang);
     }
-    store3(fl + 6 * lb, f.lin);
-    store3(fl + 6 * lb + 3, f.ang);
   }
   // per-body (contact, side) counts for the ordering lists
   for (int c = tid; c < nc; c += BS) {
@@ -643,14 +753,14 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   }
   __syncthreads();
 
-  // ---- integrate in place (solver.py:98-157); static body data of the lane's bodies stays in registers
+  // ---- integrate in place (solver.py:98-157)
 #pragma unroll
   for (int r = 0; r < BPL; ++r) {
     int lb = tid + r * BS;
     if (lb < nb) {
       int b = b0 + lb;
       for (int k = 0; k < 9; ++k) L.I[9 * lb + k] = d.body_inertia[9 * b + k];
-      if (!(L.flags[lb] & BODY_KINEMATIC)) {
+      if (MODE != XM_WARM && !(L.flags[lb] & BODY_KINEMATIC)) {
         int wi = d.body_world[b];
         int gi = wi > 0 ? wi : 0;
         Spatial f{load3(fl + 6 * lb), load3(fl + 6 * lb + 3)};
@@ -693,116 +803,247 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   }
   __syncthreads();
 
+  // ---- warm-start inputs at the integrated state
+  if (MODE == XM_LOG || MODE == XM_PREP) {
+    for (int c = tid; c < nc; c += BS) {
+      CReg r;
+      if (c < CH) {
+        r = creg[0];
+#pragma unroll
+        for (int jj = 1; jj < CPL; ++jj)
+          if (c >= jj * BS) r = creg[jj];
+      } else {
+        r = load_contact(a, cstart + c, b0, nb);
+      }
+      const int gc = cstart + c;
+      float lam_prev = 0.0f;
+      if (lrn.prev_key && a.c.sort_key) {
+        // the previous substep's contact with the same key in this world (keys ascend per world)
+        const long long key = a.c.sort_key[gc];
+        int lo = lrn.prev_start[w], hi = lo + lrn.prev_count[w] - 1;
+        while (lo <= hi) {
+          int mid = (lo + hi) >> 1;
+          long long km = lrn.prev_key[mid];
+          if (km == key) { lam_prev = lrn.prev_lambda[mid]; break; }
+          if (km < key) lo = mid + 1; else hi = mid - 1;
+        }
+      }
+      int cnt_a = r.la >= 0 ? seg[r.la + 1] - seg[r.la] : 0;
+      int cnt_b = r.lb >= 0 ? seg[r.lb + 1] - seg[r.lb] : 0;
+      contact_features(L, r, cnt_a, cnt_b, lam_prev, dt, lrn.feat + 16ll * gc);
+    }
+  }
+  if (MODE == XM_PREP) {
+    for (int lb = tid; lb < nb; lb += BS) {
+      int b = b0 + lb;
+      for (int k = 0; k < 7; ++k) lrn.q_mid[7 * b + k] = L.q[7 * lb + k];
+      for (int k = 0; k < 6; ++k) lrn.qd_mid[6 * b + k] = L.qd[6 * lb + k];
+    }
+    return;
+  }
+
   const float relax = a.p.rigid_contact_relaxation;
   const bool weighted = a.p.rigid_contact_con_weighting != 0;
   const bool articulated = s_articulated != 0;
-  for (int it = 0; it < a.p.iterations; ++it) {
-    // ---------------- contacts (kernels.py:2062-2299) + apply (kernels.py:809-879)
-    if (has_c) {
-      Spatial acc[BPL];
-      float cnt[BPL];
-      int ep[BPL], ee[BPL];
+  constexpr bool ACC = MODE == XM_LOG || MODE == XM_WARM;  // accumulate normal multipliers
+  float lamacc[CPL];
 #pragma unroll
-      for (int r = 0; r < BPL; ++r) {
-        int lb = tid + r * BS;
-        acc[r] = Spatial{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-        cnt[r] = 0.0f;
-        ep[r] = lb < nb ? seg[lb] : 0;
-        ee[r] = lb < nb ? seg[lb + 1] : 0;
+  for (int jj = 0; jj < CPL; ++jj) lamacc[jj] = 0.0f;
+  if (ACC)
+    for (int c = CH + tid; c < nc; c += BS) lrn.lambda_sum[cstart + c] = 0.0f;
+
+  // pass 0: the step (XM_WARM: warm-start pass first, warm_iterations iterations); pass 1 (XM_WARM
+  // only, when the world's residual exceeds the tolerance): cold solve from the integrated state
+  constexpr int PASSES = MODE == XM_WARM ? 2 : 1;
+  for (int pass = 0; pass < PASSES; ++pass) {
+    int iters = a.p.iterations;
+    if (MODE == XM_WARM && pass == 0) iters = lrn.warm_iterations;
+    if (MODE == XM_WARM && pass == 1) {
+      if (!s_fallback) break;
+      for (int lb = tid; lb < nb; lb += BS) {
+        int b = b0 + lb;
+        for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = lrn.q_mid[7 * b + k];
+        for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = lrn.qd_mid[6 * b + k];
       }
-      for (int cb = 0; cb < nc; cb += CH) {
 #pragma unroll
-        for (int jj = 0; jj < CPL; ++jj) {
-          int c = cb + jj * BS + tid;
-          int slot = jj * BS + tid;
-          int act = 0;
-          if (c < nc) {
-            CReg r = cb == 0 ? creg[jj] : load_contact(a, cstart + c, b0, nb);
-            Spatial da, db;
-            if (r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu, r.mut, r.mur,
-                                         relax, dt, da, db)) {
-              act = 1;
-              float* o = contrib + 12 * slot;
-              store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
-            }
-          }
-          active[slot] = act;
-        }
-        __syncthreads();
-        const unsigned cend = static_cast<unsigned>(cb + CH);
+      for (int jj = 0; jj < CPL; ++jj) lamacc[jj] = 0.0f;
+      for (int c = CH + tid; c < nc; c += BS) lrn.lambda_sum[cstart + c] = 0.0f;
+      __syncthreads();
+    }
+    // it == -1: the warm-start pass (XM_WARM, pass 0)
+    const int it0 = (MODE == XM_WARM && pass == 0) ? -1 : 0;
+    for (int it = it0; it < iters; ++it) {
+      const bool warm = MODE == XM_WARM && it < 0;
+      // ---------------- contacts (kernels.py:2062-2299) + apply (kernels.py:809-879)
+      if (has_c) {
+        Spatial acc[BPL];
+        float cnt[BPL];
+        int ep[BPL], ee[BPL];
 #pragma unroll
         for (int r = 0; r < BPL; ++r) {
-          while (ep[r] < ee[r]) {
-            unsigned ent = keys[ep[r]];
-            unsigned cc = ent >> 1;
-            if (cc >= cend) break;
-            int slot = static_cast<int>(cc) - cb;
-            if (active[slot]) {
-              const float* o = contrib + 12 * slot + 6 * (ent & 1u);
-              acc[r].lin = acc[r].lin + load3(o);
-              acc[r].ang = acc[r].ang + load3(o + 3);
-              cnt[r] += 1.0f;
+          int lb = tid + r * BS;
+          acc[r] = Spatial{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+          cnt[r] = 0.0f;
+          ep[r] = lb < nb ? seg[lb] : 0;
+          ee[r] = lb < nb ? seg[lb + 1] : 0;
+        }
+        for (int cb = 0; cb < nc; cb += CH) {
+#pragma unroll
+          for (int jj = 0; jj < CPL; ++jj) {
+            int c = cb + jj * BS + tid;
+            int slot = jj * BS + tid;
+            int act = 0;
+            if (c < nc) {
+              CReg r = cb == 0 ? creg[jj] : load_contact(a, cstart + c, b0, nb);
+              Spatial da, db;
+              bool on;
+              float lam = 0.0f;
+              if (warm) {
+                on = warm_contrib(L, r, lrn.lambda0[cstart + c], da, db);
+              } else {
+                on = r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu, r.mut,
+                                              r.mur, relax, dt, da, db, lam);
+              }
+              if (on) {
+                act = 1;
+                float* o = contrib + 12 * slot;
+                store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
+                if (ACC && !warm) {
+                  if (cb == 0) lamacc[jj] += lam;
+                  else lrn.lambda_sum[cstart + c] += lam;
+                }
+              }
             }
-            ep[r] += 1;
+            active[slot] = act;
+          }
+          __syncthreads();
+          const unsigned cend = static_cast<unsigned>(cb + CH);
+#pragma unroll
+          for (int r = 0; r < BPL; ++r) {
+            while (ep[r] < ee[r]) {
+              unsigned ent = keys[ep[r]];
+              unsigned cc = ent >> 1;
+              if (cc >= cend) break;
+              int slot = static_cast<int>(cc) - cb;
+              if (active[slot]) {
+                const float* o = contrib + 12 * slot + 6 * (ent & 1u);
+                acc[r].lin = acc[r].lin + load3(o);
+                acc[r].ang = acc[r].ang + load3(o + 3);
+                cnt[r] += 1.0f;
+              }
+              ep[r] += 1;
+            }
+          }
+          if (cb + CH < nc) __syncthreads();
+        }
+#pragma unroll
+        for (int r = 0; r < BPL; ++r) {
+          int lb = tid + r * BS;
+          // the warm pass leaves bodies without a warm contact untouched (no renormalisation / clamp)
+          if (lb < nb && !(warm && cnt[r] == 0.0f)) {
+            BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
+                                       lds_m33(L.inv_I + 9 * lb), acc[r], weighted, cnt[r], dt);
+            lds_store_state(L, lb, o);
           }
         }
-        if (cb + CH < nc) __syncthreads();
       }
+      if (warm) {
+        __syncthreads();
+        continue;
+      }
+      // ---------------- joints (kernels.py:1441-1943) + apply without weights
+      if (d.joint_count) {
+        if (articulated) {
+          __syncthreads();
+          for (int jl = tid; jl < nj; jl += BS) {
+            int j = j0 + jl;
+            int id_p = d.joint_parent[j], id_c = d.joint_child[j];
+            Spatial dp, dc;
+            bool ok = joint_solve(d, L, j, id_p >= 0 ? id_p - b0 : -1, id_c - b0, a.tpos, a.tvel,
+                                  a.p.joint_linear_compliance, a.p.joint_angular_compliance,
+                                  a.p.joint_angular_relaxation, a.p.joint_linear_relaxation, dt, dp, dc);
+            jactive[jl] = ok ? 1 : 0;
+            if (ok) {
+              float* o = jcontrib + 12 * jl;
+              store3(o, dp.lin); store3(o + 3, dp.ang); store3(o + 6, dc.lin); store3(o + 9, dc.ang);
+            }
+          }
+          __syncthreads();
+        }
 #pragma unroll
-      for (int r = 0; r < BPL; ++r) {
-        int lb = tid + r * BS;
-        if (lb < nb) {
-          BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
-                                     lds_m33(L.inv_I + 9 * lb), acc[r], weighted, cnt[r], dt);
+        for (int r = 0; r < BPL; ++r) {
+          int lb = tid + r * BS;
+          if (lb >= nb) continue;
+          int b = b0 + lb;
+          Spatial acc{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+          bool any = false;
+          if (articulated) {
+            for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
+              int ent = d.body_joint_entry[e];
+              int jl = (ent >> 1) - j0;
+              if (!jactive[jl]) continue;
+              const float* o = jcontrib + 12 * jl + 6 * (ent & 1);
+              acc.lin = acc.lin + load3(o);
+              acc.ang = acc.ang + load3(o + 3);
+              any = true;
+            }
+          }
+          BodyState o = any ? apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
+                                           lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt)
+                            : apply_zero_delta(lds_state(L, lb), load3(L.com + 3 * lb), L.inv_m[lb]);
           lds_store_state(L, lb, o);
         }
       }
+      __syncthreads();
     }
-    // ---------------- joints (kernels.py:1441-1943) + apply without weights
-    if (d.joint_count) {
-      if (articulated) {
-        __syncthreads();
-        for (int jl = tid; jl < nj; jl += BS) {
-          int j = j0 + jl;
-          int id_p = d.joint_parent[j], id_c = d.joint_child[j];
-          Spatial dp, dc;
-          bool ok = joint_solve(d, L, j, id_p >= 0 ? id_p - b0 : -1, id_c - b0, a.tpos, a.tvel,
-                                a.p.joint_linear_compliance, a.p.joint_angular_compliance,
-                                a.p.joint_angular_relaxation, a.p.joint_linear_relaxation, dt, dp, dc);
-          jactive[jl] = ok ? 1 : 0;
-          if (ok) {
-            float* o = jcontrib + 12 * jl;
-            store3(o, dp.lin); store3(o + 3, dp.ang); store3(o + 6, dc.lin); store3(o + 9, dc.ang);
-          }
-        }
-        __syncthreads();
-      }
+    // ---- per-world penetration residual: max over the world's contacts of max(0, -d)
+    if (MODE == XM_WARM || lrn.residual) {
+      float pen = 0.0f;
+      for (int c = tid; c < nc; c += BS) {
+        CReg r;
+        if (c < CH) {
+          r = creg[0];
 #pragma unroll
-      for (int r = 0; r < BPL; ++r) {
-        int lb = tid + r * BS;
-        if (lb >= nb) continue;
-        int b = b0 + lb;
-        Spatial acc{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-        bool any = false;
-        if (articulated) {
-          for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
-            int ent = d.body_joint_entry[e];
-            int jl = (ent >> 1) - j0;
-            if (!jactive[jl]) continue;
-            const float* o = jcontrib + 12 * jl + 6 * (ent & 1);
-            acc.lin = acc.lin + load3(o);
-            acc.ang = acc.ang + load3(o + 3);
-            any = true;
-          }
+          for (int jj = 1; jj < CPL; ++jj)
+            if (c >= jj * BS) r = creg[jj];
+        } else {
+          r = load_contact(a, cstart + c, b0, nb);
         }
-        BodyState o = any ? apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
-                                         lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt)
-                          : apply_zero_delta(lds_state(L, lb), load3(L.com + 3 * lb), L.inv_m[lb]);
-        lds_store_state(L, lb, o);
+        pen = fmax2(pen, contact_penetration(L, r));
+      }
+      for (int off = 32; off > 0; off >>= 1) pen = fmax2(pen, __shfl_down(pen, off, 64));
+      if ((tid & 63) == 0) s_red[tid >> 6] = pen;
+      __syncthreads();
+      if (tid == 0) {
+        float m = 0.0f;
+        for (int i = 0; i < BS / 64; ++i) m = fmax2(m, s_red[i]);
+        if (MODE == XM_WARM && pass == 0 && m > lrn.residual_tol) s_fallback = 1;
+        if (lrn.residual) lrn.residual[w] = m;
+      }
+      __syncthreads();
+    }
+  }
+  if (ACC) {
+#pragma unroll
+    for (int jj = 0; jj < CPL; ++jj) {
+      int c = jj * BS + tid;
+      if (c < nc) {
+        float l0 = 0.0f;
+        if (MODE == XM_WARM && !s_fallback) {
+          float v = lrn.lambda0[cstart + c];
+          l0 = v > 0.0f ? v : 0.0f;
+        }
+        lrn.lambda_sum[cstart + c] = l0 + lamacc[jj];
       }
     }
-    __syncthreads();
+    if (MODE == XM_WARM && !s_fallback) {
+      for (int c = CH + tid; c < nc; c += BS) {
+        float v = lrn.lambda0[cstart + c];
+        lrn.lambda_sum[cstart + c] = (v > 0.0f ? v : 0.0f) + lrn.lambda_sum[cstart + c];
+      }
+    }
   }
+  if (MODE == XM_WARM && lrn.fallback && tid == 0) lrn.fallback[w] = s_fallback;
 
   // ---- store (copy_kinematic_body_state_kernel, kernels.py:18-31: kinematic bodies pass through)
   for (int lb = tid; lb < nb; lb += BS) {
@@ -841,9 +1082,20 @@ size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap) {
   return (b + 15) & ~size_t(15);
 }
 
+template <int MODE>
+static void launch_mode(const XpbdShape& sh, int worlds, size_t lds, hipStream_t stream, const XpbdArgs& a) {
+  if (sh.bs == 64)
+    hipLaunchKernelGGL((xpbd_world_kernel<64, 2, 1, MODE>), dim3(worlds), dim3(64), lds, stream, a);
+  else if (sh.bpl == 1)
+    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 1, MODE>), dim3(worlds), dim3(256), lds, stream, a);
+  else
+    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 2, MODE>), dim3(worlds), dim3(256), lds, stream, a);
+}
+
 hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
                        const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
-                       const NmcContacts* c, float dt, float* q_out, float* qd_out, hipStream_t stream) {
+                       const NmcContacts* c, float dt, float* q_out, float* qd_out, const NmcLearnIO* lrn,
+                       hipStream_t stream) {
   XpbdArgs a;
   a.d = m->d;
   a.p = *params;
@@ -861,14 +1113,15 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   a.keys_global = m->xpbd_keys;
   a.keys_global_cap = m->xpbd_keys_cap;
   a.counters = m->counters;
+  if (lrn) a.lrn = *lrn; else a.lrn = NmcLearnIO{};
   int worlds = m->d.world_count + 1;
   XpbdShape sh = xpbd_shape(m->d.max_world_bodies);
-  if (sh.bs == 64)
-    hipLaunchKernelGGL((xpbd_world_kernel<64, 2, 1>), dim3(worlds), dim3(64), m->xpbd_lds, stream, a);
-  else if (sh.bpl == 1)
-    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 1>), dim3(worlds), dim3(256), m->xpbd_lds, stream, a);
-  else
-    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 2>), dim3(worlds), dim3(256), m->xpbd_lds, stream, a);
+  switch (a.lrn.mode) {
+    case NMC_STEP_LOG: launch_mode<XM_LOG>(sh, worlds, m->xpbd_lds, stream, a); break;
+    case NMC_STEP_PREP: launch_mode<XM_PREP>(sh, worlds, m->xpbd_lds, stream, a); break;
+    case NMC_STEP_WARM: launch_mode<XM_WARM>(sh, worlds, m->xpbd_lds, stream, a); break;
+    default: launch_mode<XM_PLAIN>(sh, worlds, m->xpbd_lds, stream, a); break;
+  }
   return hipGetLastError();
 }
 

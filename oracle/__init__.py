@@ -229,3 +229,80 @@ class Oracle:
            _p(c["point1"], tp), _p(c["offset0"], tp), _p(c["offset1"], tp), _p(c["normal"], tp), _p(c["margin0"], tp),
            _p(c["margin1"], tp), dtc, _p(qo, tp), _p(qdo, tp))
         return qo, qdo
+
+    def step_learn(self, body_q, body_qd, body_f, contacts: dict, dt: float, mode: int, iterations: int = 10,
+                   lambda_prev=None, lambda0=None, warm_iterations: int = 3, residual_tol: float = 1e-4,
+                   rigid_contact_relaxation: float = 0.8, rigid_contact_con_weighting: bool = True):
+        """FP32 step of the learned-warm-start restatement (oracle_learn.h).
+
+        mode 0 plain, 1 plain + features / multiplier logging, 2 warm start.  Returns
+        (q, qd, features [n,16], lambda_sum [n], residual [world_count+1], fallback [world_count+1])."""
+        h = self.model.host
+        L = self.lib
+        p = OracleXpbdParams(iterations, rigid_contact_relaxation, int(rigid_contact_con_weighting), 0.0, 0.7, 0.4,
+                             0.0, 0.0)
+        q = np.ascontiguousarray(body_q, np.float32)
+        qd = np.ascontiguousarray(body_qd, np.float32)
+        f = np.ascontiguousarray(body_f, np.float32)
+        one = np.zeros(1, np.float32)
+        jf = np.ascontiguousarray(h["joint_f"], np.float32) if h["joint_f"].size else one
+        tpos = np.ascontiguousarray(h["joint_target_pos"], np.float32) if h["joint_target_pos"].size else one
+        tvel = np.ascontiguousarray(h["joint_target_vel"], np.float32) if h["joint_target_vel"].size else one
+        n = len(contacts["shape0"])
+        nn = max(n, 1)
+        c = {}
+        for k in ("point0", "point1", "offset0", "offset1", "normal"):
+            c[k] = np.zeros((nn, 3), np.float32)
+            c[k][:n] = np.asarray(contacts[k], np.float32).reshape(-1, 3)
+        for k in ("margin0", "margin1"):
+            c[k] = np.zeros(nn, np.float32)
+            c[k][:n] = contacts[k]
+        for k in ("shape0", "shape1"):
+            c[k] = np.zeros(nn, np.int32)
+            c[k][:n] = contacts[k]
+        lp = np.zeros(nn, np.float32) if lambda_prev is None else np.ascontiguousarray(lambda_prev, np.float32)
+        l0 = np.zeros(nn, np.float32) if lambda0 is None else np.ascontiguousarray(lambda0, np.float32)
+        feat = np.zeros((nn, 16), np.float32)
+        lam = np.zeros(nn, np.float32)
+        W = self.model.world_count + 1
+        res = np.zeros(W, np.float32)
+        fb = np.zeros(W, np.int32)
+        qo = np.zeros_like(q)
+        qdo = np.zeros_like(qd)
+        L.oracle_xpbd_step_learn_f32.restype = None
+        L.oracle_xpbd_step_learn_f32(
+            ctypes.byref(self.m), ctypes.byref(p), _p(q, _f32p), _p(qd, _f32p), _p(f, _f32p), _p(jf, _f32p),
+            _p(tpos, _f32p), _p(tvel, _f32p), n, _p(c["shape0"], _i32p), _p(c["shape1"], _i32p),
+            _p(c["point0"], _f32p), _p(c["point1"], _f32p), _p(c["offset0"], _f32p), _p(c["offset1"], _f32p),
+            _p(c["normal"], _f32p), _p(c["margin0"], _f32p), _p(c["margin1"], _f32p), ctypes.c_float(dt),
+            ctypes.c_int(mode), _p(lp, _f32p), _p(l0, _f32p), ctypes.c_int(warm_iterations),
+            ctypes.c_float(residual_tol), _p(feat, _f32p), _p(lam, _f32p), _p(res, _f32p), _p(fb, _i32p),
+            _p(qo, _f32p), _p(qdo, _f32p))
+        return qo, qdo, feat[:n], lam[:n], res, fb
+
+
+def mlp_forward(params: dict, x: np.ndarray) -> np.ndarray:
+    """The restatement's MLP forward (BF16 inputs/weights of layers 1-2, FP32 accumulate)."""
+    L = lib()
+    L.oracle_mlp_forward.restype = None
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.zeros(len(x), np.float32)
+    P = {k: np.ascontiguousarray(v, np.float32) for k, v in params.items()}
+    L.oracle_mlp_forward(*[_p(P[k], _f32p) for k in ("w1", "b1", "w2", "b2", "w3", "b3")], _p(x, _f32p),
+                         ctypes.c_int(len(x)), _p(y, _f32p))
+    return y
+
+
+def mlp_backward(params: dict, x: np.ndarray, t: np.ndarray) -> tuple[dict, float]:
+    """Gradients of 0.5 * mean((y - t)^2) (FP32 restatement, FP64 sums)."""
+    L = lib()
+    L.oracle_mlp_backward.restype = None
+    x = np.ascontiguousarray(x, np.float32)
+    t = np.ascontiguousarray(t, np.float32)
+    P = {k: np.ascontiguousarray(v, np.float32) for k, v in params.items()}
+    g = {k: np.zeros_like(P[k]) for k in P}
+    loss = np.zeros(1, np.float32)
+    L.oracle_mlp_backward(*[_p(P[k], _f32p) for k in ("w1", "b1", "w2", "b2", "w3", "b3")], _p(x, _f32p),
+                          _p(t, _f32p), ctypes.c_int(len(x)),
+                          *[_p(g[k], _f32p) for k in ("w1", "b1", "w2", "b2", "w3", "b3")], _p(loss, _f32p))
+    return g, float(loss[0])

@@ -127,3 +127,65 @@ extern "C" void oracle_asin_f32(const float* x, int n, float* out) {
 extern "C" void oracle_acos_f32(const float* x, int n, float* out) {
   for (int i = 0; i < n; ++i) out[i] = tacos(x[i]);
 }
+
+// ---------------------------------------------------------------- learned warm start (oracle_learn.h)
+#include "oracle_learn.h"
+
+extern "C" void oracle_xpbd_step_learn_f32(const OracleModel* m, const OracleXpbdParams* p, const float* q_in,
+                                           const float* qd_in, const float* f_in, const float* joint_f,
+                                           const float* tpos, const float* tvel, int nc, const int* shape0,
+                                           const int* shape1, const float* point0, const float* point1,
+                                           const float* offset0, const float* offset1, const float* normal,
+                                           const float* margin0, const float* margin1, float dt, int mode,
+                                           const float* lambda_prev, const float* lambda0, int warm_iterations,
+                                           float residual_tol, float* feat, float* lambda_sum, float* residual,
+                                           int* fallback, float* q_out, float* qd_out) {
+  int B = m->body_count;
+  std::vector<Body<float>> state(B);
+  std::vector<Spatial<float>> f(B);
+  for (int b = 0; b < B; ++b) {
+    state[b].q = load_xf_t<float, float>(q_in + 7 * b);
+    state[b].v = V3<float>{qd_in[6 * b], qd_in[6 * b + 1], qd_in[6 * b + 2]};
+    state[b].w = V3<float>{qd_in[6 * b + 3], qd_in[6 * b + 4], qd_in[6 * b + 5]};
+    f[b].lin = V3<float>{f_in[6 * b], f_in[6 * b + 1], f_in[6 * b + 2]};
+    f[b].ang = V3<float>{f_in[6 * b + 3], f_in[6 * b + 4], f_in[6 * b + 5]};
+  }
+  std::vector<ContactIn<float>> cs(nc > 0 ? nc : 0);
+  auto ld = [](const float* a, int i) { return V3<float>{a[3 * i], a[3 * i + 1], a[3 * i + 2]}; };
+  for (int i = 0; i < nc; ++i) {
+    cs[i].shape0 = shape0[i];
+    cs[i].shape1 = shape1[i];
+    cs[i].p0 = ld(point0, i);
+    cs[i].p1 = ld(point1, i);
+    cs[i].o0 = ld(offset0, i);
+    cs[i].o1 = ld(offset1, i);
+    cs[i].n = ld(normal, i);
+    cs[i].m0 = margin0[i];
+    cs[i].m1 = margin1[i];
+  }
+  xpbd_step_learn<float>(m, p, state, f, joint_f, tpos, tvel, cs, dt, mode, lambda_prev, lambda0, warm_iterations,
+                         residual_tol, feat, lambda_sum, residual, fallback);
+  for (int b = 0; b < B; ++b) {
+    const Body<float>& s = state[b];
+    float* q = q_out + 7 * b;
+    q[0] = s.q.p.x; q[1] = s.q.p.y; q[2] = s.q.p.z;
+    q[3] = s.q.q.x; q[4] = s.q.q.y; q[5] = s.q.q.z; q[6] = s.q.q.w;
+    float* qd = qd_out + 6 * b;
+    qd[0] = s.v.x; qd[1] = s.v.y; qd[2] = s.v.z;
+    qd[3] = s.w.x; qd[4] = s.w.y; qd[5] = s.w.z;
+  }
+}
+
+extern "C" void oracle_mlp_forward(const float* w1, const float* b1, const float* w2, const float* b2, const float* w3,
+                                   const float* b3, const float* x, int n, float* y) {
+  MlpParams P{w1, b1, w2, b2, w3, b3};
+  mlp_forward(P, x, n, y);
+}
+
+extern "C" void oracle_mlp_backward(const float* w1, const float* b1, const float* w2, const float* b2,
+                                    const float* w3, const float* b3, const float* x, const float* t, int n,
+                                    float* gw1, float* gb1, float* gw2, float* gb2, float* gw3, float* gb3,
+                                    float* loss) {
+  MlpParams P{w1, b1, w2, b2, w3, b3};
+  mlp_backward(P, x, t, n, gw1, gb1, gw2, gb2, gw3, gb3, loss);
+}

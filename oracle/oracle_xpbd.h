@@ -180,7 +180,8 @@ template <class T> struct ContactIn {
 
 // solve_body_contact_positions (kernels.py:2062-2299); returns false when the contact is inactive
 template <class T> bool contact_solve(const XModel<T>& X, const std::vector<Body<T>>& bodies, const ContactIn<T>& c,
-                                      T relax, T dt, int& body_a, int& body_b, Spatial<T>& da, Spatial<T>& db) {
+                                      T relax, T dt, int& body_a, int& body_b, Spatial<T>& da, Spatial<T>& db,
+                                      T* lambda_out = nullptr) {
   const OracleModel* m = X.m;
   int sa = c.shape0, sb = c.shape1;
   if (sa == sb) return false;
@@ -214,6 +215,7 @@ template <class T> bool contact_solve(const XModel<T>& X, const std::vector<Body
   V3<T> ang_a = -cross(r_a, n);
   V3<T> ang_b = cross(r_b, n);
   T lambda_n = contact_delta(d, Xa, Xb, mia, mib, Ia, Ib, -n, n, ang_a, ang_b, relax, dt);
+  if (lambda_out) *lambda_out = lambda_n;
   V3<T> lda = (-n) * lambda_n, ldb = n * lambda_n;
   V3<T> ada = ang_a * lambda_n, adb = ang_b * lambda_n;
   if (mu > T(0)) {
@@ -520,15 +522,14 @@ template <class T> void joint_forces(const XModel<T>& X, const std::vector<Body<
   }
 }
 
+// integrate_bodies (solver.py:98-157) after the joint forces (kernels.py:890-1004); `in` keeps the
+// start-of-step state for the kinematic pass-through
 template <class T>
-void xpbd_step(const OracleModel* m, const OracleXpbdParams* p, std::vector<Body<T>>& state,
-               std::vector<Spatial<T>> body_f, const float* joint_f, const float* tpos, const float* tvel,
-               const std::vector<ContactIn<T>>& contacts, bool has_contacts, T dt) {
-  XModel<T> X = make_xmodel<T>(m);
+void integrate_all(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p, std::vector<Body<T>>& state,
+                   std::vector<Spatial<T>> body_f, const float* joint_f, T dt) {
   int B = m->body_count;
   std::vector<Body<T>> in = state;
   if (m->joint_count) joint_forces(X, in, joint_f, body_f);
-  // integrate_bodies
   for (int b = 0; b < B; ++b) {
     if (m->body_flags[b] & KINEMATIC) continue;
     int wi = m->body_world[b];
@@ -537,16 +538,28 @@ void xpbd_step(const OracleModel* m, const OracleXpbdParams* p, std::vector<Body
     state[b] = integrate_body(in[b], body_f[b], X.com[b], X.I[b], T(m->body_inv_mass[b]),
                               load_m3<T>(m->body_inv_inertia + 9 * b), g, T(p->angular_damping), dt);
   }
+}
+
+// `iterations` x (contact solve + weighted apply, joint solve + apply) (solver_xpbd.py:518-650).  When
+// lambda_sum is given, every contact's normal multiplier is added to it on each iteration it is active.
+template <class T>
+void iterate(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p, std::vector<Body<T>>& state,
+             const float* tpos, const float* tvel, const std::vector<ContactIn<T>>& contacts, bool has_contacts,
+             T dt, int iterations, T* lambda_sum) {
+  int B = m->body_count;
   V3<T> zero{T(0), T(0), T(0)};
   std::vector<Spatial<T>> deltas(B);
   std::vector<T> weights(B);
-  for (int it = 0; it < p->iterations; ++it) {
+  for (int it = 0; it < iterations; ++it) {
     for (int b = 0; b < B; ++b) { deltas[b] = Spatial<T>{zero, zero}; weights[b] = T(0); }
     if (B && has_contacts) {
-      for (const ContactIn<T>& c : contacts) {
+      for (size_t ci = 0; ci < contacts.size(); ++ci) {
+        const ContactIn<T>& c = contacts[ci];
         int ba, bb;
         Spatial<T> da, db;
-        if (!contact_solve(X, state, c, T(p->rigid_contact_relaxation), dt, ba, bb, da, db)) continue;
+        T lam = T(0);
+        if (!contact_solve(X, state, c, T(p->rigid_contact_relaxation), dt, ba, bb, da, db, &lam)) continue;
+        if (lambda_sum) lambda_sum[ci] += lam;
         if (p->rigid_contact_con_weighting) {
           if (ba >= 0) weights[ba] += T(1);
           if (bb >= 0) weights[bb] += T(1);
@@ -577,7 +590,17 @@ void xpbd_step(const OracleModel* m, const OracleXpbdParams* p, std::vector<Body
       state = next;
     }
   }
-  for (int b = 0; b < B; ++b)
+}
+
+template <class T>
+void xpbd_step(const OracleModel* m, const OracleXpbdParams* p, std::vector<Body<T>>& state,
+               std::vector<Spatial<T>> body_f, const float* joint_f, const float* tpos, const float* tvel,
+               const std::vector<ContactIn<T>>& contacts, bool has_contacts, T dt) {
+  XModel<T> X = make_xmodel<T>(m);
+  std::vector<Body<T>> in = state;
+  integrate_all(X, m, p, state, body_f, joint_f, dt);
+  iterate(X, m, p, state, tpos, tvel, contacts, has_contacts, dt, p->iterations, static_cast<T*>(nullptr));
+  for (int b = 0; b < m->body_count; ++b)
     if (m->body_flags[b] & KINEMATIC) state[b] = in[b];
 }
 

@@ -44,7 +44,9 @@ def test_structs_match_header_field_order():
         return names
 
     for cname, pyc in (("NmcModelDesc", _native.NmcModelDesc), ("NmcContacts", _native.NmcContacts),
-                       ("NmcXpbdParams", _native.NmcXpbdParams)):
+                       ("NmcXpbdParams", _native.NmcXpbdParams), ("NmcLearnIO", _native.NmcLearnIO),
+                       ("NmcMlp", _native.NmcMlp), ("NmcMlpTrainBuffers", _native.NmcMlpTrainBuffers),
+                       ("NmcMlpGrads", _native.NmcMlpGrads)):
         assert fields(cname) == [f[0] for f in pyc._fields_], cname
 
 
