@@ -46,6 +46,8 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--no-learned", action="store_true", help="skip the learned warm-start form")
     p.add_argument("--warm-iterations", type=int, default=3)
+    p.add_argument("--residual-tol", type=float, default=5e-3,
+                   help="per-world penetration residual above which the learned form falls back to a cold solve")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="only run a device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return p.parse_args()
@@ -243,22 +245,27 @@ def learned_form(args, model, dev, plain_ws_per_s):
         return None
     mlp = WarmStartMLP.load(WEIGHTS_PATH, device=dev)
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
-    solver = SolverXPBDWarmStart(model, mlp, iterations=args.iterations, warm_iterations=args.warm_iterations)
-    contacts = pipe.contacts()
+    solver = SolverXPBDWarmStart(model, mlp, iterations=args.iterations, warm_iterations=args.warm_iterations,
+                                 residual_tol=args.residual_tol)
+    # two Contacts buffers alternate, so the previous substep's contacts stay readable for the matching
+    cbufs = [pipe.contacts(), pipe.contacts()]
     states = [model.state(), model.state()]
     sdt = (1.0 / 60.0) / args.substeps
     W = model.world_count
     res_acc = torch.zeros(1, dtype=torch.float64, device=dev)
     fb_acc = torch.zeros(1, dtype=torch.float64, device=dev)
+    stats = [False]
 
     def step():
         for s in range(args.substeps):
             a, b = states[s % 2], states[(s + 1) % 2]
+            contacts = cbufs[s % 2]
             pipe.collide(a, contacts)
             solver.step(a, b, None, contacts, sdt)
-            buf = solver.buffers
-            res_acc.add_(buf["residual"][:W].mean())
-            fb_acc.add_(buf["fallback"][:W].sum())
+            if stats[0]:
+                buf = solver.buffers
+                res_acc.add_(buf["residual"][:W].mean())
+                fb_acc.add_(buf["fallback"][:W].sum())
         if args.substeps % 2:
             states[0].body_q.copy_(states[1].body_q)
             states[0].body_qd.copy_(states[1].body_qd)
@@ -289,11 +296,18 @@ def learned_form(args, model, dev, plain_ws_per_s):
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
     ws = args.worlds * args.steps / (ms / 1e3)
-    n_sub = args.steps * args.substeps
+    # residual and fallback statistics over further (untimed, eager) steps of the same run
+    stats[0] = True
+    n_stat = 30
+    for _ in range(n_stat):
+        step()
+    torch.cuda.synchronize()
+    n_sub = n_stat * args.substeps
     warm_res = float(res_acc.item()) / n_sub
     fb_frac = float(fb_acc.item()) / (n_sub * W)
     # plain residual at the learned run's end state: the plain step with 10 and with warm_iterations iterations
     a, tmp = states[0], states[1]
+    contacts = cbufs[0]
     pipe.collide(a, contacts)
     solver.log_step(a, tmp, None, contacts, sdt, remember=False)
     plain10 = float(solver.buffers["residual"][:W].mean())
@@ -310,6 +324,7 @@ def learned_form(args, model, dev, plain_ws_per_s):
         "speedup_vs_plain": round(ws / plain_ws_per_s, 3),
         "warm_iterations": args.warm_iterations,
         "fallback_iterations": args.iterations,
+        "residual_tol_m": args.residual_tol,
         "fallback_world_fraction": round(fb_frac, 4),
         "mean_world_residual_m": warm_res,
         "end_state_residual_m": {"learned": warm_here, f"plain_{args.warm_iterations}": plain_k,

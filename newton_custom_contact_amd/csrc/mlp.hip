@@ -31,7 +31,7 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {
   return static_cast<unsigned short>((u + r) >> 16);
 }
 __device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float(static_cast<unsigned>(b) << 16); }
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float silu_grad(float z) {
   float s = 1.0f / (1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
@@ -86,8 +86,13 @@ __device__ void mlp_load_weights(const MlpWeights& W, MlpLds& S, unsigned char* 
   S.b1 = reinterpret_cast<float*>(p); p += sizeof(float) * NH;
   S.b2 = reinterpret_cast<float*>(p); p += sizeof(float) * NH;
   S.w3 = reinterpret_cast<float*>(p);
-  for (int i = threadIdx.x; i < NH * NF; i += blockDim.x) S.w1[i] = W.w1[i];
-  for (int i = threadIdx.x; i < NH * NH; i += blockDim.x) S.w2[(i / NH) * W2S + (i % NH)] = W.w2[i];
+  // 16-byte copies: W1 is 256 chunks of 8 BF16, W2 128 rows x 16 chunks (rows padded to W2S in LDS)
+  for (int i = threadIdx.x; i < NH * NF / 8; i += blockDim.x)
+    reinterpret_cast<uint4*>(S.w1)[i] = reinterpret_cast<const uint4*>(W.w1)[i];
+  for (int i = threadIdx.x; i < NH * NH / 8; i += blockDim.x) {
+    int row = i >> 4, ch = i & 15;
+    *reinterpret_cast<uint4*>(S.w2 + row * W2S + 8 * ch) = reinterpret_cast<const uint4*>(W.w2)[i];
+  }
   for (int i = threadIdx.x; i < NH; i += blockDim.x) {
     S.b1[i] = W.b1[i];
     S.b2[i] = W.b2[i];
@@ -99,15 +104,21 @@ __device__ void mlp_load_weights(const MlpWeights& W, MlpLds& S, unsigned char* 
 // One wave: 16 rows starting at `row0` (rows >= n are zero-padded).  Returns in `y4` the outputs of rows
 // 4*kg .. 4*kg+3 on lanes with l % 16 == 0.  z1 / z2 (optional, FP32 [n][128]) receive the
 // pre-activations for training.
-__device__ void mlp_tile(const MlpWeights& W, const MlpLds& S, unsigned short* h, const float* __restrict__ feat,
-                         int row0, int n, float* __restrict__ z1, float* __restrict__ z2, float y4[4]) {
+// The lane's four inputs of tile row0: row row0 + l % 16, features 4 (l / 16) .. +3.
+__device__ __forceinline__ float4 mlp_load_input(const float* __restrict__ feat, int row0, int n) {
+  const int l = threadIdx.x & 63;
+  const int r = row0 + (l & 15);
+  return r < n ? *reinterpret_cast<const float4*>(feat + NF * static_cast<long long>(r) + 4 * (l >> 4))
+               : make_float4(0, 0, 0, 0);
+}
+
+__device__ void mlp_tile(const MlpWeights& W, const MlpLds& S, unsigned short* h, float4 f, int row0, int n,
+                         float* __restrict__ z1, float* __restrict__ z2, float y4[4]) {
   const int l = threadIdx.x & 63, col = l & 15, kg = l >> 4;
   // ---- input tile (A operand): row col, features 4kg .. 4kg+3, normalized, BF16
   bf16x4 ax;
   {
     int r = row0 + col;
-    float4 f = r < n ? *reinterpret_cast<const float4*>(feat + NF * static_cast<long long>(r) + 4 * kg)
-                     : make_float4(0, 0, 0, 0);
     float v[4] = {f.x, f.y, f.z, f.w};
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
@@ -181,9 +192,14 @@ __global__ void __launch_bounds__(MLP_BLOCK) mlp_forward_kernel(MlpWeights W, co
   const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
   unsigned short* h = S.h + wave * 16 * W2S;
   const int tiles = (n + 15) / 16;
-  for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
+  const int stride = gridDim.x * 4;
+  int t = blockIdx.x * 4 + wave;
+  float4 f = t < tiles ? mlp_load_input(feat, t * 16, n) : make_float4(0, 0, 0, 0);
+  for (; t < tiles; t += stride) {
+    const float4 cur = f;
+    if (t + stride < tiles) f = mlp_load_input(feat, (t + stride) * 16, n);  // next tile in flight
     float y4[4];
-    mlp_tile(W, S, h, feat, t * 16, n, nullptr, nullptr, y4);
+    mlp_tile(W, S, h, cur, t * 16, n, nullptr, nullptr, y4);
     if ((l & 15) == 0) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -206,7 +222,7 @@ __global__ void __launch_bounds__(MLP_BLOCK) mlp_train_forward_kernel(MlpWeights
   const int tiles = (n + 15) / 16;
   for (int t = blockIdx.x * 4 + wave; t < tiles; t += gridDim.x * 4) {
     float y4[4];
-    mlp_tile(W, S, h, feat, t * 16, n, z1, z2, y4);
+    mlp_tile(W, S, h, mlp_load_input(feat, t * 16, n), t * 16, n, z1, z2, y4);
     if ((l & 15) == 0) {
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -353,7 +369,7 @@ extern "C" int nmc_mlp_forward(const NmcMlp* w, const float* feat, const int* co
   if (capacity <= 0) return 0;
   int tiles = (capacity + 15) / 16;
   int blocks = (tiles + 3) / 4;
-  if (blocks > 1024) blocks = 1024;  // 4 workgroups per CU; the kernel strides over the rest
+  if (blocks > 512) blocks = 512;  // two workgroups per CU (LDS-bound) resident at once; the rest is strided
   hipLaunchKernelGGL(mlp_forward_kernel, dim3(blocks), dim3(MLP_BLOCK), MLP_LDS, reinterpret_cast<hipStream_t>(stream),
                      make_weights(w), feat, count, capacity, lambda0);
   hipError_t e = hipGetLastError();

@@ -192,7 +192,7 @@ class SolverXPBDWarmStart(SolverXPBD):
     cold fallback with ``iterations``).  Requires a deterministic CollisionPipeline (sort keys)."""
 
     def __init__(self, model, mlp: WarmStartMLP, iterations: int = 10, warm_iterations: int = 3,
-                 residual_tol: float = 2.5e-3, **kw):
+                 residual_tol: float = 5e-3, **kw):
         super().__init__(model, iterations=iterations, **kw)
         if not 0 <= warm_iterations <= iterations:
             raise ValueError("warm_iterations must be in [0, iterations]")
@@ -212,19 +212,42 @@ class SolverXPBDWarmStart(SolverXPBD):
         S = m.world_count + 1
         self._buf = {
             "C": C, "feat": torch.zeros((C, N_FEATURES), **f), "lambda0": torch.zeros(C, **f),
-            "lambda_sum": torch.zeros(C, **f), "q_mid": torch.zeros((m.body_count, 7), **f),
+            "lam": [torch.zeros(C, **f), torch.zeros(C, **f)], "q_mid": torch.zeros((m.body_count, 7), **f),
             "qd_mid": torch.zeros((m.body_count, 6), **f), "residual": torch.zeros(S, **f),
-            "fallback": torch.zeros(S, **i), "prev_key": torch.zeros(C, device=dev, dtype=torch.int64),
-            "prev_start": torch.zeros(S, **i), "prev_count": torch.zeros(S, **i), "prev_lambda": torch.zeros(C, **f),
+            "fallback": torch.zeros(S, **i), "snap_key": None, "snap_start": torch.zeros(S, **i),
+            "snap_count": torch.zeros(S, **i),
         }
+        self._cur = 0
+        self._prev = None
+        self._snapshot = False
+        self._last_remembered = True
         return self._buf
 
-    def _io(self, mode: int, b: dict, warm_iterations: int = 0, residual_tol: float | None = None) -> NmcLearnIO:
+    # History for the "previous multiplier" input: the previous step's contacts (keys, per-world ranges)
+    # and its multipliers.  Callers that alternate two Contacts buffers between substeps cost nothing; a
+    # caller that reuses one buffer switches the solver to snapshot copies of the keys after each step
+    # (the first reuse is detected too late and that step sees no history).
+    def _history(self, contacts, b):
+        lam_prev = b["lam"][1 - self._cur]
+        if self._prev is None:
+            return None, b["snap_start"], b["snap_count"], lam_prev
+        if self._prev is contacts:
+            if not self._snapshot:
+                self._snapshot = True
+                b["snap_count"].zero_()
+                b["snap_key"] = torch.zeros_like(contacts.rigid_contact_sort_key)
+                return None, b["snap_start"], b["snap_count"], lam_prev
+            return b["snap_key"], b["snap_start"], b["snap_count"], lam_prev
+        pc = self._prev
+        return pc.rigid_contact_sort_key, pc.world_contact_start, pc.world_contact_count, lam_prev
+
+    def _io(self, mode: int, b: dict, contacts, warm_iterations: int = 0,
+            residual_tol: float | None = None) -> NmcLearnIO:
         tol = self.residual_tol if residual_tol is None else float(residual_tol)
-        return NmcLearnIO(mode, warm_iterations, tol, ptr(b["feat"]), ptr(b["lambda_sum"]),
+        key, start, count, lam_prev = self._history(contacts, b)
+        return NmcLearnIO(mode, warm_iterations, tol, ptr(b["feat"]), ptr(b["lam"][self._cur]),
                           ptr(b["lambda0"]), ptr(b["residual"]), ptr(b["fallback"]), ptr(b["q_mid"]),
-                          ptr(b["qd_mid"]), ptr(b["prev_key"]), ptr(b["prev_start"]), ptr(b["prev_count"]),
-                          ptr(b["prev_lambda"]))
+                          ptr(b["qd_mid"]), ptr(key), ptr(start), ptr(count), ptr(lam_prev))
 
     def _launch(self, state_in, state_out, control, contacts, dt, io, iterations: int | None = None):
         model = self.model
@@ -246,10 +269,12 @@ class SolverXPBDWarmStart(SolverXPBD):
             ctypes.c_void_p(stream_handle(model.device))), "nmc_xpbd_step_learn")
 
     def _remember(self, contacts, b):
-        b["prev_key"].copy_(contacts.rigid_contact_sort_key)
-        b["prev_start"].copy_(contacts.world_contact_start)
-        b["prev_count"].copy_(contacts.world_contact_count)
-        b["prev_lambda"].copy_(b["lambda_sum"])
+        if self._snapshot:
+            b["snap_key"].copy_(contacts.rigid_contact_sort_key)
+            b["snap_start"].copy_(contacts.world_contact_start)
+            b["snap_count"].copy_(contacts.world_contact_count)
+        self._prev = contacts
+        self._cur = 1 - self._cur
 
     def _check_contacts(self, contacts):
         if contacts is None or contacts.rigid_contact_sort_key is None:
@@ -264,9 +289,11 @@ class SolverXPBDWarmStart(SolverXPBD):
         k = self.warm_iterations if warm_iterations is None else int(warm_iterations)
         if not 0 <= k <= self.iterations:
             raise ValueError("warm_iterations must be in [0, iterations]")
-        self._launch(state_in, None, control, contacts, dt, self._io(NMC_STEP_PREP, b))
+        self._launch(state_in, None, control, contacts, dt, self._io(NMC_STEP_PREP, b, contacts))
         self.mlp.predict(b["feat"], contacts.rigid_contact_count, contacts.rigid_contact_max, b["lambda0"])
-        self._launch(state_in, state_out, control, contacts, dt, self._io(NMC_STEP_WARM, b, k, residual_tol))
+        self._launch(state_in, state_out, control, contacts, dt,
+                     self._io(NMC_STEP_WARM, b, contacts, k, residual_tol))
+        self._last_remembered = remember
         if remember:
             self._remember(contacts, b)
 
@@ -277,18 +304,22 @@ class SolverXPBDWarmStart(SolverXPBD):
         (``buffers['residual']``): training data and the plain reference of the comparison."""
         self._check_contacts(contacts)
         b = self._buffers(contacts)
-        self._launch(state_in, state_out, control, contacts, dt, self._io(NMC_STEP_LOG, b), iterations)
+        self._launch(state_in, state_out, control, contacts, dt, self._io(NMC_STEP_LOG, b, contacts), iterations)
+        self._last_remembered = remember
         if remember:
             self._remember(contacts, b)
 
     @property
     def buffers(self) -> dict:
-        return self._buf
+        """feat, lambda0, residual, fallback, q_mid, qd_mid, and lambda_sum: the total normal multipliers of
+        the last step."""
+        b = dict(self._buf)
+        b["lambda_sum"] = self._buf["lam"][1 - self._cur] if self._last_remembered else self._buf["lam"][self._cur]
+        return b
 
     def reset_history(self):
         """Forget the previous substep's contacts (no multiplier is matched on the next step)."""
-        if self._buf is not None:
-            self._buf["prev_count"].zero_()
+        self._prev = None
 
 
 __all__ = ["FEATURES", "SolverXPBDWarmStart", "WarmStartMLP", "WEIGHTS_PATH"]

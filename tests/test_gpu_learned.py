@@ -56,7 +56,7 @@ def test_log_step_matches_oracle(cuda_device):
     f = np.zeros_like(qd)
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
     solver = SolverXPBDWarmStart(model, WarmStartMLP(device=cuda_device), iterations=10)
-    c = pipe.contacts()
+    c, c_next = pipe.contacts(), pipe.contacts()  # alternating buffers keep step A's contacts for matching
     # step A (history), then step B with the matched previous multipliers
     sa, sb, sc = _state(model, q, qd), model.state(), model.state()
     pipe.collide(sa, c)
@@ -75,6 +75,7 @@ def test_log_step_matches_oracle(cuda_device):
     assert np.array_equal(fa[og], feat_a[orf])
     assert np.array_equal(solver.buffers["residual"].cpu().numpy(), res_a)
     # step B
+    c = c_next
     pipe.collide(sb, c)
     solver.log_step(sb, sc, None, c, DT)
     torch.cuda.synchronize()
@@ -92,12 +93,14 @@ def test_log_step_matches_oracle(cuda_device):
 
 
 def _warm(solver, s_in, s_out, c, lam0, k, tol):
+    """The warm launch with a given lambda0 in place of the network's prediction."""
     b = solver._buffers(c)
-    solver._launch(s_in, None, None, c, DT, solver._io(NMC_STEP_PREP, b))
+    solver._launch(s_in, None, None, c, DT, solver._io(NMC_STEP_PREP, b, c))
     b["lambda0"][:len(lam0)].copy_(torch.from_numpy(lam0))
-    solver._launch(s_in, s_out, None, c, DT, solver._io(NMC_STEP_WARM, b, k, tol))
+    solver._launch(s_in, s_out, None, c, DT, solver._io(NMC_STEP_WARM, b, c, k, tol))
+    solver._last_remembered = False
     torch.cuda.synchronize()
-    return b
+    return solver.buffers
 
 
 def test_warm_step_matches_oracle(cuda_device):
