@@ -93,6 +93,24 @@ typedef struct NmcModelDesc {
    * nmc_collide fails. */
   int rigid_contact_max;
   int shape_pairs_max;
+  /* triangle meshes (GeoType MESH, newton/_src/geometry/types.py Mesh): shape_mesh[s] = mesh index or -1;
+   * local AABB of each shape with its scale baked in (collide.py:215-245); mesh vertices and unique edges
+   * (builder.py:10316-10351, edge order of Mesh.edges); one dense FP32 SDF grid per mesh: (cells+1)
+   * vertices per axis from `lower` with spacing `voxel`, x fastest, negative inside. */
+  const int* shape_mesh;
+  const float* shape_aabb_lower; /* S x 3 */
+  const float* shape_aabb_upper; /* S x 3 */
+  int mesh_count;
+  const float* mesh_vertex;      /* V x 3 */
+  const int* mesh_vertex_start;  /* M + 1 */
+  const int* mesh_edge;          /* E x 2, vertex indices local to the mesh */
+  const int* mesh_edge_start;    /* M + 1 */
+  const float* mesh_sdf;         /* all grids */
+  const int64_t* mesh_sdf_start; /* M */
+  const float* mesh_sdf_lower;   /* M x 3 */
+  const float* mesh_sdf_voxel;   /* M */
+  const int* mesh_sdf_cells;     /* M x 3 */
+  int mesh_contact_max;          /* contacts kept per mesh pair (deepest first), at most 255 */
 } NmcModelDesc;
 
 /* Device-resident contact buffers (layout of newton.Contacts rigid part). */
@@ -223,8 +241,16 @@ int nmc_mlp_forward(const NmcMlp* weights, const float* feat, const int* count, 
 int nmc_mlp_train_grad(const NmcMlp* weights, const float* feat, const float* target, int n,
                        const NmcMlpTrainBuffers* buffers, NmcMlpGrads* grads, nmc_stream_t stream);
 
+/* Dense signed distance grid of one triangle mesh (the cook of Mesh.build_sdf, sdf_texture.py:959-1370):
+ * grid vertex (ix, iy, iz) at lower + voxel * (ix, iy, iz) gets the distance to the nearest triangle,
+ * negative where the generalized winding number exceeds 0.5.  out has (cells[2]+1)(cells[1]+1)(cells[0]+1)
+ * floats, x fastest.  No counterpart as a C entry point in the reference (a Warp kernel there). */
+int nmc_sdf_build(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
+                  const float lower[3], float voxel, const int cells[3], float* out, nmc_stream_t stream);
+
 /* Copies the counters accumulated since model creation to host (synchronises the stream):
- * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = reserved (0),
+ * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = mesh-pair candidate
+ * contacts dropped (candidate buffer full; cannot happen for models nmc_model_create accepted),
  * out[2] = contacts beyond the Contacts capacity, out[3] = worlds whose contacts did not fit the
  * solver's ordering buffer (cannot happen when contacts->capacity <= rigid_contact_max). */
 int nmc_get_counters(NmcModel* model, int* out, nmc_stream_t stream);

@@ -17,8 +17,8 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnmc.so")
-SOURCES = ["collide.hip", "xpbd.hip", "mlp.hip", "nmc_abi.hip"]
-HEADERS = ["nmc_math.h", "nmc_internal.h", "narrow.h"]
+SOURCES = ["collide.hip", "mesh.hip", "xpbd.hip", "mlp.hip", "nmc_abi.hip"]
+HEADERS = ["nmc_math.h", "nmc_internal.h", "narrow.h", "sdf.h"]
 
 
 def hipcc() -> str:

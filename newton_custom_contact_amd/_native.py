@@ -41,6 +41,10 @@ class NmcModelDesc(ctypes.Structure):
         ("joint_axis", _vp), ("joint_limit_lower", _vp), ("joint_limit_upper", _vp),
         ("joint_target_ke", _vp), ("joint_target_kd", _vp),
         ("rigid_contact_max", ctypes.c_int), ("shape_pairs_max", ctypes.c_int),
+        ("shape_mesh", _vp), ("shape_aabb_lower", _vp), ("shape_aabb_upper", _vp), ("mesh_count", ctypes.c_int),
+        ("mesh_vertex", _vp), ("mesh_vertex_start", _vp), ("mesh_edge", _vp), ("mesh_edge_start", _vp),
+        ("mesh_sdf", _vp), ("mesh_sdf_start", _vp), ("mesh_sdf_lower", _vp), ("mesh_sdf_voxel", _vp),
+        ("mesh_sdf_cells", _vp), ("mesh_contact_max", ctypes.c_int),
     ]
 
 
@@ -89,7 +93,9 @@ class NmcMlpGrads(ctypes.Structure):
 NMC_STEP_PLAIN, NMC_STEP_LOG, NMC_STEP_PREP, NMC_STEP_WARM = 0, 1, 2, 3
 
 EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_step", "nmc_xpbd_step_learn",
-           "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_get_counters", "nmc_last_error")
+           "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_get_counters", "nmc_last_error")
+
+MESH_CONTACT_MAX = 64  # contacts kept per mesh pair (the configuration's cap; the reference caps at 255)
 
 
 def lib_path() -> str:
@@ -124,6 +130,9 @@ def load() -> ctypes.CDLL:
         L.nmc_mlp_train_grad.argtypes = [ctypes.POINTER(NmcMlp), _vp, _vp, ctypes.c_int,
                                          ctypes.POINTER(NmcMlpTrainBuffers), ctypes.POINTER(NmcMlpGrads), _vp]
         L.nmc_mlp_train_grad.restype = ctypes.c_int
+        L.nmc_sdf_build.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
+                                    ctypes.c_float, ctypes.POINTER(ctypes.c_int), _vp, _vp]
+        L.nmc_sdf_build.restype = ctypes.c_int
         L.nmc_get_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.nmc_get_counters.restype = ctypes.c_int
         L.nmc_last_error.argtypes = []
@@ -150,6 +159,25 @@ def stream_handle(device: torch.device) -> int:
     return torch.cuda.current_stream(device).cuda_stream
 
 
+def check_mesh_pairs(model):
+    """Refuse models whose mesh shapes could meet a shape other than a mesh or an infinite plane: the
+    reference routes those pairs to its mesh-triangle + GJK/MPR path (narrow_phase.py:315-320), which is
+    not part of this library."""
+    h = model.host
+    if not getattr(model, "meshes", []):
+        return
+    coll = (h["shape_flags"] & 2) != 0  # ShapeFlags.COLLIDE_SHAPES
+    t = h["shape_type"]
+    sc = h["shape_scale"]
+    infinite_plane = (t == 1) & (sc[:, 0] == 0.0) & (sc[:, 1] == 0.0)
+    other = coll & (t != 8) & ~infinite_plane
+    if other.any():
+        raise NotImplementedError(
+            "a model with mesh shapes may only hold other mesh shapes and infinite planes as colliders "
+            f"(shape {int(np.nonzero(other)[0][0])} has GeoType {int(t[other][0])}); mesh-vs-primitive "
+            "contacts are not supported")
+
+
 class NativeModel:
     """Owns one NmcModel handle plus the extra device tables it points at."""
 
@@ -164,6 +192,7 @@ class NativeModel:
                 "bodies outside every world (world -1) next to regular worlds are not supported; "
                 "place them inside a world (static global geometry uses body -1 and is fine)"
             )
+        check_mesh_pairs(model)
         L = load()
         h = model.host
         dev = model.device
@@ -242,6 +271,17 @@ class NativeModel:
         d.joint_target_ke, d.joint_target_kd = ptr(m.joint_target_ke), ptr(m.joint_target_kd)
         d.rigid_contact_max = int(rigid_contact_max)
         d.shape_pairs_max = int(min(shape_pairs_max, 2**31 - 1))
+        d.shape_mesh = ptr(m.shape_mesh)
+        d.shape_aabb_lower, d.shape_aabb_upper = ptr(m.shape_aabb_lower), ptr(m.shape_aabb_upper)
+        d.mesh_count = len(getattr(m, "meshes", []))
+        if d.mesh_count and not m.mesh_has_sdf:
+            raise RuntimeError("mesh shapes need their SDF (Mesh.build_sdf) before the collision pipeline")
+        d.mesh_vertex, d.mesh_vertex_start = ptr(m.mesh_vertex), ptr(m.mesh_vertex_start)
+        d.mesh_edge, d.mesh_edge_start = ptr(m.mesh_edge), ptr(m.mesh_edge_start)
+        d.mesh_sdf, d.mesh_sdf_start = ptr(m.mesh_sdf), ptr(m.mesh_sdf_start)
+        d.mesh_sdf_lower, d.mesh_sdf_voxel, d.mesh_sdf_cells = (ptr(m.mesh_sdf_lower), ptr(m.mesh_sdf_voxel),
+                                                               ptr(m.mesh_sdf_cells))
+        d.mesh_contact_max = MESH_CONTACT_MAX
         self.desc = d
         self.model = model
         handle = _vp()

@@ -201,6 +201,7 @@ class ModelBuilder:
         self.shape_collision_group: list[int] = []
         self.shape_collision_radius: list[float] = []
         self.shape_world: list[int] = []
+        self.shape_source: list = []  # Mesh of a MESH shape, else None
         self.shape_label: list[str] = []
         self.shape_collision_filter_pairs: set[tuple[int, int]] = set()
 
@@ -343,6 +344,7 @@ class ModelBuilder:
             self.shape_material_restitution.append(builder.shape_material_restitution[s])
             self.shape_collision_group.append(builder.shape_collision_group[s])
             self.shape_collision_radius.append(builder.shape_collision_radius[s])
+            self.shape_source.append(builder.shape_source[s])
             self.shape_world.append(world)
             lbl = builder.shape_label[s]
             self.shape_label.append(f"{label_prefix}/{lbl}" if label_prefix else lbl)
@@ -558,8 +560,10 @@ class ModelBuilder:
         self.shape_collision_filter_pairs.add((int(a), int(b)))
 
     def add_shape(self, *, body: int, type: int, xform=None, cfg: ShapeConfig | None = None, scale=None,
-                  is_static: bool = False, label: str | None = None) -> int:
-        """builder.py:5449-5609."""
+                  src=None, is_static: bool = False, label: str | None = None) -> int:
+        """builder.py:5449-5609 (``src``: the Mesh of a MESH shape)."""
+        if type == GeoType.MESH and src is None:
+            raise ValueError("a MESH shape needs its Mesh (src)")
         xform = as_transform(xform)
         cfg = self.default_shape_cfg if cfg is None else cfg
         scale = (1.0, 1.0, 1.0) if scale is None else scale
@@ -581,7 +585,13 @@ class ModelBuilder:
         self.shape_material_mu_rolling.append(float(cfg.mu_rolling))
         self.shape_gap.append(float(cfg.gap) if cfg.gap is not None else float(self.rigid_gap))
         self.shape_collision_group.append(int(cfg.collision_group))
-        self.shape_collision_radius.append(compute_shape_radius(type, scale))
+        if type == GeoType.MESH:
+            # bounding sphere of the scaled local AABB (geometry/utils.py:86-97)
+            lo, hi = src.aabb(scale)
+            self.shape_collision_radius.append(float(0.5 * np.linalg.norm(hi - lo)))
+        else:
+            self.shape_collision_radius.append(compute_shape_radius(type, scale))
+        self.shape_source.append(src)
         self.shape_world.append(self._current_world)
         if cfg.has_shape_collision and cfg.collision_filter_parent:
             for parent_body, joint_idx in self.joint_parents.get(body, ()):
@@ -595,10 +605,18 @@ class ModelBuilder:
                 for cs in self.body_shapes.get(child_body, []):
                     self.add_shape_collision_filter_pair(shape, cs)
         if not is_static and cfg.density > 0.0 and body >= 0 and not self.body_lock_inertia[body]:
-            m, c, inertia = compute_inertia_shape(type, scale, cfg.density)
+            if type == GeoType.MESH:
+                m, c, inertia = src.shape_mass_properties(cfg.density, scale)
+            else:
+                m, c, inertia = compute_inertia_shape(type, scale, cfg.density)
             com_body = transform_point(xform, c)
             self._update_body_mass(body, m, inertia, com_body, xform[3:])
         return shape
+
+    def add_shape_mesh(self, body: int, xform=None, mesh=None, scale=None, cfg=None, label=None) -> int:
+        """builder.py:6052-6120: a triangle mesh (mass properties of the solid mesh, inertia.py:361-440)."""
+        return self.add_shape(body=body, type=GeoType.MESH, xform=xform, cfg=cfg,
+                              scale=(1.0, 1.0, 1.0) if scale is None else scale, src=mesh, label=label)
 
     def add_shape_plane(self, plane=(0.0, 0.0, 1.0, 0.0), xform=None, width: float = 10.0, length: float = 10.0,
                         body: int = -1, cfg=None, label=None) -> int:

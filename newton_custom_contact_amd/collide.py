@@ -5,10 +5,13 @@ Mirrors newton/_src/sim/collide.py (class ``CollisionPipeline``, line 464): cons
 capacity estimators (``_estimate_rigid_contact_max`` ``:278-376``, ``_compute_per_world_shape_pairs_max``
 ``:379-418``, ``_resolve_shape_pairs_max`` ``:421-439``).
 
-``collide`` launches three kernels from libnmc.so on the current torch stream and never synchronises the
-host, so it can be captured in a graph.  The broad phase always runs per-world sweep-and-prune; the
-``broad_phase`` argument accepts the reference's names and ``"nxn"``/``"explicit"`` produce the same pair
-set (every one of them reports the AABB-overlapping, filter-passing pairs).
+``collide`` launches its kernels from libnmc.so (segment broad phase, route scan, primitive narrow phase,
+mesh narrow phase when the model has meshes, contact count / scan / compaction) on the current torch
+stream and never synchronises the host, so it can be captured in a graph.  The broad phase always runs
+per-world sweep-and-prune; the ``broad_phase`` argument accepts the reference's names and
+``"nxn"``/``"explicit"`` produce the same pair set (every one of them reports the AABB-overlapping,
+filter-passing pairs).  Mesh shapes collide with meshes (SDF edge search) and infinite planes (vertex
+test); a model where a mesh could meet any other shape type is refused.
 """
 
 from __future__ import annotations
@@ -21,7 +24,7 @@ from ._native import NativeModel, check, load, ptr, stream_handle
 from ._types import GeoType, ShapeFlags
 from .contacts import Contacts
 
-_SUPPORTED = {int(GeoType.PLANE), int(GeoType.SPHERE), int(GeoType.CAPSULE), int(GeoType.BOX)}
+_SUPPORTED = {int(GeoType.PLANE), int(GeoType.SPHERE), int(GeoType.CAPSULE), int(GeoType.BOX), int(GeoType.MESH)}
 
 
 def estimate_rigid_contact_max(model) -> int:
@@ -151,8 +154,8 @@ class CollisionPipeline:
     def overflow_counters(self) -> dict:
         """Overflow counters of every collide so far (verify_narrow_phase_buffers, narrow_phase.py:1334)."""
         c = self.native.counters()
-        return {"pairs_dropped": int(c[0]), "contacts_over_capacity": int(c[2]),
-                "solver_ordering_overflow": int(c[3])}
+        return {"pairs_dropped": int(c[0]), "mesh_candidates_dropped": int(c[1]),
+                "contacts_over_capacity": int(c[2]), "solver_ordering_overflow": int(c[3])}
 
 
 __all__ = ["CollisionPipeline", "estimate_rigid_contact_max", "per_world_shape_pairs_max"]

@@ -16,6 +16,7 @@
 //   collide_segment_kernel  <<<segments, 256>>>   AABB + SAP + pair sort -> pair region, route list
 //   route_scan_kernel       <<<1, 1024>>>         route offsets over segments
 //   narrow_kernel           <<<<=2048, 256>>>     narrow phase -> contact slots, per-pair counts
+//   mesh_narrow_kernel      <<<<=2048, 256>>>     mesh pairs, one workgroup per pair (models with meshes)
 //   contact_count_kernel    <<<segments, 64>>>    contacts per segment
 //   contact_scan_kernel     <<<1, 1024>>>         Contacts.world_start / world_count / count
 //   contact_compact_kernel  <<<segments, 256>>>   slots -> Contacts arrays
@@ -105,38 +106,52 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int* s_wave) {
   return v + add;
 }
 
-// Exclusive scans of two per-segment count arrays by one 1024-lane workgroup; returns the totals.
+// Exclusive scans of N per-segment count arrays by one 1024-lane workgroup; writes the totals.  A null
+// input array scans as zeros, a null output array is skipped.
 constexpr int SCAN_BLOCK = 1024;
-__device__ void segment_scan2(const int* __restrict__ a, const int* __restrict__ b, int* __restrict__ a_out,
-                              int* __restrict__ b_out, int n, int& a_total, int& b_total) {
-  __shared__ int s_wa[SCAN_BLOCK / 64], s_wb[SCAN_BLOCK / 64];
-  __shared__ int s_carry[2];
+template <int N>
+__device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], int n, int (&total)[N]) {
+  __shared__ int s_w[N][SCAN_BLOCK / 64];
+  __shared__ int s_carry[N];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  if (threadIdx.x == 0) { s_carry[0] = 0; s_carry[1] = 0; }
+  if (threadIdx.x < N) s_carry[threadIdx.x] = 0;
   __syncthreads();
   for (int base = 0; base < n; base += SCAN_BLOCK) {
     const int i = base + threadIdx.x;
-    const int va = i < n ? a[i] : 0;
-    const int vb = (b && i < n) ? b[i] : 0;
-    int ia = va, ib = vb;
+    int v[N], inc[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      v[k] = (in[k] && i < n) ? in[k][i] : 0;
+      inc[k] = v[k];
+    }
     for (int off = 1; off < 64; off <<= 1) {
-      int ta = __shfl_up(ia, off, 64), tb = __shfl_up(ib, off, 64);
-      if (lane >= off) { ia += ta; ib += tb; }
+#pragma unroll
+      for (int k = 0; k < N; ++k) {
+        int t = __shfl_up(inc[k], off, 64);
+        if (lane >= off) inc[k] += t;
+      }
     }
-    if (lane == 63) { s_wa[w] = ia; s_wb[w] = ib; }
-    __syncthreads();
-    int ca = s_carry[0], cb = s_carry[1];
-    for (int k = 0; k < w; ++k) { ca += s_wa[k]; cb += s_wb[k]; }
-    if (i < n) {
-      a_out[i] = ca + ia - va;
-      if (b_out) b_out[i] = cb + ib - vb;
+    if (lane == 63) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) s_w[k][w] = inc[k];
     }
     __syncthreads();
-    if (threadIdx.x == SCAN_BLOCK - 1) { s_carry[0] = ca + ia; s_carry[1] = cb + ib; }
+    int c[N];
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+      c[k] = s_carry[k];
+      for (int q = 0; q < w; ++q) c[k] += s_w[k][q];
+      if (i < n && out[k]) out[k][i] = c[k] + inc[k] - v[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == SCAN_BLOCK - 1) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) s_carry[k] = c[k] + inc[k];
+    }
     __syncthreads();
   }
-  a_total = s_carry[0];
-  b_total = s_carry[1];
+#pragma unroll
+  for (int k = 0; k < N; ++k) total[k] = s_carry[k];
 }
 
 struct SegLds {
@@ -154,24 +169,6 @@ struct SegLds {
   unsigned* pairs;            // pair keys (pair_cap)
 };
 
-__device__ __forceinline__ void store_wc(float* __restrict__ o, const ShapeW& w) {
-  store_xform(o, w.X);
-  store3(o + 7, w.gscale);
-  o[10] = w.margin;
-  store3(o + 11, w.lo);
-  store3(o + 14, w.hi);
-}
-
-__device__ __forceinline__ ShapeW load_wc(const float* __restrict__ o) {
-  ShapeW s;
-  s.X = load_xform(o);
-  s.gscale = load3(o + 7);
-  s.margin = o[10];
-  s.lo = load3(o + 11);
-  s.hi = load3(o + 14);
-  return s;
-}
-
 // Broad phase of one segment: shape world data (kept in shape_wc for the narrow phase), SAP, the
 // segment's pairs in key order at staging pairs [base, base + np), and their route entries.
 __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d, const float* __restrict__ body_q,
@@ -180,12 +177,13 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
                                                                   const int* __restrict__ seg_pair_cap,
                                                                   int* __restrict__ seg_pair_count,
                                                                   int* __restrict__ seg_conv_count,
+                                                                  int* __restrict__ seg_mesh_count,
                                                                   int* __restrict__ counters, int pair_cap,
                                                                   int max_n) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_npairs;
-  __shared__ int s_cls[2];
-  __shared__ int s_rbase[2];
+  __shared__ int s_cls[3];
+  __shared__ int s_rbase[3];
   const int seg = blockIdx.x;
   const int s0 = d.segment_start[seg];
   const int n = d.segment_start[seg + 1] - s0;
@@ -211,6 +209,7 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
     s_npairs = 0;
     s_cls[0] = 0;
     s_cls[1] = 0;
+    s_cls[2] = 0;
   }
 
   // ---- shape world transforms, AABBs, SAP projections (collide.py:150-275, broad_phase_sap.py:43-63)
@@ -271,6 +270,8 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
       // narrow-phase order: lower GeoType first (narrow_phase.py:212-215); ranks order like global ids
       int ra = la, rb = lb;
       if (L.type[la] > L.type[lb]) { ra = lb; rb = la; }
+      // mesh-plane pairs are kept as (mesh, plane) (narrow_phase.py:303-313)
+      if (L.type[ra] == GEO_PLANE && L.type[rb] == GEO_MESH) { int t = ra; ra = rb; rb = t; }
       int slot = atomicAdd(&s_npairs, 1);
       if (slot < pair_cap) L.pairs[slot] = (static_cast<unsigned>(ra) << 16) | static_cast<unsigned>(rb);
     }
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
   __syncthreads();
   bitonic_sort(L.pairs, pn2);
 
-  // ---- pairs out into the segment's region (key order) + its route list (MPR/GJK first)
+  // ---- pairs out into the segment's region (key order) + its route list (MPR/GJK, analytic, mesh)
   const long long poff = seg_pair_off[seg];
   const int pcap = seg_pair_cap[seg];
   if (np > pcap) np = pcap;  // cannot happen: np <= n(n-1)/2 and np <= pair_cap
@@ -295,38 +296,47 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
     int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
     st.pair_a[poff + pi] = L.gid[ra];
     st.pair_b[poff + pi] = L.gid[rb];
-    atomicAdd(&s_cls[convex_route(L.type[ra], L.type[rb]) ? 1 : 0], 1);
+    atomicAdd(&s_cls[route_class(L.type[ra], L.type[rb])], 1);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     seg_pair_count[seg] = np;
     seg_conv_count[seg] = s_cls[1];
+    seg_mesh_count[seg] = s_cls[2];
     s_rbase[1] = 0;
     s_rbase[0] = s_cls[1];
+    s_rbase[2] = s_cls[1] + s_cls[0];
     s_cls[0] = 0;
     s_cls[1] = 0;
+    s_cls[2] = 0;
   }
   __syncthreads();
   for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
     unsigned key = L.pairs[pi];
-    int cls = convex_route(L.type[key >> 16], L.type[key & 0xffffu]) ? 1 : 0;
+    int cls = route_class(L.type[key >> 16], L.type[key & 0xffffu]);
     st.route[poff + s_rbase[cls] + atomicAdd(&s_cls[cls], 1)] = pi;
   }
 }
 
-// Route offsets: exclusive prefix of MPR/GJK and analytic pair counts over segments.
+// Route offsets: exclusive prefix of MPR/GJK, analytic and mesh pair counts over segments.
 __global__ void __launch_bounds__(SCAN_BLOCK) route_scan_kernel(const int* __restrict__ seg_pair_count,
                                                                const int* __restrict__ seg_conv_count,
+                                                               const int* __restrict__ seg_mesh_count,
                                                                int* __restrict__ seg_conv_off,
                                                                int* __restrict__ seg_ana_off,
+                                                               int* __restrict__ seg_mesh_off,
                                                                int* __restrict__ route_count, int segments) {
-  int conv_total, pair_total;
-  // pair offsets minus conv offsets = analytic offsets
-  segment_scan2(seg_conv_count, seg_pair_count, seg_conv_off, seg_ana_off, segments, conv_total, pair_total);
-  for (int i = threadIdx.x; i < segments; i += blockDim.x) seg_ana_off[i] -= seg_conv_off[i];
+  // pair offsets minus conv and mesh offsets = analytic offsets
+  const int* const in[3] = {seg_conv_count, seg_pair_count, seg_mesh_count};
+  int* const out[3] = {seg_conv_off, seg_ana_off, seg_mesh_off};
+  int total[3];
+  segment_scan<3>(in, out, segments, total);
+  for (int i = threadIdx.x; i < segments; i += blockDim.x)
+    seg_ana_off[i] -= seg_conv_off[i] + (seg_mesh_off ? seg_mesh_off[i] : 0);
   if (threadIdx.x == 0) {
-    route_count[0] = pair_total - conv_total;
-    route_count[1] = conv_total;
+    route_count[0] = total[1] - total[0] - total[2];
+    route_count[1] = total[0];
+    route_count[2] = total[2];
   }
 }
 
@@ -378,8 +388,11 @@ __global__ void __launch_bounds__(64) contact_count_kernel(StageBuffers st, cons
 __global__ void __launch_bounds__(SCAN_BLOCK) contact_scan_kernel(const int* __restrict__ seg_contact_count,
                                                                  NmcContacts c, int segments,
                                                                  int* __restrict__ counters) {
-  int total, unused;
-  segment_scan2(seg_contact_count, nullptr, c.world_start, nullptr, segments, total, unused);
+  const int* const in[1] = {seg_contact_count};
+  int* const out[1] = {c.world_start};
+  int tot[1];
+  segment_scan<1>(in, out, segments, tot);
+  const int total = tot[0];
   for (int i = threadIdx.x; i < segments; i += blockDim.x) {
     int start = c.world_start[i];
     int room = c.capacity - start;
@@ -393,11 +406,14 @@ __global__ void __launch_bounds__(SCAN_BLOCK) contact_scan_kernel(const int* __r
   }
 }
 
-// Staging -> Contacts at canonical indices, one lane per output contact (coalesced stores).
+// Staging -> Contacts at canonical indices, one lane per output contact (coalesced stores).  MESH:
+// the model has mesh shapes, whose pairs (mesh first) keep their records in the mesh slots.
+template <bool MESH>
 __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st,
                                                                   const long long* __restrict__ seg_pair_off,
                                                                   const int* __restrict__ seg_pair_count,
-                                                                  NmcContacts c) {
+                                                                  const int* __restrict__ shape_type,
+                                                                  int mesh_k, NmcContacts c) {
   __shared__ int s_scan[CBLOCK];
   __shared__ int s_wave[CBLOCK / 64];
   const int seg = blockIdx.x;
@@ -422,9 +438,10 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
       }
       const int k = t - (lo > 0 ? s_scan[lo - 1] : 0);
       const long long g = poff + b0 + lo;
-      const long long s = 5 * g + k;
       const int o = start + local_idx;
       const int s0 = st.pair_a[g], s1 = st.pair_b[g];
+      long long s = 5 * g + k;
+      if (MESH && shape_type[s0] == GEO_MESH) s = st.mesh_slot_base + static_cast<long long>(st.mesh_index[g]) * mesh_k + k;
       const float4* r = st.rec + 5 * s;
       const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
       c.shape0[o] = s0;
@@ -454,26 +471,38 @@ size_t collide_lds_bytes(int max_n, int pair_cap) {
   return (b + 15) & ~size_t(15);
 }
 
+hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t stream);
+
 hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContacts* c, hipStream_t stream) {
   const NmcModelDesc& d = m->d;
   int segs = d.segment_count;
   if (segs <= 0) return hipSuccess;
   hipLaunchKernelGGL(collide_segment_kernel, dim3(segs), dim3(CBLOCK), m->collide_lds, stream, d, body_q, m->stage,
-                     m->seg_pair_off, m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count, m->counters,
-                     m->pair_cap, d.max_segment_shapes);
+                     m->seg_pair_off, m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count, m->seg_mesh_count,
+                     m->counters, m->pair_cap, d.max_segment_shapes);
+  const bool mesh = d.mesh_count > 0;
   hipLaunchKernelGGL(route_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, m->seg_pair_count, m->seg_conv_count,
-                     m->seg_conv_off, m->seg_ana_off, m->stage.route_count, segs);
+                     mesh ? m->seg_mesh_count : nullptr, m->seg_conv_off, m->seg_ana_off,
+                     mesh ? m->seg_mesh_off : nullptr, m->stage.route_count, segs);
   long long pairs = m->stage.pair_capacity;
   long long nblk = (pairs + CBLOCK - 1) / CBLOCK;
   int narrow_blocks = static_cast<int>(nblk < NARROW_MAX_BLOCKS ? nblk : NARROW_MAX_BLOCKS);
   hipLaunchKernelGGL(narrow_kernel, dim3(narrow_blocks), dim3(CBLOCK), 0, stream, d, body_q, m->stage,
                      m->seg_pair_off, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, segs);
+  if (mesh) {
+    hipError_t e = launch_mesh_narrow(m, body_q, stream);
+    if (e != hipSuccess) return e;
+  }
   hipLaunchKernelGGL(contact_count_kernel, dim3(segs), dim3(64), 0, stream, m->stage, m->seg_pair_off,
                      m->seg_pair_count, m->seg_contact_count);
   hipLaunchKernelGGL(contact_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, m->seg_contact_count, *c, segs,
                      m->counters);
-  hipLaunchKernelGGL(contact_compact_kernel, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_off,
-                     m->seg_pair_count, *c);
+  if (mesh)
+    hipLaunchKernelGGL(contact_compact_kernel<true>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_off,
+                       m->seg_pair_count, d.shape_type, d.mesh_contact_max, *c);
+  else
+    hipLaunchKernelGGL(contact_compact_kernel<false>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage,
+                       m->seg_pair_off, m->seg_pair_count, d.shape_type, d.mesh_contact_max, *c);
   return hipGetLastError();
 }
 

@@ -110,6 +110,18 @@ __device__ __forceinline__ ShapeW shape_world(int s, const float* __restrict__ b
     f3 he{rad, rad, rad};
     r.lo = r.X.p - he - mvec;
     r.hi = r.X.p + he + mvec;
+  } else if (type == GEO_MESH) {
+    // local AABB (scale baked in) rotated into the world frame (collide.py:215-245)
+    f3 llo = load3(d.shape_aabb_lower + 3 * s), lhi = load3(d.shape_aabb_upper + 3 * s);
+    f3 c = (llo + lhi) * 0.5f, h = (lhi - llo) * 0.5f;
+    f3 wc = rot(r.X.q, c) + r.X.p;
+    f3 r0 = rot(r.X.q, f3{1.0f, 0.0f, 0.0f}), r1 = rot(r.X.q, f3{0.0f, 1.0f, 0.0f});
+    f3 r2 = rot(r.X.q, f3{0.0f, 0.0f, 1.0f});
+    f3 wh{fabsf(r0.x) * h.x + fabsf(r1.x) * h.y + fabsf(r2.x) * h.z,
+          fabsf(r0.y) * h.x + fabsf(r1.y) * h.y + fabsf(r2.y) * h.z,
+          fabsf(r0.z) * h.x + fabsf(r1.z) * h.y + fabsf(r2.z) * h.z};
+    r.lo = wc - wh - mvec;
+    r.hi = wc + wh + mvec;
   } else {
     GShape g;
     g.type = type;
@@ -123,6 +135,25 @@ __device__ __forceinline__ ShapeW shape_world(int s, const float* __restrict__ b
   r.gscale = gscale;
   r.margin = margin;
   return r;
+}
+
+// Shape world data as kept in StageBuffers.shape_wc (20 floats per shape).
+__device__ __forceinline__ void store_wc(float* __restrict__ o, const ShapeW& w) {
+  store_xform(o, w.X);
+  store3(o + 7, w.gscale);
+  o[10] = w.margin;
+  store3(o + 11, w.lo);
+  store3(o + 14, w.hi);
+}
+
+__device__ __forceinline__ ShapeW load_wc(const float* __restrict__ o) {
+  ShapeW s;
+  s.X = load_xform(o);
+  s.gscale = load3(o + 7);
+  s.margin = o[10];
+  s.lo = load3(o + 11);
+  s.hi = load3(o + 14);
+  return s;
 }
 
 // ------------------------------------------------------------------------- contact records
@@ -142,14 +173,8 @@ __device__ __forceinline__ bool passes_gap(const CData& c) {
   return d <= c.gap_sum;
 }
 
-// write_contact (collide.py:62-148) into the pair's next slot.
-__device__ __forceinline__ void emit(const CData& c, WriteCtx& w, bool check_gap) {
-  f3 n, aw, bw;
-  float d = contact_d(c, n, aw, bw);
-  float gap = w.shape_gap[c.shape_a] + w.shape_gap[c.shape_b];
-  if (check_gap && d > gap) return;
-  if (w.n >= 5) return;
-  const long long s = w.slot0 + w.n;
+// write_contact (collide.py:62-148) body: the staging record of contact c at slot s.
+__device__ __forceinline__ void write_record(const CData& c, f3 n, f3 aw, f3 bw, const WriteCtx& w, long long s) {
   float off_a = c.r_a + c.m_a, off_b = c.r_b + c.m_b;
   int b0 = w.shape_body[c.shape_a], b1 = w.shape_body[c.shape_b];
   xform Xa = b0 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b0));
@@ -162,6 +187,16 @@ __device__ __forceinline__ void emit(const CData& c, WriteCtx& w, bool check_gap
   r[2] = make_float4(o0.z, o1.x, o1.y, o1.z);
   r[3] = make_float4(n.x, n.y, n.z, off_a);
   r[4] = make_float4(off_b, __int_as_float(c.sub_key), 0.0f, 0.0f);
+}
+
+// write_contact (collide.py:62-148) into the pair's next slot.
+__device__ __forceinline__ void emit(const CData& c, WriteCtx& w, bool check_gap) {
+  f3 n, aw, bw;
+  float d = contact_d(c, n, aw, bw);
+  float gap = w.shape_gap[c.shape_a] + w.shape_gap[c.shape_b];
+  if (check_gap && d > gap) return;
+  if (w.n >= 5) return;
+  write_record(c, n, aw, bw, w, w.slot0 + w.n);
   w.n += 1;
 }
 
@@ -1027,6 +1062,13 @@ __device__ void solve_convex(const PairCtx& P, float threshold, bool skip_multi,
 // type pair to an analytic collider).
 __device__ __forceinline__ bool convex_route(int ta, int tb) {
   return tb == GEO_BOX && (ta == GEO_BOX || ta == GEO_CAPSULE);
+}
+
+// Route class of a pair: 0 analytic, 1 MPR/GJK, 2 mesh (mesh-mesh or mesh-infinite plane,
+// narrow_phase.py:290-320; other mesh pairs are refused when the model is created).
+__device__ __forceinline__ int route_class(int ta, int tb) {
+  if (ta == GEO_MESH || tb == GEO_MESH) return 2;
+  return convex_route(ta, tb) ? 1 : 0;
 }
 
 // One broad-phase pair (sa, sb already ordered by GeoType) -> up to 5 records at w.slot0.

@@ -11,6 +11,7 @@
 namespace nmc {
 
 size_t collide_lds_bytes(int max_n, int pair_cap);
+size_t mesh_lds_bytes(int cand_cap);
 size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap);
 hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContacts* c, hipStream_t stream);
 hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
@@ -56,6 +57,14 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   if (d.segment_count != d.world_count + 1) return fail("segment_count must equal world_count + 1");
   if (d.max_world_bodies > 512) return fail("worlds with more than 512 bodies are not supported by the XPBD kernel");
   if (d.max_segment_shapes > 65535) return fail("a broad-phase segment may hold at most 65535 shapes");
+  if (d.mesh_count < 0) return fail("mesh_count must be >= 0");
+  if (d.mesh_count > 0) {
+    if (!d.shape_mesh || !d.shape_aabb_lower || !d.shape_aabb_upper || !d.mesh_vertex || !d.mesh_vertex_start ||
+        !d.mesh_edge || !d.mesh_edge_start || !d.mesh_sdf || !d.mesh_sdf_start || !d.mesh_sdf_lower ||
+        !d.mesh_sdf_voxel || !d.mesh_sdf_cells)
+      return fail("a model with meshes needs every mesh table (vertices, edges, local AABBs, SDF grids)");
+    if (d.mesh_contact_max < 1 || d.mesh_contact_max > 255) return fail("mesh_contact_max must be in [1, 255]");
+  }
   NmcModel* m = new NmcModel();
   m->d = d;
   hipError_t e = hipGetDevice(&m->device);
@@ -101,9 +110,52 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     long long slots = 5 * pairs;
     StageBuffers& s = m->stage;
     s.pair_capacity = pairs;
+    s.mesh_slot_base = slots;
+    s.mesh_pair_capacity = 0;
+    if (d.mesh_count > 0) {
+      // mesh pairs: per segment at most the pairs that involve one of its m mesh shapes,
+      // n(n-1)/2 - (n-m)(n-m-1)/2, within the segment's region; candidates per pair: both meshes' edges
+      // (mesh-mesh) or the mesh's vertices (mesh-plane)
+      const int nss = starts[segs];
+      std::vector<int> seg_shape(nss > 0 ? nss : 1), stype(d.shape_count > 0 ? d.shape_count : 1);
+      std::vector<int> vstart(d.mesh_count + 1), estart(d.mesh_count + 1);
+      if ((nss > 0 && (e = hipMemcpy(seg_shape.data(), d.segment_shape, sizeof(int) * nss, hipMemcpyDeviceToHost)) != hipSuccess) ||
+          (d.shape_count > 0 && (e = hipMemcpy(stype.data(), d.shape_type, sizeof(int) * d.shape_count, hipMemcpyDeviceToHost)) != hipSuccess) ||
+          (e = hipMemcpy(vstart.data(), d.mesh_vertex_start, sizeof(int) * (d.mesh_count + 1), hipMemcpyDeviceToHost)) != hipSuccess ||
+          (e = hipMemcpy(estart.data(), d.mesh_edge_start, sizeof(int) * (d.mesh_count + 1), hipMemcpyDeviceToHost)) != hipSuccess) {
+        nmc_model_destroy(m);
+        return hip_fail(e, "nmc_model_create: mesh tables");
+      }
+      long long mesh_pairs = 0;
+      for (int sg = 0; sg < segs; ++sg) {
+        long long n = starts[sg + 1] - starts[sg], mm = 0;
+        for (int k = starts[sg]; k < starts[sg + 1]; ++k) mm += stype[seg_shape[k]] == GEO_MESH;
+        long long w = n * (n - 1) / 2 - (n - mm) * (n - mm - 1) / 2;
+        mesh_pairs += w < cap[sg] ? w : cap[sg];
+      }
+      int need = d.mesh_contact_max;
+      int max_e = 0;
+      for (int k = 0; k < d.mesh_count; ++k) {
+        int nv = vstart[k + 1] - vstart[k], ne = estart[k + 1] - estart[k];
+        max_e = ne > max_e ? ne : max_e;
+        need = nv > need ? nv : need;
+      }
+      need = 2 * max_e > need ? 2 * max_e : need;
+      if (need > 4096) {
+        nmc_model_destroy(m);
+        return fail("mesh narrow phase: a pair may gather " + std::to_string(need) +
+                    " candidate contacts (2 x edges or vertices of the largest mesh); the limit is 4096");
+      }
+      m->mesh_cand_cap = next_pow2(need);
+      m->mesh_lds = mesh_lds_bytes(m->mesh_cand_cap);
+      s.mesh_pair_capacity = mesh_pairs;
+      slots += mesh_pairs * d.mesh_contact_max;
+    }
     if ((e = dmalloc(&s.rec, 5 * slots)) != hipSuccess || (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
         (e = dmalloc(&s.pair_b, pairs)) != hipSuccess || (e = dmalloc(&s.route, pairs)) != hipSuccess ||
-        (e = dmalloc(&s.route_count, 2)) != hipSuccess ||
+        (e = dmalloc(&s.route_count, 3)) != hipSuccess ||
+        (d.mesh_count > 0 && (e = dmalloc(&s.mesh_index, pairs)) != hipSuccess) ||
+        (e = dmalloc(&m->seg_mesh_count, segs)) != hipSuccess || (e = dmalloc(&m->seg_mesh_off, segs)) != hipSuccess ||
         (e = dmalloc(&s.shape_wc, 20 * (size_t)(d.shape_count > 0 ? d.shape_count : 1))) != hipSuccess ||
         (e = dmalloc(&m->seg_pair_off, segs)) != hipSuccess || (e = dmalloc(&m->seg_pair_cap, segs)) != hipSuccess ||
         (e = dmalloc(&m->seg_pair_count, segs)) != hipSuccess || (e = dmalloc(&m->seg_conv_count, segs)) != hipSuccess ||
@@ -123,7 +175,8 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
   if (!m) return;
   StageBuffers& s = m->stage;
   void* ptrs[] = {s.rec, s.mask,
-                  s.pair_a, s.pair_b, s.route, s.route_count, s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
+                  s.pair_a, s.pair_b, s.route, s.route_count, s.mesh_index, m->seg_mesh_count, m->seg_mesh_off,
+                  s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
                   m->seg_pair_count, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, m->seg_contact_count,
                   m->counters, m->xpbd_keys};
   for (void* p : ptrs)

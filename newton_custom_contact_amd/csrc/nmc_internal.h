@@ -9,7 +9,7 @@
 namespace nmc {
 
 // GeoType values (newton/_src/geometry/types.py:38-78)
-enum GeoTypeId { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7 };
+enum GeoTypeId { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7, GEO_MESH = 8 };
 // ShapeFlags.COLLIDE_SHAPES (newton/_src/geometry/flags.py:25), BodyFlags.KINEMATIC (sim/enums.py:21)
 enum { SHAPE_COLLIDE_SHAPES = 2, BODY_KINEMATIC = 2 };
 // JointType (newton/_src/sim/enums.py:29-56)
@@ -20,17 +20,22 @@ constexpr float MAXVAL = 1.0e10f;  // newton/_src/core/types.py:72
 
 // Broad-phase pairs and their staged contact records.  Every segment owns a fixed region of pair
 // entries (its worst case n(n-1)/2, capped by the LDS pair buffer) at seg_pair_off[seg]; pair g owns
-// contact slots [5g, 5g + 5).  Fixed regions need no global allocation counter.
+// contact slots [5g, 5g + 5), except mesh pairs: the i-th mesh pair of the call (route order) owns
+// slots [mesh_slot_base + i K, mesh_slot_base + (i + 1) K), K = mesh_contact_max.  Fixed regions need
+// no global allocation counter.
 struct StageBuffers {
   // one 80-byte record per contact slot: point0 3, point1 3, offset0 3, offset1 3, normal 3, margin0,
   // margin1, sub_key (int bits), pad -- written and read as five 16-byte vectors
   float4* rec;
-  unsigned char* mask;   // per pair: number of records in its slots (0-5)
+  unsigned char* mask;   // per pair: number of records in its slots (0-5, mesh pairs 0-K)
   int* pair_a;           // first shape (lower GeoType), global id
   int* pair_b;
   long long pair_capacity;  // total pair entries (sum of the segment regions)
-  int* route;            // per segment region: local pair indices, MPR/GJK pairs first, then analytic
-  int* route_count;      // [2] analytic, MPR/GJK pairs over all segments
+  int* route;            // per segment region: local pair indices, MPR/GJK pairs, analytic, then mesh
+  int* route_count;      // [3] analytic, MPR/GJK and mesh pairs over all segments
+  int* mesh_index;       // per pair entry: mesh pair index (route order), written for mesh pairs only
+  long long mesh_slot_base;  // first mesh slot (= 5 x pair_capacity)
+  long long mesh_pair_capacity;  // mesh pairs the mesh slots hold (worst case over the segment regions)
   float* shape_wc;       // [shape_count x 20] world transform 7, scale 3, margin 1, AABB lo 3 / hi 3, pad 3
 };
 
@@ -45,6 +50,10 @@ struct NmcModelImpl {
   int* seg_conv_count;      // [segments] of which MPR/GJK pairs
   int* seg_conv_off;        // [segments] exclusive prefix of MPR/GJK pairs over segments
   int* seg_ana_off;         // [segments] exclusive prefix of analytic pairs over segments
+  int* seg_mesh_count;      // [segments] mesh pairs (mesh-mesh, mesh-infinite plane)
+  int* seg_mesh_off;        // [segments] exclusive prefix of mesh pairs over segments
+  int mesh_cand_cap;        // candidate contacts one mesh pair gathers in LDS (power of two)
+  size_t mesh_lds;
   int* seg_contact_count;   // [segments] contacts produced this call
   int* counters;         // [4] overflow counters (see nmc_get_counters)
   int pair_cap;          // per-segment pair buffer (LDS) capacity

@@ -165,14 +165,75 @@ class Model:
         m.shape_collision_filter_pairs = set(b.shape_collision_filter_pairs)
         fp = sorted(m.shape_collision_filter_pairs)
         h["shape_filter_pairs"] = arr(fp, _I32, (len(fp), 2))
+        m._mesh_tables(list(getattr(b, "shape_source", [None] * S)), [tuple(x) for x in b.shape_scale])
         m._finish()
         return m
+
+    def _mesh_tables(self, sources: list, scales: list | None = None):
+        """Per-shape mesh index and local AABB (scale baked, collide.py:215-245), and the concatenated
+        mesh vertex / edge tables (builder.py:10316-10351)."""
+        h = self.host
+        S = self.shape_count
+        meshes, index = [], {}
+        shape_mesh = np.full(S, -1, _I32)
+        lo = np.zeros((S, 3), _F32)
+        hi = np.zeros((S, 3), _F32)
+        for s, src in enumerate(sources):
+            if src is None:
+                continue
+            if id(src) not in index:
+                index[id(src)] = len(meshes)
+                meshes.append(src)
+            shape_mesh[s] = index[id(src)]
+            a, b_ = src.local_aabb(h["shape_scale"][s] if scales is None else scales[s])
+            lo[s], hi[s] = a, b_
+        self.meshes = meshes
+        h["shape_mesh"] = shape_mesh
+        h["shape_aabb_lower"] = lo
+        h["shape_aabb_upper"] = hi
+        vs = [np.zeros((0, 3), _F32)] + [mm.vertices for mm in meshes]
+        es = [np.zeros((0, 2), _I32)] + [mm.edges.astype(_I32) for mm in meshes]
+        h["mesh_vertex"] = np.concatenate(vs).astype(_F32) if meshes else np.zeros((1, 3), _F32)
+        h["mesh_vertex_start"] = np.concatenate([[0], np.cumsum([len(mm.vertices) for mm in meshes])]).astype(_I32)
+        h["mesh_edge"] = np.concatenate(es).astype(_I32) if meshes else np.zeros((1, 2), _I32)
+        h["mesh_edge_start"] = np.concatenate([[0], np.cumsum([len(mm.edges) for mm in meshes])]).astype(_I32)
+
+    def _mesh_sdf_tables(self):
+        """Concatenate the meshes' SDF grids on the device (cooked on the GPU when missing)."""
+        meshes = getattr(self, "meshes", [])
+        M = len(meshes)
+        if M and self.device.type == "cuda":
+            for mm in meshes:
+                if mm.sdf is None:
+                    mm.build_sdf(device=self.device)
+        have = M > 0 and all(mm.sdf is not None for mm in meshes)
+        if have:
+            grids = [mm.sdf.grid.reshape(-1).to(self.device) for mm in meshes]
+            self.mesh_sdf = torch.cat(grids)
+            starts = np.concatenate([[0], np.cumsum([g.numel() for g in grids])[:-1]]).astype(np.int64)
+            lower = np.stack([mm.sdf.lower for mm in meshes]).astype(_F32)
+            voxel = np.asarray([mm.sdf.voxel_size for mm in meshes], _F32)
+            cells = np.stack([mm.sdf.cells for mm in meshes]).astype(_I32)
+        else:
+            self.mesh_sdf = torch.zeros(1, dtype=torch.float32, device=self.device)
+            starts = np.zeros(max(M, 1), np.int64)
+            lower = np.zeros((max(M, 1), 3), _F32)
+            voxel = np.ones(max(M, 1), _F32)
+            cells = np.ones((max(M, 1), 3), _I32)
+        self.mesh_has_sdf = have
+        for k, v in (("mesh_sdf_start", starts), ("mesh_sdf_lower", lower), ("mesh_sdf_voxel", voxel),
+                     ("mesh_sdf_cells", cells)):
+            self.host[k] = v
+            setattr(self, k, torch.from_numpy(np.ascontiguousarray(v)).to(self.device))
 
     def _finish(self):
         """Derive the per-world tables and upload every host array to the device."""
         self._build_world_tables()
+        if "shape_mesh" not in self.host:
+            self._mesh_tables([None] * self.shape_count)
         for name, a in self.host.items():
             setattr(self, name, torch.from_numpy(np.ascontiguousarray(a)).to(self.device))
+        self._mesh_sdf_tables()
         self.shape_contact_pair_count = 0
 
     @classmethod
@@ -186,6 +247,8 @@ class Model:
         """
         th = template.host
         W = int(world_count)
+        if getattr(template, "meshes", []):
+            raise ValueError("templates with mesh shapes are replicated with the ModelBuilder")
         if template.world_count != 1:
             raise ValueError("template must contain exactly one world")
         bw, sw, jw = th["body_world"], th["shape_world"], th["joint_world"]

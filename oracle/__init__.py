@@ -40,6 +40,10 @@ class OracleModel(ctypes.Structure):
         ("joint_X_p", _f32p), ("joint_X_c", _f32p), ("joint_qd_start", _i32p), ("joint_dof_dim", _i32p),
         ("joint_axis", _f32p), ("joint_limit_lower", _f32p), ("joint_limit_upper", _f32p),
         ("joint_target_ke", _f32p), ("joint_target_kd", _f32p),
+        ("shape_mesh", _i32p), ("shape_aabb_lower", _f32p), ("shape_aabb_upper", _f32p), ("mesh_count", ctypes.c_int),
+        ("mesh_vertex", _f32p), ("mesh_vertex_start", _i32p), ("mesh_edge", _i32p), ("mesh_edge_start", _i32p),
+        ("mesh_sdf", _f32p), ("mesh_sdf_start", _i64p), ("mesh_sdf_lower", _f32p), ("mesh_sdf_voxel", _f32p),
+        ("mesh_sdf_cells", _i32p), ("mesh_contact_max", ctypes.c_int),
     ]
 
 
@@ -100,7 +104,7 @@ def _p(a: np.ndarray, t):
 class Oracle:
     """Holds a model's host arrays (kept alive) and runs the restatement on them."""
 
-    def __init__(self, model):
+    def __init__(self, model, mesh_contact_max: int = 64):
         h = model.host
         self.model = model
         keep = {}
@@ -142,9 +146,44 @@ class Oracle:
         m.joint_axis = f32("joint_axis")
         m.joint_limit_lower, m.joint_limit_upper = f32("joint_limit_lower"), f32("joint_limit_upper")
         m.joint_target_ke, m.joint_target_kd = f32("joint_target_ke"), f32("joint_target_kd")
+        m.shape_mesh, m.shape_aabb_lower, m.shape_aabb_upper = (i32("shape_mesh"), f32("shape_aabb_lower"),
+                                                                f32("shape_aabb_upper"))
+        m.mesh_count = len(getattr(model, "meshes", []))
+        m.mesh_vertex, m.mesh_vertex_start = f32("mesh_vertex"), i32("mesh_vertex_start")
+        m.mesh_edge, m.mesh_edge_start = i32("mesh_edge"), i32("mesh_edge_start")
+        sdf = np.ascontiguousarray(model.mesh_sdf.detach().cpu().numpy(), np.float32)
+        keep["mesh_sdf"] = sdf
+        m.mesh_sdf = _p(sdf, _f32p)
+        st = np.ascontiguousarray(h["mesh_sdf_start"], np.int64)
+        keep["mesh_sdf_start"] = st
+        m.mesh_sdf_start = _p(st, _i64p)
+        m.mesh_sdf_lower, m.mesh_sdf_voxel, m.mesh_sdf_cells = (f32("mesh_sdf_lower"), f32("mesh_sdf_voxel"),
+                                                                i32("mesh_sdf_cells"))
+        m.mesh_contact_max = mesh_contact_max
         self._keep = keep
         self.m = m
         self.lib = lib()
+
+    def sdf_sample(self, mesh: int, points: np.ndarray):
+        """SDF value and gradient of the model's mesh ``mesh`` at local points (n x 3)."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        v = np.zeros(len(p), np.float32)
+        g = np.zeros((len(p), 3), np.float32)
+        self.lib.oracle_sdf_sample_f32.restype = None
+        self.lib.oracle_sdf_sample_f32(ctypes.byref(self.m), ctypes.c_int(mesh), _p(p, _f32p), ctypes.c_int(len(p)),
+                                       _p(v, _f32p), _p(g, _f32p))
+        return v, g
+
+    def edge_search(self, mesh: int, v0: np.ndarray, v1: np.ndarray):
+        """Deepest SDF point of each segment v0 -> v1 (mesh-local coordinates)."""
+        a = np.ascontiguousarray(v0, np.float32).reshape(-1, 3)
+        b = np.ascontiguousarray(v1, np.float32).reshape(-1, 3)
+        v = np.zeros(len(a), np.float32)
+        pt = np.zeros((len(a), 3), np.float32)
+        self.lib.oracle_edge_search_f32.restype = None
+        self.lib.oracle_edge_search_f32(ctypes.byref(self.m), ctypes.c_int(mesh), _p(a, _f32p), _p(b, _f32p),
+                                        ctypes.c_int(len(a)), _p(v, _f32p), _p(pt, _f32p))
+        return v, pt
 
     def broadphase(self, body_q: np.ndarray, capacity: int | None = None) -> np.ndarray:
         q = np.ascontiguousarray(body_q, dtype=np.float32)
@@ -279,6 +318,54 @@ class Oracle:
             ctypes.c_float(residual_tol), _p(feat, _f32p), _p(lam, _f32p), _p(res, _f32p), _p(fb, _i32p),
             _p(qo, _f32p), _p(qdo, _f32p))
         return qo, qdo, feat[:n], lam[:n], res, fb
+
+
+def sdf_build(vertices, triangles, lower, voxel: float, cells, precision: str = "f32") -> np.ndarray:
+    """The restatement's dense SDF cook; returns the grid as (nz + 1, ny + 1, nx + 1)."""
+    L = lib()
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.int32).reshape(-1)
+    lo = np.ascontiguousarray(lower, np.float32)
+    ce = np.ascontiguousarray(cells, np.int32)
+    shape = (int(ce[2]) + 1, int(ce[1]) + 1, int(ce[0]) + 1)
+    dt, tp, fn = ((np.float32, _f32p, L.oracle_sdf_build_f32) if precision == "f32"
+                  else (np.float64, _f64p, L.oracle_sdf_build_f64))
+    out = np.zeros(shape, dt)
+    fn.restype = None
+    fn(_p(v, _f32p), ctypes.c_int(len(v)), _p(t, _i32p), ctypes.c_int(len(t) // 3), _p(lo, _f32p),
+       ctypes.c_float(float(voxel)), _p(ce, _i32p), _p(out, tp))
+    return out
+
+
+def mesh_distance_f64(vertices, triangles, points) -> np.ndarray:
+    """FP64 signed distance of points to a triangle mesh (nearest triangle, winding-number sign)."""
+    L = lib()
+    v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+    t = np.ascontiguousarray(triangles, np.int32).reshape(-1)
+    p = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    out = np.zeros(len(p), np.float64)
+    L.oracle_mesh_distance_f64.restype = None
+    L.oracle_mesh_distance_f64(_p(v, _f32p), ctypes.c_int(len(v)), _p(t, _i32p), ctypes.c_int(len(t) // 3),
+                               _p(p, _f64p), ctypes.c_int(len(p)), _p(out, _f64p))
+    return out
+
+
+def closest_on_triangle_f64(p, a, b, c) -> np.ndarray:
+    L = lib()
+    arrs = [np.ascontiguousarray(x, np.float64).reshape(-1, 3) for x in (p, a, b, c)]
+    out = np.zeros_like(arrs[0])
+    L.oracle_closest_on_triangle_f64.restype = None
+    L.oracle_closest_on_triangle_f64(*[_p(x, _f64p) for x in arrs], ctypes.c_int(len(out)), _p(out, _f64p))
+    return out
+
+
+def solid_angle_f64(a, b, c) -> np.ndarray:
+    L = lib()
+    arrs = [np.ascontiguousarray(x, np.float64).reshape(-1, 3) for x in (a, b, c)]
+    out = np.zeros(len(arrs[0]), np.float64)
+    L.oracle_solid_angle_f64.restype = None
+    L.oracle_solid_angle_f64(*[_p(x, _f64p) for x in arrs], ctypes.c_int(len(out)), _p(out, _f64p))
+    return out
 
 
 def mlp_forward(params: dict, x: np.ndarray) -> np.ndarray:

@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "oracle_collide.h"
+#include "oracle_mesh.h"
 #include "oracle_xpbd.h"
 
 using namespace oracle;
@@ -188,4 +189,79 @@ extern "C" void oracle_mlp_backward(const float* w1, const float* b1, const floa
                                     float* loss) {
   MlpParams P{w1, b1, w2, b2, w3, b3};
   mlp_backward(P, x, t, n, gw1, gb1, gw2, gb2, gw3, gb3, loss);
+}
+
+extern "C" void oracle_sdf_build_f32(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
+                                     const float lower[3], float voxel, const int cells[3], float* out) {
+  sdf_cook<float>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, out);
+}
+
+extern "C" void oracle_sdf_build_f64(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
+                                     const float lower[3], float voxel, const int cells[3], double* out) {
+  sdf_cook<double>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, out);
+}
+
+extern "C" void oracle_sdf_sample_f32(const OracleModel* m, int mesh, const float* p, int n, float* value,
+                                      float* grad) {
+  SdfG<float> g = sdf_grid<float>(m, mesh);
+  for (int i = 0; i < n; ++i) {
+    V3<float> x{p[3 * i], p[3 * i + 1], p[3 * i + 2]};
+    V3<float> gr;
+    value[i] = sdf_sample_grad(g, x, gr);
+    grad[3 * i] = gr.x;
+    grad[3 * i + 1] = gr.y;
+    grad[3 * i + 2] = gr.z;
+  }
+}
+
+extern "C" void oracle_mesh_distance_f64(const float* vertices, int vertex_count, const int* triangles,
+                                         int triangle_count, const double* points, int n, double* out) {
+  const double pi = 3.14159265358979323846;
+  for (int i = 0; i < n; ++i) {
+    V3<double> p{points[3 * i], points[3 * i + 1], points[3 * i + 2]};
+    double best = 1e300, omega = 0.0;
+    for (int t = 0; t < triangle_count; ++t) {
+      int i0 = triangles[3 * t], i1 = triangles[3 * t + 1], i2 = triangles[3 * t + 2];
+      V3<double> a{vertices[3 * i0], vertices[3 * i0 + 1], vertices[3 * i0 + 2]};
+      V3<double> b{vertices[3 * i1], vertices[3 * i1 + 1], vertices[3 * i1 + 2]};
+      V3<double> c{vertices[3 * i2], vertices[3 * i2 + 1], vertices[3 * i2 + 2]};
+      double dd = length(p - closest_on_triangle(p, a, b, c));
+      best = dd < best ? dd : best;
+      omega += solid_angle(a - p, b - p, c - p);
+    }
+    out[i] = omega / (4.0 * pi) > 0.5 ? -best : best;
+  }
+}
+
+extern "C" void oracle_closest_on_triangle_f64(const double* p, const double* a, const double* b, const double* c,
+                                               int n, double* out) {
+  for (int i = 0; i < n; ++i) {
+    V3<double> q = closest_on_triangle(V3<double>{p[3 * i], p[3 * i + 1], p[3 * i + 2]},
+                                       V3<double>{a[3 * i], a[3 * i + 1], a[3 * i + 2]},
+                                       V3<double>{b[3 * i], b[3 * i + 1], b[3 * i + 2]},
+                                       V3<double>{c[3 * i], c[3 * i + 1], c[3 * i + 2]});
+    out[3 * i] = q.x;
+    out[3 * i + 1] = q.y;
+    out[3 * i + 2] = q.z;
+  }
+}
+
+extern "C" void oracle_solid_angle_f64(const double* a, const double* b, const double* c, int n, double* out) {
+  for (int i = 0; i < n; ++i)
+    out[i] = solid_angle(V3<double>{a[3 * i], a[3 * i + 1], a[3 * i + 2]},
+                         V3<double>{b[3 * i], b[3 * i + 1], b[3 * i + 2]},
+                         V3<double>{c[3 * i], c[3 * i + 1], c[3 * i + 2]});
+}
+
+extern "C" void oracle_edge_search_f32(const OracleModel* m, int mesh, const float* v0, const float* v1, int n,
+                                       float* value, float* point) {
+  SdfG<float> g = sdf_grid<float>(m, mesh);
+  for (int i = 0; i < n; ++i) {
+    V3<float> a{v0[3 * i], v0[3 * i + 1], v0[3 * i + 2]}, b{v1[3 * i], v1[3 * i + 1], v1[3 * i + 2]};
+    V3<float> mid = (a + b) * 0.5f, pt;
+    value[i] = edge_sdf_search(g, a, b, sdf_sample(g, mid), pt);
+    point[3 * i] = pt.x;
+    point[3 * i + 1] = pt.y;
+    point[3 * i + 2] = pt.z;
+  }
 }

@@ -60,6 +60,22 @@ typedef struct OracleModel {
   const float* joint_limit_upper;
   const float* joint_target_ke;
   const float* joint_target_kd;
+  /* triangle meshes: per-shape mesh index (-1 otherwise) and local AABB with the scale baked in;
+   * concatenated vertices / unique edges; one dense SDF grid per mesh ((cells+1)^3 vertices, x fastest) */
+  const int* shape_mesh;
+  const float* shape_aabb_lower; /* S x 3 */
+  const float* shape_aabb_upper; /* S x 3 */
+  int mesh_count;
+  const float* mesh_vertex;
+  const int* mesh_vertex_start; /* M + 1 */
+  const int* mesh_edge;         /* E x 2 */
+  const int* mesh_edge_start;   /* M + 1 */
+  const float* mesh_sdf;
+  const int64_t* mesh_sdf_start;
+  const float* mesh_sdf_lower; /* M x 3 */
+  const float* mesh_sdf_voxel;
+  const int* mesh_sdf_cells; /* M x 3 */
+  int mesh_contact_max;
 } OracleModel;
 
 typedef struct OracleXpbdParams {
@@ -101,6 +117,27 @@ void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const
                           const double* offset0, const double* offset1, const double* normal,
                           const double* margin0, const double* margin1, double dt, double* body_q_out,
                           double* body_qd_out);
+
+/* Dense signed distance grid of a triangle mesh (distance to the nearest triangle, negative where the
+ * winding number exceeds 0.5); out holds (cells[2]+1)(cells[1]+1)(cells[0]+1) values, x fastest. */
+void oracle_sdf_build_f32(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
+                          const float lower[3], float voxel, const int cells[3], float* out);
+void oracle_sdf_build_f64(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
+                          const float lower[3], float voxel, const int cells[3], double* out);
+
+/* SDF value and gradient of mesh `mesh` of the model at local points p (n x 3). */
+void oracle_sdf_sample_f32(const OracleModel* m, int mesh, const float* p, int n, float* value, float* grad);
+
+/* FP64 signed distance of points to a triangle mesh (nearest triangle, winding-number sign). */
+void oracle_mesh_distance_f64(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
+                              const double* points, int n, double* out);
+/* Closest point on triangles (a, b, c) to points p and solid angles of triangles seen from the origin. */
+void oracle_closest_on_triangle_f64(const double* p, const double* a, const double* b, const double* c, int n,
+                                    double* out);
+void oracle_solid_angle_f64(const double* a, const double* b, const double* c, int n, double* out);
+/* Deepest point of segments v0 -> v1 in the SDF of mesh `mesh` (Brent search from the midpoint). */
+void oracle_edge_search_f32(const OracleModel* m, int mesh, const float* v0, const float* v1, int n, float* value,
+                            float* point);
 
 // Elementwise FP32 asin / acos of the restatement (the revolute joint's twist and swing angles).
 void oracle_asin_f32(const float* x, int n, float* out);

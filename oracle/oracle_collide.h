@@ -27,7 +27,7 @@
 
 namespace oracle {
 
-enum { PLANE = 1, SPHERE = 3, CAPSULE = 4, BOX = 7 };
+enum { PLANE = 1, SPHERE = 3, CAPSULE = 4, BOX = 7, MESH = 8 };
 enum { COLLIDE_SHAPES = 2 };
 static const double MAXVAL = 1.0e10;
 
@@ -137,6 +137,20 @@ ShapeWorld<T> compute_shape_world(const OracleModel* m, const S* body_q, int s) 
     V3<T> he{rad, rad, rad};
     r.lo = pos - he - mvec;
     r.hi = pos + he + mvec;
+  } else if (type == MESH) {
+    // pre-computed local AABB rotated into the world frame (collide.py:215-245)
+    const float* l = m->shape_aabb_lower + 3 * s;
+    const float* u = m->shape_aabb_upper + 3 * s;
+    V3<T> llo{T(l[0]), T(l[1]), T(l[2])}, lhi{T(u[0]), T(u[1]), T(u[2])};
+    V3<T> c = (llo + lhi) * T(0.5), h = (lhi - llo) * T(0.5);
+    V3<T> wc = qrot(q, c) + pos;
+    V3<T> r0 = qrot(q, V3<T>{T(1), T(0), T(0)}), r1 = qrot(q, V3<T>{T(0), T(1), T(0)});
+    V3<T> r2 = qrot(q, V3<T>{T(0), T(0), T(1)});
+    V3<T> wh{tabs(r0.x) * h.x + tabs(r1.x) * h.y + tabs(r2.x) * h.z,
+             tabs(r0.y) * h.x + tabs(r1.y) * h.y + tabs(r2.y) * h.z,
+             tabs(r0.z) * h.x + tabs(r1.z) * h.y + tabs(r2.z) * h.z};
+    r.lo = wc - wh - mvec;
+    r.hi = wc + wh + mvec;
   } else {
     Shape<T> g;
     g.type = type;
@@ -1146,11 +1160,20 @@ template <class T> void solve_convex(const PairCtx<T>& P, T threshold, bool skip
 }
 
 // ---------------------------------------------------------------- narrow phase per pair
+template <class T>
+void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int sa, int sb, Writer<T>& w);
+
 template <class T> void narrow_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int sa, int sb,
                                     Writer<T>& w) {
   if (sa == sb || sa < 0 || sb < 0) return;
   int ta = m->shape_type[sa], tb = m->shape_type[sb];
   if (ta > tb) { std::swap(sa, sb); std::swap(ta, tb); }
+  // mesh pairs (narrow_phase.py:290-320): mesh-mesh, and mesh-infinite plane kept as (mesh, plane)
+  if (ta == MESH || tb == MESH) {
+    if (ta == PLANE) std::swap(sa, sb);
+    mesh_pair(m, sw, sa, sb, w);
+    return;
+  }
   const ShapeWorld<T>& A = sw[sa];
   const ShapeWorld<T>& B = sw[sb];
   V3<T> scale_a = A.gscale, scale_b = B.gscale;

@@ -1,0 +1,352 @@
+// CPU oracle -- TEST INFRASTRUCTURE ONLY.  Triangle-mesh contact restatement.
+//
+// Follows:
+//   texture_sample_sdf / _grad (dense grid)   newton/_src/geometry/sdf_texture.py:776-955
+//   scale_sdf_result_to_world                 newton/_src/geometry/sdf_contact.py:51-82
+//   get_edge_bounding_sphere, edge culling    newton/_src/geometry/sdf_contact.py:447-460, 1010-1045
+//   do_edge_sdf_collision (Brent + endpoints) newton/_src/geometry/sdf_contact.py:526-681
+//   mesh-mesh contact per edge                newton/_src/geometry/sdf_contact.py:870-1150
+//   mesh-plane contact per vertex             newton/_src/geometry/narrow_phase.py:1121-1190
+//   SDF cook (distance to the nearest triangle, winding-number sign)
+//                                             newton/_src/geometry/sdf_texture.py:959-1370 (dense grid)
+// The reference keeps a mesh pair's contacts through a normal-bin reduction in a hash table
+// (contact_reduction_global.py:671-1580); this restatement keeps the K deepest (distance, then sub key)
+// in sub-key order, the rule the product implements.
+#pragma once
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "oracle_collide.h"
+
+namespace oracle {
+
+template <class T> struct SdfG {
+  const float* v;
+  V3<T> lower, upper;
+  T inv_dx;
+  int cx, cy, cz;
+};
+
+template <class T> SdfG<T> sdf_grid(const OracleModel* m, int mesh) {
+  SdfG<T> g;
+  g.v = m->mesh_sdf + m->mesh_sdf_start[mesh];
+  g.lower = V3<T>{T(m->mesh_sdf_lower[3 * mesh]), T(m->mesh_sdf_lower[3 * mesh + 1]), T(m->mesh_sdf_lower[3 * mesh + 2])};
+  T vx = T(m->mesh_sdf_voxel[mesh]);
+  g.cx = m->mesh_sdf_cells[3 * mesh];
+  g.cy = m->mesh_sdf_cells[3 * mesh + 1];
+  g.cz = m->mesh_sdf_cells[3 * mesh + 2];
+  g.upper = V3<T>{g.lower.x + vx * T(g.cx), g.lower.y + vx * T(g.cy), g.lower.z + vx * T(g.cz)};
+  g.inv_dx = T(1) / vx;
+  return g;
+}
+
+template <class T> struct SdfCellO {
+  T c[8];
+  T tx, ty, tz;
+  T diff_mag;
+  V3<T> diff;
+};
+
+template <class T> SdfCellO<T> sdf_cell(const SdfG<T>& g, V3<T> p) {
+  SdfCellO<T> s;
+  V3<T> cl{tclamp(p.x, g.lower.x, g.upper.x), tclamp(p.y, g.lower.y, g.upper.y), tclamp(p.z, g.lower.z, g.upper.z)};
+  s.diff = p - cl;
+  s.diff_mag = length(s.diff);
+  V3<T> f = (cl - g.lower) * g.inv_dx;
+  T fx = tclamp(f.x, T(0), T(g.cx));
+  T fy = tclamp(f.y, T(0), T(g.cy));
+  T fz = tclamp(f.z, T(0), T(g.cz));
+  int ix = std::max(0, std::min(int(std::floor(fx)), g.cx - 1));
+  int iy = std::max(0, std::min(int(std::floor(fy)), g.cy - 1));
+  int iz = std::max(0, std::min(int(std::floor(fz)), g.cz - 1));
+  s.tx = fx - T(ix);
+  s.ty = fy - T(iy);
+  s.tz = fz - T(iz);
+  const long long nx = g.cx + 1, ny = g.cy + 1;
+  const float* b = g.v + ((long long)iz * ny + iy) * nx + ix;
+  const long long sy = nx, sz = nx * ny;
+  const long long o[8] = {0, 1, sy, sy + 1, sz, sz + 1, sz + sy, sz + sy + 1};
+  for (int k = 0; k < 8; ++k) s.c[k] = T(b[o[k]]);
+  return s;
+}
+
+template <class T> T sdf_trilinear(const SdfCellO<T>& s) {
+  T c00 = s.c[0] + (s.c[1] - s.c[0]) * s.tx;
+  T c10 = s.c[2] + (s.c[3] - s.c[2]) * s.tx;
+  T c01 = s.c[4] + (s.c[5] - s.c[4]) * s.tx;
+  T c11 = s.c[6] + (s.c[7] - s.c[6]) * s.tx;
+  T c0 = c00 + (c10 - c00) * s.ty;
+  T c1 = c01 + (c11 - c01) * s.ty;
+  return c0 + (c1 - c0) * s.tz;
+}
+
+template <class T> T sdf_sample(const SdfG<T>& g, V3<T> p) {
+  SdfCellO<T> s = sdf_cell(g, p);
+  return sdf_trilinear(s) + s.diff_mag;
+}
+
+template <class T> T sdf_sample_grad(const SdfG<T>& g, V3<T> p, V3<T>& grad) {
+  SdfCellO<T> s = sdf_cell(g, p);
+  T val = sdf_trilinear(s);
+  T omtx = T(1) - s.tx, omty = T(1) - s.ty, omtz = T(1) - s.tz;
+  const T* c = s.c;
+  T gx = omty * omtz * (c[1] - c[0]) + s.ty * omtz * (c[3] - c[2]) + omty * s.tz * (c[5] - c[4]) +
+         s.ty * s.tz * (c[7] - c[6]);
+  T gy = omtx * omtz * (c[2] - c[0]) + s.tx * omtz * (c[3] - c[1]) + omtx * s.tz * (c[6] - c[4]) +
+         s.tx * s.tz * (c[7] - c[5]);
+  T gz = omtx * omty * (c[4] - c[0]) + s.tx * omty * (c[5] - c[1]) + omtx * s.ty * (c[6] - c[2]) +
+         s.tx * s.ty * (c[7] - c[3]);
+  grad = V3<T>{gx * g.inv_dx, gy * g.inv_dx, gz * g.inv_dx};
+  if (s.diff_mag > T(0)) {
+    val = val + s.diff_mag;
+    grad = s.diff / s.diff_mag;
+  }
+  return val;
+}
+
+// do_edge_sdf_collision (sdf_contact.py:526-681): Brent on t in [0, 1] from the midpoint, 5 iterations,
+// then the nearer endpoint when the minimum lies in the outer fifths.
+template <class T> T edge_sdf_search(const SdfG<T>& g, V3<T> v0, V3<T> v1, T midpoint_sdf, V3<T>& point) {
+  const T golden = T(0.3819660112501051);
+  V3<T> e = v1 - v0;
+  T a = T(0), b = T(1), x = T(0.5), w = T(0.5), vb = T(0.5);
+  T fx = midpoint_sdf, fw = fx, fv = fx;
+  T dstep = T(0), estep = T(0);
+  for (int it = 0; it < 5; ++it) {
+    T mid = T(0.5) * (a + b);
+    T tol = T(1.0e-2) * tabs(x) + T(1.0e-8);
+    T tol2 = T(2) * tol;
+    if (tabs(x - mid) <= tol2 - T(0.5) * (b - a)) break;
+    bool parabolic = false;
+    T pn = T(0), qd = T(0);
+    if (tabs(estep) > tol) {
+      T r = (x - w) * (fx - fv);
+      qd = (x - vb) * (fx - fw);
+      pn = (x - vb) * qd - (x - w) * r;
+      qd = T(2) * (qd - r);
+      if (qd > T(0)) pn = -pn; else qd = -qd;
+      if (tabs(pn) < T(0.5) * tabs(qd * estep)) {
+        T trial = pn / qd;
+        T ut = x + trial;
+        if (ut - a >= tol2 && b - ut >= tol2) parabolic = true;
+      }
+    }
+    if (parabolic) {
+      estep = dstep;
+      dstep = pn / qd;
+    } else {
+      estep = x >= mid ? a - x : b - x;
+      dstep = golden * estep;
+    }
+    T u;
+    if (tabs(dstep) >= tol) u = x + dstep;
+    else u = dstep > T(0) ? x + tol : x - tol;
+    T fu = sdf_sample(g, v0 + e * u);
+    if (fu <= fx) {
+      if (u < x) b = x; else a = x;
+      vb = w; fv = fw;
+      w = x; fw = fx;
+      x = u; fx = fu;
+    } else {
+      if (u < x) a = u; else b = u;
+      if (fu <= fw || w == x) {
+        vb = w; fv = fw;
+        w = u; fw = fu;
+      } else if (fu <= fv || vb == x || vb == w) {
+        vb = u; fv = fu;
+      }
+    }
+  }
+  T bt = x, bf = fx;
+  if (x < T(0.2) || x > T(0.8)) {
+    T ct = x < T(0.5) ? T(0) : T(1);
+    T fe = sdf_sample(g, v0 + e * ct);
+    if (fe < bf) { bt = ct; bf = fe; }
+  }
+  point = v0 + e * bt;
+  return bf;
+}
+
+// ---------------------------------------------------------------- SDF cook
+// Closest point on triangle abc (Voronoi regions of the barycentric coordinates, Ericson 2005, 5.1.5).
+template <class T> V3<T> closest_on_triangle(V3<T> p, V3<T> a, V3<T> b, V3<T> c) {
+  V3<T> ab = b - a, ac = c - a, ap = p - a;
+  T d1 = dot(ab, ap), d2 = dot(ac, ap);
+  if (d1 <= T(0) && d2 <= T(0)) return a;
+  V3<T> bp = p - b;
+  T d3 = dot(ab, bp), d4 = dot(ac, bp);
+  if (d3 >= T(0) && d4 <= d3) return b;
+  T vc = d1 * d4 - d3 * d2;
+  if (vc <= T(0) && d1 >= T(0) && d3 <= T(0)) {
+    T v = d1 / (d1 - d3);
+    return a + ab * v;
+  }
+  V3<T> cp = p - c;
+  T d5 = dot(ab, cp), d6 = dot(ac, cp);
+  if (d6 >= T(0) && d5 <= d6) return c;
+  T vb = d5 * d2 - d1 * d6;
+  if (vb <= T(0) && d2 >= T(0) && d6 <= T(0)) {
+    T w = d2 / (d2 - d6);
+    return a + ac * w;
+  }
+  T va = d3 * d6 - d5 * d4;
+  if (va <= T(0) && (d4 - d3) >= T(0) && (d5 - d6) >= T(0)) {
+    T w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+    return b + (c - b) * w;
+  }
+  T denom = T(1) / (va + vb + vc);
+  T v = vb * denom, w = vc * denom;
+  return a + ab * v + ac * w;
+}
+
+// Solid angle of triangle abc seen from the origin (Van Oosterom & Strackee 1983).
+template <class T> T solid_angle(V3<T> a, V3<T> b, V3<T> c) {
+  T la = length(a), lb = length(b), lc = length(c);
+  T num = dot(a, cross(b, c));
+  T den = la * lb * lc + dot(a, b) * lc + dot(a, c) * lb + dot(b, c) * la;
+  return T(2) * std::atan2(num, den);
+}
+
+template <class T>
+void sdf_cook(const float* vtx, int nv, const int* tri, int nt, const float lower[3], float voxel, const int cells[3],
+              T* out) {
+  const long long nx = cells[0] + 1, ny = cells[1] + 1, nz = cells[2] + 1;
+  const T pi = T(3.14159265358979323846);
+  for (long long i = 0; i < nx * ny * nz; ++i) {
+    int ix = int(i % nx), iy = int((i / nx) % ny), iz = int(i / (nx * ny));
+    V3<T> p{T(lower[0]) + T(voxel) * T(ix), T(lower[1]) + T(voxel) * T(iy), T(lower[2]) + T(voxel) * T(iz)};
+    T best = T(3.0e38f), omega = T(0);
+    for (int t = 0; t < nt; ++t) {
+      int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
+      if (i0 < 0 || i0 >= nv || i1 < 0 || i1 >= nv || i2 < 0 || i2 >= nv) continue;
+      V3<T> a{T(vtx[3 * i0]), T(vtx[3 * i0 + 1]), T(vtx[3 * i0 + 2])};
+      V3<T> b{T(vtx[3 * i1]), T(vtx[3 * i1 + 1]), T(vtx[3 * i1 + 2])};
+      V3<T> c{T(vtx[3 * i2]), T(vtx[3 * i2 + 1]), T(vtx[3 * i2 + 2])};
+      V3<T> q = closest_on_triangle(p, a, b, c);
+      T dd = length(p - q);
+      best = dd < best ? dd : best;
+      omega += solid_angle(a - p, b - p, c - p);
+    }
+    T wn = omega / (T(4) * pi);
+    out[i] = wn > T(0.5) ? -best : best;
+  }
+}
+
+// ---------------------------------------------------------------- mesh pairs
+template <class T> struct MeshCandO {
+  V3<T> center, normal;
+  T dist;
+  int sub;
+};
+
+// One mesh pair (sa = mesh, sb = mesh or infinite plane): candidates -> K deepest -> sub-key order.
+template <class T>
+void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int sa, int sb, Writer<T>& w) {
+  const ShapeWorld<T>& A = sw[sa];
+  const ShapeWorld<T>& B = sw[sb];
+  const T gap_sum = T(m->shape_gap[sa]) + T(m->shape_gap[sb]);
+  ContactData<T> c;
+  c.r_a = T(0);
+  c.r_b = T(0);
+  c.m_a = A.margin;
+  c.m_b = B.margin;
+  c.shape_a = sa;
+  c.shape_b = sb;
+  c.gap_sum = gap_sum;
+  c.sub_key = 0;
+  std::vector<MeshCandO<T>> cand;
+  auto keep = [&](V3<T> center, V3<T> normal, T dist, int sub) {
+    c.center = center;
+    c.normal = normal;
+    c.dist = dist;
+    if (passes_gap(c)) cand.push_back(MeshCandO<T>{center, normal, dist, sub});
+  };
+  if (m->shape_type[sb] == PLANE) {
+    const int mesh = m->shape_mesh[sa];
+    const int v0 = m->mesh_vertex_start[mesh], nv = m->mesh_vertex_start[mesh + 1] - v0;
+    const Xf<T> Xp_inv = xf_inverse(B.X);
+    const V3<T> pn = xf_vector(B.X, V3<T>{T(0), T(0), T(1)});
+    const T total_margin = A.margin + B.margin;
+    for (int v = 0; v < nv; ++v) {
+      const float* pv = m->mesh_vertex + 3 * (v0 + v);
+      V3<T> vl{T(pv[0]) * A.gscale.x, T(pv[1]) * A.gscale.y, T(pv[2]) * A.gscale.z};
+      V3<T> vw = xf_point(A.X, vl);
+      V3<T> vp = xf_point(Xp_inv, vw);
+      V3<T> pp = xf_point(B.X, V3<T>{vp.x, vp.y, T(0)});
+      T dist = dot(vw - pp, pn);
+      if (dist < gap_sum + total_margin) keep((vw + pp) * T(0.5), -pn, dist, v);
+    }
+  } else {
+    for (int mode = 0; mode < 2; ++mode) {
+      const ShapeWorld<T>& Tr = mode == 0 ? A : B;
+      const ShapeWorld<T>& S = mode == 0 ? B : A;
+      const int tri_mesh = m->shape_mesh[mode == 0 ? sa : sb];
+      const int sdf_mesh = m->shape_mesh[mode == 0 ? sb : sa];
+      const SdfG<T> G = sdf_grid<T>(m, sdf_mesh);
+      const V3<T> ss{tmax(S.gscale.x, T(1e-10f)), tmax(S.gscale.y, T(1e-10f)), tmax(S.gscale.z, T(1e-10f))};
+      const V3<T> inv_s{T(1) / ss.x, T(1) / ss.y, T(1) / ss.z};
+      const T min_s = tmin(tmin(ss.x, ss.y), ss.z);
+      const Xf<T> X_ts = xf_mul(xf_inverse(S.X), Tr.X);
+      const T threshold = gap_sum + Tr.margin + S.margin;
+      const T threshold_u = threshold / min_s;
+      const int e0 = m->mesh_edge_start[tri_mesh], ne = m->mesh_edge_start[tri_mesh + 1] - e0;
+      const int vb = m->mesh_vertex_start[tri_mesh];
+      for (int e = 0; e < ne; ++e) {
+        const int ia = m->mesh_edge[2 * (e0 + e)], ib = m->mesh_edge[2 * (e0 + e) + 1];
+        const float* qa = m->mesh_vertex + 3 * (vb + ia);
+        const float* qb = m->mesh_vertex + 3 * (vb + ib);
+        V3<T> pa{T(qa[0]) * Tr.gscale.x, T(qa[1]) * Tr.gscale.y, T(qa[2]) * Tr.gscale.z};
+        V3<T> pb{T(qb[0]) * Tr.gscale.x, T(qb[1]) * Tr.gscale.y, T(qb[2]) * Tr.gscale.z};
+        V3<T> ea = xf_point(X_ts, pa), eb = xf_point(X_ts, pb);
+        ea = V3<T>{ea.x * inv_s.x, ea.y * inv_s.y, ea.z * inv_s.z};
+        eb = V3<T>{eb.x * inv_s.x, eb.y * inv_s.y, eb.z * inv_s.z};
+        V3<T> mid = (ea + eb) * T(0.5);
+        T rad = length(eb - ea) * T(0.5);
+        T cull = rad + threshold_u;
+        V3<T> cl{tclamp(mid.x, G.lower.x, G.upper.x), tclamp(mid.y, G.lower.y, G.upper.y),
+                 tclamp(mid.z, G.lower.z, G.upper.z)};
+        if (length_sq(mid - cl) > cull * cull) continue;
+        T msdf = sdf_sample(G, mid);
+        if (!(msdf <= cull)) continue;
+        V3<T> pu;
+        T du0 = edge_sdf_search(G, ea, eb, msdf, pu);
+        if (!(du0 * min_s < threshold)) continue;
+        V3<T> gu;
+        T du = sdf_sample_grad(G, pu, gu);
+        T dist = du * min_s;
+        V3<T> gsc{gu.x * inv_s.x, gu.y * inv_s.y, gu.z * inv_s.z};
+        T gl = length(gsc);
+        V3<T> dir = gl > T(0) ? gsc / gl : gu;
+        V3<T> pw = xf_point(S.X, V3<T>{pu.x * ss.x, pu.y * ss.y, pu.z * ss.z});
+        V3<T> dw = xf_vector(S.X, dir);
+        T dl = length(dw);
+        if (dl > T(0)) {
+          dw = dw / dl;
+        } else {
+          V3<T> fb = pw - S.X.p;
+          T fl = length(fb);
+          dw = fl > T(0) ? fb / fl : V3<T>{T(0), T(1), T(0)};
+        }
+        keep(pw, mode == 0 ? -dw : dw, dist, (e << 2) | (mode << 1));
+      }
+    }
+  }
+  const int K = m->mesh_contact_max;
+  std::stable_sort(cand.begin(), cand.end(), [](const MeshCandO<T>& x, const MeshCandO<T>& y) {
+    if (x.dist < y.dist) return true;
+    if (y.dist < x.dist) return false;
+    return x.sub < y.sub;
+  });
+  if ((int)cand.size() > K) cand.resize(K);
+  std::sort(cand.begin(), cand.end(), [](const MeshCandO<T>& x, const MeshCandO<T>& y) { return x.sub < y.sub; });
+  for (const MeshCandO<T>& k : cand) {
+    c.center = k.center;
+    c.normal = k.normal;
+    c.dist = k.dist;
+    c.sub_key = k.sub;
+    write_contact(c, w, false);
+  }
+}
+
+}  // namespace oracle
