@@ -16,6 +16,8 @@ bounded sample: a snapshot of the first worlds of the timed state, stepped on th
 ``learned`` times the same workload with the learned warm start (SolverXPBDWarmStart: prep launch,
 network, warm launch with ``--warm-iterations`` iterations and the per-world residual check) and
 reports its penetration residual beside the plain solver's at the same state.
+``sdf`` times the triangle-mesh workload (1024 worlds x 8 convex-hull meshes with 64^3 SDFs, mesh-mesh and
+mesh-plane contacts) with the same step.
 """
 
 from __future__ import annotations
@@ -48,6 +50,10 @@ def parse():
     p.add_argument("--warm-iterations", type=int, default=3)
     p.add_argument("--residual-tol", type=float, default=5e-3,
                    help="per-world penetration residual above which the learned form falls back to a cold solve")
+    p.add_argument("--no-sdf", action="store_true", help="skip the SDF (triangle-mesh) workload")
+    p.add_argument("--sdf-worlds", type=int, default=1024)
+    p.add_argument("--sdf-steps", type=int, default=60)
+    p.add_argument("--sdf-warmup", type=int, default=60)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="only run a device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return p.parse_args()
@@ -190,6 +196,10 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(model, states[0], args, sdt)
 
+    sdf = None
+    if not args.no_sdf:
+        sdf = sdf_form(args, dev)
+
     line = {
         "metric": "world-steps/s @4096 worlds; iters to 1e-4 m pen residual, learned vs plain",
         "value": round(ws_per_s, 1),
@@ -228,6 +238,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "learned": learned,
+        "sdf": sdf,
     }
     print(json.dumps(line), flush=True)
 
@@ -330,6 +341,108 @@ def learned_form(args, model, dev, plain_ws_per_s):
         "end_state_residual_m": {"learned": warm_here, f"plain_{args.warm_iterations}": plain_k,
                                  f"plain_{args.iterations}": plain10},
         "network": "16-128-128-1 SiLU, BF16 MFMA / FP32 accumulate, newton_custom_contact_amd/data/warmstart_mlp.npz",
+    }
+
+
+def sdf_form(args, dev):
+    """The triangle-mesh workload: ``--sdf-worlds`` worlds x 8 convex-hull meshes (64-256 points on a sphere
+    of radius 0.1-0.3 m, from a seeded library of 64 hulls, 64^3 FP32 SDF each) dropped on a ground plane,
+    mesh-mesh and mesh-plane contacts (at most 64 per pair), 4 substeps, 10 iterations, hipGraph step.
+    Timed after ``--sdf-warmup`` steps, when the hulls have landed and pile up."""
+    import torch
+
+    from newton_custom_contact_amd import CollisionPipeline, scenes
+    from newton_custom_contact_amd.autotune import tune_block_lanes
+    from newton_custom_contact_amd.solvers import SolverXPBD
+
+    t0 = time.time()
+    lib = scenes.mesh_library(64, seed=args.seed)
+    for mm in lib:
+        mm.build_sdf(max_resolution=64, device=dev)
+    torch.cuda.synchronize()
+    t_cook = time.time() - t0
+    model = scenes.mesh_worlds(args.sdf_worlds, seed=args.seed, device=dev, library=lib)
+    print(f"[bench] sdf scene: {args.sdf_worlds} worlds, {model.body_count} mesh bodies, 64 hulls "
+          f"(library + cook {t_cook:.1f}s, total {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    pipe = CollisionPipeline(model, broad_phase="sap")
+    solver = SolverXPBD(model, iterations=args.iterations)
+    contacts = pipe.contacts()
+    states = [model.state(), model.state()]
+    sdt = (1.0 / 60.0) / args.substeps
+    nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step():
+        for s in range(args.substeps):
+            a, b = states[s % 2], states[(s + 1) % 2]
+            pipe.collide(a, contacts)
+            solver.step(a, b, None, contacts, sdt)
+            nc_acc.add_(contacts.rigid_contact_count)
+        if args.substeps % 2:
+            states[0].body_q.copy_(states[1].body_q)
+            states[0].body_qd.copy_(states[1].body_qd)
+
+    # land the hulls (eager), pick the step kernel's launch shape on that state, then capture the step
+    for _ in range(args.sdf_warmup):
+        step()
+    pipe.collide(states[0], contacts)
+    tune = tune_block_lanes(solver, states[0], states[1], contacts, sdt)
+    print(f"[bench] sdf: step kernel launch shapes {tune}", file=sys.stderr, flush=True)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = None
+    if not args.no_graph:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step()
+        torch.cuda.synchronize()
+    for _ in range(5):
+        graph.replay() if graph is not None else step()
+    torch.cuda.synchronize()
+    nc_acc.zero_()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.sdf_steps):
+        graph.replay() if graph is not None else step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ws = args.sdf_worlds * args.sdf_steps / (ms / 1e3)
+    n_c = float(nc_acc.item()) / (args.sdf_steps * args.substeps * args.sdf_worlds)
+    stream = torch.cuda.current_stream()
+    reps = args.kernel_reps
+    a, b = states[0], states[1]
+    pipe.collide(a, contacts)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        pipe.collide(a, contacts)
+    ev[1].record(stream)
+    ev[2].record(stream)
+    for _ in range(reps):
+        solver.step(a, b, None, contacts, sdt)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    oc = pipe.overflow_counters()
+    print(f"[bench] sdf: {ws:.1f} world-steps/s, N_c={n_c:.1f}", file=sys.stderr, flush=True)
+    return {
+        "value": round(ws, 1),
+        "unit": "world-steps/s",
+        "ms_per_step": round(ms / args.sdf_steps, 4),
+        "workload": f"{args.sdf_worlds} worlds x 8 convex-hull meshes (64-hull library, 64^3 fp32 SDF) on a ground "
+                    f"plane, <=64 contacts per pair, dt=1/60 s, {args.substeps} substeps, {args.iterations} "
+                    f"iterations, timed after {args.sdf_warmup} steps",
+        "mean_contacts_per_world_substep": round(n_c, 2),
+        "collide_ms_per_substep": round(ev[0].elapsed_time(ev[1]) / reps, 4),
+        "xpbd_ms_per_substep": round(ev[2].elapsed_time(ev[3]) / reps, 4),
+        "overflow": oc,
+        "step_kernel_lanes": tune,
+        "lowest_body_z_m": round(float(states[0].body_q[:, 2].min()), 4),
     }
 
 

@@ -141,6 +141,8 @@ typedef struct NmcXpbdParams {
   float joint_linear_compliance;
   float joint_angular_compliance;
   int has_contacts; /* 0: step(..., contacts=None, ...) */
+  int block_lanes;  /* lanes per world: 0 = automatic (64 up to 64 bodies, else 256), 64 or 256; the
+                     * result does not depend on it (an autotuner picks the faster one per model) */
 } NmcXpbdParams;
 
 /* Learned warm start of the XPBD contact solve (no counterpart in the reference, whose contact solve

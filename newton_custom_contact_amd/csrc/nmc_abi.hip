@@ -39,6 +39,10 @@ static hipError_t dmalloc(T** p, size_t count) {
   return e;
 }
 
+static bool valid_lanes(const NmcModelImpl* m, const NmcXpbdParams* p) {
+  return p->block_lanes == 0 || p->block_lanes == 256 || (p->block_lanes == 64 && m->d.max_world_bodies <= 64);
+}
+
 static int next_pow2(int n) {
   int p = 1;
   while (p < n) p <<= 1;
@@ -78,8 +82,35 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   m->pair_cap = next_pow2(pair_cap);
   m->collide_lds = collide_lds_bytes(d.max_segment_shapes > 0 ? d.max_segment_shapes : 1, m->pair_cap);
   // solver ordering buffer: 2 entries per contact; worlds with up to 512 contacts order them in LDS,
-  // denser worlds in a global buffer of 2 entries per contact slot
+  // denser worlds in a global buffer of 2 entries per contact slot.  Models with meshes (up to
+  // mesh_contact_max contacts per pair, few bodies per world) size the LDS buffer for their worst world,
+  // up to 8192 entries, so that each body's contact list is walked out of LDS.
   m->xpbd_contact_cap = 1024;
+  if (d.mesh_count > 0 && segs > 1) {
+    std::vector<int> starts(segs + 1), seg_shape, stype(d.shape_count > 0 ? d.shape_count : 1);
+    if ((e = hipMemcpy(starts.data(), d.segment_start, sizeof(int) * (segs + 1), hipMemcpyDeviceToHost)) != hipSuccess) {
+      delete m;
+      return hip_fail(e, "nmc_model_create: segment_start");
+    }
+    seg_shape.resize(starts[segs] > 0 ? starts[segs] : 1);
+    if ((starts[segs] > 0 &&
+         (e = hipMemcpy(seg_shape.data(), d.segment_shape, sizeof(int) * starts[segs], hipMemcpyDeviceToHost)) != hipSuccess) ||
+        (d.shape_count > 0 &&
+         (e = hipMemcpy(stype.data(), d.shape_type, sizeof(int) * d.shape_count, hipMemcpyDeviceToHost)) != hipSuccess)) {
+      delete m;
+      return hip_fail(e, "nmc_model_create: shape tables");
+    }
+    long long worst_world = 0;
+    for (int sg = 0; sg + 1 < segs; ++sg) {
+      long long n = starts[sg + 1] - starts[sg], mm = 0;
+      for (int k = starts[sg]; k < starts[sg + 1]; ++k) mm += stype[seg_shape[k]] == GEO_MESH;
+      long long all = n * (n - 1) / 2, mesh_pairs = all - (n - mm) * (n - mm - 1) / 2;
+      long long c = 5 * (all - mesh_pairs) + (long long)d.mesh_contact_max * mesh_pairs;
+      worst_world = c > worst_world ? c : worst_world;
+    }
+    long long want = (2 * worst_world + 63) / 64 * 64;
+    m->xpbd_contact_cap = static_cast<int>(want < 1024 ? 1024 : (want > 8192 ? 8192 : want));
+  }
   m->xpbd_keys_cap = 2ll * (d.rigid_contact_max > 0 ? d.rigid_contact_max : 1);
   m->xpbd_lds = xpbd_lds_bytes(d.max_world_bodies, d.max_world_joints, m->xpbd_contact_cap);
   if (m->collide_lds > 160 * 1024 || m->xpbd_lds > 160 * 1024) {
@@ -207,6 +238,7 @@ extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q
   if (q_in == q_out || qd_in == qd_out) return fail("nmc_xpbd_step: state_in and state_out must not alias");
   if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
     return fail("nmc_xpbd_step: contacts capacity exceeds the rigid_contact_max the model was created with");
+  if (!valid_lanes(m, p)) return fail("nmc_xpbd_step: block_lanes must be 0, 256, or 64 (worlds of up to 64 bodies)");
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
                              qd_out, nullptr, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step launch");
@@ -224,6 +256,8 @@ extern "C" int nmc_xpbd_step_learn(NmcModel* m, const NmcXpbdParams* p, const fl
   if (p->has_contacts && !c) return fail("nmc_xpbd_step_learn: has_contacts set but contacts is null");
   if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
     return fail("nmc_xpbd_step_learn: contacts capacity exceeds the rigid_contact_max the model was created with");
+  if (!valid_lanes(m, p))
+    return fail("nmc_xpbd_step_learn: block_lanes must be 0, 256, or 64 (worlds of up to 64 bodies)");
   const int mode = lrn->mode;
   if (mode < NMC_STEP_PLAIN || mode > NMC_STEP_WARM) return fail("nmc_xpbd_step_learn: unknown mode");
   if (mode != NMC_STEP_PREP) {

@@ -772,33 +772,75 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       }
     }
   }
-  // ---- ordering lists: counting sort by body, then each body sorts its (short) list by contact
+  // ---- ordering lists: counting sort by body into per-body lists in contact order
   block_exclusive_scan(cur, seg, nb, scan_tmp);
   for (int lb = tid; lb < nb; lb += BS) cur[lb] = seg[lb];
   __syncthreads();
-  for (int c = tid; c < nc; c += BS) {
-    CReg r;
-    if (c < CH) {
-      r = creg[0];
+  if (BS == 64 && nc > 256) {
+    // one wave, many contacts: contacts in order, 64 at a time; per body touched in the chunk, a ballot
+    // gives every touching lane its rank, so each list is filled in contact order (stable, no sort
+    // afterwards; the insertion sort below is quadratic in a body's list length)
+    for (int cb2 = 0; cb2 < nc; cb2 += 64) {
+      const int c = cb2 + tid;
+      int la = -1, lbb = -1;
+      if (c < nc) {
+        CReg r;
+        if (c < CH) {
+          r = creg[0];
 #pragma unroll
-      for (int jj = 1; jj < CPL; ++jj)
-        if (c >= jj * BS) r = creg[jj];
-    } else {
-      r = load_contact(a, cstart + c, b0, nb);
+          for (int jj = 1; jj < CPL; ++jj)
+            if (c >= jj * BS) r = creg[jj];
+        } else {
+          r = load_contact(a, cstart + c, b0, nb);
+        }
+        if (r.valid) {
+          la = r.la;
+          lbb = r.lb;
+        }
+      }
+      bool pa = la >= 0, pb = lbb >= 0;
+      const unsigned long long below = (1ull << tid) - 1ull;
+      for (;;) {
+        const unsigned long long pend = __ballot(pa || pb);
+        if (pend == 0ull) break;
+        const int leader = __ffsll(static_cast<long long>(pend)) - 1;
+        const int x = __shfl(pa ? la : lbb, leader, 64);
+        const bool ha = pa && la == x, hb = pb && lbb == x;
+        const unsigned long long hit = __ballot(ha || hb);
+        const int base = cur[x];
+        if (ha || hb) keys[base + __popcll(hit & below)] = static_cast<unsigned>(2 * c + (ha ? 0 : 1));
+        __builtin_amdgcn_wave_barrier();
+        if (tid == leader) cur[x] = base + __popcll(hit);
+        __builtin_amdgcn_wave_barrier();
+        pa = pa && !ha;
+        pb = pb && !hb;
+      }
     }
-    if (r.valid) {
-      if (r.la >= 0) keys[atomicAdd(&cur[r.la], 1)] = static_cast<unsigned>(2 * c);
-      if (r.lb >= 0) keys[atomicAdd(&cur[r.lb], 1)] = static_cast<unsigned>(2 * c + 1);
+  } else {
+    for (int c = tid; c < nc; c += BS) {
+      CReg r;
+      if (c < CH) {
+        r = creg[0];
+#pragma unroll
+        for (int jj = 1; jj < CPL; ++jj)
+          if (c >= jj * BS) r = creg[jj];
+      } else {
+        r = load_contact(a, cstart + c, b0, nb);
+      }
+      if (r.valid) {
+        if (r.la >= 0) keys[atomicAdd(&cur[r.la], 1)] = static_cast<unsigned>(2 * c);
+        if (r.lb >= 0) keys[atomicAdd(&cur[r.lb], 1)] = static_cast<unsigned>(2 * c + 1);
+      }
     }
-  }
-  __syncthreads();
-  for (int lb = tid; lb < nb; lb += BS) {
-    int s0 = seg[lb], s1 = seg[lb + 1];
-    for (int i = s0 + 1; i < s1; ++i) {
-      unsigned k = keys[i];
-      int j = i - 1;
-      while (j >= s0 && keys[j] > k) { keys[j + 1] = keys[j]; --j; }
-      keys[j + 1] = k;
+    __syncthreads();
+    for (int lb = tid; lb < nb; lb += BS) {
+      int s0 = seg[lb], s1 = seg[lb + 1];
+      for (int i = s0 + 1; i < s1; ++i) {
+        unsigned k = keys[i];
+        int j = i - 1;
+        while (j >= s0 && keys[j] > k) { keys[j + 1] = keys[j]; --j; }
+        keys[j + 1] = k;
+      }
     }
   }
   __syncthreads();
@@ -1062,17 +1104,26 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 // wave (64 lanes, 128 contacts held in registers, barriers cost one wave's issue); worlds up to 256
 // bodies run one body per lane on 256 lanes; larger worlds (up to 512 bodies) give each lane two.
 // Measured on 4096 worlds x 32 bodies: 64 lanes 0.377 ms per substep, 128 lanes 0.499 ms.
+// block_lanes (NmcXpbdParams) picks 64 or 256 lanes where both fit; contact-heavy worlds with few bodies
+// (triangle meshes: hundreds of contacts per world) may run faster on 256.
 struct XpbdShape { int bs, cpl, bpl; };
-static XpbdShape xpbd_shape(int max_bodies) {
-  if (max_bodies <= 64) return {64, 2, 1};
+static XpbdShape xpbd_shape(int max_bodies, int block_lanes = 0) {
+  if (max_bodies <= 64 && block_lanes != 256) return {64, 2, 1};
   if (max_bodies <= 256) return {256, 1, 1};
   return {256, 1, 2};
 }
 
 int xpbd_block_size(int max_bodies) { return xpbd_shape(max_bodies).bs; }
 
+// LDS of the largest launch shape the model may use (both shapes when it has at most 64 bodies a world).
+size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints, int key_cap);
 size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap) {
-  XpbdShape sh = xpbd_shape(max_bodies);
+  size_t a = xpbd_lds_bytes_shape(xpbd_shape(max_bodies, 0), max_bodies, max_joints, key_cap);
+  size_t b = xpbd_lds_bytes_shape(xpbd_shape(max_bodies, 256), max_bodies, max_joints, key_cap);
+  return a > b ? a : b;
+}
+
+size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints, int key_cap) {
   int bs = sh.bs;
   size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 1;
   size_t ch = (size_t)bs * sh.cpl;
@@ -1115,12 +1166,13 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   a.counters = m->counters;
   if (lrn) a.lrn = *lrn; else a.lrn = NmcLearnIO{};
   int worlds = m->d.world_count + 1;
-  XpbdShape sh = xpbd_shape(m->d.max_world_bodies);
+  XpbdShape sh = xpbd_shape(m->d.max_world_bodies, params->block_lanes);
+  const size_t lds = xpbd_lds_bytes_shape(sh, m->d.max_world_bodies, m->d.max_world_joints, m->xpbd_contact_cap);
   switch (a.lrn.mode) {
-    case NMC_STEP_LOG: launch_mode<XM_LOG>(sh, worlds, m->xpbd_lds, stream, a); break;
-    case NMC_STEP_PREP: launch_mode<XM_PREP>(sh, worlds, m->xpbd_lds, stream, a); break;
-    case NMC_STEP_WARM: launch_mode<XM_WARM>(sh, worlds, m->xpbd_lds, stream, a); break;
-    default: launch_mode<XM_PLAIN>(sh, worlds, m->xpbd_lds, stream, a); break;
+    case NMC_STEP_LOG: launch_mode<XM_LOG>(sh, worlds, lds, stream, a); break;
+    case NMC_STEP_PREP: launch_mode<XM_PREP>(sh, worlds, lds, stream, a); break;
+    case NMC_STEP_WARM: launch_mode<XM_WARM>(sh, worlds, lds, stream, a); break;
+    default: launch_mode<XM_PLAIN>(sh, worlds, lds, stream, a); break;
   }
   return hipGetLastError();
 }

@@ -47,8 +47,11 @@ class SolverXPBD(SolverBase):
                  joint_angular_relaxation: float = 0.4, joint_linear_compliance: float = 0.0,
                  joint_angular_compliance: float = 0.0, rigid_contact_relaxation: float = 0.8,
                  rigid_contact_con_weighting: bool = True, angular_damping: float = 0.0,
-                 enable_restitution: bool = False):
+                 enable_restitution: bool = False, block_lanes: int = 0):
         super().__init__(model)
+        if block_lanes not in (0, 64, 256):
+            raise ValueError("block_lanes must be 0 (automatic), 64 or 256")
+        self.block_lanes = int(block_lanes)  # lanes per world of the step kernel (see autotune.tune_block_lanes)
         if enable_restitution:
             raise NotImplementedError("restitution is not part of this build")
         self.iterations = int(iterations)
@@ -77,7 +80,8 @@ class SolverXPBD(SolverBase):
     def params(self, has_contacts: bool) -> NmcXpbdParams:
         return NmcXpbdParams(self.iterations, self.rigid_contact_relaxation, int(self.rigid_contact_con_weighting),
                              self.angular_damping, self.joint_linear_relaxation, self.joint_angular_relaxation,
-                             self.joint_linear_compliance, self.joint_angular_compliance, int(has_contacts))
+                             self.joint_linear_compliance, self.joint_angular_compliance, int(has_contacts),
+                             int(getattr(self, "block_lanes", 0)))
 
     def step(self, state_in, state_out, control, contacts, dt: float) -> None:
         """solver_xpbd.py:259-764."""

@@ -1,0 +1,59 @@
+"""The step kernel's launch shapes give bit-identical results, and the autotuner picks one of them."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+from newton_custom_contact_amd import CollisionPipeline, scenes
+from newton_custom_contact_amd.autotune import candidate_block_lanes, tune_block_lanes
+from newton_custom_contact_amd.solvers import SolverXPBD
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 240.0
+
+
+def _mesh_model(dev):
+    lib = scenes.mesh_library(4, seed=31, n_min=48, n_max=96)
+    for mm in lib:
+        mm.build_sdf(max_resolution=24, device=dev)
+    return scenes.mesh_worlds(4, seed=31, device=dev, library=lib)
+
+
+MODELS = [
+    ("mixed_8w", lambda dev: scenes.mixed_worlds(8, seed=5, device=dev)),
+    ("mesh_4w", _mesh_model),
+]
+
+
+@pytest.mark.parametrize("name,make", MODELS, ids=[m[0] for m in MODELS])
+def test_block_lanes_bit_identical(cuda_device, name, make):
+    print(f"[block lanes] {name}", flush=True)
+    model = make(cuda_device)
+    pipe = CollisionPipeline(model, broad_phase="sap")
+    solver = SolverXPBD(model, iterations=10)
+    assert candidate_block_lanes(solver) == [64, 256]
+    c = pipe.contacts()
+    s0, s1 = model.state(), model.state()
+    # settle a little so that worlds carry contacts
+    for _ in range(120):
+        pipe.collide(s0, c)
+        solver.step(s0, s1, None, c, DT)
+        s0, s1 = s1, s0
+    pipe.collide(s0, c)
+    outs = {}
+    for lanes in (64, 256):
+        solver.block_lanes = lanes
+        a, b = model.state(), model.state()
+        a.body_q.copy_(s0.body_q)
+        a.body_qd.copy_(s0.body_qd)
+        solver.step(a, b, None, c, DT)
+        torch.cuda.synchronize()
+        outs[lanes] = (b.body_q.cpu().numpy(), b.body_qd.cpu().numpy())
+    assert int(c.rigid_contact_count.item()) > 0
+    assert np.array_equal(outs[64][0], outs[256][0]) and np.array_equal(outs[64][1], outs[256][1])
+    res = tune_block_lanes(solver, s0, s1, c, DT, reps=2)
+    print(name, res, flush=True)
+    assert res["block_lanes"] in (64, 256) and solver.block_lanes == res["block_lanes"]
