@@ -13,7 +13,9 @@
 // key inside a segment) with no float atomics and no global sort.
 //
 // Launch sequence of nmc_collide (all on the caller's stream, graph-capturable):
-//   collide_segment_kernel  <<<segments, 256>>>   AABB + SAP + pair sort -> pair region, route list
+//   collide_segment_wave_kernel <<<segments/4, 256>>> one wave per segment of <= 64 shapes:
+//                                                 AABB + SAP + pair sort -> pair region, route list
+//   collide_segment_kernel  <<<big segments, 256>>> the same for larger segments, one workgroup each
 //   route_scan_kernel       <<<1, 1024>>>         route offsets over segments
 //   narrow_kernel           <<<<=2048, 256>>>     narrow phase -> contact slots, per-pair counts
 //   mesh_narrow_kernel      <<<<=2048, 256>>>     mesh pairs, one workgroup per pair (models with meshes)
@@ -64,12 +66,29 @@ __device__ __forceinline__ bool excluded(int b, const int* __restrict__ fp, int 
   return false;
 }
 
+// Lane groups that run one segment: the whole workgroup (barrier) or one wave (wave-local ordering of
+// its LDS accesses; a wave's LDS operations complete in issue order).
+struct BlockGroup {
+  int lane;
+  static constexpr int size = CBLOCK;
+  __device__ __forceinline__ void sync() const { __syncthreads(); }
+};
+struct WaveGroup {
+  int lane;
+  static constexpr int size = 64;
+  __device__ __forceinline__ void sync() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+};
+
 // In-LDS bitonic sort of n2 (power of two) keys, ascending.
-template <class K>
-__device__ void bitonic_sort(K* keys, int n2) {
+template <class K, class G = BlockGroup>
+__device__ void bitonic_sort(K* keys, int n2, G g = G{static_cast<int>(threadIdx.x)}) {
   for (int k = 2; k <= n2; k <<= 1) {
     for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+      for (int i = g.lane; i < n2; i += G::size) {
         int ixj = i ^ j;
         if (ixj > i) {
           K a = keys[i], b = keys[ixj];
@@ -80,7 +99,7 @@ __device__ void bitonic_sort(K* keys, int n2) {
           }
         }
       }
-      __syncthreads();
+      g.sync();
     }
   }
 }
@@ -169,22 +188,19 @@ struct SegLds {
   unsigned* pairs;            // pair keys (pair_cap)
 };
 
-// Broad phase of one segment: shape world data (kept in shape_wc for the narrow phase), SAP, the
-// segment's pairs in key order at staging pairs [base, base + np), and their route entries.
-__global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d, const float* __restrict__ body_q,
-                                                                  StageBuffers st,
-                                                                  const long long* __restrict__ seg_pair_off,
-                                                                  const int* __restrict__ seg_pair_cap,
-                                                                  int* __restrict__ seg_pair_count,
-                                                                  int* __restrict__ seg_conv_count,
-                                                                  int* __restrict__ seg_mesh_count,
-                                                                  int* __restrict__ counters, int pair_cap,
-                                                                  int max_n) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int s_npairs;
-  __shared__ int s_cls[3];
-  __shared__ int s_rbase[3];
-  const int seg = blockIdx.x;
+// Broad phase of one segment by lane group g: shape world data (kept in shape_wc for the narrow phase),
+// SAP, the segment's pairs in key order at staging pairs [base, base + np), and their route entries.
+// smem: the group's dynamic LDS; sc: its 7 counters (pair count, 3 class counts, 3 route bases).
+template <class G>
+__device__ void segment_broad_phase(const G g, const NmcModelDesc& d, const float* __restrict__ body_q,
+                                    const StageBuffers& st, const long long* __restrict__ seg_pair_off,
+                                    const int* __restrict__ seg_pair_cap, int* __restrict__ seg_pair_count,
+                                    int* __restrict__ seg_conv_count, int* __restrict__ seg_mesh_count,
+                                    int* __restrict__ counters, int pair_cap, int max_n, int seg, unsigned char* smem,
+                                    int* sc) {
+  int& s_npairs = sc[0];
+  int* s_cls = sc + 1;
+  int* s_rbase = sc + 4;
   const int s0 = d.segment_start[seg];
   const int n = d.segment_start[seg + 1] - s0;
   const bool regular = seg < d.segment_count - 1;
@@ -205,7 +221,7 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
   L.fend = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
   L.pairs = reinterpret_cast<unsigned*>(p);
 
-  if (threadIdx.x == 0) {
+  if (g.lane == 0) {
     s_npairs = 0;
     s_cls[0] = 0;
     s_cls[1] = 0;
@@ -214,7 +230,7 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
 
   // ---- shape world transforms, AABBs, SAP projections (collide.py:150-275, broad_phase_sap.py:43-63)
   const f3 dir = normalize(f3{0.5935f, 0.7790f, 0.1235f});
-  for (int l = threadIdx.x; l < n; l += blockDim.x) {
+  for (int l = g.lane; l < n; l += G::size) {
     int s = d.segment_shape[s0 + l];
     ShapeW w = shape_world(s, body_q, d);
     // a world's shapes are written by its own segment, global shapes by the dedicated last segment
@@ -242,12 +258,12 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
     L.skeys[l] = (static_cast<unsigned long long>(sortable_float(plo)) << 32) | static_cast<unsigned>(l);
   }
   const int n2 = next_pow2(n > 1 ? n : 1);
-  for (int l = n + threadIdx.x; l < n2; l += blockDim.x) L.skeys[l] = ~0ull;
-  __syncthreads();
-  bitonic_sort(L.skeys, n2);
+  for (int l = n + g.lane; l < n2; l += G::size) L.skeys[l] = ~0ull;
+  g.sync();
+  bitonic_sort(L.skeys, n2, g);
 
   // ---- sweep (broad_phase_sap.py:189-394): element i sweeps forward while lower_j < upper_i
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = g.lane; i < n; i += G::size) {
     int li = static_cast<int>(L.skeys[i] & 0xffffffffu);
     float upper = L.phi[li];
     for (int j = i + 1; j < n; ++j) {
@@ -276,30 +292,30 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
       if (slot < pair_cap) L.pairs[slot] = (static_cast<unsigned>(ra) << 16) | static_cast<unsigned>(rb);
     }
   }
-  __syncthreads();
+  g.sync();
   int np = s_npairs;
   if (np > pair_cap) {
-    if (threadIdx.x == 0) atomicAdd(&counters[0], np - pair_cap);
+    if (g.lane == 0) atomicAdd(&counters[0], np - pair_cap);
     np = pair_cap;
   }
   const int pn2 = next_pow2(np > 1 ? np : 1);
-  for (int l = np + threadIdx.x; l < pn2; l += blockDim.x) L.pairs[l] = 0xffffffffu;
-  __syncthreads();
-  bitonic_sort(L.pairs, pn2);
+  for (int l = np + g.lane; l < pn2; l += G::size) L.pairs[l] = 0xffffffffu;
+  g.sync();
+  bitonic_sort(L.pairs, pn2, g);
 
   // ---- pairs out into the segment's region (key order) + its route list (MPR/GJK, analytic, mesh)
   const long long poff = seg_pair_off[seg];
   const int pcap = seg_pair_cap[seg];
   if (np > pcap) np = pcap;  // cannot happen: np <= n(n-1)/2 and np <= pair_cap
-  for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
+  for (int pi = g.lane; pi < np; pi += G::size) {
     unsigned key = L.pairs[pi];
     int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
     st.pair_a[poff + pi] = L.gid[ra];
     st.pair_b[poff + pi] = L.gid[rb];
     atomicAdd(&s_cls[route_class(L.type[ra], L.type[rb])], 1);
   }
-  __syncthreads();
-  if (threadIdx.x == 0) {
+  g.sync();
+  if (g.lane == 0) {
     seg_pair_count[seg] = np;
     seg_conv_count[seg] = s_cls[1];
     seg_mesh_count[seg] = s_cls[2];
@@ -310,12 +326,51 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
     s_cls[1] = 0;
     s_cls[2] = 0;
   }
-  __syncthreads();
-  for (int pi = threadIdx.x; pi < np; pi += blockDim.x) {
+  g.sync();
+  for (int pi = g.lane; pi < np; pi += G::size) {
     unsigned key = L.pairs[pi];
     int cls = route_class(L.type[key >> 16], L.type[key & 0xffffu]);
     st.route[poff + s_rbase[cls] + atomicAdd(&s_cls[cls], 1)] = pi;
   }
+}
+
+// One 256-lane workgroup per listed segment (segments above 64 shapes).
+__global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d, const float* __restrict__ body_q,
+                                                                  StageBuffers st,
+                                                                  const long long* __restrict__ seg_pair_off,
+                                                                  const int* __restrict__ seg_pair_cap,
+                                                                  int* __restrict__ seg_pair_count,
+                                                                  int* __restrict__ seg_conv_count,
+                                                                  int* __restrict__ seg_mesh_count,
+                                                                  int* __restrict__ counters, int pair_cap,
+                                                                  int max_n, const int* __restrict__ seg_list) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int sc[7];
+  segment_broad_phase(BlockGroup{static_cast<int>(threadIdx.x)}, d, body_q, st, seg_pair_off, seg_pair_cap,
+                      seg_pair_count, seg_conv_count, seg_mesh_count, counters, pair_cap, max_n, seg_list[blockIdx.x],
+                      smem, sc);
+}
+
+// One wave per listed segment (segments of at most 64 shapes), four segments per workgroup: the sorts and
+// the sweep synchronise one wave instead of the workgroup.  lds_per_wave: the dynamic LDS of one wave.
+__global__ void __launch_bounds__(CBLOCK) collide_segment_wave_kernel(NmcModelDesc d, const float* __restrict__ body_q,
+                                                                       StageBuffers st,
+                                                                       const long long* __restrict__ seg_pair_off,
+                                                                       const int* __restrict__ seg_pair_cap,
+                                                                       int* __restrict__ seg_pair_count,
+                                                                       int* __restrict__ seg_conv_count,
+                                                                       int* __restrict__ seg_mesh_count,
+                                                                       int* __restrict__ counters, int pair_cap,
+                                                                       int max_n, const int* __restrict__ seg_list,
+                                                                       int n_list, int lds_per_wave) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ int sc[CBLOCK / 64][8];
+  const int wave = threadIdx.x >> 6;
+  const int idx = blockIdx.x * (CBLOCK / 64) + wave;
+  if (idx >= n_list) return;  // whole waves only; the kernel has no workgroup barrier
+  segment_broad_phase(WaveGroup{static_cast<int>(threadIdx.x & 63)}, d, body_q, st, seg_pair_off, seg_pair_cap,
+                      seg_pair_count, seg_conv_count, seg_mesh_count, counters, pair_cap, max_n, seg_list[idx],
+                      smem + static_cast<size_t>(wave) * lds_per_wave, sc[wave]);
 }
 
 // Route offsets: exclusive prefix of MPR/GJK, analytic and mesh pair counts over segments.
@@ -477,9 +532,18 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
   const NmcModelDesc& d = m->d;
   int segs = d.segment_count;
   if (segs <= 0) return hipSuccess;
-  hipLaunchKernelGGL(collide_segment_kernel, dim3(segs), dim3(CBLOCK), m->collide_lds, stream, d, body_q, m->stage,
-                     m->seg_pair_off, m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count, m->seg_mesh_count,
-                     m->counters, m->pair_cap, d.max_segment_shapes);
+  if (m->n_seg_small > 0) {
+    const int per = CBLOCK / 64;
+    hipLaunchKernelGGL(collide_segment_wave_kernel, dim3((m->n_seg_small + per - 1) / per), dim3(CBLOCK),
+                       static_cast<size_t>(per) * m->seg_small_lds, stream, d, body_q, m->stage, m->seg_pair_off,
+                       m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count, m->seg_mesh_count, m->counters,
+                       m->seg_small_pair_cap, m->seg_small_max_n, m->seg_small, m->n_seg_small,
+                       static_cast<int>(m->seg_small_lds));
+  }
+  if (m->n_seg_big > 0)
+    hipLaunchKernelGGL(collide_segment_kernel, dim3(m->n_seg_big), dim3(CBLOCK), m->collide_lds, stream, d, body_q,
+                       m->stage, m->seg_pair_off, m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count,
+                       m->seg_mesh_count, m->counters, m->pair_cap, d.max_segment_shapes, m->seg_big);
   const bool mesh = d.mesh_count > 0;
   hipLaunchKernelGGL(route_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, m->seg_pair_count, m->seg_conv_count,
                      mesh ? m->seg_mesh_count : nullptr, m->seg_conv_off, m->seg_ana_off,

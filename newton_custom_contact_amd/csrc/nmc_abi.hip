@@ -137,6 +137,29 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       total += cap[sg];
     }
     if (total >= (1ll << 31)) { nmc_model_destroy(m); return fail("broad-phase pair regions exceed 2^31 entries"); }
+    // segments of at most 64 shapes run one wave each (their whole pair set fits the wave's LDS buffer)
+    std::vector<int> small, big;
+    int small_max = 1;
+    for (int sg = 0; sg < segs; ++sg) {
+      int n = starts[sg + 1] - starts[sg];
+      if (n <= 64) {
+        small.push_back(sg);
+        small_max = n > small_max ? n : small_max;
+      } else {
+        big.push_back(sg);
+      }
+    }
+    m->n_seg_small = static_cast<int>(small.size());
+    m->n_seg_big = static_cast<int>(big.size());
+    m->seg_small_max_n = small_max;
+    m->seg_small_pair_cap = next_pow2(small_max * (small_max - 1) / 2 > 1 ? small_max * (small_max - 1) / 2 : 1);
+    m->seg_small_lds = collide_lds_bytes(small_max, m->seg_small_pair_cap);
+    if ((e = dmalloc(&m->seg_small, small.size())) != hipSuccess || (e = dmalloc(&m->seg_big, big.size())) != hipSuccess ||
+        (!small.empty() && (e = hipMemcpy(m->seg_small, small.data(), sizeof(int) * small.size(), hipMemcpyHostToDevice)) != hipSuccess) ||
+        (!big.empty() && (e = hipMemcpy(m->seg_big, big.data(), sizeof(int) * big.size(), hipMemcpyHostToDevice)) != hipSuccess)) {
+      nmc_model_destroy(m);
+      return hip_fail(e, "nmc_model_create: segment lists");
+    }
     long long pairs = total > 0 ? total : 1;
     long long slots = 5 * pairs;
     StageBuffers& s = m->stage;
@@ -207,6 +230,7 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
   StageBuffers& s = m->stage;
   void* ptrs[] = {s.rec, s.mask,
                   s.pair_a, s.pair_b, s.route, s.route_count, s.mesh_index, m->seg_mesh_count, m->seg_mesh_off,
+                  m->seg_small, m->seg_big,
                   s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
                   m->seg_pair_count, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, m->seg_contact_count,
                   m->counters, m->xpbd_keys};
