@@ -126,8 +126,10 @@ __device__ __forceinline__ int block_inclusive_scan(int v, int* s_wave) {
 }
 
 // Exclusive scans of N per-segment count arrays by one 1024-lane workgroup; writes the totals.  A null
-// input array scans as zeros, a null output array is skipped.
+// input array scans as zeros, a null output array is skipped.  Each lane scans 4 consecutive entries, so
+// a pass covers 4096 segments.
 constexpr int SCAN_BLOCK = 1024;
+constexpr int SCAN_ITEMS = 4;
 template <int N>
 __device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], int n, int (&total)[N]) {
   __shared__ int s_w[N][SCAN_BLOCK / 64];
@@ -135,13 +137,18 @@ __device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], in
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (threadIdx.x < N) s_carry[threadIdx.x] = 0;
   __syncthreads();
-  for (int base = 0; base < n; base += SCAN_BLOCK) {
-    const int i = base + threadIdx.x;
-    int v[N], inc[N];
+  for (int base = 0; base < n; base += SCAN_BLOCK * SCAN_ITEMS) {
+    const int i0 = base + threadIdx.x * SCAN_ITEMS;
+    int v[N][SCAN_ITEMS], sum[N], inc[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
-      v[k] = (in[k] && i < n) ? in[k][i] : 0;
-      inc[k] = v[k];
+      sum[k] = 0;
+#pragma unroll
+      for (int t = 0; t < SCAN_ITEMS; ++t) {
+        v[k][t] = (in[k] && i0 + t < n) ? in[k][i0 + t] : 0;
+        sum[k] += v[k][t];
+      }
+      inc[k] = sum[k];
     }
     for (int off = 1; off < 64; off <<= 1) {
 #pragma unroll
@@ -160,7 +167,14 @@ __device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], in
     for (int k = 0; k < N; ++k) {
       c[k] = s_carry[k];
       for (int q = 0; q < w; ++q) c[k] += s_w[k][q];
-      if (i < n && out[k]) out[k][i] = c[k] + inc[k] - v[k];
+      int ex = c[k] + inc[k] - sum[k];
+      if (out[k]) {
+#pragma unroll
+        for (int t = 0; t < SCAN_ITEMS; ++t) {
+          if (i0 + t < n) out[k][i0 + t] = ex;
+          ex += v[k][t];
+        }
+      }
     }
     __syncthreads();
     if (threadIdx.x == SCAN_BLOCK - 1) {
@@ -386,6 +400,7 @@ __global__ void __launch_bounds__(SCAN_BLOCK) route_scan_kernel(const int* __res
   int* const out[3] = {seg_conv_off, seg_ana_off, seg_mesh_off};
   int total[3];
   segment_scan<3>(in, out, segments, total);
+  __syncthreads();  // the scan's writes by other lanes
   for (int i = threadIdx.x; i < segments; i += blockDim.x)
     seg_ana_off[i] -= seg_conv_off[i] + (seg_mesh_off ? seg_mesh_off[i] : 0);
   if (threadIdx.x == 0) {

@@ -165,8 +165,10 @@ __device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f
   if (lb >= 0) {
     com_b = load3(L.com + 3 * lb); mib = L.inv_m[lb]; Ib = lds_m33(L.inv_I + 9 * lb); om_b = load3(L.qd + 6 * lb + 3);
   }
-  f3 r_a = bx_a - xpoint(Xa, com_a);
-  f3 r_b = bx_b - xpoint(Xb, com_b);
+  // world centres of mass (the reference transforms them again for the friction rows; same values)
+  const f3 wcom_a = xpoint(Xa, com_a), wcom_b = xpoint(Xb, com_b);
+  f3 r_a = bx_a - wcom_a;
+  f3 r_b = bx_b - wcom_b;
   f3 ang_a = -cross(r_a, n);
   f3 ang_b = cross(r_b, n);
   float lambda_n = contact_delta(d, Xa.q, Xb.q, mia, mib, Ia, Ib, -n, n, ang_a, ang_b, relax, dt);
@@ -178,8 +180,8 @@ __device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f
     bx_b = bx_b + xvector(Xb, o1);
     f3 delta = bx_b - bx_a;
     f3 fd = delta - dot(n, delta) * n;
-    r_a = bx_a - xpoint(Xa, com_a);
-    r_b = bx_b - xpoint(Xb, com_b);
+    r_a = bx_a - wcom_a;
+    r_b = bx_b - wcom_b;
     f3 rel_v = zero;
     if (la >= 0 && (L.flags[la] & BODY_KINEMATIC)) {
       f3 va = load3(L.qd + 6 * la) + cross(load3(L.qd + 6 * la + 3), r_a);
