@@ -250,6 +250,29 @@ int nmc_mlp_train_grad(const NmcMlp* weights, const float* feat, const float* ta
 int nmc_sdf_build(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
                   const float lower[3], float voxel, const int cells[3], float* out, nmc_stream_t stream);
 
+/* Frame-to-frame contact matching (CollisionPipeline(contact_matching="latest" | "sticky", contact_report),
+ * newton/_src/geometry/contact_match.py:570-992 ContactMatcher; collide.py:1084-1184).  The matcher keeps the
+ * previous frame's contacts (keys, world midpoints, normals; body-frame points/offsets when sticky).
+ * nmc_contact_match runs after nmc_collide on the same Contacts buffer (sort keys required): it writes
+ * match_index (>= 0: index of the matched contact in the previous frame's buffer; -1 MATCH_NOT_FOUND: the
+ * shape pair had no contact; -2 MATCH_BROKEN: none within pos_threshold / normal_dot_threshold, or a closer
+ * new contact won it), replays the matched rows' previous point0/1, offset0/1 and normal when sticky, fills
+ * the report when the matcher has one, and saves this frame for the next. */
+typedef struct NmcMatcher NmcMatcher;
+typedef struct NmcContactReport {
+  int* new_indices;    /* capacity: contacts of this frame with match_index < 0, in buffer order */
+  int* new_count;      /* [1] */
+  int* broken_indices; /* capacity: previous-frame contacts no new contact matched, in that buffer's order */
+  int* broken_count;   /* [1] */
+} NmcContactReport;
+int nmc_matcher_create(const NmcModel* model, int capacity, float pos_threshold, float normal_dot_threshold,
+                       int sticky, int report, NmcMatcher** out);
+void nmc_matcher_destroy(NmcMatcher* matcher);
+/* ContactMatcher.reset (contact_match.py:745-758): the next frame matches nothing. */
+int nmc_matcher_reset(NmcMatcher* matcher, nmc_stream_t stream);
+int nmc_contact_match(NmcMatcher* matcher, const NmcModel* model, const float* body_q, const NmcContacts* contacts,
+                      int* match_index, const NmcContactReport* report, nmc_stream_t stream);
+
 /* Copies the counters accumulated since model creation to host (synchronises the stream):
  * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = mesh-pair candidate
  * contacts dropped (candidate buffer full; cannot happen for models nmc_model_create accepted),

@@ -93,7 +93,12 @@ class NmcMlpGrads(ctypes.Structure):
 NMC_STEP_PLAIN, NMC_STEP_LOG, NMC_STEP_PREP, NMC_STEP_WARM = 0, 1, 2, 3
 
 EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_step", "nmc_xpbd_step_learn",
-           "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_get_counters", "nmc_last_error")
+           "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_matcher_create", "nmc_matcher_destroy",
+           "nmc_matcher_reset", "nmc_contact_match", "nmc_get_counters", "nmc_last_error")
+
+
+class NmcContactReport(ctypes.Structure):
+    _fields_ = [("new_indices", _vp), ("new_count", _vp), ("broken_indices", _vp), ("broken_count", _vp)]
 
 MESH_CONTACT_MAX = 64  # contacts kept per mesh pair (the configuration's cap; the reference caps at 255)
 
@@ -133,6 +138,16 @@ def load() -> ctypes.CDLL:
         L.nmc_sdf_build.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                     ctypes.c_float, ctypes.POINTER(ctypes.c_int), _vp, _vp]
         L.nmc_sdf_build.restype = ctypes.c_int
+        L.nmc_matcher_create.argtypes = [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                         ctypes.c_int, ctypes.POINTER(_vp)]
+        L.nmc_matcher_create.restype = ctypes.c_int
+        L.nmc_matcher_destroy.argtypes = [_vp]
+        L.nmc_matcher_destroy.restype = None
+        L.nmc_matcher_reset.argtypes = [_vp, _vp]
+        L.nmc_matcher_reset.restype = ctypes.c_int
+        L.nmc_contact_match.argtypes = [_vp, _vp, _vp, ctypes.POINTER(NmcContacts), _vp,
+                                        ctypes.POINTER(NmcContactReport), _vp]
+        L.nmc_contact_match.restype = ctypes.c_int
         L.nmc_get_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.nmc_get_counters.restype = ctypes.c_int
         L.nmc_last_error.argtypes = []

@@ -20,7 +20,7 @@ import ctypes
 
 import numpy as np
 
-from ._native import NativeModel, check, load, ptr, stream_handle
+from ._native import NativeModel, NmcContactReport, check, load, ptr, stream_handle
 from ._types import GeoType, ShapeFlags
 from .contacts import Contacts
 
@@ -81,10 +81,17 @@ class CollisionPipeline:
                  contact_report: bool = False, verify_buffers: bool = True):
         if contact_matching not in ("disabled", "latest", "sticky"):
             raise ValueError(f"contact_matching must be one of 'disabled', 'latest', 'sticky', got {contact_matching!r}")
-        if contact_matching != "disabled":
-            raise NotImplementedError("contact matching is not part of this build yet")
-        if contact_report:
+        if contact_matching_pos_threshold < 0.0:
+            raise ValueError(
+                f"contact_matching_pos_threshold must be non-negative, got {contact_matching_pos_threshold}")
+        if not -1.0 <= contact_matching_normal_dot_threshold <= 1.0:
+            raise ValueError(
+                f"contact_matching_normal_dot_threshold must be in [-1, 1], got {contact_matching_normal_dot_threshold}")
+        matching_enabled = contact_matching != "disabled"
+        if contact_report and not matching_enabled:
             raise ValueError('contact_report=True requires contact_matching != "disabled"')
+        if matching_enabled:
+            deterministic = True  # any matching mode implies the deterministic (sort-keyed) contact order
         mode = "explicit" if broad_phase is None else str(broad_phase).lower()
         if mode not in ("nxn", "sap", "explicit"):
             raise ValueError(f"Unsupported broad phase mode: {broad_phase!r}")
@@ -118,6 +125,21 @@ class CollisionPipeline:
         self._colliding = int(np.count_nonzero(colliding))
         self.native = NativeModel(model, self._rigid_contact_max, self.shape_pairs_max)
         self._lib = load()
+        self.contact_matching = contact_matching
+        self._matching_enabled = matching_enabled
+        self._matching_sticky = contact_matching == "sticky"
+        self.contact_report = bool(contact_report)
+        self._contact_matcher = (
+            ContactMatcher(self.native, self._rigid_contact_max, contact_matching_pos_threshold,
+                           contact_matching_normal_dot_threshold, sticky=self._matching_sticky,
+                           contact_report=self.contact_report)
+            if matching_enabled else None)
+
+    @property
+    def contact_matcher(self):
+        """The frame-to-frame matcher (None when contact_matching is "disabled"); ``reset()`` forgets the
+        previous frame (after teleporting bodies or an episode reset)."""
+        return self._contact_matcher
 
     @property
     def rigid_contact_max(self) -> int:
@@ -132,7 +154,8 @@ class CollisionPipeline:
         return Contacts(self.rigid_contact_max, self.soft_contact_max, requires_grad=self.requires_grad,
                         device=self.device, segment_count=len(self.model.host["segment_start"]) - 1,
                         deterministic=self.deterministic,
-                        requested_attributes=self.model.get_requested_contact_attributes())
+                        requested_attributes=self.model.get_requested_contact_attributes(),
+                        contact_matching=self._matching_enabled, contact_report=self.contact_report)
 
     def collide(self, state, contacts: Contacts, *, soft_contact_margin: float | None = None):
         """collide.py:902-1230 (rigid part): writes ``contacts`` from ``state.body_q``."""
@@ -149,6 +172,14 @@ class CollisionPipeline:
         rc = self._lib.nmc_collide(self.native.handle, ctypes.c_void_p(ptr(q)), ctypes.byref(contacts.native_view()),
                                    ctypes.c_void_p(stream_handle(self.device)))
         check(rc, "nmc_collide")
+        if self._contact_matcher is not None:
+            if contacts.rigid_contact_match_index is None:
+                raise ValueError("CollisionPipeline has contact_matching enabled but the Contacts buffer was created "
+                                 "without contact_matching. Use pipeline.contacts() to create a compatible buffer.")
+            if self.contact_report and contacts.rigid_contact_new_indices is None:
+                raise ValueError("CollisionPipeline has contact_report enabled but the Contacts buffer was created "
+                                 "without contact_report=True. Use pipeline.contacts() to create a compatible buffer.")
+            self._contact_matcher.match(q, contacts)
         contacts.contact_generation += 1
 
     def overflow_counters(self) -> dict:
@@ -158,4 +189,45 @@ class CollisionPipeline:
                 "contacts_over_capacity": int(c[2]), "solver_ordering_overflow": int(c[3])}
 
 
-__all__ = ["CollisionPipeline", "estimate_rigid_contact_max", "per_world_shape_pairs_max"]
+class ContactMatcher:
+    """Frame-to-frame contact matching (contact_match.py:570-992) over ``nmc_contact_match``.  Owned by the
+    pipeline; keeps the previous frame's contacts on the device."""
+
+    def __init__(self, native: NativeModel, capacity: int, pos_threshold: float, normal_dot_threshold: float,
+                 sticky: bool, contact_report: bool):
+        self._lib = load()
+        self._native = native
+        self.handle = ctypes.c_void_p()
+        self.is_sticky = sticky
+        self.has_report = contact_report
+        check(self._lib.nmc_matcher_create(native.handle, int(capacity), ctypes.c_float(pos_threshold),
+                                           ctypes.c_float(normal_dot_threshold), int(sticky), int(contact_report),
+                                           ctypes.byref(self.handle)), "nmc_matcher_create")
+
+    def reset(self) -> None:
+        """Forget the previous frame: the next match reports every contact as MATCH_NOT_FOUND."""
+        check(self._lib.nmc_matcher_reset(self.handle, ctypes.c_void_p(stream_handle(self._native.model.device))),
+              "nmc_matcher_reset")
+
+    def match(self, body_q, contacts) -> None:
+        rep = None
+        if self.has_report:
+            rep = NmcContactReport(ptr(contacts.rigid_contact_new_indices), ptr(contacts.rigid_contact_new_count),
+                                   ptr(contacts.rigid_contact_broken_indices), ptr(contacts.rigid_contact_broken_count))
+        rc = self._lib.nmc_contact_match(self.handle, self._native.handle, ctypes.c_void_p(ptr(body_q)),
+                                         ctypes.byref(contacts.native_view()),
+                                         ctypes.c_void_p(ptr(contacts.rigid_contact_match_index)),
+                                         ctypes.byref(rep) if rep is not None else None,
+                                         ctypes.c_void_p(stream_handle(self._native.model.device)))
+        check(rc, "nmc_contact_match")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self._lib.nmc_matcher_destroy(self.handle)
+                self.handle = ctypes.c_void_p()
+        except Exception:  # noqa: BLE001 (interpreter shutdown)
+            pass
+
+
+__all__ = ["CollisionPipeline", "ContactMatcher", "estimate_rigid_contact_max", "per_world_shape_pairs_max"]

@@ -9,11 +9,13 @@ segment-major (world 0, world 1, ..., then the global-only segment) and ascendin
 
 from __future__ import annotations
 
-import ctypes
-
 import torch
 
 from ._native import NmcContacts, ptr
+
+
+MATCH_NOT_FOUND = -1  # contact_match.py:116-120
+MATCH_BROKEN = -2
 
 
 class Contacts:
@@ -21,7 +23,8 @@ class Contacts:
 
     def __init__(self, rigid_contact_max: int, soft_contact_max: int = 0, requires_grad: bool = False,
                  device=None, segment_count: int = 1, deterministic: bool = False,
-                 requested_attributes: set[str] | None = None):
+                 requested_attributes: set[str] | None = None, contact_matching: bool = False,
+                 contact_report: bool = False):
         dev = torch.device(device) if device is not None else torch.device("cuda:0")
         C = int(rigid_contact_max)
         self.device = dev
@@ -46,7 +49,21 @@ class Contacts:
         self.rigid_contact_margin1 = torch.zeros((C,), **f)
         self.rigid_contact_tids = torch.zeros((C,), **i)
         self.rigid_contact_sort_key = torch.zeros((C,), device=dev, dtype=torch.int64) if deterministic else None
-        self.rigid_contact_match_index = None
+        # contact matching (contacts.py:226-270): match index per contact (>= 0 matched previous-frame
+        # index, -1 MATCH_NOT_FOUND, -2 MATCH_BROKEN) and the optional new / broken report
+        self.contact_matching = bool(contact_matching)
+        self.contact_report = bool(contact_report)
+        self.rigid_contact_match_index = torch.full((C,), -1, **i) if contact_matching else None
+        if contact_report:
+            self.rigid_contact_new_indices = torch.zeros((C,), **i)
+            self.rigid_contact_new_count = torch.zeros(1, **i)
+            self.rigid_contact_broken_indices = torch.zeros((C,), **i)
+            self.rigid_contact_broken_count = torch.zeros(1, **i)
+        else:
+            self.rigid_contact_new_indices = None
+            self.rigid_contact_new_count = None
+            self.rigid_contact_broken_indices = None
+            self.rigid_contact_broken_count = None
         self.world_contact_start = torch.zeros((segment_count,), **i)
         self.world_contact_count = torch.zeros((segment_count,), **i)
         attrs = requested_attributes or set()
@@ -93,8 +110,14 @@ class Contacts:
         }
         if self.rigid_contact_sort_key is not None:
             out["sort_key"] = self.rigid_contact_sort_key[:n].cpu().numpy()
+        if self.rigid_contact_match_index is not None:
+            out["match_index"] = self.rigid_contact_match_index[:n].cpu().numpy()
+        if self.rigid_contact_new_indices is not None:
+            nn = int(self.rigid_contact_new_count.item())
+            nbk = int(self.rigid_contact_broken_count.item())
+            out["new_indices"] = self.rigid_contact_new_indices[:nn].cpu().numpy()
+            out["broken_indices"] = self.rigid_contact_broken_indices[:nbk].cpu().numpy()
         return out
 
 
-__all__ = ["Contacts"]
-_ = ctypes
+__all__ = ["Contacts", "MATCH_NOT_FOUND", "MATCH_BROKEN"]

@@ -368,6 +368,63 @@ def solid_angle_f64(a, b, c) -> np.ndarray:
     return out
 
 
+class OracleMatcher:
+    """The restatement's frame-to-frame contact matcher (contact_match.py), fed key-sorted contacts."""
+
+    def __init__(self, oracle: "Oracle", pos_threshold: float = 0.0005, normal_dot_threshold: float = 0.995,
+                 sticky: bool = False):
+        L = lib()
+        L.oracle_matcher_create.restype = ctypes.c_void_p
+        L.oracle_matcher_destroy.argtypes = [ctypes.c_void_p]
+        L.oracle_matcher_reset.argtypes = [ctypes.c_void_p]
+        L.oracle_contact_match.restype = ctypes.c_int
+        self.L = L
+        self.oracle = oracle
+        self.sticky = sticky
+        self.h = ctypes.c_void_p(L.oracle_matcher_create(ctypes.c_float(pos_threshold),
+                                                         ctypes.c_float(normal_dot_threshold), int(sticky)))
+
+    def reset(self):
+        self.L.oracle_matcher_reset(self.h)
+
+    def match(self, body_q: np.ndarray, contacts: dict) -> dict:
+        """Returns {match_index, matched_key, broken_keys} and, when sticky, the replayed contact fields."""
+        order = np.argsort(contacts["sort_key"], kind="stable")
+        n = len(order)
+        keys = np.ascontiguousarray(contacts["sort_key"][order], np.int64)
+        s0 = np.ascontiguousarray(contacts["shape0"][order], np.int32)
+        s1 = np.ascontiguousarray(contacts["shape1"][order], np.int32)
+        f = {k: np.ascontiguousarray(np.asarray(contacts[k], np.float32).reshape(-1, 3)[order])
+             for k in ("point0", "point1", "offset0", "offset1", "normal")}
+        q = np.ascontiguousarray(body_q, np.float32)
+        mi = np.zeros(max(n, 1), np.int32)
+        mk = np.zeros(max(n, 1), np.int64)
+        cap = 1 << 22
+        bk = np.zeros(cap, np.int64)
+        for k in f:
+            if f[k].size == 0:
+                f[k] = np.zeros((1, 3), np.float32)
+        if keys.size == 0:
+            keys = np.zeros(1, np.int64)
+            s0 = np.zeros(1, np.int32)
+            s1 = np.zeros(1, np.int32)
+        nb = self.L.oracle_contact_match(self.h, ctypes.byref(self.oracle.m), _p(q, _f32p), ctypes.c_int(n),
+                                         _p(keys, _i64p), _p(s0, _i32p), _p(s1, _i32p), _p(f["point0"], _f32p),
+                                         _p(f["point1"], _f32p), _p(f["offset0"], _f32p), _p(f["offset1"], _f32p),
+                                         _p(f["normal"], _f32p), _p(mi, _i32p), _p(mk, _i64p), _p(bk, _i64p),
+                                         ctypes.c_int(cap))
+        out = {"sort_key": keys[:n], "match_index": mi[:n], "matched_key": mk[:n], "broken_keys": bk[:min(nb, cap)]}
+        if self.sticky:
+            out.update({k: v[:n] for k, v in f.items()})
+        return out
+
+    def __del__(self):
+        try:
+            self.L.oracle_matcher_destroy(self.h)
+        except Exception:  # noqa: BLE001
+            pass
+
+
 def mlp_forward(params: dict, x: np.ndarray) -> np.ndarray:
     """The restatement's MLP forward (BF16 inputs/weights of layers 1-2, FP32 accumulate)."""
     L = lib()

@@ -6,6 +6,7 @@
 
 #include "oracle_collide.h"
 #include "oracle_mesh.h"
+#include "oracle_match.h"
 #include "oracle_xpbd.h"
 
 using namespace oracle;
@@ -264,4 +265,32 @@ extern "C" void oracle_edge_search_f32(const OracleModel* m, int mesh, const flo
     point[3 * i + 1] = pt.y;
     point[3 * i + 2] = pt.z;
   }
+}
+
+extern "C" void* oracle_matcher_create(float pos_threshold, float normal_dot_threshold, int sticky) {
+  MatcherState* S = new MatcherState();
+  S->pos_threshold_sq = pos_threshold * pos_threshold;
+  S->normal_dot_threshold = normal_dot_threshold;
+  S->sticky = sticky != 0;
+  return S;
+}
+
+extern "C" void oracle_matcher_destroy(void* h) { delete static_cast<MatcherState*>(h); }
+
+extern "C" void oracle_matcher_reset(void* h) {
+  MatcherState* S = static_cast<MatcherState*>(h);
+  S->keys.clear();
+}
+
+// Returns the number of broken previous contacts written to broken_keys (at most broken_cap).
+extern "C" int oracle_contact_match(void* h, const OracleModel* m, const float* body_q, int n, const int64_t* keys,
+                                    const int* shape0, const int* shape1, float* point0, float* point1,
+                                    float* offset0, float* offset1, float* normal, int* match_index,
+                                    int64_t* matched_key, int64_t* broken_keys, int broken_cap) {
+  std::vector<int64_t> broken;
+  contact_match(*static_cast<MatcherState*>(h), m, body_q, n, keys, shape0, shape1, point0, point1, offset0, offset1,
+                normal, match_index, matched_key, broken);
+  int k = int(broken.size());
+  for (int i = 0; i < k && i < broken_cap; ++i) broken_keys[i] = broken[i];
+  return k;
 }

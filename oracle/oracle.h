@@ -139,6 +139,19 @@ void oracle_solid_angle_f64(const double* a, const double* b, const double* c, i
 void oracle_edge_search_f32(const OracleModel* m, int mesh, const float* v0, const float* v1, int n, float* value,
                             float* point);
 
+/* Frame-to-frame contact matching (contact_match.py).  The matcher keeps the previous frame key-sorted.
+ * Contacts are passed key-sorted; point0/1, offset0/1, normal are replayed in place when sticky.  Outputs
+ * per contact the match index (into the previous key-sorted buffer, -1 not found, -2 broken) and the
+ * matched previous key (-1 when none); returns the count of previous contacts no new contact won
+ * (their keys, ascending, in broken_keys up to broken_cap). */
+void* oracle_matcher_create(float pos_threshold, float normal_dot_threshold, int sticky);
+void oracle_matcher_destroy(void* matcher);
+void oracle_matcher_reset(void* matcher);
+int oracle_contact_match(void* matcher, const OracleModel* m, const float* body_q, int n, const int64_t* keys,
+                         const int* shape0, const int* shape1, float* point0, float* point1, float* offset0,
+                         float* offset1, float* normal, int* match_index, int64_t* matched_key,
+                         int64_t* broken_keys, int broken_cap);
+
 // Elementwise FP32 asin / acos of the restatement (the revolute joint's twist and swing angles).
 void oracle_asin_f32(const float* x, int n, float* out);
 void oracle_acos_f32(const float* x, int n, float* out);
