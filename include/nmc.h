@@ -250,6 +250,29 @@ int nmc_mlp_train_grad(const NmcMlp* weights, const float* feat, const float* ta
 int nmc_sdf_build(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
                   const float lower[3], float voxel, const int cells[3], float* out, nmc_stream_t stream);
 
+/* Reported forces of the XPBD step (SolverXPBD.update_contacts, newton/_src/solvers/xpbd/solver_xpbd.py:767-821,
+ * and State.body_parent_f, :632-655).  contact_impulse (capacity x 6: lin, ang) accumulates, over the
+ * iterations, each active contact's shape-0 side delta scaled by the contact weighting: 1/N of its dynamic
+ * body, or 2/(N_a + N_b) for two (kernels.py:2301-2361); contact_scratch (capacity x 8) holds one iteration.
+ * body_parent_f (body_count x 6) receives each enabled non-FREE joint's accumulated child-side impulse / dt
+ * (joint_impulse: joint_count x 6 scratch), zero elsewhere (kernels.py:2397-2435). */
+typedef struct NmcXpbdReport {
+  float* contact_impulse; /* or null */
+  float* contact_scratch;
+  float* joint_impulse;
+  float* body_parent_f; /* or null */
+} NmcXpbdReport;
+/* nmc_xpbd_step with the reported impulses. */
+int nmc_xpbd_step_report(NmcModel* model, const NmcXpbdParams* params, const float* body_q_in,
+                         const float* body_qd_in, const float* body_f_in, const float* joint_f,
+                         const float* joint_target_pos, const float* joint_target_vel, const NmcContacts* contacts,
+                         float dt, float* body_q_out, float* body_qd_out, const NmcXpbdReport* report,
+                         nmc_stream_t stream);
+/* convert_contact_impulse_to_force (kernels.py:2364-2394): force = impulse / dt for the first *count
+ * contacts, zero for the rest of the capacity (contacts.force, 6 floats per contact). */
+int nmc_contact_force(const float* contact_impulse, const int* count, int capacity, float dt, float* force,
+                      nmc_stream_t stream);
+
 /* Frame-to-frame contact matching (CollisionPipeline(contact_matching="latest" | "sticky", contact_report),
  * newton/_src/geometry/contact_match.py:570-992 ContactMatcher; collide.py:1084-1184).  The matcher keeps the
  * previous frame's contacts (keys, world midpoints, normals; body-frame points/offsets when sticky).

@@ -94,7 +94,12 @@ NMC_STEP_PLAIN, NMC_STEP_LOG, NMC_STEP_PREP, NMC_STEP_WARM = 0, 1, 2, 3
 
 EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_step", "nmc_xpbd_step_learn",
            "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_matcher_create", "nmc_matcher_destroy",
-           "nmc_matcher_reset", "nmc_contact_match", "nmc_get_counters", "nmc_last_error")
+           "nmc_matcher_reset", "nmc_contact_match", "nmc_xpbd_step_report", "nmc_contact_force",
+           "nmc_get_counters", "nmc_last_error")
+
+
+class NmcXpbdReport(ctypes.Structure):
+    _fields_ = [("contact_impulse", _vp), ("contact_scratch", _vp), ("joint_impulse", _vp), ("body_parent_f", _vp)]
 
 
 class NmcContactReport(ctypes.Structure):
@@ -148,6 +153,11 @@ def load() -> ctypes.CDLL:
         L.nmc_contact_match.argtypes = [_vp, _vp, _vp, ctypes.POINTER(NmcContacts), _vp,
                                         ctypes.POINTER(NmcContactReport), _vp]
         L.nmc_contact_match.restype = ctypes.c_int
+        L.nmc_xpbd_step_report.argtypes = [_vp, ctypes.POINTER(NmcXpbdParams)] + [_vp] * 6 + \
+            [ctypes.POINTER(NmcContacts), ctypes.c_float, _vp, _vp, ctypes.POINTER(NmcXpbdReport), _vp]
+        L.nmc_xpbd_step_report.restype = ctypes.c_int
+        L.nmc_contact_force.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_float, _vp, _vp]
+        L.nmc_contact_force.restype = ctypes.c_int
         L.nmc_get_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.nmc_get_counters.restype = ctypes.c_int
         L.nmc_last_error.argtypes = []

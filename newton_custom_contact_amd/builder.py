@@ -652,8 +652,21 @@ class ModelBuilder:
                               scale=(radius, half_height, 0.0), label=label)
 
     # ------------------------------------------------------------------ finalize
+    def request_state_attributes(self, *attributes: str) -> None:
+        """builder.py: optional State arrays the finalized model allocates (``body_parent_f``)."""
+        self._requested_state_attributes = getattr(self, "_requested_state_attributes", set()) | set(attributes)
+
+    def request_contact_attributes(self, *attributes: str) -> None:
+        """builder.py: optional Contacts arrays (``force``)."""
+        self._requested_contact_attributes = getattr(self, "_requested_contact_attributes", set()) | set(attributes)
+
     def finalize(self, device=None, requires_grad: bool = False):
         """builder.py:9745: upload the SoA arrays.  ``device`` is a torch device string."""
         from .model import Model  # noqa: PLC0415
 
-        return Model._from_builder(self, device)
+        m = Model._from_builder(self, device)
+        if getattr(self, "_requested_state_attributes", None):
+            m.request_state_attributes(*self._requested_state_attributes)
+        if getattr(self, "_requested_contact_attributes", None):
+            m.request_contact_attributes(*self._requested_contact_attributes)
+        return m

@@ -17,7 +17,7 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
 hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
                        const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
                        const NmcContacts* c, float dt, float* q_out, float* qd_out, const NmcLearnIO* lrn,
-                       hipStream_t stream);
+                       const NmcXpbdReport* rep, hipStream_t stream);
 
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
@@ -264,7 +264,7 @@ extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q
     return fail("nmc_xpbd_step: contacts capacity exceeds the rigid_contact_max the model was created with");
   if (!valid_lanes(m, p)) return fail("nmc_xpbd_step: block_lanes must be 0, 256, or 64 (worlds of up to 64 bodies)");
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
-                             qd_out, nullptr, reinterpret_cast<hipStream_t>(stream));
+                             qd_out, nullptr, nullptr, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step launch");
   return 0;
 }
@@ -299,9 +299,59 @@ extern "C" int nmc_xpbd_step_learn(NmcModel* m, const NmcXpbdParams* p, const fl
   if (lrn->prev_key && (!lrn->prev_start || !lrn->prev_count || !lrn->prev_lambda))
     return fail("nmc_xpbd_step_learn: prev_key given without prev_start / prev_count / prev_lambda");
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
-                             qd_out, lrn, reinterpret_cast<hipStream_t>(stream));
+                             qd_out, lrn, nullptr, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step_learn launch");
   return 0;
+}
+
+extern "C" int nmc_xpbd_step_report(NmcModel* m, const NmcXpbdParams* p, const float* q_in, const float* qd_in,
+                                    const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
+                                    const NmcContacts* c, float dt, float* q_out, float* qd_out,
+                                    const NmcXpbdReport* rep, nmc_stream_t stream) {
+  if (!m || !p || !rep) return fail("nmc_xpbd_step_report: null argument");
+  if (m->d.body_count > 0 && (!q_in || !qd_in || !f_in || !q_out || !qd_out))
+    return fail("nmc_xpbd_step_report: body state array is null");
+  if (m->d.joint_count > 0 && (!joint_f || !tpos || !tvel)) return fail("nmc_xpbd_step_report: joint array is null");
+  if (p->has_contacts && !c) return fail("nmc_xpbd_step_report: has_contacts set but contacts is null");
+  if (q_in == q_out || qd_in == qd_out) return fail("nmc_xpbd_step_report: state_in and state_out must not alias");
+  if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
+    return fail("nmc_xpbd_step_report: contacts capacity exceeds the rigid_contact_max the model was created with");
+  if (!valid_lanes(m, p))
+    return fail("nmc_xpbd_step_report: block_lanes must be 0, 256, or 64 (worlds of up to 64 bodies)");
+  if (rep->contact_impulse && !rep->contact_scratch)
+    return fail("nmc_xpbd_step_report: contact_impulse needs contact_scratch");
+  if (rep->body_parent_f && m->d.joint_count > 0 && !rep->joint_impulse)
+    return fail("nmc_xpbd_step_report: body_parent_f needs joint_impulse");
+  NmcXpbdReport r = *rep;
+  if (!p->has_contacts) r.contact_impulse = nullptr;
+  hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
+                             qd_out, nullptr, &r, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step_report launch");
+  return 0;
+}
+
+namespace nmc {
+__global__ void __launch_bounds__(256) contact_force_kernel(const float* __restrict__ impulse,
+                                                           const int* __restrict__ count, int capacity, float dt,
+                                                           float* __restrict__ force) {
+  const int n = count[0];
+  const float inv_dt = 1.0f / dt;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < capacity; i += gridDim.x * blockDim.x) {
+    for (int k = 0; k < 6; ++k) force[6ll * i + k] = i < n ? impulse[6ll * i + k] * inv_dt : 0.0f;
+  }
+}
+}  // namespace nmc
+
+extern "C" int nmc_contact_force(const float* contact_impulse, const int* count, int capacity, float dt, float* force,
+                                 nmc_stream_t stream) {
+  if (!contact_impulse || !count || !force) return fail("nmc_contact_force: null argument");
+  if (capacity <= 0) return 0;
+  long long blocks = (capacity + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(contact_force_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), contact_impulse, count, capacity, dt, force);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : hip_fail(e, "nmc_contact_force launch");
 }
 
 extern "C" int nmc_get_counters(NmcModel* m, int* out, nmc_stream_t stream) {

@@ -435,10 +435,12 @@ struct XpbdArgs {
   long long keys_global_cap;
   int* counters;
   NmcLearnIO lrn;  // learned warm start I/O (all null / 0 for the plain step)
+  NmcXpbdReport rep;  // XM_REPORT: contact impulses, joint impulses, body_parent_f
 };
 
-// Solver modes (template parameter of the world kernel).
-enum { XM_PLAIN = 0, XM_LOG = 1, XM_PREP = 2, XM_WARM = 3 };
+// Solver modes (template parameter of the world kernel).  XM_REPORT is the plain step that also
+// accumulates the reported contact and joint impulses.
+enum { XM_PLAIN = 0, XM_LOG = 1, XM_PREP = 2, XM_WARM = 3, XM_REPORT = 4 };
 
 // Per-contact data that is constant for the whole step (solve_body_contact_positions inputs).
 struct CReg {
@@ -657,6 +659,12 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   if (tid == 0) {
     s_articulated = 0;
     s_fallback = 0;
+  }
+  if (MODE == XM_REPORT) {
+    if (a.rep.contact_impulse)
+      for (int k = tid; k < 6 * nc; k += BS) a.rep.contact_impulse[6ll * cstart + k] = 0.0f;
+    if (a.rep.body_parent_f)
+      for (int k = tid; k < 6 * nj; k += BS) a.rep.joint_impulse[6ll * j0 + k] = 0.0f;
   }
 
   // ---- load (solver.py:160-175: kinematic bodies get zero effective inverse mass/inertia)
@@ -948,6 +956,15 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
                 on = r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu, r.mut,
                                               r.mur, relax, dt, da, db, lam);
               }
+              if (MODE == XM_REPORT && a.rep.contact_impulse) {
+                // this iteration's shape-0 side delta (zero when inactive) and the contact's bodies
+                float* sc = a.rep.contact_scratch + 8ll * (cstart + c);
+                const bool hit = on;
+                sc[0] = hit ? da.lin.x : 0.0f; sc[1] = hit ? da.lin.y : 0.0f; sc[2] = hit ? da.lin.z : 0.0f;
+                sc[3] = hit ? da.ang.x : 0.0f; sc[4] = hit ? da.ang.y : 0.0f; sc[5] = hit ? da.ang.z : 0.0f;
+                sc[6] = __int_as_float(hit ? r.la : -2);
+                sc[7] = __int_as_float(r.lb);
+              }
               if (on) {
                 act = 1;
                 float* o = contrib + 12 * slot;
@@ -980,6 +997,35 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
           }
           if (cb + CH < nc) __syncthreads();
         }
+        if (MODE == XM_REPORT && a.rep.contact_impulse) {
+          // accumulate_weighted_contact_impulse (kernels.py:2301-2361): per contact, this iteration's
+          // shape-0 delta scaled by 1/N of its dynamic body (harmonic mean 2/(N_a+N_b) for two)
+#pragma unroll
+          for (int r = 0; r < BPL; ++r) {
+            int lb = tid + r * BS;
+            if (lb < nb) fl[lb] = cnt[r];
+          }
+          __syncthreads();
+          for (int c = tid; c < nc; c += BS) {
+            const float* sc = a.rep.contact_scratch + 8ll * (cstart + c);
+            const int la = __float_as_int(sc[6]), lbb = __float_as_int(sc[7]);
+            if (la == -2) continue;  // inactive this iteration
+            float wgt = 1.0f;
+            if (weighted) {
+              const float na = la >= 0 ? fl[la] : 0.0f, nbc = lbb >= 0 ? fl[lbb] : 0.0f;
+              const float ns = na + nbc;
+              if (ns > 0.0f) {
+                if (na == 0.0f) wgt = 1.0f / nbc;
+                else if (nbc == 0.0f) wgt = 1.0f / na;
+                else wgt = 2.0f / ns;
+              }
+            }
+            float* acc_c = a.rep.contact_impulse + 6ll * (cstart + c);
+#pragma unroll
+            for (int k = 0; k < 6; ++k) acc_c[k] = acc_c[k] + sc[k] * wgt;
+          }
+          __syncthreads();
+        }
 #pragma unroll
         for (int r = 0; r < BPL; ++r) {
           int lb = tid + r * BS;
@@ -1010,6 +1056,11 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
             if (ok) {
               float* o = jcontrib + 12 * jl;
               store3(o, dp.lin); store3(o + 3, dp.ang); store3(o + 6, dc.lin); store3(o + 9, dc.ang);
+              if (MODE == XM_REPORT && a.rep.body_parent_f) {  // child-side impulse (kernels.py:1937-1942)
+                float* ji = a.rep.joint_impulse + 6ll * j;
+                ji[0] = ji[0] + dc.lin.x; ji[1] = ji[1] + dc.lin.y; ji[2] = ji[2] + dc.lin.z;
+                ji[3] = ji[3] + dc.ang.x; ji[4] = ji[4] + dc.ang.y; ji[5] = ji[5] + dc.ang.z;
+              }
             }
           }
           __syncthreads();
@@ -1089,6 +1140,20 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   }
   if (MODE == XM_WARM && lrn.fallback && tid == 0) lrn.fallback[w] = s_fallback;
 
+  // ---- reported parent-joint wrench (convert_joint_impulse_to_parent_f, kernels.py:2397-2435)
+  if (MODE == XM_REPORT && a.rep.body_parent_f) {
+    for (int k = tid; k < 6 * nb; k += BS) a.rep.body_parent_f[6ll * b0 + k] = 0.0f;
+    __syncthreads();
+    const float inv_dt = 1.0f / dt;
+    for (int jl = tid; jl < nj; jl += BS) {
+      const int j = j0 + jl;
+      if (!d.joint_enabled[j] || d.joint_type[j] == JT_FREE) continue;
+      const int id_c = d.joint_child[j];
+      if (id_c < 0) continue;
+      for (int k = 0; k < 6; ++k) a.rep.body_parent_f[6ll * id_c + k] = a.rep.joint_impulse[6ll * j + k] * inv_dt;
+    }
+  }
+
   // ---- store (copy_kinematic_body_state_kernel, kernels.py:18-31: kinematic bodies pass through)
   for (int lb = tid; lb < nb; lb += BS) {
     int b = b0 + lb;
@@ -1148,7 +1213,7 @@ static void launch_mode(const XpbdShape& sh, int worlds, size_t lds, hipStream_t
 hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
                        const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
                        const NmcContacts* c, float dt, float* q_out, float* qd_out, const NmcLearnIO* lrn,
-                       hipStream_t stream) {
+                       const NmcXpbdReport* rep, hipStream_t stream) {
   XpbdArgs a;
   a.d = m->d;
   a.p = *params;
@@ -1167,9 +1232,14 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   a.keys_global_cap = m->xpbd_keys_cap;
   a.counters = m->counters;
   if (lrn) a.lrn = *lrn; else a.lrn = NmcLearnIO{};
+  if (rep) a.rep = *rep; else a.rep = NmcXpbdReport{};
   int worlds = m->d.world_count + 1;
   XpbdShape sh = xpbd_shape(m->d.max_world_bodies, params->block_lanes);
   const size_t lds = xpbd_lds_bytes_shape(sh, m->d.max_world_bodies, m->d.max_world_joints, m->xpbd_contact_cap);
+  if (rep) {
+    launch_mode<XM_REPORT>(sh, worlds, lds, stream, a);
+    return hipGetLastError();
+  }
   switch (a.lrn.mode) {
     case NMC_STEP_LOG: launch_mode<XM_LOG>(sh, worlds, lds, stream, a); break;
     case NMC_STEP_PREP: launch_mode<XM_PREP>(sh, worlds, lds, stream, a); break;

@@ -92,6 +92,8 @@ class Model:
         self.particle_count = 0
         self.shape_collision_filter_pairs: set[tuple[int, int]] = set()
         self._collision_pipeline = None
+        self._requested_state_attributes: set[str] = set()
+        self._requested_contact_attributes: set[str] = set()
 
     # ------------------------------------------------------------------ construction
     @classmethod
@@ -439,7 +441,28 @@ class Model:
         s.body_f = torch.zeros_like(self.body_qd)
         s.joint_q = self.joint_q.clone()
         s.joint_qd = self.joint_qd.clone()
+        if "body_parent_f" in self._requested_state_attributes:
+            s.body_parent_f = torch.zeros_like(self.body_qd)
         return s
+
+    def request_state_attributes(self, *attributes: str) -> None:
+        """model.py:1024-1034: optional State arrays (here: ``body_parent_f``)."""
+        unknown = set(attributes) - {"body_parent_f"}
+        if unknown:
+            raise ValueError(f"unsupported state attributes: {sorted(unknown)}")
+        self._requested_state_attributes.update(attributes)
+
+    def get_requested_state_attributes(self) -> list[str]:
+        return sorted(self._requested_state_attributes)
+
+    def request_contact_attributes(self, *attributes: str) -> None:
+        """model.py:1036-1044: optional Contacts arrays (here: ``force``, filled by update_contacts)."""
+        from .contacts import Contacts  # noqa: PLC0415
+
+        unknown = set(attributes) - set(Contacts.EXTENDED_ATTRIBUTES)
+        if unknown:
+            raise ValueError(f"unsupported contact attributes: {sorted(unknown)}")
+        self._requested_contact_attributes.update(attributes)
 
     def control(self, clone_variables: bool = True) -> Control:
         """model.py:880-921."""
@@ -475,7 +498,7 @@ class Model:
         return contacts
 
     def get_requested_contact_attributes(self) -> set[str]:
-        return set()
+        return set(self._requested_contact_attributes)
 
 
 def geo_type_counts(model: Model) -> dict[str, int]:

@@ -18,7 +18,7 @@ import ctypes
 
 import torch
 
-from ._native import NativeModel, NmcXpbdParams, check, load, ptr, stream_handle
+from ._native import NativeModel, NmcXpbdParams, NmcXpbdReport, check, load, ptr, stream_handle
 
 
 class SolverNotifyFlags:
@@ -73,6 +73,12 @@ class SolverXPBD(SolverBase):
         self.native = NativeModel(model, rigid_contact_max=self._contact_capacity, shape_pairs_max=0)
         self._lib = load()
         self._control = None
+        self._impulse = None
+        self._impulse_scratch = None
+        self._joint_impulse = None
+        self._impulse_valid = False
+        self._impulse_capacity = 0
+        self._last_dt = 0.0
 
     def notify_model_changed(self, flags: int) -> None:
         """Kinematic flags and inverse masses are read on every step; nothing is cached."""
@@ -102,17 +108,54 @@ class SolverXPBD(SolverBase):
         for t in (state_in.body_q, state_in.body_qd, state_in.body_f, state_out.body_q, state_out.body_qd):
             if t.device != model.device or not t.is_contiguous():
                 raise ValueError("state arrays must be contiguous tensors on the model device")
-        rc = self._lib.nmc_xpbd_step(
-            self.native.handle, ctypes.byref(p), ctypes.c_void_p(ptr(state_in.body_q)),
-            ctypes.c_void_p(ptr(state_in.body_qd)), ctypes.c_void_p(ptr(state_in.body_f)),
-            ctypes.c_void_p(ptr(control.joint_f)), ctypes.c_void_p(ptr(control.joint_target_pos)),
-            ctypes.c_void_p(ptr(control.joint_target_vel)), cv, ctypes.c_float(dt),
-            ctypes.c_void_p(ptr(state_out.body_q)), ctypes.c_void_p(ptr(state_out.body_qd)),
-            ctypes.c_void_p(stream_handle(model.device)))
-        check(rc, "nmc_xpbd_step")
+        args = (self.native.handle, ctypes.byref(p), ctypes.c_void_p(ptr(state_in.body_q)),
+                ctypes.c_void_p(ptr(state_in.body_qd)), ctypes.c_void_p(ptr(state_in.body_f)),
+                ctypes.c_void_p(ptr(control.joint_f)), ctypes.c_void_p(ptr(control.joint_target_pos)),
+                ctypes.c_void_p(ptr(control.joint_target_vel)), cv, ctypes.c_float(dt),
+                ctypes.c_void_p(ptr(state_out.body_q)), ctypes.c_void_p(ptr(state_out.body_qd)))
+        want_force = contacts is not None and getattr(contacts, "force", None) is not None
+        want_parent = getattr(state_out, "body_parent_f", None) is not None
+        if want_force or want_parent:
+            # reported forces (solver_xpbd.py:275-296, 632-655): the step also accumulates the per-contact
+            # and per-joint impulses
+            rep = NmcXpbdReport()
+            if want_force:
+                C = contacts.rigid_contact_max
+                if self._impulse is None or self._impulse.shape[0] != C:
+                    self._impulse = torch.zeros((C, 6), device=model.device, dtype=torch.float32)
+                    self._impulse_scratch = torch.zeros((C, 8), device=model.device, dtype=torch.float32)
+                rep.contact_impulse, rep.contact_scratch = ptr(self._impulse), ptr(self._impulse_scratch)
+            if want_parent:
+                if self._joint_impulse is None:
+                    self._joint_impulse = torch.zeros((max(model.joint_count, 1), 6), device=model.device,
+                                                      dtype=torch.float32)
+                rep.joint_impulse, rep.body_parent_f = ptr(self._joint_impulse), ptr(state_out.body_parent_f)
+            check(self._lib.nmc_xpbd_step_report(*args, ctypes.byref(rep), ctypes.c_void_p(stream_handle(model.device))),
+                  "nmc_xpbd_step_report")
+            self._impulse_capacity = contacts.rigid_contact_max if want_force else 0
+            self._last_dt = float(dt)
+            self._impulse_valid = want_force
+        else:
+            check(self._lib.nmc_xpbd_step(*args, ctypes.c_void_p(stream_handle(model.device))), "nmc_xpbd_step")
+            self._impulse_valid = False
 
     def update_contacts(self, contacts, state=None) -> None:
-        raise NotImplementedError("contact force reporting (update_contacts) is not part of this build yet")
+        """solver_xpbd.py:767-821: ``contacts.force`` [N, N·m] from the contact impulses of the last step
+        (linear: normal + friction; angular: torsional + rolling), scaled for the 1/N contact weighting."""
+        if contacts.force is None:
+            raise ValueError("contacts.force is not allocated. Call model.request_contact_attributes('force') "
+                             "before creating the Contacts object.")
+        if not self._impulse_valid:
+            raise ValueError("No contact impulse data available. Call step() before update_contacts().")
+        if contacts.rigid_contact_max != self._impulse_capacity:
+            raise ValueError(f"Contacts capacity mismatch: update_contacts() received rigid_contact_max="
+                             f"{contacts.rigid_contact_max}, but step() used {self._impulse_capacity}. "
+                             "Pass the same Contacts instance to both step() and update_contacts().")
+        check(self._lib.nmc_contact_force(ctypes.c_void_p(ptr(self._impulse)),
+                                          ctypes.c_void_p(ptr(contacts.rigid_contact_count)),
+                                          int(contacts.rigid_contact_max), ctypes.c_float(self._last_dt),
+                                          ctypes.c_void_p(ptr(contacts.force)),
+                                          ctypes.c_void_p(stream_handle(self.model.device))), "nmc_contact_force")
 
 
 __all__ = ["SolverBase", "SolverNotifyFlags", "SolverXPBD"]

@@ -217,8 +217,9 @@ class Oracle:
              angular_damping: float = 0.0, joint_linear_relaxation: float = 0.7,
              joint_angular_relaxation: float = 0.4, joint_linear_compliance: float = 0.0,
              joint_angular_compliance: float = 0.0, joint_f=None, joint_target_pos=None, joint_target_vel=None,
-             precision: str = "f32"):
-        """One SolverXPBD.step; returns (body_q_out, body_qd_out)."""
+             precision: str = "f32", report: bool = False):
+        """One SolverXPBD.step; returns (body_q_out, body_qd_out), plus (contact_force (n, 6), body_parent_f
+        (B, 6)) with ``report`` (FP32 only)."""
         h = self.model.host
         dtp = np.float32 if precision == "f32" else np.float64
         tp = _f32p if precision == "f32" else _f64p
@@ -261,12 +262,20 @@ class Oracle:
                     c[k] = np.zeros(1, np.int32)
         qo = np.zeros_like(q)
         qdo = np.zeros_like(qd)
-        fn = self.lib.oracle_xpbd_step_f32 if precision == "f32" else self.lib.oracle_xpbd_step_f64
+        args = [ctypes.byref(self.m), ctypes.byref(p), _p(q, tp), _p(qd, tp), _p(f, tp), _p(jf, _f32p),
+                _p(tpos, _f32p), _p(tvel, _f32p), nc, _p(c["shape0"], _i32p), _p(c["shape1"], _i32p),
+                _p(c["point0"], tp), _p(c["point1"], tp), _p(c["offset0"], tp), _p(c["offset1"], tp),
+                _p(c["normal"], tp), _p(c["margin0"], tp), _p(c["margin1"], tp)]
         dtc = ctypes.c_float(dt) if precision == "f32" else ctypes.c_double(dt)
-        fn(ctypes.byref(self.m), ctypes.byref(p), _p(q, tp), _p(qd, tp), _p(f, tp), _p(jf, _f32p), _p(tpos, _f32p),
-           _p(tvel, _f32p), nc, _p(c["shape0"], _i32p), _p(c["shape1"], _i32p), _p(c["point0"], tp),
-           _p(c["point1"], tp), _p(c["offset0"], tp), _p(c["offset1"], tp), _p(c["normal"], tp), _p(c["margin0"], tp),
-           _p(c["margin1"], tp), dtc, _p(qo, tp), _p(qdo, tp))
+        if report:
+            assert precision == "f32"
+            cf = np.zeros((max(nc, 1), 6), np.float32)
+            pf = np.zeros((max(len(q), 1), 6), np.float32)
+            self.lib.oracle_xpbd_step_report_f32.restype = None
+            self.lib.oracle_xpbd_step_report_f32(*args, dtc, _p(qo, tp), _p(qdo, tp), _p(cf, _f32p), _p(pf, _f32p))
+            return qo, qdo, cf[:max(nc, 0)], pf[:len(q)]
+        fn = self.lib.oracle_xpbd_step_f32 if precision == "f32" else self.lib.oracle_xpbd_step_f64
+        fn(*args, dtc, _p(qo, tp), _p(qdo, tp))
         return qo, qdo
 
     def step_learn(self, body_q, body_qd, body_f, contacts: dict, dt: float, mode: int, iterations: int = 10,

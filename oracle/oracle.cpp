@@ -66,7 +66,8 @@ template <class T, class S>
 static void step_impl(const OracleModel* m, const OracleXpbdParams* p, const S* q_in, const S* qd_in, const S* f_in,
                       const float* joint_f, const float* tpos, const float* tvel, int nc, const int* shape0,
                       const int* shape1, const S* point0, const S* point1, const S* offset0, const S* offset1,
-                      const S* normal, const S* margin0, const S* margin1, S dt, S* q_out, S* qd_out) {
+                      const S* normal, const S* margin0, const S* margin1, S dt, S* q_out, S* qd_out,
+                      S* contact_force = nullptr, S* body_parent_f = nullptr) {
   int B = m->body_count;
   std::vector<Body<T>> state(B);
   std::vector<Spatial<T>> f(B);
@@ -90,7 +91,14 @@ static void step_impl(const OracleModel* m, const OracleXpbdParams* p, const S* 
     cs[i].m0 = T(margin0[i]);
     cs[i].m1 = T(margin1[i]);
   }
-  xpbd_step<T>(m, p, state, f, joint_f, tpos, tvel, cs, nc >= 0, T(dt));
+  std::vector<Spatial<T>> cf(contact_force ? cs.size() : 0), pf(body_parent_f ? B : 0);
+  xpbd_step<T>(m, p, state, f, joint_f, tpos, tvel, cs, nc >= 0, T(dt), contact_force ? cf.data() : nullptr,
+               body_parent_f ? pf.data() : nullptr);
+  auto put6 = [](S* o, const Spatial<T>& v) {
+    o[0] = S(v.lin.x); o[1] = S(v.lin.y); o[2] = S(v.lin.z); o[3] = S(v.ang.x); o[4] = S(v.ang.y); o[5] = S(v.ang.z);
+  };
+  for (size_t i = 0; i < cf.size(); ++i) put6(contact_force + 6 * i, cf[i]);
+  for (size_t b = 0; b < pf.size(); ++b) put6(body_parent_f + 6 * b, pf[b]);
   for (int b = 0; b < B; ++b) {
     const Body<T>& s = state[b];
     S* q = q_out + 7 * b;
@@ -110,6 +118,17 @@ extern "C" void oracle_xpbd_step_f32(const OracleModel* m, const OracleXpbdParam
                                      const float* margin1, float dt, float* q_out, float* qd_out) {
   step_impl<float, float>(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, nc, shape0, shape1, point0, point1, offset0,
                           offset1, normal, margin0, margin1, dt, q_out, qd_out);
+}
+
+extern "C" void oracle_xpbd_step_report_f32(const OracleModel* m, const OracleXpbdParams* p, const float* q_in,
+                                            const float* qd_in, const float* f_in, const float* joint_f,
+                                            const float* tpos, const float* tvel, int nc, const int* shape0,
+                                            const int* shape1, const float* point0, const float* point1,
+                                            const float* offset0, const float* offset1, const float* normal,
+                                            const float* margin0, const float* margin1, float dt, float* q_out,
+                                            float* qd_out, float* contact_force, float* body_parent_f) {
+  step_impl<float, float>(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, nc, shape0, shape1, point0, point1, offset0,
+                          offset1, normal, margin0, margin1, dt, q_out, qd_out, contact_force, body_parent_f);
 }
 
 extern "C" void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const double* q_in,

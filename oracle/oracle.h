@@ -110,6 +110,15 @@ void oracle_xpbd_step_f32(const OracleModel* m, const OracleXpbdParams* p, const
                           const int* shape0, const int* shape1, const float* point0, const float* point1,
                           const float* offset0, const float* offset1, const float* normal, const float* margin0,
                           const float* margin1, float dt, float* body_q_out, float* body_qd_out);
+/* The same FP32 step, also writing the contact forces (update_contacts, per contact lin + ang, 6 floats;
+ * solver_xpbd.py:767-821) and body_parent_f (per body, 6 floats; solver_xpbd.py:636-655). */
+void oracle_xpbd_step_report_f32(const OracleModel* m, const OracleXpbdParams* p, const float* body_q_in,
+                                 const float* body_qd_in, const float* body_f_in, const float* joint_f,
+                                 const float* joint_target_pos, const float* joint_target_vel, int contact_count,
+                                 const int* shape0, const int* shape1, const float* point0, const float* point1,
+                                 const float* offset0, const float* offset1, const float* normal,
+                                 const float* margin0, const float* margin1, float dt, float* body_q_out,
+                                 float* body_qd_out, float* contact_force, float* body_parent_f);
 void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const double* body_q_in,
                           const double* body_qd_in, const double* body_f_in, const float* joint_f,
                           const float* joint_target_pos, const float* joint_target_vel, int contact_count,

@@ -542,14 +542,21 @@ void integrate_all(const XModel<T>& X, const OracleModel* m, const OracleXpbdPar
 
 // `iterations` x (contact solve + weighted apply, joint solve + apply) (solver_xpbd.py:518-650).  When
 // lambda_sum is given, every contact's normal multiplier is added to it on each iteration it is active.
+// contact_impulse (per contact) accumulates each iteration's shape-0 side delta scaled by the contact
+// weighting (accumulate_weighted_contact_impulse, kernels.py:2301-2361); joint_impulse (per joint) the
+// child side delta (kernels.py:1937-1942).
 template <class T>
 void iterate(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p, std::vector<Body<T>>& state,
              const float* tpos, const float* tvel, const std::vector<ContactIn<T>>& contacts, bool has_contacts,
-             T dt, int iterations, T* lambda_sum) {
+             T dt, int iterations, T* lambda_sum, Spatial<T>* contact_impulse = nullptr,
+             Spatial<T>* joint_impulse = nullptr) {
   int B = m->body_count;
   V3<T> zero{T(0), T(0), T(0)};
   std::vector<Spatial<T>> deltas(B);
   std::vector<T> weights(B);
+  std::vector<Spatial<T>> iter_imp(contact_impulse ? contacts.size() : 0);
+  std::vector<char> iter_on(contact_impulse ? contacts.size() : 0);
+  std::vector<int> iter_ba(contact_impulse ? contacts.size() : 0), iter_bb(contact_impulse ? contacts.size() : 0);
   for (int it = 0; it < iterations; ++it) {
     for (int b = 0; b < B; ++b) { deltas[b] = Spatial<T>{zero, zero}; weights[b] = T(0); }
     if (B && has_contacts) {
@@ -558,7 +565,14 @@ void iterate(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p
         int ba, bb;
         Spatial<T> da, db;
         T lam = T(0);
+        if (contact_impulse) iter_on[ci] = 0;
         if (!contact_solve(X, state, c, T(p->rigid_contact_relaxation), dt, ba, bb, da, db, &lam)) continue;
+        if (contact_impulse) {
+          iter_on[ci] = 1;
+          iter_imp[ci] = da;
+          iter_ba[ci] = ba;
+          iter_bb[ci] = bb;
+        }
         if (lambda_sum) lambda_sum[ci] += lam;
         if (p->rigid_contact_con_weighting) {
           if (ba >= 0) weights[ba] += T(1);
@@ -566,6 +580,24 @@ void iterate(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p
         }
         if (ba >= 0) { deltas[ba].lin = deltas[ba].lin + da.lin; deltas[ba].ang = deltas[ba].ang + da.ang; }
         if (bb >= 0) { deltas[bb].lin = deltas[bb].lin + db.lin; deltas[bb].ang = deltas[bb].ang + db.ang; }
+      }
+      if (contact_impulse) {
+        for (size_t ci = 0; ci < contacts.size(); ++ci) {
+          if (!iter_on[ci]) continue;
+          T w = T(1);
+          if (p->rigid_contact_con_weighting) {
+            T na = iter_ba[ci] >= 0 ? weights[iter_ba[ci]] : T(0);
+            T nb = iter_bb[ci] >= 0 ? weights[iter_bb[ci]] : T(0);
+            T ns = na + nb;
+            if (ns > T(0)) {
+              if (na == T(0)) w = T(1) / nb;
+              else if (nb == T(0)) w = T(1) / na;
+              else w = T(2) / ns;
+            }
+          }
+          contact_impulse[ci].lin = contact_impulse[ci].lin + iter_imp[ci].lin * w;
+          contact_impulse[ci].ang = contact_impulse[ci].ang + iter_imp[ci].ang * w;
+        }
       }
       std::vector<Body<T>> next(B);
       for (int b = 0; b < B; ++b)
@@ -581,6 +613,10 @@ void iterate(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p
                          T(p->joint_angular_relaxation), T(p->joint_linear_relaxation), dt, dp, dc))
           continue;
         int id_p = m->joint_parent[j], id_c = m->joint_child[j];
+        if (joint_impulse) {
+          joint_impulse[j].lin = joint_impulse[j].lin + dc.lin;
+          joint_impulse[j].ang = joint_impulse[j].ang + dc.ang;
+        }
         if (id_p >= 0) { deltas[id_p].lin = deltas[id_p].lin + dp.lin; deltas[id_p].ang = deltas[id_p].ang + dp.ang; }
         if (id_c >= 0) { deltas[id_c].lin = deltas[id_c].lin + dc.lin; deltas[id_c].ang = deltas[id_c].ang + dc.ang; }
       }
@@ -592,14 +628,34 @@ void iterate(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p
   }
 }
 
+// contact_force (per contact, lin + ang) and body_parent_f (per body): update_contacts and the reported
+// parent-joint wrench (solver_xpbd.py:632-655, 767-821; kernels.py:2364-2435), when given.
 template <class T>
 void xpbd_step(const OracleModel* m, const OracleXpbdParams* p, std::vector<Body<T>>& state,
                std::vector<Spatial<T>> body_f, const float* joint_f, const float* tpos, const float* tvel,
-               const std::vector<ContactIn<T>>& contacts, bool has_contacts, T dt) {
+               const std::vector<ContactIn<T>>& contacts, bool has_contacts, T dt,
+               Spatial<T>* contact_force = nullptr, Spatial<T>* body_parent_f = nullptr) {
   XModel<T> X = make_xmodel<T>(m);
   std::vector<Body<T>> in = state;
+  V3<T> zero{T(0), T(0), T(0)};
+  std::vector<Spatial<T>> cimp(contact_force ? contacts.size() : 0, Spatial<T>{zero, zero});
+  std::vector<Spatial<T>> jimp(body_parent_f ? m->joint_count : 0, Spatial<T>{zero, zero});
   integrate_all(X, m, p, state, body_f, joint_f, dt);
-  iterate(X, m, p, state, tpos, tvel, contacts, has_contacts, dt, p->iterations, static_cast<T*>(nullptr));
+  iterate(X, m, p, state, tpos, tvel, contacts, has_contacts, dt, p->iterations, static_cast<T*>(nullptr),
+          contact_force ? cimp.data() : nullptr, body_parent_f && m->joint_count ? jimp.data() : nullptr);
+  const T inv_dt = T(1) / dt;
+  if (contact_force)
+    for (size_t ci = 0; ci < contacts.size(); ++ci)
+      contact_force[ci] = Spatial<T>{cimp[ci].lin * inv_dt, cimp[ci].ang * inv_dt};
+  if (body_parent_f) {
+    for (int b = 0; b < m->body_count; ++b) body_parent_f[b] = Spatial<T>{zero, zero};
+    for (int j = 0; j < m->joint_count; ++j) {
+      if (!m->joint_enabled[j] || m->joint_type[j] == 4 /* FREE */) continue;
+      int id_c = m->joint_child[j];
+      if (id_c < 0) continue;
+      body_parent_f[id_c] = Spatial<T>{jimp[j].lin * inv_dt, jimp[j].ang * inv_dt};
+    }
+  }
   for (int b = 0; b < m->body_count; ++b)
     if (m->body_flags[b] & KINEMATIC) state[b] = in[b];
 }
