@@ -96,7 +96,9 @@ typedef struct NmcModelDesc {
   /* triangle meshes (GeoType MESH, newton/_src/geometry/types.py Mesh): shape_mesh[s] = mesh index or -1;
    * local AABB of each shape with its scale baked in (collide.py:215-245); mesh vertices and unique edges
    * (builder.py:10316-10351, edge order of Mesh.edges); one dense FP32 SDF grid per mesh: (cells+1)
-   * vertices per axis from `lower` with spacing `voxel`, x fastest, negative inside. */
+   * vertices per axis from `lower` with spacing `voxel`, x fastest, negative inside; mesh_sdf_flags bit 0
+   * set when the shape scale was multiplied into the vertices before the cook (TextureSDFData.scale_baked,
+   * sdf_contact.py:918-930: the SDF side is then sampled with unit scale). */
   const int* shape_mesh;
   const float* shape_aabb_lower; /* S x 3 */
   const float* shape_aabb_upper; /* S x 3 */
@@ -110,6 +112,7 @@ typedef struct NmcModelDesc {
   const float* mesh_sdf_lower;   /* M x 3 */
   const float* mesh_sdf_voxel;   /* M */
   const int* mesh_sdf_cells;     /* M x 3 */
+  const int* mesh_sdf_flags;     /* M */
   int mesh_contact_max;          /* contacts kept per mesh pair (deepest first), at most 255 */
 } NmcModelDesc;
 
@@ -245,10 +248,14 @@ int nmc_mlp_train_grad(const NmcMlp* weights, const float* feat, const float* ta
 
 /* Dense signed distance grid of one triangle mesh (the cook of Mesh.build_sdf, sdf_texture.py:959-1370):
  * grid vertex (ix, iy, iz) at lower + voxel * (ix, iy, iz) gets the distance to the nearest triangle,
- * negative where the generalized winding number exceeds 0.5.  out has (cells[2]+1)(cells[1]+1)(cells[0]+1)
- * floats, x fastest.  No counterpart as a C entry point in the reference (a Warp kernel there). */
+ * negative inside.  Inside is decided by the generalized winding number w against winding_threshold
+ * (get_distance_to_mesh, sdf_utils.py:640-651): w > threshold for threshold >= 0, and w <= threshold for a
+ * negative threshold (inward-wound meshes, -0.5 when the signed volume is negative, sdf_utils.py:506-510).
+ * out has (cells[2]+1)(cells[1]+1)(cells[0]+1) floats, x fastest.  No counterpart as a C entry point in
+ * the reference (a Warp kernel there). */
 int nmc_sdf_build(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                  const float lower[3], float voxel, const int cells[3], float* out, nmc_stream_t stream);
+                  const float lower[3], float voxel, const int cells[3], float winding_threshold, float* out,
+                  nmc_stream_t stream);
 
 /* Reported forces of the XPBD step (SolverXPBD.update_contacts, newton/_src/solvers/xpbd/solver_xpbd.py:767-821,
  * and State.body_parent_f, :632-655).  contact_impulse (capacity x 6: lin, ang) accumulates, over the

@@ -44,7 +44,7 @@ class NmcModelDesc(ctypes.Structure):
         ("shape_mesh", _vp), ("shape_aabb_lower", _vp), ("shape_aabb_upper", _vp), ("mesh_count", ctypes.c_int),
         ("mesh_vertex", _vp), ("mesh_vertex_start", _vp), ("mesh_edge", _vp), ("mesh_edge_start", _vp),
         ("mesh_sdf", _vp), ("mesh_sdf_start", _vp), ("mesh_sdf_lower", _vp), ("mesh_sdf_voxel", _vp),
-        ("mesh_sdf_cells", _vp), ("mesh_contact_max", ctypes.c_int),
+        ("mesh_sdf_cells", _vp), ("mesh_sdf_flags", _vp), ("mesh_contact_max", ctypes.c_int),
     ]
 
 
@@ -141,7 +141,7 @@ def load() -> ctypes.CDLL:
                                          ctypes.POINTER(NmcMlpTrainBuffers), ctypes.POINTER(NmcMlpGrads), _vp]
         L.nmc_mlp_train_grad.restype = ctypes.c_int
         L.nmc_sdf_build.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
-                                    ctypes.c_float, ctypes.POINTER(ctypes.c_int), _vp, _vp]
+                                    ctypes.c_float, ctypes.POINTER(ctypes.c_int), ctypes.c_float, _vp, _vp]
         L.nmc_sdf_build.restype = ctypes.c_int
         L.nmc_matcher_create.argtypes = [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                          ctypes.c_int, ctypes.POINTER(_vp)]
@@ -306,6 +306,7 @@ class NativeModel:
         d.mesh_sdf, d.mesh_sdf_start = ptr(m.mesh_sdf), ptr(m.mesh_sdf_start)
         d.mesh_sdf_lower, d.mesh_sdf_voxel, d.mesh_sdf_cells = (ptr(m.mesh_sdf_lower), ptr(m.mesh_sdf_voxel),
                                                                ptr(m.mesh_sdf_cells))
+        d.mesh_sdf_flags = ptr(m.mesh_sdf_flags)
         d.mesh_contact_max = MESH_CONTACT_MAX
         self.desc = d
         self.model = model

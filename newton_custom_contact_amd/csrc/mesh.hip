@@ -1,7 +1,8 @@
 // Triangle-mesh contacts on gfx950: the SDF cook and the mesh narrow phase.
 //
 // sdf_build_kernel: one lane per grid vertex; distance to the nearest triangle (closest point on each
-// triangle), sign from the generalized winding number (sum of solid angles / 4 pi > 0.5: inside).
+// triangle), sign from the generalized winding number w (sum of solid angles / 4 pi): inside when
+// w > threshold, or w <= threshold for a negative threshold (inward-wound meshes).
 //
 // mesh_narrow_kernel: one 256-lane workgroup per mesh pair (grid-stride over the call's mesh pairs).
 //   mesh (shape0) - infinite plane (shape1), narrow_phase.py:1073-1190: every mesh vertex closer to the
@@ -56,7 +57,7 @@ __device__ void mesh_bitonic(K* keys, int n2) {
 // ------------------------------------------------------------------ SDF cook
 __global__ void __launch_bounds__(256) sdf_build_kernel(const float* __restrict__ vtx, int nvtx,
                                                         const int* __restrict__ tri, int ntri, f3 lower, float voxel, int cx, int cy, int cz,
-                                                        float* __restrict__ out) {
+                                                        float threshold, float* __restrict__ out) {
   const long long nx = cx + 1, ny = cy + 1, nz = cz + 1;
   const long long total = nx * ny * nz;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
@@ -78,7 +79,8 @@ __global__ void __launch_bounds__(256) sdf_build_kernel(const float* __restrict_
       omega += solid_angle(a - p, b - p, c - p);
     }
     const float wn = omega / (4.0f * 3.14159265358979323846f);
-    out[i] = wn > 0.5f ? -best : best;
+    const bool above = wn > threshold;
+    out[i] = (threshold >= 0.0f ? above : !above) ? -best : best;
   }
 }
 
@@ -100,7 +102,9 @@ __device__ __forceinline__ EdgeSide edge_side(const NmcModelDesc& d, const Shape
                                               int sdf_mesh, float gap_sum) {
   EdgeSide e;
   e.G = sdf_grid(d, sdf_mesh);
-  e.ss = f3{fmax2(S.gscale.x, 1e-10f), fmax2(S.gscale.y, 1e-10f), fmax2(S.gscale.z, 1e-10f)};
+  // a grid cooked from scaled vertices is sampled with unit scale (sdf_contact.py:918-930)
+  const f3 s = (d.mesh_sdf_flags[sdf_mesh] & 1) ? f3{1.0f, 1.0f, 1.0f} : S.gscale;
+  e.ss = f3{fmax2(s.x, 1e-10f), fmax2(s.y, 1e-10f), fmax2(s.z, 1e-10f)};
   e.inv_s = f3{1.0f / e.ss.x, 1.0f / e.ss.y, 1.0f / e.ss.z};
   e.min_s = fmin2(fmin2(e.ss.x, e.ss.y), e.ss.z);
   e.X_ts = xmul(xinverse(S.X), T.X);
@@ -333,7 +337,8 @@ hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t 
 using namespace nmc;
 
 extern "C" int nmc_sdf_build(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                             const float lower[3], float voxel, const int cells[3], float* out, nmc_stream_t stream) {
+                             const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                             float* out, nmc_stream_t stream) {
   if (!vertices || !triangles || !lower || !cells || !out) return nmc::fail("nmc_sdf_build: null argument");
   if (vertex_count <= 0 || triangle_count <= 0) return nmc::fail("nmc_sdf_build: empty mesh");
   if (!(voxel > 0.0f) || cells[0] <= 0 || cells[1] <= 0 || cells[2] <= 0)
@@ -343,7 +348,7 @@ extern "C" int nmc_sdf_build(const float* vertices, int vertex_count, const int*
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(sdf_build_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
                      reinterpret_cast<hipStream_t>(stream), vertices, vertex_count, triangles, triangle_count,
-                     f3{lower[0], lower[1], lower[2]}, voxel, cells[0], cells[1], cells[2], out);
+                     f3{lower[0], lower[1], lower[2]}, voxel, cells[0], cells[1], cells[2], winding_threshold, out);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : nmc::hip_fail(e, "nmc_sdf_build launch");
 }

@@ -65,7 +65,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   if (d.mesh_count > 0) {
     if (!d.shape_mesh || !d.shape_aabb_lower || !d.shape_aabb_upper || !d.mesh_vertex || !d.mesh_vertex_start ||
         !d.mesh_edge || !d.mesh_edge_start || !d.mesh_sdf || !d.mesh_sdf_start || !d.mesh_sdf_lower ||
-        !d.mesh_sdf_voxel || !d.mesh_sdf_cells)
+        !d.mesh_sdf_voxel || !d.mesh_sdf_cells || !d.mesh_sdf_flags)
       return fail("a model with meshes needs every mesh table (vertices, edges, local AABBs, SDF grids)");
     if (d.mesh_contact_max < 1 || d.mesh_contact_max > 255) return fail("mesh_contact_max must be in [1, 255]");
   }

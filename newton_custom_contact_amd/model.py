@@ -216,15 +216,17 @@ class Model:
             lower = np.stack([mm.sdf.lower for mm in meshes]).astype(_F32)
             voxel = np.asarray([mm.sdf.voxel_size for mm in meshes], _F32)
             cells = np.stack([mm.sdf.cells for mm in meshes]).astype(_I32)
+            flags = np.asarray([1 if mm.sdf.scale_baked else 0 for mm in meshes], _I32)
         else:
             self.mesh_sdf = torch.zeros(1, dtype=torch.float32, device=self.device)
             starts = np.zeros(max(M, 1), np.int64)
             lower = np.zeros((max(M, 1), 3), _F32)
             voxel = np.ones(max(M, 1), _F32)
             cells = np.ones((max(M, 1), 3), _I32)
+            flags = np.zeros(max(M, 1), _I32)
         self.mesh_has_sdf = have
         for k, v in (("mesh_sdf_start", starts), ("mesh_sdf_lower", lower), ("mesh_sdf_voxel", voxel),
-                     ("mesh_sdf_cells", cells)):
+                     ("mesh_sdf_cells", cells), ("mesh_sdf_flags", flags)):
             self.host[k] = v
             setattr(self, k, torch.from_numpy(np.ascontiguousarray(v)).to(self.device))
 

@@ -43,7 +43,7 @@ class OracleModel(ctypes.Structure):
         ("shape_mesh", _i32p), ("shape_aabb_lower", _f32p), ("shape_aabb_upper", _f32p), ("mesh_count", ctypes.c_int),
         ("mesh_vertex", _f32p), ("mesh_vertex_start", _i32p), ("mesh_edge", _i32p), ("mesh_edge_start", _i32p),
         ("mesh_sdf", _f32p), ("mesh_sdf_start", _i64p), ("mesh_sdf_lower", _f32p), ("mesh_sdf_voxel", _f32p),
-        ("mesh_sdf_cells", _i32p), ("mesh_contact_max", ctypes.c_int),
+        ("mesh_sdf_cells", _i32p), ("mesh_sdf_flags", _i32p), ("mesh_contact_max", ctypes.c_int),
     ]
 
 
@@ -159,6 +159,7 @@ class Oracle:
         m.mesh_sdf_start = _p(st, _i64p)
         m.mesh_sdf_lower, m.mesh_sdf_voxel, m.mesh_sdf_cells = (f32("mesh_sdf_lower"), f32("mesh_sdf_voxel"),
                                                                 i32("mesh_sdf_cells"))
+        m.mesh_sdf_flags = i32("mesh_sdf_flags")
         m.mesh_contact_max = mesh_contact_max
         self._keep = keep
         self.m = m
@@ -329,7 +330,8 @@ class Oracle:
         return qo, qdo, feat[:n], lam[:n], res, fb
 
 
-def sdf_build(vertices, triangles, lower, voxel: float, cells, precision: str = "f32") -> np.ndarray:
+def sdf_build(vertices, triangles, lower, voxel: float, cells, precision: str = "f32",
+              winding_threshold: float = 0.5) -> np.ndarray:
     """The restatement's dense SDF cook; returns the grid as (nz + 1, ny + 1, nx + 1)."""
     L = lib()
     v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
@@ -342,7 +344,7 @@ def sdf_build(vertices, triangles, lower, voxel: float, cells, precision: str = 
     out = np.zeros(shape, dt)
     fn.restype = None
     fn(_p(v, _f32p), ctypes.c_int(len(v)), _p(t, _i32p), ctypes.c_int(len(t) // 3), _p(lo, _f32p),
-       ctypes.c_float(float(voxel)), _p(ce, _i32p), _p(out, tp))
+       ctypes.c_float(float(voxel)), _p(ce, _i32p), ctypes.c_float(float(winding_threshold)), _p(out, tp))
     return out
 
 

@@ -212,13 +212,15 @@ extern "C" void oracle_mlp_backward(const float* w1, const float* b1, const floa
 }
 
 extern "C" void oracle_sdf_build_f32(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                                     const float lower[3], float voxel, const int cells[3], float* out) {
-  sdf_cook<float>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, out);
+                                     const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                                     float* out) {
+  sdf_cook<float>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, winding_threshold, out);
 }
 
 extern "C" void oracle_sdf_build_f64(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                                     const float lower[3], float voxel, const int cells[3], double* out) {
-  sdf_cook<double>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, out);
+                                     const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                                     double* out) {
+  sdf_cook<double>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, winding_threshold, out);
 }
 
 extern "C" void oracle_sdf_sample_f32(const OracleModel* m, int mesh, const float* p, int n, float* value,

@@ -75,6 +75,7 @@ typedef struct OracleModel {
   const float* mesh_sdf_lower; /* M x 3 */
   const float* mesh_sdf_voxel;
   const int* mesh_sdf_cells; /* M x 3 */
+  const int* mesh_sdf_flags; /* M: bit 0 = scale baked into the grid */
   int mesh_contact_max;
 } OracleModel;
 
@@ -130,9 +131,11 @@ void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const
 /* Dense signed distance grid of a triangle mesh (distance to the nearest triangle, negative where the
  * winding number exceeds 0.5); out holds (cells[2]+1)(cells[1]+1)(cells[0]+1) values, x fastest. */
 void oracle_sdf_build_f32(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                          const float lower[3], float voxel, const int cells[3], float* out);
+                          const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                          float* out);
 void oracle_sdf_build_f64(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                          const float lower[3], float voxel, const int cells[3], double* out);
+                          const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                          double* out);
 
 /* SDF value and gradient of mesh `mesh` of the model at local points p (n x 3). */
 void oracle_sdf_sample_f32(const OracleModel* m, int mesh, const float* p, int n, float* value, float* grad);

@@ -210,7 +210,7 @@ template <class T> T solid_angle(V3<T> a, V3<T> b, V3<T> c) {
 
 template <class T>
 void sdf_cook(const float* vtx, int nv, const int* tri, int nt, const float lower[3], float voxel, const int cells[3],
-              T* out) {
+              float threshold, T* out) {
   const long long nx = cells[0] + 1, ny = cells[1] + 1, nz = cells[2] + 1;
   const T pi = T(3.14159265358979323846);
   for (long long i = 0; i < nx * ny * nz; ++i) {
@@ -228,8 +228,11 @@ void sdf_cook(const float* vtx, int nv, const int* tri, int nt, const float lowe
       best = dd < best ? dd : best;
       omega += solid_angle(a - p, b - p, c - p);
     }
+    // get_distance_to_mesh (sdf_utils.py:640-651): the sign of `w > threshold`, flipped back for a
+    // negative threshold
     T wn = omega / (T(4) * pi);
-    out[i] = wn > T(0.5) ? -best : best;
+    const bool above = wn > T(threshold);
+    out[i] = (threshold >= 0.0f ? above : !above) ? -best : best;
   }
 }
 
@@ -284,7 +287,9 @@ void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int s
       const int tri_mesh = m->shape_mesh[mode == 0 ? sa : sb];
       const int sdf_mesh = m->shape_mesh[mode == 0 ? sb : sa];
       const SdfG<T> G = sdf_grid<T>(m, sdf_mesh);
-      const V3<T> ss{tmax(S.gscale.x, T(1e-10f)), tmax(S.gscale.y, T(1e-10f)), tmax(S.gscale.z, T(1e-10f))};
+      // a grid cooked from scaled vertices is sampled with unit scale (sdf_contact.py:918-930)
+      const V3<T> sc = (m->mesh_sdf_flags[sdf_mesh] & 1) ? V3<T>{T(1), T(1), T(1)} : S.gscale;
+      const V3<T> ss{tmax(sc.x, T(1e-10f)), tmax(sc.y, T(1e-10f)), tmax(sc.z, T(1e-10f))};
       const V3<T> inv_s{T(1) / ss.x, T(1) / ss.y, T(1) / ss.z};
       const T min_s = tmin(tmin(ss.x, ss.y), ss.z);
       const Xf<T> X_ts = xf_mul(xf_inverse(S.X), Tr.X);

@@ -328,3 +328,56 @@ def test_mesh_contacts_keep_k_deepest():
     assert mm.any()
     sub = full["sort_key"][mm] & 0x7FFFFF
     assert np.all(sub & 1 == 0)
+
+
+# ---------------------------------------------------------------- winding threshold and baked scale
+def test_inward_winding_threshold():
+    """An inward-wound mesh has negative signed volume and cooks, with threshold -0.5
+    (sdf_utils.py:506-510, 640-651), to the outward mesh's grid; with +0.5 it would be outside everywhere."""
+    m = box_mesh(0.3, 0.2, 0.1)
+    inv = Mesh(m.vertices, m.indices.reshape(-1, 3)[:, [0, 2, 1]].reshape(-1))
+    np.testing.assert_allclose(m.signed_volume(), 0.6 * 0.4 * 0.2, rtol=1e-6)
+    np.testing.assert_allclose(inv.signed_volume(), -0.6 * 0.4 * 0.2, rtol=1e-6)
+    lo, vx, cells = m.sdf_grid_geometry(16, 0.05)
+    g = oracle.sdf_build(m.vertices, m.indices, lo, vx, cells)
+    gi = oracle.sdf_build(inv.vertices, inv.indices, lo, vx, cells, winding_threshold=-0.5)
+    wrong = oracle.sdf_build(inv.vertices, inv.indices, lo, vx, cells, winding_threshold=0.5)
+    np.testing.assert_allclose(gi, g, rtol=0, atol=1e-7)
+    assert np.array_equal(np.sign(gi[np.abs(g) > 1e-6]), np.sign(g[np.abs(g) > 1e-6]))
+    assert (g < 0).sum() > 0 and np.all(wrong >= 0)
+    cook_cpu(inv, 16)
+    np.testing.assert_allclose(inv.sdf.grid.numpy(), g, rtol=0, atol=1e-7)
+
+
+def _stacked_boxes(baked: bool):
+    """Box A (0.2, 0.2, 0.05) at shape scale (1, 1, 2) -> half extents (0.2, 0.2, 0.1), static at the
+    origin; unit-scale box B (0.1) on a body at z = 0.19, 1 cm into A's top face."""
+    a = box_mesh(0.2, 0.2, 0.05)
+    cook_cpu(a, 16, scale=(1.0, 1.0, 2.0) if baked else None)
+    bm = cook_cpu(box_mesh(0.1, 0.1, 0.1), 16)
+    b = ModelBuilder()
+    cfg = ModelBuilder.ShapeConfig()
+    cfg.gap = 0.005
+    b.add_shape_mesh(-1, mesh=a, scale=(1.0, 1.0, 2.0), cfg=cfg)
+    body = b.add_body(xform=(0.0, 0.0, 0.19, 0.0, 0.0, 0.0, 1.0))
+    b.add_shape_mesh(body, mesh=bm, cfg=cfg)
+    model = b.finalize("cpu")
+    assert model.host["mesh_sdf_flags"][0] == (1 if baked else 0)
+    return model
+
+
+def test_baked_scale_sdf_depth():
+    """Edges of B in the SDF of A (mode 1): with the scale baked into A's grid the depth is the true 1 cm;
+    sampled through the unit grid it is the conservative min-scale estimate, 0.5 cm (sdf_contact.py:51-82,
+    918-930)."""
+    for baked, depth in ((True, -0.01), (False, -0.005)):
+        model = _stacked_boxes(baked)
+        o = oracle.Oracle(model)
+        q = model.host["body_q"]
+        c = o.collide(q)
+        sub = c["sort_key"] & 0x7FFFFF
+        mode1 = (sub >> 1) & 1 == 1
+        assert mode1.any()
+        d = _contact_depth(model, q.astype(np.float64), c)
+        np.testing.assert_allclose(d[mode1].min(), depth, atol=2e-6)
+        np.testing.assert_allclose(c["normal"][mode1][np.argmin(d[mode1])], [0.0, 0.0, 1.0], atol=1e-5)
