@@ -303,6 +303,31 @@ int nmc_matcher_reset(NmcMatcher* matcher, nmc_stream_t stream);
 int nmc_contact_match(NmcMatcher* matcher, const NmcModel* model, const float* body_q, const NmcContacts* contacts,
                       int* match_index, const NmcContactReport* report, nmc_stream_t stream);
 
+/* Standalone broad phases over caller AABBs (newton.geometry BroadPhaseSAP, BroadPhaseAllPairs,
+ * BroadPhaseExplicit; newton/_src/geometry/broad_phase_sap.py:397-677, broad_phase_nxn.py:217-472).
+ * nmc_broad_phase_create replaces the constructors: mode 0 SAP or 1 all pairs take the host index map and
+ * slice ends of precompute_world_map (broad_phase_common.py:183-300; the last slice is the dedicated
+ * world -1 segment); mode 2 explicit takes the largest pair count it will be launched with.  Scratch is
+ * allocated here, never per launch. */
+typedef struct NmcBroadPhase NmcBroadPhase;
+int nmc_broad_phase_create(int mode, const int* index_map, int index_count, const int* slice_ends, int segment_count,
+                           int explicit_pair_capacity, NmcBroadPhase** out);
+void nmc_broad_phase_destroy(NmcBroadPhase* broad_phase);
+/* BroadPhaseSAP.launch / BroadPhaseAllPairs.launch (broad_phase_sap.py:510-677, broad_phase_nxn.py:305-391):
+ * lower/upper S x 3, gap S or null (AABBs already expanded), group and world S, filter_pairs F x 2 sorted
+ * (min, max) or null.  Writes candidate_pair (max x 2) and candidate_pair_count[0] = pairs found (may exceed
+ * max; the pairs past it are dropped).  Deterministic order: segment by segment; all pairs by (row, column)
+ * in index-map order, SAP by sweep origin in sorted order. */
+int nmc_broad_phase_launch(NmcBroadPhase* broad_phase, int shape_count, const float* lower, const float* upper,
+                           const float* gap, const int* group, const int* world, const int* filter_pairs,
+                           int filter_count, int* candidate_pair, int max_candidate_pair, int* candidate_pair_count,
+                           nmc_stream_t stream);
+/* BroadPhaseExplicit.launch (broad_phase_nxn.py:408-472): the given pairs whose AABBs overlap, in input
+ * order; a pair naming a shape outside [0, shape_count) is skipped. */
+int nmc_broad_phase_explicit(NmcBroadPhase* broad_phase, int shape_count, const float* lower, const float* upper,
+                             const float* gap, const int* shape_pairs, int shape_pair_count, int* candidate_pair,
+                             int max_candidate_pair, int* candidate_pair_count, nmc_stream_t stream);
+
 /* Copies the counters accumulated since model creation to host (synchronises the stream):
  * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = mesh-pair candidate
  * contacts dropped (candidate buffer full; cannot happen for models nmc_model_create accepted),

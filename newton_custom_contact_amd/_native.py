@@ -95,6 +95,7 @@ NMC_STEP_PLAIN, NMC_STEP_LOG, NMC_STEP_PREP, NMC_STEP_WARM = 0, 1, 2, 3
 EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_step", "nmc_xpbd_step_learn",
            "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_matcher_create", "nmc_matcher_destroy",
            "nmc_matcher_reset", "nmc_contact_match", "nmc_xpbd_step_report", "nmc_contact_force",
+           "nmc_broad_phase_create", "nmc_broad_phase_destroy", "nmc_broad_phase_launch", "nmc_broad_phase_explicit",
            "nmc_get_counters", "nmc_last_error")
 
 
@@ -158,6 +159,18 @@ def load() -> ctypes.CDLL:
         L.nmc_xpbd_step_report.restype = ctypes.c_int
         L.nmc_contact_force.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_float, _vp, _vp]
         L.nmc_contact_force.restype = ctypes.c_int
+        _ip = ctypes.POINTER(ctypes.c_int)
+        L.nmc_broad_phase_create.argtypes = [ctypes.c_int, _ip, ctypes.c_int, _ip, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(_vp)]
+        L.nmc_broad_phase_create.restype = ctypes.c_int
+        L.nmc_broad_phase_destroy.argtypes = [_vp]
+        L.nmc_broad_phase_destroy.restype = None
+        L.nmc_broad_phase_launch.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
+                                             ctypes.c_int, _vp, _vp]
+        L.nmc_broad_phase_launch.restype = ctypes.c_int
+        L.nmc_broad_phase_explicit.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
+                                               ctypes.c_int, _vp, _vp]
+        L.nmc_broad_phase_explicit.restype = ctypes.c_int
         L.nmc_get_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.nmc_get_counters.restype = ctypes.c_int
         L.nmc_last_error.argtypes = []

@@ -348,6 +348,39 @@ def sdf_build(vertices, triangles, lower, voxel: float, cells, precision: str = 
     return out
 
 
+def broad_phase(mode: str, lower, upper, gap, group, world, filter_pairs=None, index_map=None, slice_ends=None,
+                shape_pairs=None) -> np.ndarray:
+    """The restatement's standalone broad phase ("sap", "nxn" or "explicit"); returns the (n, 2) pairs in the
+    order one thread finds them."""
+    L = lib()
+    lo = np.ascontiguousarray(lower, np.float32).reshape(-1, 3)
+    up = np.ascontiguousarray(upper, np.float32).reshape(-1, 3)
+    ga = None if gap is None else np.ascontiguousarray(gap, np.float32)
+    gr = np.ascontiguousarray(group, np.int32)
+    wo = np.ascontiguousarray(world, np.int32)
+    fp = None if filter_pairs is None or len(filter_pairs) == 0 else np.ascontiguousarray(filter_pairs, np.int32)
+    im = np.ascontiguousarray(index_map if index_map is not None else np.zeros(1), np.int32)
+    se = np.ascontiguousarray(slice_ends if slice_ends is not None else np.zeros(1), np.int32)
+    sp = np.ascontiguousarray(shape_pairs if shape_pairs is not None else np.zeros((1, 2)), np.int32)
+    m = {"sap": 0, "nxn": 1, "explicit": 2}[mode]
+    fn = L.oracle_broad_phase
+    fn.restype = ctypes.c_int
+
+    def run(cap):
+        out = np.zeros((max(cap, 1), 2), np.int32)
+        n = fn(ctypes.c_int(m), _p(lo, _f32p), _p(up, _f32p), None if ga is None else _p(ga, _f32p), _p(gr, _i32p),
+               _p(wo, _i32p), None if fp is None else _p(fp, _i32p), ctypes.c_int(0 if fp is None else len(fp)),
+               _p(im, _i32p), _p(se, _i32p), ctypes.c_int(0 if slice_ends is None else len(slice_ends)),
+               _p(sp, _i32p), ctypes.c_int(0 if shape_pairs is None else len(shape_pairs)), _p(out, _i32p),
+               ctypes.c_int(cap))
+        return n, out
+
+    n, out = run(1024)
+    if n > 1024:
+        n, out = run(n)
+    return out[:n].copy()
+
+
 def mesh_distance_f64(vertices, triangles, points) -> np.ndarray:
     """FP64 signed distance of points to a triangle mesh (nearest triangle, winding-number sign)."""
     L = lib()

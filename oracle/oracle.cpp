@@ -4,6 +4,7 @@
 
 #include <vector>
 
+#include "oracle_broad.h"
 #include "oracle_collide.h"
 #include "oracle_mesh.h"
 #include "oracle_match.h"
@@ -314,4 +315,21 @@ extern "C" int oracle_contact_match(void* h, const OracleModel* m, const float* 
   int k = int(broken.size());
   for (int i = 0; i < k && i < broken_cap; ++i) broken_keys[i] = broken[i];
   return k;
+}
+
+extern "C" int oracle_broad_phase(int mode, const float* lower, const float* upper, const float* gap, const int* group,
+                                  const int* world, const int* filter, int filter_count, const int* index_map,
+                                  const int* slice_ends, int segments, const int* pairs, int pair_count, int* out,
+                                  int capacity) {
+  BroadInput in{lower, upper, gap, group, world, filter, filter_count};
+  std::vector<int> res;
+  if (mode == 0) broad_sap(in, index_map, slice_ends, segments, res);
+  else if (mode == 1) broad_all_pairs(in, index_map, slice_ends, segments, res);
+  else broad_explicit(in, pairs, pair_count, res);
+  const int found = int(res.size() / 2);
+  for (int k = 0; k < found && k < capacity; ++k) {
+    out[2 * k] = res[2 * k];
+    out[2 * k + 1] = res[2 * k + 1];
+  }
+  return found;
 }

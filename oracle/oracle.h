@@ -164,6 +164,13 @@ int oracle_contact_match(void* matcher, const OracleModel* m, const float* body_
                          float* offset1, float* normal, int* match_index, int64_t* matched_key,
                          int64_t* broken_keys, int broken_cap);
 
+/* Standalone broad phases over given AABBs (broad_phase_sap.py / broad_phase_nxn.py): mode 0 SAP, 1 all
+ * pairs (both over the index map / slice ends of precompute_world_map), 2 explicit (pairs, pair_count).
+ * gap and filter may be null.  Writes up to capacity pairs (x 2 ints) and returns the number found. */
+int oracle_broad_phase(int mode, const float* lower, const float* upper, const float* gap, const int* group,
+                       const int* world, const int* filter, int filter_count, const int* index_map,
+                       const int* slice_ends, int segments, const int* pairs, int pair_count, int* out, int capacity);
+
 // Elementwise FP32 asin / acos of the restatement (the revolute joint's twist and swing angles).
 void oracle_asin_f32(const float* x, int n, float* out);
 void oracle_acos_f32(const float* x, int n, float* out);

@@ -1,0 +1,107 @@
+"""Standalone broad phases on the GPU (nmc_broad_phase_launch / nmc_broad_phase_explicit) against the oracle.
+
+Every mode reports the oracle's count and pair set (each pair once) on seeded multi-world scenes with shared
+shapes, collision groups, per-shape gaps and a filter list; a world above the LDS limit (2 100 shapes) runs
+the all-pairs global path and the SAP predicate path; two launches write identical arrays (fixed order); with
+too small an output the count is still the number found and the written prefix equals the full run's;
+explicit keeps the oracle's (input) order exactly; the reference's per-shape gap known answer holds.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from newton_custom_contact_amd.geometry import (BroadPhaseAllPairs, BroadPhaseExplicit, BroadPhaseSAP,
+                                                precompute_world_map)
+from tests.test_oracle_broad import _gap_scene, random_scene
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dev, dt):
+    return torch.from_numpy(np.ascontiguousarray(a, dt)).to(dev)
+
+
+def _run(bp, dev, lower, upper, gap, group, world, filt, cap):
+    out = torch.full((max(cap, 1), 2), -7, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    bp.launch(_dev(lower, dev, np.float32), _dev(upper, dev, np.float32),
+              None if gap is None else _dev(gap, dev, np.float32), _dev(group, dev, np.int32),
+              _dev(world, dev, np.int32), len(world), out[:cap] if cap else out[:0], cnt,
+              filter_pairs=None if filt is None else _dev(filt, dev, np.int32))
+    torch.cuda.synchronize()
+    return int(cnt.item()), out.cpu().numpy()
+
+
+SCENES = [("3w_12", dict(worlds=3, per_world=12, shared=3)),
+          ("64w_33", dict(worlds=64, per_world=32, shared=1, extent=2.0)),
+          ("2w_300", dict(worlds=2, per_world=300, shared=4, extent=4.0)),
+          ("1w_2100", dict(worlds=1, per_world=2100, shared=2, extent=12.0))]
+
+
+@pytest.mark.parametrize("name,kw", SCENES, ids=[s[0] for s in SCENES])
+@pytest.mark.parametrize("mode", ["sap", "nxn"])
+def test_broad_phase_matches_oracle(cuda_device, name, kw, mode):
+    print(f"[broad phase] {mode} {name}", flush=True)
+    rng = np.random.Generator(np.random.PCG64(sum(map(ord, name))))
+    lower, upper, gap, group, world, filt = random_scene(rng, **kw)
+    im, se = precompute_world_map(world)
+    ref = oracle.broad_phase(mode, lower, upper, gap, group, world, filt, im, se)
+    bp = (BroadPhaseSAP if mode == "sap" else BroadPhaseAllPairs)(world)
+    cap = len(ref) + 16
+    n, got = _run(bp, cuda_device, lower, upper, gap, group, world, filt, cap)
+    assert n == len(ref) and n > 0
+    pairs = [tuple(p) for p in got[:n].tolist()]
+    assert len(set(pairs)) == n
+    assert set(pairs) == {tuple(p) for p in ref.tolist()}
+    assert np.all(got[n:cap] == -7)
+    n2, got2 = _run(bp, cuda_device, lower, upper, gap, group, world, filt, cap)
+    np.testing.assert_array_equal(got2, got)
+    # too small an output: full count, the same prefix
+    k = n // 3
+    n3, got3 = _run(bp, cuda_device, lower, upper, gap, group, world, filt, k)
+    assert n3 == n
+    np.testing.assert_array_equal(got3[:k], got[:k])
+    # no gaps, no filter
+    ref0 = oracle.broad_phase(mode, lower, upper, None, group, world, None, im, se)
+    n4, got4 = _run(bp, cuda_device, lower, upper, None, group, world, None, len(ref0) + 1)
+    assert n4 == len(ref0)
+    assert {tuple(p) for p in got4[:n4].tolist()} == {tuple(p) for p in ref0.tolist()}
+
+
+def test_explicit_matches_oracle(cuda_device):
+    print("[broad phase] explicit", flush=True)
+    rng = np.random.Generator(np.random.PCG64(5))
+    lower, upper, gap, group, world, _ = random_scene(rng, worlds=8, per_world=24, shared=0)
+    n = len(world)
+    pairs = np.array([(a, b) for a in range(n) for b in range(a + 1, n) if world[a] == world[b]], np.int32)
+    rng.shuffle(pairs)
+    ref = oracle.broad_phase("explicit", lower, upper, gap, group, world, shape_pairs=pairs)
+    bp = BroadPhaseExplicit()
+    dev = cuda_device
+    out = torch.zeros((len(pairs), 2), dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    bp.launch(_dev(lower, dev, np.float32), _dev(upper, dev, np.float32), _dev(gap, dev, np.float32),
+              _dev(pairs, dev, np.int32), len(pairs), out, cnt)
+    torch.cuda.synchronize()
+    k = int(cnt.item())
+    assert k == len(ref) and k > 0
+    np.testing.assert_array_equal(out[:k].cpu().numpy(), ref)
+    # a pair naming a shape past the AABB arrays is skipped, never read
+    bad = np.concatenate([pairs[:5], [[0, n + 5]]]).astype(np.int32)
+    ref_b = oracle.broad_phase("explicit", lower, upper, gap, group, world, shape_pairs=bad[:5])
+    bp.launch(_dev(lower, dev, np.float32), _dev(upper, dev, np.float32), _dev(gap, dev, np.float32),
+              _dev(bad, dev, np.int32), len(bad), out, cnt)
+    torch.cuda.synchronize()
+    k = int(cnt.item())
+    np.testing.assert_array_equal(out[:k].cpu().numpy(), ref_b)
+
+
+def test_per_shape_gap_known_answer_gpu(cuda_device):
+    lower, upper, gap, group, world = _gap_scene()
+    for cls in (BroadPhaseAllPairs, BroadPhaseSAP):
+        n, got = _run(cls(world), cuda_device, lower, upper, gap, group, world, None, 8)
+        assert n == 1 and tuple(got[0]) == (0, 2)

@@ -1,0 +1,206 @@
+"""CPU tests of the standalone broad phases' restatement (oracle/oracle_broad.h) and host logic.
+
+Pins:
+  * precompute_world_map against the layout the reference documents (broad_phase_common.py:183-300): world
+    shapes then shared shapes per world, a last segment of shared shapes, flags filter, invalid world ids;
+  * all pairs against an independent numpy brute force of the reference's rules (world -1 pairs only in the
+    dedicated segment, test_world_and_group_pair, overlap with the sum of the two gaps, the filter list) on
+    random scenes with several worlds, shared shapes, groups, gaps and filters;
+  * SAP reports a subset of all pairs, and the same set on generic scenes (random extents: every overlap is
+    far above rounding); a corner-touching pair is reported by SAP exactly when the projected intervals
+    overlap strictly;
+  * the reference's own known answers: per-shape gaps (test_broad_phase.py:2024-2118: the large-gap sphere
+    meets the ground, the small-gap one does not, in every mode) and boxes exactly touching on x;
+  * explicit = the given pairs whose AABBs overlap, in input order.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import torch
+
+import oracle
+from newton_custom_contact_amd.geometry import (BroadPhaseAllPairs, BroadPhaseExplicit, BroadPhaseSAP,
+                                                precompute_world_map)
+
+
+def test_world_map_layout():
+    world = np.array([0, -1, 1, 0, -1, 2, 1])
+    im, se = precompute_world_map(world)
+    np.testing.assert_array_equal(im, [0, 3, 1, 4, 2, 6, 1, 4, 5, 1, 4, 1, 4])
+    np.testing.assert_array_equal(se, [4, 8, 11, 13])
+    flags = np.array([2, 2, 0, 3, 2, 2, 2])  # shape 2 does not collide
+    im, se = precompute_world_map(world, flags)
+    np.testing.assert_array_equal(im, [0, 3, 1, 4, 6, 1, 4, 5, 1, 4, 1, 4])
+    np.testing.assert_array_equal(se, [4, 7, 10, 12])
+    im, se = precompute_world_map(np.array([-1, -1]))
+    np.testing.assert_array_equal(im, [0, 1])
+    np.testing.assert_array_equal(se, [2])
+    with pytest.raises(ValueError, match="Invalid world IDs"):
+        precompute_world_map(np.array([0, -2]))
+    with pytest.raises(ValueError, match="same length"):
+        precompute_world_map(np.array([0, 1]), np.array([2]))
+    bp = BroadPhaseSAP(world, sort_type="tile")
+    assert bp.num_regular_worlds == 3 and bp.max_shapes_per_world == 4
+    with pytest.raises(ValueError, match="sort type"):
+        BroadPhaseSAP(world, sort_type="radix")
+    nxn = BroadPhaseAllPairs(world)
+    assert nxn.num_kernel_threads == 6 + 6 + 3 + 1
+
+
+def random_scene(rng, worlds=3, per_world=12, shared=3, extent=1.5):
+    n = worlds * per_world + shared
+    world = np.concatenate([np.repeat(np.arange(worlds), per_world), np.full(shared, -1)]).astype(np.int32)
+    rng.shuffle(world)
+    c = rng.uniform(0, extent, (n, 3))
+    h = rng.uniform(0.05, 0.6, (n, 3))
+    lower = (c - h).astype(np.float32)
+    upper = (c + h).astype(np.float32)
+    gap = rng.uniform(0.0, 0.1, n).astype(np.float32)
+    group = rng.choice(np.array([-2, -1, 0, 1, 2, 1, 1, 1, 1, 1], np.int32), n).astype(np.int32)
+    # about a tenth of all pairs, as sorted canonical (min, max) rows
+    ab = rng.integers(0, n, size=(max(n * (n - 1) // 20, 1), 2))
+    ab = np.sort(ab[ab[:, 0] != ab[:, 1]], axis=1)
+    filt = np.unique(ab, axis=0).astype(np.int32).reshape(-1, 2)
+    return lower, upper, gap, group, world, filt
+
+
+def brute_all_pairs(lower, upper, gap, group, world, filt):
+    """The reference's rules, spelled out over every unordered pair of shapes."""
+    def grp(a, b):
+        if a == 0 or b == 0:
+            return False
+        if a > 0:
+            return a == b or b < 0
+        return a != b
+
+    fset = {tuple(p) for p in filt.tolist()}
+    out = set()
+    n = len(world)
+    for a in range(n):
+        for b in range(a + 1, n):
+            wa, wb = world[a], world[b]
+            if wa != -1 and wb != -1 and wa != wb:
+                continue
+            if not grp(group[a], group[b]):
+                continue
+            c = np.float32(gap[a]) + np.float32(gap[b])
+            if not all(lower[a, k] <= upper[b, k] + c and upper[a, k] >= lower[b, k] - c for k in range(3)):
+                continue
+            if (a, b) in fset:
+                continue
+            out.add((a, b))
+    return out
+
+
+def _pairs(arr):
+    return [tuple(p) for p in np.asarray(arr).tolist()]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3, 4])
+def test_all_pairs_and_sap_vs_brute_force(seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lower, upper, gap, group, world, filt = random_scene(rng)
+    im, se = precompute_world_map(world)
+    nxn = _pairs(oracle.broad_phase("nxn", lower, upper, gap, group, world, filt, im, se))
+    sap = _pairs(oracle.broad_phase("sap", lower, upper, gap, group, world, filt, im, se))
+    ref = brute_all_pairs(lower, upper, gap, group, world, filt)
+    assert len(nxn) == len(set(nxn)) and len(sap) == len(set(sap))  # every pair once
+    assert set(nxn) == ref and len(ref) > 20
+    assert all(a < b for a, b in nxn + sap)
+    assert set(sap) == set(nxn)
+    # without gaps or filters too
+    nxn0 = _pairs(oracle.broad_phase("nxn", lower, upper, None, group, world, None, im, se))
+    assert set(nxn0) == brute_all_pairs(lower, upper, np.zeros_like(gap), group, world, np.zeros((0, 2), np.int32))
+
+
+def test_sap_touching_corners_follow_the_sweep_predicate():
+    """Corner-touching unit boxes: the AABB test passes, so all pairs reports every one; SAP reports a pair
+    only when the lower projection of the later box is below the upper projection of the earlier one."""
+    d = np.array([0.5935, 0.7790, 0.1235], np.float32)
+    d = d / np.float32(np.sqrt(np.float32(d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]))
+    seen = {True: 0, False: 0}
+    for off in np.linspace(0.1, 7.3, 97).astype(np.float32):
+        lower = np.array([[off, off, off], [off + 1, off + 1, off + 1]], np.float32)
+        upper = lower + np.float32(1.0)
+        group = np.ones(2, np.int32)
+        world = np.zeros(2, np.int32)
+        im, se = precompute_world_map(world)
+        nxn = _pairs(oracle.broad_phase("nxn", lower, upper, None, group, world, None, im, se))
+        sap = _pairs(oracle.broad_phase("sap", lower, upper, None, group, world, None, im, se))
+        assert nxn == [(0, 1)]
+
+        def proj(k):
+            half = np.float32(0.5) * (upper[k] - lower[k])
+            c = np.float32(0.5) * (lower[k] + upper[k])
+            r = np.float32(d[0] * half[0] + d[1] * half[1]) + np.float32(d[2] * half[2])
+            cc = np.float32(d[0] * c[0] + d[1] * c[1]) + np.float32(d[2] * c[2])
+            return np.float32(cc - r), np.float32(cc + r)
+
+        (l0, u0), (l1, _u1) = proj(0), proj(1)
+        strict = bool(l1 < u0)
+        assert (sap == [(0, 1)]) == strict, off
+        seen[strict] += 1
+    assert seen[True] > 0 and seen[False] > 0  # both outcomes occur (35 / 62 of these offsets)
+
+
+def _gap_scene():
+    lower = np.array([[-1000.0, -1000.0, 0.0], [-0.2, -0.2, 0.04], [9.8, -0.2, 0.04]], np.float32)
+    upper = np.array([[1000.0, 1000.0, 0.0], [0.2, 0.2, 0.44], [10.2, 0.2, 0.44]], np.float32)
+    gap = np.array([0.01, 0.02, 0.06], np.float32)
+    group = np.ones(3, np.int32)
+    world = np.zeros(3, np.int32)
+    return lower, upper, gap, group, world
+
+
+def test_per_shape_gap_known_answer():
+    lower, upper, gap, group, world = _gap_scene()
+    im, se = precompute_world_map(world)
+    for mode in ("nxn", "sap"):
+        assert _pairs(oracle.broad_phase(mode, lower, upper, gap, group, world, None, im, se)) == [(0, 2)]
+    ex = oracle.broad_phase("explicit", lower, upper, gap, group, world,
+                            shape_pairs=np.array([[0, 1], [0, 2], [1, 2]], np.int32))
+    assert _pairs(ex) == [(0, 2)]
+
+
+def test_touching_boxes_on_x_known_answer():
+    lower = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [3.2, 0.0, 0.0]], np.float32)
+    upper = np.array([[1.0, 1.0, 1.0], [2.0, 1.0, 1.0], [4.0, 1.0, 1.0]], np.float32)
+    group = np.ones(3, np.int32)
+    world = np.zeros(3, np.int32)
+    im, se = precompute_world_map(world)
+    for mode in ("nxn", "sap"):
+        assert _pairs(oracle.broad_phase(mode, lower, upper, None, group, world, None, im, se)) == [(0, 1)]
+        # a cutoff of 0.65 per shape closes the 1.2 m gap between boxes 1 and 2
+        got = _pairs(oracle.broad_phase(mode, lower, upper, np.full(3, 0.65, np.float32), group, world, None, im, se))
+        assert set(got) == {(0, 1), (1, 2)}
+
+
+def test_explicit_keeps_input_order():
+    rng = np.random.Generator(np.random.PCG64(7))
+    lower, upper, gap, group, world, _ = random_scene(rng, worlds=1, per_world=20, shared=0)
+    n = len(world)
+    pairs = np.array([(a, b) for a in range(n) for b in range(n) if a != b], np.int32)
+    rng.shuffle(pairs)
+    got = oracle.broad_phase("explicit", lower, upper, gap, group, world, shape_pairs=pairs)
+    keep = [tuple(p) for p in pairs.tolist()
+            if all(lower[p[0], k] <= upper[p[1], k] + (gap[p[0]] + gap[p[1]]) and
+                   upper[p[0], k] >= lower[p[1], k] - (gap[p[0]] + gap[p[1]]) for k in range(3))]
+    assert _pairs(got) == keep and len(keep) > 10
+
+
+def test_launch_argument_checks():
+    world = torch.zeros(3, dtype=torch.int32)
+    bp = BroadPhaseAllPairs(world.numpy())
+    lo = torch.zeros((3, 3))
+    out = torch.zeros((4, 2), dtype=torch.int32)
+    cnt = torch.zeros(1, dtype=torch.int32)
+    with pytest.raises(RuntimeError, match="GPU"):
+        bp.launch(lo, lo, None, world, world, 3, out, cnt)
+    with pytest.raises(TypeError, match="dtype"):
+        bp.launch(lo.double(), lo, None, world, world, 3, out, cnt)
+    with pytest.raises(TypeError):
+        bp.launch(lo.numpy(), lo, None, world, world, 3, out, cnt)
+    with pytest.raises(RuntimeError, match="GPU"):
+        BroadPhaseExplicit().launch(lo, lo, None, torch.zeros((1, 2), dtype=torch.int32), 1, out, cnt)
