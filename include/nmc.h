@@ -114,6 +114,15 @@ typedef struct NmcModelDesc {
   const int* mesh_sdf_cells;     /* M x 3 */
   const int* mesh_sdf_flags;     /* M */
   int mesh_contact_max;          /* contacts kept per mesh pair (deepest first), at most 255 */
+  /* CollisionPipeline broad_phase (collide.py:605-735): 0 "sap" -- a segment's shapes sorted by their
+   * projection on the fixed axis, a pair tested only when the later one's lower bound is below the earlier
+   * one's upper bound (broad_phase_sap.py:189-394); 1 "nxn" -- every pair of a segment (broad_phase_nxn.py:
+   * 129-214; also "explicit" with the model's own shape_contact_pairs, the same set); 2 "explicit" with a
+   * caller's list -- the pairs of explicit_pairs (sorted canonical (min, max)) whose AABBs overlap, without
+   * the group and filter tests (broad_phase_nxn.py:28-65). */
+  int broad_phase_mode;
+  const int* explicit_pairs;
+  int explicit_pair_count;
 } NmcModelDesc;
 
 /* Device-resident contact buffers (layout of newton.Contacts rigid part). */

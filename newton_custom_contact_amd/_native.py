@@ -45,6 +45,7 @@ class NmcModelDesc(ctypes.Structure):
         ("mesh_vertex", _vp), ("mesh_vertex_start", _vp), ("mesh_edge", _vp), ("mesh_edge_start", _vp),
         ("mesh_sdf", _vp), ("mesh_sdf_start", _vp), ("mesh_sdf_lower", _vp), ("mesh_sdf_voxel", _vp),
         ("mesh_sdf_cells", _vp), ("mesh_sdf_flags", _vp), ("mesh_contact_max", ctypes.c_int),
+        ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _vp), ("explicit_pair_count", ctypes.c_int),
     ]
 
 
@@ -219,7 +220,8 @@ def check_mesh_pairs(model):
 class NativeModel:
     """Owns one NmcModel handle plus the extra device tables it points at."""
 
-    def __init__(self, model, rigid_contact_max: int, shape_pairs_max: int):
+    def __init__(self, model, rigid_contact_max: int, shape_pairs_max: int, broad_phase_mode: int = 0,
+                 explicit_pairs: torch.Tensor | None = None):
         if model.device.type != "cuda":
             raise RuntimeError(
                 "newton_custom_contact_amd runs its contact step on the GPU only (HIP kernels in libnmc.so); "
@@ -321,6 +323,10 @@ class NativeModel:
                                                                ptr(m.mesh_sdf_cells))
         d.mesh_sdf_flags = ptr(m.mesh_sdf_flags)
         d.mesh_contact_max = MESH_CONTACT_MAX
+        d.broad_phase_mode = int(broad_phase_mode)
+        self.explicit_pairs = explicit_pairs
+        d.explicit_pairs = ptr(explicit_pairs)
+        d.explicit_pair_count = 0 if explicit_pairs is None else int(explicit_pairs.shape[0])
         self.desc = d
         self.model = model
         handle = _vp()

@@ -7,10 +7,11 @@ capacity estimators (``_estimate_rigid_contact_max`` ``:278-376``, ``_compute_pe
 
 ``collide`` launches its kernels from libnmc.so (segment broad phase, route scan, primitive narrow phase,
 mesh narrow phase when the model has meshes, contact count / scan / compaction) on the current torch
-stream and never synchronises the host, so it can be captured in a graph.  The broad phase always runs
-per-world sweep-and-prune; the ``broad_phase`` argument accepts the reference's names and
-``"nxn"``/``"explicit"`` produce the same pair set (every one of them reports the AABB-overlapping,
-filter-passing pairs).  Mesh shapes collide with meshes (SDF edge search) and infinite planes (vertex
+stream and never synchronises the host, so it can be captured in a graph.  The broad phase runs per
+segment out of LDS with the reference's pair sets: ``"sap"`` tests a pair when the sweep on the fixed axis
+reaches it (projected intervals overlapping strictly), ``"nxn"`` tests every pair of a segment, and
+``"explicit"`` (the default) tests ``model.shape_contact_pairs`` -- the ``"nxn"`` set -- or, given
+``shape_pairs_filtered``, exactly the listed pairs (AABB overlap only).  Mesh shapes collide with meshes (SDF edge search) and infinite planes (vertex
 test); a model where a mesh could meet any other shape type is refused.
 """
 
@@ -19,6 +20,7 @@ from __future__ import annotations
 import ctypes
 
 import numpy as np
+import torch
 
 from ._native import NativeModel, NmcContactReport, check, load, ptr, stream_handle
 from ._types import GeoType, ShapeFlags
@@ -69,6 +71,27 @@ def per_world_shape_pairs_max(model) -> int:
     return int(np.sum(seg * (seg - 1) // 2))
 
 
+def _explicit_pair_list(model, pairs) -> torch.Tensor:
+    """A caller's pair list for broad_phase="explicit": canonical (min, max), sorted, without repeats.  The
+    pairs are found inside broad-phase segments, so each must join two colliding shapes of one world (or a
+    world and a world -1 shape, or two world -1 shapes); the reference would also test other pairs."""
+    p = pairs.detach().cpu().numpy() if isinstance(pairs, torch.Tensor) else np.asarray(pairs)
+    p = np.asarray(p, np.int64).reshape(-1, 2)
+    S = model.shape_count
+    if len(p) and (p.min() < 0 or p.max() >= S):
+        raise ValueError("shape_pairs_filtered names shapes outside the model")
+    p = p[p[:, 0] != p[:, 1]]
+    p = np.unique(np.sort(p, axis=1), axis=0)
+    h = model.host
+    sw = h["shape_world"]
+    coll = (h["shape_flags"] & int(ShapeFlags.COLLIDE_SHAPES)) != 0
+    wa, wb = sw[p[:, 0]], sw[p[:, 1]]
+    if np.any((wa != -1) & (wb != -1) & (wa != wb)) or not np.all(coll[p[:, 0]] & coll[p[:, 1]]):
+        raise NotImplementedError("shape_pairs_filtered may only join colliding shapes of one world "
+                                  "(or with world -1 shapes)")
+    return torch.from_numpy(np.ascontiguousarray(p, np.int32).reshape(-1, 2))
+
+
 class CollisionPipeline:
     """GPU collision pipeline with the reference's constructor and ``collide`` API."""
 
@@ -92,11 +115,33 @@ class CollisionPipeline:
             raise ValueError('contact_report=True requires contact_matching != "disabled"')
         if matching_enabled:
             deterministic = True  # any matching mode implies the deterministic (sort-keyed) contact order
-        mode = "explicit" if broad_phase is None else str(broad_phase).lower()
-        if mode not in ("nxn", "sap", "explicit"):
-            raise ValueError(f"Unsupported broad phase mode: {broad_phase!r}")
-        if narrow_phase is not None:
-            raise ValueError("expert narrow_phase objects are not supported")
+        from .geometry import BroadPhaseAllPairs, BroadPhaseExplicit, BroadPhaseSAP  # noqa: PLC0415
+
+        if broad_phase is None or isinstance(broad_phase, str):
+            mode = "explicit" if broad_phase is None else str(broad_phase).lower()
+            if mode not in ("nxn", "sap", "explicit"):
+                raise ValueError(f"Unsupported broad phase mode: {broad_phase!r}")
+            expert = narrow_phase is not None
+        else:
+            kinds = ((BroadPhaseAllPairs, "nxn"), (BroadPhaseSAP, "sap"), (BroadPhaseExplicit, "explicit"))
+            hit = [name for cls, name in kinds if isinstance(broad_phase, cls)]
+            if not hit:
+                raise TypeError("broad_phase must be a BroadPhaseAllPairs, BroadPhaseSAP, or BroadPhaseExplicit "
+                                f"instance (got {type(broad_phase)!r})")
+            mode = hit[0]
+            expert = True
+        if expert:
+            if narrow_phase is None or (broad_phase is None or isinstance(broad_phase, str)):
+                raise ValueError("Provide both broad_phase and narrow_phase for expert component construction")
+            raise NotImplementedError("expert narrow_phase objects are not supported: the narrow phase runs in "
+                                      "libnmc.so")
+        # "explicit" tests model.shape_contact_pairs, the same set "nxn" visits; a caller's own list is tested
+        # as given (AABB overlap only, collide.py:695-709)
+        explicit_pairs = None
+        bp_mode = {"sap": 0, "nxn": 1, "explicit": 1}[mode]
+        if mode == "explicit" and shape_pairs_filtered is not None:
+            explicit_pairs = _explicit_pair_list(model, shape_pairs_filtered)
+            bp_mode = 2
         types = set(np.unique(model.host["shape_type"]).tolist())
         bad = types - _SUPPORTED
         if bad:
@@ -123,7 +168,9 @@ class CollisionPipeline:
         colliding = (model.host["shape_flags"] & int(ShapeFlags.COLLIDE_SHAPES)) != 0
         self.shape_count = model.shape_count
         self._colliding = int(np.count_nonzero(colliding))
-        self.native = NativeModel(model, self._rigid_contact_max, self.shape_pairs_max)
+        self.shape_pairs_filtered = explicit_pairs
+        self.native = NativeModel(model, self._rigid_contact_max, self.shape_pairs_max, bp_mode,
+                                  None if explicit_pairs is None else explicit_pairs.to(model.device))
         self._lib = load()
         self.contact_matching = contact_matching
         self._matching_enabled = matching_enabled

@@ -74,18 +74,6 @@ __device__ __forceinline__ BpShape bp_load(const BpArgs& a, int id) {
   return s;
 }
 
-// is_pair_excluded (broad_phase_common.py:82-113): binary search of (a, b) in the sorted filter list.
-__device__ __forceinline__ bool bp_excluded(const int* __restrict__ fp, int n, int a, int b) {
-  int lo = 0, hi = n - 1;
-  while (lo <= hi) {
-    const int mid = (lo + hi) >> 1;
-    const int ma = fp[2 * mid], mb = fp[2 * mid + 1];
-    if (a == ma && b == mb) return true;
-    if (a < ma || (a == ma && b < mb)) hi = mid - 1; else lo = mid + 1;
-  }
-  return false;
-}
-
 // The tests every broad phase applies to the pair of two segment members.
 __device__ __forceinline__ bool bp_pair_ok(const BpArgs& a, const BpShape& p, const BpShape& q, bool dedicated,
                                            int& s1, int& s2) {
@@ -101,7 +89,7 @@ __device__ __forceinline__ bool bp_pair_ok(const BpArgs& a, const BpShape& p, co
   const bool ov = A.lo.x <= B.hi.x + c && A.hi.x >= B.lo.x - c && A.lo.y <= B.hi.y + c && A.hi.y >= B.lo.y - c &&
                   A.lo.z <= B.hi.z + c && A.hi.z >= B.lo.z - c;
   if (!ov) return false;
-  if (a.filter_count > 0 && bp_excluded(a.filter, a.filter_count, s1, s2)) return false;
+  if (a.filter_count > 0 && pair_listed(a.filter, a.filter_count, s1, s2)) return false;
   return true;
 }
 

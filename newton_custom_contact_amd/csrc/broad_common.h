@@ -40,6 +40,19 @@ __device__ __forceinline__ bool excluded(int b, const int* __restrict__ fp, int 
   return false;
 }
 
+// is_pair_excluded (broad_phase_common.py:82-113): binary search of the canonical pair (a, b) in a list of
+// n lexicographically sorted (min, max) pairs.
+__device__ __forceinline__ bool pair_listed(const int* __restrict__ fp, int n, int a, int b) {
+  int lo = 0, hi = n - 1;
+  while (lo <= hi) {
+    const int mid = (lo + hi) >> 1;
+    const int ma = fp[2 * mid], mb = fp[2 * mid + 1];
+    if (a == ma && b == mb) return true;
+    if (a < ma || (a == ma && b < mb)) hi = mid - 1; else lo = mid + 1;
+  }
+  return false;
+}
+
 // Lane groups that run one segment: the whole workgroup (barrier) or one wave (wave-local ordering of
 // its LDS accesses; a wave's LDS operations complete in issue order).
 struct BlockGroup {

@@ -145,27 +145,36 @@ __device__ void segment_broad_phase(const G g, const NmcModelDesc& d, const floa
   g.sync();
   bitonic_sort(L.skeys, n2, g);
 
-  // ---- sweep (broad_phase_sap.py:189-394): element i sweeps forward while lower_j < upper_i
+  // ---- sweep (broad_phase_sap.py:189-394): element i sweeps forward while lower_j < upper_i ("sap"), or
+  // over every later element ("nxn", "explicit")
+  const bool sweep = d.broad_phase_mode == 0;
+  const bool listed = d.broad_phase_mode == 2;
   for (int i = g.lane; i < n; i += G::size) {
     int li = static_cast<int>(L.skeys[i] & 0xffffffffu);
     float upper = L.phi[li];
     for (int j = i + 1; j < n; ++j) {
       int lj = static_cast<int>(L.skeys[j] & 0xffffffffu);
-      if (!(L.plo[lj] < upper)) break;
+      if (sweep && !(L.plo[lj] < upper)) break;
       int t1 = L.gid[li], t2 = L.gid[lj];
       if (t1 == t2) continue;
       int la = t1 < t2 ? li : lj, lb = t1 < t2 ? lj : li;
       // the reference's tests form one conjunction; the LDS-only ones go first
       int w1 = L.world[la], w2 = L.world[lb];
       if (w1 == -1 && w2 == -1 && regular) continue;
-      if (w1 != -1 && w2 != -1 && w1 != w2) continue;
-      if (!group_pair(L.group[la], L.group[lb])) continue;
+      if (!listed) {
+        if (w1 != -1 && w2 != -1 && w1 != w2) continue;
+        if (!group_pair(L.group[la], L.group[lb])) continue;
+      }
       const float c = 0.0f + 0.0f;
       bool ov = L.lo[3 * la] <= L.hi[3 * lb] + c && L.hi[3 * la] >= L.lo[3 * lb] - c &&
                 L.lo[3 * la + 1] <= L.hi[3 * lb + 1] + c && L.hi[3 * la + 1] >= L.lo[3 * lb + 1] - c &&
                 L.lo[3 * la + 2] <= L.hi[3 * lb + 2] + c && L.hi[3 * la + 2] >= L.lo[3 * lb + 2] - c;
       if (!ov) continue;
-      if (excluded(L.gid[lb], d.filter_pairs, L.fbeg[la], L.fend[la])) continue;
+      if (listed) {
+        if (!pair_listed(d.explicit_pairs, d.explicit_pair_count, L.gid[la], L.gid[lb])) continue;
+      } else if (excluded(L.gid[lb], d.filter_pairs, L.fbeg[la], L.fend[la])) {
+        continue;
+      }
       // narrow-phase order: lower GeoType first (narrow_phase.py:212-215); ranks order like global ids
       int ra = la, rb = lb;
       if (L.type[la] > L.type[lb]) { ra = lb; rb = la; }

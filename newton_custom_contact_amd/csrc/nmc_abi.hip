@@ -62,6 +62,9 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   if (d.max_world_bodies > 512) return fail("worlds with more than 512 bodies are not supported by the XPBD kernel");
   if (d.max_segment_shapes > 65535) return fail("a broad-phase segment may hold at most 65535 shapes");
   if (d.mesh_count < 0) return fail("mesh_count must be >= 0");
+  if (d.broad_phase_mode < 0 || d.broad_phase_mode > 2) return fail("broad_phase_mode must be 0 (sap), 1 (nxn) or 2 (explicit)");
+  if (d.broad_phase_mode == 2 && (d.explicit_pair_count < 0 || (d.explicit_pair_count > 0 && !d.explicit_pairs)))
+    return fail("an explicit broad phase needs its pair list");
   if (d.mesh_count > 0) {
     if (!d.shape_mesh || !d.shape_aabb_lower || !d.shape_aabb_upper || !d.mesh_vertex || !d.mesh_vertex_start ||
         !d.mesh_edge || !d.mesh_edge_start || !d.mesh_sdf || !d.mesh_sdf_start || !d.mesh_sdf_lower ||

@@ -92,6 +92,7 @@ class Model:
         self.particle_count = 0
         self.shape_collision_filter_pairs: set[tuple[int, int]] = set()
         self._collision_pipeline = None
+        self._shape_contact_pairs = None
         self._requested_state_attributes: set[str] = set()
         self._requested_contact_attributes: set[str] = set()
 
@@ -238,7 +239,47 @@ class Model:
         for name, a in self.host.items():
             setattr(self, name, torch.from_numpy(np.ascontiguousarray(a)).to(self.device))
         self._mesh_sdf_tables()
-        self.shape_contact_pair_count = 0
+        self._shape_contact_pairs = None
+
+    @property
+    def shape_contact_pairs(self) -> torch.Tensor:
+        """Every shape pair the collision pipeline may test, ``(K, 2)`` int32 canonical (min, max), sorted
+        (find_shape_contact_pairs, builder.py:10855-10913: colliding shapes, the world and group rules,
+        minus the filter pairs).  Computed on first use; the explicit broad phase tests exactly these."""
+        if self._shape_contact_pairs is None:
+            self._shape_contact_pairs = torch.from_numpy(self._find_shape_contact_pairs()).to(self.device)
+        return self._shape_contact_pairs
+
+    @property
+    def shape_contact_pair_count(self) -> int:
+        return int(self.shape_contact_pairs.shape[0])
+
+    def _find_shape_contact_pairs(self) -> np.ndarray:
+        h = self.host
+        ss, sh = h["segment_start"], h["segment_shape"]
+        sw, grp = h["shape_world"], h["shape_collision_group"]
+        nseg = len(ss) - 1
+        out = []
+        for seg in range(nseg):
+            ids = sh[ss[seg]:ss[seg + 1]]
+            if len(ids) < 2:
+                continue
+            i, j = np.triu_indices(len(ids), 1)
+            a, b = np.minimum(ids[i], ids[j]), np.maximum(ids[i], ids[j])
+            wa, wb = sw[a], sw[b]
+            ga, gb = grp[a], grp[b]
+            keep = (wa == -1) & (wb == -1) if seg == nseg - 1 else ~((wa == -1) & (wb == -1))
+            keep &= ~((wa != -1) & (wb != -1) & (wa != wb))
+            # test_group_pair (broad_phase_common.py:132-150)
+            keep &= (ga != 0) & (gb != 0) & np.where(ga > 0, (ga == gb) | (gb < 0), ga != gb)
+            out.append(np.stack([a[keep], b[keep]], 1))
+        pairs = np.concatenate(out).astype(np.int64) if out else np.zeros((0, 2), np.int64)
+        fp = h["shape_filter_pairs"].astype(np.int64)
+        if len(pairs) and len(fp):
+            key = pairs[:, 0] * (1 << 32) + pairs[:, 1]
+            pairs = pairs[~np.isin(key, fp[:, 0] * (1 << 32) + fp[:, 1])]
+        pairs = np.unique(pairs, axis=0) if len(pairs) else pairs
+        return np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
 
     @classmethod
     def _replicate(cls, template: Model, world_count: int, body_q: np.ndarray, device) -> Model:
@@ -483,7 +524,7 @@ class Model:
         if self._collision_pipeline is None:
             from .collide import CollisionPipeline  # noqa: PLC0415
 
-            self._collision_pipeline = CollisionPipeline(self, broad_phase="sap")
+            self._collision_pipeline = CollisionPipeline(self, broad_phase="explicit")
         return self._collision_pipeline
 
     def contacts(self, collision_pipeline=None):

@@ -44,6 +44,7 @@ class OracleModel(ctypes.Structure):
         ("mesh_vertex", _f32p), ("mesh_vertex_start", _i32p), ("mesh_edge", _i32p), ("mesh_edge_start", _i32p),
         ("mesh_sdf", _f32p), ("mesh_sdf_start", _i64p), ("mesh_sdf_lower", _f32p), ("mesh_sdf_voxel", _f32p),
         ("mesh_sdf_cells", _i32p), ("mesh_sdf_flags", _i32p), ("mesh_contact_max", ctypes.c_int),
+        ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _i32p), ("explicit_pair_count", ctypes.c_int),
     ]
 
 
@@ -104,7 +105,7 @@ def _p(a: np.ndarray, t):
 class Oracle:
     """Holds a model's host arrays (kept alive) and runs the restatement on them."""
 
-    def __init__(self, model, mesh_contact_max: int = 64):
+    def __init__(self, model, mesh_contact_max: int = 64, broad_phase: str = "sap", explicit_pairs=None):
         h = model.host
         self.model = model
         keep = {}
@@ -161,6 +162,16 @@ class Oracle:
                                                                 i32("mesh_sdf_cells"))
         m.mesh_sdf_flags = i32("mesh_sdf_flags")
         m.mesh_contact_max = mesh_contact_max
+        # CollisionPipeline broad_phase: "explicit" without a list is the model's own pair list, the "nxn" set
+        if broad_phase == "explicit" and explicit_pairs is None:
+            broad_phase = "nxn"
+        m.broad_phase_mode = {"sap": 0, "nxn": 1, "explicit": 2}[broad_phase]
+        if explicit_pairs is not None:
+            ep = np.unique(np.sort(np.asarray(explicit_pairs, np.int64).reshape(-1, 2), axis=1), axis=0)
+            ep = np.ascontiguousarray(ep, np.int32)
+            keep["explicit_pairs"] = ep
+            m.explicit_pairs = _p(ep, _i32p)
+            m.explicit_pair_count = len(ep)
         self._keep = keep
         self.m = m
         self.lib = lib()

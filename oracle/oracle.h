@@ -77,6 +77,9 @@ typedef struct OracleModel {
   const int* mesh_sdf_cells; /* M x 3 */
   const int* mesh_sdf_flags; /* M: bit 0 = scale baked into the grid */
   int mesh_contact_max;
+  int broad_phase_mode;       /* 0 sap, 1 nxn (and explicit with the model's pairs), 2 explicit list */
+  const int* explicit_pairs;  /* sorted (min, max) */
+  int explicit_pair_count;
 } OracleModel;
 
 typedef struct OracleXpbdParams {

@@ -213,16 +213,23 @@ void sap_pairs(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, std::
       order[l] = l;
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plo[a] < plo[b]; });
+    // "sap" sweeps while lower_j < upper_i; "nxn" (broad_phase_nxn.py:129-214) and "explicit"
+    // (:28-65, the pairs of a list, AABB test only) visit every pair of the segment
+    const bool sweep = m->broad_phase_mode == 0, listed = m->broad_phase_mode == 2;
     for (int i = 0; i < n; ++i) {
       T upper = phi[order[i]];
-      for (int j = i + 1; j < n && plo[order[j]] < upper; ++j) {
+      for (int j = i + 1; j < n && (!sweep || plo[order[j]] < upper); ++j) {
         int t1 = m->segment_shape[s0 + order[i]], t2 = m->segment_shape[s0 + order[j]];
         if (t1 == t2) continue;
         int a = std::min(t1, t2), b = std::max(t1, t2);
         int w1 = m->shape_world[a], w2 = m->shape_world[b];
         if (w1 == -1 && w2 == -1 && seg < regular) continue;
-        if (!test_world_and_group_pair(w1, w2, m->shape_group[a], m->shape_group[b])) continue;
-        if (is_pair_excluded(a, b, m->filter_pairs, m->filter_count)) continue;
+        if (listed) {
+          if (!is_pair_excluded(a, b, m->explicit_pairs, m->explicit_pair_count)) continue;  // membership
+        } else {
+          if (!test_world_and_group_pair(w1, w2, m->shape_group[a], m->shape_group[b])) continue;
+          if (is_pair_excluded(a, b, m->filter_pairs, m->filter_count)) continue;
+        }
         if (!aabb_overlap(sw[a], sw[b])) continue;
         out.push_back({a, b});
       }

@@ -105,3 +105,51 @@ def test_per_shape_gap_known_answer_gpu(cuda_device):
     for cls in (BroadPhaseAllPairs, BroadPhaseSAP):
         n, got = _run(cls(world), cuda_device, lower, upper, gap, group, world, None, 8)
         assert n == 1 and tuple(got[0]) == (0, 2)
+
+
+# ---------------------------------------------------------------- the pipeline's broad-phase modes
+def _pipe_vs_oracle(model, mode, q=None, explicit=None):
+    from newton_custom_contact_amd import CollisionPipeline
+    from parity_helpers import compare_contacts
+
+    q = model.host["body_q"] if q is None else q
+    pipe = CollisionPipeline(model, broad_phase=mode, deterministic=True, shape_pairs_filtered=explicit)
+    o = oracle.Oracle(model, broad_phase=mode, explicit_pairs=explicit)
+    state = model.state()
+    state.body_q.copy_(torch.from_numpy(np.ascontiguousarray(q, np.float32)))
+    c = pipe.contacts()
+    pipe.collide(state, c)
+    torch.cuda.synchronize()
+    ref = o.collide(q, capacity=c.rigid_contact_max)
+    return compare_contacts(c.numpy(), ref)
+
+
+def test_pipeline_modes_corner_touching(cuda_device):
+    from tests.test_oracle_broad import corner_boxes_model
+
+    print("[pipeline broad phase] corner-touching boxes", flush=True)
+    counts = {"sap": 0, "nxn": 0, "explicit": 0}
+    for off in np.arange(2, 12) / 8.0:
+        model = corner_boxes_model(float(off), cuda_device)
+        for mode in counts:
+            counts[mode] += _pipe_vs_oracle(model, mode)["count"]
+    print(counts, flush=True)
+    assert counts["nxn"] == counts["explicit"] == 10 and counts["sap"] < 10
+
+
+def test_pipeline_explicit_list_matches_oracle(cuda_device):
+    from newton_custom_contact_amd import scenes
+
+    print("[pipeline broad phase] explicit list", flush=True)
+    model = scenes.mixed_worlds(8, seed=3, device=cuda_device)
+    allp = model.shape_contact_pairs.cpu().numpy()
+    rng = np.random.Generator(np.random.PCG64(3))
+    sub = allp[rng.random(len(allp)) < 0.4]
+    q = model.host["body_q"].copy()
+    q[:, 2] *= 0.3  # bodies lowered into one another and the ground
+    q[:, :2] *= 0.4
+    rep_all = _pipe_vs_oracle(model, "explicit", q)
+    rep_nxn = _pipe_vs_oracle(model, "nxn", q)
+    rep_sub = _pipe_vs_oracle(model, "explicit", q, explicit=sub)
+    print(rep_all, rep_nxn, rep_sub, flush=True)
+    assert rep_all["count"] == rep_nxn["count"] > rep_sub["count"] > 0
