@@ -51,6 +51,9 @@ def parse():
     p.add_argument("--residual-tol", type=float, default=5e-3,
                    help="per-world penetration residual above which the learned form falls back to a cold solve")
     p.add_argument("--no-sdf", action="store_true", help="skip the SDF (triangle-mesh) workload")
+    p.add_argument("--no-components", action="store_true",
+                   help="skip the per-launch timings of contact matching, force reporting, standalone broad "
+                        "phases and the SDF cache")
     p.add_argument("--sdf-worlds", type=int, default=1024)
     p.add_argument("--sdf-steps", type=int, default=60)
     p.add_argument("--sdf-warmup", type=int, default=60)
@@ -178,7 +181,7 @@ def main():
     b_step = 76 * N_b + 96 * N_b + 16 * N_s + 76 * n_c + 52 * N_b + 120 * N_j
     b_ws = args.substeps * (b_collide + b_step)
     roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
-    collide_kernels = ("collide_segment_kernel", "route_scan_kernel", "narrow_kernel", "contact_count_kernel",
+    collide_kernels = ("collide_segment", "route_scan_kernel", "narrow_kernel", "contact_count_kernel",
                        "contact_scan_kernel", "contact_compact_kernel")
     if t_xp_ms >= t_col_ms:
         dom, dom_ms, dom_bytes = "xpbd_world_kernel", t_xp_ms, args.worlds * b_step
@@ -199,6 +202,10 @@ def main():
     sdf = None
     if not args.no_sdf:
         sdf = sdf_form(args, dev)
+
+    components = None
+    if not args.no_components:
+        components = components_form(args, model, states[0], sdt, dev)
 
     line = {
         "metric": "world-steps/s @4096 worlds; iters to 1e-4 m pen residual, learned vs plain",
@@ -239,6 +246,7 @@ def main():
         "cpu_baseline": cpu,
         "learned": learned,
         "sdf": sdf,
+        "components": components,
     }
     print(json.dumps(line), flush=True)
 
@@ -342,6 +350,104 @@ def learned_form(args, model, dev, plain_ws_per_s):
                                  f"plain_{args.iterations}": plain10},
         "network": "16-128-128-1 SiLU, BF16 MFMA / FP32 accumulate, newton_custom_contact_amd/data/warmstart_mlp.npz",
     }
+
+
+def components_form(args, model, state, sdt, dev):
+    """Per-launch cost of the path's other entry points on the plain workload's end state (HIP events on the
+    launch stream, ``--kernel-reps`` back-to-back launches): contact matching (collide with matching minus the
+    deterministic collide), the reporting XPBD step and ``update_contacts``, the standalone broad phases over
+    the end state's bounding boxes; and the SDF cache (host wall time of a 64^3 cook + write and of a load).
+    Runs last: it requests ``Contacts.force`` and ``State.body_parent_f`` on the model."""
+    import tempfile
+
+    import torch
+
+    from newton_custom_contact_amd import CollisionPipeline, scenes
+    from newton_custom_contact_amd.geometry import BroadPhaseAllPairs, BroadPhaseExplicit, BroadPhaseSAP
+    from newton_custom_contact_amd.solvers import SolverXPBD
+
+    stream = torch.cuda.current_stream()
+    reps = args.kernel_reps
+
+    def timed_us(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return round(e0.elapsed_time(e1) / reps * 1e3, 1)
+
+    out = {"workload": f"end state of the plain workload ({args.worlds} worlds x 32 bodies)"}
+    # contact matching
+    det = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    cd = det.contacts()
+    t_det = timed_us(lambda: det.collide(state, cd))
+    match = {"deterministic_collide_us": t_det}
+    for mode in ("latest", "sticky"):
+        mp = CollisionPipeline(model, broad_phase="sap", contact_matching=mode, contact_report=True)
+        cm = mp.contacts()
+        match[f"{mode}_with_report_us"] = round(timed_us(lambda: mp.collide(state, cm)) - t_det, 1)
+    out["contact_matching_us_per_substep"] = match
+    # reported forces
+    solver = SolverXPBD(model, iterations=args.iterations)
+    c_plain = det.contacts()
+    det.collide(state, c_plain)
+    s_out = model.state()
+    t_plain = timed_us(lambda: solver.step(state, s_out, None, c_plain, sdt))
+    model.request_contact_attributes("force")
+    model.request_state_attributes("body_parent_f")
+    c_rep = det.contacts()
+    det.collide(state, c_rep)
+    s_rep = model.state()
+    t_rep = timed_us(lambda: solver.step(state, s_rep, None, c_rep, sdt))
+    t_upd = timed_us(lambda: solver.update_contacts(c_rep))
+    out["force_reporting_us_per_substep"] = {"plain_step": t_plain, "reporting_step": t_rep,
+                                             "update_contacts": t_upd}
+    # standalone broad phases over the end state's bounding spheres' boxes
+    h = model.host
+    sb = torch.from_numpy(h["shape_body"].astype(np.int64)).to(dev)
+    xf = torch.from_numpy(np.ascontiguousarray(h["shape_transform"][:, :3], np.float32)).to(dev)
+    pos = torch.where((sb >= 0)[:, None], state.body_q[sb.clamp(min=0), :3], xf)
+    r = torch.from_numpy(np.ascontiguousarray(h["shape_collision_radius"], np.float32)).to(dev)[:, None]
+    lower, upper = (pos - r).contiguous(), (pos + r).contiguous()
+    gap = model.shape_gap
+    group, world = model.shape_collision_group, model.shape_world
+    fpairs = model.shape_filter_pairs if len(h["shape_filter_pairs"]) else None
+    cap = 4 << 20
+    pairs = torch.zeros((cap, 2), dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    bp = {}
+    for name, cls in (("sap", BroadPhaseSAP), ("nxn", BroadPhaseAllPairs)):
+        inst = cls(h["shape_world"], h["shape_flags"])
+        t = timed_us(lambda: inst.launch(lower, upper, gap, group, world, model.shape_count, pairs, cnt,
+                                         filter_pairs=fpairs))
+        bp[name] = {"us": t, "pairs": int(cnt.item())}
+    ex = BroadPhaseExplicit()
+    sp = model.shape_contact_pairs
+    t = timed_us(lambda: ex.launch(lower, upper, gap, sp, sp.shape[0], pairs, cnt))
+    bp["explicit"] = {"us": t, "pairs": int(cnt.item()), "input_pairs": int(sp.shape[0])}
+    out["standalone_broad_phase_us"] = bp
+    # SDF cache: cook + publish, then load, of one 64^3 grid
+    with tempfile.TemporaryDirectory() as d:
+        m1 = scenes.mesh_library(1, seed=args.seed)[0]
+        m2 = scenes.mesh_library(1, seed=args.seed)[0]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m1.build_sdf(max_resolution=64, device=dev, cache_dir=d)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        m2.build_sdf(max_resolution=64, device=dev, cache_dir=d)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        same = bool(torch.equal(m1.sdf.grid, m2.sdf.grid))
+    out["sdf_cache_ms"] = {"cook_and_write": round((t1 - t0) * 1e3, 2), "load": round((t2 - t1) * 1e3, 2),
+                           "grid": "64 cells on the longest axis, fp32", "load_equals_cook": same}
+    print(f"[bench] components: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
 
 
 def sdf_form(args, dev):

@@ -6,7 +6,8 @@
 // broad_phase_common.py:116-128).  The reference appends through one global atomic counter, so its order
 // depends on scheduling; every launch here writes one fixed order, with no atomics:
 //   all pairs  segment by segment; inside a segment row i, then column j (i < j, index-map order)
-//   SAP        segment by segment; inside a segment by sweep origin in sorted order, then sweep position
+//   SAP        segment by segment; inside a segment as all pairs, except segments of BP_ENUM_MAX..BP_LDS_MAX
+//              shapes: by sweep origin in sorted order, then sweep position
 //   explicit   input order
 // Each call is three launches: count (one workgroup per segment, or per block of input pairs), an exclusive
 // scan of the counts by one workgroup, and write (the workgroup counts again, scans its lanes' counts and
@@ -18,8 +19,10 @@
 // SAP adds its sweep: each segment is sorted by the projection of its AABBs on the fixed axis (ties keep
 // index-map order, the stable segmented sort) and element a sweeps the elements b after it while
 // lower(b) < upper(a) (broad_phase_sap.py:189-250).  A pair is therefore found iff, with a the earlier of
-// the two in that order, lower(b) < upper(a) and the tests pass.  Segments of up to BP_LDS_MAX shapes sort
-// in LDS and sweep; larger ones evaluate that predicate for every pair (the same set, O(n^2) work).
+// the two in that order, lower(b) < upper(a) and the tests pass.  Segments of up to BP_ENUM_MAX shapes
+// evaluate that predicate on every pair (all 256 lanes busy, no sort: faster than a sort and a serial sweep
+// at these sizes); segments up to BP_LDS_MAX sort in LDS and sweep; larger ones enumerate again, reading
+// global memory (the same set, O(n^2) work).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -33,6 +36,7 @@
 namespace nmc {
 
 constexpr int BP_LDS_MAX = 2048;         // shapes per segment held in LDS (56 B each for SAP)
+constexpr int BP_ENUM_MAX = 256;         // SAP segments up to this size enumerate their pairs (no sort)
 constexpr int BP_EXPLICIT_ITEMS = 4;     // explicit pairs per lane per block
 constexpr int BP_MAX_BLOCKS = 4096;
 
@@ -134,16 +138,43 @@ __device__ __forceinline__ void bp_emit(const BpArgs& a, long long at, int s1, i
   }
 }
 
-// ---------------------------------------------------------------------------------------- all pairs
-// Lane L takes the contiguous run [L c, (L + 1) c) of the segment's n(n-1)/2 pairs in row order.
+// ---------------------------------------------------------------------------------------- pair enumeration
+// Lane L takes the contiguous run [L c, (L + 1) c) of the segment's n(n-1)/2 pairs in row order.  SAP adds
+// the sweep predicate: the earlier of the two in sort order (projected lower bound, then index-map position)
+// sweeps the later one, so the pair counts when lower(later) < upper(earlier).  Shapes (and projections)
+// come from LDS when the segment fits, else from global memory.
 __device__ __forceinline__ long long bp_row_start(long long i, long long n) { return i * (2 * n - i - 1) / 2; }
 
-template <bool WRITE>
-__device__ int bp_all_pairs_segment(const BpArgs& a, int seg, const BpShape* sh, int* s_wave, long long out_base) {
+struct BpMember {
+  BpShape s;
+  float lo, hi;  // projection (SAP)
+};
+
+template <bool SAP>
+__device__ __forceinline__ BpMember bp_member(const BpArgs& a, int s0, int l, bool in_lds, const BpShape* sh,
+                                              const float* plo, const float* phi, f3 dir) {
+  BpMember m;
+  if (in_lds) {
+    m.s = sh[l];
+    if (SAP) {
+      m.lo = plo[l];
+      m.hi = phi[l];
+    }
+  } else {
+    m.s = bp_load(a, a.index_map[s0 + l]);
+    if (SAP) bp_project(m.s, dir, m.lo, m.hi);
+  }
+  return m;
+}
+
+template <bool WRITE, bool SAP>
+__device__ int bp_enum_segment(const BpArgs& a, int seg, const BpShape* sh, const float* plo, const float* phi,
+                               int* s_wave, long long out_base) {
   const int s0 = a.seg_start[seg];
   const int n = a.seg_start[seg + 1] - s0;
   const bool dedicated = seg == a.segments - 1;
   const bool in_lds = n <= a.lds_n;
+  const f3 dir = normalize(f3{0.5935f, 0.7790f, 0.1235f});
   const long long P = static_cast<long long>(n) * (n - 1) / 2;
   const long long chunk = (P + CBLOCK - 1) / CBLOCK;
   const long long p0 = chunk * threadIdx.x;
@@ -167,18 +198,25 @@ __device__ int bp_all_pairs_segment(const BpArgs& a, int seg, const BpShape* sh,
       bp_block_scan(found, ex, s_wave);
       at = out_base + ex;
     }
-    BpShape P_i = in_lds ? sh[ii < n ? ii : 0] : bp_load(a, a.index_map[s0 + (ii < n ? ii : 0)]);
-    for (long long p = p0; p < p1; ++p) {
-      const BpShape Q = in_lds ? sh[jj] : bp_load(a, a.index_map[s0 + jj]);
-      int s1, s2;
-      if (bp_pair_ok(a, P_i, Q, dedicated, s1, s2)) {
-        if (WRITE && pass == 1) bp_emit(a, at + k, s1, s2);
-        ++k;
-      }
-      if (++jj == n) {
-        ++ii;
-        jj = ii + 1;
-        if (ii < n) P_i = in_lds ? sh[ii] : bp_load(a, a.index_map[s0 + ii]);
+    if (p0 < p1) {
+      BpMember A = bp_member<SAP>(a, s0, ii, in_lds, sh, plo, phi, dir);
+      for (long long p = p0; p < p1; ++p) {
+        const BpMember B = bp_member<SAP>(a, s0, jj, in_lds, sh, plo, phi, dir);
+        bool take = true;
+        if (SAP) {
+          const bool a_first = bp_order_key(A.lo, ii) < bp_order_key(B.lo, jj);
+          take = a_first ? (B.lo < A.hi) : (A.lo < B.hi);
+        }
+        int s1, s2;
+        if (take && bp_pair_ok(a, A.s, B.s, dedicated, s1, s2)) {
+          if (WRITE && pass == 1) bp_emit(a, at + k, s1, s2);
+          ++k;
+        }
+        if (++jj == n) {
+          ++ii;
+          jj = ii + 1;
+          if (ii < n - 1) A = bp_member<SAP>(a, s0, ii, in_lds, sh, plo, phi, dir);
+        }
       }
     }
     found = k;
@@ -243,65 +281,6 @@ __device__ int bp_sap_segment_lds(const BpArgs& a, int seg, const BpShape* sh, c
   return 0;
 }
 
-// Segments above BP_LDS_MAX: every pair in row order, with the sweep predicate.
-template <bool WRITE>
-__device__ int bp_sap_segment_big(const BpArgs& a, int seg, int* s_wave, long long out_base) {
-  const int s0 = a.seg_start[seg];
-  const int n = a.seg_start[seg + 1] - s0;
-  const bool dedicated = seg == a.segments - 1;
-  const f3 dir = normalize(f3{0.5935f, 0.7790f, 0.1235f});
-  const long long P = static_cast<long long>(n) * (n - 1) / 2;
-  const long long chunk = (P + CBLOCK - 1) / CBLOCK;
-  const long long p0 = chunk * threadIdx.x;
-  const long long p1 = p0 + chunk < P ? p0 + chunk : P;
-  int i = 0, j = 0;
-  if (p0 < p1) {
-    int lo = 0, hi = n - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (bp_row_start(mid, n) <= p0) lo = mid; else hi = mid - 1;
-    }
-    i = lo;
-    j = static_cast<int>(p0 - bp_row_start(i, n)) + i + 1;
-  }
-  int found = 0;
-  for (int pass = 0; pass < (WRITE ? 2 : 1); ++pass) {
-    int ii = i, jj = j, k = 0;
-    long long at = 0;
-    if (WRITE && pass == 1) {
-      int ex;
-      bp_block_scan(found, ex, s_wave);
-      at = out_base + ex;
-    }
-    BpShape P_i = bp_load(a, a.index_map[s0 + (ii < n ? ii : 0)]);
-    float lo_i, hi_i;
-    bp_project(P_i, dir, lo_i, hi_i);
-    for (long long p = p0; p < p1; ++p) {
-      const BpShape Q = bp_load(a, a.index_map[s0 + jj]);
-      float lo_j, hi_j;
-      bp_project(Q, dir, lo_j, hi_j);
-      // the earlier of the two in sort order sweeps the later one
-      const bool i_first = bp_order_key(lo_i, ii) < bp_order_key(lo_j, jj);
-      const bool swept = i_first ? (lo_j < hi_i) : (lo_i < hi_j);
-      int s1, s2;
-      if (swept && bp_pair_ok(a, P_i, Q, dedicated, s1, s2)) {
-        if (WRITE && pass == 1) bp_emit(a, at + k, s1, s2);
-        ++k;
-      }
-      if (++jj == n) {
-        ++ii;
-        jj = ii + 1;
-        if (ii < n) {
-          P_i = bp_load(a, a.index_map[s0 + ii]);
-          bp_project(P_i, dir, lo_i, hi_i);
-        }
-      }
-    }
-    found = k;
-  }
-  return found;
-}
-
 // One workgroup per segment (grid-stride).  WRITE = false: pairs per segment into seg_count; WRITE = true:
 // pairs at seg_off[seg].
 template <bool WRITE>
@@ -319,6 +298,8 @@ __global__ void __launch_bounds__(CBLOCK) bp_segment_kernel(BpArgs a) {
     const int s0 = a.seg_start[seg];
     const int n = a.seg_start[seg + 1] - s0;
     const bool in_lds = n <= a.lds_n;
+    // SAP segments above BP_ENUM_MAX shapes held in LDS sort and sweep; every other segment enumerates pairs
+    const bool sweep = a.sap && in_lds && n > BP_ENUM_MAX;
     if (in_lds) {
       for (int l = threadIdx.x; l < n; l += CBLOCK) {
         const BpShape s = bp_load(a, a.index_map[s0 + l]);
@@ -331,7 +312,7 @@ __global__ void __launch_bounds__(CBLOCK) bp_segment_kernel(BpArgs a) {
           keys[l] = bp_order_key(lo, l);
         }
       }
-      if (a.sap) {
+      if (sweep) {
         int n2 = 1;
         while (n2 < n) n2 <<= 1;
         for (int l = n + threadIdx.x; l < n2; l += CBLOCK) keys[l] = ~0ull;
@@ -342,11 +323,12 @@ __global__ void __launch_bounds__(CBLOCK) bp_segment_kernel(BpArgs a) {
     }
     const long long base = WRITE ? a.seg_off[seg] : 0;
     int c;
-    if (a.sap) {
-      c = in_lds ? bp_sap_segment_lds<WRITE>(a, seg, sh, plo, phi, keys, s_wave, base)
-                 : bp_sap_segment_big<WRITE>(a, seg, s_wave, base);
+    if (sweep) {
+      c = bp_sap_segment_lds<WRITE>(a, seg, sh, plo, phi, keys, s_wave, base);
+    } else if (a.sap) {
+      c = bp_enum_segment<WRITE, true>(a, seg, sh, plo, phi, s_wave, base);
     } else {
-      c = bp_all_pairs_segment<WRITE>(a, seg, sh, s_wave, base);
+      c = bp_enum_segment<WRITE, false>(a, seg, sh, plo, phi, s_wave, base);
     }
     if (!WRITE) {
       if (threadIdx.x == 0) s_sum = 0;

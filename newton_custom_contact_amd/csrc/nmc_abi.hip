@@ -1,6 +1,7 @@
 // extern "C" entry points of libnmc.so (declared in include/nmc.h).
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -337,10 +338,30 @@ namespace nmc {
 __global__ void __launch_bounds__(256) contact_force_kernel(const float* __restrict__ impulse,
                                                            const int* __restrict__ count, int capacity, float dt,
                                                            float* __restrict__ force) {
-  const int n = count[0];
+  // the buffers as flat float arrays of 6 x capacity, written 16 bytes per lane; float f belongs to contact f / 6
+  const long long n6 = 6ll * static_cast<long long>(count[0] < capacity ? count[0] : capacity);
+  const long long total = 6ll * capacity;
   const float inv_dt = 1.0f / dt;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < capacity; i += gridDim.x * blockDim.x) {
-    for (int k = 0; k < 6; ++k) force[6ll * i + k] = i < n ? impulse[6ll * i + k] * inv_dt : 0.0f;
+  const long long nq = total >> 2;
+  const float4* __restrict__ in4 = reinterpret_cast<const float4*>(impulse);
+  float4* __restrict__ out4 = reinterpret_cast<float4*>(force);
+  for (long long q = blockIdx.x * static_cast<long long>(blockDim.x) + threadIdx.x; q < nq;
+       q += static_cast<long long>(gridDim.x) * blockDim.x) {
+    const long long f = 4 * q;
+    float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (f < n6) {
+      const float4 im = in4[q];
+      v.x = im.x * inv_dt;
+      v.y = f + 1 < n6 ? im.y * inv_dt : 0.0f;
+      v.z = f + 2 < n6 ? im.z * inv_dt : 0.0f;
+      v.w = f + 3 < n6 ? im.w * inv_dt : 0.0f;
+    }
+    out4[q] = v;
+  }
+  // 6 x capacity is even: at most two trailing floats
+  if (blockIdx.x == 0 && threadIdx.x < static_cast<unsigned>(total - 4 * nq)) {
+    const long long f = 4 * nq + threadIdx.x;
+    force[f] = f < n6 ? impulse[f] * inv_dt : 0.0f;
   }
 }
 }  // namespace nmc
@@ -349,7 +370,11 @@ extern "C" int nmc_contact_force(const float* contact_impulse, const int* count,
                                  nmc_stream_t stream) {
   if (!contact_impulse || !count || !force) return fail("nmc_contact_force: null argument");
   if (capacity <= 0) return 0;
-  long long blocks = (capacity + 255) / 256;
+  // flat float4 access needs 16-byte aligned buffers
+  if ((reinterpret_cast<uintptr_t>(contact_impulse) | reinterpret_cast<uintptr_t>(force)) & 15)
+    return fail("nmc_contact_force: impulse and force buffers must be 16-byte aligned");
+  long long blocks = (6ll * capacity / 4 + 255) / 256;
+  if (blocks < 1) blocks = 1;
   if (blocks > 4096) blocks = 4096;
   hipLaunchKernelGGL(contact_force_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
                      reinterpret_cast<hipStream_t>(stream), contact_impulse, count, capacity, dt, force);
