@@ -116,7 +116,18 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     m->xpbd_contact_cap = static_cast<int>(want < 1024 ? 1024 : (want > 8192 ? 8192 : want));
   }
   m->xpbd_keys_cap = 2ll * (d.rigid_contact_max > 0 ? d.rigid_contact_max : 1);
-  m->xpbd_lds = xpbd_lds_bytes(d.max_world_bodies, d.max_world_joints, m->xpbd_contact_cap);
+  m->xpbd_joint_lds = 0;
+  if (d.joint_count > 0) {
+    std::vector<int> jt(d.joint_count), je(d.joint_count);
+    if ((e = hipMemcpy(jt.data(), d.joint_type, sizeof(int) * d.joint_count, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(je.data(), d.joint_enabled, sizeof(int) * d.joint_count, hipMemcpyDeviceToHost)) != hipSuccess) {
+      delete m;
+      return hip_fail(e, "nmc_model_create: joint tables");
+    }
+    for (int j = 0; j < d.joint_count; ++j)
+      if (je[j] && jt[j] != JT_FREE && jt[j] != JT_DISTANCE) m->xpbd_joint_lds = d.max_world_joints > 0 ? d.max_world_joints : 1;
+  }
+  m->xpbd_lds = xpbd_lds_bytes(d.max_world_bodies, m->xpbd_joint_lds, m->xpbd_contact_cap);
   if (m->collide_lds > 160 * 1024 || m->xpbd_lds > 160 * 1024) {
     delete m;
     return fail("per-world LDS requirement exceeds 160 KiB");

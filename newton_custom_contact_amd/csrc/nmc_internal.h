@@ -65,6 +65,8 @@ struct NmcModelImpl {
   size_t collide_lds;
   size_t xpbd_lds;
   int xpbd_contact_cap;  // (contact, side) entries a world orders in LDS
+  int xpbd_joint_lds;    // joints per world with LDS delta slots: max_world_joints when some enabled joint is
+                         // not FREE / DISTANCE (the only joints the joint pass solves), else 0
   unsigned* xpbd_keys;   // [xpbd_keys_cap] ordering entries of denser worlds, at 2 x their contact offset
   long long xpbd_keys_cap;
 };

@@ -431,6 +431,7 @@ struct XpbdArgs {
   float* q_out;
   float* qd_out;
   int key_cap;  // (contact, side) entries the LDS ordering buffer holds
+  int joint_lds;  // joints per world with LDS delta slots (0 when no world has a non-FREE joint)
   unsigned* keys_global;  // ordering buffer of worlds whose entries exceed key_cap (2 per contact slot)
   long long keys_global_cap;
   int* counters;
@@ -620,7 +621,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   const int j0 = d.world_joint_first[w];
   const int nj = d.world_joint_count[w];
   const int mb = d.max_world_bodies > 0 ? d.max_world_bodies : 1;
-  const int mj = d.max_world_joints > 0 ? d.max_world_joints : 1;
+  const int mj = a.joint_lds;
   constexpr int CH = BS * CPL;  // contacts per chunk
   const float dt = a.dt;
   const int tid = threadIdx.x;
@@ -1192,7 +1193,7 @@ size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap) {
 
 size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints, int key_cap) {
   int bs = sh.bs;
-  size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 1;
+  size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 0;
   size_t ch = (size_t)bs * sh.cpl;
   size_t b = sizeof(float) * (7 + 6 + 6 + 3 + 1 + 9 + 9) * mb + sizeof(int) * mb + 2 * sizeof(int) * (mb + 1);
   b += sizeof(int) * bs + sizeof(float) * 12 * ch + sizeof(int) * ch + sizeof(float) * 12 * mj + sizeof(int) * mj;
@@ -1228,6 +1229,7 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   a.q_out = q_out;
   a.qd_out = qd_out;
   a.key_cap = m->xpbd_contact_cap;
+  a.joint_lds = m->xpbd_joint_lds;
   a.keys_global = m->xpbd_keys;
   a.keys_global_cap = m->xpbd_keys_cap;
   a.counters = m->counters;
@@ -1235,7 +1237,7 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   if (rep) a.rep = *rep; else a.rep = NmcXpbdReport{};
   int worlds = m->d.world_count + 1;
   XpbdShape sh = xpbd_shape(m->d.max_world_bodies, params->block_lanes);
-  const size_t lds = xpbd_lds_bytes_shape(sh, m->d.max_world_bodies, m->d.max_world_joints, m->xpbd_contact_cap);
+  const size_t lds = xpbd_lds_bytes_shape(sh, m->d.max_world_bodies, m->xpbd_joint_lds, m->xpbd_contact_cap);
   if (rep) {
     launch_mode<XM_REPORT>(sh, worlds, lds, stream, a);
     return hipGetLastError();
