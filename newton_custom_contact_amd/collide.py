@@ -93,7 +93,10 @@ def _explicit_pair_list(model, pairs) -> torch.Tensor:
 
 
 class CollisionPipeline:
-    """GPU collision pipeline with the reference's constructor and ``collide`` API."""
+    """GPU collision pipeline with the reference's constructor and ``collide`` API.
+
+    ``collide`` may be captured in a graph; the graph then points into this pipeline's device scratch, so the
+    pipeline (and the model, states and contacts it was captured with) must outlive the graph."""
 
     def __init__(self, model, *, reduce_contacts: bool = True, rigid_contact_max: int | None = None,
                  max_triangle_pairs: int = 1000000, shape_pairs_filtered=None, soft_contact_max: int | None = None,
