@@ -51,6 +51,12 @@ def parse():
     p.add_argument("--residual-tol", type=float, default=5e-3,
                    help="per-world penetration residual above which the learned form falls back to a cold solve")
     p.add_argument("--no-sdf", action="store_true", help="skip the SDF (triangle-mesh) workload")
+    p.add_argument("--uneven", action="store_true",
+                   help="also run the uneven-worlds workload (log-uniform 8-256 bodies per world + revolute chains; "
+                        "its scene build takes minutes on the host, so it is not part of the default run)")
+    p.add_argument("--uneven-worlds", type=int, default=2048)
+    p.add_argument("--uneven-steps", type=int, default=30)
+    p.add_argument("--uneven-warmup", type=int, default=30)
     p.add_argument("--no-components", action="store_true",
                    help="skip the per-launch timings of contact matching, force reporting, standalone broad "
                         "phases and the SDF cache")
@@ -203,6 +209,10 @@ def main():
     if not args.no_sdf:
         sdf = sdf_form(args, dev)
 
+    uneven = None
+    if args.uneven:
+        uneven = uneven_form(args, dev)
+
     components = None
     if not args.no_components:
         components = components_form(args, model, states[0], sdt, dev)
@@ -246,6 +256,7 @@ def main():
         "cpu_baseline": cpu,
         "learned": learned,
         "sdf": sdf,
+        "uneven": uneven,
         "components": components,
     }
     print(json.dumps(line), flush=True)
@@ -448,6 +459,101 @@ def components_form(args, model, state, sdt, dev):
                            "grid": "64 cells on the longest axis, fp32", "load_equals_cook": same}
     print(f"[bench] components: {json.dumps(out)}", file=sys.stderr, flush=True)
     return out
+
+
+def uneven_form(args, dev):
+    """Uneven worlds: ``--uneven-worlds`` worlds of log-uniform 8-256 primitives (spheres, boxes, capsules) plus
+    a 6-link revolute capsule chain with +-90 degree limits each, on a ground plane; 4 substeps, 10 iterations,
+    the step kernel's launch shape autotuned, hipGraph step, timed after ``--uneven-warmup`` steps."""
+    import torch
+
+    from newton_custom_contact_amd import CollisionPipeline, scenes
+    from newton_custom_contact_amd.autotune import tune_block_lanes
+    from newton_custom_contact_amd.solvers import SolverXPBD
+
+    t0 = time.time()
+    model = scenes.uneven_worlds(args.uneven_worlds, seed=args.seed, device=dev)
+    W = args.uneven_worlds
+    print(f"[bench] uneven scene: {W} worlds, {model.body_count} bodies, {model.joint_count} joints "
+          f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    pipe = CollisionPipeline(model, broad_phase="sap")
+    solver = SolverXPBD(model, iterations=args.iterations)
+    contacts = pipe.contacts()
+    states = [model.state(), model.state()]
+    sdt = (1.0 / 60.0) / args.substeps
+    nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step():
+        for s in range(args.substeps):
+            a, b = states[s % 2], states[(s + 1) % 2]
+            pipe.collide(a, contacts)
+            solver.step(a, b, None, contacts, sdt)
+            nc_acc.add_(contacts.rigid_contact_count)
+        if args.substeps % 2:
+            states[0].body_q.copy_(states[1].body_q)
+            states[0].body_qd.copy_(states[1].body_qd)
+
+    for _ in range(args.uneven_warmup):
+        step()
+    pipe.collide(states[0], contacts)
+    tune = tune_block_lanes(solver, states[0], states[1], contacts, sdt)
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        step()
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    graph = None
+    if not args.no_graph:
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(graph):
+            step()
+        torch.cuda.synchronize()
+    for _ in range(3):
+        graph.replay() if graph is not None else step()
+    torch.cuda.synchronize()
+    nc_acc.zero_()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.uneven_steps):
+        graph.replay() if graph is not None else step()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    ws = W * args.uneven_steps / (ms / 1e3)
+    n_c = float(nc_acc.item()) / (args.uneven_steps * args.substeps * W)
+    stream = torch.cuda.current_stream()
+    reps = args.kernel_reps
+    a, b = states[0], states[1]
+    pipe.collide(a, contacts)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    ev[0].record(stream)
+    for _ in range(reps):
+        pipe.collide(a, contacts)
+    ev[1].record(stream)
+    ev[2].record(stream)
+    for _ in range(reps):
+        solver.step(a, b, None, contacts, sdt)
+    ev[3].record(stream)
+    torch.cuda.synchronize()
+    oc = pipe.overflow_counters()
+    print(f"[bench] uneven: {ws:.1f} world-steps/s, N_c={n_c:.1f}", file=sys.stderr, flush=True)
+    del graph
+    return {
+        "value": round(ws, 1),
+        "unit": "world-steps/s",
+        "ms_per_step": round(ms / args.uneven_steps, 4),
+        "workload": f"{W} worlds of log-uniform 8-256 primitives + a 6-link revolute chain each "
+                    f"({model.body_count} bodies, {model.joint_count} joints) on a ground plane, dt=1/60 s, "
+                    f"{args.substeps} substeps, {args.iterations} iterations, timed after {args.uneven_warmup} steps",
+        "mean_contacts_per_world_substep": round(n_c, 2),
+        "collide_ms_per_substep": round(ev[0].elapsed_time(ev[1]) / reps, 4),
+        "xpbd_ms_per_substep": round(ev[2].elapsed_time(ev[3]) / reps, 4),
+        "overflow": oc,
+        "step_kernel_lanes": tune,
+    }
 
 
 def sdf_form(args, dev):

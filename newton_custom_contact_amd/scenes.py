@@ -52,13 +52,35 @@ class _Placer:
         self.rad = np.zeros(0)
 
     def place(self, radius: float) -> np.ndarray:
+        """Up to 2000 candidates (x, y, z uniform, drawn in that order) per height band; the first whose sphere
+        clears every placed one is taken.  Candidates are tested in batches; the generator is then left
+        exactly after the taken candidate's three draws, as if they had been drawn one by one."""
+        bg = self.rng.bit_generator
         while True:
-            for _ in range(2000):
-                p = np.array([self.rng.uniform(-1, 1), self.rng.uniform(-1, 1), self.rng.uniform(0.5, self.z_hi)])
-                if np.all(np.linalg.norm(self.pts - p, axis=1) > radius + self.rad):
+            tried = 0
+            while tried < 2000:
+                k = min(64, 2000 - tried)
+                state = bg.state
+                u = self.rng.random((k, 3))
+                cand = np.empty((k, 3))
+                cand[:, 0] = -1.0 + 2.0 * u[:, 0]
+                cand[:, 1] = -1.0 + 2.0 * u[:, 1]
+                cand[:, 2] = 0.5 + (self.z_hi - 0.5) * u[:, 2]
+                if len(self.pts):
+                    ok = np.all(np.linalg.norm(self.pts[None, :, :] - cand[:, None, :], axis=2) > radius + self.rad,
+                                axis=1)
+                else:
+                    ok = np.ones(k, bool)
+                hit = np.flatnonzero(ok)
+                if hit.size:
+                    i = int(hit[0])
+                    bg.state = state
+                    bg.advance(3 * (i + 1))
+                    p = cand[i].copy()
                     self.pts = np.vstack([self.pts, p])
                     self.rad = np.append(self.rad, radius)
                     return p
+                tried += k
             self.z_hi += 0.25
 
 
