@@ -472,7 +472,9 @@ def uneven_form(args, dev):
     from newton_custom_contact_amd.solvers import SolverXPBD
 
     t0 = time.time()
-    model = scenes.uneven_worlds(args.uneven_worlds, seed=args.seed, device=dev)
+    model = scenes.uneven_worlds(args.uneven_worlds, seed=args.seed, device=dev,
+                                 progress=lambda k, n: print(f"[bench] uneven scene: {k}/{n} worlds built",
+                                                             file=sys.stderr, flush=True))
     W = args.uneven_worlds
     print(f"[bench] uneven scene: {W} worlds, {model.body_count} bodies, {model.joint_count} joints "
           f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)

@@ -184,8 +184,9 @@ def mixed_worlds(world_count: int = 4096, seed: int = 0, mu: float = 0.5, device
 
 
 def uneven_worlds(world_count: int = 2048, seed: int = 0, mu: float = 0.5, device=None, lo: int = 8,
-                  hi: int = 256, chain_links: int = 6, chain_limit: float = 0.5 * math.pi):
-    """Worlds with log-uniform body counts in [lo, hi] plus an optional revolute capsule chain."""
+                  hi: int = 256, chain_links: int = 6, chain_limit: float = 0.5 * math.pi, progress=None):
+    """Worlds with log-uniform body counts in [lo, hi] plus an optional revolute capsule chain.
+    ``progress(worlds_done, world_count)`` is called every 128 worlds when given."""
     rng = np.random.Generator(np.random.PCG64(seed))
     b = ModelBuilder()
     cfg = _cfg(mu)
@@ -204,6 +205,8 @@ def uneven_worlds(world_count: int = 2048, seed: int = 0, mu: float = 0.5, devic
         if chain_links > 0:
             _add_chain(b, rng, chain_links, chain_limit, cfg)
         b.end_world()
+        if progress is not None and (_w + 1) % 128 == 0:
+            progress(_w + 1, world_count)
     return b.finalize(device)
 
 
