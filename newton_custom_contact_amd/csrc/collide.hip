@@ -1,11 +1,11 @@
 // Collision pipeline kernels for gfx950.
 //
-// One workgroup owns one broad-phase segment (a world's shapes plus the global shapes, reference
-// precompute_world_map, newton/_src/geometry/broad_phase_common.py:183-300) and runs, out of LDS:
-// shape world transforms + AABBs (collide.py:150-275) and sweep-and-prune on the reference's fixed
-// axis (broad_phase_sap.py:43-394: project, bitonic sort of (lower, index) keys, sweep with the
-// world / group / exclusion / AABB tests), then a bitonic sort of the surviving pairs by contact sort
-// key.  The narrow phase then runs over all pairs of all segments at once, one pair per lane, with the
+// One wave (segments of up to 64 shapes) or one workgroup owns one broad-phase segment (a world's shapes
+// plus the global shapes, reference precompute_world_map, newton/_src/geometry/broad_phase_common.py:183-300)
+// and runs, out of LDS: shape world transforms + AABBs (collide.py:150-275) and the pair tests of the
+// configured broad phase (broad_phase_sap.py:43-394 sweep on the fixed axis, broad_phase_nxn.py every pair
+// or listed pairs) over an enumeration of the segment's pairs in contact-sort-key order, so the pairs come
+// out sorted with no sort (segment_broad_phase_enum).  The narrow phase then runs over all pairs of all segments at once, one pair per lane, with the
 // MPR/GJK pairs grouped apart from the analytic ones.  Each segment owns a fixed region of pair
 // entries and each pair five contact slots, so no kernel uses a global atomic counter (same-address
 // atomics serialise across the chip's eight L2 slices); the two cross-segment prefix sums are done by
@@ -14,7 +14,7 @@
 //
 // Launch sequence of nmc_collide (all on the caller's stream, graph-capturable):
 //   collide_segment_wave_kernel <<<segments/4, 256>>> one wave per segment of <= 64 shapes:
-//                                                 AABB + SAP + pair sort -> pair region, route list
+//                                                 AABB + pair tests in key order -> pair region, route list
 //   collide_segment_kernel  <<<big segments, 256>>> the same for larger segments, one workgroup each
 //   route_scan_kernel       <<<1, 1024>>>         route offsets over segments
 //   narrow_kernel           <<<<=2048, 256>>>     narrow phase -> contact slots, per-pair counts
@@ -34,12 +34,6 @@
 namespace nmc {
 
 constexpr int NARROW_MAX_BLOCKS = 2048;  // 8 workgroups per CU; the narrow kernel strides over the rest
-
-__device__ __forceinline__ int next_pow2(int n) {
-  int p = 1;
-  while (p < n) p <<= 1;
-  return p;
-}
 
 // Inclusive scan of one value per lane over a CBLOCK-lane workgroup (wave shuffles + 4 wave totals).
 __device__ __forceinline__ int block_inclusive_scan(int v, int* s_wave) {
@@ -68,30 +62,121 @@ struct SegLds {
   int* fbeg;   // filter-pair entries with this shape first: [fbeg, fend)
   int* fend;
   unsigned long long* skeys;  // sort keys (n2)
-  unsigned* pairs;            // pair keys (pair_cap)
 };
 
-// Broad phase of one segment by lane group g: shape world data (kept in shape_wc for the narrow phase),
-// SAP, the segment's pairs in key order at staging pairs [base, base + np), and their route entries.
-// smem: the group's dynamic LDS; sc: its 7 counters (pair count, 3 class counts, 3 route bases).
+// Exclusive scan over a lane group of four counters per lane (x: all pairs, y/z/w: route classes 0/1/2);
+// returns this lane's exclusive prefix, `total` gets the group's sums.  s_tmp: CBLOCK / 64 entries (block).
 template <class G>
-__device__ void segment_broad_phase(const G g, const NmcModelDesc& d, const float* __restrict__ body_q,
-                                    const StageBuffers& st, const long long* __restrict__ seg_pair_off,
-                                    const int* __restrict__ seg_pair_cap, int* __restrict__ seg_pair_count,
-                                    int* __restrict__ seg_conv_count, int* __restrict__ seg_mesh_count,
-                                    int* __restrict__ counters, int pair_cap, int max_n, int seg, unsigned char* smem,
-                                    int* sc) {
-  int& s_npairs = sc[0];
-  int* s_cls = sc + 1;
-  int* s_rbase = sc + 4;
+__device__ __forceinline__ int4 group_scan4(const G g, int4 v, int4& total, int4* s_tmp) {
+  const int lane = threadIdx.x & 63;
+  int4 inc = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int tx = __shfl_up(inc.x, off, 64), ty = __shfl_up(inc.y, off, 64);
+    const int tz = __shfl_up(inc.z, off, 64), tw = __shfl_up(inc.w, off, 64);
+    if (lane >= off) {
+      inc.x += tx;
+      inc.y += ty;
+      inc.z += tz;
+      inc.w += tw;
+    }
+  }
+  int4 before = make_int4(0, 0, 0, 0);
+  if (G::size == 64) {
+    total = make_int4(__shfl(inc.x, 63, 64), __shfl(inc.y, 63, 64), __shfl(inc.z, 63, 64), __shfl(inc.w, 63, 64));
+  } else {
+    const int w = threadIdx.x >> 6;
+    if (lane == 63) s_tmp[w] = inc;
+    g.sync();
+    total = make_int4(0, 0, 0, 0);
+    for (int i = 0; i < G::size / 64; ++i) {
+      const int4 t = s_tmp[i];
+      if (i < w) {
+        before.x += t.x;
+        before.y += t.y;
+        before.z += t.z;
+        before.w += t.w;
+      }
+      total.x += t.x;
+      total.y += t.y;
+      total.z += t.z;
+      total.w += t.w;
+    }
+    g.sync();
+  }
+  return make_int4(before.x + inc.x - v.x, before.y + inc.y - v.y, before.z + inc.z - v.z, before.w + inc.w - v.w);
+}
+
+// Broad phase of one segment by lane group g, enumerating pairs instead of sorting and sweeping: the
+// segment's shapes (ranks ordered like global ids) form an n x n matrix of (first, second) candidates,
+// row-major, where `first` is the shape that leads the pair in the narrow phase (lower GeoType, then lower
+// rank; a mesh before an infinite plane, narrow_phase.py:212-215, 303-313).  Row-major order over that
+// matrix IS the pair key order, so each lane takes a contiguous run, counts its accepted pairs per route
+// class, and after one group scan writes them and their route entries at their final positions -- no
+// atomics, no sort, no LDS pair buffer.  The tests are the sweep's (broad_phase_sap.py:189-394): "sap"
+// keeps a pair when the later of the two in (projected lower bound, rank) order starts below the earlier
+// one's upper bound; "nxn" and "explicit" test every pair.  Pairs past the region's capacity are dropped,
+// and counted, in key order.
+struct EnumRun {
+  long long p0, p1;  // this lane's run of the n x n matrix
+  int n;
+  bool regular, sweep, listed;
+};
+
+// Visits the accepted pairs of the lane's run in order: f(a, b, route class).
+template <class F>
+__device__ __forceinline__ void enum_pairs(const NmcModelDesc& d, const SegLds& L, const EnumRun& r, F&& f) {
+  const int n = r.n;
+  if (r.p0 >= r.p1) return;
+  int a = static_cast<int>(r.p0 / n), b = static_cast<int>(r.p0 % n);
+  int ta = L.type[a];
+  for (long long q = r.p0; q < r.p1; ++q) {
+    const int tb = L.type[b];
+    bool first = a != b && (ta < tb || (ta == tb && a < b));
+    if (ta == GEO_PLANE && tb == GEO_MESH) first = false;
+    if (ta == GEO_MESH && tb == GEO_PLANE) first = true;
+    if (first) {
+      const int la = a < b ? a : b, lb = a < b ? b : a;  // lower rank first for the tests
+      const int w1 = L.world[la], w2 = L.world[lb];
+      bool keep = !(w1 == -1 && w2 == -1 && r.regular);
+      if (keep && !r.listed) keep = !(w1 != -1 && w2 != -1 && w1 != w2) && group_pair(L.group[la], L.group[lb]);
+      if (keep && r.sweep) {
+        const bool a_early = L.skeys[a] < L.skeys[b];
+        keep = a_early ? (L.plo[b] < L.phi[a]) : (L.plo[a] < L.phi[b]);
+      }
+      if (keep) {
+        const float c = 0.0f + 0.0f;
+        keep = L.lo[3 * la] <= L.hi[3 * lb] + c && L.hi[3 * la] >= L.lo[3 * lb] - c &&
+               L.lo[3 * la + 1] <= L.hi[3 * lb + 1] + c && L.hi[3 * la + 1] >= L.lo[3 * lb + 1] - c &&
+               L.lo[3 * la + 2] <= L.hi[3 * lb + 2] + c && L.hi[3 * la + 2] >= L.lo[3 * lb + 2] - c;
+      }
+      if (keep) {
+        if (r.listed) keep = pair_listed(d.explicit_pairs, d.explicit_pair_count, L.gid[la], L.gid[lb]);
+        else keep = !excluded(L.gid[lb], d.filter_pairs, L.fbeg[la], L.fend[la]);
+      }
+      if (keep) f(a, b, route_class(ta, tb));
+    }
+    if (++b == n) {
+      b = 0;
+      ++a;
+      if (a < n) ta = L.type[a];
+    }
+  }
+}
+
+template <class G>
+__device__ void segment_broad_phase_enum(const G g, const NmcModelDesc& d, const float* __restrict__ body_q,
+                                         const StageBuffers& st, const long long* __restrict__ seg_pair_off,
+                                         const int* __restrict__ seg_pair_cap, int* __restrict__ seg_pair_count,
+                                         int* __restrict__ seg_conv_count, int* __restrict__ seg_mesh_count,
+                                         int* __restrict__ counters, int max_n, int seg, unsigned char* smem,
+                                         int4* s_tmp) {
   const int s0 = d.segment_start[seg];
   const int n = d.segment_start[seg + 1] - s0;
   const bool regular = seg < d.segment_count - 1;
-  const int n2max = next_pow2(max_n > 1 ? max_n : 1);
 
   SegLds L;
   unsigned char* p = smem;
-  L.skeys = reinterpret_cast<unsigned long long*>(p); p += sizeof(unsigned long long) * n2max;
+  L.skeys = reinterpret_cast<unsigned long long*>(p); p += sizeof(unsigned long long) * max_n;
   L.lo = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
   L.hi = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * max_n;
   L.plo = reinterpret_cast<float*>(p); p += sizeof(float) * max_n;
@@ -101,15 +186,7 @@ __device__ void segment_broad_phase(const G g, const NmcModelDesc& d, const floa
   L.world = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
   L.group = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
   L.fbeg = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
-  L.fend = reinterpret_cast<int*>(p); p += sizeof(int) * max_n;
-  L.pairs = reinterpret_cast<unsigned*>(p);
-
-  if (g.lane == 0) {
-    s_npairs = 0;
-    s_cls[0] = 0;
-    s_cls[1] = 0;
-    s_cls[2] = 0;
-  }
+  L.fend = reinterpret_cast<int*>(p);
 
   // ---- shape world transforms, AABBs, SAP projections (collide.py:150-275, broad_phase_sap.py:43-63)
   const f3 dir = normalize(f3{0.5935f, 0.7790f, 0.1235f});
@@ -138,95 +215,70 @@ __device__ void segment_broad_phase(const G g, const NmcModelDesc& d, const floa
     float plo = center - radius;
     L.plo[l] = plo;
     L.phi[l] = center + radius;
+    // the stable sort's order: projected lower bound, then rank
     L.skeys[l] = (static_cast<unsigned long long>(sortable_float(plo)) << 32) | static_cast<unsigned>(l);
   }
-  const int n2 = next_pow2(n > 1 ? n : 1);
-  for (int l = n + g.lane; l < n2; l += G::size) L.skeys[l] = ~0ull;
   g.sync();
-  bitonic_sort(L.skeys, n2, g);
 
-  // ---- sweep (broad_phase_sap.py:189-394): element i sweeps forward while lower_j < upper_i ("sap"), or
-  // over every later element ("nxn", "explicit")
-  const bool sweep = d.broad_phase_mode == 0;
-  const bool listed = d.broad_phase_mode == 2;
-  for (int i = g.lane; i < n; i += G::size) {
-    int li = static_cast<int>(L.skeys[i] & 0xffffffffu);
-    float upper = L.phi[li];
-    for (int j = i + 1; j < n; ++j) {
-      int lj = static_cast<int>(L.skeys[j] & 0xffffffffu);
-      if (sweep && !(L.plo[lj] < upper)) break;
-      int t1 = L.gid[li], t2 = L.gid[lj];
-      if (t1 == t2) continue;
-      int la = t1 < t2 ? li : lj, lb = t1 < t2 ? lj : li;
-      // the reference's tests form one conjunction; the LDS-only ones go first
-      int w1 = L.world[la], w2 = L.world[lb];
-      if (w1 == -1 && w2 == -1 && regular) continue;
-      if (!listed) {
-        if (w1 != -1 && w2 != -1 && w1 != w2) continue;
-        if (!group_pair(L.group[la], L.group[lb])) continue;
-      }
-      const float c = 0.0f + 0.0f;
-      bool ov = L.lo[3 * la] <= L.hi[3 * lb] + c && L.hi[3 * la] >= L.lo[3 * lb] - c &&
-                L.lo[3 * la + 1] <= L.hi[3 * lb + 1] + c && L.hi[3 * la + 1] >= L.lo[3 * lb + 1] - c &&
-                L.lo[3 * la + 2] <= L.hi[3 * lb + 2] + c && L.hi[3 * la + 2] >= L.lo[3 * lb + 2] - c;
-      if (!ov) continue;
-      if (listed) {
-        if (!pair_listed(d.explicit_pairs, d.explicit_pair_count, L.gid[la], L.gid[lb])) continue;
-      } else if (excluded(L.gid[lb], d.filter_pairs, L.fbeg[la], L.fend[la])) {
-        continue;
-      }
-      // narrow-phase order: lower GeoType first (narrow_phase.py:212-215); ranks order like global ids
-      int ra = la, rb = lb;
-      if (L.type[la] > L.type[lb]) { ra = lb; rb = la; }
-      // mesh-plane pairs are kept as (mesh, plane) (narrow_phase.py:303-313)
-      if (L.type[ra] == GEO_PLANE && L.type[rb] == GEO_MESH) { int t = ra; ra = rb; rb = t; }
-      int slot = atomicAdd(&s_npairs, 1);
-      if (slot < pair_cap) L.pairs[slot] = (static_cast<unsigned>(ra) << 16) | static_cast<unsigned>(rb);
-    }
-  }
-  g.sync();
-  int np = s_npairs;
-  if (np > pair_cap) {
-    if (g.lane == 0) atomicAdd(&counters[0], np - pair_cap);
-    np = pair_cap;
-  }
-  const int pn2 = next_pow2(np > 1 ? np : 1);
-  for (int l = np + g.lane; l < pn2; l += G::size) L.pairs[l] = 0xffffffffu;
-  g.sync();
-  bitonic_sort(L.pairs, pn2, g);
-
-  // ---- pairs out into the segment's region (key order) + its route list (MPR/GJK, analytic, mesh)
+  EnumRun r;
+  r.n = n;
+  r.regular = regular;
+  r.sweep = d.broad_phase_mode == 0;
+  r.listed = d.broad_phase_mode == 2;
+  const long long N2 = static_cast<long long>(n) * n;
+  const long long chunk = (N2 + G::size - 1) / G::size;
+  r.p0 = chunk * g.lane;
+  r.p1 = r.p0 + chunk < N2 ? r.p0 + chunk : N2;
+  if (r.p0 > N2) r.p0 = N2;
   const long long poff = seg_pair_off[seg];
   const int pcap = seg_pair_cap[seg];
-  if (np > pcap) np = pcap;  // cannot happen: np <= n(n-1)/2 and np <= pair_cap
-  for (int pi = g.lane; pi < np; pi += G::size) {
-    unsigned key = L.pairs[pi];
-    int ra = static_cast<int>(key >> 16), rb = static_cast<int>(key & 0xffffu);
-    st.pair_a[poff + pi] = L.gid[ra];
-    st.pair_b[poff + pi] = L.gid[rb];
-    atomicAdd(&s_cls[route_class(L.type[ra], L.type[rb])], 1);
+
+  // pass 0: accepted pairs per route class
+  int4 mine = make_int4(0, 0, 0, 0);
+  enum_pairs(d, L, r, [&](int, int, int cls) {
+    mine.x += 1;
+    if (cls == 0) mine.y += 1; else if (cls == 1) mine.z += 1; else mine.w += 1;
+  });
+  int4 tot;
+  int4 at = group_scan4(g, mine, tot, s_tmp);
+  int np = tot.x;
+  if (np > pcap) {
+    // rare: the region is full; recount the classes over the first pcap pairs only
+    if (g.lane == 0) atomicAdd(&counters[0], np - pcap);
+    np = pcap;
+    int4 kept = make_int4(0, 0, 0, 0);
+    int pi = at.x;
+    enum_pairs(d, L, r, [&](int, int, int cls) {
+      if (pi < pcap) {
+        kept.x += 1;
+        if (cls == 0) kept.y += 1; else if (cls == 1) kept.z += 1; else kept.w += 1;
+      }
+      ++pi;
+    });
+    const int ax = at.x;
+    at = group_scan4(g, kept, tot, s_tmp);
+    at.x = ax;  // pair positions stay those of the full order
   }
-  g.sync();
+  // route order: MPR/GJK (class 1), then analytic (class 0), then mesh (class 2)
+  const int base0 = tot.z, base1 = 0, base2 = tot.z + tot.y;
+  int pi = at.x, c0 = at.y, c1 = at.z, c2 = at.w;
+  enum_pairs(d, L, r, [&](int a, int b, int cls) {
+    if (pi < pcap) {
+      st.pair_a[poff + pi] = L.gid[a];
+      st.pair_b[poff + pi] = L.gid[b];
+      const int slot = cls == 1 ? base1 + c1++ : (cls == 0 ? base0 + c0++ : base2 + c2++);
+      st.route[poff + slot] = pi;
+    }
+    ++pi;
+  });
   if (g.lane == 0) {
     seg_pair_count[seg] = np;
-    seg_conv_count[seg] = s_cls[1];
-    seg_mesh_count[seg] = s_cls[2];
-    s_rbase[1] = 0;
-    s_rbase[0] = s_cls[1];
-    s_rbase[2] = s_cls[1] + s_cls[0];
-    s_cls[0] = 0;
-    s_cls[1] = 0;
-    s_cls[2] = 0;
-  }
-  g.sync();
-  for (int pi = g.lane; pi < np; pi += G::size) {
-    unsigned key = L.pairs[pi];
-    int cls = route_class(L.type[key >> 16], L.type[key & 0xffffu]);
-    st.route[poff + s_rbase[cls] + atomicAdd(&s_cls[cls], 1)] = pi;
+    seg_conv_count[seg] = tot.z;
+    seg_mesh_count[seg] = tot.w;
   }
 }
 
-// One 256-lane workgroup per listed segment (segments above 64 shapes).
+// One 256-lane workgroup per listed segment (segments above 64 shapes), pairs by enumeration.
 __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d, const float* __restrict__ body_q,
                                                                   StageBuffers st,
                                                                   const long long* __restrict__ seg_pair_off,
@@ -234,13 +286,13 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
                                                                   int* __restrict__ seg_pair_count,
                                                                   int* __restrict__ seg_conv_count,
                                                                   int* __restrict__ seg_mesh_count,
-                                                                  int* __restrict__ counters, int pair_cap,
-                                                                  int max_n, const int* __restrict__ seg_list) {
+                                                                  int* __restrict__ counters, int max_n,
+                                                                  const int* __restrict__ seg_list) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int sc[7];
-  segment_broad_phase(BlockGroup{static_cast<int>(threadIdx.x)}, d, body_q, st, seg_pair_off, seg_pair_cap,
-                      seg_pair_count, seg_conv_count, seg_mesh_count, counters, pair_cap, max_n, seg_list[blockIdx.x],
-                      smem, sc);
+  __shared__ int4 s_tmp[CBLOCK / 64];
+  segment_broad_phase_enum(BlockGroup{static_cast<int>(threadIdx.x)}, d, body_q, st, seg_pair_off, seg_pair_cap,
+                           seg_pair_count, seg_conv_count, seg_mesh_count, counters, max_n, seg_list[blockIdx.x],
+                           smem, s_tmp);
 }
 
 // One wave per listed segment (segments of at most 64 shapes), four segments per workgroup: the sorts and
@@ -252,17 +304,16 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_wave_kernel(NmcModelDe
                                                                        int* __restrict__ seg_pair_count,
                                                                        int* __restrict__ seg_conv_count,
                                                                        int* __restrict__ seg_mesh_count,
-                                                                       int* __restrict__ counters, int pair_cap,
-                                                                       int max_n, const int* __restrict__ seg_list,
+                                                                       int* __restrict__ counters, int max_n,
+                                                                       const int* __restrict__ seg_list,
                                                                        int n_list, int lds_per_wave) {
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ int sc[CBLOCK / 64][8];
   const int wave = threadIdx.x >> 6;
   const int idx = blockIdx.x * (CBLOCK / 64) + wave;
   if (idx >= n_list) return;  // whole waves only; the kernel has no workgroup barrier
-  segment_broad_phase(WaveGroup{static_cast<int>(threadIdx.x & 63)}, d, body_q, st, seg_pair_off, seg_pair_cap,
-                      seg_pair_count, seg_conv_count, seg_mesh_count, counters, pair_cap, max_n, seg_list[idx],
-                      smem + static_cast<size_t>(wave) * lds_per_wave, sc[wave]);
+  segment_broad_phase_enum(WaveGroup{static_cast<int>(threadIdx.x & 63)}, d, body_q, st, seg_pair_off, seg_pair_cap,
+                           seg_pair_count, seg_conv_count, seg_mesh_count, counters, max_n, seg_list[idx],
+                           smem + static_cast<size_t>(wave) * lds_per_wave, nullptr);
 }
 
 // Route offsets: exclusive prefix of MPR/GJK, analytic and mesh pair counts over segments.
@@ -411,13 +462,12 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
   }
 }
 
-size_t collide_lds_bytes(int max_n, int pair_cap) {
-  int n2 = 1;
-  while (n2 < max_n) n2 <<= 1;
-  size_t b = sizeof(unsigned long long) * n2 + sizeof(float) * 8 * (size_t)max_n + sizeof(int) * 6 * (size_t)max_n;
-  b += sizeof(unsigned) * (size_t)pair_cap;
+// LDS of segment_broad_phase_enum: order key 8 B, AABB 24 B, projection 8 B, six ints per shape.
+size_t collide_lds_enum_bytes(int max_n) {
+  size_t b = (sizeof(unsigned long long) + sizeof(float) * 8 + sizeof(int) * 6) * (size_t)max_n;
   return (b + 15) & ~size_t(15);
 }
+
 
 hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t stream);
 
@@ -430,13 +480,13 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
     hipLaunchKernelGGL(collide_segment_wave_kernel, dim3((m->n_seg_small + per - 1) / per), dim3(CBLOCK),
                        static_cast<size_t>(per) * m->seg_small_lds, stream, d, body_q, m->stage, m->seg_pair_off,
                        m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count, m->seg_mesh_count, m->counters,
-                       m->seg_small_pair_cap, m->seg_small_max_n, m->seg_small, m->n_seg_small,
+                       m->seg_small_max_n, m->seg_small, m->n_seg_small,
                        static_cast<int>(m->seg_small_lds));
   }
   if (m->n_seg_big > 0)
     hipLaunchKernelGGL(collide_segment_kernel, dim3(m->n_seg_big), dim3(CBLOCK), m->collide_lds, stream, d, body_q,
                        m->stage, m->seg_pair_off, m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count,
-                       m->seg_mesh_count, m->counters, m->pair_cap, d.max_segment_shapes, m->seg_big);
+                       m->seg_mesh_count, m->counters, d.max_segment_shapes, m->seg_big);
   const bool mesh = d.mesh_count > 0;
   hipLaunchKernelGGL(route_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, m->seg_pair_count, m->seg_conv_count,
                      mesh ? m->seg_mesh_count : nullptr, m->seg_conv_off, m->seg_ana_off,

@@ -11,7 +11,7 @@
 
 namespace nmc {
 
-size_t collide_lds_bytes(int max_n, int pair_cap);
+size_t collide_lds_enum_bytes(int max_n);
 size_t mesh_lds_bytes(int cand_cap);
 size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap);
 hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContacts* c, hipStream_t stream);
@@ -78,13 +78,14 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   hipError_t e = hipGetDevice(&m->device);
   if (e != hipSuccess) { delete m; return hip_fail(e, "hipGetDevice"); }
   int segs = d.segment_count;
-  // per-segment pair buffer: the worst case n(n-1)/2 of the largest segment, capped at 16384 entries
+  // per-segment pair region: the segment's worst case n(n-1)/2, capped at 16384 pairs (a pair entry stages 5
+  // contact records of 80 B)
   long long maxn = d.max_segment_shapes;
   long long worst = maxn * (maxn - 1) / 2;
   int pair_cap = static_cast<int>(worst < 16384 ? worst : 16384);
   if (pair_cap < 1) pair_cap = 1;
   m->pair_cap = next_pow2(pair_cap);
-  m->collide_lds = collide_lds_bytes(d.max_segment_shapes > 0 ? d.max_segment_shapes : 1, m->pair_cap);
+  m->collide_lds = collide_lds_enum_bytes(d.max_segment_shapes > 0 ? d.max_segment_shapes : 1);
   // solver ordering buffer: 2 entries per contact; worlds with up to 512 contacts order them in LDS,
   // denser worlds in a global buffer of 2 entries per contact slot.  Models with meshes (up to
   // mesh_contact_max contacts per pair, few bodies per world) size the LDS buffer for their worst world,
@@ -167,8 +168,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     m->n_seg_small = static_cast<int>(small.size());
     m->n_seg_big = static_cast<int>(big.size());
     m->seg_small_max_n = small_max;
-    m->seg_small_pair_cap = next_pow2(small_max * (small_max - 1) / 2 > 1 ? small_max * (small_max - 1) / 2 : 1);
-    m->seg_small_lds = collide_lds_bytes(small_max, m->seg_small_pair_cap);
+    m->seg_small_lds = collide_lds_enum_bytes(small_max);
     if ((e = dmalloc(&m->seg_small, small.size())) != hipSuccess || (e = dmalloc(&m->seg_big, big.size())) != hipSuccess ||
         (!small.empty() && (e = hipMemcpy(m->seg_small, small.data(), sizeof(int) * small.size(), hipMemcpyHostToDevice)) != hipSuccess) ||
         (!big.empty() && (e = hipMemcpy(m->seg_big, big.data(), sizeof(int) * big.size(), hipMemcpyHostToDevice)) != hipSuccess)) {

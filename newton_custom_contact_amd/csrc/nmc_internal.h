@@ -58,7 +58,7 @@ struct NmcModelImpl {
   int* seg_small;           // segments of at most 64 shapes (one wave each), and the others (one workgroup)
   int* seg_big;
   int n_seg_small, n_seg_big;
-  int seg_small_max_n, seg_small_pair_cap;
+  int seg_small_max_n;
   size_t seg_small_lds;     // dynamic LDS of one wave
   int* counters;         // [4] overflow counters (see nmc_get_counters)
   int pair_cap;          // per-segment pair buffer (LDS) capacity
