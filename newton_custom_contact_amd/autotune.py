@@ -3,8 +3,9 @@
 The step kernel runs one workgroup per world.  Two shapes exist for worlds of up to 64 bodies: one
 64-lane wave holding 128 contacts in registers (best when a world has a few hundred contacts at most),
 and 256 lanes holding 256 (more lanes per world for contact-heavy worlds, e.g. triangle meshes with up
-to 64 contacts per pair).  Both give bit-identical results, so the choice is purely a timing question,
-answered here by timing the solver on the caller's own state and contacts.
+to 64 contacts per pair).  Models with larger worlds can launch by size tier (automatic, 0) or run every
+world on 256 lanes.  All give bit-identical results, so the choice is purely a timing question, answered
+here by timing the solver on the caller's own state and contacts.
 """
 
 from __future__ import annotations
@@ -13,8 +14,10 @@ import torch
 
 
 def candidate_block_lanes(solver) -> list[int]:
-    """Launch shapes the solver's model admits (64 lanes needs worlds of at most 64 bodies)."""
-    return [64, 256] if solver.native.desc.max_world_bodies <= 64 else [256]
+    """Launch shapes the solver's model admits: 64 lanes needs worlds of at most 64 bodies; 0 (automatic)
+    launches worlds by size tier (<= 64 bodies on one wave, larger on 256 lanes), 256 runs every world on
+    256 lanes."""
+    return [64, 256] if solver.native.desc.max_world_bodies <= 64 else [0, 256]
 
 
 def tune_block_lanes(solver, state_in, state_out, contacts, dt: float, reps: int = 5) -> dict:

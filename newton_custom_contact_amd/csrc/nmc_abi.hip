@@ -129,6 +129,37 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       if (je[j] && jt[j] != JT_FREE && jt[j] != JT_DISTANCE) m->xpbd_joint_lds = d.max_world_joints > 0 ? d.max_world_joints : 1;
   }
   m->xpbd_lds = xpbd_lds_bytes(d.max_world_bodies, m->xpbd_joint_lds, m->xpbd_contact_cap);
+  // world size tiers for the step kernel's automatic launch shape
+  if (d.max_world_bodies > 64 && d.world_count >= 0) {
+    const int W1 = d.world_count + 1;
+    std::vector<int> nb(W1);
+    if ((e = hipMemcpy(nb.data(), d.world_body_count, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess) {
+      delete m;
+      return hip_fail(e, "nmc_model_create: world_body_count");
+    }
+    std::vector<int> tier[3];
+    for (int w = 0; w < W1; ++w) {
+      const int t = nb[w] <= 64 ? 0 : (nb[w] <= 256 ? 1 : 2);
+      tier[t].push_back(w);
+      m->xpbd_tier_max[t] = std::max(m->xpbd_tier_max[t], std::max(nb[w], 1));
+    }
+    int used = 0;
+    for (int t = 0; t < 3; ++t) used += tier[t].empty() ? 0 : 1;
+    m->xpbd_tiered = used > 1;
+    if (m->xpbd_tiered) {
+      for (int t = 0; t < 3; ++t) {
+        m->xpbd_tier_count[t] = static_cast<int>(tier[t].size());
+        if (tier[t].empty()) continue;
+        if ((e = dmalloc(&m->xpbd_tier_worlds[t], tier[t].size())) != hipSuccess ||
+            (e = hipMemcpy(m->xpbd_tier_worlds[t], tier[t].data(), sizeof(int) * tier[t].size(),
+                           hipMemcpyHostToDevice)) != hipSuccess) {
+          for (int k = 0; k < 3; ++k) (void)hipFree(m->xpbd_tier_worlds[k]);
+          delete m;
+          return hip_fail(e, "nmc_model_create: world tiers");
+        }
+      }
+    }
+  }
   if (m->collide_lds > 160 * 1024 || m->xpbd_lds > 160 * 1024) {
     delete m;
     return fail("per-world LDS requirement exceeds 160 KiB");
@@ -248,7 +279,7 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
                   m->seg_small, m->seg_big,
                   s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
                   m->seg_pair_count, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, m->seg_contact_count,
-                  m->counters, m->xpbd_keys};
+                  m->counters, m->xpbd_keys, m->xpbd_tier_worlds[0], m->xpbd_tier_worlds[1], m->xpbd_tier_worlds[2]};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;

@@ -65,6 +65,12 @@ struct NmcModelImpl {
   size_t collide_lds;
   size_t xpbd_lds;
   int xpbd_contact_cap;  // (contact, side) entries a world orders in LDS
+  // worlds by size tier (<= 64, <= 256, <= 512 bodies) for the automatic launch shape; xpbd_tiered when the
+  // model has worlds above 64 bodies and worlds of another tier (else one launch covers all)
+  int* xpbd_tier_worlds[3];
+  int xpbd_tier_count[3];
+  int xpbd_tier_max[3];
+  bool xpbd_tiered;
   int xpbd_joint_lds;    // joints per world with LDS delta slots: max_world_joints when some enabled joint is
                          // not FREE / DISTANCE (the only joints the joint pass solves), else 0
   unsigned* xpbd_keys;   // [xpbd_keys_cap] ordering entries of denser worlds, at 2 x their contact offset

@@ -432,6 +432,8 @@ struct XpbdArgs {
   float* qd_out;
   int key_cap;  // (contact, side) entries the LDS ordering buffer holds
   int joint_lds;  // joints per world with LDS delta slots (0 when no world has a non-FREE joint)
+  int lds_bodies;  // bodies per world the launch's LDS layout holds
+  const int* world_list;  // the launch's worlds (null: workgroup w runs world w)
   unsigned* keys_global;  // ordering buffer of worlds whose entries exceed key_cap (2 per contact slot)
   long long keys_global_cap;
   int* counters;
@@ -615,12 +617,13 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   __shared__ int s_fallback;
   const NmcModelDesc& d = a.d;
   const NmcLearnIO& lrn = a.lrn;
-  const int w = blockIdx.x;  // world index; w == world_count owns the global bodies
+  // world index; w == world_count owns the global bodies
+  const int w = a.world_list ? a.world_list[blockIdx.x] : static_cast<int>(blockIdx.x);
   const int b0 = d.world_body_first[w];
   const int nb = d.world_body_count[w];
   const int j0 = d.world_joint_first[w];
   const int nj = d.world_joint_count[w];
-  const int mb = d.max_world_bodies > 0 ? d.max_world_bodies : 1;
+  const int mb = a.lds_bodies > 0 ? a.lds_bodies : 1;
   const int mj = a.joint_lds;
   constexpr int CH = BS * CPL;  // contacts per chunk
   const float dt = a.dt;
@@ -1235,18 +1238,35 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   a.counters = m->counters;
   if (lrn) a.lrn = *lrn; else a.lrn = NmcLearnIO{};
   if (rep) a.rep = *rep; else a.rep = NmcXpbdReport{};
-  int worlds = m->d.world_count + 1;
-  XpbdShape sh = xpbd_shape(m->d.max_world_bodies, params->block_lanes);
-  const size_t lds = xpbd_lds_bytes_shape(sh, m->d.max_world_bodies, m->xpbd_joint_lds, m->xpbd_contact_cap);
-  if (rep) {
-    launch_mode<XM_REPORT>(sh, worlds, lds, stream, a);
-    return hipGetLastError();
+  // launches: one per size tier of worlds (automatic shape: <= 64 bodies on one wave, <= 256 and <= 512 on
+  // 256 lanes), or all worlds in one launch of the requested shape
+  struct Launch { XpbdShape sh; int worlds, bodies; const int* list; };
+  Launch ls[3];
+  int nl = 0;
+  if (params->block_lanes == 0 && m->xpbd_tiered) {
+    for (int t = 0; t < 3; ++t)
+      if (m->xpbd_tier_count[t] > 0)
+        ls[nl++] = Launch{xpbd_shape(m->xpbd_tier_max[t], 0), m->xpbd_tier_count[t], m->xpbd_tier_max[t],
+                          m->xpbd_tier_worlds[t]};
+  } else {
+    ls[nl++] = Launch{xpbd_shape(m->d.max_world_bodies, params->block_lanes), m->d.world_count + 1,
+                      m->d.max_world_bodies, nullptr};
   }
-  switch (a.lrn.mode) {
-    case NMC_STEP_LOG: launch_mode<XM_LOG>(sh, worlds, lds, stream, a); break;
-    case NMC_STEP_PREP: launch_mode<XM_PREP>(sh, worlds, lds, stream, a); break;
-    case NMC_STEP_WARM: launch_mode<XM_WARM>(sh, worlds, lds, stream, a); break;
-    default: launch_mode<XM_PLAIN>(sh, worlds, lds, stream, a); break;
+  for (int i = 0; i < nl; ++i) {
+    const Launch& L = ls[i];
+    a.lds_bodies = L.bodies;
+    a.world_list = L.list;
+    const size_t lds = xpbd_lds_bytes_shape(L.sh, L.bodies, m->xpbd_joint_lds, m->xpbd_contact_cap);
+    if (rep) {
+      launch_mode<XM_REPORT>(L.sh, L.worlds, lds, stream, a);
+      continue;
+    }
+    switch (a.lrn.mode) {
+      case NMC_STEP_LOG: launch_mode<XM_LOG>(L.sh, L.worlds, lds, stream, a); break;
+      case NMC_STEP_PREP: launch_mode<XM_PREP>(L.sh, L.worlds, lds, stream, a); break;
+      case NMC_STEP_WARM: launch_mode<XM_WARM>(L.sh, L.worlds, lds, stream, a); break;
+      default: launch_mode<XM_PLAIN>(L.sh, L.worlds, lds, stream, a); break;
+    }
   }
   return hipGetLastError();
 }

@@ -57,3 +57,35 @@ def test_block_lanes_bit_identical(cuda_device, name, make):
     res = tune_block_lanes(solver, s0, s1, c, DT, reps=2)
     print(name, res, flush=True)
     assert res["block_lanes"] in (64, 256) and solver.block_lanes == res["block_lanes"]
+
+
+def test_tiered_launch_bit_identical_uneven(cuda_device):
+    """Worlds of 8-262 bodies: launching by size tier (automatic) equals running every world on 256 lanes."""
+    print("[block lanes] uneven tiers", flush=True)
+    model = scenes.uneven_worlds(12, seed=8, device=cuda_device)
+    pipe = CollisionPipeline(model, broad_phase="sap")
+    solver = SolverXPBD(model, iterations=10)
+    assert candidate_block_lanes(solver) == [0, 256]
+    counts = model.host["world_body_count"]
+    assert counts.max() > 64 and (counts[:-1] <= 64).any()
+    c = pipe.contacts()
+    s0, s1 = model.state(), model.state()
+    for _ in range(60):
+        pipe.collide(s0, c)
+        solver.step(s0, s1, None, c, DT)
+        s0, s1 = s1, s0
+    pipe.collide(s0, c)
+    outs = {}
+    for lanes in (0, 256):
+        solver.block_lanes = lanes
+        a, b = model.state(), model.state()
+        a.body_q.copy_(s0.body_q)
+        a.body_qd.copy_(s0.body_qd)
+        solver.step(a, b, None, c, DT)
+        torch.cuda.synchronize()
+        outs[lanes] = (b.body_q.cpu().numpy(), b.body_qd.cpu().numpy())
+    assert int(c.rigid_contact_count.item()) > 0
+    assert np.array_equal(outs[0][0], outs[256][0]) and np.array_equal(outs[0][1], outs[256][1])
+    res = tune_block_lanes(solver, s0, s1, c, DT, reps=2)
+    print(res, flush=True)
+    assert res["block_lanes"] in (0, 256)
