@@ -9,7 +9,7 @@ C-ABI and are compared inside the kernels:
 * ``JointType``    -- newton/_src/sim/enums.py:29-56
 
 Transforms are ``(px, py, pz, qx, qy, qz, qw)`` and spatial vectors are ``(linear, angular)``,
-the reference's layouts (SURVEY of newton/_src/sim/state.py, Warp ``transform``/``spatial_vector``).
+the reference's layouts (newton/_src/sim/state.py; Warp ``transform`` / ``spatial_vector``).
 """
 
 from __future__ import annotations
