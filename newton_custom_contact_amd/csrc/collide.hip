@@ -28,6 +28,7 @@
 
 #include "broad_common.h"
 #include "narrow.h"
+static_assert(nmc::kNarrowBlock == nmc::CBLOCK, "the narrow kernel's clip buffers are laid out for CBLOCK lanes");
 #include "nmc_internal.h"
 #include "scan.h"
 
@@ -324,16 +325,23 @@ __global__ void __launch_bounds__(SCAN_BLOCK) route_scan_kernel(const int* __res
                                                                int* __restrict__ seg_ana_off,
                                                                int* __restrict__ seg_mesh_off,
                                                                int* __restrict__ route_count, int segments) {
-  // pair offsets minus conv and mesh offsets = analytic offsets
-  const int* const in[3] = {seg_conv_count, seg_pair_count, seg_mesh_count};
-  int* const out[3] = {seg_conv_off, seg_ana_off, seg_mesh_off};
+  // scans MPR/GJK, analytic (all pairs minus MPR/GJK minus mesh) and mesh counts in one pass
   int total[3];
-  segment_scan<3>(in, out, segments, total);
-  __syncthreads();  // the scan's writes by other lanes
-  for (int i = threadIdx.x; i < segments; i += blockDim.x)
-    seg_ana_off[i] -= seg_conv_off[i] + (seg_mesh_off ? seg_mesh_off[i] : 0);
+  segment_scan_fn<3>(
+      segments,
+      [&](int i, int (&v)[3]) {
+        v[0] = seg_conv_count[i];
+        v[2] = seg_mesh_count ? seg_mesh_count[i] : 0;
+        v[1] = seg_pair_count[i] - v[0] - v[2];
+      },
+      [&](int i, const int (&ex)[3], const int (&)[3]) {
+        seg_conv_off[i] = ex[0];
+        seg_ana_off[i] = ex[1];
+        if (seg_mesh_off) seg_mesh_off[i] = ex[2];
+      },
+      total);
   if (threadIdx.x == 0) {
-    route_count[0] = total[1] - total[0] - total[2];
+    route_count[0] = total[1];
     route_count[1] = total[0];
     route_count[2] = total[2];
   }
@@ -387,18 +395,18 @@ __global__ void __launch_bounds__(64) contact_count_kernel(StageBuffers st, cons
 __global__ void __launch_bounds__(SCAN_BLOCK) contact_scan_kernel(const int* __restrict__ seg_contact_count,
                                                                  NmcContacts c, int segments,
                                                                  int* __restrict__ counters) {
-  const int* const in[1] = {seg_contact_count};
-  int* const out[1] = {c.world_start};
+  // world_start, and world_count clipped to the capacity, written as the scan emits each segment
   int tot[1];
-  segment_scan<1>(in, out, segments, tot);
+  segment_scan_fn<1>(
+      segments, [&](int i, int (&v)[1]) { v[0] = seg_contact_count[i]; },
+      [&](int i, const int (&ex)[1], const int (&v)[1]) {
+        c.world_start[i] = ex[0];
+        const int room = c.capacity - ex[0];
+        const int stored = v[0] < room ? v[0] : room;
+        c.world_count[i] = stored > 0 ? stored : 0;
+      },
+      tot);
   const int total = tot[0];
-  for (int i = threadIdx.x; i < segments; i += blockDim.x) {
-    int start = c.world_start[i];
-    int room = c.capacity - start;
-    int v = seg_contact_count[i];
-    int stored = v < room ? v : room;
-    c.world_count[i] = stored > 0 ? stored : 0;
-  }
   if (threadIdx.x == 0) {
     c.count[0] = total;
     if (total > c.capacity) atomicAdd(&counters[2], total - c.capacity);

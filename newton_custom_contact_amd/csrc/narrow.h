@@ -692,8 +692,19 @@ __device__ __forceinline__ f2 isect(f2 s, f2 e, f2 a, f2 b) {
 // guards below never fire on well-formed input; they keep every index inside the buffer regardless.
 constexpr int kClipMax = 12;
 
+// The clip buffer of one lane, kept in LDS rather than in scratch (the polygon loops index it by value):
+// entry i of lane t sits at s_poly[i * kNarrowBlock + t], so a wave's accesses to one entry hit
+// consecutive 8-byte words.  The narrow kernel runs kNarrowBlock lanes per workgroup.
+constexpr int kNarrowBlock = 256;
+struct LanePoly {
+  f2* base;
+  __device__ __forceinline__ f2& operator[](int i) const { return base[i * kNarrowBlock]; }
+  __device__ __forceinline__ LanePoly operator+(int o) const { return LanePoly{base + o * kNarrowBlock}; }
+};
+
 // Sutherland-Hodgman clip of `loop` (in place, in the shared 12-entry buffer) by the line s->e.
-__device__ int trim_in_place(f2 s, f2 e, f2* buf, int lo, int loop_count) {
+template <class Buf>
+__device__ int trim_in_place(f2 s, f2 e, const Buf& buf, int lo, int loop_count) {
   if (loop_count < 3) return loop_count;
   f2 ia{0.0f, 0.0f}, ib{0.0f, 0.0f};
   int change_a = -1, change_b = -1;
@@ -744,7 +755,8 @@ __device__ int trim_in_place(f2 s, f2 e, f2* buf, int lo, int loop_count) {
   return nlc;
 }
 
-__device__ __forceinline__ bool seg_rect(f2* buf, int o) {
+template <class Buf>
+__device__ __forceinline__ bool seg_rect(const Buf& buf, int o) {
   const float move = 1e-5f;
   f2 p0 = buf[o], p1 = buf[o + 1];
   float dx = p1.x - p0.x, dy = p1.y - p0.y;
@@ -760,7 +772,8 @@ __device__ __forceinline__ bool seg_rect(f2* buf, int o) {
 }
 
 // trim polygon lives at buf[5..], loop at buf[0..] (the reference aliases them the same way)
-__device__ int trim_all(f2* buf, int tcount, int lcount) {
+template <class Buf>
+__device__ int trim_all(const Buf& buf, int tcount, int lcount) {
   const int T0 = 5;
   if (tcount <= 1) return lcount < 1 ? lcount : 1;
   if (tcount == 2) {
@@ -781,7 +794,8 @@ __device__ int trim_all(f2* buf, int tcount, int lcount) {
   return cur;
 }
 
-__device__ void calipers(const f2* hull, int n, int out[4]) {
+template <class Buf>
+__device__ void calipers(const Buf& hull, int n, int out[4]) {
   int p1 = 0, p3 = 1;
   f2 d0 = hull[p1] - hull[p3];
   float max_dist_sq = d0.x * d0.x + d0.y * d0.y;
@@ -813,7 +827,8 @@ __device__ void calipers(const f2* hull, int n, int out[4]) {
   out[0] = p1; out[1] = p2; out[2] = p3; out[3] = p4;
 }
 
-__device__ int remove_zero_edges(f2* loop, int count, float eps) {
+template <class Buf>
+__device__ int remove_zero_edges(const Buf& loop, int count, float eps) {
   if (count < 2) return 0;
   int w = 0;
   for (int r = 1; r < count; ++r) {
@@ -826,7 +841,8 @@ __device__ int remove_zero_edges(f2* loop, int count, float eps) {
   return nc;
 }
 
-__device__ __forceinline__ int add_nodup(f2* arr, int count, f2 v, float eps, bool& added) {
+template <class Buf>
+__device__ __forceinline__ int add_nodup(const Buf& arr, int count, f2 v, float eps, bool& added) {
   added = false;
   if (count > 0 && len_sq(arr[0] - v) < eps) return count;
   if (count > 1 && len_sq(arr[count - 1] - v) < eps) return count;
@@ -936,8 +952,9 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
   f3 zero{0.0f, 0.0f, 0.0f};
   Tracker tra{zero, zero, zero, 0.0f}, trb{zero, zero, zero, 0.0f};
   f3 center = 0.5f * (p_a + p_b);
-  f2 buf[12];
-  for (int i = 0; i < 12; ++i) buf[i] = f2{0.0f, 0.0f};
+  __shared__ f2 s_poly[kClipMax * kNarrowBlock];
+  const LanePoly buf{s_poly + threadIdx.x};
+  for (int i = 0; i < kClipMax; ++i) buf[i] = f2{0.0f, 0.0f};
   int a_count = 0, b_count = 0;
   f3 lnb = rot_inv(rel_ob, -normal);
   f3 ltab = rot_inv(rel_ob, -ta);

@@ -1,16 +1,20 @@
-// One-workgroup exclusive scan over per-segment counts, shared by the collide and contact-match kernels.
+// One-workgroup exclusive scan over per-segment counts, shared by the collide, broad-phase and
+// contact-match kernels.
 #pragma once
 #include <hip/hip_runtime.h>
 
 namespace nmc {
 
-// Exclusive scans of N per-segment count arrays by one 1024-lane workgroup; writes the totals.  A null
-// input array scans as zeros, a null output array is skipped.  Each lane scans 4 consecutive entries, so
-// a pass covers 4096 segments.
+// Exclusive scans of N per-segment count arrays by one 1024-lane workgroup.  Each lane scans 8
+// consecutive entries, so one pass covers 8192 segments (larger counts take several passes).
+//
+// segment_scan_fn: load(i, v) fills v[0..N) with the counts of segment i (i < n); emit(i, ex, v) receives
+// the exclusive prefixes ex[0..N) of segment i with its counts; the totals come back in total[].  Doing the
+// per-segment follow-up work in emit saves the second pass over global memory a separate loop would take.
 constexpr int SCAN_BLOCK = 1024;
-constexpr int SCAN_ITEMS = 4;
-template <int N>
-__device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], int n, int (&total)[N]) {
+constexpr int SCAN_ITEMS = 8;
+template <int N, class Load, class Emit>
+__device__ void segment_scan_fn(int n, Load load, Emit emit, int (&total)[N]) {
   __shared__ int s_w[N][SCAN_BLOCK / 64];
   __shared__ int s_carry[N];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -18,17 +22,19 @@ __device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], in
   __syncthreads();
   for (int base = 0; base < n; base += SCAN_BLOCK * SCAN_ITEMS) {
     const int i0 = base + threadIdx.x * SCAN_ITEMS;
-    int v[N][SCAN_ITEMS], sum[N], inc[N];
+    int v[SCAN_ITEMS][N], sum[N], inc[N];
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-      sum[k] = 0;
+    for (int k = 0; k < N; ++k) sum[k] = 0;
 #pragma unroll
-      for (int t = 0; t < SCAN_ITEMS; ++t) {
-        v[k][t] = (in[k] && i0 + t < n) ? in[k][i0 + t] : 0;
-        sum[k] += v[k][t];
-      }
-      inc[k] = sum[k];
+    for (int t = 0; t < SCAN_ITEMS; ++t) {
+#pragma unroll
+      for (int k = 0; k < N; ++k) v[t][k] = 0;
+      if (i0 + t < n) load(i0 + t, v[t]);
+#pragma unroll
+      for (int k = 0; k < N; ++k) sum[k] += v[t][k];
     }
+#pragma unroll
+    for (int k = 0; k < N; ++k) inc[k] = sum[k];
     for (int off = 1; off < 64; off <<= 1) {
 #pragma unroll
       for (int k = 0; k < N; ++k) {
@@ -41,19 +47,18 @@ __device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], in
       for (int k = 0; k < N; ++k) s_w[k][w] = inc[k];
     }
     __syncthreads();
-    int c[N];
+    int c[N], ex[N];
 #pragma unroll
     for (int k = 0; k < N; ++k) {
       c[k] = s_carry[k];
       for (int q = 0; q < w; ++q) c[k] += s_w[k][q];
-      int ex = c[k] + inc[k] - sum[k];
-      if (out[k]) {
+      ex[k] = c[k] + inc[k] - sum[k];
+    }
 #pragma unroll
-        for (int t = 0; t < SCAN_ITEMS; ++t) {
-          if (i0 + t < n) out[k][i0 + t] = ex;
-          ex += v[k][t];
-        }
-      }
+    for (int t = 0; t < SCAN_ITEMS; ++t) {
+      if (i0 + t < n) emit(i0 + t, ex, v[t]);
+#pragma unroll
+      for (int k = 0; k < N; ++k) ex[k] += v[t][k];
     }
     __syncthreads();
     if (threadIdx.x == SCAN_BLOCK - 1) {
@@ -64,6 +69,23 @@ __device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], in
   }
 #pragma unroll
   for (int k = 0; k < N; ++k) total[k] = s_carry[k];
+}
+
+// Plain form: N input arrays (a null one scans as zeros) into N output arrays (a null one is skipped).
+template <int N>
+__device__ void segment_scan(const int* const (&in)[N], int* const (&out)[N], int n, int (&total)[N]) {
+  segment_scan_fn<N>(
+      n,
+      [&](int i, int (&v)[N]) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = in[k] ? in[k][i] : 0;
+      },
+      [&](int i, const int (&ex)[N], const int (&)[N]) {
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+          if (out[k]) out[k][i] = ex[k];
+      },
+      total);
 }
 
 }  // namespace nmc
