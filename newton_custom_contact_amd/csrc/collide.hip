@@ -372,7 +372,7 @@ __global__ void __launch_bounds__(CBLOCK) narrow_kernel(NmcModelDesc d, const fl
     int sa = st.pair_a[g], sb = st.pair_b[g];
     ShapeW A = load_wc(st.shape_wc + 20ll * sa), B = load_wc(st.shape_wc + 20ll * sb);
     float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
-    WriteCtx w{body_q, d.shape_body, d.shape_gap, st, 5ll * g, 0};
+    WriteCtx w{d.shape_gap, st, 5ll * g, 0};
     narrow_pair(sa, sb, d.shape_type[sa], d.shape_type[sb], A, B, gap_sum, w);
     st.mask[g] = static_cast<unsigned char>(w.n);
   }
@@ -420,6 +420,8 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
                                                                   const long long* __restrict__ seg_pair_off,
                                                                   const int* __restrict__ seg_pair_count,
                                                                   const int* __restrict__ shape_type,
+                                                                  const int* __restrict__ shape_body,
+                                                                  const float* __restrict__ body_q,
                                                                   int mesh_k, NmcContacts c) {
   __shared__ int s_scan[CBLOCK];
   __shared__ int s_wave[CBLOCK / 64];
@@ -449,20 +451,28 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
       const int s0 = st.pair_a[g], s1 = st.pair_b[g];
       long long s = 5 * g + k;
       if (MESH && shape_type[s0] == GEO_MESH) s = st.mesh_slot_base + static_cast<long long>(st.mesh_index[g]) * mesh_k + k;
-      const float4* r = st.rec + 5 * s;
-      const float4 r0 = r[0], r1 = r[1], r2 = r[2], r3 = r[3], r4 = r[4];
+      const float4* r = st.rec + 3 * s;
+      const float4 r0 = r[0], r1 = r[1], r2 = r[2];
+      // write_contact (collide.py:62-148), second half: body-frame points and offsets
+      const f3 aw{r0.x, r0.y, r0.z}, bw{r0.w, r1.x, r1.y}, n{r1.z, r1.w, r2.x};
+      const float off_a = r2.y, off_b = r2.z;
+      const int b0 = shape_body[s0], b1 = shape_body[s1];
+      const xform Xa = b0 == -1 ? xident() : xinverse(load_xform(body_q + 7 * b0));
+      const xform Xb = b1 == -1 ? xident() : xinverse(load_xform(body_q + 7 * b1));
+      const f3 p0 = xpoint(Xa, aw), p1 = xpoint(Xb, bw);
+      const f3 o0 = xvector(Xa, off_a * n), o1 = xvector(Xb, (-off_b) * n);
       c.shape0[o] = s0;
       c.shape1[o] = s1;
-      c.point0[3 * o] = r0.x; c.point0[3 * o + 1] = r0.y; c.point0[3 * o + 2] = r0.z;
-      c.point1[3 * o] = r0.w; c.point1[3 * o + 1] = r1.x; c.point1[3 * o + 2] = r1.y;
-      c.offset0[3 * o] = r1.z; c.offset0[3 * o + 1] = r1.w; c.offset0[3 * o + 2] = r2.x;
-      c.offset1[3 * o] = r2.y; c.offset1[3 * o + 1] = r2.z; c.offset1[3 * o + 2] = r2.w;
-      c.normal[3 * o] = r3.x; c.normal[3 * o + 1] = r3.y; c.normal[3 * o + 2] = r3.z;
-      c.margin0[o] = r3.w;
-      c.margin1[o] = r4.x;
+      c.point0[3 * o] = p0.x; c.point0[3 * o + 1] = p0.y; c.point0[3 * o + 2] = p0.z;
+      c.point1[3 * o] = p1.x; c.point1[3 * o + 1] = p1.y; c.point1[3 * o + 2] = p1.z;
+      c.offset0[3 * o] = o0.x; c.offset0[3 * o + 1] = o0.y; c.offset0[3 * o + 2] = o0.z;
+      c.offset1[3 * o] = o1.x; c.offset1[3 * o + 1] = o1.y; c.offset1[3 * o + 2] = o1.z;
+      c.normal[3 * o] = n.x; c.normal[3 * o + 1] = n.y; c.normal[3 * o + 2] = n.z;
+      c.margin0[o] = off_a;
+      c.margin1[o] = off_b;
       if (c.sort_key) {
         c.sort_key[o] = ((static_cast<long long>(s0) & 0xFFFFF) << 43) | ((static_cast<long long>(s1) & 0xFFFFF) << 23) |
-                        (static_cast<long long>(__float_as_int(r4.y)) & 0x7FFFFF);
+                        (static_cast<long long>(__float_as_int(r2.w)) & 0x7FFFFF);
       }
     }
     run += chunk;
@@ -514,10 +524,11 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
                      m->counters);
   if (mesh)
     hipLaunchKernelGGL(contact_compact_kernel<true>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_off,
-                       m->seg_pair_count, d.shape_type, d.mesh_contact_max, *c);
+                       m->seg_pair_count, d.shape_type, d.shape_body, body_q, d.mesh_contact_max, *c);
   else
     hipLaunchKernelGGL(contact_compact_kernel<false>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage,
-                       m->seg_pair_off, m->seg_pair_count, d.shape_type, d.mesh_contact_max, *c);
+                       m->seg_pair_off, m->seg_pair_count, d.shape_type, d.shape_body, body_q, d.mesh_contact_max,
+                       *c);
   return hipGetLastError();
 }
 

@@ -291,7 +291,7 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
     __syncthreads();
     mesh_bitonic(sel, k2);
     // ---- pass 2: the kept candidates again, written in sub-key order
-    const WriteCtx w{body_q, d.shape_body, d.shape_gap, st, st.mesh_slot_base + static_cast<long long>(i) * K, 0};
+    const WriteCtx w{d.shape_gap, st, st.mesh_slot_base + static_cast<long long>(i) * K, 0};
     for (int k = threadIdx.x; k < kept; k += blockDim.x) {
       const int sub = static_cast<int>(sel[k]);
       f3 center, normal;

@@ -43,8 +43,6 @@ struct CData {  // ContactData (contact_data.py:17-56)
 // Output of one pair: up to 5 records in sub-key order.
 // Destination of one pair's contacts: five consecutive staging slots starting at slot0.
 struct WriteCtx {
-  const float* __restrict__ body_q;
-  const int* __restrict__ shape_body;
   const float* __restrict__ shape_gap;
   StageBuffers st;
   long long slot0;
@@ -173,20 +171,15 @@ __device__ __forceinline__ bool passes_gap(const CData& c) {
   return d <= c.gap_sum;
 }
 
-// write_contact (collide.py:62-148) body: the staging record of contact c at slot s.
+// write_contact (collide.py:62-148), first half: the staging record of contact c at slot s holds the
+// world-frame points, the unit normal, both offsets and the sub key; the compaction applies the inverse
+// body transforms (the second half) as it writes the contact out.
 __device__ __forceinline__ void write_record(const CData& c, f3 n, f3 aw, f3 bw, const WriteCtx& w, long long s) {
   float off_a = c.r_a + c.m_a, off_b = c.r_b + c.m_b;
-  int b0 = w.shape_body[c.shape_a], b1 = w.shape_body[c.shape_b];
-  xform Xa = b0 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b0));
-  xform Xb = b1 == -1 ? xident() : xinverse(load_xform(w.body_q + 7 * b1));
-  f3 p0 = xpoint(Xa, aw), p1 = xpoint(Xb, bw);
-  f3 o0 = xvector(Xa, off_a * n), o1 = xvector(Xb, (-off_b) * n);
-  float4* r = w.st.rec + 5 * s;
-  r[0] = make_float4(p0.x, p0.y, p0.z, p1.x);
-  r[1] = make_float4(p1.y, p1.z, o0.x, o0.y);
-  r[2] = make_float4(o0.z, o1.x, o1.y, o1.z);
-  r[3] = make_float4(n.x, n.y, n.z, off_a);
-  r[4] = make_float4(off_b, __int_as_float(c.sub_key), 0.0f, 0.0f);
+  float4* r = w.st.rec + 3 * s;
+  r[0] = make_float4(aw.x, aw.y, aw.z, bw.x);
+  r[1] = make_float4(bw.y, bw.z, n.x, n.y);
+  r[2] = make_float4(n.z, off_a, off_b, __int_as_float(c.sub_key));
 }
 
 // write_contact (collide.py:62-148) into the pair's next slot.

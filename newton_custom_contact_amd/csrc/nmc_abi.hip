@@ -79,7 +79,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   if (e != hipSuccess) { delete m; return hip_fail(e, "hipGetDevice"); }
   int segs = d.segment_count;
   // per-segment pair region: the segment's worst case n(n-1)/2, capped at 16384 pairs (a pair entry stages 5
-  // contact records of 80 B)
+  // contact records of 48 B)
   long long maxn = d.max_segment_shapes;
   long long worst = maxn * (maxn - 1) / 2;
   int pair_cap = static_cast<int>(worst < 16384 ? worst : 16384);
@@ -251,7 +251,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       s.mesh_pair_capacity = mesh_pairs;
       slots += mesh_pairs * d.mesh_contact_max;
     }
-    if ((e = dmalloc(&s.rec, 5 * slots)) != hipSuccess || (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
+    if ((e = dmalloc(&s.rec, 3 * slots)) != hipSuccess || (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
         (e = dmalloc(&s.pair_b, pairs)) != hipSuccess || (e = dmalloc(&s.route, pairs)) != hipSuccess ||
         (e = dmalloc(&s.route_count, 3)) != hipSuccess ||
         (d.mesh_count > 0 && (e = dmalloc(&s.mesh_index, pairs)) != hipSuccess) ||

@@ -24,8 +24,8 @@ constexpr float MAXVAL = 1.0e10f;  // newton/_src/core/types.py:72
 // slots [mesh_slot_base + i K, mesh_slot_base + (i + 1) K), K = mesh_contact_max.  Fixed regions need
 // no global allocation counter.
 struct StageBuffers {
-  // one 80-byte record per contact slot: point0 3, point1 3, offset0 3, offset1 3, normal 3, margin0,
-  // margin1, sub_key (int bits), pad -- written and read as five 16-byte vectors
+  // one 48-byte record per contact slot: world-frame point on A 3, on B 3, unit normal 3, offset A,
+  // offset B (radius + margin), sub_key (int bits) -- written and read as three 16-byte vectors
   float4* rec;
   unsigned char* mask;   // per pair: number of records in its slots (0-5, mesh pairs 0-K)
   int* pair_a;           // first shape (lower GeoType), global id
