@@ -25,10 +25,10 @@ constexpr int NF = 16;    // inputs
 constexpr int NH = 128;   // hidden width
 constexpr int W2S = 136;  // padded LDS row stride (bf16) of W2 and the hidden tile
 
+// FP32 -> BF16, round to nearest even: gfx950's v_cvt_pk_bf16_f32 (one instruction instead of the integer
+// rounding sequence; the same bits for every finite input).
 __device__ __forceinline__ unsigned short to_bf16(float x) {
-  unsigned u = __float_as_uint(x);
-  unsigned r = ((u >> 16) & 1u) + 0x7FFFu;
-  return static_cast<unsigned short>((u + r) >> 16);
+  return __builtin_bit_cast(unsigned short, static_cast<__bf16>(x));
 }
 __device__ __forceinline__ float from_bf16(unsigned short b) { return __uint_as_float(static_cast<unsigned>(b) << 16); }
 __device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
