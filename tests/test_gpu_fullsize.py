@@ -1,7 +1,8 @@
-"""Parity at the benchmark's full size: mixed primitives, 4096 worlds x 32 bodies (11 spheres, 11 boxes and 10
-capsules per world on a ground plane), the workload the benchmark times.
+"""Parity at the benchmark's full sizes: the primitive workload (below) and the SDF workload
+(`test_fullsize_sdf_parity`).
 
-The GPU advances the scene 480 substeps (120 frames of 4 substeps, dt = 1/240 s) from its initial state.  At that
+Mixed primitives, 4096 worlds x 32 bodies (11 spheres, 11 boxes and 10 capsules per world on a ground plane), the
+workload the benchmark times.  The GPU advances the scene 480 substeps (120 frames of 4 substeps, dt = 1/240 s) from its initial state.  At that
 settled state (about 120 contacts per world):
 
 * the GPU contact set equals the oracle's: count, sort keys and shape pairs exactly; the float fields bit for bit
@@ -100,3 +101,50 @@ def test_fullsize_mixed_parity_and_layout(cuda_device):
     assert np.array_equal(bad_gpu, bad_ref)
     assert len(bad_gpu) <= 4
     assert exact
+
+
+def test_fullsize_sdf_parity(cuda_device):
+    """The SDF workload at the benchmark's size: 1024 worlds x 8 convex-hull meshes (64-hull library, 64^3
+    SDFs), advanced 240 substeps (60 frames) on the GPU until the hulls pile up.  At that state the GPU's
+    mesh-mesh and mesh-plane contacts equal the oracle's (count, keys, pairs exactly; float fields within 1e-5
+    relative, bit-exactness reported), and one step from that state agrees within 1e-5 relative."""
+    print("[full size sdf] 64-hull library, 64^3 SDFs, 1024 worlds", flush=True)
+    lib = scenes.mesh_library(64, seed=0)
+    for mm in lib:
+        mm.build_sdf(max_resolution=64, device=cuda_device)
+    model = scenes.mesh_worlds(1024, seed=0, device=cuda_device, library=lib)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    solver = SolverXPBD(model, iterations=10)
+    c = pipe.contacts()
+    a, b = model.state(), model.state()
+    print("[full size sdf] 240 GPU substeps", flush=True)
+    for _ in range(240):
+        pipe.collide(a, c)
+        solver.step(a, b, None, c, DT)
+        a, b = b, a
+    torch.cuda.synchronize()
+    q = a.body_q.cpu().numpy()
+    qd = a.body_qd.cpu().numpy()
+    assert np.isfinite(q).all() and np.isfinite(qd).all()
+
+    print("[full size sdf] collide parity", flush=True)
+    pipe.collide(a, c)
+    torch.cuda.synchronize()
+    got = c.numpy()
+    oracle = Oracle(model)
+    ref = oracle.collide(q, capacity=c.rigid_contact_max)
+    report = compare_contacts(got, ref)
+    print("contacts", report, flush=True)
+    assert report["count"] > 300 * 1024
+    oc = pipe.overflow_counters()
+    assert oc["pairs_dropped"] == 0 and oc["contacts_over_capacity"] == 0
+
+    print("[full size sdf] step parity", flush=True)
+    out = model.state()
+    solver.step(a, out, None, c, DT)
+    torch.cuda.synchronize()
+    q_ref, qd_ref = oracle.step(q, qd, np.zeros_like(qd), ref, DT, iterations=10)
+    q_gpu, qd_gpu = out.body_q.cpu().numpy(), out.body_qd.cpu().numpy()
+    print("step bit-exact", np.array_equal(q_gpu, q_ref) and np.array_equal(qd_gpu, qd_ref), flush=True)
+    np.testing.assert_allclose(q_gpu, q_ref, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(qd_gpu, qd_ref, rtol=1e-5, atol=1e-5)
