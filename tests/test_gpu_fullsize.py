@@ -1,5 +1,5 @@
-"""Parity at the benchmark's full sizes: the primitive workload (below) and the SDF workload
-(`test_fullsize_sdf_parity`).
+"""Parity at the benchmark's full sizes: the primitive workload (below), the SDF workload
+(`test_fullsize_sdf_parity`) and the learned warm start on the primitive workload (`test_fullsize_learned_step`).
 
 Mixed primitives, 4096 worlds x 32 bodies (11 spheres, 11 boxes and 10 capsules per world on a ground plane), the
 workload the benchmark times.  The GPU advances the scene 480 substeps (120 frames of 4 substeps, dt = 1/240 s) from its initial state.  At that
@@ -148,3 +148,77 @@ def test_fullsize_sdf_parity(cuda_device):
     print("step bit-exact", np.array_equal(q_gpu, q_ref) and np.array_equal(qd_gpu, qd_ref), flush=True)
     np.testing.assert_allclose(q_gpu, q_ref, rtol=1e-5, atol=1e-6)
     np.testing.assert_allclose(qd_gpu, qd_ref, rtol=1e-5, atol=1e-5)
+
+
+def test_fullsize_learned_step(cuda_device):
+    """The learned warm start on the primitive workload at full size, with the shipped network: at the settled
+    state (480 GPU substeps), one learned step (PREP -> network -> WARM: 3 warm iterations, 5 mm fallback
+    tolerance) gives features equal to the oracle's bit for bit, predictions within the BF16 bound of the
+    small-size network test, and bodies, normal multipliers, per-world residuals and fallback flags equal to
+    the oracle's warm step fed the GPU's predictions, bit for bit (NaN-aware: the diverged boxes of this seed)."""
+    from newton_custom_contact_amd.learned import WEIGHTS_PATH, SolverXPBDWarmStart, WarmStartMLP
+    from oracle import mlp_forward
+
+    print(f"[full size learned] {WORLDS} worlds", flush=True)
+    model = scenes.mixed_worlds(WORLDS, seed=0, device=cuda_device)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    plain = SolverXPBD(model, iterations=10)
+    c = pipe.contacts()
+    a, b = model.state(), model.state()
+    for _ in range(480):
+        pipe.collide(a, c)
+        plain.step(a, b, None, c, DT)
+        a, b = b, a
+    torch.cuda.synchronize()
+    q = a.body_q.cpu().numpy()
+    qd = a.body_qd.cpu().numpy()
+    f = np.zeros_like(qd)
+
+    mlp = WarmStartMLP.load(WEIGHTS_PATH, device=cuda_device)
+    solver = SolverXPBDWarmStart(model, mlp, iterations=10, warm_iterations=3, residual_tol=5e-3)
+    pipe.collide(a, c)
+    out = model.state()
+    solver.step(a, out, None, c, DT, remember=False)
+    torch.cuda.synchronize()
+    cg = c.numpy()
+    n = cg["count"]
+    buf = solver.buffers
+    oracle = Oracle(model)
+    ref = oracle.collide(q)
+    og = np.argsort(cg["sort_key"], kind="stable")
+    orf = np.argsort(ref["sort_key"], kind="stable")
+    assert np.array_equal(cg["sort_key"][og], ref["sort_key"][orf])
+
+    print("[full size learned] features and predictions", flush=True)
+    _, _, feat_ref, _, _, _ = oracle.step_learn(q, qd, f, ref, DT, mode=1)
+    feat = buf["feat"][:n].cpu().numpy()
+    fg, fr = feat[og], feat_ref[orf]
+    same = (fg == fr) | (np.isnan(fg) & np.isnan(fr))
+    bad_rows = np.nonzero(~same.all(axis=1))[0]
+    print("feature rows differing", len(bad_rows), "rows with NaN", int(np.isnan(fg).any(axis=1).sum()), flush=True)
+    if len(bad_rows):
+        print("first", bad_rows[:5].tolist(), fg[bad_rows[:2]].tolist(), fr[bad_rows[:2]].tolist(), flush=True)
+    assert len(bad_rows) == 0
+    lam0 = buf["lambda0"][:n].cpu().numpy()
+    params = {k: v.detach().cpu().numpy() for k, v in mlp.params.items()}
+    x = (feat - mlp.x_mean.cpu().numpy()) * mlp.x_inv_std.cpu().numpy()
+    pred_ref = mlp_forward(params, x) * mlp.out_scale
+    fin = np.isfinite(pred_ref)
+    assert np.array_equal(fin, np.isfinite(lam0))  # the NaN rows: contacts of a diverged box
+    scale = max(1e-3, float(np.abs(pred_ref[fin]).max()))
+    err = float(np.abs(lam0[fin] - pred_ref[fin]).max())
+    print("prediction max abs error", err, "scale", scale, flush=True)
+    assert err <= 1e-2 * scale
+
+    print("[full size learned] warm step", flush=True)
+    lam0_ref = np.empty_like(lam0)
+    lam0_ref[orf] = lam0[og]
+    qw, qdw, _, lam_w, res_w, fb_w = oracle.step_learn(q, qd, f, ref, DT, mode=2, lambda0=lam0_ref,
+                                                       warm_iterations=3, residual_tol=5e-3)
+    assert np.array_equal(out.body_q.cpu().numpy(), qw, equal_nan=True)
+    assert np.array_equal(out.body_qd.cpu().numpy(), qdw, equal_nan=True)
+    lam_g = buf["lambda_sum"][:n].cpu().numpy()
+    assert np.array_equal(lam_g[og], lam_w[orf], equal_nan=True)
+    assert np.array_equal(buf["residual"].cpu().numpy(), res_w, equal_nan=True)
+    assert np.array_equal(buf["fallback"].cpu().numpy(), fb_w)
+    print("fallback worlds", int(np.asarray(fb_w).sum()), flush=True)
