@@ -87,7 +87,9 @@ inline void broad_sap(const BroadInput& in, const int* index_map, const int* sli
       phi[l] = center + radius;
       order[l] = l;
     }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plo[a] < plo[b]; });
+    std::vector<uint64_t> okey(n);
+    for (int l = 0; l < n; ++l) okey[l] = sap_order_key(plo[l]);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return okey[a] < okey[b]; });
     std::vector<float> sorted(n);
     for (int i = 0; i < n; ++i) sorted[i] = plo[order[i]];
     for (int i = 0; i < n; ++i) {

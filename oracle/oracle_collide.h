@@ -212,7 +212,9 @@ void sap_pairs(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, std::
       phi[l] = center + radius;
       order[l] = l;
     }
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return plo[a] < plo[b]; });
+    std::vector<uint64_t> okey(n);
+    for (int l = 0; l < n; ++l) okey[l] = sap_order_key(plo[l]);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return okey[a] < okey[b]; });
     // "sap" sweeps while lower_j < upper_i; "nxn" (broad_phase_nxn.py:129-214) and "explicit"
     // (:28-65, the pairs of a list, AABB test only) visit every pair of the segment
     const bool sweep = m->broad_phase_mode == 0, listed = m->broad_phase_mode == 2;

@@ -20,8 +20,27 @@
 #pragma once
 #include <cmath>
 #include <cstdint>
+#include <cstring>
 
 namespace oracle {
+
+// Order of the SAP sort (broad_phase_sap.py:600-640 sorts with wp.utils.segmented_sort_pairs, a radix sort
+// on the float keys' sign-flipped bit patterns): -NaN < -inf < ... < -0 = +0 < ... < +inf < +NaN, ties in
+// index order under a stable sort.  For finite keys this is the order of operator<; unlike operator< it is a
+// strict weak order when a key is NaN (a body whose state has diverged), so the sweep then stays the
+// reference's.
+inline uint64_t sap_order_key(float f) {
+  if (f == 0.0f) f = 0.0f;
+  uint32_t u;
+  std::memcpy(&u, &f, sizeof u);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+inline uint64_t sap_order_key(double f) {
+  if (f == 0.0) f = 0.0;
+  uint64_t u;
+  std::memcpy(&u, &f, sizeof u);
+  return (u >> 63) ? ~u : (u | (uint64_t(1) << 63));
+}
 
 template <class T> struct V2 { T x, y; };
 template <class T> struct V3 { T x, y, z; };

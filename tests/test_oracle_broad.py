@@ -265,3 +265,22 @@ def test_shape_contact_pairs_and_explicit_lists():
     lst = np.array([[1, 2], [3, 0], [5, 6]], np.int32)
     sub = _pairs(oracle.Oracle(model, broad_phase="explicit", explicit_pairs=lst).broadphase(q))
     assert sorted(sub) == [(0, 3), (1, 2), (5, 6)]
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_sap_with_non_finite_boxes(seed):
+    """A diverged body gives NaN bounds.  The SAP sort orders keys like the reference's radix sort (NaN after
+    +inf, or before -inf when the sign bit is set), so the sweep still finds every pair among the finite boxes:
+    the set equals the brute force, in which a NaN box overlaps nothing."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    lower, upper, gap, group, world, filt = random_scene(rng)
+    n = len(world)
+    for k, i in enumerate(rng.choice(n, size=6, replace=False)):
+        nan = np.float32(np.nan) if k % 2 == 0 else -np.float32(np.nan)
+        lower[i, :] = nan
+        upper[i, :] = nan
+    im, se = precompute_world_map(world)
+    sap = _pairs(oracle.broad_phase("sap", lower, upper, gap, group, world, filt, im, se))
+    ref = brute_all_pairs(lower, upper, gap, group, world, filt)
+    assert len(sap) == len(set(sap)) and len(ref) > 20
+    assert set(sap) == ref
