@@ -203,17 +203,19 @@ def uneven_worlds(world_count: int = 2048, seed: int = 0, mu: float = 0.5, devic
             p = placer.place(_bound_radius(kind, dims))
             _add_primitive(b, kind, dims, p, shoemake_quat(rng), cfg)
         if chain_links > 0:
-            _add_chain(b, rng, chain_links, chain_limit, cfg)
+            # hang the chain clear of the pile: a dense world's placer raises its band above z = 3
+            top = float(np.max(placer.pts[:, 2] + placer.rad)) if len(placer.rad) else 0.0
+            _add_chain(b, rng, chain_links, chain_limit, cfg, z=max(3.0, top + 0.1))
         b.end_world()
         if progress is not None and (_w + 1) % 128 == 0:
             progress(_w + 1, world_count)
     return b.finalize(device)
 
 
-def _add_chain(b: ModelBuilder, rng, links: int, limit: float, cfg):
-    """A revolute chain of capsule links hanging from a world anchor above the pile."""
+def _add_chain(b: ModelBuilder, rng, links: int, limit: float, cfg, z: float = 3.0):
+    """A revolute chain of capsule links hanging from a world anchor at height ``z`` above the pile."""
     r, hh = 0.04, 0.1
-    base = np.array([rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), 3.0])
+    base = np.array([rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), z])
     # links lie along +X; each capsule's local +Z axis is rotated onto X
     q_link = quat_from_axis_angle((0.0, 1.0, 0.0), 0.5 * math.pi)
     prev = -1

@@ -229,9 +229,8 @@ def test_fullsize_uneven_parity(cuda_device):
     """The uneven workload at its benchmark size: 2048 worlds of log-uniform 8-256 primitives plus a 6-link
     revolute capsule chain each (157 k bodies and joints), advanced 120 substeps (30 frames) on the GPU.  At that
     state the contacts equal the oracle's (count, keys, pairs exactly; float fields within 1e-5 relative,
-    bit-exactness reported) and one step with the joints equals the oracle's bit for bit (NaN-aware: six of this
-    seed's worlds diverge within the first substeps, on the oracle identically, and their NaN bounds must not
-    change the sweep's pairs among the finite shapes of a segment)."""
+    bit-exactness reported) and one step with the joints equals the oracle's bit for bit (NaN-aware, should a
+    world diverge: its NaN bounds must not change the sweep's pairs among the finite shapes of a segment)."""
     worlds = 2048
 
     def progress(done, total):
