@@ -326,16 +326,20 @@ void nmc_broad_phase_destroy(NmcBroadPhase* broad_phase);
  * lower/upper S x 3, gap S or null (AABBs already expanded), group and world S, filter_pairs F x 2 sorted
  * (min, max) or null.  Writes candidate_pair (max x 2) and candidate_pair_count[0] = pairs found (may exceed
  * max; the pairs past it are dropped).  Deterministic order: segment by segment; all pairs by (row, column)
- * in index-map order, SAP by sweep origin in sorted order. */
+ * in index-map order, SAP by sweep origin in sorted order.  append = the reference's skip_count_zero
+ * (broad_phase_sap.py:562-563, broad_phase_nxn.py:353-354): 0 counts from zero; non-zero keeps the
+ * candidate_pair_count[0] already on the device, writes the new pairs after that many and adds their number
+ * to it (a negative count on entry counts as 0). */
 int nmc_broad_phase_launch(NmcBroadPhase* broad_phase, int shape_count, const float* lower, const float* upper,
                            const float* gap, const int* group, const int* world, const int* filter_pairs,
                            int filter_count, int* candidate_pair, int max_candidate_pair, int* candidate_pair_count,
-                           nmc_stream_t stream);
+                           int append, nmc_stream_t stream);
 /* BroadPhaseExplicit.launch (broad_phase_nxn.py:408-472): the given pairs whose AABBs overlap, in input
- * order; a pair naming a shape outside [0, shape_count) is skipped. */
+ * order; a pair naming a shape outside [0, shape_count) is skipped.  append: skip_count_zero
+ * (broad_phase_nxn.py:448-449), as for nmc_broad_phase_launch. */
 int nmc_broad_phase_explicit(NmcBroadPhase* broad_phase, int shape_count, const float* lower, const float* upper,
                              const float* gap, const int* shape_pairs, int shape_pair_count, int* candidate_pair,
-                             int max_candidate_pair, int* candidate_pair_count, nmc_stream_t stream);
+                             int max_candidate_pair, int* candidate_pair_count, int append, nmc_stream_t stream);
 
 /* Copies the counters accumulated since model creation to host (synchronises the stream):
  * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = mesh-pair candidate

@@ -13,6 +13,7 @@ from __future__ import annotations
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
@@ -47,13 +48,28 @@ def build(force: bool = True, verbose: bool = False) -> str:
         newest = max(os.path.getmtime(p) for p in srcs + [os.path.join(CSRC, h) for h in HEADERS])
         if os.path.getmtime(LIB) >= newest:
             return LIB
+    # one hipcc per translation unit, in parallel, then one link
+    common = [hipcc()] + [f for f in flags() if f != "-shared"]
+    objs = [os.path.join(CSRC, os.path.splitext(s)[0] + ".o") for s in SOURCES]
+
+    def compile_one(k: int) -> subprocess.CompletedProcess:
+        return subprocess.run(common + ["-c", "-o", objs[k], srcs[k]], capture_output=True, text=True)
+
+    jobs = max(1, min(len(srcs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(compile_one, range(len(srcs))))
+    log = "".join(r.stdout + r.stderr for r in results)
+    if any(r.returncode != 0 for r in results):
+        raise RuntimeError("hipcc failed:\n" + log)
     tmp = LIB + ".tmp"
-    cmd = [hipcc()] + flags() + ["-o", tmp] + srcs
-    r = subprocess.run(cmd, capture_output=True, text=True)
+    r = subprocess.run([hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs,
+                       capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    if verbose and (r.stdout or r.stderr):
-        print(r.stdout + r.stderr)
+        raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
+    for o in objs:
+        os.remove(o)
+    if verbose and (log or r.stdout or r.stderr):
+        print(log + r.stdout + r.stderr)
     os.replace(tmp, LIB)
     return LIB
 

@@ -167,10 +167,10 @@ def load() -> ctypes.CDLL:
         L.nmc_broad_phase_destroy.argtypes = [_vp]
         L.nmc_broad_phase_destroy.restype = None
         L.nmc_broad_phase_launch.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
-                                             ctypes.c_int, _vp, _vp]
+                                             ctypes.c_int, _vp, ctypes.c_int, _vp]
         L.nmc_broad_phase_launch.restype = ctypes.c_int
         L.nmc_broad_phase_explicit.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
-                                               ctypes.c_int, _vp, _vp]
+                                               ctypes.c_int, _vp, ctypes.c_int, _vp]
         L.nmc_broad_phase_explicit.restype = ctypes.c_int
         L.nmc_get_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.nmc_get_counters.restype = ctypes.c_int

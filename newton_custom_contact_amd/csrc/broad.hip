@@ -26,6 +26,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 #include <vector>
 
 #include "broad_common.h"
@@ -37,7 +38,6 @@ namespace nmc {
 
 constexpr int BP_LDS_MAX = 2048;         // shapes per segment held in LDS (56 B each for SAP)
 constexpr int BP_ENUM_MAX = 256;         // SAP segments up to this size enumerate their pairs (no sort)
-constexpr int BP_EXPLICIT_ITEMS = 4;     // explicit pairs per lane per block
 constexpr int BP_MAX_BLOCKS = 4096;
 
 struct BpArgs {
@@ -345,16 +345,29 @@ size_t bp_lds_bytes(int lds_n, bool sap) {
   return static_cast<size_t>(lds_n) * (sizeof(BpShape) + (sap ? 2 * sizeof(float) + sizeof(unsigned long long) : 0));
 }
 
+// Exclusive scan of the per-segment (or per-block) counts into write offsets.  append != 0: the pairs go after
+// the out_count[0] pairs already in the output (a caller that skips the zeroing of the count and chains
+// launches into one buffer), and the count becomes that base plus the pairs found; a negative count on entry
+// counts as 0.  The base is read into LDS before any lane writes out_count.
 __global__ void __launch_bounds__(SCAN_BLOCK) bp_scan_kernel(const int* __restrict__ count, int n,
                                                              int* __restrict__ off, int* __restrict__ total,
-                                                             int* __restrict__ out_count) {
-  const int* in[1] = {count};
-  int* out[1] = {off};
-  int t[1];
-  segment_scan<1>(in, out, n, t);
+                                                             int* __restrict__ out_count, int append) {
+  __shared__ int s_base;
   if (threadIdx.x == 0) {
-    total[0] = t[0];
-    out_count[0] = t[0];
+    const int b = append ? out_count[0] : 0;
+    s_base = b > 0 ? b : 0;
+  }
+  __syncthreads();
+  const int base = s_base;
+  int t[1];
+  segment_scan_fn<1>(
+      n, [&](int i, int (&v)[1]) { v[0] = count[i]; },
+      [&](int i, const int (&ex)[1], const int (&)[1]) { off[i] = base + ex[0]; }, t);
+  if (threadIdx.x == 0) {
+    const long long sum = static_cast<long long>(base) + t[0];
+    const int tot = sum > 0x7fffffffLL ? 0x7fffffff : static_cast<int>(sum);
+    total[0] = tot;
+    out_count[0] = tot;
   }
 }
 
@@ -525,7 +538,7 @@ extern "C" void nmc_broad_phase_destroy(NmcBroadPhase* bp) { bp_free(bp); }
 extern "C" int nmc_broad_phase_launch(NmcBroadPhase* bp, int shape_count, const float* lower, const float* upper,
                                       const float* gap, const int* group, const int* world, const int* filter_pairs,
                                       int filter_count, int* candidate_pair, int max_candidate_pair,
-                                      int* candidate_pair_count, nmc_stream_t stream) {
+                                      int* candidate_pair_count, int append, nmc_stream_t stream) {
   if (!bp || bp->mode == 2) return fail("nmc_broad_phase_launch: needs a SAP or all-pairs broad phase");
   if (bp->max_index >= shape_count)
     return fail("nmc_broad_phase_launch: the index map names shapes beyond shape_count");
@@ -557,26 +570,19 @@ extern "C" int nmc_broad_phase_launch(NmcBroadPhase* bp, int shape_count, const 
   const size_t lds = bp_lds_bytes(bp->lds_n, bp->mode == 0);
   hipLaunchKernelGGL(bp_segment_kernel<false>, dim3(grid), dim3(CBLOCK), lds, s, a);
   hipLaunchKernelGGL(bp_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, s, bp->count, bp->segments, bp->off, bp->total,
-                     candidate_pair_count);
+                     candidate_pair_count, append);
   hipLaunchKernelGGL(bp_segment_kernel<true>, dim3(grid), dim3(CBLOCK), lds, s, a);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : hip_fail(e, "nmc_broad_phase_launch");
 }
 
-extern "C" int nmc_broad_phase_explicit(NmcBroadPhase* bp, int shape_count, const float* lower, const float* upper,
-                                        const float* gap, const int* shape_pairs, int shape_pair_count,
-                                        int* candidate_pair, int max_candidate_pair, int* candidate_pair_count,
-                                        nmc_stream_t stream) {
-  if (!bp || bp->mode != 2) return fail("nmc_broad_phase_explicit: needs an explicit broad phase");
-  if (!lower || !upper || !candidate_pair_count || (shape_pair_count > 0 && !shape_pairs) ||
-      (max_candidate_pair > 0 && !candidate_pair))
-    return fail("nmc_broad_phase_explicit: null argument");
-  if (shape_pair_count < 0 || max_candidate_pair < 0) return fail("nmc_broad_phase_explicit: negative size");
-  const long long per = static_cast<long long>(CBLOCK) * BP_EXPLICIT_ITEMS;
-  const int blocks = static_cast<int>((shape_pair_count + per - 1) / per);
-  if (blocks > bp->capacity)
-    return fail("nmc_broad_phase_explicit: more pairs than the broad phase was created for");
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+// The three launches of an explicit broad phase over per-block scratch (count, off: `blocks` entries each,
+// total: 1).  Every entry of count is written by the first launch before the scan reads it, so the scratch
+// needs no clearing.
+int nmc::bp_explicit_run(int shape_count, const float* lower, const float* upper, const float* gap,
+                           const int* shape_pairs, int shape_pair_count, int blocks, int* count, int* off,
+                           int* total, int* candidate_pair, int max_candidate_pair, int* candidate_pair_count,
+                           int append, hipStream_t s, const char* what) {
   BpExplicitArgs a;
   a.shape_count = shape_count;
   a.lower = lower;
@@ -584,17 +590,46 @@ extern "C" int nmc_broad_phase_explicit(NmcBroadPhase* bp, int shape_count, cons
   a.gap = gap;
   a.pairs = shape_pairs;
   a.pair_count = shape_pair_count;
-  a.block_count = bp->count;
-  a.block_off = bp->off;
-  a.total = bp->total;
+  a.block_count = count;
+  a.block_off = off;
+  a.total = total;
   a.out_pairs = candidate_pair;
   a.max_pairs = max_candidate_pair;
   a.out_count = candidate_pair_count;
   const int grid = blocks < 1 ? 1 : (blocks < BP_MAX_BLOCKS ? blocks : BP_MAX_BLOCKS);
   hipLaunchKernelGGL(bp_explicit_kernel<false>, dim3(grid), dim3(CBLOCK), 0, s, a);
-  hipLaunchKernelGGL(bp_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, s, bp->count, blocks, bp->off, bp->total,
-                     candidate_pair_count);
+  hipLaunchKernelGGL(bp_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, s, count, blocks, off, total,
+                     candidate_pair_count, append);
   hipLaunchKernelGGL(bp_explicit_kernel<true>, dim3(grid), dim3(CBLOCK), 0, s, a);
   const hipError_t e = hipGetLastError();
-  return e == hipSuccess ? 0 : hip_fail(e, "nmc_broad_phase_explicit");
+  return e == hipSuccess ? 0 : hip_fail(e, what);
 }
+
+int nmc::bp_explicit_check(int shape_count, const float* lower, const float* upper, const int* shape_pairs,
+                             int shape_pair_count, const int* candidate_pair, int max_candidate_pair,
+                             const int* candidate_pair_count, const char* what) {
+  if (!lower || !upper || !candidate_pair_count || (shape_pair_count > 0 && !shape_pairs) ||
+      (max_candidate_pair > 0 && !candidate_pair))
+    return fail(std::string(what) + ": null argument");
+  if (shape_pair_count < 0 || max_candidate_pair < 0 || shape_count < 0)
+    return fail(std::string(what) + ": negative size");
+  return 0;
+}
+
+extern "C" int nmc_broad_phase_explicit(NmcBroadPhase* bp, int shape_count, const float* lower, const float* upper,
+                                        const float* gap, const int* shape_pairs, int shape_pair_count,
+                                        int* candidate_pair, int max_candidate_pair, int* candidate_pair_count,
+                                        int append, nmc_stream_t stream) {
+  if (!bp || bp->mode != 2) return fail("nmc_broad_phase_explicit: needs an explicit broad phase");
+  if (bp_explicit_check(shape_count, lower, upper, shape_pairs, shape_pair_count, candidate_pair,
+                        max_candidate_pair, candidate_pair_count, "nmc_broad_phase_explicit"))
+    return -1;
+  const long long per = static_cast<long long>(CBLOCK) * BP_EXPLICIT_ITEMS;
+  const int blocks = static_cast<int>((shape_pair_count + per - 1) / per);
+  if (blocks > bp->capacity)
+    return fail("nmc_broad_phase_explicit: more pairs than the broad phase was created for");
+  return bp_explicit_run(shape_count, lower, upper, gap, shape_pairs, shape_pair_count, blocks, bp->count, bp->off,
+                         bp->total, candidate_pair, max_candidate_pair, candidate_pair_count, append,
+                         reinterpret_cast<hipStream_t>(stream), "nmc_broad_phase_explicit");
+}
+

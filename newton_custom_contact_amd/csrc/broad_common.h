@@ -7,6 +7,7 @@
 namespace nmc {
 
 constexpr int CBLOCK = 256;
+constexpr int BP_EXPLICIT_ITEMS = 4;  // explicit broad phase: input pairs per lane per block
 
 __device__ __forceinline__ unsigned sortable_float(float f) {
   if (f == 0.0f) f = 0.0f;  // -0 and +0 compare equal: give them one key

@@ -77,6 +77,16 @@ struct NmcModelImpl {
   long long xpbd_keys_cap;
 };
 
+// Explicit broad phase (broad.hip): argument checks, and the count / scan / write launches over caller-given
+// per-block scratch (count and off: `blocks` ints each, total: 1 int).
+int bp_explicit_check(int shape_count, const float* lower, const float* upper, const int* shape_pairs,
+                      int shape_pair_count, const int* candidate_pair, int max_candidate_pair,
+                      const int* candidate_pair_count, const char* what);
+int bp_explicit_run(int shape_count, const float* lower, const float* upper, const float* gap,
+                    const int* shape_pairs, int shape_pair_count, int blocks, int* count, int* off, int* total,
+                    int* candidate_pair, int max_candidate_pair, int* candidate_pair_count, int append,
+                    hipStream_t s, const char* what);
+
 void set_error(const std::string& msg);
 int fail(const std::string& msg);               // sets the error string, returns -1
 int hip_fail(hipError_t e, const char* what);  // same, with the HIP error text

@@ -17,8 +17,9 @@ order varies):
 * ``BroadPhaseExplicit``: the given pairs whose AABBs overlap, in input order.
 
 ``sort_type``, ``tile_block_dim`` and ``sweep_thread_count_multiplier`` select the reference's Warp launch
-strategies; they are validated and kept, and do not change the result.  ``skip_count_zero`` is accepted: the
-count is always written whole.
+strategies; they are validated and kept, and do not change the result.  ``skip_count_zero=True`` leaves
+``candidate_pair_count`` as the caller set it: the pairs found go after that many and the count grows by their
+number (the reference appends through its atomic counter the same way; its callers pass 0 there).
 """
 
 from __future__ import annotations
@@ -163,7 +164,7 @@ class _SegmentBroadPhase:
         check(ctx.L.nmc_broad_phase_launch(ctx.h, int(n), ptr(shape_lower), ptr(shape_upper), ptr(shape_gap),
                                            ptr(shape_collision_group), ptr(shape_world), ptr(filter_pairs), nf,
                                            ptr(candidate_pair), int(candidate_pair.shape[0]),
-                                           ptr(candidate_pair_count), stream_handle(dev)),
+                                           ptr(candidate_pair_count), int(bool(skip_count_zero)), stream_handle(dev)),
               "nmc_broad_phase_launch")
 
 
@@ -225,5 +226,5 @@ class BroadPhaseExplicit:
         check(ctx.L.nmc_broad_phase_explicit(ctx.h, int(n), ptr(shape_lower), ptr(shape_upper), ptr(shape_gap),
                                              ptr(shape_pairs), count, ptr(candidate_pair),
                                              int(candidate_pair.shape[0]), ptr(candidate_pair_count),
-                                             stream_handle(dev)),
+                                             int(bool(skip_count_zero)), stream_handle(dev)),
               "nmc_broad_phase_explicit")

@@ -153,3 +153,52 @@ def test_pipeline_explicit_list_matches_oracle(cuda_device):
     rep_sub = _pipe_vs_oracle(model, "explicit", q, explicit=sub)
     print(rep_all, rep_nxn, rep_sub, flush=True)
     assert rep_all["count"] == rep_nxn["count"] > rep_sub["count"] > 0
+
+
+def test_skip_count_zero_appends(cuda_device):
+    """skip_count_zero=True: the pairs found go after the count already in the buffer and the count grows by
+    their number (two launches chained into one buffer hold both full results, in launch order)."""
+    print("[broad phase] skip_count_zero chaining", flush=True)
+    dev = cuda_device
+    rng = np.random.Generator(np.random.PCG64(11))
+    lower, upper, gap, group, world, filt = random_scene(rng, worlds=6, per_world=20, shared=2)
+    im, se = precompute_world_map(world)
+    bp = BroadPhaseAllPairs(world)
+    n1, got1 = _run(bp, dev, lower, upper, gap, group, world, filt, 4096)
+    assert n1 > 0
+    args = (_dev(lower, dev, np.float32), _dev(upper, dev, np.float32), _dev(gap, dev, np.float32),
+            _dev(group, dev, np.int32), _dev(world, dev, np.int32), len(world))
+    out = torch.full((2 * n1 + 8, 2), -7, dtype=torch.int32, device=dev)
+    cnt = torch.zeros(1, dtype=torch.int32, device=dev)
+    fp = _dev(filt, dev, np.int32)
+    bp.launch(*args, out, cnt, filter_pairs=fp)
+    bp.launch(*args, out, cnt, filter_pairs=fp, skip_count_zero=True)
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == 2 * n1
+    o = out.cpu().numpy()
+    np.testing.assert_array_equal(o[:n1], got1[:n1])
+    np.testing.assert_array_equal(o[n1:2 * n1], got1[:n1])
+    assert np.all(o[2 * n1:] == -7)
+    # without skip_count_zero the count restarts from zero
+    bp.launch(*args, out, cnt, filter_pairs=fp)
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == n1
+    # explicit: a count of 5 on entry puts the pairs at [5, 5 + k); an output too small keeps the full count
+    n = len(world)
+    pairs = np.array([(a, b) for a in range(n) for b in range(a + 1, n) if world[a] == world[b]], np.int32)
+    ref = oracle.broad_phase("explicit", lower, upper, gap, group, world, shape_pairs=pairs)
+    ex = BroadPhaseExplicit()
+    out = torch.full((len(ref) + 5, 2), -7, dtype=torch.int32, device=dev)
+    cnt = torch.full((1,), 5, dtype=torch.int32, device=dev)
+    ex.launch(args[0], args[1], args[2], _dev(pairs, dev, np.int32), len(pairs), out, cnt, skip_count_zero=True)
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == len(ref) + 5
+    o = out.cpu().numpy()
+    assert np.all(o[:5] == -7)
+    np.testing.assert_array_equal(o[5:], ref)
+    small = torch.full((len(ref), 2), -7, dtype=torch.int32, device=dev)
+    cnt.fill_(5)
+    ex.launch(args[0], args[1], args[2], _dev(pairs, dev, np.int32), len(pairs), small, cnt, skip_count_zero=True)
+    torch.cuda.synchronize()
+    assert int(cnt.item()) == len(ref) + 5
+    np.testing.assert_array_equal(small.cpu().numpy()[5:], ref[:len(ref) - 5])
