@@ -340,6 +340,16 @@ int nmc_broad_phase_launch(NmcBroadPhase* broad_phase, int shape_count, const fl
 int nmc_broad_phase_explicit(NmcBroadPhase* broad_phase, int shape_count, const float* lower, const float* upper,
                              const float* gap, const int* shape_pairs, int shape_pair_count, int* candidate_pair,
                              int max_candidate_pair, int* candidate_pair_count, int append, nmc_stream_t stream);
+/* nmc_broad_phase_explicit without a broad-phase object and without allocating anything: no counterpart in the
+ * reference; for callers that cannot call hipMalloc (code under graph capture, tight serving loops, harnesses
+ * that forbid it).  Same arguments and results as nmc_broad_phase_explicit.  Its per-block scratch is one
+ * static device array per GPU (128 KiB, held until the process ends), so one call takes at most 16 777 216
+ * input pairs (more is refused with an error before any launch), and calls that overlap on one GPU race
+ * (each call rewrites the whole part of the scratch it reads; nothing carries over between calls). */
+int nmc_broad_phase_explicit_noalloc(int shape_count, const float* lower, const float* upper, const float* gap,
+                                     const int* shape_pairs, int shape_pair_count, int* candidate_pair,
+                                     int max_candidate_pair, int* candidate_pair_count, int append,
+                                     nmc_stream_t stream);
 
 /* Copies the counters accumulated since model creation to host (synchronises the stream):
  * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = mesh-pair candidate

@@ -97,6 +97,7 @@ EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_ste
            "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_matcher_create", "nmc_matcher_destroy",
            "nmc_matcher_reset", "nmc_contact_match", "nmc_xpbd_step_report", "nmc_contact_force",
            "nmc_broad_phase_create", "nmc_broad_phase_destroy", "nmc_broad_phase_launch", "nmc_broad_phase_explicit",
+           "nmc_broad_phase_explicit_noalloc",
            "nmc_get_counters", "nmc_last_error")
 
 
@@ -172,6 +173,9 @@ def load() -> ctypes.CDLL:
         L.nmc_broad_phase_explicit.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
                                                ctypes.c_int, _vp, ctypes.c_int, _vp]
         L.nmc_broad_phase_explicit.restype = ctypes.c_int
+        L.nmc_broad_phase_explicit_noalloc.argtypes = [ctypes.c_int, _vp, _vp, _vp, _vp, ctypes.c_int, _vp,
+                                                       ctypes.c_int, _vp, ctypes.c_int, _vp]
+        L.nmc_broad_phase_explicit_noalloc.restype = ctypes.c_int
         L.nmc_get_counters.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp]
         L.nmc_get_counters.restype = ctypes.c_int
         L.nmc_last_error.argtypes = []
