@@ -113,7 +113,12 @@ typedef struct NmcModelDesc {
   const float* mesh_sdf_voxel;   /* M */
   const int* mesh_sdf_cells;     /* M x 3 */
   const int* mesh_sdf_flags;     /* M */
-  int mesh_contact_max;          /* contacts kept per mesh pair (deepest first), at most 255 */
+  /* CollisionPipeline(reduce_contacts, deterministic) for mesh pairs: 0 every contact the writer keeps
+   * (narrow_phase.py:1090-1190, sdf_contact.py:830-1065); 1 the reference's normal-bin / voxel reduction with
+   * the fast value packing, 2 with the deterministic packing (contact_reduction_global.py:232-1440); reduced,
+   * a pair keeps at most 240 contacts. */
+  int mesh_reduce;
+  const int* shape_voxel_resolution; /* S x 3: the reduction's voxel grid per shape (builder.py:9924-10065) */
   /* CollisionPipeline broad_phase (collide.py:605-735): 0 "sap" -- a segment's shapes sorted by their
    * projection on the fixed axis, a pair tested only when the later one's lower bound is below the earlier
    * one's upper bound (broad_phase_sap.py:189-394); 1 "nxn" -- every pair of a segment (broad_phase_nxn.py:

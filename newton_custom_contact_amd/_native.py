@@ -44,7 +44,8 @@ class NmcModelDesc(ctypes.Structure):
         ("shape_mesh", _vp), ("shape_aabb_lower", _vp), ("shape_aabb_upper", _vp), ("mesh_count", ctypes.c_int),
         ("mesh_vertex", _vp), ("mesh_vertex_start", _vp), ("mesh_edge", _vp), ("mesh_edge_start", _vp),
         ("mesh_sdf", _vp), ("mesh_sdf_start", _vp), ("mesh_sdf_lower", _vp), ("mesh_sdf_voxel", _vp),
-        ("mesh_sdf_cells", _vp), ("mesh_sdf_flags", _vp), ("mesh_contact_max", ctypes.c_int),
+        ("mesh_sdf_cells", _vp), ("mesh_sdf_flags", _vp), ("mesh_reduce", ctypes.c_int),
+        ("shape_voxel_resolution", _vp),
         ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _vp), ("explicit_pair_count", ctypes.c_int),
     ]
 
@@ -107,8 +108,6 @@ class NmcXpbdReport(ctypes.Structure):
 
 class NmcContactReport(ctypes.Structure):
     _fields_ = [("new_indices", _vp), ("new_count", _vp), ("broken_indices", _vp), ("broken_count", _vp)]
-
-MESH_CONTACT_MAX = 64  # contacts kept per mesh pair (the configuration's cap; the reference caps at 255)
 
 
 def lib_path() -> str:
@@ -225,7 +224,8 @@ class NativeModel:
     """Owns one NmcModel handle plus the extra device tables it points at."""
 
     def __init__(self, model, rigid_contact_max: int, shape_pairs_max: int, broad_phase_mode: int = 0,
-                 explicit_pairs: torch.Tensor | None = None):
+                 explicit_pairs: torch.Tensor | None = None, reduce_contacts: bool = True,
+                 deterministic: bool = False):
         if model.device.type != "cuda":
             raise RuntimeError(
                 "newton_custom_contact_amd runs its contact step on the GPU only (HIP kernels in libnmc.so); "
@@ -326,7 +326,9 @@ class NativeModel:
         d.mesh_sdf_lower, d.mesh_sdf_voxel, d.mesh_sdf_cells = (ptr(m.mesh_sdf_lower), ptr(m.mesh_sdf_voxel),
                                                                ptr(m.mesh_sdf_cells))
         d.mesh_sdf_flags = ptr(m.mesh_sdf_flags)
-        d.mesh_contact_max = MESH_CONTACT_MAX
+        # mesh contact reduction and its value packing (the reducer's deterministic flag is the pipeline's)
+        d.mesh_reduce = 0 if not reduce_contacts else (2 if deterministic else 1)
+        d.shape_voxel_resolution = ptr(m.shape_voxel_resolution)
         d.broad_phase_mode = int(broad_phase_mode)
         self.explicit_pairs = explicit_pairs
         d.explicit_pairs = ptr(explicit_pairs)

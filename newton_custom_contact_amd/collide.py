@@ -173,7 +173,8 @@ class CollisionPipeline:
         self._colliding = int(np.count_nonzero(colliding))
         self.shape_pairs_filtered = explicit_pairs
         self.native = NativeModel(model, self._rigid_contact_max, self.shape_pairs_max, bp_mode,
-                                  None if explicit_pairs is None else explicit_pairs.to(model.device))
+                                  None if explicit_pairs is None else explicit_pairs.to(model.device),
+                                  reduce_contacts=bool(reduce_contacts), deterministic=self.deterministic)
         self._lib = load()
         self.contact_matching = contact_matching
         self._matching_enabled = matching_enabled

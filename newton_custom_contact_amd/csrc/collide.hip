@@ -374,7 +374,7 @@ __global__ void __launch_bounds__(CBLOCK) narrow_kernel(NmcModelDesc d, const fl
     float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
     WriteCtx w{d.shape_gap, st, 5ll * g, 0};
     narrow_pair(sa, sb, d.shape_type[sa], d.shape_type[sb], A, B, gap_sum, w);
-    st.mask[g] = static_cast<unsigned char>(w.n);
+    st.mask[g] = static_cast<unsigned short>(w.n);
   }
 }
 
@@ -524,10 +524,10 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
                      m->counters);
   if (mesh)
     hipLaunchKernelGGL(contact_compact_kernel<true>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_off,
-                       m->seg_pair_count, d.shape_type, d.shape_body, body_q, d.mesh_contact_max, *c);
+                       m->seg_pair_count, d.shape_type, d.shape_body, body_q, m->mesh_k, *c);
   else
     hipLaunchKernelGGL(contact_compact_kernel<false>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage,
-                       m->seg_pair_off, m->seg_pair_count, d.shape_type, d.shape_body, body_q, d.mesh_contact_max,
+                       m->seg_pair_off, m->seg_pair_count, d.shape_type, d.shape_body, body_q, m->mesh_k,
                        *c);
   return hipGetLastError();
 }

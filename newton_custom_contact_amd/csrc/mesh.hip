@@ -5,22 +5,25 @@
 // w > threshold, or w <= threshold for a negative threshold (inward-wound meshes).
 //
 // mesh_narrow_kernel: one 256-lane workgroup per mesh pair (grid-stride over the call's mesh pairs).
-//   mesh (shape0) - infinite plane (shape1), narrow_phase.py:1073-1190: every mesh vertex closer to the
-//     plane than gap + margins is a contact at the midpoint of the vertex and its projection, normal
+//   mesh (shape0) - infinite plane (shape1), narrow_phase.py:1073-1326: every mesh vertex closer to the
+//     plane than gap + margins is a candidate at the midpoint of the vertex and its projection, normal
 //     mesh -> plane (minus the plane normal), sub key = vertex index.
-//   mesh - mesh, sdf_contact.py:813-1172: both ways (mode 0: edges of shape0 in the SDF of shape1,
+//   mesh - mesh, sdf_contact.py:813-1514: both ways (mode 0: edges of shape0 in the SDF of shape1,
 //     mode 1: the reverse).  An edge is culled by its bounding sphere against the SDF box and the SDF at
-//     its midpoint, searched for its deepest point (Brent), and becomes a contact at that point with the
-//     SDF gradient as normal (negated in mode 0); sub key = (edge << 2) | (mode << 1).
-// Pass 1 keeps (distance, sub key) of every candidate that passes the writer's gap test (collide.py:62-148)
-// in LDS; the pair keeps the mesh_contact_max deepest (ordered by distance, then sub key) and pass 2
-// evaluates those again and writes them in sub-key order into the pair's staging slots; the compaction
-// then places them in canonical order.  The reference instead reduces a
-// pair's contacts by normal bins in a global hash table (contact_reduction_global.py:671-1580).
+//     its midpoint, searched for its deepest point (Brent), and becomes a candidate at that point with the
+//     SDF gradient as normal (negated in mode 0) when the search distance is below gap + margins;
+//     sub key = (edge << 2) | (mode << 1).
+// reduce_contacts=False: every candidate the writer keeps (collide.py:62-148) is a contact.  Pass 1 lists
+//   their sub keys in LDS, they are sorted, pass 2 evaluates each again and writes it.
+// reduce_contacts=True: pass 1 enters every candidate into the pair's 240-slot reduction table in LDS
+//   (reduce.h); the distinct winners are sorted by sub key, evaluated again, given the octahedral round trip
+//   of their normal the reference's reducer buffer applies, kept when the writer keeps them and written.
+// Contacts go to the pair's staging slots in sub-key order; the compaction places them in canonical order.
 #include <hip/hip_runtime.h>
 
 #include "narrow.h"
 #include "nmc_internal.h"
+#include "reduce.h"
 #include "sdf.h"
 
 namespace nmc {
@@ -92,6 +95,7 @@ struct EdgeSide {
   float min_s;
   xform X_ts;    // T frame -> S frame
   xform XS;      // S world transform
+  xform XT;      // T world transform
   f3 tscale;     // T scale
   float threshold, threshold_u;
   const int* edge;
@@ -109,6 +113,7 @@ __device__ __forceinline__ EdgeSide edge_side(const NmcModelDesc& d, const Shape
   e.min_s = fmin2(fmin2(e.ss.x, e.ss.y), e.ss.z);
   e.X_ts = xmul(xinverse(S.X), T.X);
   e.XS = S.X;
+  e.XT = T.X;
   e.tscale = T.gscale;
   e.threshold = gap_sum + T.margin + S.margin;
   e.threshold_u = e.threshold / e.min_s;
@@ -182,12 +187,55 @@ __device__ __forceinline__ bool vertex_contact(const PlaneSide& P, int v, f3& ce
   return true;
 }
 
-// The writer's gap test (collide.py:62-148) for a contact of this pair.
-__device__ __forceinline__ bool mesh_keeps(CData& c, f3 center, f3 normal, float dist) {
-  c.center = center;
-  c.normal = normal;
-  c.dist = dist;
-  return passes_gap(c);
+__device__ __forceinline__ int mesh_block_scan(bool f, int* s_w, int& total) {
+  const unsigned long long b = __ballot(f);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int pre = __popcll(b & ((1ull << lane) - 1ull));
+  if (lane == 0) s_w[w] = __popcll(b);
+  __syncthreads();
+  int base = 0;
+  total = 0;
+  for (int q = 0; q < MESH_BLOCK / 64; ++q) {
+    base += q < w ? s_w[q] : 0;
+    total += s_w[q];
+  }
+  __syncthreads();
+  return base + pre;
+}
+
+// The reduction frame of one side of a pair: where its candidates compete (contact_reduction_global.py
+// :968-1059 for mesh-plane, :1092-1227 for mesh-mesh).
+struct RedFrame {
+  bool centered;  // mesh-mesh: project relative to the shapes' midpoint, fixed beta
+  f3 midpoint;
+  float beta;     // depth below which a candidate competes in the scan slots
+  xform X_inv;    // world -> frame of the voxel shape
+  f3 lo, hi;      // voxel shape's local collision AABB
+  int res[3];
+};
+
+__device__ __forceinline__ RedFrame red_frame(const NmcModelDesc& d, int shape, const xform& X, bool centered,
+                                              f3 midpoint) {
+  RedFrame f;
+  f.centered = centered;
+  f.midpoint = midpoint;
+  f.lo = load3(d.shape_aabb_lower + 3 * shape);
+  f.hi = load3(d.shape_aabb_upper + 3 * shape);
+  f.beta = centered ? RED_BETA : RED_BETA * len(f.hi - f.lo);
+  f.X_inv = xinverse(X);
+  f.res[0] = d.shape_voxel_resolution[3 * shape];
+  f.res[1] = d.shape_voxel_resolution[3 * shape + 1];
+  f.res[2] = d.shape_voxel_resolution[3 * shape + 2];
+  return f;
+}
+
+__device__ __forceinline__ void red_candidate(unsigned long long* slots, const RedParams& rp, const RedFrame& f,
+                                              f3 center, f3 normal, float dist, int sub) {
+  // mesh-plane candidates are binned by their stored (octahedral) normal, mesh-mesh ones by the computed one
+  const f3 bn = f.centered ? normal : red_oct_roundtrip(normal);
+  const f3 proj = f.centered ? center - f.midpoint : center;
+  const int vox = red_voxel(xpoint(f.X_inv, center), f.lo, f.hi, f.res[0], f.res[1], f.res[2]);
+  red_insert(slots, rp, bn, proj, dist, dist < f.beta, vox, sub);
 }
 
 __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d, const float* __restrict__ body_q,
@@ -196,14 +244,16 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
                                                                  const int* __restrict__ seg_pair_count,
                                                                  const int* __restrict__ seg_mesh_count,
                                                                  const int* __restrict__ seg_mesh_off, int segments,
-                                                                 int cand_cap, int* __restrict__ counters) {
+                                                                 int cand_cap, int mesh_k, RedParams rp,
+                                                                 int* __restrict__ counters) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_n;
-  // (sortable distance << 32) | sub key, then the kept sub keys
-  unsigned long long* keys = reinterpret_cast<unsigned long long*>(smem);
-  unsigned* sel = reinterpret_cast<unsigned*>(smem + sizeof(unsigned long long) * cand_cap);
+  __shared__ int s_w[MESH_BLOCK / 64];
+  // reduced: the 240 slots, then 256 winner sub keys; unreduced: cand_cap candidate sub keys
+  unsigned long long* slots = reinterpret_cast<unsigned long long*>(smem);
+  unsigned* sel = reinterpret_cast<unsigned*>(smem + (rp.mode ? sizeof(unsigned long long) * 256 : 0));
   const int total = st.route_count[2];
-  const int K = d.mesh_contact_max;
+  const bool reduce = rp.mode != 0;
   for (int i = blockIdx.x; i < total; i += gridDim.x) {
     int lo = 0, hi = segments - 1;  // last segment whose mesh offset is <= i
     while (lo < hi) {
@@ -220,6 +270,8 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
     const int sa = st.pair_a[g], sb = st.pair_b[g];
     const bool plane = d.shape_type[sb] == GEO_PLANE;
     if (threadIdx.x == 0) s_n = 0;
+    if (reduce)
+      for (int k = threadIdx.x; k < RED_SLOTS; k += blockDim.x) slots[k] = 0ull;
     __syncthreads();
     const ShapeW A = load_wc(st.shape_wc + 20ll * sa), B = load_wc(st.shape_wc + 20ll * sb);
     const float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
@@ -244,83 +296,108 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
       P.limit = gap_sum + (A.margin + B.margin);
       P.vtx = d.mesh_vertex + 3 * d.mesh_vertex_start[ma];
     }
-    // ---- pass 1: every candidate's distance, kept ones as keys (sub key: vertex, or edge and mode)
+    // ---- pass 1: every candidate into the reduction table, or its sub key into the list when the writer keeps it
     auto push = [&](int sub, f3 center, f3 normal, float dist) {
-      if (!mesh_keeps(c, center, normal, dist)) return;
+      c.center = center;
+      c.normal = normal;
+      c.dist = dist;
+      if (!passes_gap(c)) return;
       const int k = atomicAdd(&s_n, 1);
-      if (k < cand_cap) keys[k] = (static_cast<unsigned long long>(mesh_sortable(dist)) << 32) | static_cast<unsigned>(sub);
+      if (k < cand_cap) sel[k] = static_cast<unsigned>(sub);
     };
     if (plane) {
       const int nv = d.mesh_vertex_start[ma + 1] - d.mesh_vertex_start[ma];
+      RedFrame F;
+      if (reduce) F = red_frame(d, sa, A.X, false, f3{0.0f, 0.0f, 0.0f});
       for (int v = threadIdx.x; v < nv; v += blockDim.x) {
         f3 center, normal;
         float dist;
-        if (vertex_contact(P, v, center, normal, dist)) push(v, center, normal, dist);
+        if (!vertex_contact(P, v, center, normal, dist)) continue;
+        if (reduce) red_candidate(slots, rp, F, center, normal, dist, v);
+        else push(v, center, normal, dist);
       }
     } else {
       for (int mode = 0; mode < 2; ++mode) {
         const int tm = mode == 0 ? ma : mb;
         const EdgeSide E = mode == 0 ? edge_side(d, A, B, ma, mb, gap_sum) : edge_side(d, B, A, mb, ma, gap_sum);
+        RedFrame F;
+        if (reduce) F = red_frame(d, mode == 0 ? sa : sb, E.XT, true, (E.XT.p + E.XS.p) * 0.5f);
         const int ne = d.mesh_edge_start[tm + 1] - d.mesh_edge_start[tm];
         for (int e = threadIdx.x; e < ne; e += blockDim.x) {
           f3 center, normal;
           float dist;
-          if (edge_contact(E, mode, e, center, normal, dist)) push((e << 2) | (mode << 1), center, normal, dist);
+          if (!edge_contact(E, mode, e, center, normal, dist)) continue;
+          const int sub = (e << 2) | (mode << 1);
+          if (reduce) red_candidate(slots, rp, F, center, normal, dist, sub);
+          else push(sub, center, normal, dist);
         }
       }
     }
     __syncthreads();
-    int n = s_n;
-    if (n > cand_cap) {
-      if (threadIdx.x == 0) atomicAdd(&counters[1], n - cand_cap);
-      n = cand_cap;
-    }
-    // ---- keep the K deepest (distance, then sub key); order them by sub key
-    const int kept = n < K ? n : K;
-    if (n > K) {
-      int n2 = 1;
-      while (n2 < n) n2 <<= 1;
-      for (int k = n + threadIdx.x; k < n2; k += blockDim.x) keys[k] = ~0ull;
-      __syncthreads();
-      mesh_bitonic(keys, n2);
-    }
-    int k2 = 1;
-    while (k2 < kept) k2 <<= 1;
-    for (int k = threadIdx.x; k < k2; k += blockDim.x)
-      sel[k] = k < kept ? static_cast<unsigned>(keys[k] & 0xffffffffu) : 0xffffffffu;
-    __syncthreads();
-    mesh_bitonic(sel, k2);
-    // ---- pass 2: the kept candidates again, written in sub-key order
-    const WriteCtx w{d.shape_gap, st, st.mesh_slot_base + static_cast<long long>(i) * K, 0};
-    for (int k = threadIdx.x; k < kept; k += blockDim.x) {
-      const int sub = static_cast<int>(sel[k]);
-      f3 center, normal;
-      float dist;
-      if (plane) {
-        (void)vertex_contact(P, sub, center, normal, dist);
-      } else {
-        const int mode = (sub >> 1) & 1;
-        const EdgeSide E = mode == 0 ? edge_side(d, A, B, ma, mb, gap_sum) : edge_side(d, B, A, mb, ma, gap_sum);
-        (void)edge_contact(E, mode, sub >> 2, center, normal, dist);
+    // ---- the sub keys to write, sorted: the list, or the table's winners (each once)
+    int n, n2;
+    if (reduce) {
+      for (int k = threadIdx.x; k < 256; k += blockDim.x) {
+        const unsigned long long v = k < RED_SLOTS ? slots[k] : 0ull;
+        sel[k] = v ? static_cast<unsigned>(red_winner(v, rp.mode)) : 0xffffffffu;
       }
+      n = 256;
+      n2 = 256;
+    } else {
+      n = s_n;
+      if (n > cand_cap) {
+        if (threadIdx.x == 0) atomicAdd(&counters[1], n - cand_cap);
+        n = cand_cap;
+      }
+      n2 = 1;
+      while (n2 < n) n2 <<= 1;
+      for (int k = n + threadIdx.x; k < n2; k += blockDim.x) sel[k] = 0xffffffffu;
+    }
+    __syncthreads();
+    mesh_bitonic(sel, n2);
+    // ---- pass 2: evaluate each again; reduced: first of equal sub keys, octahedral normal, writer test
+    const WriteCtx w{d.shape_gap, st, st.mesh_slot_base + static_cast<long long>(i) * mesh_k, 0};
+    int written = 0;
+    for (int k0 = 0; k0 < n; k0 += blockDim.x) {
+      const int k = k0 + threadIdx.x;
+      const unsigned sub_u = k < n ? sel[k] : 0xffffffffu;
+      bool live = sub_u != 0xffffffffu && (!reduce || k == 0 || sel[k - 1] != sub_u);
       CData o = c;
-      o.center = center;
-      o.normal = normal;
-      o.dist = dist;
-      o.sub_key = sub;
       f3 nn, aw, bw;
-      (void)contact_d(o, nn, aw, bw);
-      write_record(o, nn, aw, bw, w, w.slot0 + k);
+      if (live) {
+        const int sub = static_cast<int>(sub_u);
+        f3 center, normal;
+        float dist;
+        if (plane) {
+          (void)vertex_contact(P, sub, center, normal, dist);
+        } else {
+          const int mode = (sub >> 1) & 1;
+          const EdgeSide E = mode == 0 ? edge_side(d, A, B, ma, mb, gap_sum) : edge_side(d, B, A, mb, ma, gap_sum);
+          (void)edge_contact(E, mode, sub >> 2, center, normal, dist);
+        }
+        o.center = center;
+        o.normal = reduce ? red_oct_roundtrip(normal) : normal;
+        o.dist = dist;
+        o.sub_key = sub;
+        const float dd = contact_d(o, nn, aw, bw);
+        if (reduce) live = dd <= o.gap_sum;
+      }
+      int cnt;
+      const int at = mesh_block_scan(live, s_w, cnt);
+      if (live && written + at < mesh_k) write_record(o, nn, aw, bw, w, w.slot0 + written + at);
+      written += cnt;
     }
     if (threadIdx.x == 0) {
-      st.mask[g] = static_cast<unsigned char>(kept);
+      st.mask[g] = static_cast<unsigned short>(written < mesh_k ? written : mesh_k);
       st.mesh_index[g] = i;
     }
     __syncthreads();
   }
 }
 
-size_t mesh_lds_bytes(int cand_cap) { return sizeof(unsigned long long) * (size_t)cand_cap + sizeof(unsigned) * 256; }
+size_t mesh_lds_bytes(int cand_cap, bool reduce) {
+  return reduce ? sizeof(unsigned long long) * 256 + sizeof(unsigned) * 256 : sizeof(unsigned) * (size_t)cand_cap;
+}
 
 hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t stream) {
   if (m->stage.mesh_pair_capacity <= 0) return hipSuccess;
@@ -328,7 +405,7 @@ hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t 
   const int blocks = static_cast<int>(cap < MESH_MAX_BLOCKS ? cap : MESH_MAX_BLOCKS);
   hipLaunchKernelGGL(mesh_narrow_kernel, dim3(blocks), dim3(MESH_BLOCK), m->mesh_lds, stream, m->d, body_q, m->stage,
                      m->seg_pair_off, m->seg_pair_count, m->seg_mesh_count, m->seg_mesh_off, m->d.segment_count,
-                     m->mesh_cand_cap, m->counters);
+                     m->mesh_cand_cap, m->mesh_k, m->red, m->counters);
   return hipGetLastError();
 }
 

@@ -1,6 +1,8 @@
 // extern "C" entry points of libnmc.so (declared in include/nmc.h).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+#include <algorithm>
 #include <cstdint>
 #include <cstring>
 #include <mutex>
@@ -12,7 +14,7 @@
 namespace nmc {
 
 size_t collide_lds_enum_bytes(int max_n);
-size_t mesh_lds_bytes(int cand_cap);
+size_t mesh_lds_bytes(int cand_cap, bool reduce);
 size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap);
 hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContacts* c, hipStream_t stream);
 hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,
@@ -71,13 +73,49 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
         !d.mesh_edge || !d.mesh_edge_start || !d.mesh_sdf || !d.mesh_sdf_start || !d.mesh_sdf_lower ||
         !d.mesh_sdf_voxel || !d.mesh_sdf_cells || !d.mesh_sdf_flags)
       return fail("a model with meshes needs every mesh table (vertices, edges, local AABBs, SDF grids)");
-    if (d.mesh_contact_max < 1 || d.mesh_contact_max > 255) return fail("mesh_contact_max must be in [1, 255]");
+    if (d.mesh_reduce < 0 || d.mesh_reduce > 2) return fail("mesh_reduce must be 0, 1 or 2");
+    if (!d.shape_voxel_resolution) return fail("a model with meshes needs shape_voxel_resolution");
   }
   NmcModel* m = new NmcModel();
   m->d = d;
   hipError_t e = hipGetDevice(&m->device);
   if (e != hipSuccess) { delete m; return hip_fail(e, "hipGetDevice"); }
   int segs = d.segment_count;
+  // mesh pairs: candidates per pair are at most both meshes' edges (mesh-mesh) or the mesh's vertices
+  // (mesh-plane); staging slots per pair: the reduction's 240, or every candidate unreduced
+  m->mesh_k = 0;
+  m->mesh_cand_cap = 0;
+  m->red.mode = d.mesh_count > 0 ? d.mesh_reduce : 0;
+  {
+    // get_spatial_direction_2d: angle k (2 pi / 6) in float32, cosine and sine correctly rounded to float32
+    const float step = (2.0f * 3.14159265358979323846f) / 6.0f;
+    for (int k = 0; k < 6; ++k) {
+      const float a = static_cast<float>(k) * step;
+      m->red.dirs.c[k] = static_cast<float>(std::cos(static_cast<double>(a)));
+      m->red.dirs.s[k] = static_cast<float>(std::sin(static_cast<double>(a)));
+    }
+  }
+  if (d.mesh_count > 0) {
+    std::vector<int> vstart(d.mesh_count + 1), estart(d.mesh_count + 1);
+    if ((e = hipMemcpy(vstart.data(), d.mesh_vertex_start, sizeof(int) * (d.mesh_count + 1), hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(estart.data(), d.mesh_edge_start, sizeof(int) * (d.mesh_count + 1), hipMemcpyDeviceToHost)) != hipSuccess) {
+      delete m;
+      return hip_fail(e, "nmc_model_create: mesh tables");
+    }
+    int need = 1;
+    for (int k = 0; k < d.mesh_count; ++k) {
+      const int nv = vstart[k + 1] - vstart[k], ne = estart[k + 1] - estart[k];
+      need = std::max(need, std::max(nv, 2 * ne));
+    }
+    if (need > 4096) {
+      delete m;
+      return fail("mesh narrow phase: a pair may gather " + std::to_string(need) +
+                  " candidate contacts (2 x edges or vertices of the largest mesh); the limit is 4096");
+    }
+    m->mesh_cand_cap = next_pow2(need);
+    m->mesh_k = d.mesh_reduce ? 240 : need;
+    m->mesh_lds = mesh_lds_bytes(m->mesh_cand_cap, d.mesh_reduce != 0);
+  }
   // per-segment pair region: the segment's worst case n(n-1)/2, capped at 16384 pairs (a pair entry stages 5
   // contact records of 48 B)
   long long maxn = d.max_segment_shapes;
@@ -88,7 +126,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   m->collide_lds = collide_lds_enum_bytes(d.max_segment_shapes > 0 ? d.max_segment_shapes : 1);
   // solver ordering buffer: 2 entries per contact; worlds with up to 512 contacts order them in LDS,
   // denser worlds in a global buffer of 2 entries per contact slot.  Models with meshes (up to
-  // mesh_contact_max contacts per pair, few bodies per world) size the LDS buffer for their worst world,
+  // mesh_k contacts per pair, few bodies per world) size the LDS buffer for their worst world,
   // up to 8192 entries, so that each body's contact list is walked out of LDS.
   m->xpbd_contact_cap = 1024;
   if (d.mesh_count > 0 && segs > 1) {
@@ -110,7 +148,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       long long n = starts[sg + 1] - starts[sg], mm = 0;
       for (int k = starts[sg]; k < starts[sg + 1]; ++k) mm += stype[seg_shape[k]] == GEO_MESH;
       long long all = n * (n - 1) / 2, mesh_pairs = all - (n - mm) * (n - mm - 1) / 2;
-      long long c = 5 * (all - mesh_pairs) + (long long)d.mesh_contact_max * mesh_pairs;
+      long long c = 5 * (all - mesh_pairs) + (long long)m->mesh_k * mesh_pairs;
       worst_world = c > worst_world ? c : worst_world;
     }
     long long want = (2 * worst_world + 63) / 64 * 64;
@@ -214,15 +252,11 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     s.mesh_pair_capacity = 0;
     if (d.mesh_count > 0) {
       // mesh pairs: per segment at most the pairs that involve one of its m mesh shapes,
-      // n(n-1)/2 - (n-m)(n-m-1)/2, within the segment's region; candidates per pair: both meshes' edges
-      // (mesh-mesh) or the mesh's vertices (mesh-plane)
+      // n(n-1)/2 - (n-m)(n-m-1)/2, within the segment's region
       const int nss = starts[segs];
       std::vector<int> seg_shape(nss > 0 ? nss : 1), stype(d.shape_count > 0 ? d.shape_count : 1);
-      std::vector<int> vstart(d.mesh_count + 1), estart(d.mesh_count + 1);
       if ((nss > 0 && (e = hipMemcpy(seg_shape.data(), d.segment_shape, sizeof(int) * nss, hipMemcpyDeviceToHost)) != hipSuccess) ||
-          (d.shape_count > 0 && (e = hipMemcpy(stype.data(), d.shape_type, sizeof(int) * d.shape_count, hipMemcpyDeviceToHost)) != hipSuccess) ||
-          (e = hipMemcpy(vstart.data(), d.mesh_vertex_start, sizeof(int) * (d.mesh_count + 1), hipMemcpyDeviceToHost)) != hipSuccess ||
-          (e = hipMemcpy(estart.data(), d.mesh_edge_start, sizeof(int) * (d.mesh_count + 1), hipMemcpyDeviceToHost)) != hipSuccess) {
+          (d.shape_count > 0 && (e = hipMemcpy(stype.data(), d.shape_type, sizeof(int) * d.shape_count, hipMemcpyDeviceToHost)) != hipSuccess)) {
         nmc_model_destroy(m);
         return hip_fail(e, "nmc_model_create: mesh tables");
       }
@@ -233,23 +267,8 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
         long long w = n * (n - 1) / 2 - (n - mm) * (n - mm - 1) / 2;
         mesh_pairs += w < cap[sg] ? w : cap[sg];
       }
-      int need = d.mesh_contact_max;
-      int max_e = 0;
-      for (int k = 0; k < d.mesh_count; ++k) {
-        int nv = vstart[k + 1] - vstart[k], ne = estart[k + 1] - estart[k];
-        max_e = ne > max_e ? ne : max_e;
-        need = nv > need ? nv : need;
-      }
-      need = 2 * max_e > need ? 2 * max_e : need;
-      if (need > 4096) {
-        nmc_model_destroy(m);
-        return fail("mesh narrow phase: a pair may gather " + std::to_string(need) +
-                    " candidate contacts (2 x edges or vertices of the largest mesh); the limit is 4096");
-      }
-      m->mesh_cand_cap = next_pow2(need);
-      m->mesh_lds = mesh_lds_bytes(m->mesh_cand_cap);
       s.mesh_pair_capacity = mesh_pairs;
-      slots += mesh_pairs * d.mesh_contact_max;
+      slots += mesh_pairs * m->mesh_k;
     }
     if ((e = dmalloc(&s.rec, 3 * slots)) != hipSuccess || (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
         (e = dmalloc(&s.pair_b, pairs)) != hipSuccess || (e = dmalloc(&s.route, pairs)) != hipSuccess ||

@@ -21,13 +21,26 @@ constexpr float MAXVAL = 1.0e10f;  // newton/_src/core/types.py:72
 // Broad-phase pairs and their staged contact records.  Every segment owns a fixed region of pair
 // entries (its worst case n(n-1)/2, capped by the LDS pair buffer) at seg_pair_off[seg]; pair g owns
 // contact slots [5g, 5g + 5), except mesh pairs: the i-th mesh pair of the call (route order) owns
-// slots [mesh_slot_base + i K, mesh_slot_base + (i + 1) K), K = mesh_contact_max.  Fixed regions need
+// slots [mesh_slot_base + i K, mesh_slot_base + (i + 1) K), K = mesh_k (240 reduced, the largest candidate count
+// of a pair unreduced).  Fixed regions need
 // no global allocation counter.
+// Mesh contact reduction parameters (reduce.h): the 6 scan directions (cos, sin) of angle k (2 pi / 6) taken in
+// float32 (get_spatial_direction_2d, contact_reduction.py:334-345), correctly rounded and filled on the host,
+// and the value packing: 0 no reduction, 1 fast (full score, sub key), 2 deterministic (22-bit score,
+// 22-bit fingerprint = sub key, 20-bit id = sub key).
+struct RedDirs {
+  float c[6], s[6];
+};
+struct RedParams {
+  RedDirs dirs;
+  int mode;
+};
+
 struct StageBuffers {
   // one 48-byte record per contact slot: world-frame point on A 3, on B 3, unit normal 3, offset A,
   // offset B (radius + margin), sub_key (int bits) -- written and read as three 16-byte vectors
   float4* rec;
-  unsigned char* mask;   // per pair: number of records in its slots (0-5, mesh pairs 0-K)
+  unsigned short* mask;  // per pair: number of records in its slots (0-5, mesh pairs 0-K)
   int* pair_a;           // first shape (lower GeoType), global id
   int* pair_b;
   long long pair_capacity;  // total pair entries (sum of the segment regions)
@@ -53,6 +66,8 @@ struct NmcModelImpl {
   int* seg_mesh_count;      // [segments] mesh pairs (mesh-mesh, mesh-infinite plane)
   int* seg_mesh_off;        // [segments] exclusive prefix of mesh pairs over segments
   int mesh_cand_cap;        // candidate contacts one mesh pair gathers in LDS (power of two)
+  int mesh_k;               // staging slots per mesh pair
+  RedParams red;            // reduction of mesh-pair contacts
   size_t mesh_lds;
   int* seg_contact_count;   // [segments] contacts produced this call
   int* seg_small;           // segments of at most 64 shapes (one wave each), and the others (one workgroup)

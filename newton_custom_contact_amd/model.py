@@ -172,12 +172,44 @@ class Model:
         m._finish()
         return m
 
+    @staticmethod
+    def voxel_resolution(lower, upper, budget: int = 100) -> tuple[int, int, int]:
+        """Voxel grid of a shape's contact reduction (compute_voxel_resolution_from_aabb, builder.py:9936-9961):
+        about cubic voxels, at most ``budget`` (NUM_VOXEL_DEPTH_SLOTS), shrinking the largest axis first."""
+        size = np.maximum(np.asarray(upper, np.float64) - np.asarray(lower, np.float64), 1e-6)
+        volume = size[0] * size[1] * size[2]
+        v = max((volume / budget) ** (1.0 / 3.0), 1e-6)
+        nx, ny, nz = (max(1, round(float(x) / v)) for x in size)
+        while nx * ny * nz > budget:
+            if nx >= ny and nx >= nz and nx > 1:
+                nx -= 1
+            elif ny >= nz and ny > 1:
+                ny -= 1
+            elif nz > 1:
+                nz -= 1
+            else:
+                break
+        return nx, ny, nz
+
+    def _voxel_table(self, mesh_aabb: dict):
+        """Per-shape voxel resolution of the contact reduction (builder.py:9924-10065) for the mesh shapes, from
+        their local AABB (scaled vertices, in float64 as the builder computes it).  Only mesh pairs are reduced
+        in this build, so the other shapes keep 1 x 1 x 1."""
+        res = np.ones((self.shape_count, 3), _I32)
+        cache = {}
+        for s, (lo, hi) in mesh_aabb.items():
+            key = (tuple(np.asarray(lo, np.float64)), tuple(np.asarray(hi, np.float64)))
+            if key not in cache:
+                cache[key] = self.voxel_resolution(lo, hi)
+            res[s] = cache[key]
+        self.host["shape_voxel_resolution"] = res
+
     def _mesh_tables(self, sources: list, scales: list | None = None):
         """Per-shape mesh index and local AABB (scale baked, collide.py:215-245), and the concatenated
         mesh vertex / edge tables (builder.py:10316-10351)."""
         h = self.host
         S = self.shape_count
-        meshes, index = [], {}
+        meshes, index, mesh_aabb = [], {}, {}
         shape_mesh = np.full(S, -1, _I32)
         lo = np.zeros((S, 3), _F32)
         hi = np.zeros((S, 3), _F32)
@@ -190,6 +222,7 @@ class Model:
             shape_mesh[s] = index[id(src)]
             a, b_ = src.local_aabb(h["shape_scale"][s] if scales is None else scales[s])
             lo[s], hi[s] = a, b_
+            mesh_aabb[s] = (a, b_)
         self.meshes = meshes
         h["shape_mesh"] = shape_mesh
         h["shape_aabb_lower"] = lo
@@ -200,6 +233,7 @@ class Model:
         h["mesh_vertex_start"] = np.concatenate([[0], np.cumsum([len(mm.vertices) for mm in meshes])]).astype(_I32)
         h["mesh_edge"] = np.concatenate(es).astype(_I32) if meshes else np.zeros((1, 2), _I32)
         h["mesh_edge_start"] = np.concatenate([[0], np.cumsum([len(mm.edges) for mm in meshes])]).astype(_I32)
+        self._voxel_table(mesh_aabb)
 
     def _mesh_sdf_tables(self):
         """Concatenate the meshes' SDF grids on the device (cooked on the GPU when missing)."""

@@ -43,7 +43,8 @@ class OracleModel(ctypes.Structure):
         ("shape_mesh", _i32p), ("shape_aabb_lower", _f32p), ("shape_aabb_upper", _f32p), ("mesh_count", ctypes.c_int),
         ("mesh_vertex", _f32p), ("mesh_vertex_start", _i32p), ("mesh_edge", _i32p), ("mesh_edge_start", _i32p),
         ("mesh_sdf", _f32p), ("mesh_sdf_start", _i64p), ("mesh_sdf_lower", _f32p), ("mesh_sdf_voxel", _f32p),
-        ("mesh_sdf_cells", _i32p), ("mesh_sdf_flags", _i32p), ("mesh_contact_max", ctypes.c_int),
+        ("mesh_sdf_cells", _i32p), ("mesh_sdf_flags", _i32p), ("mesh_reduce", ctypes.c_int),
+        ("shape_voxel_res", _i32p),
         ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _i32p), ("explicit_pair_count", ctypes.c_int),
     ]
 
@@ -105,7 +106,8 @@ def _p(a: np.ndarray, t):
 class Oracle:
     """Holds a model's host arrays (kept alive) and runs the restatement on them."""
 
-    def __init__(self, model, mesh_contact_max: int = 64, broad_phase: str = "sap", explicit_pairs=None):
+    def __init__(self, model, reduce_contacts: bool = True, deterministic: bool = True, broad_phase: str = "sap",
+                 explicit_pairs=None):
         h = model.host
         self.model = model
         keep = {}
@@ -161,7 +163,10 @@ class Oracle:
         m.mesh_sdf_lower, m.mesh_sdf_voxel, m.mesh_sdf_cells = (f32("mesh_sdf_lower"), f32("mesh_sdf_voxel"),
                                                                 i32("mesh_sdf_cells"))
         m.mesh_sdf_flags = i32("mesh_sdf_flags")
-        m.mesh_contact_max = mesh_contact_max
+        # the reference's mesh contact reduction and its value packing (CollisionPipeline reduce_contacts /
+        # deterministic)
+        m.mesh_reduce = 0 if not reduce_contacts else (2 if deterministic else 1)
+        m.shape_voxel_res = i32("shape_voxel_resolution")
         # CollisionPipeline broad_phase: "explicit" without a list is the model's own pair list, the "nxn" set
         if broad_phase == "explicit" and explicit_pairs is None:
             broad_phase = "nxn"
@@ -505,3 +510,83 @@ def mlp_backward(params: dict, x: np.ndarray, t: np.ndarray) -> tuple[dict, floa
                           _p(t, _f32p), ctypes.c_int(len(x)),
                           *[_p(g[k], _f32p) for k in ("w1", "b1", "w2", "b2", "w3", "b3")], _p(loss, _f32p))
     return g, float(loss[0])
+
+
+# ---------------------------------------------------------------- contact reduction primitives
+class reduction:
+    """The restatement's contact-reduction primitives (oracle_reduce.h), for tests."""
+
+    @staticmethod
+    def get_slot(normals) -> np.ndarray:
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        out = np.zeros(len(n), np.int32)
+        lib().oracle_red_get_slot(_p(n, _f32p), ctypes.c_int(len(n)), _p(out, _i32p))
+        return out
+
+    @staticmethod
+    def project(bins, points) -> np.ndarray:
+        b = np.ascontiguousarray(bins, np.int32).reshape(-1)
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        out = np.zeros((len(p), 2), np.float32)
+        lib().oracle_red_project(_p(b, _i32p), _p(p, _f32p), ctypes.c_int(len(p)), _p(out, _f32p))
+        return out
+
+    @staticmethod
+    def directions() -> np.ndarray:
+        out = np.zeros((6, 2), np.float32)
+        lib().oracle_red_direction(_p(out, _f32p))
+        return out
+
+    @staticmethod
+    def oct(normals):
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        enc = np.zeros((len(n), 2), np.float32)
+        dec = np.zeros((len(n), 3), np.float32)
+        lib().oracle_red_oct(_p(n, _f32p), ctypes.c_int(len(n)), _p(enc, _f32p), _p(dec, _f32p))
+        return enc, dec
+
+    @staticmethod
+    def value(score, fingerprint, cid, det: bool) -> np.ndarray:
+        s = np.ascontiguousarray(score, np.float32).reshape(-1)
+        f = np.ascontiguousarray(fingerprint, np.int32).reshape(-1)
+        c = np.ascontiguousarray(cid, np.uint32).reshape(-1)
+        out = np.zeros(len(s), np.uint64)
+        lib().oracle_red_value(_p(s, _f32p), _p(f, _i32p), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                               ctypes.c_int(len(s)), ctypes.c_int(int(det)),
+                               out.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)))
+        return out
+
+    @staticmethod
+    def voxel_index(points, lower, upper, res) -> np.ndarray:
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        lo = np.ascontiguousarray(lower, np.float32)
+        hi = np.ascontiguousarray(upper, np.float32)
+        r = np.ascontiguousarray(res, np.int32)
+        out = np.zeros(len(p), np.int32)
+        lib().oracle_red_voxel_index(_p(p, _f32p), ctypes.c_int(len(p)), _p(lo, _f32p), _p(hi, _f32p),
+                                     _p(r, _i32p), _p(out, _i32p))
+        return out
+
+    @staticmethod
+    def voxel_resolution(lower, upper, budget: int = 100):
+        lo = np.ascontiguousarray(lower, np.float64)
+        hi = np.ascontiguousarray(upper, np.float64)
+        out = np.zeros(3, np.int32)
+        lib().oracle_red_voxel_resolution(_p(lo, _f64p), _p(hi, _f64p), ctypes.c_int(budget), _p(out, _i32p))
+        return tuple(int(x) for x in out)
+
+    @staticmethod
+    def reduce(points, normals, depth, sub, use_beta, voxel, det: bool = True) -> np.ndarray:
+        """Candidate indices the reduction keeps (points are both the stored and the projected position)."""
+        p = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        d = np.ascontiguousarray(depth, np.float32).reshape(-1)
+        s = np.ascontiguousarray(sub, np.int32).reshape(-1)
+        b = np.ascontiguousarray(use_beta, np.int32).reshape(-1)
+        v = np.ascontiguousarray(voxel, np.int32).reshape(-1)
+        out = np.zeros(max(len(p), 1), np.int32)
+        L = lib()
+        L.oracle_red_reduce.restype = ctypes.c_int
+        k = L.oracle_red_reduce(_p(p, _f32p), _p(n, _f32p), _p(d, _f32p), _p(s, _i32p), _p(b, _i32p), _p(v, _i32p),
+                                ctypes.c_int(len(p)), ctypes.c_int(int(det)), _p(out, _i32p))
+        return np.sort(out[:k])

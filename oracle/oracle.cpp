@@ -333,3 +333,72 @@ extern "C" int oracle_broad_phase(int mode, const float* lower, const float* upp
   }
   return found;
 }
+
+// ---------------------------------------------------------------- contact reduction primitives (oracle_reduce.h)
+extern "C" void oracle_red_get_slot(const float* n, int count, int* out) {
+  for (int i = 0; i < count; ++i) out[i] = red_get_slot(V3<float>{n[3 * i], n[3 * i + 1], n[3 * i + 2]});
+}
+
+extern "C" void oracle_red_project(const int* bin, const float* p, int count, float* out) {
+  for (int i = 0; i < count; ++i) {
+    V2<float> r = red_project(bin[i], V3<float>{p[3 * i], p[3 * i + 1], p[3 * i + 2]});
+    out[2 * i] = r.x;
+    out[2 * i + 1] = r.y;
+  }
+}
+
+extern "C" void oracle_red_direction(float* out) {
+  for (int i = 0; i < RED_DIRS; ++i) {
+    V2<float> d = red_direction(i);
+    out[2 * i] = d.x;
+    out[2 * i + 1] = d.y;
+  }
+}
+
+extern "C" void oracle_red_oct(const float* n, int count, float* enc, float* dec) {
+  for (int i = 0; i < count; ++i) {
+    V2<float> e = red_encode_oct(V3<float>{n[3 * i], n[3 * i + 1], n[3 * i + 2]});
+    V3<float> d = red_decode_oct(e);
+    enc[2 * i] = e.x;
+    enc[2 * i + 1] = e.y;
+    dec[3 * i] = d.x;
+    dec[3 * i + 1] = d.y;
+    dec[3 * i + 2] = d.z;
+  }
+}
+
+extern "C" void oracle_red_value(const float* score, const int* fingerprint, const unsigned* id, int count, int det,
+                                 unsigned long long* out) {
+  for (int i = 0; i < count; ++i) out[i] = red_value(score[i], fingerprint[i], id[i], det != 0);
+}
+
+extern "C" void oracle_red_voxel_index(const float* p, int count, const float lo[3], const float hi[3],
+                                       const int res[3], int* out) {
+  const V3<float> l{lo[0], lo[1], lo[2]}, h{hi[0], hi[1], hi[2]};
+  for (int i = 0; i < count; ++i) out[i] = red_voxel_index(V3<float>{p[3 * i], p[3 * i + 1], p[3 * i + 2]}, l, h, res);
+}
+
+extern "C" void oracle_red_voxel_resolution(const double lo[3], const double hi[3], int budget, int out[3]) {
+  red_voxel_resolution(lo, hi, budget, out);
+}
+
+// The reduction over given candidates (position, normal used for the bin = normal, projected point = position,
+// use_beta, voxel, depth, sub key); writes the winners' candidate indices, returns their number.
+extern "C" int oracle_red_reduce(const float* proj_point, const float* bin_normal, const float* depth,
+                                 const int* sub, const int* use_beta, const int* voxel, int count, int det,
+                                 int* winners) {
+  std::vector<RedCand<float>> c(count);
+  for (int i = 0; i < count; ++i) {
+    c[i].pos = V3<float>{proj_point[3 * i], proj_point[3 * i + 1], proj_point[3 * i + 2]};
+    c[i].proj_point = c[i].pos;
+    c[i].bin_normal = V3<float>{bin_normal[3 * i], bin_normal[3 * i + 1], bin_normal[3 * i + 2]};
+    c[i].normal = c[i].bin_normal;
+    c[i].depth = depth[i];
+    c[i].sub = sub[i];
+    c[i].use_beta = use_beta[i] != 0;
+    c[i].voxel = voxel[i];
+  }
+  std::vector<int> w = red_reduce(c, det != 0);
+  for (size_t k = 0; k < w.size(); ++k) winners[k] = w[k];
+  return static_cast<int>(w.size());
+}

@@ -76,7 +76,10 @@ typedef struct OracleModel {
   const float* mesh_sdf_voxel;
   const int* mesh_sdf_cells; /* M x 3 */
   const int* mesh_sdf_flags; /* M: bit 0 = scale baked into the grid */
-  int mesh_contact_max;
+  /* mesh contact reduction (reduce_contacts): 0 off (every contact), 1 fast packing, 2 deterministic packing;
+   * per-shape voxel grid of the reduction (S x 3), builder.py:9936-10065 */
+  int mesh_reduce;
+  const int* shape_voxel_res;
   int broad_phase_mode;       /* 0 sap, 1 nxn (and explicit with the model's pairs), 2 explicit list */
   const int* explicit_pairs;  /* sorted (min, max) */
   int explicit_pair_count;

@@ -9,15 +9,16 @@
 //   mesh-plane contact per vertex             newton/_src/geometry/narrow_phase.py:1121-1190
 //   SDF cook (distance to the nearest triangle, winding-number sign)
 //                                             newton/_src/geometry/sdf_texture.py:959-1370 (dense grid)
-// The reference keeps a mesh pair's contacts through a normal-bin reduction in a hash table
-// (contact_reduction_global.py:671-1580); this restatement keeps the K deepest (distance, then sub key)
-// in sub-key order, the rule the product implements.
+//   reduce_contacts=False: every candidate the writer keeps   narrow_phase.py:1090-1190, sdf_contact.py:830-1065
+//   reduce_contacts=True: the normal-bin / voxel reduction    oracle_reduce.h (contact_reduction_global.py)
+// A pair's contacts are written in sub-key order.
 #pragma once
 #include <algorithm>
 #include <cmath>
 #include <vector>
 
 #include "oracle_collide.h"
+#include "oracle_reduce.h"
 
 namespace oracle {
 
@@ -243,12 +244,20 @@ template <class T> struct MeshCandO {
   int sub;
 };
 
-// One mesh pair (sa = mesh, sb = mesh or infinite plane): candidates -> K deepest -> sub-key order.
+// One mesh pair (sa = mesh, sb = mesh or infinite plane).  Candidates are the contacts the reference's mesh
+// kernels produce (vertex below gap + margins; edge whose search distance is below gap + margins).  Unreduced
+// (mesh_reduce 0): the candidates the writer keeps (collide.py:62-148).  Reduced (1 fast, 2 deterministic
+// packing): the reduction's winners, exported with the octahedral round trip of their normal, that the writer
+// keeps (contact_reduction_global.py:1327-1440).  Mesh-plane candidates compete as reduce_contact_in_hashtable
+// places them (bin of the stored normal, world point, beta scaled by the mesh's AABB diagonal, voxel of the
+// mesh); mesh-mesh candidates as export_and_reduce_contact_centered does (bin of the normal, point relative to
+// the midpoint of the two shape origins, beta 1e-4 m, voxel of the shape whose edge made the contact).
 template <class T>
 void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int sa, int sb, Writer<T>& w) {
   const ShapeWorld<T>& A = sw[sa];
   const ShapeWorld<T>& B = sw[sb];
   const T gap_sum = T(m->shape_gap[sa]) + T(m->shape_gap[sb]);
+  const int reduce = m->mesh_reduce;
   ContactData<T> c;
   c.r_a = T(0);
   c.r_b = T(0);
@@ -258,12 +267,17 @@ void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int s
   c.shape_b = sb;
   c.gap_sum = gap_sum;
   c.sub_key = 0;
-  std::vector<MeshCandO<T>> cand;
-  auto keep = [&](V3<T> center, V3<T> normal, T dist, int sub) {
+  std::vector<MeshCandO<T>> kept;   // unreduced
+  std::vector<RedCand<T>> cand;     // reduced
+  auto aabb = [&](int s, V3<T>& lo, V3<T>& hi) {
+    lo = V3<T>{T(m->shape_aabb_lower[3 * s]), T(m->shape_aabb_lower[3 * s + 1]), T(m->shape_aabb_lower[3 * s + 2])};
+    hi = V3<T>{T(m->shape_aabb_upper[3 * s]), T(m->shape_aabb_upper[3 * s + 1]), T(m->shape_aabb_upper[3 * s + 2])};
+  };
+  auto keep_plain = [&](V3<T> center, V3<T> normal, T dist, int sub) {
     c.center = center;
     c.normal = normal;
     c.dist = dist;
-    if (passes_gap(c)) cand.push_back(MeshCandO<T>{center, normal, dist, sub});
+    if (passes_gap(c)) kept.push_back(MeshCandO<T>{center, normal, dist, sub});
   };
   if (m->shape_type[sb] == PLANE) {
     const int mesh = m->shape_mesh[sa];
@@ -271,6 +285,10 @@ void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int s
     const Xf<T> Xp_inv = xf_inverse(B.X);
     const V3<T> pn = xf_vector(B.X, V3<T>{T(0), T(0), T(1)});
     const T total_margin = A.margin + B.margin;
+    V3<T> lo, hi;
+    aabb(sa, lo, hi);
+    const T beta = T(RED_BETA) * length(hi - lo);
+    const Xf<T> X_inv = xf_inverse(A.X);
     for (int v = 0; v < nv; ++v) {
       const float* pv = m->mesh_vertex + 3 * (v0 + v);
       V3<T> vl{T(pv[0]) * A.gscale.x, T(pv[1]) * A.gscale.y, T(pv[2]) * A.gscale.z};
@@ -278,13 +296,29 @@ void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int s
       V3<T> vp = xf_point(Xp_inv, vw);
       V3<T> pp = xf_point(B.X, V3<T>{vp.x, vp.y, T(0)});
       T dist = dot(vw - pp, pn);
-      if (dist < gap_sum + total_margin) keep((vw + pp) * T(0.5), -pn, dist, v);
+      if (!(dist < gap_sum + total_margin)) continue;
+      const V3<T> center = (vw + pp) * T(0.5);
+      if (!reduce) {
+        keep_plain(center, -pn, dist, v);
+        continue;
+      }
+      RedCand<T> k;
+      k.pos = center;
+      k.normal = -pn;
+      k.depth = dist;
+      k.sub = v;
+      k.bin_normal = red_decode_oct(red_encode_oct(k.normal));
+      k.proj_point = center;
+      k.use_beta = dist < beta;
+      k.voxel = red_voxel_index(xf_point(X_inv, center), lo, hi, m->shape_voxel_res + 3 * sa);
+      cand.push_back(k);
     }
   } else {
     for (int mode = 0; mode < 2; ++mode) {
       const ShapeWorld<T>& Tr = mode == 0 ? A : B;
       const ShapeWorld<T>& S = mode == 0 ? B : A;
-      const int tri_mesh = m->shape_mesh[mode == 0 ? sa : sb];
+      const int tri_shape = mode == 0 ? sa : sb;
+      const int tri_mesh = m->shape_mesh[tri_shape];
       const int sdf_mesh = m->shape_mesh[mode == 0 ? sb : sa];
       const SdfG<T> G = sdf_grid<T>(m, sdf_mesh);
       // a grid cooked from scaled vertices is sampled with unit scale (sdf_contact.py:918-930)
@@ -297,6 +331,11 @@ void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int s
       const T threshold_u = threshold / min_s;
       const int e0 = m->mesh_edge_start[tri_mesh], ne = m->mesh_edge_start[tri_mesh + 1] - e0;
       const int vb = m->mesh_vertex_start[tri_mesh];
+      // reduction frame of this side (sdf_contact.py:1286-1324)
+      const V3<T> midpoint = (Tr.X.p + S.X.p) * T(0.5);
+      const Xf<T> X_ws_tri = xf_inverse(Tr.X);
+      V3<T> tlo, thi;
+      aabb(tri_shape, tlo, thi);
       for (int e = 0; e < ne; ++e) {
         const int ia = m->mesh_edge[2 * (e0 + e)], ib = m->mesh_edge[2 * (e0 + e) + 1];
         const float* qa = m->mesh_vertex + 3 * (vb + ia);
@@ -333,24 +372,38 @@ void mesh_pair(const OracleModel* m, const std::vector<ShapeWorld<T>>& sw, int s
           T fl = length(fb);
           dw = fl > T(0) ? fb / fl : V3<T>{T(0), T(1), T(0)};
         }
-        keep(pw, mode == 0 ? -dw : dw, dist, (e << 2) | (mode << 1));
+        const V3<T> normal = mode == 0 ? -dw : dw;
+        const int sub = (e << 2) | (mode << 1);
+        if (!reduce) {
+          keep_plain(pw, normal, dist, sub);
+          continue;
+        }
+        RedCand<T> k;
+        k.pos = pw;
+        k.normal = normal;
+        k.depth = dist;
+        k.sub = sub;
+        k.bin_normal = normal;
+        k.proj_point = pw - midpoint;
+        k.use_beta = dist < T(RED_BETA);
+        k.voxel = red_voxel_index(xf_point(X_ws_tri, pw), tlo, thi, m->shape_voxel_res + 3 * tri_shape);
+        cand.push_back(k);
       }
     }
   }
-  const int K = m->mesh_contact_max;
-  std::stable_sort(cand.begin(), cand.end(), [](const MeshCandO<T>& x, const MeshCandO<T>& y) {
-    if (x.dist < y.dist) return true;
-    if (y.dist < x.dist) return false;
-    return x.sub < y.sub;
-  });
-  if ((int)cand.size() > K) cand.resize(K);
-  std::sort(cand.begin(), cand.end(), [](const MeshCandO<T>& x, const MeshCandO<T>& y) { return x.sub < y.sub; });
-  for (const MeshCandO<T>& k : cand) {
+  if (reduce) {
+    for (int q : red_reduce(cand, reduce == 2)) {
+      const RedCand<T>& k = cand[q];
+      kept.push_back(MeshCandO<T>{k.pos, red_decode_oct(red_encode_oct(k.normal)), k.depth, k.sub});
+    }
+  }
+  std::sort(kept.begin(), kept.end(), [](const MeshCandO<T>& x, const MeshCandO<T>& y) { return x.sub < y.sub; });
+  for (const MeshCandO<T>& k : kept) {
     c.center = k.center;
     c.normal = k.normal;
     c.dist = k.dist;
     c.sub_key = k.sub;
-    write_contact(c, w, false);
+    write_contact(c, w, reduce != 0);
   }
 }
 

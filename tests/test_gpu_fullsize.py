@@ -104,17 +104,21 @@ def test_fullsize_mixed_parity_and_layout(cuda_device):
     assert exact
 
 
-def test_fullsize_sdf_parity(cuda_device):
+@pytest.mark.parametrize("reduce", [True, False], ids=["reduced", "unreduced"])
+def test_fullsize_sdf_parity(cuda_device, reduce):
     """The SDF workload at the benchmark's size: 1024 worlds x 8 convex-hull meshes (64-hull library, 64^3
-    SDFs), advanced 240 substeps (60 frames) on the GPU until the hulls pile up.  At that state the GPU's
-    mesh-mesh and mesh-plane contacts equal the oracle's (count, keys, pairs exactly; float fields within 1e-5
-    relative, bit-exactness reported), and one step from that state agrees within 1e-5 relative."""
-    print("[full size sdf] 64-hull library, 64^3 SDFs, 1024 worlds", flush=True)
+    SDFs), advanced 240 substeps (60 frames) on the GPU until the hulls pile up, with the reference's contact
+    reduction (deterministic packing) and without it.  At that state the GPU's mesh-mesh and mesh-plane
+    contacts equal the oracle's (count, keys, pairs exactly; float fields within 1e-5 relative, bit-exactness
+    reported), and one step from that state agrees within 1e-5 relative."""
+    print(f"[full size sdf] 64-hull library, 64^3 SDFs, 1024 worlds, reduce_contacts={reduce}", flush=True)
     lib = scenes.mesh_library(64, seed=0)
     for mm in lib:
         mm.build_sdf(max_resolution=64, device=cuda_device)
     model = scenes.mesh_worlds(1024, seed=0, device=cuda_device, library=lib)
-    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    # unreduced, pairs keep every contact (about 7 000 a world once piled): a capacity beyond the estimate
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True, reduce_contacts=reduce,
+                             rigid_contact_max=None if reduce else 12_000_000)
     solver = SolverXPBD(model, iterations=10)
     c = pipe.contacts()
     a, b = model.state(), model.state()
@@ -132,7 +136,7 @@ def test_fullsize_sdf_parity(cuda_device):
     pipe.collide(a, c)
     torch.cuda.synchronize()
     got = c.numpy()
-    oracle = Oracle(model)
+    oracle = Oracle(model, reduce_contacts=reduce)
     ref = oracle.collide(q, capacity=c.rigid_contact_max)
     report = compare_contacts(got, ref)
     print("contacts", report, flush=True)

@@ -12,7 +12,9 @@ Pins (none of them retypes the restatement's formula):
   * trilinear sampling and gradient reproduce a linear field exactly, extrapolate by the distance to the
     grid box outside it, and match finite differences inside a cell;
   * the Brent edge search against a brute-force minimum along the segment;
-  * mesh-plane contacts of a box resting on the ground (closed form) and the keep-the-K-deepest rule.
+  * mesh-plane contacts of a box resting on the ground (closed form, reduced and unreduced); reduced contacts
+    are a subset of the unreduced ones that keeps each pair's deepest contact (the reduction's own pins are in
+    test_oracle_reduction.py).
 """
 
 from __future__ import annotations
@@ -252,7 +254,7 @@ def test_edge_search_vs_brute_force():
 
 
 # ---------------------------------------------------------------- mesh contacts
-def _box_on_plane(z, gap=0.1, K=64):
+def _box_on_plane(z, gap=0.1, reduce_contacts=True):
     mesh = cook_cpu(box_mesh(0.1, 0.1, 0.1), 8)
     b = ModelBuilder()
     cfg = ModelBuilder.ShapeConfig()
@@ -261,21 +263,24 @@ def _box_on_plane(z, gap=0.1, K=64):
     body = b.add_body(xform=(0.0, 0.0, z, 0.0, 0.0, 0.0, 1.0))
     b.add_shape_mesh(body, mesh=mesh, cfg=cfg)
     model = b.finalize("cpu")
-    return model, oracle.Oracle(model, mesh_contact_max=K)
+    return model, oracle.Oracle(model, reduce_contacts=reduce_contacts)
 
 
 def test_mesh_plane_contacts_closed_form():
-    model, o = _box_on_plane(0.12)
-    c = o.collide(model.host["body_q"])
-    # the 4 bottom vertices (0..3) at height 0.02; the top ones at 0.22 exceed gap_sum = 0.2
-    assert c["count"] == 4
-    np.testing.assert_array_equal(c["shape0"], [1] * 4)  # (mesh, plane)
-    np.testing.assert_array_equal(c["shape1"], [0] * 4)
-    np.testing.assert_array_equal(c["sort_key"] & 0x7FFFFF, [0, 1, 2, 3])
-    np.testing.assert_allclose(c["normal"], np.tile([0, 0, -1], (4, 1)), atol=0)
-    # point0 = vertex (mesh frame), point1 = its projection on the plane (world frame)
-    np.testing.assert_allclose(c["point0"], model.host["mesh_vertex"][:4], atol=1e-6)
-    np.testing.assert_allclose(c["point1"][:, 2], 0.0, atol=1e-6)
+    for reduce in (False, True):
+        model, o = _box_on_plane(0.12, reduce_contacts=reduce)
+        c = o.collide(model.host["body_q"])
+        # the 4 bottom vertices (0..3) at height 0.02; the top ones at 0.22 exceed gap_sum = 0.2.  Reduced, each
+        # bottom corner is alone in its voxel (a 0.2 m cube is split 4 x 5 x 5), so all four stay; the octahedral
+        # round trip of (0, 0, -1) is exact
+        assert c["count"] == 4
+        np.testing.assert_array_equal(c["shape0"], [1] * 4)  # (mesh, plane)
+        np.testing.assert_array_equal(c["shape1"], [0] * 4)
+        np.testing.assert_array_equal(c["sort_key"] & 0x7FFFFF, [0, 1, 2, 3])
+        np.testing.assert_allclose(c["normal"], np.tile([0, 0, -1], (4, 1)), atol=0)
+        # point0 = vertex (mesh frame), point1 = its projection on the plane (world frame)
+        np.testing.assert_allclose(c["point0"], model.host["mesh_vertex"][:4], atol=1e-6)
+        np.testing.assert_allclose(c["point1"][:, 2], 0.0, atol=1e-6)
 
 
 def _contact_depth(model, q, c):
@@ -297,9 +302,11 @@ def _contact_depth(model, q, c):
     return np.einsum("ij,ij->i", p1 - p0, c["normal"]) - c["margin0"] - c["margin1"]
 
 
-def test_mesh_contacts_keep_k_deepest():
+def test_reduced_mesh_contacts_subset_of_unreduced():
+    """Reduced contacts are a subset (by sort key) of the unreduced ones, in sub-key order, at most 240 per pair;
+    each pair keeps its deepest contact (the max-depth slot of that contact's normal bin)."""
     rng = np.random.Generator(np.random.PCG64(5))
-    lib = [cook_cpu(random_convex_hull(rng, 24, 40), 12) for _ in range(2)]
+    lib = [cook_cpu(random_convex_hull(rng, 60, 90), 12) for _ in range(2)]
     b = ModelBuilder()
     b.add_ground_plane()
     for k, z in enumerate((0.12, 0.3)):
@@ -307,22 +314,20 @@ def test_mesh_contacts_keep_k_deepest():
         b.add_shape_mesh(body, mesh=lib[k])
     model = b.finalize("cpu")
     q = model.host["body_q"].astype(np.float64)
-    full = oracle.Oracle(model, mesh_contact_max=255).collide(model.host["body_q"])
-    cut = oracle.Oracle(model, mesh_contact_max=5).collide(model.host["body_q"])
+    full = oracle.Oracle(model, reduce_contacts=False).collide(model.host["body_q"])
     pairs = sorted({(int(a), int(b_)) for a, b_ in zip(full["shape0"], full["shape1"])})
     assert (1, 0) in pairs and (1, 2) in pairs  # a mesh-plane pair and the mesh-mesh pair
     d_full = _contact_depth(model, q, full)
-    d_cut = _contact_depth(model, q, cut)
-    for a, b_ in pairs:
-        sel = (full["shape0"] == a) & (full["shape1"] == b_)
-        sel_c = (cut["shape0"] == a) & (cut["shape1"] == b_)
-        assert sel_c.sum() == min(5, sel.sum())
-        kept = set(cut["sort_key"][sel_c].tolist())
-        assert kept.issubset(set(full["sort_key"][sel].tolist()))
-        assert np.all(np.diff(cut["sort_key"][sel_c]) > 0)  # sub-key order
-        dropped = [d for k, d in zip(full["sort_key"][sel], d_full[sel]) if k not in kept]
-        if dropped:
-            assert d_cut[sel_c].max() <= min(dropped) + 1e-5  # the kept ones are the deepest
+    for det in (True, False):
+        red = oracle.Oracle(model, reduce_contacts=True, deterministic=det).collide(model.host["body_q"])
+        for a, b_ in pairs:
+            sel = (full["shape0"] == a) & (full["shape1"] == b_)
+            sel_r = (red["shape0"] == a) & (red["shape1"] == b_)
+            assert 0 < sel_r.sum() <= min(240, sel.sum())
+            assert set(red["sort_key"][sel_r].tolist()).issubset(set(full["sort_key"][sel].tolist()))
+            assert np.all(np.diff(red["sort_key"][sel_r]) > 0)  # sub-key order
+            deepest = full["sort_key"][sel][np.argmin(d_full[sel])]
+            assert deepest in set(red["sort_key"][sel_r].tolist())
     # mesh-mesh sub keys carry (edge << 2) | (mode << 1)
     mm = full["shape1"] != 0
     assert mm.any()
