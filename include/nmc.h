@@ -144,8 +144,15 @@ typedef struct NmcContacts {
   float* margin0;
   float* margin1;
   int64_t* sort_key; /* optional (NULL): (shape0<<43 | shape1<<23 | sub_key) */
-  int* world_start;  /* segment_count: first contact of each segment */
-  int* world_count;  /* segment_count: contacts of each segment stored in the buffer */
+  int* world_start;  /* segment_count: first canonical position of each segment's contacts */
+  int* world_count;  /* segment_count: contacts of each segment (canonical positions below the capacity) */
+  /* capacity: buffer position of the contact at each canonical position (-1: past the capacity, not stored).
+   * Required with sort_key: the buffer is then in the reference's deterministic layout, the whole buffer
+   * ascending by sort key (collide.py:1105-1124, contact_sort.py:194-445).  Null without sort keys: the buffer
+   * is in the canonical order itself.  The canonical order is segment by segment (world order, the dedicated
+   * world -1 segment last), by sort key inside a segment; a body's contacts are in the same relative order
+   * in both. */
+  int* order;
 } NmcContacts;
 
 typedef struct NmcXpbdParams {
@@ -173,8 +180,8 @@ typedef struct NmcXpbdParams {
  *                   q_mid/qd_mid with the full iteration count.  `lambda_sum` = applied normal multipliers.
  * Per-contact arrays are indexed like the Contacts buffer; `feat` holds 16 floats per contact (see
  * newton_custom_contact_amd/learned.py FEATURES).  prev_* (optional): the previous substep's sort keys,
- * world_start / world_count and lambda_sum, used to fill the "previous multiplier" feature (requires
- * contacts->sort_key). */
+ * world_start / world_count, lambda_sum and order array, used to fill the "previous multiplier" feature
+ * (requires contacts->sort_key). */
 enum { NMC_STEP_PLAIN = 0, NMC_STEP_LOG = 1, NMC_STEP_PREP = 2, NMC_STEP_WARM = 3 };
 typedef struct NmcLearnIO {
   int mode;
@@ -191,6 +198,7 @@ typedef struct NmcLearnIO {
   const int* prev_start;
   const int* prev_count;
   const float* prev_lambda;
+  const int* prev_order;   /* the previous Contacts' order array (null: canonical layout) */
 } NmcLearnIO;
 
 typedef struct NmcModel NmcModel;
@@ -199,8 +207,9 @@ typedef struct NmcModel NmcModel;
 int nmc_model_create(const NmcModelDesc* desc, NmcModel** out);
 void nmc_model_destroy(NmcModel* model);
 
-/* Shape AABBs -> SAP broad phase -> primitive / MPR-GJK narrow phase -> contacts written in
- * canonical order: segment-major, and inside a segment ascending by the 64-bit sort key. */
+/* Shape AABBs -> SAP broad phase -> primitive / MPR-GJK / mesh narrow phase -> contacts.  With sort keys the
+ * buffer is in global sort-key order (contacts->order maps canonical positions to it); without, in canonical
+ * order (segment-major, ascending by the 64-bit sort key inside a segment). */
 int nmc_collide(NmcModel* model, const float* body_q, const NmcContacts* contacts, nmc_stream_t stream);
 
 /* One XPBD step: integrate, `iterations` x (contact solve + apply, joint solve + apply).  Fails when
@@ -304,9 +313,10 @@ int nmc_contact_force(const float* contact_impulse, const int* count, int capaci
  * the report when the matcher has one, and saves this frame for the next. */
 typedef struct NmcMatcher NmcMatcher;
 typedef struct NmcContactReport {
-  int* new_indices;    /* capacity: contacts of this frame with match_index < 0, in buffer order */
+  int* new_indices;    /* capacity: buffer indices of this frame's contacts with match_index < 0 (canonical order;
+                        * the reference appends through an atomic counter, in no fixed order) */
   int* new_count;      /* [1] */
-  int* broken_indices; /* capacity: previous-frame contacts no new contact matched, in that buffer's order */
+  int* broken_indices; /* capacity: previous-frame buffer indices no new contact matched (its canonical order) */
   int* broken_count;   /* [1] */
 } NmcContactReport;
 int nmc_matcher_create(const NmcModel* model, int capacity, float pos_threshold, float normal_dot_threshold,

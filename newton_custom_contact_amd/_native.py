@@ -54,7 +54,7 @@ class NmcContacts(ctypes.Structure):
     _fields_ = [
         ("capacity", ctypes.c_int), ("count", _vp), ("shape0", _vp), ("shape1", _vp), ("point0", _vp),
         ("point1", _vp), ("offset0", _vp), ("offset1", _vp), ("normal", _vp), ("margin0", _vp),
-        ("margin1", _vp), ("sort_key", _vp), ("world_start", _vp), ("world_count", _vp),
+        ("margin1", _vp), ("sort_key", _vp), ("world_start", _vp), ("world_count", _vp), ("order", _vp),
     ]
 
 
@@ -73,7 +73,7 @@ class NmcLearnIO(ctypes.Structure):
         ("mode", ctypes.c_int), ("warm_iterations", ctypes.c_int), ("residual_tol", ctypes.c_float),
         ("feat", _vp), ("lambda_sum", _vp), ("lambda0", _vp), ("residual", _vp), ("fallback", _vp),
         ("q_mid", _vp), ("qd_mid", _vp), ("prev_key", _vp), ("prev_start", _vp), ("prev_count", _vp),
-        ("prev_lambda", _vp),
+        ("prev_lambda", _vp), ("prev_order", _vp),
     ]
 
 

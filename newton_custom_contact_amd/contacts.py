@@ -1,9 +1,11 @@
 """``Contacts``: rigid contact buffers (newton/_src/sim/contacts.py:43-358, rigid part).
 
-Field names, dtypes and shapes follow the reference.  Contacts are stored in canonical order:
-segment-major (world 0, world 1, ..., then the global-only segment) and ascending by the 64-bit sort key
-``shape0 << 43 | shape1 << 23 | sub_key`` inside each segment.  Two extra per-segment arrays,
-``world_contact_start`` / ``world_contact_count``, index that layout for the per-world solver.
+Field names, dtypes and shapes follow the reference.  With sort keys (a deterministic pipeline) the buffer is
+in the reference's deterministic layout: ascending by the 64-bit sort key ``shape0 << 43 | shape1 << 23 |
+sub_key`` over the whole buffer (collide.py:1105-1124).  Without sort keys it is in the canonical order:
+segment-major (world 0, world 1, ..., then the global-only segment), key order inside each segment.  Two extra
+per-segment arrays, ``world_contact_start`` / ``world_contact_count``, index the canonical order for the
+per-world solver, and ``rigid_contact_order`` maps canonical positions to buffer indices (with sort keys).
 ``rigid_contact_tids`` is zero, as the reference's writer always stores (collide.py:134).
 """
 
@@ -66,6 +68,11 @@ class Contacts:
             self.rigid_contact_broken_count = None
         self.world_contact_start = torch.zeros((segment_count,), **i)
         self.world_contact_count = torch.zeros((segment_count,), **i)
+        # with sort keys the buffer is in the reference's deterministic layout, ascending by sort key over the
+        # whole buffer (collide.py:1105-1124); rigid_contact_order[j] is the buffer index of the j-th contact in
+        # the canonical order (world by world, key order inside a world; -1 when past the capacity), by which
+        # the solver walks each world's contacts
+        self.rigid_contact_order = torch.full((C,), -1, **i) if deterministic else None
         attrs = requested_attributes or set()
         self.force = torch.zeros((C, 6), **f) if "force" in attrs else None
         self._view = None
@@ -82,6 +89,7 @@ class Contacts:
             v.margin0, v.margin1 = ptr(self.rigid_contact_margin0), ptr(self.rigid_contact_margin1)
             v.sort_key = ptr(self.rigid_contact_sort_key)
             v.world_start, v.world_count = ptr(self.world_contact_start), ptr(self.world_contact_count)
+            v.order = ptr(self.rigid_contact_order)
             self._view = v
         return self._view
 
@@ -110,6 +118,7 @@ class Contacts:
         }
         if self.rigid_contact_sort_key is not None:
             out["sort_key"] = self.rigid_contact_sort_key[:n].cpu().numpy()
+            out["order"] = self.rigid_contact_order[:n].cpu().numpy()
         if self.rigid_contact_match_index is not None:
             out["match_index"] = self.rigid_contact_match_index[:n].cpu().numpy()
         if self.rigid_contact_new_indices is not None:

@@ -391,6 +391,114 @@ __global__ void __launch_bounds__(64) contact_count_kernel(StageBuffers st, cons
   if (threadIdx.x == 0) seg_contact_count[seg] = local;
 }
 
+// Global sort-key layout, counting pass (one wave per segment, pairs in key order): per-segment counts for
+// the scan below.  The segment's contacts with a world shape0 form one run per shape0, all inside this
+// segment; for each run's shape0 it records minus the number of world -1 shape0 contacts before the run
+// (ord_delta0).  For each world -1 shape0 it records the segment's contacts and the first one's segment-local
+// position (ord_M, ord_gstart; LDS accumulation), and in the last (world -1 only) segment each pair's count by
+// (shape0, shape1) (ord_H).
+struct OrderArgs {
+  int G, U, W;
+  const int* gid;
+  const int* unit_order;
+  int* delta0;
+  int* M;
+  int* gstart;
+  int* L;
+  int* H;
+  int* unit_off;
+  int* base;
+};
+
+__global__ void __launch_bounds__(64) order_count_kernel(StageBuffers st, const long long* __restrict__ seg_pair_off,
+                                                         const int* __restrict__ seg_pair_count, OrderArgs o) {
+  extern __shared__ int s_gm[];  // [G] contacts, [G] first position
+  const int seg = blockIdx.x;
+  const int np = seg_pair_count[seg];
+  const long long poff = seg_pair_off[seg];
+  const int G = o.G;
+  const bool last = seg == o.W;
+  for (int k = threadIdx.x; k < G; k += 64) {
+    s_gm[k] = 0;
+    s_gm[G + k] = 0x7fffffff;
+  }
+  if (last)
+    for (int k = threadIdx.x; k < G * G; k += 64) o.H[k] = 0;
+  __syncthreads();
+  const int lane = threadIdx.x;
+  int carry = 0, carry_g = 0;
+  for (int b0 = 0; b0 < np; b0 += 64) {
+    const int p = b0 + lane;
+    const bool valid = p < np;
+    const int cnt = valid ? static_cast<int>(st.mask[poff + p]) : 0;
+    const int a = valid ? st.pair_a[poff + p] : -1;
+    const int gi = valid ? o.gid[a] : -1;
+    const int gc = gi >= 0 ? cnt : 0;
+    int ic = cnt, ig = gc;
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(ic, off, 64), tg = __shfl_up(ig, off, 64);
+      if (lane >= off) {
+        ic += t;
+        ig += tg;
+      }
+    }
+    const int ex = carry + ic - cnt, exg = carry_g + ig - gc;
+    if (valid) {
+      const bool run_start = p == 0 || st.pair_a[poff + p - 1] != a;
+      if (gi < 0) {
+        if (run_start) o.delta0[a] = -exg;
+      } else {
+        if (cnt > 0) {
+          atomicAdd(&s_gm[gi], cnt);  // LDS
+          atomicMin(&s_gm[G + gi], ex);
+        }
+        if (last) {
+          const int gj = o.gid[st.pair_b[poff + p]];
+          if (gj >= 0) o.H[gi * G + gj] = cnt;
+        }
+      }
+    }
+    carry += __shfl(ic, 63, 64);
+    carry_g += __shfl(ig, 63, 64);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < G; k += 64) {
+    o.M[static_cast<long long>(seg) * G + k] = s_gm[k];
+    o.gstart[static_cast<long long>(seg) * G + k] = s_gm[G + k];
+  }
+  if (threadIdx.x == 0) o.L[seg] = carry - carry_g;
+}
+
+// Global sort-key layout, scans (one workgroup): for each world -1 shape0 g, the exclusive prefix of its
+// contacts over the units in shape-index order (world w: segment w's contacts with shape0 g; world -1 shape t:
+// the last segment's (g, t) contacts) -> ord_unit_off, total T[g]; then the shape0 blocks' first positions
+// over the units (world w: its world-shape0 contacts; world -1 shape g: T[g]) -> ord_base.
+__global__ void __launch_bounds__(SCAN_BLOCK) order_scan_kernel(OrderArgs o) {
+  __shared__ int s_T[256];
+  const int G = o.G, U = o.U, W = o.W;
+  for (int g = 0; g < G; ++g) {
+    int tot[1];
+    segment_scan_fn<1>(
+        U,
+        [&](int i, int (&v)[1]) {
+          const int u = o.unit_order[i];
+          v[0] = u < W ? o.M[static_cast<long long>(u) * G + g] : o.H[g * G + (u - W)];
+        },
+        [&](int i, const int (&ex)[1], const int (&)[1]) { o.unit_off[static_cast<long long>(g) * U + o.unit_order[i]] = ex[0]; },
+        tot);
+    if (threadIdx.x == 0) s_T[g] = tot[0];
+    __syncthreads();
+  }
+  int tot[1];
+  segment_scan_fn<1>(
+      U,
+      [&](int i, int (&v)[1]) {
+        const int u = o.unit_order[i];
+        v[0] = u < W ? o.L[u] : s_T[u - W];
+      },
+      [&](int i, const int (&ex)[1], const int (&)[1]) { o.base[o.unit_order[i]] = ex[0]; }, tot);
+}
+
 // Contact offsets: exclusive prefix over segments -> Contacts.world_start / world_count / count.
 __global__ void __launch_bounds__(SCAN_BLOCK) contact_scan_kernel(const int* __restrict__ seg_contact_count,
                                                                  NmcContacts c, int segments,
@@ -415,20 +523,25 @@ __global__ void __launch_bounds__(SCAN_BLOCK) contact_scan_kernel(const int* __r
 
 // Staging -> Contacts at canonical indices, one lane per output contact (coalesced stores).  MESH:
 // the model has mesh shapes, whose pairs (mesh first) keep their records in the mesh slots.
-template <bool MESH>
+// ORDER: the global sort-key layout; each contact goes to its buffer position (below the capacity) and
+// Contacts.order records it.
+template <bool MESH, bool ORDER>
 __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st,
                                                                   const long long* __restrict__ seg_pair_off,
                                                                   const int* __restrict__ seg_pair_count,
                                                                   const int* __restrict__ shape_type,
                                                                   const int* __restrict__ shape_body,
                                                                   const float* __restrict__ body_q,
-                                                                  int mesh_k, NmcContacts c) {
+                                                                  int mesh_k, NmcContacts c, OrderArgs ord,
+                                                                  const int* __restrict__ seg_total) {
   __shared__ int s_scan[CBLOCK];
   __shared__ int s_wave[CBLOCK / 64];
   const int seg = blockIdx.x;
   const int np = seg_pair_count[seg];
   const long long poff = seg_pair_off[seg];
-  const int start = c.world_start[seg], stored = c.world_count[seg];
+  // ORDER: every contact of the segment is placed (those whose buffer position is below the capacity are
+  // stored, so the buffer holds the first contacts of the key order); else the canonical range only
+  const int start = c.world_start[seg], stored = ORDER ? seg_total[seg] : c.world_count[seg];
   int run = 0;  // contacts of the earlier chunks
   for (int b0 = 0; b0 < np && run < stored; b0 += CBLOCK) {
     const int pi = b0 + threadIdx.x;
@@ -447,8 +560,26 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
       }
       const int k = t - (lo > 0 ? s_scan[lo - 1] : 0);
       const long long g = poff + b0 + lo;
-      const int o = start + local_idx;
       const int s0 = st.pair_a[g], s1 = st.pair_b[g];
+      int o = start + local_idx;
+      if (ORDER) {
+        // buffer position: shape0 block base + rank inside it
+        const int gi = ord.gid[s0];
+        long long pos;
+        if (gi < 0) {
+          pos = static_cast<long long>(ord.base[seg]) + ord.delta0[s0] + local_idx;
+        } else if (seg < ord.W) {
+          pos = static_cast<long long>(ord.base[ord.W + gi]) + ord.unit_off[static_cast<long long>(gi) * ord.U + seg] +
+                (local_idx - ord.gstart[static_cast<long long>(seg) * ord.G + gi]);
+        } else {
+          pos = static_cast<long long>(ord.base[ord.W + gi]) +
+                ord.unit_off[static_cast<long long>(gi) * ord.U + ord.W + ord.gid[s1]] + k;
+        }
+        const bool kept = pos < c.capacity;
+        if (o < c.capacity) c.order[o] = kept ? static_cast<int>(pos) : -1;
+        if (!kept) continue;
+        o = static_cast<int>(pos);
+      }
       long long s = 5 * g + k;
       if (MESH && shape_type[s0] == GEO_MESH) s = st.mesh_slot_base + static_cast<long long>(st.mesh_index[g]) * mesh_k + k;
       const float4* r = st.rec + 3 * s;
@@ -522,13 +653,24 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
                      m->seg_pair_count, m->seg_contact_count);
   hipLaunchKernelGGL(contact_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, m->seg_contact_count, *c, segs,
                      m->counters);
-  if (mesh)
-    hipLaunchKernelGGL(contact_compact_kernel<true>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_off,
-                       m->seg_pair_count, d.shape_type, d.shape_body, body_q, m->mesh_k, *c);
-  else
-    hipLaunchKernelGGL(contact_compact_kernel<false>, dim3(segs), dim3(CBLOCK), 0, stream, m->stage,
-                       m->seg_pair_off, m->seg_pair_count, d.shape_type, d.shape_body, body_q, m->mesh_k,
-                       *c);
+  const bool order = c->order != nullptr;
+  OrderArgs oa{};
+  if (order) {
+    oa = OrderArgs{m->ord_G, m->ord_U, d.world_count, m->ord_gid, m->ord_unit_order, m->ord_delta0, m->ord_M,
+                   m->ord_gstart, m->ord_L, m->ord_H, m->ord_unit_off, m->ord_base};
+    hipLaunchKernelGGL(order_count_kernel, dim3(segs), dim3(64), sizeof(int) * 2 * (m->ord_G > 0 ? m->ord_G : 1),
+                       stream, m->stage, m->seg_pair_off, m->seg_pair_count, oa);
+    hipLaunchKernelGGL(order_scan_kernel, dim3(1), dim3(SCAN_BLOCK), 0, stream, oa);
+  }
+  auto compact = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(segs), dim3(CBLOCK), 0, stream, m->stage, m->seg_pair_off, m->seg_pair_count,
+                       d.shape_type, d.shape_body, body_q, m->mesh_k, *c, oa, m->seg_contact_count);
+  };
+  if (mesh) {
+    if (order) compact(contact_compact_kernel<true, true>); else compact(contact_compact_kernel<true, false>);
+  } else {
+    if (order) compact(contact_compact_kernel<false, true>); else compact(contact_compact_kernel<false, false>);
+  }
   return hipGetLastError();
 }
 

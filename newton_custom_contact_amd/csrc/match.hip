@@ -8,16 +8,16 @@
 // the one with the smallest packed claim (float_flip(dist^2) << 32 | low 32 bits of its sort key,
 // `atomic_min`, :160-214) and demotes the others to MATCH_BROKEN (:341-370).
 //
-// Here the Contacts buffer is already in canonical order (segment-major, sort key inside a segment), and a
-// shape pair lives in exactly one segment, so both the previous pair range and the competing new contacts
-// of a pair are contiguous runs inside one segment.  One 64-lane workgroup per segment:
+// Here the contacts are walked in canonical order (segment-major, sort key inside a segment; Contacts.order maps
+// a canonical position to the buffer), and a shape pair lives in exactly one segment, so both the previous
+// pair range and the competing new contacts of a pair are contiguous runs inside one segment.  One 64-lane workgroup per segment:
 //   match_kernel    candidate + its squared distance per contact (binary search over the segment's
 //                   previous keys, linear scan of the pair range)
 //   resolve_kernel  claims settled by scanning the pair's own new contacts (no atomics), sticky replay of
 //                   the winners' previous body-frame points/offsets and normal, report counts per segment
 //   report: one-workgroup scan of the per-segment counts, then the ordered index lists
 //   save_kernel     this frame's keys, midpoints, normals (and sticky columns) for the next frame
-// Match indices refer to the previous frame's Contacts layout (this library's canonical order).
+// Match indices and report entries are buffer positions (the previous frame's for a match, as in the reference).
 #include <hip/hip_runtime.h>
 
 #include "nmc_internal.h"
@@ -44,6 +44,7 @@ struct MatchState {
   float* o1;
   int* seg_start;
   int* seg_count;
+  int* bufpos;       // buffer position of each previous contact in the previous Contacts
   int* count;        // [1]
   // this frame's scratch
   int* cand;
@@ -87,14 +88,15 @@ __global__ void __launch_bounds__(64) match_kernel(MatchState m, NmcContacts c, 
   for (int i = ws + static_cast<int>(threadIdx.x); i < ws + wc; i += 64) {
     int best = MATCH_NOT_FOUND;
     float best_d2 = m.pos_threshold_sq;
-    if (pe > ps) {
-      const long long key = c.sort_key[i];
+    const int bi = cpos(c, i);
+    if (pe > ps && bi >= 0) {
+      const long long key = c.sort_key[bi];
       const long long prefix = key & ~SUB_KEY_MASK;
       const int lo = lower_bound64(m.keys, ps, pe, prefix);
       const int hi = lower_bound64(m.keys, lo, pe, prefix + SUB_KEY_MASK + 1);
       if (lo < hi) {
-        const f3 pw = midpoint(c, body_q, shape_body, i);
-        const f3 nn = load3(c.normal + 3 * i);
+        const f3 pw = midpoint(c, body_q, shape_body, bi);
+        const f3 nn = load3(c.normal + 3 * bi);
         best = MATCH_BROKEN;
         int found = -1;
         for (int j = lo; j < hi; ++j) {
@@ -133,32 +135,39 @@ __global__ void __launch_bounds__(64) resolve_kernel(MatchState m, NmcContacts c
   if (threadIdx.x == 0) s_new = 0;
   __syncthreads();
   int my_new = 0;
+  // the sort key at canonical position j, or -1 when that contact is not stored (no pair prefix matches it)
+  auto ckey = [&](int j) -> long long {
+    const int b = cpos(c, j);
+    return b >= 0 ? c.sort_key[b] : -1ll;
+  };
   for (int i = ws + static_cast<int>(threadIdx.x); i < ws + wc; i += 64) {
+    const int bi = cpos(c, i);
+    if (bi < 0) continue;
     int r = m.cand[i];
     if (r >= 0) {
       // competitors: the pair's other new contacts (a contiguous run around i) with the same candidate
-      const long long key = c.sort_key[i];
+      const long long key = c.sort_key[bi];
       const long long prefix = key & ~SUB_KEY_MASK;
       const float d2 = m.dist_sq[i];
       bool win = true;
-      for (int j = i - 1; j >= ws && (c.sort_key[j] & ~SUB_KEY_MASK) == prefix && win; --j)
-        if (m.cand[j] == r && claim_less(m.dist_sq[j], c.sort_key[j], d2, key)) win = false;
-      for (int j = i + 1; j < ws + wc && (c.sort_key[j] & ~SUB_KEY_MASK) == prefix && win; ++j)
-        if (m.cand[j] == r && claim_less(m.dist_sq[j], c.sort_key[j], d2, key)) win = false;
+      for (int j = i - 1; j >= ws && (ckey(j) & ~SUB_KEY_MASK) == prefix && win; --j)
+        if (m.cand[j] == r && claim_less(m.dist_sq[j], ckey(j), d2, key)) win = false;
+      for (int j = i + 1; j < ws + wc && (ckey(j) & ~SUB_KEY_MASK) == prefix && win; ++j)
+        if (m.cand[j] == r && claim_less(m.dist_sq[j], ckey(j), d2, key)) win = false;
       if (!win) {
         r = MATCH_BROKEN;
       } else {
         if (m.report) m.prev_matched[r] = 1;
         if (m.sticky) {
-          store3(c.point0 + 3 * i, load3(m.p0 + 3 * r));
-          store3(c.point1 + 3 * i, load3(m.p1 + 3 * r));
-          store3(c.offset0 + 3 * i, load3(m.o0 + 3 * r));
-          store3(c.offset1 + 3 * i, load3(m.o1 + 3 * r));
-          store3(c.normal + 3 * i, load3(m.normal + 3 * r));
+          store3(c.point0 + 3 * bi, load3(m.p0 + 3 * r));
+          store3(c.point1 + 3 * bi, load3(m.p1 + 3 * r));
+          store3(c.offset0 + 3 * bi, load3(m.o0 + 3 * r));
+          store3(c.offset1 + 3 * bi, load3(m.o1 + 3 * r));
+          store3(c.normal + 3 * bi, load3(m.normal + 3 * r));
         }
       }
     }
-    match_index[i] = r;
+    match_index[bi] = r >= 0 ? m.bufpos[r] : r;
     my_new += r < 0 ? 1 : 0;
   }
   if (!m.report) return;
@@ -186,8 +195,9 @@ __global__ void __launch_bounds__(SCAN_BLOCK) report_scan_kernel(MatchState m, i
   }
 }
 
-// Ordered index lists: new contacts (match index < 0) in this frame's order, broken previous contacts
-// (not won by any new contact) in the previous frame's order.  One wave per segment, ballot ranks.
+// Index lists (buffer positions): new contacts (match index < 0) in this frame's canonical order, broken
+// previous contacts (not won by any new contact) in the previous frame's canonical order.  The reference
+// fills both through atomic counters, in no fixed order.  One wave per segment, ballot ranks.
 __global__ void __launch_bounds__(64) report_write_kernel(MatchState m, NmcContacts c,
                                                          const int* __restrict__ match_index,
                                                          int* __restrict__ new_indices,
@@ -198,9 +208,10 @@ __global__ void __launch_bounds__(64) report_write_kernel(MatchState m, NmcConta
   int base = m.seg_new_off[seg];
   for (int i0 = ws; i0 < ws + wc; i0 += 64) {
     const int i = i0 + static_cast<int>(threadIdx.x);
-    const bool is_new = i < ws + wc && match_index[i] < 0;
+    const int bi = i < ws + wc ? cpos(c, i) : -1;
+    const bool is_new = bi >= 0 && match_index[bi] < 0;
     const unsigned long long bal = __ballot(is_new);
-    if (is_new) new_indices[base + __popcll(bal & below)] = i;
+    if (is_new) new_indices[base + __popcll(bal & below)] = bi;
     base += __popcll(bal);
   }
   const int ps = m.seg_start[seg], pe = ps + m.seg_count[seg];
@@ -209,7 +220,7 @@ __global__ void __launch_bounds__(64) report_write_kernel(MatchState m, NmcConta
     const int p = p0 + static_cast<int>(threadIdx.x);
     const bool brk = p < pe && !m.prev_matched[p];
     const unsigned long long bal = __ballot(brk);
-    if (brk) broken_indices[base + __popcll(bal & below)] = p;
+    if (brk) broken_indices[base + __popcll(bal & below)] = m.bufpos[p];
     base += __popcll(bal);
   }
 }
@@ -226,14 +237,20 @@ __global__ void __launch_bounds__(256) save_kernel(MatchState m, NmcContacts c, 
   const int n = n0 < c.capacity ? n0 : c.capacity;
   if (blockIdx.x == 0 && threadIdx.x == 0) m.count[0] = n;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    m.keys[i] = c.sort_key[i];
-    store3(m.pos + 3 * i, midpoint(c, body_q, shape_body, i));
-    store3(m.normal + 3 * i, load3(c.normal + 3 * i));
+    const int bi = cpos(c, i);
+    m.bufpos[i] = bi;
+    if (bi < 0) {  // not stored: a key no contact carries, so nothing matches it
+      m.keys[i] = -1ll;
+      continue;
+    }
+    m.keys[i] = c.sort_key[bi];
+    store3(m.pos + 3 * i, midpoint(c, body_q, shape_body, bi));
+    store3(m.normal + 3 * i, load3(c.normal + 3 * bi));
     if (m.sticky) {
-      store3(m.p0 + 3 * i, load3(c.point0 + 3 * i));
-      store3(m.p1 + 3 * i, load3(c.point1 + 3 * i));
-      store3(m.o0 + 3 * i, load3(c.offset0 + 3 * i));
-      store3(m.o1 + 3 * i, load3(c.offset1 + 3 * i));
+      store3(m.p0 + 3 * i, load3(c.point0 + 3 * bi));
+      store3(m.p1 + 3 * i, load3(c.point1 + 3 * bi));
+      store3(m.o0 + 3 * i, load3(c.offset0 + 3 * bi));
+      store3(m.o1 + 3 * i, load3(c.offset1 + 3 * bi));
     }
   }
 }
@@ -259,7 +276,7 @@ static hipError_t mmalloc(T** p, size_t count) {
 extern "C" void nmc_matcher_destroy(NmcMatcher* mt) {
   if (!mt) return;
   MatchState& s = mt->s;
-  void* ptrs[] = {s.keys, s.pos, s.normal, s.p0, s.p1, s.o0, s.o1, s.seg_start, s.seg_count, s.count, s.cand,
+  void* ptrs[] = {s.keys, s.pos, s.normal, s.p0, s.p1, s.o0, s.o1, s.seg_start, s.seg_count, s.bufpos, s.count, s.cand,
                   s.dist_sq, s.prev_matched, s.seg_new, s.seg_broken, s.seg_new_off, s.seg_broken_off};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -288,6 +305,7 @@ extern "C" int nmc_matcher_create(const NmcModel* model, int capacity, float pos
   if ((e = mmalloc(&s.keys, C)) != hipSuccess || (e = mmalloc(&s.pos, 3 * C)) != hipSuccess ||
       (e = mmalloc(&s.normal, 3 * C)) != hipSuccess || (e = mmalloc(&s.seg_start, S)) != hipSuccess ||
       (e = mmalloc(&s.seg_count, S)) != hipSuccess || (e = mmalloc(&s.count, 1)) != hipSuccess ||
+      (e = mmalloc(&s.bufpos, C)) != hipSuccess ||
       (e = mmalloc(&s.cand, C)) != hipSuccess || (e = mmalloc(&s.dist_sq, C)) != hipSuccess ||
       (s.sticky && ((e = mmalloc(&s.p0, 3 * C)) != hipSuccess || (e = mmalloc(&s.p1, 3 * C)) != hipSuccess ||
                     (e = mmalloc(&s.o0, 3 * C)) != hipSuccess || (e = mmalloc(&s.o1, 3 * C)) != hipSuccess)) ||
@@ -322,7 +340,8 @@ extern "C" int nmc_contact_match(NmcMatcher* mt, const NmcModel* model, const fl
                                  int* match_index, const NmcContactReport* report, nmc_stream_t stream) {
   if (!mt || !model || !c || !match_index) return nmc::fail("nmc_contact_match: null argument");
   const MatchState& s = mt->s;
-  if (!c->sort_key) return nmc::fail("nmc_contact_match: the contacts buffer needs sort keys (deterministic pipeline)");
+  if (!c->sort_key || !c->order)
+    return nmc::fail("nmc_contact_match: the contacts buffer needs sort keys and its order array (deterministic pipeline)");
   if (c->capacity > s.capacity) return nmc::fail("nmc_contact_match: contacts capacity exceeds the matcher's");
   if (model->d.segment_count != s.segments) return nmc::fail("nmc_contact_match: model does not match the matcher");
   if (model->d.body_count > 0 && !body_q) return nmc::fail("nmc_contact_match: body_q is null");

@@ -52,6 +52,56 @@ static int next_pow2(int n) {
   return p;
 }
 
+// Tables of the global sort-key layout (collide.hip order_*).  The layout needs every world's shapes to form one
+// contiguous index block (world -1 shapes lie between blocks) and at most 256 world -1 shapes; otherwise
+// ord_G stays -1 and a collide that asks for the layout fails.
+static hipError_t order_setup(NmcModelImpl* m) {
+  const NmcModelDesc& d = m->d;
+  const int S = d.shape_count, W = d.world_count, segs = d.segment_count;
+  std::vector<int> sw(S > 0 ? S : 1);
+  hipError_t e = hipSuccess;
+  if (S > 0 && (e = hipMemcpy(sw.data(), d.shape_world, sizeof(int) * S, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+  std::vector<int> gid(S > 0 ? S : 1, -1), gshape;
+  std::vector<int> lo(W > 0 ? W : 1, 0x7fffffff), hi(W > 0 ? W : 1, -1), n(W > 0 ? W : 1, 0);
+  for (int s = 0; s < S; ++s) {
+    if (sw[s] < 0) {
+      gid[s] = static_cast<int>(gshape.size());
+      gshape.push_back(s);
+    } else if (sw[s] < W) {
+      lo[sw[s]] = std::min(lo[sw[s]], s);
+      hi[sw[s]] = std::max(hi[sw[s]], s);
+      n[sw[s]] += 1;
+    }
+  }
+  const int G = static_cast<int>(gshape.size());
+  for (int w = 0; w < W; ++w)
+    if (n[w] > 0 && hi[w] - lo[w] + 1 != n[w]) return hipSuccess;  // a world's shapes are not one block
+  if (G > 256) return hipSuccess;
+  // units by position: world w at its first shape, world -1 shape g at its index
+  const int U = W + G;
+  std::vector<std::pair<long long, int>> pos;
+  pos.reserve(U);
+  for (int w = 0; w < W; ++w) pos.push_back({n[w] > 0 ? lo[w] : (1ll << 40) + w, w});
+  for (int g = 0; g < G; ++g) pos.push_back({gshape[g], W + g});
+  std::stable_sort(pos.begin(), pos.end());
+  std::vector<int> unit_order(U > 0 ? U : 1);
+  for (int u = 0; u < U; ++u) unit_order[u] = pos[u].second;
+  const size_t G1 = G > 0 ? G : 1;
+  if ((e = dmalloc(&m->ord_gid, S)) != hipSuccess || (e = dmalloc(&m->ord_unit_order, U)) != hipSuccess ||
+      (e = dmalloc(&m->ord_delta0, S)) != hipSuccess || (e = dmalloc(&m->ord_M, segs * G1)) != hipSuccess ||
+      (e = dmalloc(&m->ord_gstart, segs * G1)) != hipSuccess || (e = dmalloc(&m->ord_L, segs)) != hipSuccess ||
+      (e = dmalloc(&m->ord_H, G1 * G1)) != hipSuccess || (e = dmalloc(&m->ord_unit_off, G1 * (U > 0 ? U : 1))) != hipSuccess ||
+      (e = dmalloc(&m->ord_base, U)) != hipSuccess)
+    return e;
+  if ((S > 0 && (e = hipMemcpy(m->ord_gid, gid.data(), sizeof(int) * S, hipMemcpyHostToDevice)) != hipSuccess) ||
+      (U > 0 && (e = hipMemcpy(m->ord_unit_order, unit_order.data(), sizeof(int) * U, hipMemcpyHostToDevice)) != hipSuccess))
+    return e;
+  (void)segs;
+  m->ord_G = G;
+  m->ord_U = U;
+  return hipSuccess;
+}
+
 }  // namespace nmc
 
 using namespace nmc;
@@ -286,6 +336,14 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       return hip_fail(e, "nmc_model_create: device allocation");
     }
   }
+  m->ord_G = -1;
+  if (d.shape_pairs_max > 0) {
+    e = order_setup(m);
+    if (e != hipSuccess) {
+      nmc_model_destroy(m);
+      return hip_fail(e, "nmc_model_create: sort-key layout tables");
+    }
+  }
   *out = m;
   return 0;
 }
@@ -298,7 +356,9 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
                   m->seg_small, m->seg_big,
                   s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
                   m->seg_pair_count, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, m->seg_contact_count,
-                  m->counters, m->xpbd_keys, m->xpbd_tier_worlds[0], m->xpbd_tier_worlds[1], m->xpbd_tier_worlds[2]};
+                  m->counters, m->xpbd_keys, m->xpbd_tier_worlds[0], m->xpbd_tier_worlds[1], m->xpbd_tier_worlds[2],
+                  m->ord_gid, m->ord_unit_order, m->ord_delta0, m->ord_M, m->ord_gstart, m->ord_L, m->ord_H,
+                  m->ord_unit_off, m->ord_base};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   delete m;
@@ -311,6 +371,11 @@ extern "C" int nmc_collide(NmcModel* m, const float* body_q, const NmcContacts* 
   if (!c->count || !c->shape0 || !c->shape1 || !c->point0 || !c->point1 || !c->offset0 || !c->offset1 ||
       !c->normal || !c->margin0 || !c->margin1 || !c->world_start || !c->world_count)
     return fail("nmc_collide: contacts buffer has a null array");
+  if (c->sort_key && !c->order)
+    return fail("nmc_collide: a contacts buffer with sort keys needs its order array (the global key layout)");
+  if (c->order && m->ord_G < 0)
+    return fail("nmc_collide: the global sort-key layout needs each world's shapes in one index block and at most "
+                "256 world -1 shapes");
   hipError_t e = launch_collide(m, body_q, c, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_collide launch");
   return 0;

@@ -75,6 +75,21 @@ struct NmcModelImpl {
   int n_seg_small, n_seg_big;
   int seg_small_max_n;
   size_t seg_small_lds;     // dynamic LDS of one wave
+  // global sort-key layout of deterministic Contacts buffers (collide.hip: order_count / order_scan / compaction).
+  // Units are the W worlds' shape blocks and the G world -1 shapes, ordered by shape index: a contact's
+  // buffer position is the contacts of lower shape0 (unit block bases, then the shape0's rank inside its
+  // world block), then its rank among the contacts of its shape0 (for a world -1 shape0: by unit of shape1).
+  int ord_G;              // world -1 shapes (-1: the layout is unavailable for this model)
+  int ord_U;              // W + G units
+  int* ord_gid;           // [S] index among the world -1 shapes, -1 for world shapes
+  int* ord_unit_order;    // [U] unit ids (world w = w, world -1 shape g = W + g) in shape-index order
+  int* ord_delta0;        // [S] world shape0: minus the world -1 shape0 contacts before its run in its segment
+  int* ord_M;             // [segments x G] contacts of each world -1 shape0 in each segment
+  int* ord_gstart;        // [segments x G] first segment-local contact of each world -1 shape0
+  int* ord_L;             // [segments] contacts whose shape0 is a world shape
+  int* ord_H;             // [G x G] last segment: contacts of each (world -1 shape0, world -1 shape1) pair
+  int* ord_unit_off;      // [G x U] rank base of each unit among the contacts of one world -1 shape0
+  int* ord_base;          // [U] first buffer position of each unit's shape0 block
   int* counters;         // [4] overflow counters (see nmc_get_counters)
   int pair_cap;          // per-segment pair buffer (LDS) capacity
   size_t collide_lds;
@@ -101,6 +116,10 @@ int bp_explicit_run(int shape_count, const float* lower, const float* upper, con
                     const int* shape_pairs, int shape_pair_count, int blocks, int* count, int* off, int* total,
                     int* candidate_pair, int max_candidate_pair, int* candidate_pair_count, int append,
                     hipStream_t s, const char* what);
+
+// Buffer position of the contact at canonical (segment-major, key-sorted per segment) position i: the identity
+// for the segment-major layout, Contacts.order for the global key-order layout; -1 when not stored.
+__device__ __forceinline__ int cpos(const NmcContacts& c, int i) { return c.order ? c.order[i] : i; }
 
 void set_error(const std::string& msg);
 int fail(const std::string& msg);               // sets the error string, returns -1

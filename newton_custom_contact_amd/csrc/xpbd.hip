@@ -453,9 +453,19 @@ struct CReg {
   int valid;
 };
 
-__device__ __forceinline__ CReg load_contact(const XpbdArgs& a, int gc, int b0, int nb) {
+// Contact at canonical position j (the world's range in key order); its buffer position is cpos(a.c, j).  A
+// contact not stored (past the capacity in the global key layout) is inert.
+__device__ __forceinline__ CReg load_contact(const XpbdArgs& a, int j, int b0, int nb) {
   const NmcModelDesc& d = a.d;
   CReg r;
+  const int gc = cpos(a.c, j);
+  if (gc < 0) {
+    r.valid = 0;
+    r.la = r.lb = -1;
+    r.p0 = r.p1 = r.o0 = r.o1 = r.n = f3{0.0f, 0.0f, 0.0f};
+    r.m0 = r.m1 = r.mu = r.mut = r.mur = 0.0f;
+    return r;
+  }
   int s0 = a.c.shape0[gc], s1 = a.c.shape1[gc];
   int ba = s0 >= 0 ? d.shape_body[s0] : -1;
   int bb = s1 >= 0 ? d.shape_body[s1] : -1;
@@ -666,7 +676,10 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   }
   if (MODE == XM_REPORT) {
     if (a.rep.contact_impulse)
-      for (int k = tid; k < 6 * nc; k += BS) a.rep.contact_impulse[6ll * cstart + k] = 0.0f;
+      for (int k = tid; k < 6 * nc; k += BS) {
+        const int pc = cpos(a.c, cstart + k / 6);
+        if (pc >= 0) a.rep.contact_impulse[6ll * pc + k % 6] = 0.0f;
+      }
     if (a.rep.body_parent_f)
       for (int k = tid; k < 6 * nj; k += BS) a.rep.joint_impulse[6ll * j0 + k] = 0.0f;
   }
@@ -871,16 +884,20 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       } else {
         r = load_contact(a, cstart + c, b0, nb);
       }
-      const int gc = cstart + c;
+      const int gc = cpos(a.c, cstart + c);
+      if (gc < 0) continue;
       float lam_prev = 0.0f;
       if (lrn.prev_key && a.c.sort_key) {
-        // the previous substep's contact with the same key in this world (keys ascend per world)
+        // the previous substep's contact with the same key in this world (keys ascend over the world's
+        // canonical range; prev_order maps it to that buffer)
         const long long key = a.c.sort_key[gc];
         int lo = lrn.prev_start[w], hi = lo + lrn.prev_count[w] - 1;
         while (lo <= hi) {
           int mid = (lo + hi) >> 1;
-          long long km = lrn.prev_key[mid];
-          if (km == key) { lam_prev = lrn.prev_lambda[mid]; break; }
+          const int pm = lrn.prev_order ? lrn.prev_order[mid] : mid;
+          if (pm < 0) break;
+          long long km = lrn.prev_key[pm];
+          if (km == key) { lam_prev = lrn.prev_lambda[pm]; break; }
           if (km < key) lo = mid + 1; else hi = mid - 1;
         }
       }
@@ -906,7 +923,10 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 #pragma unroll
   for (int jj = 0; jj < CPL; ++jj) lamacc[jj] = 0.0f;
   if (ACC)
-    for (int c = CH + tid; c < nc; c += BS) lrn.lambda_sum[cstart + c] = 0.0f;
+    for (int c = CH + tid; c < nc; c += BS) {
+      const int pc = cpos(a.c, cstart + c);
+      if (pc >= 0) lrn.lambda_sum[pc] = 0.0f;
+    }
 
   // pass 0: the step (XM_WARM: warm-start pass first, warm_iterations iterations); pass 1 (XM_WARM
   // only, when the world's residual exceeds the tolerance): cold solve from the integrated state
@@ -923,7 +943,10 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       }
 #pragma unroll
       for (int jj = 0; jj < CPL; ++jj) lamacc[jj] = 0.0f;
-      for (int c = CH + tid; c < nc; c += BS) lrn.lambda_sum[cstart + c] = 0.0f;
+      for (int c = CH + tid; c < nc; c += BS) {
+      const int pc = cpos(a.c, cstart + c);
+      if (pc >= 0) lrn.lambda_sum[pc] = 0.0f;
+    }
       __syncthreads();
     }
     // it == -1: the warm-start pass (XM_WARM, pass 0)
@@ -955,7 +978,8 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
               bool on;
               float lam = 0.0f;
               if (warm) {
-                on = warm_contrib(L, r, lrn.lambda0[cstart + c], da, db);
+                const int pc = cpos(a.c, cstart + c);
+                on = pc >= 0 && warm_contrib(L, r, lrn.lambda0[pc], da, db);
               } else {
                 on = r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu, r.mut,
                                               r.mur, relax, dt, da, db, lam);
@@ -975,7 +999,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
                 store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
                 if (ACC && !warm) {
                   if (cb == 0) lamacc[jj] += lam;
-                  else lrn.lambda_sum[cstart + c] += lam;
+                  else lrn.lambda_sum[cpos(a.c, cstart + c)] += lam;
                 }
               }
             }
@@ -1024,7 +1048,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
                 else wgt = 2.0f / ns;
               }
             }
-            float* acc_c = a.rep.contact_impulse + 6ll * (cstart + c);
+            float* acc_c = a.rep.contact_impulse + 6ll * cpos(a.c, cstart + c);
 #pragma unroll
             for (int k = 0; k < 6; ++k) acc_c[k] = acc_c[k] + sc[k] * wgt;
           }
@@ -1126,19 +1150,22 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 #pragma unroll
     for (int jj = 0; jj < CPL; ++jj) {
       int c = jj * BS + tid;
-      if (c < nc) {
+      const int pc = c < nc ? cpos(a.c, cstart + c) : -1;
+      if (pc >= 0) {
         float l0 = 0.0f;
         if (MODE == XM_WARM && !s_fallback) {
-          float v = lrn.lambda0[cstart + c];
+          float v = lrn.lambda0[pc];
           l0 = v > 0.0f ? v : 0.0f;
         }
-        lrn.lambda_sum[cstart + c] = l0 + lamacc[jj];
+        lrn.lambda_sum[pc] = l0 + lamacc[jj];
       }
     }
     if (MODE == XM_WARM && !s_fallback) {
       for (int c = CH + tid; c < nc; c += BS) {
-        float v = lrn.lambda0[cstart + c];
-        lrn.lambda_sum[cstart + c] = (v > 0.0f ? v : 0.0f) + lrn.lambda_sum[cstart + c];
+        const int pc = cpos(a.c, cstart + c);
+        if (pc < 0) continue;
+        float v = lrn.lambda0[pc];
+        lrn.lambda_sum[pc] = (v > 0.0f ? v : 0.0f) + lrn.lambda_sum[pc];
       }
     }
   }

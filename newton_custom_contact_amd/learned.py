@@ -230,24 +230,26 @@ class SolverXPBDWarmStart(SolverXPBD):
     def _history(self, contacts, b):
         lam_prev = b["lam"][1 - self._cur]
         if self._prev is None:
-            return None, b["snap_start"], b["snap_count"], lam_prev
+            return None, b["snap_start"], b["snap_count"], lam_prev, None
         if self._prev is contacts:
             if not self._snapshot:
                 self._snapshot = True
                 b["snap_count"].zero_()
                 b["snap_key"] = torch.zeros_like(contacts.rigid_contact_sort_key)
-                return None, b["snap_start"], b["snap_count"], lam_prev
-            return b["snap_key"], b["snap_start"], b["snap_count"], lam_prev
+                b["snap_order"] = torch.zeros_like(contacts.rigid_contact_order)
+                return None, b["snap_start"], b["snap_count"], lam_prev, None
+            return b["snap_key"], b["snap_start"], b["snap_count"], lam_prev, b["snap_order"]
         pc = self._prev
-        return pc.rigid_contact_sort_key, pc.world_contact_start, pc.world_contact_count, lam_prev
+        return (pc.rigid_contact_sort_key, pc.world_contact_start, pc.world_contact_count, lam_prev,
+                pc.rigid_contact_order)
 
     def _io(self, mode: int, b: dict, contacts, warm_iterations: int = 0,
             residual_tol: float | None = None) -> NmcLearnIO:
         tol = self.residual_tol if residual_tol is None else float(residual_tol)
-        key, start, count, lam_prev = self._history(contacts, b)
+        key, start, count, lam_prev, order = self._history(contacts, b)
         return NmcLearnIO(mode, warm_iterations, tol, ptr(b["feat"]), ptr(b["lam"][self._cur]),
                           ptr(b["lambda0"]), ptr(b["residual"]), ptr(b["fallback"]), ptr(b["q_mid"]),
-                          ptr(b["qd_mid"]), ptr(key), ptr(start), ptr(count), ptr(lam_prev))
+                          ptr(b["qd_mid"]), ptr(key), ptr(start), ptr(count), ptr(lam_prev), ptr(order))
 
     def _launch(self, state_in, state_out, control, contacts, dt, io, iterations: int | None = None):
         model = self.model
@@ -271,6 +273,7 @@ class SolverXPBDWarmStart(SolverXPBD):
     def _remember(self, contacts, b):
         if self._snapshot:
             b["snap_key"].copy_(contacts.rigid_contact_sort_key)
+            b["snap_order"].copy_(contacts.rigid_contact_order)
             b["snap_start"].copy_(contacts.world_contact_start)
             b["snap_count"].copy_(contacts.world_contact_count)
         self._prev = contacts

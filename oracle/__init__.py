@@ -209,7 +209,23 @@ class Oracle:
         n = self.lib.oracle_broadphase_f32(ctypes.byref(self.m), _p(q, _f32p), cap, _p(out, _i32p))
         return out[: min(n, cap)]
 
-    def collide(self, body_q: np.ndarray, capacity: int | None = None, precision: str = "f32") -> dict:
+    def collide(self, body_q: np.ndarray, capacity: int | None = None, precision: str = "f32",
+                layout: str = "canonical") -> dict:
+        """CollisionPipeline.collide.  layout "canonical": segment by segment, key order inside a segment;
+        "global": the deterministic pipeline's buffer, every contact found sorted by its 64-bit sort key and the
+        first `capacity` kept (ContactSorter, contact_sort.py:194-445, as collide.py:1105-1124 applies it)."""
+        if layout == "global":
+            full = self._collide(body_q, None, precision)
+            n = full["count"]
+            full = self._collide(body_q, max(n, 1), precision)
+            order = np.argsort(full["sort_key"], kind="stable")
+            cap = n if capacity is None else min(n, capacity)
+            res = {k: v[order][:cap] for k, v in full.items() if isinstance(v, np.ndarray)}
+            res["count"] = n
+            return res
+        return self._collide(body_q, capacity, precision)
+
+    def _collide(self, body_q: np.ndarray, capacity: int | None = None, precision: str = "f32") -> dict:
         dt = np.float32 if precision == "f32" else np.float64
         tp = _f32p if precision == "f32" else _f64p
         q = np.ascontiguousarray(body_q, dtype=dt)

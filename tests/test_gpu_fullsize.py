@@ -73,18 +73,25 @@ def test_fullsize_mixed_parity_and_layout(cuda_device):
     again = c2.numpy()
     for k in ("shape0", "shape1", "sort_key", "point0", "point1", "offset0", "offset1", "normal", "margin0", "margin1"):
         assert np.array_equal(again[k], got[k]), f"second collide differs in {k}"
+    # the reference's deterministic layout: keys strictly ascending over the whole buffer; the canonical
+    # per-world ranges map onto it through the order array, each a permutation slice of one world's contacts in
+    # key order
+    key = got["sort_key"]
+    assert np.all(np.diff(key) > 0), "buffer keys do not ascend"
     start = c.world_contact_start.cpu().numpy().astype(np.int64)
     count = c.world_contact_count.cpu().numpy().astype(np.int64)
+    order = got["order"]
     assert int(count.sum()) == n
     assert np.array_equal(start, np.concatenate([[0], np.cumsum(count)[:-1]]))
-    key = got["sort_key"]
+    assert np.array_equal(np.sort(order), np.arange(n)), "order is not a permutation of the buffer"
     shape_world = model.host["shape_world"]
     for w in range(len(start)):
         s, e = int(start[w]), int(start[w] + count[w])
+        pos = order[s:e]
         if e - s > 1:
-            assert np.all(np.diff(key[s:e]) > 0), f"world {w}: keys do not ascend"
+            assert np.all(np.diff(key[pos]) > 0), f"world {w}: keys do not ascend"
         for col in ("shape0", "shape1"):
-            sw = shape_world[got[col][s:e]]
+            sw = shape_world[got[col][pos]]
             assert np.all((sw == w) | (sw == -1)), f"world {w}: a contact names another world's shape"
 
     print("[full size] step parity", flush=True)

@@ -136,3 +136,75 @@ def test_step_parity_joint_forces_and_targets(cuda_device):
         np.testing.assert_allclose(q_gpu, q_ref, rtol=1e-5, atol=1e-6)
         np.testing.assert_allclose(qd_gpu, qd_ref, rtol=1e-5, atol=1e-5)
     print("joint forces/targets bit-exact", exact, flush=True)
+
+
+def _layout_scene(dev):
+    """World -1 shapes between and after the worlds: a static sphere after world 0 (resting 1 cm into the ground,
+    so the last, world -1 only, segment has a sphere-plane pair) and the ground plane as the last shape; three
+    worlds of boxes and spheres piled onto both."""
+    from newton_custom_contact_amd.builder import ModelBuilder
+
+    rng = np.random.Generator(np.random.PCG64(17))
+    b = ModelBuilder()
+    cfg = ModelBuilder.ShapeConfig()
+    cfg.mu = 0.5
+
+    def world():
+        b.begin_world()
+        for k in range(10):
+            p = (rng.uniform(-0.5, 0.5), rng.uniform(-0.5, 0.5), rng.uniform(0.05, 0.9))
+            body = b.add_body(xform=(*p, 0.0, 0.0, 0.0, 1.0))
+            if k % 2:
+                b.add_shape_box(body, hx=0.08, hy=0.1, hz=0.06, cfg=cfg)
+            else:
+                b.add_shape_sphere(body, radius=0.09, cfg=cfg)
+        b.end_world()
+
+    world()
+    b.add_shape_sphere(-1, xform=(0.1, 0.0, 0.29, 0.0, 0.0, 0.0, 1.0), radius=0.3, cfg=cfg)
+    world()
+    world()
+    b.add_ground_plane(cfg=cfg)
+    return b.finalize(dev)
+
+
+LAYOUT_SCENES = [("mixed_8w_plane_first", lambda dev: scenes.mixed_worlds(8, seed=3, device=dev)),
+                 ("global_shapes_between_and_last", _layout_scene)]
+
+
+@pytest.mark.parametrize("name,make", LAYOUT_SCENES, ids=[s[0] for s in LAYOUT_SCENES])
+def test_deterministic_layout_element_by_element(cuda_device, name, make):
+    """deterministic=True: the Contacts arrays equal, element by element with no sorting of the GPU side, the
+    oracle's contacts in the reference's global sort-key order (collide.py:1105-1124); the order array maps the
+    canonical per-world ranges onto that buffer; a step on it equals the oracle's step bit for bit."""
+    print(f"[deterministic layout] {name}", flush=True)
+    model = make(cuda_device)
+    oracle = Oracle(model)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    solver = SolverXPBD(model, iterations=10)
+    sw = model.host["shape_world"]
+    exercised_global_shape0 = False
+    for q, qd in oracle_states(oracle, model, n_substeps=160, dt=DT, every=40):
+        ref = oracle.collide(q, layout="global")
+        c, got, state = _gpu_collide(pipe, model, q)
+        n = ref["count"]
+        assert got["count"] == n and n > 0
+        for k in ("sort_key", "shape0", "shape1"):
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+        for k in ("point0", "point1", "offset0", "offset1", "normal", "margin0", "margin1"):
+            np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+        exercised_global_shape0 |= bool(np.any(sw[got["shape0"]] == -1))
+        # the canonical order: world by world (the world -1 only segment last), key order inside a world
+        order = got["order"]
+        canon = oracle.collide(q)
+        cw = np.maximum(sw[canon["shape0"]], sw[canon["shape1"]])
+        cw = np.where(cw < 0, model.world_count, cw)
+        np.testing.assert_array_equal(got["sort_key"][order], canon["sort_key"][np.lexsort((canon["sort_key"], cw))])
+        state.body_qd.copy_(torch.from_numpy(np.ascontiguousarray(qd, np.float32)))
+        out = model.state()
+        solver.step(state, out, None, c, DT)
+        torch.cuda.synchronize()
+        q1, qd1 = oracle.step(q, qd, np.zeros_like(qd), canon, DT, iterations=10)
+        np.testing.assert_array_equal(out.body_q.cpu().numpy(), q1)
+        np.testing.assert_array_equal(out.body_qd.cpu().numpy(), qd1)
+    assert exercised_global_shape0
