@@ -10,14 +10,23 @@ algorithmic bytes of the API boundary (each substep's collide/step read their in
 outputs once):
     B_collide = 28 N_b + 64 N_s + 76 N_c + 4          bytes per world-substep
     B_step    = 76 N_b + 96 N_b + 16 N_s + 76 N_c + 52 N_b + 120 N_j
-N_c is the measured mean contact count per world per substep in the timed region.
-``cpu_baseline`` times the C++ oracle (scalar restatement of the reference algorithm, 1 thread) on a
-bounded sample: a snapshot of the first worlds of the timed state, stepped on the host.
+N_c is the measured mean contact count per world per substep in the timed region.  Every form first runs an
+untimed settle phase (``--settle`` steps, 2 s of simulated time by default, whatever ``--warmup`` is) so that the
+timed region measures the piled-up scene, not bodies still falling.
+``iterations`` sweeps the solver's iteration count at the settled state: the mean over worlds of the largest
+remaining penetration after one substep, plain (1..``--iters-max`` iterations) and learned warm start (1..10 warm
+iterations, no fallback), and reports the iterations each needs to reach 1e-4 m and the plain 10-iteration
+residual ("not reached" when it does not within the sweep).
+``cpu_baseline`` times the C++ oracle (scalar restatement of the reference algorithm) on a bounded sample: a
+snapshot of the first worlds of the timed state, stepped on the host, one thread and then one thread per core of
+the box's CPU share.
 ``learned`` times the same workload with the learned warm start (SolverXPBDWarmStart: prep launch,
 network, warm launch with ``--warm-iterations`` iterations and the per-world residual check) and
 reports its penetration residual beside the plain solver's at the same state.
 ``sdf`` times the triangle-mesh workload (1024 worlds x 8 convex-hull meshes with 64^3 SDFs, mesh-mesh and
-mesh-plane contacts) with the same step.
+mesh-plane contacts, the reference's contact reduction) with the same step; ``uneven`` the uneven-worlds workload
+(2048 worlds of 8-256 primitives plus a 6-link revolute chain each; its scene is built on the host in a separate
+process while the other forms run); ``single_world`` the one-world 64-box case.
 """
 
 from __future__ import annotations
@@ -25,7 +34,9 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -37,7 +48,10 @@ def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
     p.add_argument("--steps", type=int, default=240)
-    p.add_argument("--warmup", type=int, default=120)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--settle", type=int, default=120,
+                   help="untimed steps before warmup and timing (the pile settles; independent of --warmup)")
+    p.add_argument("--iters-max", type=int, default=30, help="largest plain iteration count of the sweep")
     p.add_argument("--worlds", type=int, default=4096)
     p.add_argument("--iterations", type=int, default=10)
     p.add_argument("--substeps", type=int, default=4)
@@ -51,18 +65,18 @@ def parse():
     p.add_argument("--residual-tol", type=float, default=5e-3,
                    help="per-world penetration residual above which the learned form falls back to a cold solve")
     p.add_argument("--no-sdf", action="store_true", help="skip the SDF (triangle-mesh) workload")
-    p.add_argument("--uneven", action="store_true",
-                   help="also run the uneven-worlds workload (log-uniform 8-256 bodies per world + revolute chains; "
-                        "its scene build takes minutes on the host, so it is not part of the default run)")
+    p.add_argument("--no-uneven", action="store_true",
+                   help="skip the uneven-worlds workload (log-uniform 8-256 bodies per world + revolute chains)")
+    p.add_argument("--no-single", action="store_true", help="skip the one-world 64-box case")
     p.add_argument("--uneven-worlds", type=int, default=2048)
-    p.add_argument("--uneven-steps", type=int, default=30)
-    p.add_argument("--uneven-warmup", type=int, default=30)
+    p.add_argument("--uneven-steps", type=int, default=60)
+    p.add_argument("--uneven-settle", type=int, default=60)
     p.add_argument("--no-components", action="store_true",
                    help="skip the per-launch timings of contact matching, force reporting, standalone broad "
                         "phases and the SDF cache")
     p.add_argument("--sdf-worlds", type=int, default=1024)
     p.add_argument("--sdf-steps", type=int, default=60)
-    p.add_argument("--sdf-warmup", type=int, default=60)
+    p.add_argument("--sdf-settle", type=int, default=60)
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="only run a device-to-device copy of this many MiB, 5 times (HBM counter calibration)")
     return p.parse_args()
@@ -82,91 +96,37 @@ def copy_probe(mib: int) -> None:
     print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "launches": 5}), flush=True)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (worlds shard across "
-              "GPUs with no per-step exchange, so the multi-GPU figure is N x the one-GPU figure by construction)",
-              file=sys.stderr)
-        sys.exit(2)
-    if args.copy_probe_mib > 0:
-        copy_probe(args.copy_probe_mib)
-        return
+UNEVEN_BUILD = """
+import sys, time
+sys.path.insert(0, sys.argv[1])
+from newton_custom_contact_amd import scenes
+t0 = time.time()
+m = scenes.uneven_worlds(int(sys.argv[3]), seed=int(sys.argv[4]), device="cpu",
+                         progress=lambda k, n: print(f"[bench] uneven scene: {k}/{n} worlds built", file=sys.stderr,
+                                                     flush=True))
+m.save(sys.argv[2])
+print(f"[bench] uneven scene built on the host in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+"""
+
+
+def start_uneven_build(args):
+    """The uneven scene takes minutes of host Python to build: start it in a child process before this process
+    touches the GPU, so it overlaps the other forms."""
+    if args.no_uneven:
+        return None
+    d = tempfile.mkdtemp(prefix="nmc_bench_")
+    path = os.path.join(d, "uneven.npz")
+    root = os.path.dirname(os.path.abspath(__file__))
+    proc = subprocess.Popen([sys.executable, "-c", UNEVEN_BUILD, root, path, str(args.uneven_worlds), str(args.seed)])
+    return proc, path, time.time()
+
+
+def phase_times(pipe, solver, a, b, contacts, sdt, reps):
+    """Average launch durations on the launch stream (HIP events): `reps` back-to-back collides, then `reps`
+    back-to-back steps, at the given state, so the GPU never waits on the host."""
     import torch
 
-    from newton_custom_contact_amd import CollisionPipeline, scenes
-    from newton_custom_contact_amd.solvers import SolverXPBD
-
-    dev = "cuda:0"
-    t0 = time.time()
-    model = scenes.mixed_worlds(args.worlds, seed=args.seed, device=dev)
-    print(f"[bench] scene: {args.worlds} worlds, {model.body_count} bodies, {model.shape_count} shapes "
-          f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
-    pipe = CollisionPipeline(model, broad_phase="sap")
-    solver = SolverXPBD(model, iterations=args.iterations)
-    contacts = pipe.contacts()
-    states = [model.state(), model.state()]
-    dt = 1.0 / 60.0
-    sdt = dt / args.substeps
-    nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
-
-    def step():
-        for s in range(args.substeps):
-            a, b = states[s % 2], states[(s + 1) % 2]
-            pipe.collide(a, contacts)
-            solver.step(a, b, None, contacts, sdt)
-            nc_acc.add_(contacts.rigid_contact_count)
-        if args.substeps % 2:
-            states[0].body_q.copy_(states[1].body_q)
-            states[0].body_qd.copy_(states[1].body_qd)
-
-    torch.cuda.synchronize()
-    graph = None
-    if not args.no_graph:
-        side = torch.cuda.Stream()
-        side.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.stream(side):
-            step()
-        torch.cuda.current_stream().wait_stream(side)
-        torch.cuda.synchronize()
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
-            step()
-        torch.cuda.synchronize()
-
-    def run(n):
-        for _ in range(n):
-            if graph is not None:
-                graph.replay()
-            else:
-                step()
-
-    run(args.warmup)
-    torch.cuda.synchronize()
-    print(f"[bench] warmup {args.warmup} steps done", file=sys.stderr, flush=True)
-    nc_acc.zero_()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    e0.record()
-    run(args.steps)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
-    ms = e0.elapsed_time(e1)
-    ms_per_step = ms / args.steps
-    ws_per_s = args.worlds * args.steps / (ms / 1e3)
-    n_c = float(nc_acc.item()) / (args.steps * args.substeps * args.worlds)
-    print(f"[bench] timed {args.steps} steps: {ms:.2f} ms (wall {wall * 1e3:.1f} ms), N_c={n_c:.1f}",
-          file=sys.stderr, flush=True)
-
-    # ---- per-phase launch durations on the launch stream (HIP events), at the end state of the timed
-    # region: `reps` back-to-back launches of one phase between two events, so the GPU never waits on
-    # the host and the average is the kernels' own duration (collide: 4 kernels, step: 1 kernel)
     stream = torch.cuda.current_stream()
-    reps = args.kernel_reps
-    a, b = states[0], states[1]
     pipe.collide(a, contacts)
     torch.cuda.synchronize()
     ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -179,23 +139,144 @@ def main():
         solver.step(a, b, None, contacts, sdt)
     ev[3].record(stream)
     torch.cuda.synchronize()
-    t_col_ms = ev[0].elapsed_time(ev[1]) / reps
-    t_xp_ms = ev[2].elapsed_time(ev[3]) / reps
+    return ev[0].elapsed_time(ev[1]) / reps, ev[2].elapsed_time(ev[3]) / reps
+
+
+def timed_form(args, model, pipe, solver, worlds, settle, warmup, steps, dev, tune=None):
+    """Settle (untimed), warm up, then time `steps` graph replays of one step (4 substeps of collide + step).
+    `tune(state_a, state_b, contacts, sdt)` (optional) runs on the settled state and may change the step's
+    launch shape; the graph is captured again after it.  Returns (states, contacts, ms, N_c, t_collide_ms,
+    t_xpbd_ms, tune result)."""
+    import torch
+
+    contacts = pipe.contacts()
+    states = [model.state(), model.state()]
+    sdt = (1.0 / 60.0) / args.substeps
+    nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
+
+    def step():
+        for s in range(args.substeps):
+            a, b = states[s % 2], states[(s + 1) % 2]
+            pipe.collide(a, contacts)
+            solver.step(a, b, None, contacts, sdt)
+            nc_acc.add_(contacts.rigid_contact_count)
+        if args.substeps % 2:
+            states[0].body_q.copy_(states[1].body_q)
+            states[0].body_qd.copy_(states[1].body_qd)
+
+    def capture():
+        torch.cuda.synchronize()
+        if args.no_graph:
+            return None
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            step()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            step()
+        torch.cuda.synchronize()
+        return g
+
+    graph = capture()
+
+    def run(n):
+        for _ in range(n):
+            if graph is not None:
+                graph.replay()
+            else:
+                step()
+
+    run(settle)
+    tuned = None
+    if tune is not None:
+        torch.cuda.synchronize()
+        pipe.collide(states[0], contacts)
+        tuned = tune(states[0], states[1], contacts, sdt)
+        del graph
+        graph = capture()
+    run(warmup)
+    torch.cuda.synchronize()
+    nc_acc.zero_()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    run(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    n_c = float(nc_acc.item()) / (steps * args.substeps * worlds)
+    t_col, t_xp = phase_times(pipe, solver, states[0], states[1], contacts, sdt, args.kernel_reps)
+    del graph
+    return states, contacts, ms, n_c, t_col, t_xp, tuned
+
+
+def roofline_block(t_col, t_xp, worlds, b_collide, b_step, traffic=None, traffic_at=None):
+    """The dominant phase (XPBD launch or the collide sequence) priced with its share of the algorithmic bytes."""
+    collide_kernels = ("collide_segment", "route_scan_kernel", "narrow_kernel", "contact_count_kernel",
+                       "contact_scan_kernel", "contact_compact_kernel")
+    if t_xp >= t_col:
+        dom, dom_ms, dom_bytes, kernels = "xpbd_world_kernel", t_xp, worlds * b_step, ("xpbd_world_kernel",)
+    else:
+        dom, dom_ms, dom_bytes = "collide phase: " + " + ".join(collide_kernels), t_col, worlds * b_collide
+        kernels = collide_kernels
+    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    out = {"kernel": dom, "launch_ms": round(dom_ms, 4), "bound": "hbm", "achieved": round(achieved, 2),
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+           "traffic": traffic(kernels) if traffic else None}
+    if traffic_at:
+        out["traffic_at"] = traffic_at
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported (worlds shard across "
+              "GPUs with no per-step exchange, so the multi-GPU figure is N x the one-GPU figure by construction)",
+              file=sys.stderr)
+        sys.exit(2)
+    if args.copy_probe_mib > 0:
+        copy_probe(args.copy_probe_mib)
+        return
+    uneven_job = start_uneven_build(args)
+    import torch
+
+    from newton_custom_contact_amd import CollisionPipeline, scenes
+    from newton_custom_contact_amd.solvers import SolverXPBD
+
+    dev = "cuda:0"
+    t0 = time.time()
+    model = scenes.mixed_worlds(args.worlds, seed=args.seed, device=dev)
+    print(f"[bench] scene: {args.worlds} worlds, {model.body_count} bodies, {model.shape_count} shapes "
+          f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    pipe = CollisionPipeline(model, broad_phase="sap")
+    solver = SolverXPBD(model, iterations=args.iterations)
+    sdt = (1.0 / 60.0) / args.substeps
+    states, contacts, ms, n_c, t_col_ms, t_xp_ms, _ = timed_form(args, model, pipe, solver, args.worlds,
+                                                                  args.settle, args.warmup, args.steps, dev)
+    ms_per_step = ms / args.steps
+    ws_per_s = args.worlds * args.steps / (ms / 1e3)
+    nonfinite = int((~torch.isfinite(states[0].body_qd).all(dim=1)).sum())
+    print(f"[bench] timed {args.steps} steps after {args.settle} settle + {args.warmup} warmup: {ms:.2f} ms, "
+          f"N_c={n_c:.1f}, non-finite bodies {nonfinite}", file=sys.stderr, flush=True)
 
     N_b, N_s, N_j = 32, 33, 0
     b_collide = 28 * N_b + 64 * N_s + 76 * n_c + 4
     b_step = 76 * N_b + 96 * N_b + 16 * N_s + 76 * n_c + 52 * N_b + 120 * N_j
     b_ws = args.substeps * (b_collide + b_step)
     roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
-    collide_kernels = ("collide_segment", "route_scan_kernel", "narrow_kernel", "contact_count_kernel",
-                       "contact_scan_kernel", "contact_compact_kernel")
-    if t_xp_ms >= t_col_ms:
-        dom, dom_ms, dom_bytes = "xpbd_world_kernel", t_xp_ms, args.worlds * b_step
-        dom_kernels = ("xpbd_world_kernel",)
-    else:
-        dom, dom_ms, dom_bytes = "collide phase: " + " + ".join(collide_kernels), t_col_ms, args.worlds * b_collide
-        dom_kernels = collide_kernels
-    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
+    roof = roofline_block(t_col_ms, t_xp_ms, args.worlds, b_collide, b_step,
+                          traffic=lambda k: measured_traffic(args, k),
+                          traffic_at="rocprofv3 FETCH_SIZE + WRITE_SIZE passes at the settled state (N_c 123.4), "
+                                     "profiles/mixed4096_hbm_traffic.md")
+
+    iterations = None
+    if not args.no_learned:
+        iterations = iterations_form(args, model, states[0], dev)
 
     learned = None
     if not args.no_learned:
@@ -205,13 +286,17 @@ def main():
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(model, states[0], args, sdt)
 
+    single = None
+    if not args.no_single:
+        single = single_world_form(args, dev)
+
     sdf = None
     if not args.no_sdf:
         sdf = sdf_form(args, dev)
 
     uneven = None
-    if args.uneven:
-        uneven = uneven_form(args, dev)
+    if uneven_job is not None:
+        uneven = uneven_form(args, dev, uneven_job)
 
     components = None
     if not args.no_components:
@@ -233,33 +318,105 @@ def main():
         "config": {
             "workload": f"mixed primitives: {args.worlds} worlds x 32 (11 spheres, 11 boxes, 10 capsules) on a "
                         f"ground plane, dt=1/60 s, {args.substeps} substeps, {args.iterations} XPBD iterations, "
-                        f"{'hipGraph' if graph is not None else 'eager'} step",
+                        f"{'hipGraph' if not args.no_graph else 'eager'} step, timed after {args.settle} untimed "
+                        f"settle steps + {args.warmup} warmup steps",
             "worlds": args.worlds,
             "bodies_per_world": N_b,
+            "settle_steps": args.settle,
             "mean_contacts_per_world_substep": round(n_c, 2),
+            "non_finite_bodies_at_end": nonfinite,
             "bytes_per_world_step": round(b_ws, 1),
+            "bytes_per_world_substep": {"collide": round(b_collide, 1), "step": round(b_step, 1)},
             "hbm_roofline_world_steps_per_s": round(roof_ws, 1),
             "fraction_of_step_roofline": round(ws_per_s / roof_ws, 5),
             "collide_ms_per_substep": round(t_col_ms, 4),
             "xpbd_ms_per_substep": round(t_xp_ms, 4),
         },
-        "roofline": {
-            "kernel": dom,
-            "launch_ms": round(dom_ms, 4),
-            "bound": "hbm",
-            "achieved": round(achieved, 2),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5),
-            "traffic": measured_traffic(args, dom_kernels),
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
+        "iterations": iterations,
         "learned": learned,
+        "single_world": single,
         "sdf": sdf,
         "uneven": uneven,
         "components": components,
     }
     print(json.dumps(line), flush=True)
+
+
+def iterations_form(args, model, state, dev):
+    """Iterations to a penetration residual at the settled state.  Residual of one substep = mean over worlds
+    of the world's largest remaining penetration max(0, -d) after the solve, on the same state and contacts for
+    every iteration count: plain 1..iters_max iterations, learned warm start 1..10 warm iterations (no fallback,
+    the trained network)."""
+    import torch
+
+    from newton_custom_contact_amd import CollisionPipeline
+    from newton_custom_contact_amd.learned import WEIGHTS_PATH, SolverXPBDWarmStart, WarmStartMLP
+
+    if not os.path.exists(WEIGHTS_PATH):
+        return None
+    mlp = WarmStartMLP.load(WEIGHTS_PATH, device=dev)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    kmax = max(args.iters_max, args.iterations)
+    solver = SolverXPBDWarmStart(model, mlp, iterations=kmax, warm_iterations=3, residual_tol=float("inf"))
+    sdt = (1.0 / 60.0) / args.substeps
+    W = model.world_count
+    c_prev, c = pipe.contacts(), pipe.contacts()
+    tmp = model.state()
+    # the previous substep's contacts and multipliers for the "previous multiplier" feature
+    a = model.state()
+    a.body_q.copy_(state.body_q)
+    a.body_qd.copy_(state.body_qd)
+    pipe.collide(a, c_prev)
+    solver.log_step(a, tmp, None, c_prev, sdt, iterations=args.iterations)
+    a.body_q.copy_(tmp.body_q)
+    a.body_qd.copy_(tmp.body_qd)
+    pipe.collide(a, c)
+    plain = []
+    for k in range(1, kmax + 1):
+        solver.log_step(a, tmp, None, c, sdt, iterations=k, remember=False)
+        plain.append(float(solver.buffers["residual"][:W].mean()))
+    learned = []
+    for k in range(1, min(10, kmax) + 1):
+        solver.step(a, tmp, None, c, sdt, warm_iterations=k, residual_tol=float("inf"), remember=False)
+        learned.append(float(solver.buffers["residual"][:W].mean()))
+    torch.cuda.synchronize()
+    p10 = plain[args.iterations - 1]
+
+    def first_below(seq, thr):
+        for k, r in enumerate(seq, 1):
+            if r <= thr:
+                return k
+        return f"not reached within {len(seq)} iterations (smallest {min(seq):.3e} m)"
+
+    out = {
+        "state": f"settled plain workload, {W} worlds, one substep of dt {sdt:.6f} s from the same state and contacts",
+        "residual_m_by_iterations": {"plain": plain, "learned_warm_start": learned},
+        "plain_10_iteration_residual_m": p10,
+        "iters_to_1e-4_m": {"plain": first_below(plain, 1e-4), "learned": first_below(learned, 1e-4)},
+        "iters_to_plain10_residual": {"plain": first_below(plain, p10), "learned": first_below(learned, p10)},
+    }
+    print(f"[bench] iterations: {json.dumps(out)}", file=sys.stderr, flush=True)
+    return out
+
+
+def single_world_form(args, dev):
+    """The one-world case: 64 boxes (edge 0.1-0.5 m) dropped on the ground, 120 steps timed after settling (a
+    latency-bound launch sequence; the throughput metric is for the batched forms)."""
+    from newton_custom_contact_amd import CollisionPipeline, scenes
+    from newton_custom_contact_amd.solvers import SolverXPBD
+
+    model = scenes.box_pile(64, seed=args.seed, device=dev)
+    pipe = CollisionPipeline(model, broad_phase="sap")
+    solver = SolverXPBD(model, iterations=args.iterations)
+    steps = 120
+    _st, _c, ms, n_c, t_col, t_xp, _ = timed_form(args, model, pipe, solver, 1, args.settle, 5, steps, dev)
+    return {"value": round(steps / (ms / 1e3), 1), "unit": "world-steps/s", "ms_per_step": round(ms / steps, 4),
+            "workload": "1 world x 64 boxes (edge 0.1-0.5 m) on a ground plane, dt=1/60 s, 4 substeps, "
+                        f"{args.iterations} iterations, hipGraph step",
+            "mean_contacts_per_world_substep": round(n_c, 2),
+            "collide_ms_per_substep": round(t_col, 4), "xpbd_ms_per_substep": round(t_xp, 4)}
 
 
 def learned_form(args, model, dev, plain_ws_per_s):
@@ -312,7 +469,7 @@ def learned_form(args, model, dev, plain_ws_per_s):
         with torch.cuda.graph(graph):
             step()
         torch.cuda.synchronize()
-    for _ in range(args.warmup):
+    for _ in range(args.settle + args.warmup):
         graph.replay() if graph is not None else step()
     torch.cuda.synchronize()
     res_acc.zero_()
@@ -461,108 +618,77 @@ def components_form(args, model, state, sdt, dev):
     return out
 
 
-def uneven_form(args, dev):
+def uneven_form(args, dev, job):
     """Uneven worlds: ``--uneven-worlds`` worlds of log-uniform 8-256 primitives (spheres, boxes, capsules) plus
     a 6-link revolute capsule chain with +-90 degree limits each, on a ground plane; 4 substeps, 10 iterations,
-    the step kernel's launch shape autotuned, hipGraph step, timed after ``--uneven-warmup`` steps."""
-    import torch
-
-    from newton_custom_contact_amd import CollisionPipeline, scenes
+    the step kernel's launch shape autotuned on the settled state, hipGraph step, timed after
+    ``--uneven-settle`` steps.  The scene was built on the host by a child process started at the beginning."""
+    from newton_custom_contact_amd import CollisionPipeline
     from newton_custom_contact_amd.autotune import tune_block_lanes
+    from newton_custom_contact_amd.model import Model
     from newton_custom_contact_amd.solvers import SolverXPBD
 
+    proc, path, t_start = job
     t0 = time.time()
-    model = scenes.uneven_worlds(args.uneven_worlds, seed=args.seed, device=dev,
-                                 progress=lambda k, n: print(f"[bench] uneven scene: {k}/{n} worlds built",
-                                                             file=sys.stderr, flush=True))
+    rc = proc.wait()
+    if rc != 0 or not os.path.exists(path):
+        print(f"[bench] uneven scene build failed (exit {rc})", file=sys.stderr, flush=True)
+        return None
+    model = Model.load(path, dev)
+    try:
+        os.remove(path)
+        os.rmdir(os.path.dirname(path))
+    except OSError:
+        pass
     W = args.uneven_worlds
-    print(f"[bench] uneven scene: {W} worlds, {model.body_count} bodies, {model.joint_count} joints "
-          f"({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    print(f"[bench] uneven scene: {W} worlds, {model.body_count} bodies, {model.joint_count} joints (host build "
+          f"started {t0 - t_start:.1f}s before this form; waited {time.time() - t0:.1f}s for it and the upload)",
+          file=sys.stderr, flush=True)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=args.iterations)
-    contacts = pipe.contacts()
-    states = [model.state(), model.state()]
-    sdt = (1.0 / 60.0) / args.substeps
-    nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
-
-    def step():
-        for s in range(args.substeps):
-            a, b = states[s % 2], states[(s + 1) % 2]
-            pipe.collide(a, contacts)
-            solver.step(a, b, None, contacts, sdt)
-            nc_acc.add_(contacts.rigid_contact_count)
-        if args.substeps % 2:
-            states[0].body_q.copy_(states[1].body_q)
-            states[0].body_qd.copy_(states[1].body_qd)
-
-    for _ in range(args.uneven_warmup):
-        step()
-    pipe.collide(states[0], contacts)
-    tune = tune_block_lanes(solver, states[0], states[1], contacts, sdt)
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        step()
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    graph = None
-    if not args.no_graph:
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
-            step()
-        torch.cuda.synchronize()
-    for _ in range(3):
-        graph.replay() if graph is not None else step()
-    torch.cuda.synchronize()
-    nc_acc.zero_()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.uneven_steps):
-        graph.replay() if graph is not None else step()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
+    tune = lambda a, b, c, sdt: tune_block_lanes(solver, a, b, c, sdt)  # noqa: E731
+    states, contacts, ms, n_c, t_col, t_xp, tuned = timed_form(args, model, pipe, solver, W, args.uneven_settle,
+                                                                3, args.uneven_steps, dev, tune=tune)
     ws = W * args.uneven_steps / (ms / 1e3)
-    n_c = float(nc_acc.item()) / (args.uneven_steps * args.substeps * W)
-    stream = torch.cuda.current_stream()
-    reps = args.kernel_reps
-    a, b = states[0], states[1]
-    pipe.collide(a, contacts)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ev[0].record(stream)
-    for _ in range(reps):
-        pipe.collide(a, contacts)
-    ev[1].record(stream)
-    ev[2].record(stream)
-    for _ in range(reps):
-        solver.step(a, b, None, contacts, sdt)
-    ev[3].record(stream)
-    torch.cuda.synchronize()
+    h = model.host
+    n_b = model.body_count / W
+    n_s = (model.shape_count - int((h["shape_world"] == -1).sum())) / W + int((h["shape_world"] == -1).sum())
+    n_j = model.joint_count / W
+    revolute = int((h["joint_type"] == 1).sum()) / W
+    b_collide = 28 * n_b + 64 * n_s + 76 * n_c + 4
+    b_step = 76 * n_b + 96 * n_b + 16 * n_s + 76 * n_c + 52 * n_b + 120 * revolute
+    b_ws = args.substeps * (b_collide + b_step)
+    roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
     oc = pipe.overflow_counters()
     print(f"[bench] uneven: {ws:.1f} world-steps/s, N_c={n_c:.1f}", file=sys.stderr, flush=True)
-    del graph
     return {
         "value": round(ws, 1),
         "unit": "world-steps/s",
         "ms_per_step": round(ms / args.uneven_steps, 4),
         "workload": f"{W} worlds of log-uniform 8-256 primitives + a 6-link revolute chain each "
                     f"({model.body_count} bodies, {model.joint_count} joints) on a ground plane, dt=1/60 s, "
-                    f"{args.substeps} substeps, {args.iterations} iterations, timed after {args.uneven_warmup} steps",
+                    f"{args.substeps} substeps, {args.iterations} iterations, timed after {args.uneven_settle} "
+                    "untimed settle steps",
+        "mean_bodies_per_world": round(n_b, 2),
+        "mean_revolute_joints_per_world": round(revolute, 2),
         "mean_contacts_per_world_substep": round(n_c, 2),
-        "collide_ms_per_substep": round(ev[0].elapsed_time(ev[1]) / reps, 4),
-        "xpbd_ms_per_substep": round(ev[2].elapsed_time(ev[3]) / reps, 4),
+        "bytes_per_world_step": round(b_ws, 1),
+        "hbm_roofline_world_steps_per_s": round(roof_ws, 1),
+        "fraction_of_step_roofline": round(ws / roof_ws, 5),
+        "collide_ms_per_substep": round(t_col, 4),
+        "xpbd_ms_per_substep": round(t_xp, 4),
+        "roofline": roofline_block(t_col, t_xp, W, b_collide, b_step),
         "overflow": oc,
-        "step_kernel_lanes": tune,
+        "step_kernel_lanes": tuned,
     }
 
 
 def sdf_form(args, dev):
     """The triangle-mesh workload: ``--sdf-worlds`` worlds x 8 convex-hull meshes (64-256 points on a sphere
     of radius 0.1-0.3 m, from a seeded library of 64 hulls, 64^3 FP32 SDF each) dropped on a ground plane,
-    mesh-mesh and mesh-plane contacts (at most 64 per pair), 4 substeps, 10 iterations, hipGraph step.
-    Timed after ``--sdf-warmup`` steps, when the hulls have landed and pile up."""
+    mesh-mesh and mesh-plane contacts with the reference's contact reduction (at most 240 per pair), 4 substeps,
+    10 iterations, hipGraph step; the step kernel's launch shape autotuned on the settled state.  Timed after
+    ``--sdf-settle`` untimed steps, when the hulls have landed and pile up."""
     import torch
 
     from newton_custom_contact_amd import CollisionPipeline, scenes
@@ -580,68 +706,15 @@ def sdf_form(args, dev):
           f"(library + cook {t_cook:.1f}s, total {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=args.iterations)
-    contacts = pipe.contacts()
-    states = [model.state(), model.state()]
-    sdt = (1.0 / 60.0) / args.substeps
-    nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
-
-    def step():
-        for s in range(args.substeps):
-            a, b = states[s % 2], states[(s + 1) % 2]
-            pipe.collide(a, contacts)
-            solver.step(a, b, None, contacts, sdt)
-            nc_acc.add_(contacts.rigid_contact_count)
-        if args.substeps % 2:
-            states[0].body_q.copy_(states[1].body_q)
-            states[0].body_qd.copy_(states[1].body_qd)
-
-    # land the hulls (eager), pick the step kernel's launch shape on that state, then capture the step
-    for _ in range(args.sdf_warmup):
-        step()
-    pipe.collide(states[0], contacts)
-    tune = tune_block_lanes(solver, states[0], states[1], contacts, sdt)
-    print(f"[bench] sdf: step kernel launch shapes {tune}", file=sys.stderr, flush=True)
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        step()
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    graph = None
-    if not args.no_graph:
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
-            step()
-        torch.cuda.synchronize()
-    for _ in range(5):
-        graph.replay() if graph is not None else step()
-    torch.cuda.synchronize()
-    nc_acc.zero_()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(args.sdf_steps):
-        graph.replay() if graph is not None else step()
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1)
+    tune = lambda a, b, c, sdt: tune_block_lanes(solver, a, b, c, sdt)  # noqa: E731
+    states, contacts, ms, n_c, t_col, t_xp, tuned = timed_form(args, model, pipe, solver, args.sdf_worlds,
+                                                                args.sdf_settle, 5, args.sdf_steps, dev, tune=tune)
     ws = args.sdf_worlds * args.sdf_steps / (ms / 1e3)
-    n_c = float(nc_acc.item()) / (args.sdf_steps * args.substeps * args.sdf_worlds)
-    stream = torch.cuda.current_stream()
-    reps = args.kernel_reps
-    a, b = states[0], states[1]
-    pipe.collide(a, contacts)
-    torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ev[0].record(stream)
-    for _ in range(reps):
-        pipe.collide(a, contacts)
-    ev[1].record(stream)
-    ev[2].record(stream)
-    for _ in range(reps):
-        solver.step(a, b, None, contacts, sdt)
-    ev[3].record(stream)
-    torch.cuda.synchronize()
+    n_b, n_s = 8, 9
+    b_collide = 28 * n_b + 64 * n_s + 76 * n_c + 4
+    b_step = 76 * n_b + 96 * n_b + 16 * n_s + 76 * n_c + 52 * n_b
+    b_ws = args.substeps * (b_collide + b_step)
+    roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
     oc = pipe.overflow_counters()
     print(f"[bench] sdf: {ws:.1f} world-steps/s, N_c={n_c:.1f}", file=sys.stderr, flush=True)
     return {
@@ -649,13 +722,17 @@ def sdf_form(args, dev):
         "unit": "world-steps/s",
         "ms_per_step": round(ms / args.sdf_steps, 4),
         "workload": f"{args.sdf_worlds} worlds x 8 convex-hull meshes (64-hull library, 64^3 fp32 SDF) on a ground "
-                    f"plane, <=64 contacts per pair, dt=1/60 s, {args.substeps} substeps, {args.iterations} "
-                    f"iterations, timed after {args.sdf_warmup} steps",
+                    f"plane, reference contact reduction, dt=1/60 s, {args.substeps} substeps, {args.iterations} "
+                    f"iterations, timed after {args.sdf_settle} untimed settle steps",
         "mean_contacts_per_world_substep": round(n_c, 2),
-        "collide_ms_per_substep": round(ev[0].elapsed_time(ev[1]) / reps, 4),
-        "xpbd_ms_per_substep": round(ev[2].elapsed_time(ev[3]) / reps, 4),
+        "bytes_per_world_step": round(b_ws, 1),
+        "hbm_roofline_world_steps_per_s": round(roof_ws, 1),
+        "fraction_of_step_roofline": round(ws / roof_ws, 5),
+        "collide_ms_per_substep": round(t_col, 4),
+        "xpbd_ms_per_substep": round(t_xp, 4),
+        "roofline": roofline_block(t_col, t_xp, args.sdf_worlds, b_collide, b_step),
         "overflow": oc,
-        "step_kernel_lanes": tune,
+        "step_kernel_lanes": tuned,
         "lowest_body_z_m": round(float(states[0].body_q[:, 2].min()), 4),
     }
 
@@ -693,7 +770,17 @@ def cpu_baseline(model, state, args, sdt):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
+    # one GPU's share of the host is 16 cores, whatever the machine has
     threads = max(1, min(16, cores, args.worlds // n_w))
+    model_name = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model_name = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
 
     def make(k):
         sl = slice(k * n_w * nb, (k + 1) * n_w * nb)
@@ -734,6 +821,8 @@ def cpu_baseline(model, state, args, sdt):
         "value": round(vn, 2),
         "unit": "world-steps/s",
         "cores": threads,
+        "host": {"cpu_model": model_name, "logical_cpus": os.cpu_count(), "usable_cpus": cores,
+                 "threads_used": threads, "note": "one GPU's share of the host is 16 cores"},
         "kind": "port",
         "sample": f"{threads} threads x {n_w} worlds each from the GPU run's timed state, C++ oracle, "
                   f"{sn} world-batch steps in {en:.1f} s; one thread: {v1:.2f} world-steps/s "

@@ -94,11 +94,16 @@ def quat_mul(a, b) -> np.ndarray:
 
 
 def quat_rotate(q, v) -> np.ndarray:
-    q = np.asarray(q, dtype=np.float64)
-    v = np.asarray(v, dtype=np.float64)
-    u = q[:3]
-    w = q[3]
-    return v * (2.0 * w * w - 1.0) + u * (2.0 * np.dot(u, v)) + np.cross(u, v) * (2.0 * w)
+    """v (2 w^2 - 1) + u (2 u.v) + (u x v) 2 w for q = (u, w), in float64 scalar arithmetic (3-vectors are too
+    small for numpy's per-call cost)."""
+    x, y, z, w = (float(c) for c in q)
+    vx, vy, vz = (float(c) for c in v)
+    a = 2.0 * w * w - 1.0
+    d = 2.0 * (x * vx + y * vy + z * vz)
+    w2 = 2.0 * w
+    return np.array([vx * a + x * d + (y * vz - z * vy) * w2,
+                     vy * a + y * d + (z * vx - x * vz) * w2,
+                     vz * a + z * d + (x * vy - y * vx) * w2])
 
 
 def quat_from_axis_angle(axis, angle) -> np.ndarray:
@@ -128,7 +133,13 @@ def quat_between_vectors(a, b) -> np.ndarray:
 
 
 def quat_to_matrix(q) -> np.ndarray:
-    return np.stack([quat_rotate(q, e) for e in np.eye(3)], axis=1)
+    """Columns: the rotated unit vectors."""
+    x, y, z, w = (float(c) for c in q)
+    a = 2.0 * w * w - 1.0
+    w2 = 2.0 * w
+    return np.array([[a + 2.0 * x * x, 2.0 * x * y - z * w2, 2.0 * x * z + y * w2],
+                     [2.0 * y * x + z * w2, a + 2.0 * y * y, 2.0 * y * z - x * w2],
+                     [2.0 * z * x - y * w2, 2.0 * z * y + x * w2, a + 2.0 * z * z]])
 
 
 def transform_identity() -> np.ndarray:

@@ -172,6 +172,32 @@ class Model:
         m._finish()
         return m
 
+    _COUNTS = ("body_count", "shape_count", "joint_count", "joint_dof_count", "joint_coord_count", "world_count")
+
+    def save(self, path: str) -> None:
+        """The host arrays and counts of a model without mesh shapes, as one .npz (plain arrays; a model built on
+        the host of one process can be handed to another)."""
+        if getattr(self, "meshes", []):
+            raise ValueError("Model.save: models with mesh shapes are not supported")
+        meta = np.asarray([getattr(self, k) for k in self._COUNTS], np.int64)
+        up = np.asarray([str(getattr(self, "up_axis", "Z"))])
+        np.savez(path, _counts=meta, _up_axis=up, **self.host)
+
+    @classmethod
+    def load(cls, path: str, device=None) -> Model:
+        """Inverse of Model.save: the arrays are read without unpickling and the device tables rebuilt."""
+        m = cls(device)
+        with np.load(path, allow_pickle=False) as z:
+            for k, v in zip(cls._COUNTS, z["_counts"].tolist()):
+                setattr(m, k, int(v))
+            m.up_axis = str(z["_up_axis"][0])
+            m.host = {k: z[k] for k in z.files if not k.startswith("_")}
+        fp = m.host["shape_filter_pairs"].astype(np.int64)
+        m.shape_collision_filter_pairs = {(int(a), int(b)) for a, b in fp}
+        m.meshes = []
+        m._finish()
+        return m
+
     @staticmethod
     def voxel_resolution(lower, upper, budget: int = 100) -> tuple[int, int, int]:
         """Voxel grid of a shape's contact reduction (compute_voxel_resolution_from_aabb, builder.py:9936-9961):

@@ -48,37 +48,52 @@ class _Placer:
     def __init__(self, rng: np.random.Generator, z_hi: float = 2.5):
         self.rng = rng
         self.z_hi = z_hi
-        self.pts = np.zeros((0, 3))
-        self.rad = np.zeros(0)
+        self._pts = np.zeros((16, 3))
+        self._rad = np.zeros(16)
+        self.n = 0
+
+    @property
+    def pts(self) -> np.ndarray:
+        return self._pts[:self.n]
+
+    @property
+    def rad(self) -> np.ndarray:
+        return self._rad[:self.n]
+
+    _MOD = 1 << 128  # PCG64 advances modulo its 128-bit state: advancing by _MOD - n steps it back n draws
 
     def place(self, radius: float) -> np.ndarray:
         """Up to 2000 candidates (x, y, z uniform, drawn in that order) per height band; the first whose sphere
-        clears every placed one is taken.  Candidates are tested in batches; the generator is then left
-        exactly after the taken candidate's three draws, as if they had been drawn one by one."""
+        clears every placed one is taken.  Candidates are tested in batches; the generator is then stepped back to
+        just after the taken candidate's three draws, as if they had been drawn one by one."""
         bg = self.rng.bit_generator
         while True:
             tried = 0
             while tried < 2000:
                 k = min(64, 2000 - tried)
-                state = bg.state
                 u = self.rng.random((k, 3))
                 cand = np.empty((k, 3))
                 cand[:, 0] = -1.0 + 2.0 * u[:, 0]
                 cand[:, 1] = -1.0 + 2.0 * u[:, 1]
                 cand[:, 2] = 0.5 + (self.z_hi - 0.5) * u[:, 2]
-                if len(self.pts):
-                    ok = np.all(np.linalg.norm(self.pts[None, :, :] - cand[:, None, :], axis=2) > radius + self.rad,
-                                axis=1)
+                if self.n:
+                    diff = self.pts[None, :, :] - cand[:, None, :]
+                    ok = np.all(np.sqrt(np.einsum("kni,kni->kn", diff, diff)) > radius + self.rad, axis=1)
                 else:
                     ok = np.ones(k, bool)
                 hit = np.flatnonzero(ok)
                 if hit.size:
                     i = int(hit[0])
-                    bg.state = state
-                    bg.advance(3 * (i + 1))
+                    back = 3 * (k - i - 1)
+                    if back:
+                        bg.advance(self._MOD - back)
                     p = cand[i].copy()
-                    self.pts = np.vstack([self.pts, p])
-                    self.rad = np.append(self.rad, radius)
+                    if self.n == len(self._rad):
+                        self._pts = np.concatenate([self._pts, np.zeros_like(self._pts)])
+                        self._rad = np.concatenate([self._rad, np.zeros_like(self._rad)])
+                    self._pts[self.n] = p
+                    self._rad[self.n] = radius
+                    self.n += 1
                     return p
                 tried += k
             self.z_hi += 0.25
