@@ -167,6 +167,10 @@ typedef struct NmcXpbdParams {
   int has_contacts; /* 0: step(..., contacts=None, ...) */
   int block_lanes;  /* lanes per world: 0 = automatic (64 up to 64 bodies, else 256), 64 or 256; the
                      * result does not depend on it (an autotuner picks the faster one per model) */
+  int reference_arithmetic; /* 0 (default): contact rows and body updates in world-frame arithmetic (rotation
+                             * matrices, world inverse inertia, FMA, hardware reciprocal), within rounding of
+                             * the reference; 1: the reference's operation order, bit-identical to a CPU
+                             * sweep compiled without contraction.  Same method, same iteration order. */
 } NmcXpbdParams;
 
 /* Learned warm start of the XPBD contact solve (no counterpart in the reference, whose contact solve

@@ -64,7 +64,7 @@ class NmcXpbdParams(ctypes.Structure):
         ("rigid_contact_con_weighting", ctypes.c_int), ("angular_damping", ctypes.c_float),
         ("joint_linear_relaxation", ctypes.c_float), ("joint_angular_relaxation", ctypes.c_float),
         ("joint_linear_compliance", ctypes.c_float), ("joint_angular_compliance", ctypes.c_float),
-        ("has_contacts", ctypes.c_int), ("block_lanes", ctypes.c_int),
+        ("has_contacts", ctypes.c_int), ("block_lanes", ctypes.c_int), ("reference_arithmetic", ctypes.c_int),
     ]
 
 

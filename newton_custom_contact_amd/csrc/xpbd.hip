@@ -13,6 +13,7 @@
 
 #include "nmc_internal.h"
 #include "nmc_math.h"
+#include "xpbd_fast.h"
 
 namespace nmc {
 
@@ -122,12 +123,13 @@ __device__ __forceinline__ float angular_correction(float err, float derr, qf qa
 // Per-world LDS views of body data (local body index).
 struct WorldLds {
   float* q;      // 7
-  float* qd;     // 6
+  float* qd;     // 6 (slot nb: zero, the static side of a contact)
   float* com;    // 3
   float* inv_m;  // 1 (effective)
   float* inv_I;  // 9 (effective)
   float* I;      // 9
-  int* flags;
+  int* flags;    // (slot nb: 0)
+  float4* F;     // world-frame arithmetic: fastm::FRAME_F4 per body, slot nb static
 };
 
 __device__ __forceinline__ xform lds_q(const WorldLds& L, int b) {
@@ -499,6 +501,162 @@ __device__ __forceinline__ void lds_store_state(const WorldLds& L, int lb, const
   store3(L.qd + 6 * lb + 3, s.w);
 }
 
+// ---- world-frame arithmetic (FAST kernels)
+
+// Frame of body lb from its LDS state.
+__device__ __forceinline__ void frame_from_state(const WorldLds& L, int lb) {
+  using namespace fastm;
+  const xform X = lds_q(L, lb);
+  float4 r0, r1, r2;
+  rot_rows(X.q, r0, r1, r2);
+  r0.w = X.p.x;
+  r1.w = X.p.y;
+  r2.w = X.p.z;
+  const f3 c = fpoint(r0, r1, r2, load3(L.com + 3 * lb));
+  frame_store(L.F + FRAME_F4 * lb, r0, r1, r2, c, lds_m33(L.inv_I + 9 * lb), L.inv_m[lb]);
+}
+
+// solve_body_contact_positions (kernels.py:2062-2299) for one contact in world-frame arithmetic: the normal
+// row, then friction (offset contact points, tangential drift plus kinematic surface velocity), torsional
+// and rolling friction, each clamped by the normal multiplier as in the reference.  Body indices la/lb of
+// -1 read the static frame at slot nb.  db.lin = -da.lin (exactly, as in the reference).
+__device__ __forceinline__ bool contact_solve_fast(const WorldLds& L, const CReg& r, int nb, float relax, float dt,
+                                                   Spatial& da, Spatial& db, float& lam_out) {
+  using namespace fastm;
+  const int ia = r.la >= 0 ? r.la : nb, ib = r.lb >= 0 ? r.lb : nb;
+  const float4* A = L.F + FRAME_F4 * ia;
+  const float4* B = L.F + FRAME_F4 * ib;
+  const float4 a0 = A[0], a1 = A[1], a2 = A[2], b0 = B[0], b1 = B[1], b2 = B[2];
+  f3 bx_a = fpoint(a0, a1, a2, r.p0);
+  f3 bx_b = fpoint(b0, b1, b2, r.p1);
+  const f3 n = r.n;
+  const float d = dot3(n, bx_b - bx_a) - (r.m0 + r.m1);
+  if (d >= 0.0f) return false;
+  const float4 ca = A[3], cb = B[3], wa0 = A[4], wa1 = A[5], wb0 = B[4], wb1 = B[5];
+  const f3 ca3 = xyz(ca), cb3 = xyz(cb);
+  const f3 ang_a = cross3(n, bx_a - ca3);  // -(r_a x n)
+  const f3 ang_b = cross3(bx_b - cb3, n);
+  const float nn = dot3(n, n);
+  const float wn = fmaf(nn, ca.w, nn * cb.w) + quad(wa0, wa1, ang_a) + quad(wb0, wb1, ang_b);
+  const float lam_n = row_delta(d, wn, relax, dt);
+  lam_out = lam_n;
+  f3 lin = n * (-lam_n);
+  f3 anga = ang_a * lam_n, angb = ang_b * lam_n;
+  if (r.mu > 0.0f) {
+    bx_a = bx_a + fvec(a0, a1, a2, r.o0);
+    bx_b = bx_b + fvec(b0, b1, b2, r.o1);
+    const f3 delta = bx_b - bx_a;
+    f3 fd = axpy(-dot3(n, delta), n, delta);
+    const f3 ra = bx_a - ca3, rb = bx_b - cb3;
+    const int fa_flags = L.flags[ia], fb_flags = L.flags[ib];
+    if ((fa_flags | fb_flags) & BODY_KINEMATIC) {
+      f3 rel_v{0.0f, 0.0f, 0.0f};
+      if (fa_flags & BODY_KINEMATIC) {
+        const f3 va = load3(L.qd + 6 * ia) + cross3(load3(L.qd + 6 * ia + 3), ra);
+        rel_v = rel_v - axpy(-dot3(n, va), n, va);
+      }
+      if (fb_flags & BODY_KINEMATIC) {
+        const f3 vb = load3(L.qd + 6 * ib) + cross3(load3(L.qd + 6 * ib + 3), rb);
+        rel_v = rel_v + axpy(-dot3(n, vb), n, vb);
+      }
+      fd = axpy(dt, rel_v, fd);
+    }
+    const float s = dot3(fd, fd);
+    if (s > TINY_SQ) {
+      const float inv = rsq(s);
+      const f3 perp = fd * inv;
+      const f3 fa = cross3(perp, ra), fb = cross3(rb, perp);
+      const float pp = dot3(perp, perp);
+      const float wf = fmaf(pp, ca.w, pp * cb.w) + quad(wa0, wa1, fa) + quad(wb0, wb1, fb);
+      float lf = row_delta(s * inv, wf, relax, dt);
+      lf = fmax2(lf, -lam_n * r.mu);
+      lin = axpy(-lf, perp, lin);
+      anga = axpy(lf, fa, anga);
+      angb = axpy(lf, fb, angb);
+    }
+  }
+  if (r.mut > 0.0f || r.mur > 0.0f) {
+    const f3 dom = load3(L.qd + 6 * ib + 3) - load3(L.qd + 6 * ia + 3);
+    if (r.mut > 0.0f) {
+      const float err = dot3(dom, n) * dt;
+      if (fabsf(err) > 0.0f) {
+        const float wt = quad(wa0, wa1, n) + quad(wb0, wb1, n);
+        float lt = row_delta(err, wt, relax, dt);
+        lt = clampf(lt, -lam_n * r.mut, lam_n * r.mut);
+        anga = axpy(-lt, n, anga);
+        angb = axpy(lt, n, angb);
+      }
+    }
+    if (r.mur > 0.0f) {
+      const f3 dr = axpy(-dot3(n, dom), n, dom);
+      const float s = dot3(dr, dr);
+      if (s > TINY_SQ) {
+        const float inv = rsq(s);
+        const f3 rn = dr * inv;
+        const float wr = quad(wa0, wa1, rn) + quad(wb0, wb1, rn);
+        float lr = row_delta(s * inv * dt, wr, relax, dt);
+        lr = fmax2(lr, -lam_n * r.mur);
+        anga = axpy(-lr, rn, anga);
+        angb = axpy(lr, rn, angb);
+      }
+    }
+  }
+  da = Spatial{lin, anga};
+  db = Spatial{-lin, angb};
+  return true;
+}
+
+// apply_body_deltas (kernels.py:809-879) through the body's frame: the body-frame angular update with the
+// gyroscopic term, the quaternion step and renormalisation, the origin re-derived from the moved centre of
+// mass, the small-velocity clamps; then the new frame.
+__device__ __forceinline__ void apply_fast(const WorldLds& L, int lb, const Spatial& delta, bool has_weights,
+                                           float inv_weight, float dt) {
+  using namespace fastm;
+  float4* F = L.F + FRAME_F4 * lb;
+  const float4 cm = F[3];
+  const float inv_m = cm.w;
+  if (inv_m == 0.0f) return;
+  const float4 r0 = F[0], r1 = F[1], r2 = F[2];
+  const m33 J = lds_m33(L.inv_I + 9 * lb), I = lds_m33(L.I + 9 * lb);
+  const float weight = (has_weights && inv_weight > 0.0f) ? rcp(inv_weight) : 1.0f;
+  const f3 dp = delta.lin * (inv_m * weight);
+  const f3 dq = delta.ang * weight;
+  const f3 w0 = load3(L.qd + 6 * lb + 3);
+  const f3 wb = fvec_t(r0, r1, r2, w0);
+  const f3 dwb = fmv(J, fvec_t(r0, r1, r2, dq));
+  const f3 tb = cross3(dwb, fmv(I, wb + dwb)) + cross3(wb, fmv(I, dwb));
+  const f3 dw1 = fvec(r0, r1, r2, axpy(-dt, fmv(J, tb), dwb));
+  // q1 = normalize(q0 + (0.5 dt dw1, 0) * q0)
+  const float* qs = L.q + 7 * lb;
+  const f3 qv{qs[3], qs[4], qs[5]};
+  const float qw = qs[6];
+  const f3 h = dw1 * (0.5f * dt);
+  const f3 hv = axpy(qw, h, cross3(h, qv));
+  qf q1{qv.x + hv.x, qv.y + hv.y, qv.z + hv.z, qw - dot3(h, qv)};
+  const float ql = fmaf(q1.x, q1.x, fmaf(q1.y, q1.y, fmaf(q1.z, q1.z, q1.w * q1.w)));
+  if (ql > 0.0f) {
+    const float inv = rsq(ql);
+    q1 = qf{q1.x * inv, q1.y * inv, q1.z * inv, q1.w * inv};
+  } else {
+    q1 = qf{0.0f, 0.0f, 0.0f, 0.0f};
+  }
+  const f3 c1 = axpy(dt, dp, xyz(cm));
+  float4 n0, n1, n2;
+  rot_rows(q1, n0, n1, n2);
+  const f3 p1 = c1 - fvec(n0, n1, n2, load3(L.com + 3 * lb));
+  n0.w = p1.x;
+  n1.w = p1.y;
+  n2.w = p1.z;
+  f3 v1 = load3(L.qd + 6 * lb) + dp;
+  f3 w1 = w0 + dw1;
+  if (dot3(v1, v1) < 1.0e-8f) v1 = f3{0.0f, 0.0f, 0.0f};
+  if (dot3(w1, w1) < 1.0e-8f) w1 = f3{0.0f, 0.0f, 0.0f};
+  store_xform(L.q + 7 * lb, xform{p1, q1});
+  store3(L.qd + 6 * lb, v1);
+  store3(L.qd + 6 * lb + 3, w1);
+  frame_store(F, n0, n1, n2, c1, J, inv_m);
+}
+
 // Exclusive scan of cnt[0..n) into out[0..n] (out[n] = total) by the whole block.
 __device__ void block_exclusive_scan(const int* cnt, int* out, int n, int* tmp) {
   __shared__ int s_carry;
@@ -619,7 +777,10 @@ __device__ __forceinline__ bool warm_contrib(const WorldLds& L, const CReg& r, f
 // multiplier sums; XM_PREP joint forces + integration + features, integrated state out; XM_WARM from the
 // integrated state: warm-start pass, warm_iterations iterations, per-world residual check and cold
 // fallback (integrated state, full iteration count).
-template <int BS, int CPL, int BPL, int MODE>
+// FAST: contact rows and body updates in world-frame arithmetic (xpbd_fast.h); otherwise the reference's
+// operation order, bit-identical to a sequential CPU sweep compiled without contraction.  ART: the model has
+// joints other than FREE / DISTANCE (without them the joint solve is not compiled in, which frees registers).
+template <int BS, int CPL, int BPL, int MODE, bool FAST, bool ART>
 __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_articulated;
@@ -641,14 +802,15 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 
   WorldLds L;
   unsigned char* p = smem;
+  L.F = reinterpret_cast<float4*>(p); p += sizeof(float4) * fastm::FRAME_F4 * (mb + 1);
   L.q = reinterpret_cast<float*>(p); p += sizeof(float) * 7 * mb;
-  L.qd = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * mb;
+  L.qd = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * (mb + 1);
   float* fl = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * mb;
   L.com = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * mb;
   L.inv_m = reinterpret_cast<float*>(p); p += sizeof(float) * mb;
   L.inv_I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
   L.I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
-  L.flags = reinterpret_cast<int*>(p); p += sizeof(int) * mb;
+  L.flags = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* seg = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* cur = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* scan_tmp = reinterpret_cast<int*>(p); p += sizeof(int) * BS;
@@ -697,6 +859,12 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
     for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = q_src[7 * b + k];
     for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = qd_src[6 * b + k];
     cur[lb] = 0;
+  }
+  if (tid == 0) {
+    // slot nb: the static side of a contact
+    for (int k = 0; k < 6; ++k) L.qd[6 * nb + k] = 0.0f;
+    L.flags[nb] = 0;
+    if (FAST) fastm::frame_static(L.F + fastm::FRAME_F4 * nb);
   }
   for (int jl = tid; jl < nj; jl += BS) {
     int j = j0 + jl;
@@ -914,6 +1082,10 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
     }
     return;
   }
+  if (FAST) {
+    for (int lb = tid; lb < nb; lb += BS) frame_from_state(L, lb);
+    __syncthreads();
+  }
 
   const float relax = a.p.rigid_contact_relaxation;
   const bool weighted = a.p.rigid_contact_con_weighting != 0;
@@ -940,6 +1112,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
         int b = b0 + lb;
         for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = lrn.q_mid[7 * b + k];
         for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = lrn.qd_mid[6 * b + k];
+        if (FAST) frame_from_state(L, lb);
       }
 #pragma unroll
       for (int jj = 0; jj < CPL; ++jj) lamacc[jj] = 0.0f;
@@ -981,8 +1154,11 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
                 const int pc = cpos(a.c, cstart + c);
                 on = pc >= 0 && warm_contrib(L, r, lrn.lambda0[pc], da, db);
               } else {
-                on = r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu, r.mut,
-                                              r.mur, relax, dt, da, db, lam);
+                if (FAST)
+                  on = r.valid && contact_solve_fast(L, r, nb, relax, dt, da, db, lam);
+                else
+                  on = r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu,
+                                                r.mut, r.mur, relax, dt, da, db, lam);
               }
               if (MODE == XM_REPORT && a.rep.contact_impulse) {
                 // this iteration's shape-0 side delta (zero when inactive) and the contact's bodies
@@ -1059,9 +1235,13 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
           int lb = tid + r * BS;
           // the warm pass leaves bodies without a warm contact untouched (no renormalisation / clamp)
           if (lb < nb && !(warm && cnt[r] == 0.0f)) {
-            BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
-                                       lds_m33(L.inv_I + 9 * lb), acc[r], weighted, cnt[r], dt);
-            lds_store_state(L, lb, o);
+            if (FAST) {
+              apply_fast(L, lb, acc[r], weighted, cnt[r], dt);
+            } else {
+              BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb),
+                                         L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc[r], weighted, cnt[r], dt);
+              lds_store_state(L, lb, o);
+            }
           }
         }
       }
@@ -1069,9 +1249,11 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
         __syncthreads();
         continue;
       }
-      // ---------------- joints (kernels.py:1441-1943) + apply without weights
-      if (d.joint_count) {
-        if (articulated) {
+      // ---------------- joints (kernels.py:1441-1943) + apply without weights.  Without articulation the
+      // pass applies zero deltas (renormalise, re-derive the origin, clamp): the reference-order kernel runs
+      // it, the world-frame one skips it (it changes nothing beyond rounding after the contact apply).
+      if (d.joint_count && (ART || !FAST)) {
+        if (ART && articulated) {
           __syncthreads();
           for (int jl = tid; jl < nj; jl += BS) {
             int j = j0 + jl;
@@ -1100,7 +1282,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
           int b = b0 + lb;
           Spatial acc{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
           bool any = false;
-          if (articulated) {
+          if (ART && articulated) {
             for (int e = d.body_joint_start[b]; e < d.body_joint_start[b + 1]; ++e) {
               int ent = d.body_joint_entry[e];
               int jl = (ent >> 1) - j0;
@@ -1111,10 +1293,14 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
               any = true;
             }
           }
-          BodyState o = any ? apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb), L.inv_m[lb],
-                                           lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt)
-                            : apply_zero_delta(lds_state(L, lb), load3(L.com + 3 * lb), L.inv_m[lb]);
-          lds_store_state(L, lb, o);
+          if (FAST) {
+            if (any) apply_fast(L, lb, acc, false, 0.0f, dt);
+          } else {
+            BodyState o = any ? apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb),
+                                             L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt)
+                              : apply_zero_delta(lds_state(L, lb), load3(L.com + 3 * lb), L.inv_m[lb]);
+            lds_store_state(L, lb, o);
+          }
         }
       }
       __syncthreads();
@@ -1226,19 +1412,34 @@ size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints,
   size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 0;
   size_t ch = (size_t)bs * sh.cpl;
   size_t b = sizeof(float) * (7 + 6 + 6 + 3 + 1 + 9 + 9) * mb + sizeof(int) * mb + 2 * sizeof(int) * (mb + 1);
+  b += sizeof(float) * 4 * 6 * (mb + 1) + sizeof(float) * 6 + sizeof(int);  // frames (+ static slot), static qd, flags
   b += sizeof(int) * bs + sizeof(float) * 12 * ch + sizeof(int) * ch + sizeof(float) * 12 * mj + sizeof(int) * mj;
   b += sizeof(unsigned) * (size_t)key_cap;
   return (b + 15) & ~size_t(15);
 }
 
+template <int MODE, bool FAST, bool ART>
+static void launch_shape(const XpbdShape& sh, int worlds, size_t lds, hipStream_t stream, const XpbdArgs& a) {
+  if (sh.bs == 64)
+    hipLaunchKernelGGL((xpbd_world_kernel<64, 2, 1, MODE, FAST, ART>), dim3(worlds), dim3(64), lds, stream, a);
+  else if (sh.bpl == 1)
+    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 1, MODE, FAST, ART>), dim3(worlds), dim3(256), lds, stream, a);
+  else
+    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 2, MODE, FAST, ART>), dim3(worlds), dim3(256), lds, stream, a);
+}
+
+// reference_arithmetic picks the operation order; models whose joints are all FREE / DISTANCE (joint_lds 0)
+// run the kernels compiled without the joint solve
 template <int MODE>
 static void launch_mode(const XpbdShape& sh, int worlds, size_t lds, hipStream_t stream, const XpbdArgs& a) {
-  if (sh.bs == 64)
-    hipLaunchKernelGGL((xpbd_world_kernel<64, 2, 1, MODE>), dim3(worlds), dim3(64), lds, stream, a);
-  else if (sh.bpl == 1)
-    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 1, MODE>), dim3(worlds), dim3(256), lds, stream, a);
-  else
-    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 2, MODE>), dim3(worlds), dim3(256), lds, stream, a);
+  const bool art = a.joint_lds > 0;
+  if (a.p.reference_arithmetic) {
+    if (art) launch_shape<MODE, false, true>(sh, worlds, lds, stream, a);
+    else launch_shape<MODE, false, false>(sh, worlds, lds, stream, a);
+  } else {
+    if (art) launch_shape<MODE, true, true>(sh, worlds, lds, stream, a);
+    else launch_shape<MODE, true, false>(sh, worlds, lds, stream, a);
+  }
 }
 
 hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float* q_in, const float* qd_in,

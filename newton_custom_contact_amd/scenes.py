@@ -133,6 +133,30 @@ def box_pile(n_boxes: int = 64, seed: int = 0, mu: float = 0.5, device=None, wor
     return b.finalize(device)
 
 
+def box_stacks(n_stacks: int = 4, height: int = 6, seed: int = 0, mu: float = 0.5, device=None):
+    """One world of ``n_stacks`` towers of ``height`` seeded boxes resting on an infinite ground plane, 1 m apart:
+    each box (half extents 0.06-0.15 m across, 0.04-0.10 m tall) sits on the one below at a seeded horizontal
+    offset of up to a fifth of the smaller half extent and a seeded turn about the vertical."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = ModelBuilder()
+    b.add_ground_plane(cfg=_cfg(mu))
+    b.begin_world()
+    cfg = _cfg(mu)
+    for k in range(n_stacks):
+        x0, y0, z = float(k % 2), float(k // 2), 0.0
+        for _ in range(height):
+            hx, hy = rng.uniform(0.06, 0.15, size=2)
+            hz = rng.uniform(0.04, 0.10)
+            dx, dy = rng.uniform(-0.2, 0.2, size=2) * min(hx, hy)
+            yaw = rng.uniform(-np.pi, np.pi)
+            q = (0.0, 0.0, float(np.sin(0.5 * yaw)), float(np.cos(0.5 * yaw)))
+            body = b.add_body(xform=transform((x0 + dx, y0 + dy, z + hz), q))
+            b.add_shape_box(body, hx=hx, hy=hy, hz=hz, cfg=cfg)
+            z += 2.0 * hz
+    b.end_world()
+    return b.finalize(device)
+
+
 def _primitive_sizes(rng, n_sphere, n_box, n_capsule):
     sizes = []
     for _ in range(n_sphere):

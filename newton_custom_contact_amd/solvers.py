@@ -47,11 +47,14 @@ class SolverXPBD(SolverBase):
                  joint_angular_relaxation: float = 0.4, joint_linear_compliance: float = 0.0,
                  joint_angular_compliance: float = 0.0, rigid_contact_relaxation: float = 0.8,
                  rigid_contact_con_weighting: bool = True, angular_damping: float = 0.0,
-                 enable_restitution: bool = False, block_lanes: int = 0):
+                 enable_restitution: bool = False, block_lanes: int = 0, reference_arithmetic: bool = False):
         super().__init__(model)
         if block_lanes not in (0, 64, 256):
             raise ValueError("block_lanes must be 0 (automatic), 64 or 256")
         self.block_lanes = int(block_lanes)  # lanes per world of the step kernel (see autotune.tune_block_lanes)
+        # False: world-frame contact arithmetic (within rounding of the reference, the fast path); True: the
+        # reference's operation order, bit-identical to the CPU restatement
+        self.reference_arithmetic = bool(reference_arithmetic)
         if enable_restitution:
             raise NotImplementedError("restitution is not part of this build")
         self.iterations = int(iterations)
@@ -87,7 +90,7 @@ class SolverXPBD(SolverBase):
         return NmcXpbdParams(self.iterations, self.rigid_contact_relaxation, int(self.rigid_contact_con_weighting),
                              self.angular_damping, self.joint_linear_relaxation, self.joint_angular_relaxation,
                              self.joint_linear_compliance, self.joint_angular_compliance, int(has_contacts),
-                             int(getattr(self, "block_lanes", 0)))
+                             int(getattr(self, "block_lanes", 0)), int(getattr(self, "reference_arithmetic", False)))
 
     def step(self, state_in, state_out, control, contacts, dt: float) -> None:
         """solver_xpbd.py:259-764."""

@@ -45,3 +45,30 @@ def oracle_states(oracle, model, n_substeps: int, dt: float, iterations: int = 1
         if (i + 1) % every == 0:
             out.append((q.copy(), qd.copy()))
     return out
+
+
+# The solver's parity bar per step from identical inputs: 1e-5 relative, with the scene's length scale (1 m)
+# as the floor for components near zero -- poses (positions, quaternion components) within 1e-5 relative or
+# 1e-5 (m, quaternion units); velocities within 1e-5 relative or 1e-5 m per substep (XPBD velocities are
+# position corrections over dt; the FP32 oracle itself differs from its FP64 rerun by up to 1.5e-6 m and 5e-4 m/s
+# on these scenes).  Non-finite entries must be non-finite on both sides.
+STEP_RTOL = 1e-5
+Q_ATOL = 1e-5
+
+
+def assert_step_close(q, qd, q_ref, qd_ref, dt: float, what: str = "") -> None:
+    np.testing.assert_allclose(q, q_ref, rtol=STEP_RTOL, atol=Q_ATOL, equal_nan=True, err_msg=f"body_q {what}")
+    np.testing.assert_allclose(qd, qd_ref, rtol=STEP_RTOL, atol=Q_ATOL / dt, equal_nan=True,
+                               err_msg=f"body_qd {what}")
+
+
+def max_abs_diff(a, b) -> float:
+    """Largest |a - b| over the entries finite on both sides (0 when there are none)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    fin = np.isfinite(a) & np.isfinite(b)
+    return float(np.abs(a[fin] - b[fin]).max()) if fin.any() else 0.0
+
+
+def bit_equal(a, b) -> bool:
+    return bool(np.array_equal(a, b, equal_nan=True))

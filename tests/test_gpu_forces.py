@@ -1,8 +1,10 @@
 """Reported forces on the GPU: SolverXPBD.update_contacts (contacts.force) and State.body_parent_f.
 
-One step from identical inputs matches the oracle (forces within 1e-5 relative, in fact bit-identical);
-along a GPU trajectory the reference's known answers hold (static equilibrium: -m g per resting body, 1.5 m g
-under each bottom pyramid cube; a revolute pendulum reports m g in body_parent_f).
+With the reference-order kernel one step from identical inputs matches the oracle bit for bit (contact forces and
+body_parent_f); with either kernel the reference's known answers hold along a GPU trajectory (static equilibrium:
+-m g per resting body, 1.5 m g under each bottom pyramid cube; a revolute pendulum reports m g in body_parent_f).
+Per-contact forces of a resting body with several contacts are not unique (any split with the same total
+holds it), so the world-frame kernel is held to the per-body totals, not to the oracle's split.
 """
 
 from __future__ import annotations
@@ -19,12 +21,13 @@ from tests.test_oracle_forces import G, _equilibrium_model, _ground_forces
 pytestmark = pytest.mark.gpu
 
 
-def test_update_contacts_matches_oracle_and_equilibrium(cuda_device):
+@pytest.mark.parametrize("reference_arithmetic", [True, False], ids=["reference_order", "world_frame"])
+def test_update_contacts_matches_oracle_and_equilibrium(cuda_device, reference_arithmetic):
     model, bodies = _equilibrium_model(cuda_device)
     model.request_contact_attributes("force")
     o = oracle.Oracle(model)
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
-    solver = SolverXPBD(model, iterations=32)
+    solver = SolverXPBD(model, iterations=32, reference_arithmetic=reference_arithmetic)
     c = pipe.contacts()
     assert c.force is not None
     with pytest.raises(ValueError):
@@ -32,11 +35,10 @@ def test_update_contacts_matches_oracle_and_equilibrium(cuda_device):
     s0, s1 = model.state(), model.state()
     sub_dt = 1.0 / 60.0 / 8
     acc = {k: np.zeros(3) for k in bodies}
-    exact = True
     for step in range(140):
         for k in range(8):
             pipe.collide(s0, c)
-            check = step in (30, 139) and k == 7
+            check = reference_arithmetic and step in (30, 139) and k == 7
             if check:
                 torch.cuda.synchronize()
                 q, qd = s0.body_q.cpu().numpy(), s0.body_qd.cpu().numpy()
@@ -50,8 +52,7 @@ def test_update_contacts_matches_oracle_and_equilibrium(cuda_device):
                 got = c.numpy()
                 order = np.argsort(got["sort_key"], kind="stable")
                 gf = c.force[: got["count"]].cpu().numpy()[order]
-                np.testing.assert_allclose(gf, res[2], rtol=1e-5, atol=1e-3)
-                exact = exact and np.array_equal(gf, res[2])
+                assert np.array_equal(gf, res[2])
             s0, s1 = s1, s0
         if step >= 120:
             solver.update_contacts(c)
@@ -60,7 +61,6 @@ def test_update_contacts_matches_oracle_and_equilibrium(cuda_device):
             fg, _ = _ground_forces(model, got, c.force[: got["count"]].cpu().numpy(), bodies)
             for name in acc:
                 acc[name] += fg[name]
-    print("forces bit-exact vs oracle:", exact, flush=True)
     for name in acc:
         acc[name] /= 20
     m = model.host["body_mass"]
@@ -72,7 +72,8 @@ def test_update_contacts_matches_oracle_and_equilibrium(cuda_device):
     np.testing.assert_allclose(-acc["right"][2], 1.5 * mg, rtol=0.15)
 
 
-def test_body_parent_f_matches_oracle_and_weight(cuda_device):
+@pytest.mark.parametrize("reference_arithmetic", [True, False], ids=["reference_order", "world_frame"])
+def test_body_parent_f_matches_oracle_and_weight(cuda_device, reference_arithmetic):
     from newton_custom_contact_amd.builder import ModelBuilder
 
     b = ModelBuilder()
@@ -84,7 +85,7 @@ def test_body_parent_f_matches_oracle_and_weight(cuda_device):
     b.add_articulation([j])
     model = b.finalize(cuda_device)
     o = oracle.Oracle(model)
-    solver = SolverXPBD(model, iterations=8)
+    solver = SolverXPBD(model, iterations=8, reference_arithmetic=reference_arithmetic)
     s0, s1 = model.state(), model.state()
     assert s0.body_parent_f is not None
     s0.body_q[link, 2] = -1.0
@@ -92,11 +93,12 @@ def test_body_parent_f_matches_oracle_and_weight(cuda_device):
     avg = np.zeros(6)
     for step in range(90):
         for k in range(8):
-            if step == 10 and k == 0:
+            check = reference_arithmetic and step == 10 and k == 0
+            if check:
                 torch.cuda.synchronize()
                 q, qd = s0.body_q.cpu().numpy(), s0.body_qd.cpu().numpy()
             solver.step(s0, s1, None, None, sub_dt)
-            if step == 10 and k == 0:
+            if check:
                 torch.cuda.synchronize()
                 res = o.step(q, qd, np.zeros_like(qd), None, sub_dt, iterations=8, report=True)
                 np.testing.assert_allclose(s1.body_parent_f.cpu().numpy(), res[3], rtol=1e-5, atol=1e-4)

@@ -55,7 +55,8 @@ def test_log_step_matches_oracle(cuda_device):
     model, o, q, qd = _settled(cuda_device)
     f = np.zeros_like(qd)
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
-    solver = SolverXPBDWarmStart(model, WarmStartMLP(device=cuda_device), iterations=10)
+    solver = SolverXPBDWarmStart(model, WarmStartMLP(device=cuda_device), iterations=10,
+                                 reference_arithmetic=True)
     c, c_next = pipe.contacts(), pipe.contacts()  # alternating buffers keep step A's contacts for matching
     # step A (history), then step B with the matched previous multipliers
     sa, sb, sc = _state(model, q, qd), model.state(), model.state()
@@ -108,7 +109,8 @@ def test_warm_step_matches_oracle(cuda_device):
     model, o, q, qd = _settled(cuda_device, seed=5)
     f = np.zeros_like(qd)
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
-    solver = SolverXPBDWarmStart(model, WarmStartMLP(device=cuda_device), iterations=10)
+    solver = SolverXPBDWarmStart(model, WarmStartMLP(device=cuda_device), iterations=10,
+                                 reference_arithmetic=True)
     c = pipe.contacts()
     s_in, s_out = _state(model, q, qd), model.state()
     pipe.collide(s_in, c)
@@ -138,10 +140,16 @@ def test_warm_step_matches_oracle(cuda_device):
         assert np.array_equal(b["fallback"].cpu().numpy(), fb_w), k
         print("warm", k, "tol", t, "fallback worlds", int(fb_w.sum()), flush=True)
     # lambda0 = 0 and warm_iterations = iterations: the plain step
-    plain = SolverXPBD(model, iterations=10)
+    plain = SolverXPBD(model, iterations=10, reference_arithmetic=True)
     s_plain = model.state()
     plain.step(s_in, s_plain, None, c, DT)
     _warm(solver, s_in, s_out, c, np.zeros_like(lam0_gpu), 10, 1e30)
+    assert np.array_equal(s_out.body_q.cpu().numpy(), s_plain.body_q.cpu().numpy())
+    assert np.array_equal(s_out.body_qd.cpu().numpy(), s_plain.body_qd.cpu().numpy())
+    # the same identity for the world-frame kernels (the default)
+    fast_warm = SolverXPBDWarmStart(model, WarmStartMLP(device=cuda_device), iterations=10)
+    SolverXPBD(model, iterations=10).step(s_in, s_plain, None, c, DT)
+    _warm(fast_warm, s_in, s_out, c, np.zeros_like(lam0_gpu), 10, 1e30)
     assert np.array_equal(s_out.body_q.cpu().numpy(), s_plain.body_q.cpu().numpy())
     assert np.array_equal(s_out.body_qd.cpu().numpy(), s_plain.body_qd.cpu().numpy())
 

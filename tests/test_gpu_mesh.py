@@ -21,7 +21,7 @@ from newton_custom_contact_amd import CollisionPipeline, scenes
 from newton_custom_contact_amd.builder import ModelBuilder
 from newton_custom_contact_amd.mesh import random_convex_hull
 from newton_custom_contact_amd.solvers import SolverXPBD
-from parity_helpers import compare_contacts
+from parity_helpers import assert_step_close, compare_contacts
 from tests.mesh_helpers import box_mesh, icosphere
 
 pytestmark = pytest.mark.gpu
@@ -158,8 +158,7 @@ def test_mesh_step_parity(cuda_device):
         solver.step(state_in, state_out, None, c, DT)
         torch.cuda.synchronize()
         q_ref, qd_ref = o.step(q, qd, np.zeros_like(qd), ref_c, DT, iterations=10)
-        np.testing.assert_allclose(state_out.body_q.cpu().numpy(), q_ref, rtol=1e-5, atol=1e-6)
-        np.testing.assert_allclose(state_out.body_qd.cpu().numpy(), qd_ref, rtol=1e-5, atol=1e-5)
+        assert_step_close(state_out.body_q.cpu().numpy(), state_out.body_qd.cpu().numpy(), q_ref, qd_ref, DT, "mesh")
 
 
 def test_mesh_worlds_settle(cuda_device):
