@@ -814,6 +814,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   int* seg = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* cur = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* scan_tmp = reinterpret_cast<int*>(p); p += sizeof(int) * BS;
+  p = smem + ((p - smem + 15) & ~static_cast<ptrdiff_t>(15));  // 16-byte aligned contribution slots
   float* contrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * CH;
   int* active = reinterpret_cast<int*>(p); p += sizeof(int) * CH;
   float* jcontrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * mj;
@@ -1100,6 +1101,17 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       if (pc >= 0) lrn.lambda_sum[pc] = 0.0f;
     }
 
+  // World-frame kernels, one body per lane: LPB lanes walk each body's list (entries k, k + LPB, ... in lane k
+  // of the body's group) and the partial sums combine by a fixed shuffle tree, so small worlds keep their
+  // lanes busy.  LPB depends on the world's body count only, so every launch shape sums in the same order
+  // (deterministic, not the sequential order).  Contribution slots then hold (da.lin, active) | (da.ang, 0) |
+  // (db.ang, 0) as float4, db.lin = -da.lin, zeros when inactive.
+  constexpr bool SPLIT = FAST && BPL == 1;
+  const int lpb = SPLIT ? (nb <= 4 ? 16 : nb <= 8 ? 8 : nb <= 16 ? 4 : nb <= 32 ? 2 : 1) : 1;
+  const int sub = SPLIT ? (tid & (lpb - 1)) : 0;
+  const int walk_body = SPLIT ? tid / lpb : tid;
+  float4* const contrib4 = reinterpret_cast<float4*>(contrib);
+
   // pass 0: the step (XM_WARM: warm-start pass first, warm_iterations iterations); pass 1 (XM_WARM
   // only, when the world's residual exceeds the tolerance): cold solve from the integrated state
   constexpr int PASSES = MODE == XM_WARM ? 2 : 1;
@@ -1133,10 +1145,10 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
         int ep[BPL], ee[BPL];
 #pragma unroll
         for (int r = 0; r < BPL; ++r) {
-          int lb = tid + r * BS;
+          int lb = SPLIT ? walk_body : tid + r * BS;
           acc[r] = Spatial{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
           cnt[r] = 0.0f;
-          ep[r] = lb < nb ? seg[lb] : 0;
+          ep[r] = lb < nb ? seg[lb] + sub : 0;
           ee[r] = lb < nb ? seg[lb + 1] : 0;
         }
         for (int cb = 0; cb < nc; cb += CH) {
@@ -1169,45 +1181,80 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
                 sc[6] = __int_as_float(hit ? r.la : -2);
                 sc[7] = __int_as_float(r.lb);
               }
+              if (SPLIT) {
+                // every slot written (zeros when inactive): the walk needs no separate flag
+                contrib4[3 * slot] = on ? make_float4(da.lin.x, da.lin.y, da.lin.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                contrib4[3 * slot + 1] = on ? make_float4(da.ang.x, da.ang.y, da.ang.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                contrib4[3 * slot + 2] = on ? make_float4(db.ang.x, db.ang.y, db.ang.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+              }
               if (on) {
                 act = 1;
-                float* o = contrib + 12 * slot;
-                store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
+                if (!SPLIT) {
+                  float* o = contrib + 12 * slot;
+                  store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
+                }
                 if (ACC && !warm) {
                   if (cb == 0) lamacc[jj] += lam;
                   else lrn.lambda_sum[cpos(a.c, cstart + c)] += lam;
                 }
               }
             }
-            active[slot] = act;
+            if (!SPLIT) active[slot] = act;
           }
           __syncthreads();
           const unsigned cend = static_cast<unsigned>(cb + CH);
-#pragma unroll
-          for (int r = 0; r < BPL; ++r) {
-            while (ep[r] < ee[r]) {
-              unsigned ent = keys[ep[r]];
-              unsigned cc = ent >> 1;
+          if (SPLIT) {
+            while (ep[0] < ee[0]) {
+              const unsigned ent = keys[ep[0]];
+              const unsigned cc = ent >> 1;
               if (cc >= cend) break;
-              int slot = static_cast<int>(cc) - cb;
-              if (active[slot]) {
-                const float* o = contrib + 12 * slot + 6 * (ent & 1u);
-                acc[r].lin = acc[r].lin + load3(o);
-                acc[r].ang = acc[r].ang + load3(o + 3);
-                cnt[r] += 1.0f;
+              const int slot = static_cast<int>(cc) - cb;
+              const float4 u = contrib4[3 * slot];
+              const float4 v = contrib4[3 * slot + 1 + (ent & 1u)];
+              const float sgn = (ent & 1u) ? -1.0f : 1.0f;
+              acc[0].lin = fastm::axpy(sgn, f3{u.x, u.y, u.z}, acc[0].lin);
+              acc[0].ang = acc[0].ang + f3{v.x, v.y, v.z};
+              cnt[0] += u.w;
+              ep[0] += lpb;
+            }
+          } else {
+#pragma unroll
+            for (int r = 0; r < BPL; ++r) {
+              while (ep[r] < ee[r]) {
+                unsigned ent = keys[ep[r]];
+                unsigned cc = ent >> 1;
+                if (cc >= cend) break;
+                int slot = static_cast<int>(cc) - cb;
+                if (active[slot]) {
+                  const float* o = contrib + 12 * slot + 6 * (ent & 1u);
+                  acc[r].lin = acc[r].lin + load3(o);
+                  acc[r].ang = acc[r].ang + load3(o + 3);
+                  cnt[r] += 1.0f;
+                }
+                ep[r] += 1;
               }
-              ep[r] += 1;
             }
           }
           if (cb + CH < nc) __syncthreads();
+        }
+        if (SPLIT) {
+          for (int off = lpb >> 1; off > 0; off >>= 1) {
+            acc[0].lin.x += __shfl_xor(acc[0].lin.x, off, 64);
+            acc[0].lin.y += __shfl_xor(acc[0].lin.y, off, 64);
+            acc[0].lin.z += __shfl_xor(acc[0].lin.z, off, 64);
+            acc[0].ang.x += __shfl_xor(acc[0].ang.x, off, 64);
+            acc[0].ang.y += __shfl_xor(acc[0].ang.y, off, 64);
+            acc[0].ang.z += __shfl_xor(acc[0].ang.z, off, 64);
+            cnt[0] += __shfl_xor(cnt[0], off, 64);
+          }
         }
         if (MODE == XM_REPORT && a.rep.contact_impulse) {
           // accumulate_weighted_contact_impulse (kernels.py:2301-2361): per contact, this iteration's
           // shape-0 delta scaled by 1/N of its dynamic body (harmonic mean 2/(N_a+N_b) for two)
 #pragma unroll
           for (int r = 0; r < BPL; ++r) {
-            int lb = tid + r * BS;
-            if (lb < nb) fl[lb] = cnt[r];
+            int lb = SPLIT ? walk_body : tid + r * BS;
+            if (lb < nb && sub == 0) fl[lb] = cnt[r];
           }
           __syncthreads();
           for (int c = tid; c < nc; c += BS) {
@@ -1232,9 +1279,9 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
         }
 #pragma unroll
         for (int r = 0; r < BPL; ++r) {
-          int lb = tid + r * BS;
+          int lb = SPLIT ? walk_body : tid + r * BS;
           // the warm pass leaves bodies without a warm contact untouched (no renormalisation / clamp)
-          if (lb < nb && !(warm && cnt[r] == 0.0f)) {
+          if (lb < nb && sub == 0 && !(warm && cnt[r] == 0.0f)) {
             if (FAST) {
               apply_fast(L, lb, acc[r], weighted, cnt[r], dt);
             } else {
@@ -1413,6 +1460,7 @@ size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints,
   size_t ch = (size_t)bs * sh.cpl;
   size_t b = sizeof(float) * (7 + 6 + 6 + 3 + 1 + 9 + 9) * mb + sizeof(int) * mb + 2 * sizeof(int) * (mb + 1);
   b += sizeof(float) * 4 * 6 * (mb + 1) + sizeof(float) * 6 + sizeof(int);  // frames (+ static slot), static qd, flags
+  b += 16;  // alignment of the contribution slots
   b += sizeof(int) * bs + sizeof(float) * 12 * ch + sizeof(int) * ch + sizeof(float) * 12 * mj + sizeof(int) * mj;
   b += sizeof(unsigned) * (size_t)key_cap;
   return (b + 15) & ~size_t(15);
