@@ -824,6 +824,13 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   const bool has_c = a.p.has_contacts != 0 && w < d.segment_count;
   const int cstart = has_c ? a.c.world_start[w] : 0;
   int nc = has_c ? a.c.world_count[w] : 0;
+  // World-frame kernels, one body per lane: the per-body (contact, side) lists come from two bit masks per
+  // body over the world's contacts (the body is shape 0's / shape 1's), built by LDS atomic OR (order-free)
+  // in the contribution slots (free until the iterations), then written out in ascending order by the body's
+  // lane: no atomic scatter, no sort.  Worlds whose masks do not fit there use the counting sort.
+  constexpr bool SPLIT_SHAPE = FAST && BPL == 1;
+  const int mask_words = (nc + 31) >> 5;  // 32-bit words per body and side
+  const bool masks = SPLIT_SHAPE && 2 * nb * mask_words <= 12 * CH;
   if (2 * nc > a.key_cap) {
     // dense world: order its entries in the global buffer at its contact offset
     long long room = a.keys_global_cap - 2ll * cstart;
@@ -931,7 +938,12 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       store3(fl + 6 * lb + 3, f.ang);
     }
   }
-  // per-body (contact, side) counts for the ordering lists
+  // per-body (contact, side) counts for the ordering lists, or the contact masks
+  unsigned* const cmask = reinterpret_cast<unsigned*>(contrib);
+  if (masks) {
+    for (int i = tid; i < 2 * nb * mask_words; i += BS) cmask[i] = 0u;
+    __syncthreads();
+  }
   for (int c = tid; c < nc; c += BS) {
     CReg r;
     if (c < CH) {
@@ -943,8 +955,14 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       r = load_contact(a, cstart + c, b0, nb);
     }
     if (r.valid) {
-      if (r.la >= 0) atomicAdd(&cur[r.la], 1);
-      if (r.lb >= 0) atomicAdd(&cur[r.lb], 1);
+      if (masks) {
+        const unsigned bit = 1u << (c & 31);
+        if (r.la >= 0) atomicOr(&cmask[(2 * r.la) * mask_words + (c >> 5)], bit);
+        if (r.lb >= 0) atomicOr(&cmask[(2 * r.lb + 1) * mask_words + (c >> 5)], bit);
+      } else {
+        if (r.la >= 0) atomicAdd(&cur[r.la], 1);
+        if (r.lb >= 0) atomicAdd(&cur[r.lb], 1);
+      }
     }
   }
   __syncthreads();
@@ -969,6 +987,31 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
     }
   }
   // ---- ordering lists: counting sort by body into per-body lists in contact order
+  if (masks) {
+    // counts from the masks, offsets, then each body's list in ascending (contact, side) order (a contact
+    // never names the same body on both sides)
+    for (int lb = tid; lb < nb; lb += BS) {
+      int n = 0;
+      for (int k = 0; k < 2 * mask_words; ++k) n += __popc(cmask[2 * lb * mask_words + k]);
+      cur[lb] = n;
+    }
+    __syncthreads();
+    block_exclusive_scan(cur, seg, nb, scan_tmp);
+    for (int lb = tid; lb < nb; lb += BS) {
+      int o = seg[lb];
+      const unsigned* ma = cmask + 2 * lb * mask_words;
+      const unsigned* mb = ma + mask_words;
+      for (int wd = 0; wd < mask_words; ++wd) {
+        const unsigned sa = ma[wd], sb = mb[wd];
+        unsigned m = sa | sb;
+        while (m) {
+          const int bit = __ffs(m) - 1;
+          m &= m - 1u;
+          keys[o++] = static_cast<unsigned>(2 * ((wd << 5) + bit)) + ((sb >> bit) & 1u);
+        }
+      }
+    }
+  } else {
   block_exclusive_scan(cur, seg, nb, scan_tmp);
   for (int lb = tid; lb < nb; lb += BS) cur[lb] = seg[lb];
   __syncthreads();
@@ -1039,6 +1082,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       }
     }
   }
+  }
   __syncthreads();
 
   // ---- warm-start inputs at the integrated state
@@ -1106,7 +1150,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   // lanes busy.  LPB depends on the world's body count only, so every launch shape sums in the same order
   // (deterministic, not the sequential order).  Contribution slots then hold (da.lin, active) | (da.ang, 0) |
   // (db.ang, 0) as float4, db.lin = -da.lin, zeros when inactive.
-  constexpr bool SPLIT = FAST && BPL == 1;
+  constexpr bool SPLIT = SPLIT_SHAPE;
   const int lpb = SPLIT ? (nb <= 4 ? 16 : nb <= 8 ? 8 : nb <= 16 ? 4 : nb <= 32 ? 2 : 1) : 1;
   const int sub = SPLIT ? (tid & (lpb - 1)) : 0;
   const int walk_body = SPLIT ? tid / lpb : tid;
