@@ -582,7 +582,7 @@ __global__ void __launch_bounds__(CBLOCK) contact_compact_kernel(StageBuffers st
       }
       long long s = 5 * g + k;
       if (MESH && shape_type[s0] == GEO_MESH) s = st.mesh_slot_base + static_cast<long long>(st.mesh_index[g]) * mesh_k + k;
-      const float4* r = st.rec + 3 * s;
+      const float4* r = st.rec + REC_F4 * s;
       const float4 r0 = r[0], r1 = r[1], r2 = r[2];
       // write_contact (collide.py:62-148), second half: body-frame points and offsets
       const f3 aw{r0.x, r0.y, r0.z}, bw{r0.w, r1.x, r1.y}, n{r1.z, r1.w, r2.x};

@@ -176,7 +176,7 @@ __device__ __forceinline__ bool passes_gap(const CData& c) {
 // body transforms (the second half) as it writes the contact out.
 __device__ __forceinline__ void write_record(const CData& c, f3 n, f3 aw, f3 bw, const WriteCtx& w, long long s) {
   float off_a = c.r_a + c.m_a, off_b = c.r_b + c.m_b;
-  float4* r = w.st.rec + 3 * s;
+  float4* r = w.st.rec + REC_F4 * s;
   r[0] = make_float4(aw.x, aw.y, aw.z, bw.x);
   r[1] = make_float4(bw.y, bw.z, n.x, n.y);
   r[2] = make_float4(n.z, off_a, off_b, __int_as_float(c.sub_key));
@@ -227,7 +227,7 @@ __device__ __forceinline__ void sphere_sphere(f3 p1, float r1, f3 p2, float r2, 
   dist = d;
 }
 
-__device__ void capsule_capsule(f3 p1, f3 ax1, float r1, float hl1, f3 p2, f3 ax2, float r2, float hl2,
+__device__ __forceinline__ void capsule_capsule(f3 p1, f3 ax1, float r1, float hl1, f3 p2, f3 ax2, float r2, float hl2,
                                 float dist[2], f3 pos[2], f3& normal) {
   const float MINVAL = 1e-15f;
   dist[0] = dist[1] = MAXVAL;
@@ -264,10 +264,17 @@ __device__ void capsule_capsule(f3 p1, f3 ax1, float r1, float hl1, f3 p2, f3 ax
   }
 }
 
-__device__ void plane_box(f3 n, f3 pp, f3 bp, const m33& R, f3 size, float margin, float dist[4], f3 pos[4]) {
+// The four kept corners live in named registers (k0..k3) and are addressed by compile-time index only: an
+// array indexed by the running slot number would be placed in scratch memory.  Same selection as the
+// reference: the first four corners within the margin, then each further one replaces the current worst
+// (largest distance) when it is deeper.
+__device__ __forceinline__ void plane_box(f3 n, f3 pp, f3 bp, const m33& R, f3 size, float margin, float dist[4], f3 pos[4]) {
   float center_dist = dot(bp - pp, n);
-  for (int i = 0; i < 4; ++i) { dist[i] = MAXVAL; pos[i] = f3{0.0f, 0.0f, 0.0f}; }
+  float d0 = MAXVAL, d1 = MAXVAL, d2 = MAXVAL, d3 = MAXVAL;
+  const f3 zero{0.0f, 0.0f, 0.0f};
+  f3 q0 = zero, q1 = zero, q2 = zero, q3 = zero;
   int ncontact = 0, worst = 0;
+#pragma unroll
   for (int i = 0; i < 8; ++i) {
     f3 corner{(i & 1) ? size.x : -size.x, (i & 2) ? size.y : -size.y, (i & 4) ? size.z : -size.z};
     corner = mv(R, corner);
@@ -275,20 +282,32 @@ __device__ void plane_box(f3 n, f3 pp, f3 bp, const m33& R, f3 size, float margi
     float cdist = center_dist + ldist;
     if (cdist > margin) continue;
     f3 cpos = corner + bp - (0.5f * n) * cdist;
+    const float dw = worst == 0 ? d0 : (worst == 1 ? d1 : (worst == 2 ? d2 : d3));
+    int slot;
     if (ncontact < 4) {
-      dist[ncontact] = cdist;
-      pos[ncontact] = cpos;
-      if (ncontact == 0 || cdist > dist[worst]) worst = ncontact;
+      slot = ncontact;
+    } else if (cdist < dw) {
+      slot = worst;
+    } else {
+      continue;
+    }
+    if (slot == 0) { d0 = cdist; q0 = cpos; }
+    else if (slot == 1) { d1 = cdist; q1 = cpos; }
+    else if (slot == 2) { d2 = cdist; q2 = cpos; }
+    else { d3 = cdist; q3 = cpos; }
+    if (ncontact < 4) {
+      // dw is an earlier corner's distance (worst < ncontact): unchanged by the store above
+      if (ncontact == 0 || cdist > dw) worst = ncontact;
       ncontact += 1;
-    } else if (cdist < dist[worst]) {
-      dist[worst] = cdist;
-      pos[worst] = cpos;
+    } else {
       worst = 0;
-      if (dist[1] > dist[worst]) worst = 1;
-      if (dist[2] > dist[worst]) worst = 2;
-      if (dist[3] > dist[worst]) worst = 3;
+      if (d1 > d0) worst = 1;
+      if (d2 > (worst == 0 ? d0 : d1)) worst = 2;
+      if (d3 > (worst == 0 ? d0 : (worst == 1 ? d1 : d2))) worst = 3;
     }
   }
+  dist[0] = d0; dist[1] = d1; dist[2] = d2; dist[3] = d3;
+  pos[0] = q0; pos[1] = q1; pos[2] = q2; pos[3] = q3;
 }
 
 __device__ void sphere_box(f3 sp, float r, f3 bp, const m33& R, f3 size, float& dist, f3& pos_out, f3& normal) {
@@ -467,15 +486,33 @@ __device__ CvxRes mpr_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
 }
 
 // ------------------------------------------------------------------------- GJK
+// The simplex's four slots are addressed through the selects below with compile-time array indices only
+// (slot numbers are run-time values): arrays indexed at run time would live in scratch memory.
 struct Simplex {
   f3 B[4], A[4];
   float bc[4];
   unsigned mask;
 };
 
-__device__ __forceinline__ f3 smp_segment(const Simplex& s, int i0, int i1, float bc[4], unsigned& mask) {
+__device__ __forceinline__ f3 sget(const f3 (&a)[4], int i) {
+  return i == 0 ? a[0] : (i == 1 ? a[1] : (i == 2 ? a[2] : a[3]));
+}
+__device__ __forceinline__ void sset(f3 (&a)[4], int i, f3 v) {
+  a[0] = i == 0 ? v : a[0];
+  a[1] = i == 1 ? v : a[1];
+  a[2] = i == 2 ? v : a[2];
+  a[3] = i == 3 ? v : a[3];
+}
+__device__ __forceinline__ void sset(float (&a)[4], int i, float v) {
+  a[0] = i == 0 ? v : a[0];
+  a[1] = i == 1 ? v : a[1];
+  a[2] = i == 2 ? v : a[2];
+  a[3] = i == 3 ? v : a[3];
+}
+
+__device__ __forceinline__ f3 smp_segment(const Simplex& s, int i0, int i1, float (&bc)[4], unsigned& mask) {
   const float EPS = 1e-8f;
-  f3 a = s.A[i0], b = s.A[i1];
+  f3 a = sget(s.A, i0), b = sget(s.A, i1);
   f3 edge = b - a;
   float vsq = len_sq(edge);
   bool degenerate = vsq < EPS;
@@ -486,14 +523,14 @@ __device__ __forceinline__ f3 smp_segment(const Simplex& s, int i0, int i1, floa
   bc[0] = bc[1] = bc[2] = bc[3] = 0.0f;
   if (l0 < 0.0f || degenerate) { mask = 1u << i1; l0 = 0.0f; l1 = 1.0f; }
   else if (l1 < 0.0f) { mask = 1u << i0; l0 = 1.0f; l1 = 0.0f; }
-  bc[i0] = l0;
-  bc[i1] = l1;
+  sset(bc, i0, l0);
+  sset(bc, i1, l1);
   return l0 * a + l1 * b;
 }
 
-__device__ f3 smp_triangle(const Simplex& s, int i0, int i1, int i2, float bc[4], unsigned& mask) {
+__device__ __forceinline__ f3 smp_triangle(const Simplex& s, int i0, int i1, int i2, float (&bc)[4], unsigned& mask) {
   const float EPS = 1e-8f;
-  f3 a = s.A[i0], b = s.A[i1], c = s.A[i2];
+  f3 a = sget(s.A, i0), b = sget(s.A, i1), c = sget(s.A, i2);
   f3 u = a - b, w = a - c;
   f3 normal = cross(u, w);
   float t = len_sq(normal);
@@ -513,27 +550,41 @@ __device__ f3 smp_triangle(const Simplex& s, int i0, int i1, int i2, float bc[4]
   if (l0 < 0.0f || degenerate) {
     f3 cl = smp_segment(s, i1, i2, bct, m);
     float dist = len_sq(cl);
-    if (dist < best) { for (int k = 0; k < 4; ++k) bc[k] = bct[k]; mask = m; best = dist; closest_pt = cl; }
+    if (dist < best) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) bc[k] = bct[k];
+      mask = m; best = dist; closest_pt = cl;
+    }
   }
   if (l1 < 0.0f || degenerate) {
     f3 cl = smp_segment(s, i0, i2, bct, m);
     float dist = len_sq(cl);
-    if (dist < best) { for (int k = 0; k < 4; ++k) bc[k] = bct[k]; mask = m; best = dist; closest_pt = cl; }
+    if (dist < best) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) bc[k] = bct[k];
+      mask = m; best = dist; closest_pt = cl;
+    }
   }
   if (l2 < 0.0f || degenerate) {
     f3 cl = smp_segment(s, i0, i1, bct, m);
     float dist = len_sq(cl);
-    if (dist < best) { for (int k = 0; k < 4; ++k) bc[k] = bct[k]; mask = m; closest_pt = cl; }
+    if (dist < best) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) bc[k] = bct[k];
+      mask = m; closest_pt = cl;
+    }
   }
   if (mask != 0) return closest_pt;
-  bc[i0] = l0; bc[i1] = l1; bc[i2] = l2;
+  sset(bc, i0, l0);
+  sset(bc, i1, l1);
+  sset(bc, i2, l2);
   mask = (1u << i0) | (1u << i1) | (1u << i2);
   return l0 * a + l1 * b + l2 * c;
 }
 
 __device__ __forceinline__ float det4(f3 a, f3 b, f3 c, f3 d) { return dot(b - a, cross(c - a, d - a)); }
 
-__device__ f3 smp_tetra(const Simplex& s, float bc[4], unsigned& mask) {
+__device__ __forceinline__ f3 smp_tetra(const Simplex& s, float (&bc)[4], unsigned& mask) {
   const float EPS = 1e-8f;
   f3 v0 = s.A[0], v1 = s.A[1], v2 = s.A[2], v3 = s.A[3];
   float det_t = det4(v0, v1, v2, v3);
@@ -553,11 +604,13 @@ __device__ f3 smp_tetra(const Simplex& s, float bc[4], unsigned& mask) {
   mask = 0;
   unsigned m;
   const int F[4][3] = {{1, 2, 3}, {0, 2, 3}, {0, 1, 3}, {0, 1, 2}};
+#pragma unroll
   for (int f = 0; f < 4; ++f) {
     if (lam[f] < 0.0f || degenerate) {
       f3 cl = smp_triangle(s, F[f][0], F[f][1], F[f][2], bct, m);
       float dist = len_sq(cl);
       if (dist < best) {
+#pragma unroll
         for (int k = 0; k < 4; ++k) bc[k] = bct[k];
         mask = m;
         if (f < 3) best = dist;
@@ -574,6 +627,7 @@ __device__ f3 smp_tetra(const Simplex& s, float bc[4], unsigned& mask) {
 __device__ __forceinline__ void smp_closest(const Simplex& s, f3& pa, f3& pb) {
   pa = f3{0.0f, 0.0f, 0.0f};
   pb = pa;
+#pragma unroll
   for (int i = 0; i < 4; ++i) {
     if ((s.mask & (1u << i)) == 0) continue;
     float b = s.bc[i];
@@ -589,6 +643,7 @@ __device__ CvxRes gjk_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
   f3 zero{0.0f, 0.0f, 0.0f};
   f3 point_a = zero, point_b = zero, normal = zero;
   Simplex s;
+#pragma unroll
   for (int i = 0; i < 4; ++i) { s.B[i] = zero; s.A[i] = zero; s.bc[i] = 0.0f; }
   s.mask = 0;
   int iter = MAX_ITER;
@@ -609,46 +664,63 @@ __device__ CvxRes gjk_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
     float delta_dist = dot(v, v - wv);
     if (delta_dist < CE * sqrtf(dist_sq)) break;
     bool dup = false;
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (s.mask & (1u << i)) {
-        if (len_sq(s.A[i] - wv) < CE * CE) { dup = true; break; }
+      if (!dup && (s.mask & (1u << i))) {
+        if (len_sq(s.A[i] - wv) < CE * CE) dup = true;
       }
     }
     if (dup) break;
+    // used slots in ascending order, then the last free slot (idx0..idx3)
     int use_count = 0, free_slot = 0;
-    int idx[4] = {0, 0, 0, 0};
+    int idx0 = 0, idx1 = 0, idx2 = 0, idx3 = 0;
+#pragma unroll
     for (int i = 0; i < 4; ++i) {
-      if (s.mask & (1u << i)) idx[use_count++] = i;
-      else free_slot = i;
+      if (s.mask & (1u << i)) {
+        idx0 = use_count == 0 ? i : idx0;
+        idx1 = use_count == 1 ? i : idx1;
+        idx2 = use_count == 2 ? i : idx2;
+        idx3 = use_count == 3 ? i : idx3;
+        use_count += 1;
+      } else {
+        free_slot = i;
+      }
     }
-    idx[use_count] = free_slot;
+    idx0 = use_count == 0 ? free_slot : idx0;
+    idx1 = use_count == 1 ? free_slot : idx1;
+    idx2 = use_count == 2 ? free_slot : idx2;
+    idx3 = use_count == 3 ? free_slot : idx3;
     use_count += 1;
-    s.B[free_slot] = w.B;
-    s.A[free_slot] = w.A;
+    sset(s.B, free_slot, w.B);
+    sset(s.A, free_slot, w.A);
     f3 closest = zero;
     bool success = true;
     float bc[4];
     unsigned mask;
     if (use_count == 1) {
-      closest = s.A[idx[0]];
-      s.mask = 1u << idx[0];
-      s.bc[idx[0]] = 1.0f;
+      closest = sget(s.A, idx0);
+      s.mask = 1u << idx0;
+      sset(s.bc, idx0, 1.0f);
     } else if (use_count == 2) {
-      closest = smp_segment(s, idx[0], idx[1], bc, mask);
+      closest = smp_segment(s, idx0, idx1, bc, mask);
+#pragma unroll
       for (int k = 0; k < 4; ++k) s.bc[k] = bc[k];
       s.mask = mask;
     } else if (use_count == 3) {
-      closest = smp_triangle(s, idx[0], idx[1], idx[2], bc, mask);
+      closest = smp_triangle(s, idx0, idx1, idx2, bc, mask);
+#pragma unroll
       for (int k = 0; k < 4; ++k) s.bc[k] = bc[k];
       s.mask = mask;
     } else if (use_count == 4) {
       closest = smp_tetra(s, bc, mask);
+#pragma unroll
       for (int k = 0; k < 4; ++k) s.bc[k] = bc[k];
       s.mask = mask;
       success = !(mask == 15u);
     } else {
       success = false;
     }
+    (void)idx3;
     if (!success) {
       smp_closest(s, point_a, point_b);
       return {false, point_a, point_b, zero, 0.0f};
@@ -788,7 +860,7 @@ __device__ int trim_all(const Buf& buf, int tcount, int lcount) {
 }
 
 template <class Buf>
-__device__ void calipers(const Buf& hull, int n, int out[4]) {
+__device__ __forceinline__ void calipers(const Buf& hull, int n, int out[4]) {
   int p1 = 0, p3 = 1;
   f2 d0 = hull[p1] - hull[p3];
   float max_dist_sq = d0.x * d0.x + d0.y * d0.y;
@@ -931,15 +1003,18 @@ struct PairCtx {
   CData tmpl;
 };
 
+// Directions of the five tilted support probes (cos and sin of k 2 pi / 5), in constant memory: the probe
+// loop index is wave-uniform, so the loads are scalar.
+static __constant__ float c_probe_cos[5] = {1.0f, (float)0.30901699437494745, (float)-0.8090169943749473,
+                                            (float)-0.8090169943749476, (float)0.30901699437494723};
+static __constant__ float c_probe_sin[5] = {0.0f, (float)0.9510565162951535, (float)0.5877852522924732,
+                                            (float)-0.587785252292473, (float)-0.9510565162951536};
+
 __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b, f3 normal, WriteCtx& w) {
   const float EPS = 0.00001f;
   const float SIN_TILT = (float)0.03489949670250097;
   const float COS_TILT = (float)0.9993908270190958;
   const float COS_DEEP = (float)0.9999984769132877;
-  const float PC[5] = {1.0f, (float)0.30901699437494745, (float)-0.8090169943749473, (float)-0.8090169943749476,
-                       (float)0.30901699437494723};
-  const float PS[5] = {0.0f, (float)0.9510565162951535, (float)0.5877852522924732, (float)-0.587785252292473,
-                       (float)-0.9510565162951536};
   f3 ta, tb;
   onb(normal, ta, tb);
   f3 zero{0.0f, 0.0f, 0.0f};
@@ -953,7 +1028,7 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
   f3 ltab = rot_inv(rel_ob, -ta);
   f3 ltbb = rot_inv(rel_ob, -tb);
   for (int e = 0; e < 5; ++e) {
-    float c = PC[e], s = PS[e];
+    float c = c_probe_cos[e], s = c_probe_sin[e];
     float ct = COS_TILT;
     float cs = c * SIN_TILT, ss = s * SIN_TILT;
     f3 dir_a = normal * ct + cs * ta + ss * tb;
@@ -993,7 +1068,9 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
           calipers(buf, loop_count, res);
           loop_count = 4;
         }
-        for (int i = 0; i < loop_count; ++i) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          if (i >= loop_count) break;
           int ia = res[i];
           f3 pl = buf[ia].x * ta + buf[ia].y * tb + center;
           f3 a = ray_plane(pl, normal, pa.d, pa.n);
@@ -1130,6 +1207,7 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     sphere_box(pos_a, scale_a.x, pos_b, R, scale_b, cd[0], cp[0], cn);
   }
   int num = 0;
+#pragma unroll
   for (int k = 0; k < 4; ++k) num += cd[k] < MAXVAL ? 1 : 0;
   if (num > 0) {
     CData c;
@@ -1142,6 +1220,7 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     c.shape_b = sb;
     c.gap_sum = gap_sum;
     bool valid[4];
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
       valid[k] = false;
       if (cd[k] < MAXVAL) {
@@ -1150,6 +1229,7 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
         valid[k] = passes_gap(c);
       }
     }
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
       if (!valid[k]) continue;
       c.center = cp[k];

@@ -320,7 +320,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       s.mesh_pair_capacity = mesh_pairs;
       slots += mesh_pairs * m->mesh_k;
     }
-    if ((e = dmalloc(&s.rec, 3 * slots)) != hipSuccess || (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
+    if ((e = dmalloc(&s.rec, REC_F4 * slots)) != hipSuccess || (e = dmalloc(&s.mask, pairs)) != hipSuccess || (e = dmalloc(&s.pair_a, pairs)) != hipSuccess ||
         (e = dmalloc(&s.pair_b, pairs)) != hipSuccess || (e = dmalloc(&s.route, pairs)) != hipSuccess ||
         (e = dmalloc(&s.route_count, 3)) != hipSuccess ||
         (d.mesh_count > 0 && (e = dmalloc(&s.mesh_index, pairs)) != hipSuccess) ||

@@ -36,9 +36,12 @@ struct RedParams {
   int mode;
 };
 
+constexpr int REC_F4 = 4;  // float4 per staging slot
+
 struct StageBuffers {
   // one 48-byte record per contact slot: world-frame point on A 3, on B 3, unit normal 3, offset A,
-  // offset B (radius + margin), sub_key (int bits) -- written and read as three 16-byte vectors
+  // offset B (radius + margin), sub_key (int bits) -- written and read as three 16-byte vectors at a
+  // 64-byte stride (REC_F4 float4 per slot), so that a record fills one 64-byte memory sector
   float4* rec;
   unsigned short* mask;  // per pair: number of records in its slots (0-5, mesh pairs 0-K)
   int* pair_a;           // first shape (lower GeoType), global id
