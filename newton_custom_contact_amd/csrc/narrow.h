@@ -486,23 +486,35 @@ __device__ CvxRes mpr_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
 }
 
 // ------------------------------------------------------------------------- GJK
-// The simplex's four slots are addressed through the selects below with compile-time array indices only
-// (slot numbers are run-time values): arrays indexed at run time would live in scratch memory.
+// Lanes per workgroup of the kernels that run the narrow phase (the per-lane LDS buffers are sized by it).
+constexpr int kNarrowBlock = 256;
+
+// The simplex's vertices live in LDS, one lane's float f at s_smp[f * kNarrowBlock + t] (slot numbers are
+// run-time values, and a register array indexed by them would be placed in scratch memory): A of slot i at
+// floats 3i..3i+2, B at 12 + 3i..; barycentric weights and the slot mask stay in registers, written
+// through the compile-time-indexed selects of sset.
+struct SimplexVerts {
+  float* base;  // s_smp + lane
+  __device__ __forceinline__ f3 get(int f0) const {
+    return {base[f0 * kNarrowBlock], base[(f0 + 1) * kNarrowBlock], base[(f0 + 2) * kNarrowBlock]};
+  }
+  __device__ __forceinline__ void put(int f0, f3 v) const {
+    base[f0 * kNarrowBlock] = v.x;
+    base[(f0 + 1) * kNarrowBlock] = v.y;
+    base[(f0 + 2) * kNarrowBlock] = v.z;
+  }
+  __device__ __forceinline__ f3 A(int i) const { return get(3 * i); }
+  __device__ __forceinline__ f3 B(int i) const { return get(12 + 3 * i); }
+  __device__ __forceinline__ void setA(int i, f3 v) const { put(3 * i, v); }
+  __device__ __forceinline__ void setB(int i, f3 v) const { put(12 + 3 * i, v); }
+};
+
 struct Simplex {
-  f3 B[4], A[4];
+  SimplexVerts v;
   float bc[4];
   unsigned mask;
 };
 
-__device__ __forceinline__ f3 sget(const f3 (&a)[4], int i) {
-  return i == 0 ? a[0] : (i == 1 ? a[1] : (i == 2 ? a[2] : a[3]));
-}
-__device__ __forceinline__ void sset(f3 (&a)[4], int i, f3 v) {
-  a[0] = i == 0 ? v : a[0];
-  a[1] = i == 1 ? v : a[1];
-  a[2] = i == 2 ? v : a[2];
-  a[3] = i == 3 ? v : a[3];
-}
 __device__ __forceinline__ void sset(float (&a)[4], int i, float v) {
   a[0] = i == 0 ? v : a[0];
   a[1] = i == 1 ? v : a[1];
@@ -512,7 +524,7 @@ __device__ __forceinline__ void sset(float (&a)[4], int i, float v) {
 
 __device__ __forceinline__ f3 smp_segment(const Simplex& s, int i0, int i1, float (&bc)[4], unsigned& mask) {
   const float EPS = 1e-8f;
-  f3 a = sget(s.A, i0), b = sget(s.A, i1);
+  f3 a = s.v.A(i0), b = s.v.A(i1);
   f3 edge = b - a;
   float vsq = len_sq(edge);
   bool degenerate = vsq < EPS;
@@ -530,7 +542,7 @@ __device__ __forceinline__ f3 smp_segment(const Simplex& s, int i0, int i1, floa
 
 __device__ __forceinline__ f3 smp_triangle(const Simplex& s, int i0, int i1, int i2, float (&bc)[4], unsigned& mask) {
   const float EPS = 1e-8f;
-  f3 a = sget(s.A, i0), b = sget(s.A, i1), c = sget(s.A, i2);
+  f3 a = s.v.A(i0), b = s.v.A(i1), c = s.v.A(i2);
   f3 u = a - b, w = a - c;
   f3 normal = cross(u, w);
   float t = len_sq(normal);
@@ -586,7 +598,7 @@ __device__ __forceinline__ float det4(f3 a, f3 b, f3 c, f3 d) { return dot(b - a
 
 __device__ __forceinline__ f3 smp_tetra(const Simplex& s, float (&bc)[4], unsigned& mask) {
   const float EPS = 1e-8f;
-  f3 v0 = s.A[0], v1 = s.A[1], v2 = s.A[2], v3 = s.A[3];
+  f3 v0 = s.v.A(0), v1 = s.v.A(1), v2 = s.v.A(2), v3 = s.v.A(3);
   float det_t = det4(v0, v1, v2, v3);
   bool degenerate = fabsf(det_t) < EPS;
   float denom = degenerate ? EPS : det_t;
@@ -631,8 +643,9 @@ __device__ __forceinline__ void smp_closest(const Simplex& s, f3& pa, f3& pb) {
   for (int i = 0; i < 4; ++i) {
     if ((s.mask & (1u << i)) == 0) continue;
     float b = s.bc[i];
-    pa = pa + b * (s.B[i] + s.A[i]);
-    pb = pb + b * s.B[i];
+    const f3 sb = s.v.B(i);
+    pa = pa + b * (sb + s.v.A(i));
+    pb = pb + b * sb;
   }
 }
 
@@ -642,9 +655,11 @@ __device__ CvxRes gjk_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
   const float EPS = 1e-8f;
   f3 zero{0.0f, 0.0f, 0.0f};
   f3 point_a = zero, point_b = zero, normal = zero;
+  __shared__ float s_smp[24 * kNarrowBlock];
   Simplex s;
+  s.v = SimplexVerts{s_smp + threadIdx.x};
 #pragma unroll
-  for (int i = 0; i < 4; ++i) { s.B[i] = zero; s.A[i] = zero; s.bc[i] = 0.0f; }
+  for (int i = 0; i < 4; ++i) { s.v.setB(i, zero); s.v.setA(i, zero); s.bc[i] = 0.0f; }
   s.mask = 0;
   int iter = MAX_ITER;
   Vert center = geo_center(ob, pb);
@@ -667,7 +682,7 @@ __device__ CvxRes gjk_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
       if (!dup && (s.mask & (1u << i))) {
-        if (len_sq(s.A[i] - wv) < CE * CE) dup = true;
+        if (len_sq(s.v.A(i) - wv) < CE * CE) dup = true;
       }
     }
     if (dup) break;
@@ -691,14 +706,14 @@ __device__ CvxRes gjk_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
     idx2 = use_count == 2 ? free_slot : idx2;
     idx3 = use_count == 3 ? free_slot : idx3;
     use_count += 1;
-    sset(s.B, free_slot, w.B);
-    sset(s.A, free_slot, w.A);
+    s.v.setB(free_slot, w.B);
+    s.v.setA(free_slot, w.A);
     f3 closest = zero;
     bool success = true;
     float bc[4];
     unsigned mask;
     if (use_count == 1) {
-      closest = sget(s.A, idx0);
+      closest = s.v.A(idx0);
       s.mask = 1u << idx0;
       sset(s.bc, idx0, 1.0f);
     } else if (use_count == 2) {
@@ -759,8 +774,7 @@ constexpr int kClipMax = 12;
 
 // The clip buffer of one lane, kept in LDS rather than in scratch (the polygon loops index it by value):
 // entry i of lane t sits at s_poly[i * kNarrowBlock + t], so a wave's accesses to one entry hit
-// consecutive 8-byte words.  The narrow kernel runs kNarrowBlock lanes per workgroup.
-constexpr int kNarrowBlock = 256;
+// consecutive 8-byte words.
 struct LanePoly {
   f2* base;
   __device__ __forceinline__ f2& operator[](int i) const { return base[i * kNarrowBlock]; }
