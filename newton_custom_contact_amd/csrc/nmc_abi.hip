@@ -174,11 +174,11 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   if (pair_cap < 1) pair_cap = 1;
   m->pair_cap = next_pow2(pair_cap);
   m->collide_lds = collide_lds_enum_bytes(d.max_segment_shapes > 0 ? d.max_segment_shapes : 1);
-  // solver ordering buffer: 2 entries per contact; worlds with up to 512 contacts order them in LDS,
+  // solver ordering buffer: 2 entries per contact; worlds with up to 256 contacts order them in LDS,
   // denser worlds in a global buffer of 2 entries per contact slot.  Models with meshes (up to
   // mesh_k contacts per pair, few bodies per world) size the LDS buffer for their worst world,
   // up to 8192 entries, so that each body's contact list is walked out of LDS.
-  m->xpbd_contact_cap = 1024;
+  m->xpbd_contact_cap = 512;
   if (d.mesh_count > 0 && segs > 1) {
     std::vector<int> starts(segs + 1), seg_shape, stype(d.shape_count > 0 ? d.shape_count : 1);
     if ((e = hipMemcpy(starts.data(), d.segment_start, sizeof(int) * (segs + 1), hipMemcpyDeviceToHost)) != hipSuccess) {
@@ -202,7 +202,7 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       worst_world = c > worst_world ? c : worst_world;
     }
     long long want = (2 * worst_world + 63) / 64 * 64;
-    m->xpbd_contact_cap = static_cast<int>(want < 1024 ? 1024 : (want > 8192 ? 8192 : want));
+    m->xpbd_contact_cap = static_cast<int>(want < 512 ? 512 : (want > 8192 ? 8192 : want));
   }
   m->xpbd_keys_cap = 2ll * (d.rigid_contact_max > 0 ? d.rigid_contact_max : 1);
   m->xpbd_joint_lds = 0;

@@ -443,6 +443,13 @@ struct XpbdArgs {
   NmcXpbdReport rep;  // XM_REPORT: contact impulses, joint impulses, body_parent_f
 };
 
+// Floats of the contribution region: 12 per slot of a solve chunk, at least the 6 per body the body forces
+// take there before the iterations.
+__host__ __device__ inline int xpbd_contrib_floats(int chunk, int max_bodies) {
+  const int a = 12 * chunk, b = 6 * (max_bodies > 0 ? max_bodies : 1);
+  return a > b ? a : b;
+}
+
 // Solver modes (template parameter of the world kernel).  XM_REPORT is the plain step that also
 // accumulates the reported contact and joint impulses.
 enum { XM_PLAIN = 0, XM_LOG = 1, XM_PREP = 2, XM_WARM = 3, XM_REPORT = 4 };
@@ -796,7 +803,12 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   const int nj = d.world_joint_count[w];
   const int mb = a.lds_bodies > 0 ? a.lds_bodies : 1;
   const int mj = a.joint_lds;
-  constexpr int CH = BS * CPL;  // contacts per chunk
+  // CR contacts are held in registers for the whole step; each iteration solves CH of them at a time (one
+  // register set per chunk in the world-frame kernels, whose contribution slots are then a quarter of the
+  // LDS they would otherwise take; all of them at once in the reference-order kernels)
+  constexpr bool SPLIT_SHAPE = FAST && BPL == 1;
+  constexpr int CR = BS * CPL;
+  constexpr int CH = SPLIT_SHAPE ? BS : CR;
   const float dt = a.dt;
   const int tid = threadIdx.x;
 
@@ -805,18 +817,28 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   L.F = reinterpret_cast<float4*>(p); p += sizeof(float4) * fastm::FRAME_F4 * (mb + 1);
   L.q = reinterpret_cast<float*>(p); p += sizeof(float) * 7 * mb;
   L.qd = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * (mb + 1);
-  float* fl = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * mb;
   L.com = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * mb;
   L.inv_m = reinterpret_cast<float*>(p); p += sizeof(float) * mb;
   L.inv_I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
-  L.I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
+  if (FAST) {
+    // the body-frame inertia (apply only) is read from the model arrays
+    L.I = const_cast<float*>(d.body_inertia) + 9ll * b0;
+  } else {
+    L.I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
+  }
   L.flags = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* seg = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* cur = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* scan_tmp = reinterpret_cast<int*>(p); p += sizeof(int) * BS;
   p = smem + ((p - smem + 15) & ~static_cast<ptrdiff_t>(15));  // 16-byte aligned contribution slots
-  float* contrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * CH;
-  int* active = reinterpret_cast<int*>(p); p += sizeof(int) * CH;
+  // contribution slots; before the iterations the body forces use their last 6 mb floats (the report mode,
+  // which uses them during the iterations, has them apart)
+  const int cfl = xpbd_contrib_floats(CH, mb);
+  float* contrib = reinterpret_cast<float*>(p); p += sizeof(float) * cfl;
+  float* fl = contrib + (cfl - 6 * mb);
+  if (MODE == XM_REPORT) { fl = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * mb; }
+  int* active = reinterpret_cast<int*>(p);
+  if (!SPLIT_SHAPE) p += sizeof(int) * CH;
   float* jcontrib = reinterpret_cast<float*>(p); p += sizeof(float) * 12 * mj;
   int* jactive = reinterpret_cast<int*>(p); p += sizeof(int) * mj;
   unsigned* keys = reinterpret_cast<unsigned*>(p);
@@ -828,9 +850,9 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   // body over the world's contacts (the body is shape 0's / shape 1's), built by LDS atomic OR (order-free)
   // in the contribution slots (free until the iterations), then written out in ascending order by the body's
   // lane: no atomic scatter, no sort.  Worlds whose masks do not fit there use the counting sort.
-  constexpr bool SPLIT_SHAPE = FAST && BPL == 1;
   const int mask_words = (nc + 31) >> 5;  // 32-bit words per body and side
-  const bool masks = SPLIT_SHAPE && 2 * nb * mask_words <= 12 * CH;
+  // (the masks share the contribution slots with the body forces at their end)
+  const bool masks = SPLIT_SHAPE && 2 * nb * mask_words + 6 * mb <= cfl;
   if (2 * nc > a.key_cap) {
     // dense world: order its entries in the global buffer at its contact offset
     long long room = a.keys_global_cap - 2ll * cstart;
@@ -946,7 +968,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   }
   for (int c = tid; c < nc; c += BS) {
     CReg r;
-    if (c < CH) {
+    if (c < CR) {
       r = creg[0];
 #pragma unroll
       for (int jj = 1; jj < CPL; ++jj)
@@ -973,7 +995,8 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
     int lb = tid + r * BS;
     if (lb < nb) {
       int b = b0 + lb;
-      for (int k = 0; k < 9; ++k) L.I[9 * lb + k] = d.body_inertia[9 * b + k];
+      if (!FAST)
+        for (int k = 0; k < 9; ++k) L.I[9 * lb + k] = d.body_inertia[9 * b + k];
       if (MODE != XM_WARM && !(L.flags[lb] & BODY_KINEMATIC)) {
         int wi = d.body_world[b];
         int gi = wi > 0 ? wi : 0;
@@ -1024,7 +1047,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       int la = -1, lbb = -1;
       if (c < nc) {
         CReg r;
-        if (c < CH) {
+        if (c < CR) {
           r = creg[0];
 #pragma unroll
           for (int jj = 1; jj < CPL; ++jj)
@@ -1058,7 +1081,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   } else {
     for (int c = tid; c < nc; c += BS) {
       CReg r;
-      if (c < CH) {
+      if (c < CR) {
         r = creg[0];
 #pragma unroll
         for (int jj = 1; jj < CPL; ++jj)
@@ -1089,7 +1112,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   if (MODE == XM_LOG || MODE == XM_PREP) {
     for (int c = tid; c < nc; c += BS) {
       CReg r;
-      if (c < CH) {
+      if (c < CR) {
         r = creg[0];
 #pragma unroll
         for (int jj = 1; jj < CPL; ++jj)
@@ -1140,7 +1163,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 #pragma unroll
   for (int jj = 0; jj < CPL; ++jj) lamacc[jj] = 0.0f;
   if (ACC)
-    for (int c = CH + tid; c < nc; c += BS) {
+    for (int c = CR + tid; c < nc; c += BS) {
       const int pc = cpos(a.c, cstart + c);
       if (pc >= 0) lrn.lambda_sum[pc] = 0.0f;
     }
@@ -1172,7 +1195,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       }
 #pragma unroll
       for (int jj = 0; jj < CPL; ++jj) lamacc[jj] = 0.0f;
-      for (int c = CH + tid; c < nc; c += BS) {
+      for (int c = CR + tid; c < nc; c += BS) {
       const int pc = cpos(a.c, cstart + c);
       if (pc >= 0) lrn.lambda_sum[pc] = 0.0f;
     }
@@ -1197,12 +1220,21 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
         }
         for (int cb = 0; cb < nc; cb += CH) {
 #pragma unroll
-          for (int jj = 0; jj < CPL; ++jj) {
-            int c = cb + jj * BS + tid;
-            int slot = jj * BS + tid;
+          for (int jj = 0; jj < CH / BS; ++jj) {
+            const int c = cb + jj * BS + tid;
+            const int slot = jj * BS + tid;
+            const int kreg = cb / BS + jj;  // the register set that holds contact c (c < CR)
             int act = 0;
             if (c < nc) {
-              CReg r = cb == 0 ? creg[jj] : load_contact(a, cstart + c, b0, nb);
+              CReg r;
+              if (c < CR) {
+                r = creg[0];
+#pragma unroll
+                for (int k = 1; k < CPL; ++k)
+                  if (kreg == k) r = creg[k];
+              } else {
+                r = load_contact(a, cstart + c, b0, nb);
+              }
               Spatial da, db;
               bool on;
               float lam = 0.0f;
@@ -1238,8 +1270,13 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
                   store3(o, da.lin); store3(o + 3, da.ang); store3(o + 6, db.lin); store3(o + 9, db.ang);
                 }
                 if (ACC && !warm) {
-                  if (cb == 0) lamacc[jj] += lam;
-                  else lrn.lambda_sum[cpos(a.c, cstart + c)] += lam;
+                  if (c < CR) {
+#pragma unroll
+                    for (int k = 0; k < CPL; ++k)
+                      if (kreg == k) lamacc[k] += lam;
+                  } else {
+                    lrn.lambda_sum[cpos(a.c, cstart + c)] += lam;
+                  }
                 }
               }
             }
@@ -1401,7 +1438,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       float pen = 0.0f;
       for (int c = tid; c < nc; c += BS) {
         CReg r;
-        if (c < CH) {
+        if (c < CR) {
           r = creg[0];
 #pragma unroll
           for (int jj = 1; jj < CPL; ++jj)
@@ -1438,7 +1475,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
       }
     }
     if (MODE == XM_WARM && !s_fallback) {
-      for (int c = CH + tid; c < nc; c += BS) {
+      for (int c = CR + tid; c < nc; c += BS) {
         const int pc = cpos(a.c, cstart + c);
         if (pc < 0) continue;
         float v = lrn.lambda0[pc];
@@ -1482,6 +1519,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 // block_lanes (NmcXpbdParams) picks 64 or 256 lanes where both fit; contact-heavy worlds with few bodies
 // (triangle meshes: hundreds of contacts per world) may run faster on 256.
 struct XpbdShape { int bs, cpl, bpl; };
+
 static XpbdShape xpbd_shape(int max_bodies, int block_lanes = 0) {
   if (max_bodies <= 64 && block_lanes != 256) return {64, 2, 1};
   if (max_bodies <= 256) return {256, 1, 1};
@@ -1491,23 +1529,35 @@ static XpbdShape xpbd_shape(int max_bodies, int block_lanes = 0) {
 int xpbd_block_size(int max_bodies) { return xpbd_shape(max_bodies).bs; }
 
 // LDS of the largest launch shape the model may use (both shapes when it has at most 64 bodies a world).
-size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints, int key_cap);
-size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap) {
-  size_t a = xpbd_lds_bytes_shape(xpbd_shape(max_bodies, 0), max_bodies, max_joints, key_cap);
-  size_t b = xpbd_lds_bytes_shape(xpbd_shape(max_bodies, 256), max_bodies, max_joints, key_cap);
-  return a > b ? a : b;
-}
-
-size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints, int key_cap) {
-  int bs = sh.bs;
-  size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 0;
-  size_t ch = (size_t)bs * sh.cpl;
-  size_t b = sizeof(float) * (7 + 6 + 6 + 3 + 1 + 9 + 9) * mb + sizeof(int) * mb + 2 * sizeof(int) * (mb + 1);
-  b += sizeof(float) * 4 * 6 * (mb + 1) + sizeof(float) * 6 + sizeof(int);  // frames (+ static slot), static qd, flags
-  b += 16;  // alignment of the contribution slots
-  b += sizeof(int) * bs + sizeof(float) * 12 * ch + sizeof(int) * ch + sizeof(float) * 12 * mj + sizeof(int) * mj;
+// LDS of one world in the kernel's layout (xpbd_world_kernel, in allocation order).
+size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints, int key_cap, bool fast,
+                            bool report) {
+  const size_t mb = max_bodies > 0 ? max_bodies : 1, mj = max_joints > 0 ? max_joints : 0;
+  const bool split = fast && sh.bpl == 1;
+  const int ch = split ? sh.bs : sh.bs * sh.cpl;
+  size_t b = sizeof(float4) * 6 * (mb + 1);                           // frames (+ static slot)
+  b += sizeof(float) * (7 * mb + 6 * (mb + 1) + 3 * mb + mb + 9 * mb);  // q, qd (+ static), com, inv_m, inv_I
+  if (!fast) b += sizeof(float) * 9 * mb;                               // I
+  b += sizeof(int) * 3 * (mb + 1) + sizeof(int) * sh.bs;              // flags, seg, cur, scan
+  b = (b + 15) & ~size_t(15);
+  b += sizeof(float) * (size_t)xpbd_contrib_floats(ch, static_cast<int>(mb));
+  if (report) b += sizeof(float) * 6 * mb;
+  if (!split) b += sizeof(int) * ch;                                    // active flags
+  b += sizeof(float) * 12 * mj + sizeof(int) * mj;                      // joint slots
   b += sizeof(unsigned) * (size_t)key_cap;
   return (b + 15) & ~size_t(15);
+}
+
+// The largest LDS any launch of the model may take (either launch shape, either arithmetic, any mode).
+size_t xpbd_lds_bytes(int max_bodies, int max_joints, int key_cap) {
+  size_t m = 0;
+  for (int lanes : {0, 256})
+    for (int fast = 0; fast < 2; ++fast) {
+      const size_t v = xpbd_lds_bytes_shape(xpbd_shape(max_bodies, lanes), max_bodies, max_joints, key_cap,
+                                            fast != 0, true);
+      m = v > m ? v : m;
+    }
+  return m;
 }
 
 template <int MODE, bool FAST, bool ART>
@@ -1576,7 +1626,8 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
     const Launch& L = ls[i];
     a.lds_bodies = L.bodies;
     a.world_list = L.list;
-    const size_t lds = xpbd_lds_bytes_shape(L.sh, L.bodies, m->xpbd_joint_lds, m->xpbd_contact_cap);
+    const size_t lds = xpbd_lds_bytes_shape(L.sh, L.bodies, m->xpbd_joint_lds, m->xpbd_contact_cap,
+                                            !params->reference_arithmetic, rep != nullptr);
     if (rep) {
       launch_mode<XM_REPORT>(L.sh, L.worlds, lds, stream, a);
       continue;
