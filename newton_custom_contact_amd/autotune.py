@@ -15,8 +15,8 @@ import torch
 
 def candidate_block_lanes(solver) -> list[int]:
     """Launch shapes the solver's model admits: 64 lanes needs worlds of at most 64 bodies; 0 (automatic)
-    launches worlds by size tier (<= 64 bodies on one wave, larger on 256 lanes), 256 runs every world on
-    256 lanes."""
+    launches worlds by size tier (<= 64 bodies on one wave, <= 256 on 256 lanes, <= 512 on 512), 256 runs
+    every world in one launch of the large-world shape."""
     return [64, 256] if solver.native.desc.max_world_bodies <= 64 else [0, 256]
 
 

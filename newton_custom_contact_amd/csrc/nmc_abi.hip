@@ -235,6 +235,16 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     for (int t = 0; t < 3; ++t) used += tier[t].empty() ? 0 : 1;
     m->xpbd_tiered = used > 1;
     if (m->xpbd_tiered) {
+      // side streams (default priority) and fork / join events for the concurrent tier launches
+      e = hipEventCreateWithFlags(&m->tier_fork, hipEventDisableTiming);
+      for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        e = hipStreamCreateWithFlags(&m->tier_stream[k], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&m->tier_join[k], hipEventDisableTiming);
+      }
+      if (e != hipSuccess) {
+        nmc_model_destroy(m);
+        return hip_fail(e, "nmc_model_create: tier streams");
+      }
       for (int t = 0; t < 3; ++t) {
         m->xpbd_tier_count[t] = static_cast<int>(tier[t].size());
         if (tier[t].empty()) continue;
@@ -361,6 +371,11 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
                   m->ord_unit_off, m->ord_base};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  for (int k = 0; k < 2; ++k) {
+    if (m->tier_stream[k]) (void)hipStreamDestroy(m->tier_stream[k]);
+    if (m->tier_join[k]) (void)hipEventDestroy(m->tier_join[k]);
+  }
+  if (m->tier_fork) (void)hipEventDestroy(m->tier_fork);
   delete m;
 }
 

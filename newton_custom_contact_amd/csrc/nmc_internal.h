@@ -104,6 +104,10 @@ struct NmcModelImpl {
   int xpbd_tier_count[3];
   int xpbd_tier_max[3];
   bool xpbd_tiered;
+  // the tiers' launches run concurrently: tier t > 0 on tier_stream[t - 1], forked from and joined into the
+  // caller's stream through events (graph-capturable); created with the model when it is tiered
+  hipStream_t tier_stream[2];
+  hipEvent_t tier_fork, tier_join[2];
   int xpbd_joint_lds;    // joints per world with LDS delta slots: max_world_joints when some enabled joint is
                          // not FREE / DISTANCE (the only joints the joint pass solves), else 0
   unsigned* xpbd_keys;   // [xpbd_keys_cap] ordering entries of denser worlds, at 2 x their contact offset

@@ -1514,16 +1514,17 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 
 // Launch shapes: <lanes, contacts per lane, bodies per lane>.  Worlds up to 64 bodies run on one
 // wave (64 lanes, 128 contacts held in registers, barriers cost one wave's issue); worlds up to 256
-// bodies run one body per lane on 256 lanes; larger worlds (up to 512 bodies) give each lane two.
-// Measured on 4096 worlds x 32 bodies: 64 lanes 0.377 ms per substep, 128 lanes 0.499 ms.
-// block_lanes (NmcXpbdParams) picks 64 or 256 lanes where both fit; contact-heavy worlds with few bodies
-// (triangle meshes: hundreds of contacts per world) may run faster on 256.
+// bodies run one body per lane on 256 lanes, up to 512 on 512 lanes; larger worlds (up to 1024 bodies) give
+// each of 512 lanes two.  Measured on 4096 worlds x 32 bodies: 64 lanes 0.377 ms per substep, 128 lanes
+// 0.499 ms.  block_lanes (NmcXpbdParams) picks 64 or 256 lanes where both fit; contact-heavy worlds with
+// few bodies (triangle meshes: hundreds of contacts per world) may run faster on 256.
 struct XpbdShape { int bs, cpl, bpl; };
 
 static XpbdShape xpbd_shape(int max_bodies, int block_lanes = 0) {
   if (max_bodies <= 64 && block_lanes != 256) return {64, 2, 1};
   if (max_bodies <= 256) return {256, 1, 1};
-  return {256, 1, 2};
+  if (max_bodies <= 512) return {512, 1, 1};
+  return {512, 1, 2};
 }
 
 int xpbd_block_size(int max_bodies) { return xpbd_shape(max_bodies).bs; }
@@ -1564,10 +1565,12 @@ template <int MODE, bool FAST, bool ART>
 static void launch_shape(const XpbdShape& sh, int worlds, size_t lds, hipStream_t stream, const XpbdArgs& a) {
   if (sh.bs == 64)
     hipLaunchKernelGGL((xpbd_world_kernel<64, 2, 1, MODE, FAST, ART>), dim3(worlds), dim3(64), lds, stream, a);
-  else if (sh.bpl == 1)
+  else if (sh.bs == 256)
     hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 1, MODE, FAST, ART>), dim3(worlds), dim3(256), lds, stream, a);
+  else if (sh.bpl == 1)
+    hipLaunchKernelGGL((xpbd_world_kernel<512, 1, 1, MODE, FAST, ART>), dim3(worlds), dim3(512), lds, stream, a);
   else
-    hipLaunchKernelGGL((xpbd_world_kernel<256, 1, 2, MODE, FAST, ART>), dim3(worlds), dim3(256), lds, stream, a);
+    hipLaunchKernelGGL((xpbd_world_kernel<512, 1, 2, MODE, FAST, ART>), dim3(worlds), dim3(512), lds, stream, a);
 }
 
 // reference_arithmetic picks the operation order; models whose joints are all FREE / DISTANCE (joint_lds 0)
@@ -1614,7 +1617,7 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   Launch ls[3];
   int nl = 0;
   if (params->block_lanes == 0 && m->xpbd_tiered) {
-    for (int t = 0; t < 3; ++t)
+    for (int t = 2; t >= 0; --t)  // largest worlds first
       if (m->xpbd_tier_count[t] > 0)
         ls[nl++] = Launch{xpbd_shape(m->xpbd_tier_max[t], 0), m->xpbd_tier_count[t], m->xpbd_tier_max[t],
                           m->xpbd_tier_worlds[t]};
@@ -1622,24 +1625,40 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
     ls[nl++] = Launch{xpbd_shape(m->d.max_world_bodies, params->block_lanes), m->d.world_count + 1,
                       m->d.max_world_bodies, nullptr};
   }
+  // several tiers: launch them concurrently (tier i > 0 on a side stream forked from the caller's stream and
+  // joined back into it), so that each size class's launch shares the chip with the others
+  const bool fork = nl > 1;
+  if (fork) {
+    hipError_t e = hipEventRecord(m->tier_fork, stream);
+    for (int i = 1; i < nl && e == hipSuccess; ++i) e = hipStreamWaitEvent(m->tier_stream[i - 1], m->tier_fork, 0);
+    if (e != hipSuccess) return e;
+  }
   for (int i = 0; i < nl; ++i) {
     const Launch& L = ls[i];
+    hipStream_t s = i == 0 ? stream : m->tier_stream[i - 1];
     a.lds_bodies = L.bodies;
     a.world_list = L.list;
     const size_t lds = xpbd_lds_bytes_shape(L.sh, L.bodies, m->xpbd_joint_lds, m->xpbd_contact_cap,
                                             !params->reference_arithmetic, rep != nullptr);
     if (rep) {
-      launch_mode<XM_REPORT>(L.sh, L.worlds, lds, stream, a);
+      launch_mode<XM_REPORT>(L.sh, L.worlds, lds, s, a);
       continue;
     }
     switch (a.lrn.mode) {
-      case NMC_STEP_LOG: launch_mode<XM_LOG>(L.sh, L.worlds, lds, stream, a); break;
-      case NMC_STEP_PREP: launch_mode<XM_PREP>(L.sh, L.worlds, lds, stream, a); break;
-      case NMC_STEP_WARM: launch_mode<XM_WARM>(L.sh, L.worlds, lds, stream, a); break;
-      default: launch_mode<XM_PLAIN>(L.sh, L.worlds, lds, stream, a); break;
+      case NMC_STEP_LOG: launch_mode<XM_LOG>(L.sh, L.worlds, lds, s, a); break;
+      case NMC_STEP_PREP: launch_mode<XM_PREP>(L.sh, L.worlds, lds, s, a); break;
+      case NMC_STEP_WARM: launch_mode<XM_WARM>(L.sh, L.worlds, lds, s, a); break;
+      default: launch_mode<XM_PLAIN>(L.sh, L.worlds, lds, s, a); break;
     }
   }
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (fork) {
+    for (int i = 1; i < nl && e == hipSuccess; ++i) {
+      e = hipEventRecord(m->tier_join[i - 1], m->tier_stream[i - 1]);
+      if (e == hipSuccess) e = hipStreamWaitEvent(stream, m->tier_join[i - 1], 0);
+    }
+  }
+  return e;
 }
 
 }  // namespace nmc

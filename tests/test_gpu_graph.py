@@ -1,7 +1,9 @@
 """Graph capture of the substep: collide + XPBD step (plain; deterministic with contact matching and
-reported forces) captured once with torch.cuda.CUDAGraph and replayed; every replay, with unrelated eager work
-and a synchronisation between replays, equals the eager substep bit for bit.  The library records no
-memset nodes (scratch is cleared by its own kernels), so replays after the first are checked too.
+reported forces; worlds of uneven size, whose step launches its size tiers concurrently on side streams forked
+from and joined into the capturing stream) captured once with torch.cuda.CUDAGraph and replayed; every replay,
+with unrelated eager work and a synchronisation between replays, equals the eager substep bit for bit.  The
+library records no memset nodes (scratch is cleared by its own kernels), so replays after the first are
+checked too.
 """
 
 from __future__ import annotations
@@ -18,8 +20,8 @@ pytestmark = pytest.mark.gpu
 DT = 1.0 / 240.0
 
 
-def _setup(dev, report: bool):
-    model = scenes.mixed_worlds(16, seed=5, device=dev)
+def _setup(dev, report, uneven=False):
+    model = scenes.uneven_worlds(12, seed=8, device=dev) if uneven else scenes.mixed_worlds(16, seed=5, device=dev)
     if report:
         model.request_contact_attributes("force")
         model.request_state_attributes("body_parent_f")
@@ -37,12 +39,16 @@ def _outputs(c, s, report):
     return out
 
 
-@pytest.mark.parametrize("report", [False, True], ids=["plain", "matching_and_forces"])
-def test_graph_replays_equal_eager(cuda_device, report):
-    print(f"[graph] report={report}", flush=True)
+@pytest.mark.parametrize("report,uneven", [(False, False), (True, False), (False, True)],
+                         ids=["plain", "matching_and_forces", "uneven_tiers"])
+def test_graph_replays_equal_eager(cuda_device, report, uneven):
+    print(f"[graph] report={report} uneven={uneven}", flush=True)
     steps = 12
     # eager reference
-    model, pipe, solver = _setup(cuda_device, report)
+    model, pipe, solver = _setup(cuda_device, report, uneven)
+    if uneven:
+        counts = model.host["world_body_count"][:-1]
+        assert counts.max() > 64 and (counts <= 64).any(), "the scene should launch its step in several size tiers"
     c = pipe.contacts()
     a, b = model.state(), model.state()
     q0, qd0 = a.body_q.clone(), a.body_qd.clone()
@@ -57,7 +63,7 @@ def test_graph_replays_equal_eager(cuda_device, report):
         torch.cuda.synchronize()
         ref.append([t.cpu().numpy() for t in _outputs(c, b, report)])
     # the same substep captured and replayed (fresh objects, same initial state)
-    model, pipe, solver = _setup(cuda_device, report)
+    model, pipe, solver = _setup(cuda_device, report, uneven)
     c = pipe.contacts()
     a, b = model.state(), model.state()
 
