@@ -1513,16 +1513,18 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
 }
 
 // Launch shapes: <lanes, contacts per lane, bodies per lane>.  Worlds up to 64 bodies run on one
-// wave (64 lanes, 128 contacts held in registers, barriers cost one wave's issue); worlds up to 256
-// bodies run one body per lane on 256 lanes, up to 512 on 512 lanes; larger worlds (up to 1024 bodies) give
-// each of 512 lanes two.  Measured on 4096 worlds x 32 bodies: 64 lanes 0.377 ms per substep, 128 lanes
-// 0.499 ms.  block_lanes (NmcXpbdParams) picks 64 or 256 lanes where both fit; contact-heavy worlds with
-// few bodies (triangle meshes: hundreds of contacts per world) may run faster on 256.
+// wave (64 lanes, 128 contacts held in registers, barriers cost one wave's issue) or on 256 lanes
+// (block_lanes 256: contact-heavy worlds with few bodies, e.g. triangle meshes with hundreds of contacts a
+// world); larger worlds run one body per lane on 512 lanes (up to 512 bodies; two a lane up to 1024).  A
+// world that large takes most of a CU's LDS, so its workgroup should bring as many waves as it can: on the
+// uneven workload (2048 worlds of 8-262 bodies) 512 lanes in place of 256 for the worlds above 64 bodies took
+// the tiered step from 1.56 to 1.30 ms per substep.  On 4096 worlds x 32 bodies: 64 lanes 0.377 ms per
+// substep, 128 lanes 0.499.
 struct XpbdShape { int bs, cpl, bpl; };
 
 static XpbdShape xpbd_shape(int max_bodies, int block_lanes = 0) {
   if (max_bodies <= 64 && block_lanes != 256) return {64, 2, 1};
-  if (max_bodies <= 256) return {256, 1, 1};
+  if (max_bodies <= 64) return {256, 1, 1};
   if (max_bodies <= 512) return {512, 1, 1};
   return {512, 1, 2};
 }
