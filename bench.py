@@ -122,24 +122,39 @@ def start_uneven_build(args):
 
 
 def phase_times(pipe, solver, a, b, contacts, sdt, reps):
-    """Average launch durations on the launch stream (HIP events): `reps` back-to-back collides, then `reps`
-    back-to-back steps, at the given state, so the GPU never waits on the host."""
+    """Average durations on the launch stream (HIP events around graph replays, as in the timed loop, so the
+    GPU never waits on the host's launch rate): `reps` back-to-back collides, then `reps` back-to-back steps,
+    at the given state."""
     import torch
 
-    stream = torch.cuda.current_stream()
     pipe.collide(a, contacts)
     torch.cuda.synchronize()
-    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    ev[0].record(stream)
-    for _ in range(reps):
-        pipe.collide(a, contacts)
-    ev[1].record(stream)
-    ev[2].record(stream)
-    for _ in range(reps):
-        solver.step(a, b, None, contacts, sdt)
-    ev[3].record(stream)
-    torch.cuda.synchronize()
-    return ev[0].elapsed_time(ev[1]) / reps, ev[2].elapsed_time(ev[3]) / reps
+    graphs = []
+    for work in (lambda: pipe.collide(a, contacts), lambda: solver.step(a, b, None, contacts, sdt)):
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            work()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            for _ in range(reps):
+                work()
+        graphs.append(g)
+    stream = torch.cuda.current_stream()
+    out = []
+    for g in graphs:
+        g.replay()  # warm
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        g.replay()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        out.append(e0.elapsed_time(e1) / reps)
+    del graphs
+    return out[0], out[1]
 
 
 def timed_form(args, model, pipe, solver, worlds, settle, warmup, steps, dev, tune=None):
@@ -214,21 +229,31 @@ def timed_form(args, model, pipe, solver, worlds, settle, warmup, steps, dev, tu
     return states, contacts, ms, n_c, t_col, t_xp, tuned
 
 
-def roofline_block(t_col, t_xp, worlds, b_collide, b_step, traffic=None, traffic_at=None):
-    """The dominant phase (XPBD launch or the collide sequence) priced with its share of the algorithmic bytes."""
-    collide_kernels = ("collide_segment", "route_scan_kernel", "narrow_kernel", "contact_count_kernel",
-                       "contact_scan_kernel", "contact_compact_kernel")
-    if t_xp >= t_col:
-        dom, dom_ms, dom_bytes, kernels = "xpbd_world_kernel", t_xp, worlds * b_step, ("xpbd_world_kernel",)
+COLLIDE_KERNELS = ("collide_segment", "route_scan_kernel", "narrow_kernel", "contact_count_kernel",
+                   "contact_scan_kernel", "contact_compact_kernel")
+
+
+def roofline_block(t_col, t_xp, worlds, b_collide, b_step, traffic=None, traffic_at=None, xpbd_kernel=True):
+    """The dominant kernel priced with its share of the algorithmic bytes, the collide phase beside it.  On the
+    primitive workloads that is the XPBD step (one launch per substep, the largest single kernel; the collide
+    phase is six smaller ones); xpbd_kernel=False prices whichever phase takes longer (the mesh workload's
+    collide phase, whose mesh narrow phase is its largest kernel)."""
+    col = worlds * b_collide / (t_col / 1e3) / 1e9
+    collide = {"kernels": list(COLLIDE_KERNELS), "ms": round(t_col, 4), "achieved": round(col, 2),
+               "frac": round(col / HBM_PEAK_GBS, 5), "traffic": traffic(COLLIDE_KERNELS) if traffic else None}
+    if xpbd_kernel or t_xp >= t_col:
+        achieved = worlds * b_step / (t_xp / 1e3) / 1e9
+        out = {"kernel": "xpbd_world_kernel", "launch_ms": round(t_xp, 4), "bound": "hbm",
+               "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(achieved / HBM_PEAK_GBS, 5),
+               "traffic": traffic(("xpbd_world_kernel",)) if traffic else None}
     else:
-        dom, dom_ms, dom_bytes = "collide phase: " + " + ".join(collide_kernels), t_col, worlds * b_collide
-        kernels = collide_kernels
-    achieved = dom_bytes / (dom_ms / 1e3) / 1e9
-    out = {"kernel": dom, "launch_ms": round(dom_ms, 4), "bound": "hbm", "achieved": round(achieved, 2),
-           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
-           "traffic": traffic(kernels) if traffic else None}
+        out = {"kernel": "collide phase: " + " + ".join(COLLIDE_KERNELS), "launch_ms": round(t_col, 4),
+               "bound": "hbm", "achieved": collide["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": collide["frac"], "traffic": collide["traffic"]}
     if traffic_at:
         out["traffic_at"] = traffic_at
+    out["collide_phase"] = collide
     return out
 
 
@@ -730,7 +755,7 @@ def sdf_form(args, dev):
         "fraction_of_step_roofline": round(ws / roof_ws, 5),
         "collide_ms_per_substep": round(t_col, 4),
         "xpbd_ms_per_substep": round(t_xp, 4),
-        "roofline": roofline_block(t_col, t_xp, args.sdf_worlds, b_collide, b_step),
+        "roofline": roofline_block(t_col, t_xp, args.sdf_worlds, b_collide, b_step, xpbd_kernel=False),
         "overflow": oc,
         "step_kernel_lanes": tuned,
         "lowest_body_z_m": round(float(states[0].body_q[:, 2].min()), 4),

@@ -123,6 +123,35 @@ struct EnumRun {
   bool regular, sweep, listed;
 };
 
+// Whether shape a (segment rank) leads the pair {a, b} in the narrow phase: lower GeoType, then lower rank; a
+// mesh before an infinite plane.
+__device__ __forceinline__ bool pair_lead(int a, int b, int ta, int tb) {
+  bool first = a != b && (ta < tb || (ta == tb && a < b));
+  if (ta == GEO_PLANE && tb == GEO_MESH) first = false;
+  if (ta == GEO_MESH && tb == GEO_PLANE) first = true;
+  return first;
+}
+
+// The broad phase's tests of a pair other than the sweep's (ranks la < lb): world, collision group, AABB
+// overlap, filter or explicit list.
+__device__ __forceinline__ bool pair_accepted(const NmcModelDesc& d, const SegLds& L, int la, int lb, bool regular,
+                                              bool listed) {
+  const int w1 = L.world[la], w2 = L.world[lb];
+  bool keep = !(w1 == -1 && w2 == -1 && regular);
+  if (keep && !listed) keep = !(w1 != -1 && w2 != -1 && w1 != w2) && group_pair(L.group[la], L.group[lb]);
+  if (keep) {
+    const float c = 0.0f + 0.0f;
+    keep = L.lo[3 * la] <= L.hi[3 * lb] + c && L.hi[3 * la] >= L.lo[3 * lb] - c &&
+           L.lo[3 * la + 1] <= L.hi[3 * lb + 1] + c && L.hi[3 * la + 1] >= L.lo[3 * lb + 1] - c &&
+           L.lo[3 * la + 2] <= L.hi[3 * lb + 2] + c && L.hi[3 * la + 2] >= L.lo[3 * lb + 2] - c;
+  }
+  if (keep) {
+    if (listed) keep = pair_listed(d.explicit_pairs, d.explicit_pair_count, L.gid[la], L.gid[lb]);
+    else keep = !excluded(L.gid[lb], d.filter_pairs, L.fbeg[la], L.fend[la]);
+  }
+  return keep;
+}
+
 // Visits the accepted pairs of the lane's run in order: f(a, b, route class).
 template <class F>
 __device__ __forceinline__ void enum_pairs(const NmcModelDesc& d, const SegLds& L, const EnumRun& r, F&& f) {
@@ -132,28 +161,14 @@ __device__ __forceinline__ void enum_pairs(const NmcModelDesc& d, const SegLds& 
   int ta = L.type[a];
   for (long long q = r.p0; q < r.p1; ++q) {
     const int tb = L.type[b];
-    bool first = a != b && (ta < tb || (ta == tb && a < b));
-    if (ta == GEO_PLANE && tb == GEO_MESH) first = false;
-    if (ta == GEO_MESH && tb == GEO_PLANE) first = true;
-    if (first) {
+    if (pair_lead(a, b, ta, tb)) {
       const int la = a < b ? a : b, lb = a < b ? b : a;  // lower rank first for the tests
-      const int w1 = L.world[la], w2 = L.world[lb];
-      bool keep = !(w1 == -1 && w2 == -1 && r.regular);
-      if (keep && !r.listed) keep = !(w1 != -1 && w2 != -1 && w1 != w2) && group_pair(L.group[la], L.group[lb]);
-      if (keep && r.sweep) {
+      bool keep = true;
+      if (r.sweep) {
         const bool a_early = L.skeys[a] < L.skeys[b];
         keep = a_early ? (L.plo[b] < L.phi[a]) : (L.plo[a] < L.phi[b]);
       }
-      if (keep) {
-        const float c = 0.0f + 0.0f;
-        keep = L.lo[3 * la] <= L.hi[3 * lb] + c && L.hi[3 * la] >= L.lo[3 * lb] - c &&
-               L.lo[3 * la + 1] <= L.hi[3 * lb + 1] + c && L.hi[3 * la + 1] >= L.lo[3 * lb + 1] - c &&
-               L.lo[3 * la + 2] <= L.hi[3 * lb + 2] + c && L.hi[3 * la + 2] >= L.lo[3 * lb + 2] - c;
-      }
-      if (keep) {
-        if (r.listed) keep = pair_listed(d.explicit_pairs, d.explicit_pair_count, L.gid[la], L.gid[lb]);
-        else keep = !excluded(L.gid[lb], d.filter_pairs, L.fbeg[la], L.fend[la]);
-      }
+      if (keep) keep = pair_accepted(d, L, la, lb, r.regular, r.listed);
       if (keep) f(a, b, route_class(ta, tb));
     }
     if (++b == n) {
@@ -162,6 +177,181 @@ __device__ __forceinline__ void enum_pairs(const NmcModelDesc& d, const SegLds& 
       if (a < n) ta = L.type[a];
     }
   }
+}
+
+// Ascending bitonic sort of a[0, npow2) (npow2 a power of two) by the whole workgroup in LDS.
+template <class T>
+__device__ void block_bitonic_sort(T* a, int npow2) {
+  for (int k = 2; k <= npow2; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < npow2; i += blockDim.x) {
+        const int ixj = i ^ j;
+        if (ixj > i) {
+          const T x = a[i], y = a[ixj];
+          if ((x > y) == ((i & k) == 0)) {
+            a[i] = y;
+            a[ixj] = x;
+          }
+        }
+      }
+      __syncthreads();
+    }
+}
+
+constexpr int SWEEP_MAX_SHAPES = 512;  // segments the sweep path takes (above 64 shapes, up to this many)
+constexpr int SWEEP_MAX_PAIRS = 4096;  // accepted pairs its LDS buffer holds; more: the enumeration
+
+// SAP broad phase of one large segment by a 256-lane workgroup (broad_phase_sap.py:99-141, 189-394): the
+// shapes sorted by (projected lower bound, rank) in LDS, each swept against the later ones that start below
+// its upper bound, the accepted pairs put in key order (first * n + second over the segment's ranks, the
+// enumeration's row-major order: a counting sort by the leading shape, then each pair placed in its bucket by
+// the number of smaller keys there) and written like the enumeration writes them.  The pair set is the enumeration's: the
+// same tests, each pair visited once.  Returns false, having written nothing, when the segment finds more
+// than SWEEP_MAX_PAIRS pairs.
+__device__ bool segment_pairs_sweep(const BlockGroup g, const NmcModelDesc& d, const SegLds& L, int n, bool regular,
+                                    const StageBuffers& st, long long poff, int pcap, int* __restrict__ seg_pair_count,
+                                    int* __restrict__ seg_conv_count, int* __restrict__ seg_mesh_count,
+                                    int* __restrict__ counters, int seg, int4* s_tmp) {
+  __shared__ unsigned long long s_sk[SWEEP_MAX_SHAPES];
+  __shared__ unsigned s_pk[SWEEP_MAX_PAIRS];
+  __shared__ unsigned s_po[SWEEP_MAX_PAIRS];
+  __shared__ int s_cnt[SWEEP_MAX_SHAPES + 1];
+  __shared__ int s_off[SWEEP_MAX_SHAPES + 1];
+  __shared__ int s_wv[CBLOCK / 64];
+  __shared__ int s_np;
+  int npow = 1;
+  while (npow < n) npow <<= 1;
+  for (int i = g.lane; i < npow; i += CBLOCK) s_sk[i] = i < n ? L.skeys[i] : ~0ull;
+  for (int i = g.lane; i < n; i += CBLOCK) s_cnt[i] = 0;
+  if (g.lane == 0) {
+    s_cnt[SWEEP_MAX_SHAPES] = 0;
+    s_np = 0;
+  }
+  g.sync();
+  block_bitonic_sort(s_sk, npow);
+  // sweep: the earlier of two in (projected lower bound, rank) order keeps each later one that starts below
+  // its upper bound.  The lower bounds ascend in sort order (NaN bounds sit at the ends, where the test fails),
+  // so shape k's candidates are the run k+1 .. end_k - 1, end_k found by binary search; the runs are laid end
+  // to end and split evenly over the lanes (one shape -- an infinite plane -- may span the whole segment).
+  for (int base = 0; base < n; base += CBLOCK) {
+    const int k = base + g.lane;
+    int len = 0;
+    if (k < n) {
+      const float hk = L.phi[static_cast<int>(s_sk[k] & 0xFFFFFFFFull)];
+      int lo = k + 1, hi = n;  // first j in (k, n] whose lower bound does not start below hk
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (L.plo[static_cast<int>(s_sk[mid] & 0xFFFFFFFFull)] < hk) lo = mid + 1; else hi = mid;
+      }
+      len = lo - k - 1;
+    }
+    const int incl = block_inclusive_scan(len, s_wv);
+    const int carry = base == 0 ? 0 : s_off[base];
+    g.sync();
+    if (k < n) s_off[k] = carry + incl - len;
+    if (g.lane == CBLOCK - 1) s_off[(base + CBLOCK < n ? base + CBLOCK : n)] = carry + incl;
+    g.sync();
+  }
+  const int total = s_off[n];
+  const int per = (total + CBLOCK - 1) / CBLOCK;
+  int c = g.lane * per < total ? g.lane * per : total;
+  const int c_end = c + per < total ? c + per : total;
+  if (c < c_end) {
+    int k = 0, hi = n - 1;  // the last shape whose run starts at or before c
+    while (k < hi) {
+      const int mid = (k + hi + 1) >> 1;
+      if (s_off[mid] <= c) k = mid; else hi = mid - 1;
+    }
+    int j = k + 1 + (c - s_off[k]);
+    for (; c < c_end; ++c) {
+      while (c >= s_off[k + 1]) {  // past shape k's run: the next shape with a non-empty run
+        ++k;
+        j = k + 1;
+      }
+      const int rk = static_cast<int>(s_sk[k] & 0xFFFFFFFFull);
+      const int rj = static_cast<int>(s_sk[j] & 0xFFFFFFFFull);
+      ++j;
+      const int la = rk < rj ? rk : rj, lb = rk < rj ? rj : rk;
+      if (!pair_accepted(d, L, la, lb, regular, false)) continue;
+      const bool lead = pair_lead(rk, rj, L.type[rk], L.type[rj]);
+      const unsigned key = static_cast<unsigned>(lead ? rk * n + rj : rj * n + rk);
+      const int pos = atomicAdd(&s_np, 1);
+      if (pos < SWEEP_MAX_PAIRS) {
+        s_pk[pos] = key;
+        atomicAdd(&s_cnt[lead ? rk : rj], 1);
+      }
+    }
+  }
+  g.sync();
+  const int np_all = s_np;
+  if (np_all > SWEEP_MAX_PAIRS) return false;
+  // key order: offsets of the leading shapes' buckets (exclusive scan of the counts), scatter, then each
+  // bucket sorted by its shape's lane (a shape leads a handful of pairs)
+  for (int base = 0; base < n; base += CBLOCK) {
+    const int i = base + g.lane;
+    const int v = i < n ? s_cnt[i] : 0;
+    const int incl = block_inclusive_scan(v, s_wv);
+    const int carry = s_cnt[SWEEP_MAX_SHAPES];  // total of the earlier passes (slot past the shapes)
+    g.sync();
+    if (i < n) s_cnt[i] = carry + incl - v;
+    g.sync();
+    if (g.lane == CBLOCK - 1) s_cnt[SWEEP_MAX_SHAPES] = carry + incl;
+    g.sync();
+  }
+  for (int q = g.lane; q < np_all; q += CBLOCK) {
+    const unsigned key = s_pk[q];
+    const int pos = atomicAdd(&s_cnt[key / static_cast<unsigned>(n)], 1);
+    s_po[pos] = key;
+  }
+  g.sync();
+  // each pair's place in its bucket: the number of smaller keys there (keys are unique); s_cnt[a] is now the
+  // end of a's bucket, its start the previous shape's end (0 for the first)
+  for (int q = g.lane; q < np_all; q += CBLOCK) {
+    const unsigned key = s_po[q];
+    const int a = static_cast<int>(key / static_cast<unsigned>(n));
+    const int b0 = a > 0 ? s_cnt[a - 1] : 0, b1 = s_cnt[a];
+    int rank = 0;
+    for (int i = b0; i < b1; ++i) rank += s_po[i] < key ? 1 : 0;
+    s_pk[b0 + rank] = key;
+  }
+  g.sync();
+  // the first pcap pairs in key order are kept (the rest dropped and counted, as the enumeration does)
+  int np = np_all;
+  if (np > pcap) {
+    if (g.lane == 0) atomicAdd(&counters[0], np - pcap);
+    np = pcap;
+  }
+  const int chunk = (np + CBLOCK - 1) / CBLOCK;
+  const int q0 = g.lane * chunk < np ? g.lane * chunk : np;
+  const int q1 = q0 + chunk < np ? q0 + chunk : np;
+  int4 mine = make_int4(0, 0, 0, 0);
+  for (int q = q0; q < q1; ++q) {
+    const unsigned key = s_pk[q];
+    const int a = static_cast<int>(key / static_cast<unsigned>(n)), b = static_cast<int>(key % static_cast<unsigned>(n));
+    const int cls = route_class(L.type[a], L.type[b]);
+    mine.x += 1;
+    if (cls == 0) mine.y += 1; else if (cls == 1) mine.z += 1; else mine.w += 1;
+  }
+  int4 tot;
+  const int4 at = group_scan4(g, mine, tot, s_tmp);
+  // route order: MPR/GJK (class 1), then analytic (class 0), then mesh (class 2)
+  const int base0 = tot.z, base1 = 0, base2 = tot.z + tot.y;
+  int c0 = at.y, c1 = at.z, c2 = at.w;
+  for (int q = q0; q < q1; ++q) {
+    const unsigned key = s_pk[q];
+    const int a = static_cast<int>(key / static_cast<unsigned>(n)), b = static_cast<int>(key % static_cast<unsigned>(n));
+    const int cls = route_class(L.type[a], L.type[b]);
+    st.pair_a[poff + q] = L.gid[a];
+    st.pair_b[poff + q] = L.gid[b];
+    const int slot = cls == 1 ? base1 + c1++ : (cls == 0 ? base0 + c0++ : base2 + c2++);
+    st.route[poff + slot] = q;
+  }
+  if (g.lane == 0) {
+    seg_pair_count[seg] = np;
+    seg_conv_count[seg] = tot.z;
+    seg_mesh_count[seg] = tot.w;
+  }
+  return true;
 }
 
 template <class G>
@@ -226,6 +416,15 @@ __device__ void segment_broad_phase_enum(const G g, const NmcModelDesc& d, const
   r.regular = regular;
   r.sweep = d.broad_phase_mode == 0;
   r.listed = d.broad_phase_mode == 2;
+  const long long poff_s = seg_pair_off[seg];
+  const int pcap_s = seg_pair_cap[seg];
+  if constexpr (G::size == CBLOCK) {
+    // large segments under SAP: sort and sweep (the enumeration is quadratic in the shape count)
+    if (r.sweep && n > 64 && n <= SWEEP_MAX_SHAPES &&
+        segment_pairs_sweep(g, d, L, n, regular, st, poff_s, pcap_s, seg_pair_count, seg_conv_count, seg_mesh_count,
+                            counters, seg, s_tmp))
+      return;
+  }
   const long long N2 = static_cast<long long>(n) * n;
   const long long chunk = (N2 + G::size - 1) / G::size;
   r.p0 = chunk * g.lane;
