@@ -165,8 +165,10 @@ typedef struct NmcXpbdParams {
   float joint_linear_compliance;
   float joint_angular_compliance;
   int has_contacts; /* 0: step(..., contacts=None, ...) */
-  int block_lanes;  /* lanes per world: 0 = automatic (64 up to 64 bodies, else 256), 64 or 256; the
-                     * result does not depend on it (an autotuner picks the faster one per model) */
+  int block_lanes;  /* launch shape: 0 = automatic (one 64-lane wave a world up to 64 bodies; models with
+                     * larger worlds launch by size tier, worlds above 64 bodies on 512 lanes), 64 (worlds
+                     * of up to 64 bodies) or 256 (every world in one launch: 256 lanes up to 64 bodies a
+                     * world, else 512); the result does not depend on it (an autotuner picks per model) */
   int reference_arithmetic; /* 0 (default): contact rows and body updates in world-frame arithmetic (rotation
                              * matrices, world inverse inertia, FMA, hardware reciprocal), within rounding of
                              * the reference; 1: the reference's operation order, bit-identical to a CPU

@@ -4,7 +4,7 @@ The step kernel runs one workgroup per world.  Two shapes exist for worlds of up
 64-lane wave holding 128 contacts in registers (best when a world has a few hundred contacts at most),
 and 256 lanes holding 256 (more lanes per world for contact-heavy worlds, e.g. triangle meshes with up
 to 64 contacts per pair).  Models with larger worlds can launch by size tier (automatic, 0) or run every
-world on 256 lanes.  All give bit-identical results, so the choice is purely a timing question, answered
+world in one launch of the large-world shape (256: 512 lanes a world).  All give bit-identical results, so the choice is purely a timing question, answered
 here by timing the solver on the caller's own state and contacts.
 """
 
@@ -15,8 +15,8 @@ import torch
 
 def candidate_block_lanes(solver) -> list[int]:
     """Launch shapes the solver's model admits: 64 lanes needs worlds of at most 64 bodies; 0 (automatic)
-    launches worlds by size tier (<= 64 bodies on one wave, <= 256 on 256 lanes, <= 512 on 512), 256 runs
-    every world in one launch of the large-world shape."""
+    launches worlds by size tier (<= 64 bodies on one wave, larger worlds on 512 lanes), 256 runs every
+    world in one launch of the large-world shape."""
     return [64, 256] if solver.native.desc.max_world_bodies <= 64 else [0, 256]
 
 

@@ -1613,8 +1613,8 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   a.counters = m->counters;
   if (lrn) a.lrn = *lrn; else a.lrn = NmcLearnIO{};
   if (rep) a.rep = *rep; else a.rep = NmcXpbdReport{};
-  // launches: one per size tier of worlds (automatic shape: <= 64 bodies on one wave, <= 256 and <= 512 on
-  // 256 lanes), or all worlds in one launch of the requested shape
+  // launches: one per size tier of worlds (automatic shape: <= 64 bodies on one wave, larger on 512 lanes),
+  // or all worlds in one launch of the requested shape
   struct Launch { XpbdShape sh; int worlds, bodies; const int* list; };
   Launch ls[3];
   int nl = 0;
