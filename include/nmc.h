@@ -95,10 +95,18 @@ typedef struct NmcModelDesc {
   int shape_pairs_max;
   /* triangle meshes (GeoType MESH, newton/_src/geometry/types.py Mesh): shape_mesh[s] = mesh index or -1;
    * local AABB of each shape with its scale baked in (collide.py:215-245); mesh vertices and unique edges
-   * (builder.py:10316-10351, edge order of Mesh.edges); one dense FP32 SDF grid per mesh: (cells+1)
-   * vertices per axis from `lower` with spacing `voxel`, x fastest, negative inside; mesh_sdf_flags bit 0
-   * set when the shape scale was multiplied into the vertices before the cook (TextureSDFData.scale_baked,
-   * sdf_contact.py:918-930: the SDF side is then sampled with unit scale). */
+   * (builder.py:10316-10351, edge order of Mesh.edges).  One sparse SDF per mesh, the reference's texture SDF
+   * (TextureSDFData, sdf_texture.py:48-87, cooked by nmc_sdf_build / nmc_sdf_subgrids / nmc_sdf_pack):
+   *   coarse: (d+1)(h+1)(w+1) floats, x fastest -- the field at the subgrid corners;
+   *   slots: w*h*d uint32, z fastest (index (bx*h + by)*d + bz) -- texture block x | y << 10 | z << 20 of a
+   *     narrow-band subgrid, 0xFFFFFFFE for a subgrid demoted to the coarse grid, 0xFFFFFFFF for far field;
+   *   texels: ts^3 samples, x fastest, 9^3 per block, float32 or unorm16 / unorm8 (value = raw * range + min);
+   * mesh_sdf_start[3m..3m+2] = offsets of mesh m into the coarse (floats), slot (uint32) and texel (bytes)
+   * arrays; mesh_sdf_box[9m..] = sdf_box_lower, sdf_box_upper (the padded box), inv_sdf_dx;
+   * mesh_sdf_quant[2m..] = subgrids_min_sdf_value, subgrids_sdf_value_range; mesh_sdf_dims[5m..] = w, h, d,
+   * ts, bytes per texel; mesh_sdf_flags bit 0 set when the shape scale was multiplied into the vertices
+   * before the cook (TextureSDFData.scale_baked, sdf_contact.py:918-930: the SDF side is then sampled with
+   * unit scale). */
   const int* shape_mesh;
   const float* shape_aabb_lower; /* S x 3 */
   const float* shape_aabb_upper; /* S x 3 */
@@ -107,11 +115,13 @@ typedef struct NmcModelDesc {
   const int* mesh_vertex_start;  /* M + 1 */
   const int* mesh_edge;          /* E x 2, vertex indices local to the mesh */
   const int* mesh_edge_start;    /* M + 1 */
-  const float* mesh_sdf;         /* all grids */
-  const int64_t* mesh_sdf_start; /* M */
-  const float* mesh_sdf_lower;   /* M x 3 */
-  const float* mesh_sdf_voxel;   /* M */
-  const int* mesh_sdf_cells;     /* M x 3 */
+  const float* mesh_sdf_coarse;
+  const uint32_t* mesh_sdf_slots;
+  const uint8_t* mesh_sdf_texels;
+  const int64_t* mesh_sdf_start; /* M x 3 */
+  const float* mesh_sdf_box;     /* M x 9 */
+  const float* mesh_sdf_quant;   /* M x 2 */
+  const int* mesh_sdf_dims;      /* M x 5 */
   const int* mesh_sdf_flags;     /* M */
   /* CollisionPipeline(reduce_contacts, deterministic) for mesh pairs: 0 every contact the writer keeps
    * (narrow_phase.py:1090-1190, sdf_contact.py:830-1065); 1 the reference's normal-bin / voxel reduction with
@@ -275,16 +285,36 @@ int nmc_mlp_forward(const NmcMlp* weights, const float* feat, const int* count, 
 int nmc_mlp_train_grad(const NmcMlp* weights, const float* feat, const float* target, int n,
                        const NmcMlpTrainBuffers* buffers, NmcMlpGrads* grads, nmc_stream_t stream);
 
-/* Dense signed distance grid of one triangle mesh (the cook of Mesh.build_sdf, sdf_texture.py:959-1370):
- * grid vertex (ix, iy, iz) at lower + voxel * (ix, iy, iz) gets the distance to the nearest triangle,
- * negative inside.  Inside is decided by the generalized winding number w against winding_threshold
- * (get_distance_to_mesh, sdf_utils.py:640-651): w > threshold for threshold >= 0, and w <= threshold for a
- * negative threshold (inward-wound meshes, -0.5 when the signed volume is negative, sdf_utils.py:506-510).
- * out has (cells[2]+1)(cells[1]+1)(cells[0]+1) floats, x fastest.  No counterpart as a C entry point in
- * the reference (a Warp kernel there). */
+/* The reference's sparse SDF texture of one triangle mesh (create_texture_sdf_from_mesh /
+ * build_sparse_sdf_from_mesh, sdf_texture.py:959-1505), cooked in three calls; the host derives the grid
+ * geometry and, between the second and third call, sizes the texture from the count of required subgrids
+ * (the reference reads it back at the same point, :1163-1172).  No counterpart as C entry points in the
+ * reference (Warp kernels there).
+ *
+ * nmc_sdf_build: the padded fine grid.  Vertex (ix, iy, iz) at lower + (ix cell[0], iy cell[1], iz cell[2])
+ * gets the distance to the nearest triangle, negative inside; inside is decided by the generalized winding
+ * number w against winding_threshold (get_distance_to_mesh, sdf_utils.py:640-651): w > threshold for
+ * threshold >= 0, w <= threshold for a negative threshold (inward-wound meshes, -0.5 when the signed volume is
+ * negative, sdf_utils.py:506-510).  out has vertices_per_axis[2] * [1] * [0] floats, x fastest; the texture
+ * cook passes 8 w + 1, 8 h + 1, 8 d + 1 vertices (w, h, d subgrids of 8^3 cells). */
 int nmc_sdf_build(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                  const float lower[3], float voxel, const int cells[3], float winding_threshold, float* out,
-                  nmc_stream_t stream);
+                  const float lower[3], const float cell[3], const int vertices_per_axis[3], float winding_threshold,
+                  float* out, nmc_stream_t stream);
+/* nmc_sdf_subgrids: from the fine grid of subgrids[0..2] = (w, h, d) subgrids, the coarse grid (coarse:
+ * (d+1)(h+1)(w+1) floats, x fastest) and per subgrid s = (bz*h + by)*w + bx: flags[s] = occupied (the centre
+ * value inside the band: below band_hi when not negative, above band_lo otherwise) | linear << 1 (occupied and
+ * every sample within linear_threshold of the coarse interpolation; never when linear_threshold <= 0) |
+ * required << 2 (occupied and not linear); address[s] = exclusive scan of required; *count = required
+ * subgrids (device int). */
+int nmc_sdf_subgrids(const float* fine, const int subgrids[3], float band_lo, float band_hi, float linear_threshold,
+                     float* coarse, int* flags, int* address, int* count, nmc_stream_t stream);
+/* nmc_sdf_pack: slots (w*h*d, z fastest) and the subgrid texture of blocks_per_axis^3 blocks of 9^3 texels
+ * (ts = 9 blocks_per_axis texels per axis, x fastest; blocks_per_axis^3 >= the required count, the caller
+ * zero-fills texels), texel_bytes 4 (float32), 2 (unorm16) or 1 (unorm8):
+ * trunc(clamp((v - quant_min) * quant_range_inv, 0, 1) * (2^bits - 1)). */
+int nmc_sdf_pack(const float* fine, const int subgrids[3], const int* flags, const int* address, int blocks_per_axis,
+                 int texel_bytes, float quant_min, float quant_range_inv, uint32_t* slots, void* texels,
+                 nmc_stream_t stream);
 
 /* Reported forces of the XPBD step (SolverXPBD.update_contacts, newton/_src/solvers/xpbd/solver_xpbd.py:767-821,
  * and State.body_parent_f, :632-655).  contact_impulse (capacity x 6: lin, ang) accumulates, over the

@@ -43,8 +43,9 @@ class NmcModelDesc(ctypes.Structure):
         ("rigid_contact_max", ctypes.c_int), ("shape_pairs_max", ctypes.c_int),
         ("shape_mesh", _vp), ("shape_aabb_lower", _vp), ("shape_aabb_upper", _vp), ("mesh_count", ctypes.c_int),
         ("mesh_vertex", _vp), ("mesh_vertex_start", _vp), ("mesh_edge", _vp), ("mesh_edge_start", _vp),
-        ("mesh_sdf", _vp), ("mesh_sdf_start", _vp), ("mesh_sdf_lower", _vp), ("mesh_sdf_voxel", _vp),
-        ("mesh_sdf_cells", _vp), ("mesh_sdf_flags", _vp), ("mesh_reduce", ctypes.c_int),
+        ("mesh_sdf_coarse", _vp), ("mesh_sdf_slots", _vp), ("mesh_sdf_texels", _vp), ("mesh_sdf_start", _vp),
+        ("mesh_sdf_box", _vp), ("mesh_sdf_quant", _vp), ("mesh_sdf_dims", _vp), ("mesh_sdf_flags", _vp),
+        ("mesh_reduce", ctypes.c_int),
         ("shape_voxel_resolution", _vp),
         ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _vp), ("explicit_pair_count", ctypes.c_int),
     ]
@@ -95,7 +96,8 @@ class NmcMlpGrads(ctypes.Structure):
 NMC_STEP_PLAIN, NMC_STEP_LOG, NMC_STEP_PREP, NMC_STEP_WARM = 0, 1, 2, 3
 
 EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_step", "nmc_xpbd_step_learn",
-           "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_matcher_create", "nmc_matcher_destroy",
+           "nmc_mlp_forward", "nmc_mlp_train_grad", "nmc_sdf_build", "nmc_sdf_subgrids", "nmc_sdf_pack",
+           "nmc_matcher_create", "nmc_matcher_destroy",
            "nmc_matcher_reset", "nmc_contact_match", "nmc_xpbd_step_report", "nmc_contact_force",
            "nmc_broad_phase_create", "nmc_broad_phase_destroy", "nmc_broad_phase_launch", "nmc_broad_phase_explicit",
            "nmc_broad_phase_explicit_noalloc",
@@ -143,8 +145,15 @@ def load() -> ctypes.CDLL:
                                          ctypes.POINTER(NmcMlpTrainBuffers), ctypes.POINTER(NmcMlpGrads), _vp]
         L.nmc_mlp_train_grad.restype = ctypes.c_int
         L.nmc_sdf_build.argtypes = [_vp, ctypes.c_int, _vp, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
-                                    ctypes.c_float, ctypes.POINTER(ctypes.c_int), ctypes.c_float, _vp, _vp]
+                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_float, _vp,
+                                    _vp]
         L.nmc_sdf_build.restype = ctypes.c_int
+        L.nmc_sdf_subgrids.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_float, _vp, _vp, _vp, _vp, _vp]
+        L.nmc_sdf_subgrids.restype = ctypes.c_int
+        L.nmc_sdf_pack.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), _vp, _vp, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_float, ctypes.c_float, _vp, _vp, _vp]
+        L.nmc_sdf_pack.restype = ctypes.c_int
         L.nmc_matcher_create.argtypes = [_vp, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                          ctypes.c_int, ctypes.POINTER(_vp)]
         L.nmc_matcher_create.restype = ctypes.c_int
@@ -322,10 +331,10 @@ class NativeModel:
             raise RuntimeError("mesh shapes need their SDF (Mesh.build_sdf) before the collision pipeline")
         d.mesh_vertex, d.mesh_vertex_start = ptr(m.mesh_vertex), ptr(m.mesh_vertex_start)
         d.mesh_edge, d.mesh_edge_start = ptr(m.mesh_edge), ptr(m.mesh_edge_start)
-        d.mesh_sdf, d.mesh_sdf_start = ptr(m.mesh_sdf), ptr(m.mesh_sdf_start)
-        d.mesh_sdf_lower, d.mesh_sdf_voxel, d.mesh_sdf_cells = (ptr(m.mesh_sdf_lower), ptr(m.mesh_sdf_voxel),
-                                                               ptr(m.mesh_sdf_cells))
-        d.mesh_sdf_flags = ptr(m.mesh_sdf_flags)
+        d.mesh_sdf_coarse, d.mesh_sdf_slots = ptr(m.mesh_sdf_coarse), ptr(m.mesh_sdf_slots)
+        d.mesh_sdf_texels, d.mesh_sdf_start = ptr(m.mesh_sdf_texels), ptr(m.mesh_sdf_start)
+        d.mesh_sdf_box, d.mesh_sdf_quant = ptr(m.mesh_sdf_box), ptr(m.mesh_sdf_quant)
+        d.mesh_sdf_dims, d.mesh_sdf_flags = ptr(m.mesh_sdf_dims), ptr(m.mesh_sdf_flags)
         # mesh contact reduction and its value packing (the reducer's deterministic flag is the pipeline's)
         d.mesh_reduce = 0 if not reduce_contacts else (2 if deterministic else 1)
         d.shape_voxel_resolution = ptr(m.shape_voxel_resolution)

@@ -1,8 +1,12 @@
 // Triangle-mesh contacts on gfx950: the SDF cook and the mesh narrow phase.
 //
-// sdf_build_kernel: one lane per grid vertex; distance to the nearest triangle (closest point on each
-// triangle), sign from the generalized winding number w (sum of solid angles / 4 pi): inside when
-// w > threshold, or w <= threshold for a negative threshold (inward-wound meshes).
+// The cook of the reference's sparse SDF texture (sdf_texture.py:959-1310), in three entry points:
+//   sdf_build_kernel: one lane per vertex of the padded fine grid; distance to the nearest triangle (closest
+//     point on each triangle), sign from the generalized winding number w (sum of solid angles / 4 pi):
+//     inside when w > threshold, or w <= threshold for a negative threshold (inward-wound meshes);
+//   sdf_coarse_kernel, sdf_subgrid_kernel, sdf_address_kernel: the coarse grid, narrow-band occupancy,
+//     linearity demotion and texture addresses of the subgrids;
+//   sdf_pack_kernel: indirection slots and the (quantised) subgrid texture.
 //
 // mesh_narrow_kernel: one 256-lane workgroup per mesh pair (grid-stride over the call's mesh pairs).
 //   mesh (shape0) - infinite plane (shape1), narrow_phase.py:1073-1326: every mesh vertex closer to the
@@ -21,15 +25,21 @@
 // Contacts go to the pair's staging slots in sub-key order; the compaction places them in canonical order.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "narrow.h"
 #include "nmc_internal.h"
 #include "reduce.h"
+#include "scan.h"
 #include "sdf.h"
 
 namespace nmc {
 
 constexpr int MESH_BLOCK = 256;
 constexpr int MESH_MAX_BLOCKS = 2048;
+// Slot tables of up to this many subgrids (12^3: up to 96 cells a side) are staged in LDS per mesh pair, so a
+// sample's indirection is an LDS read; larger textures are read from global memory.
+constexpr int MESH_SLOT_LDS = 2048;
 
 __device__ __forceinline__ unsigned mesh_sortable(float f) {
   if (f == 0.0f) f = 0.0f;
@@ -58,15 +68,18 @@ __device__ void mesh_bitonic(K* keys, int n2) {
 }
 
 // ------------------------------------------------------------------ SDF cook
+// Fine vertex (gx, gy, gz) at lower + (gx cx, gy cy, gz cz): the reference queries the mesh only at fine
+// vertices (subgrid corners, subgrid centres, subgrid samples), so the cook evaluates the padded fine grid
+// once and every later pass reads it.
 __global__ void __launch_bounds__(256) sdf_build_kernel(const float* __restrict__ vtx, int nvtx,
-                                                        const int* __restrict__ tri, int ntri, f3 lower, float voxel, int cx, int cy, int cz,
-                                                        float threshold, float* __restrict__ out) {
-  const long long nx = cx + 1, ny = cy + 1, nz = cz + 1;
-  const long long total = nx * ny * nz;
+                                                        const int* __restrict__ tri, int ntri, f3 lower, f3 cell,
+                                                        int nx, int ny, int nz, float threshold,
+                                                        float* __restrict__ out) {
+  const long long total = (long long)nx * ny * nz;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
        i += (long long)gridDim.x * blockDim.x) {
-    const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / (nx * ny));
-    const f3 p{lower.x + voxel * (float)ix, lower.y + voxel * (float)iy, lower.z + voxel * (float)iz};
+    const int ix = (int)(i % nx), iy = (int)((i / nx) % ny), iz = (int)(i / ((long long)nx * ny));
+    const f3 p{lower.x + (float)ix * cell.x, lower.y + (float)iy * cell.y, lower.z + (float)iz * cell.z};
     float best = 3.0e38f;
     float omega = 0.0f;
     for (int t = 0; t < ntri; ++t) {
@@ -84,6 +97,137 @@ __global__ void __launch_bounds__(256) sdf_build_kernel(const float* __restrict_
     const float wn = omega / (4.0f * 3.14159265358979323846f);
     const bool above = wn > threshold;
     out[i] = (threshold >= 0.0f ? above : !above) ? -best : best;
+  }
+}
+
+struct FineGrid {
+  const float* v;
+  int nx, ny;  // fine vertices along x and y (8 w + 1, 8 h + 1)
+  __device__ __forceinline__ float at(int x, int y, int z) const {
+    return v[((long long)z * ny + y) * nx + x];
+  }
+};
+
+// The coarse grid: the field at every subgrid corner (_build_coarse_sdf_from_mesh_kernel,
+// sdf_texture.py:329-360), (d+1)(h+1)(w+1) values, x fastest.
+__global__ void __launch_bounds__(256) sdf_coarse_kernel(FineGrid F, int w, int h, int d, float* __restrict__ coarse) {
+  const int bx = w + 1, by = h + 1, total = bx * by * (d + 1);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < total; i += gridDim.x * blockDim.x) {
+    const int z = i / (bx * by), r = i - z * bx * by, y = r / bx, x = r - y * bx;
+    coarse[i] = F.at(x * SDF_SUBGRID, y * SDF_SUBGRID, z * SDF_SUBGRID);
+  }
+}
+
+// Trilinear value of the coarse grid at fine sample (lx, ly, lz) of subgrid (bx, by, bz)
+// (_interp_coarse_sdf, sdf_texture.py:119-161: coarse coordinates clamped to the last cell, weights to [0, 1]).
+__device__ __forceinline__ float coarse_interp(const FineGrid& F, int bx, int by, int bz, int lx, int ly, int lz,
+                                               int w, int h, int d) {
+  const float cfx = (float)bx + (float)lx * 0.125f;
+  const float cfy = (float)by + (float)ly * 0.125f;
+  const float cfz = (float)bz + (float)lz * 0.125f;
+  const int x0 = max(0, min((int)floorf(cfx), w - 1));
+  const int y0 = max(0, min((int)floorf(cfy), h - 1));
+  const int z0 = max(0, min((int)floorf(cfz), d - 1));
+  const float tx = clampf(cfx - (float)x0, 0.0f, 1.0f);
+  const float ty = clampf(cfy - (float)y0, 0.0f, 1.0f);
+  const float tz = clampf(cfz - (float)z0, 0.0f, 1.0f);
+  const int X0 = x0 * SDF_SUBGRID, X1 = X0 + SDF_SUBGRID, Y0 = y0 * SDF_SUBGRID, Y1 = Y0 + SDF_SUBGRID;
+  const int Z0 = z0 * SDF_SUBGRID, Z1 = Z0 + SDF_SUBGRID;
+  const float v000 = F.at(X0, Y0, Z0), v100 = F.at(X1, Y0, Z0), v010 = F.at(X0, Y1, Z0), v110 = F.at(X1, Y1, Z0);
+  const float v001 = F.at(X0, Y0, Z1), v101 = F.at(X1, Y0, Z1), v011 = F.at(X0, Y1, Z1), v111 = F.at(X1, Y1, Z1);
+  const float c00 = v000 * (1.0f - tx) + v100 * tx;
+  const float c10 = v010 * (1.0f - tx) + v110 * tx;
+  const float c01 = v001 * (1.0f - tx) + v101 * tx;
+  const float c11 = v011 * (1.0f - tx) + v111 * tx;
+  const float c0 = c00 * (1.0f - ty) + c10 * ty;
+  const float c1 = c01 * (1.0f - ty) + c11 * ty;
+  return c0 * (1.0f - tz) + c1 * tz;
+}
+
+// One workgroup per subgrid (grid-stride): narrow-band occupancy from the value at its centre
+// (_check_subgrid_occupied_kernel, :198-224, _is_in_narrow_band, :164-169), then for occupied subgrids the
+// largest deviation of its 9^3 samples from the coarse interpolation (_accumulate_subgrid_linearity_error_kernel,
+// :227-304); a subgrid below linear_threshold is demoted to the coarse grid (_apply_subgrid_linearity_kernel,
+// :307-326).  flags[s] = occupied | linear << 1 | required << 2.
+__global__ void __launch_bounds__(256) sdf_subgrid_kernel(FineGrid F, int w, int h, int d, float band_lo,
+                                                          float band_hi, float linear_threshold,
+                                                          int* __restrict__ flags) {
+  __shared__ float s_err[256 / 64];
+  const int total = w * h * d;
+  for (int s = blockIdx.x; s < total; s += gridDim.x) {
+    const int bz = s / (w * h), r = s - bz * w * h, by = r / w, bx = r - by * w;
+    const float c = F.at(bx * SDF_SUBGRID + 4, by * SDF_SUBGRID + 4, bz * SDF_SUBGRID + 4);
+    const bool occupied = !(c < 0.0f) ? c < band_hi : c > band_lo;
+    float err = 0.0f;
+    if (occupied && linear_threshold > 0.0f) {
+      for (int k = threadIdx.x; k < 729; k += blockDim.x) {
+        const int lz = k / 81, rr = k - lz * 81, ly = rr / 9, lx = rr - ly * 9;
+        const float v = F.at(bx * SDF_SUBGRID + lx, by * SDF_SUBGRID + ly, bz * SDF_SUBGRID + lz);
+        const float e = fabsf(v - coarse_interp(F, bx, by, bz, lx, ly, lz, w, h, d));
+        err = e > err ? e : err;
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(err, off, 64);
+        err = o > err ? o : err;
+      }
+      if ((threadIdx.x & 63) == 0) s_err[threadIdx.x >> 6] = err;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        for (int q = 1; q < (int)(blockDim.x >> 6); ++q) err = s_err[q] > err ? s_err[q] : err;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const bool linear = occupied && linear_threshold > 0.0f && err < linear_threshold;
+      flags[s] = (occupied ? 1 : 0) | (linear ? 2 : 0) | ((occupied && !linear) ? 4 : 0);
+    }
+  }
+}
+
+// Texture addresses of the required subgrids: exclusive scan (one workgroup); count[0] = number required.
+__global__ void __launch_bounds__(SCAN_BLOCK) sdf_address_kernel(const int* __restrict__ flags, int n,
+                                                                 int* __restrict__ address, int* __restrict__ count) {
+  int total[1];
+  segment_scan_fn<1>(
+      n, [&](int i, int (&v)[1]) { v[0] = (flags[i] >> 2) & 1; },
+      [&](int i, const int (&ex)[1], const int (&)[1]) { address[i] = ex[0]; }, total);
+  if (threadIdx.x == 0) count[0] = total[0];
+}
+
+// One lane per subgrid sample (_populate_subgrid_texture_*_kernel, :363-566, _write_subgrid_slot, :172-195):
+// a required subgrid's samples go to texture block `address` (x fastest in a blocks^3 arrangement of 9^3
+// blocks), quantised for unorm formats as trunc(clamp((v - qmin) qinv, 0, 1) * (2^bits - 1)); sample 0 of
+// every subgrid writes its slot (block coordinates, SLOT_LINEAR for a demoted subgrid, SLOT_EMPTY otherwise).
+__global__ void __launch_bounds__(256) sdf_pack_kernel(FineGrid F, int w, int h, int d, const int* __restrict__ flags,
+                                                       const int* __restrict__ address, int blocks, int ts, int fmt,
+                                                       float qmin, float qinv, unsigned* __restrict__ slots,
+                                                       void* __restrict__ texels) {
+  const long long total = (long long)w * h * d * 729;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total;
+       t += (long long)gridDim.x * blockDim.x) {
+    const int s = (int)(t / 729), k = (int)(t - (long long)s * 729);
+    const int bz = s / (w * h), r = s - bz * w * h, by = r / w, bx = r - by * w;
+    const int f = flags[s];
+    const int a = address[s];
+    const int az = a / (blocks * blocks), ar = a - az * blocks * blocks, ay = ar / blocks, ax = ar - ay * blocks;
+    if (k == 0) {
+      const unsigned slot = (f & 4) ? ((unsigned)ax | ((unsigned)ay << 10) | ((unsigned)az << 20))
+                                    : ((f & 2) ? SDF_SLOT_LINEAR : SDF_SLOT_EMPTY);
+      slots[((long long)bx * h + by) * d + bz] = slot;
+    }
+    if (!(f & 4)) continue;
+    const int lz = k / 81, rr = k - lz * 81, ly = rr / 9, lx = rr - ly * 9;
+    const float v = F.at(bx * SDF_SUBGRID + lx, by * SDF_SUBGRID + ly, bz * SDF_SUBGRID + lz);
+    const long long ti = ((long long)(az * 9 + lz) * ts + (ay * 9 + ly)) * ts + (ax * 9 + lx);
+    if (fmt == 4) {
+      static_cast<float*>(texels)[ti] = v;
+    } else {
+      const float vn = clampf((v - qmin) * qinv, 0.0f, 1.0f);
+      if (fmt == 2)
+        static_cast<unsigned short*>(texels)[ti] = (unsigned short)(vn * 65535.0f);
+      else
+        static_cast<unsigned char*>(texels)[ti] = (unsigned char)(vn * 255.0f);
+    }
   }
 }
 
@@ -238,6 +382,7 @@ __device__ __forceinline__ void red_candidate(unsigned long long* slots, const R
   red_insert(slots, rp, bn, proj, dist, dist < f.beta, vox, sub);
 }
 
+template <bool REDUCE>
 __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d, const float* __restrict__ body_q,
                                                                  StageBuffers st,
                                                                  const long long* __restrict__ seg_pair_off,
@@ -249,11 +394,12 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_n;
   __shared__ int s_w[MESH_BLOCK / 64];
+  __shared__ unsigned s_slot[2][MESH_SLOT_LDS];  // [mode]: slots of the SDF that side samples
   // reduced: the 240 slots, then 256 winner sub keys; unreduced: cand_cap candidate sub keys
   unsigned long long* slots = reinterpret_cast<unsigned long long*>(smem);
   unsigned* sel = reinterpret_cast<unsigned*>(smem + (rp.mode ? sizeof(unsigned long long) * 256 : 0));
   const int total = st.route_count[2];
-  const bool reduce = rp.mode != 0;
+  constexpr bool reduce = REDUCE;
   for (int i = blockIdx.x; i < total; i += gridDim.x) {
     int lo = 0, hi = segments - 1;  // last segment whose mesh offset is <= i
     while (lo < hi) {
@@ -286,6 +432,26 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
     c.sub_key = 0;
     const int ma = d.shape_mesh[sa];
     const int mb = plane ? -1 : d.shape_mesh[sb];
+    int staged = 0;  // bit m: side m's slot table is in LDS
+    if (!plane) {
+      for (int mode = 0; mode < 2; ++mode) {
+        const int sm = mode == 0 ? mb : ma;
+        const int* dm = d.mesh_sdf_dims + 5 * sm;
+        const int ns = dm[0] * dm[1] * dm[2];
+        if (ns <= MESH_SLOT_LDS) {
+          staged |= 1 << mode;
+          const unsigned* src = d.mesh_sdf_slots + d.mesh_sdf_start[3 * sm + 1];
+          for (int k = threadIdx.x; k < ns; k += blockDim.x) s_slot[mode][k] = src[k];
+        }
+      }
+      __syncthreads();
+    }
+    // the edges of one side against the other side's SDF, its slot table in LDS when staged
+    auto side = [&](int mode) {
+      EdgeSide E = mode == 0 ? edge_side(d, A, B, ma, mb, gap_sum) : edge_side(d, B, A, mb, ma, gap_sum);
+      if ((staged >> mode) & 1) E.G.slots = s_slot[mode];
+      return E;
+    };
     PlaneSide P;
     if (plane) {
       P.XM = A.X;
@@ -319,7 +485,7 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
     } else {
       for (int mode = 0; mode < 2; ++mode) {
         const int tm = mode == 0 ? ma : mb;
-        const EdgeSide E = mode == 0 ? edge_side(d, A, B, ma, mb, gap_sum) : edge_side(d, B, A, mb, ma, gap_sum);
+        const EdgeSide E = side(mode);
         RedFrame F;
         if (reduce) F = red_frame(d, mode == 0 ? sa : sb, E.XT, true, (E.XT.p + E.XS.p) * 0.5f);
         const int ne = d.mesh_edge_start[tm + 1] - d.mesh_edge_start[tm];
@@ -372,7 +538,7 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
           (void)vertex_contact(P, sub, center, normal, dist);
         } else {
           const int mode = (sub >> 1) & 1;
-          const EdgeSide E = mode == 0 ? edge_side(d, A, B, ma, mb, gap_sum) : edge_side(d, B, A, mb, ma, gap_sum);
+          const EdgeSide E = side(mode);
           (void)edge_contact(E, mode, sub >> 2, center, normal, dist);
         }
         o.center = center;
@@ -403,9 +569,14 @@ hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t 
   if (m->stage.mesh_pair_capacity <= 0) return hipSuccess;
   const long long cap = m->stage.mesh_pair_capacity;
   const int blocks = static_cast<int>(cap < MESH_MAX_BLOCKS ? cap : MESH_MAX_BLOCKS);
-  hipLaunchKernelGGL(mesh_narrow_kernel, dim3(blocks), dim3(MESH_BLOCK), m->mesh_lds, stream, m->d, body_q, m->stage,
-                     m->seg_pair_off, m->seg_pair_count, m->seg_mesh_count, m->seg_mesh_off, m->d.segment_count,
-                     m->mesh_cand_cap, m->mesh_k, m->red, m->counters);
+  if (m->red.mode != 0)
+    hipLaunchKernelGGL(mesh_narrow_kernel<true>, dim3(blocks), dim3(MESH_BLOCK), m->mesh_lds, stream, m->d, body_q,
+                       m->stage, m->seg_pair_off, m->seg_pair_count, m->seg_mesh_count, m->seg_mesh_off,
+                       m->d.segment_count, m->mesh_cand_cap, m->mesh_k, m->red, m->counters);
+  else
+    hipLaunchKernelGGL(mesh_narrow_kernel<false>, dim3(blocks), dim3(MESH_BLOCK), m->mesh_lds, stream, m->d, body_q,
+                       m->stage, m->seg_pair_off, m->seg_pair_count, m->seg_mesh_count, m->seg_mesh_off,
+                       m->d.segment_count, m->mesh_cand_cap, m->mesh_k, m->red, m->counters);
   return hipGetLastError();
 }
 
@@ -414,18 +585,59 @@ hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t 
 using namespace nmc;
 
 extern "C" int nmc_sdf_build(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                             const float lower[3], float voxel, const int cells[3], float winding_threshold,
-                             float* out, nmc_stream_t stream) {
-  if (!vertices || !triangles || !lower || !cells || !out) return nmc::fail("nmc_sdf_build: null argument");
+                             const float lower[3], const float cell[3], const int vertices_per_axis[3],
+                             float winding_threshold, float* out, nmc_stream_t stream) {
+  if (!vertices || !triangles || !lower || !cell || !vertices_per_axis || !out)
+    return nmc::fail("nmc_sdf_build: null argument");
   if (vertex_count <= 0 || triangle_count <= 0) return nmc::fail("nmc_sdf_build: empty mesh");
-  if (!(voxel > 0.0f) || cells[0] <= 0 || cells[1] <= 0 || cells[2] <= 0)
-    return nmc::fail("nmc_sdf_build: voxel and cells must be positive");
-  const long long total = (long long)(cells[0] + 1) * (cells[1] + 1) * (cells[2] + 1);
+  if (!(cell[0] > 0.0f) || !(cell[1] > 0.0f) || !(cell[2] > 0.0f) || vertices_per_axis[0] < 2 ||
+      vertices_per_axis[1] < 2 || vertices_per_axis[2] < 2)
+    return nmc::fail("nmc_sdf_build: cell sizes must be positive and every axis needs two vertices");
+  const long long total = (long long)vertices_per_axis[0] * vertices_per_axis[1] * vertices_per_axis[2];
   long long blocks = (total + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(sdf_build_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
                      reinterpret_cast<hipStream_t>(stream), vertices, vertex_count, triangles, triangle_count,
-                     f3{lower[0], lower[1], lower[2]}, voxel, cells[0], cells[1], cells[2], winding_threshold, out);
+                     f3{lower[0], lower[1], lower[2]}, f3{cell[0], cell[1], cell[2]}, vertices_per_axis[0],
+                     vertices_per_axis[1], vertices_per_axis[2], winding_threshold, out);
   const hipError_t e = hipGetLastError();
   return e == hipSuccess ? 0 : nmc::hip_fail(e, "nmc_sdf_build launch");
+}
+
+extern "C" int nmc_sdf_subgrids(const float* fine, const int subgrids[3], float band_lo, float band_hi,
+                                float linear_threshold, float* coarse, int* flags, int* address, int* count,
+                                nmc_stream_t stream) {
+  if (!fine || !subgrids || !coarse || !flags || !address || !count) return nmc::fail("nmc_sdf_subgrids: null argument");
+  const int w = subgrids[0], h = subgrids[1], d = subgrids[2];
+  if (w <= 0 || h <= 0 || d <= 0 || w > 1024 || h > 1024 || d > 1024 || (long long)w * h * d > (1LL << 30))
+    return nmc::fail("nmc_sdf_subgrids: subgrid counts must lie in [1, 1024]");
+  const hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const FineGrid F{fine, w * SDF_SUBGRID + 1, h * SDF_SUBGRID + 1};
+  const int nc = (w + 1) * (h + 1) * (d + 1), n = w * h * d;
+  hipLaunchKernelGGL(sdf_coarse_kernel, dim3(static_cast<unsigned>(std::min(4096, (nc + 255) / 256))), dim3(256), 0,
+                     st, F, w, h, d, coarse);
+  hipLaunchKernelGGL(sdf_subgrid_kernel, dim3(static_cast<unsigned>(std::min(n, 65535))), dim3(256), 0, st, F, w,
+                     h, d, band_lo, band_hi, linear_threshold, flags);
+  hipLaunchKernelGGL(sdf_address_kernel, dim3(1), dim3(SCAN_BLOCK), 0, st, flags, n, address, count);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : nmc::hip_fail(e, "nmc_sdf_subgrids launch");
+}
+
+extern "C" int nmc_sdf_pack(const float* fine, const int subgrids[3], const int* flags, const int* address,
+                            int blocks_per_axis, int texel_bytes, float quant_min, float quant_range_inv,
+                            uint32_t* slots, void* texels, nmc_stream_t stream) {
+  if (!fine || !subgrids || !flags || !address || !slots || !texels) return nmc::fail("nmc_sdf_pack: null argument");
+  const int w = subgrids[0], h = subgrids[1], d = subgrids[2];
+  if (w <= 0 || h <= 0 || d <= 0 || w > 1024 || h > 1024 || d > 1024 || (long long)w * h * d > (1LL << 30))
+    return nmc::fail("nmc_sdf_pack: subgrid counts must lie in [1, 1024]");
+  if (texel_bytes != 1 && texel_bytes != 2 && texel_bytes != 4) return nmc::fail("nmc_sdf_pack: texel_bytes must be 1, 2 or 4");
+  if (blocks_per_axis < 1 || blocks_per_axis > 1024) return nmc::fail("nmc_sdf_pack: blocks_per_axis must lie in [1, 1024]");
+  const FineGrid F{fine, w * SDF_SUBGRID + 1, h * SDF_SUBGRID + 1};
+  const long long total = (long long)w * h * d * 729;
+  const long long blocks = std::min<long long>((total + 255) / 256, 16384);
+  hipLaunchKernelGGL(sdf_pack_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), F, w, h, d, flags, address, blocks_per_axis,
+                     blocks_per_axis * 9, texel_bytes, quant_min, quant_range_inv, slots, texels);
+  const hipError_t e = hipGetLastError();
+  return e == hipSuccess ? 0 : nmc::hip_fail(e, "nmc_sdf_pack launch");
 }

@@ -120,9 +120,9 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     return fail("an explicit broad phase needs its pair list");
   if (d.mesh_count > 0) {
     if (!d.shape_mesh || !d.shape_aabb_lower || !d.shape_aabb_upper || !d.mesh_vertex || !d.mesh_vertex_start ||
-        !d.mesh_edge || !d.mesh_edge_start || !d.mesh_sdf || !d.mesh_sdf_start || !d.mesh_sdf_lower ||
-        !d.mesh_sdf_voxel || !d.mesh_sdf_cells || !d.mesh_sdf_flags)
-      return fail("a model with meshes needs every mesh table (vertices, edges, local AABBs, SDF grids)");
+        !d.mesh_edge || !d.mesh_edge_start || !d.mesh_sdf_coarse || !d.mesh_sdf_slots || !d.mesh_sdf_texels ||
+        !d.mesh_sdf_start || !d.mesh_sdf_box || !d.mesh_sdf_quant || !d.mesh_sdf_dims || !d.mesh_sdf_flags)
+      return fail("a model with meshes needs every mesh table (vertices, edges, local AABBs, SDF textures)");
     if (d.mesh_reduce < 0 || d.mesh_reduce > 2) return fail("mesh_reduce must be 0, 1 or 2");
     if (!d.shape_voxel_resolution) return fail("a model with meshes needs shape_voxel_resolution");
   }

@@ -3,16 +3,19 @@
 ``Mesh`` keeps the reference's vertex / index layout, its unique-edge list (the contact sort key of a
 mesh-mesh contact encodes the edge index, so the edge order follows ``Mesh.edges``,
 types.py:831-866: canonical vertex ids from coordinates quantized at 1e-7, edges (0,1), (1,2), (0,2) of
-each triangle interleaved, first occurrence of each canonical pair kept in order) and solid mass
-properties (inertia.py:361-440: signed tetrahedra against the origin).
+each triangle interleaved, first occurrence of each canonical pair kept in order), its watertight test
+(types.py:868-907) and solid mass properties (inertia.py:361-440: signed tetrahedra against the origin).
 
-``Mesh.build_sdf`` cooks a dense FP32 signed distance grid on the GPU (``nmc_sdf_build``).  Grid
-geometry follows the reference's cook (sdf_texture.py:959-1370): the mesh AABB padded by ``margin``
-(0.05 m), ``max_resolution`` cells along the longest axis, the other axes with the same voxel size;
-values are the distance to the nearest triangle, negative inside (generalized winding number above 0.5,
-or below -0.5 for an inward-wound mesh).  The reference stores the same field as a sparse narrow-band
-texture (coarse grid + 8^3 subgrids, optionally quantized); here every grid vertex holds its exact FP32
-value (64^3 fp32 = 1 MiB per mesh).  Cooked grids can be kept on disk (``cache_dir``, ``_sdf_cache``).
+``Mesh.build_sdf`` cooks the reference's sparse narrow-band SDF texture on the GPU
+(``SDF.create_from_mesh`` / ``create_texture_sdf_from_mesh`` / ``build_sparse_sdf_from_mesh``,
+sdf_utils.py:346-545, sdf_texture.py:959-1505): the mesh AABB padded by ``margin``, ``max_resolution`` cells
+along its longest axis and per-axis cell sizes that make the grid span the padded box exactly; a coarse grid
+at the corners of 8^3-cell subgrids; the subgrids whose centre lies within the narrow band (widened by the
+subgrid's half diagonal) and that the coarse grid does not already reproduce within 1e-6 of the box diagonal
+keep their 9^3 samples in a texture of float32, unorm16 or unorm8 texels.  The host arithmetic of the grid
+geometry is the reference's numpy arithmetic, operation for operation (float32 extents, float64 cell sizes),
+so the same mesh gives the same grid.  Cooked textures can be kept on disk in the reference's cache format
+(``cache_dir``, ``_sdf_cache``).
 """
 
 from __future__ import annotations
@@ -22,25 +25,140 @@ import ctypes
 import numpy as np
 import torch
 
+SUBGRID_SIZE = 8
+"""Cells per subgrid edge (the reference's default ``subgrid_size``, the only one ``Mesh.build_sdf`` uses)."""
+SLOT_EMPTY = 0xFFFFFFFF
+"""Indirection slot of a far-field subgrid (sampled from the coarse grid)."""
+SLOT_LINEAR = 0xFFFFFFFE
+"""Indirection slot of a subgrid demoted to the coarse grid by the linearity test."""
+QUANTIZATION_MODES = {"float32": 4, "uint16": 2, "uint8": 1}
+"""``texture_format`` -> the reference's ``QuantizationMode`` value (= bytes per texel)."""
+
+
+def texture_sdf_geometry(points, *, margin: float = 0.05, narrow_band_range=(-0.1, 0.1),
+                         max_resolution: int | None = None, target_voxel_size: float | None = None) -> dict | None:
+    """Grid geometry of a texture SDF cook, or ``None`` for a degenerate (point-sized) mesh.
+
+    The host arithmetic of ``create_texture_sdf_from_mesh`` (sdf_texture.py:1436-1480) and
+    ``build_sparse_sdf_from_mesh`` (:1017-1160, :1240-1244) on float32 ``points``: extents in float32,
+    ``cells + 1`` vertices per axis with float64 cell sizes ``extent / cells``, ``w, h, d`` subgrids (ceiling
+    division), the narrow band widened by the subgrid half diagonal, the linearity threshold 1e-6 times the
+    padded box diagonal, the unorm quantisation range and the padded upper corner.
+    """
+    pts = np.asarray(points, np.float32).reshape(-1, 3)
+    margin = float(margin)
+    min_ext = pts.min(axis=0) - margin
+    max_ext = pts.max(axis=0) + margin
+    ext = max_ext - min_ext
+    max_ext_scalar = np.max(ext)
+    if max_ext_scalar < 1e-10:
+        return None
+    if target_voxel_size is not None:
+        if target_voxel_size <= 0.0:
+            raise ValueError("target_voxel_size must be > 0")
+        derived = int(np.ceil(max_ext_scalar / float(target_voxel_size)))
+        max_resolution = max(8, ((derived + 7) // 8) * 8)
+    elif max_resolution is None:
+        max_resolution = 64
+    max_resolution = int(max_resolution)
+    if max_resolution <= 0:
+        raise ValueError("max_resolution must be > 0")
+    if max_resolution >= (1 << 16):
+        raise ValueError(f"max_resolution must be less than {1 << 16}")
+    cell_size_scalar = max_ext_scalar / max_resolution
+    dims = np.ceil(ext / cell_size_scalar).astype(int) + 1
+    cell_size = ext / (dims - 1)
+    nb = max(abs(float(narrow_band_range[0])), abs(float(narrow_band_range[1])))
+    w, h, d = ((int(n) - 1 + SUBGRID_SIZE - 1) // SUBGRID_SIZE for n in dims)
+    subgrid_radius = float(np.linalg.norm(SUBGRID_SIZE * 0.5 * cell_size))
+    extents = max_ext - min_ext
+    sdf_min = -nb - subgrid_radius
+    sdf_range = (nb + subgrid_radius) - sdf_min
+    if sdf_range < 1e-10:
+        sdf_range = 1.0
+    return {
+        "min_extents": min_ext, "max_extents_unpadded": max_ext, "cell_size": cell_size, "dims": dims,
+        "max_resolution": max_resolution, "coarse_dims": (w, h, d), "narrow_band_thickness": nb,
+        "band": (np.float32(-nb - subgrid_radius), np.float32(nb + subgrid_radius)),
+        "linear_threshold": float(1e-6 * np.linalg.norm(extents)),
+        "sdf_min": sdf_min, "sdf_range": sdf_range,
+        "padded_max": min_ext + np.array([w, h, d], dtype=float) * SUBGRID_SIZE * cell_size,
+    }
+
+
+def texture_blocks_per_axis(num_required: int) -> int:
+    """Texture blocks per axis for ``num_required`` subgrids (sdf_texture.py:1181-1184: the ceiling of the
+    float cube root, raised until its cube holds them all)."""
+    b = max(1, int(np.ceil(num_required ** (1.0 / 3.0))))
+    while b ** 3 < num_required:
+        b += 1
+    return b
+
 
 class SDF:
-    """Dense signed distance grid of a mesh in its local frame (unscaled unless ``scale_baked``)."""
+    """The reference's texture SDF of a mesh in its local frame (``TextureSDFData`` + the cooked arrays).
 
-    def __init__(self, grid: torch.Tensor, lower, voxel_size: float, cells, *, scale_baked: bool = False,
-                 shape_margin: float = 0.0, narrow_band_range=(-0.1, 0.1), texture_format: str = "float32"):
-        self.grid = grid                    # (nz + 1, ny + 1, nx + 1) float32, x fastest
-        self.lower = np.asarray(lower, np.float32)
-        self.voxel_size = float(voxel_size)
-        self.cells = np.asarray(cells, np.int32)
-        self.upper = (self.lower.astype(np.float64) + self.cells * self.voxel_size).astype(np.float32)
+    Device tensors: ``coarse`` (float32, (d+1, h+1, w+1)), ``slots`` (int32 holding the uint32 slots,
+    (w, h, d)), ``texels`` (uint8 bytes of the (ts, ts, ts) texture, float32 / uint16 / uint8 samples).  The
+    sampling parameters are those ``create_sparse_sdf_textures`` (sdf_texture.py:1311-1367) derives.
+    """
+
+    def __init__(self, sparse: dict, device, *, scale_baked: bool = False, shape_margin: float = 0.0,
+                 narrow_band_range=(-0.1, 0.1), texture_format: str = "uint16", device_arrays=None):
         self.scale_baked = bool(scale_baked)
         self.shape_margin = float(shape_margin)
         self.narrow_band_range = (float(narrow_band_range[0]), float(narrow_band_range[1]))
         self.texture_format = texture_format
+        self._sparse = sparse
+        w, h, d = (int(x) for x in sparse["coarse_dims"])
+        self.coarse_dims = (w, h, d)
+        self.subgrid_size = int(sparse["subgrid_size"])
+        if self.subgrid_size != SUBGRID_SIZE:
+            raise ValueError(f"only subgrid_size {SUBGRID_SIZE} is supported, got {self.subgrid_size}")
+        self.subgrid_tex_size = int(sparse["subgrid_tex_size"])
+        self.num_subgrids = int(sparse["num_subgrids"])
+        self.quantization_mode = int(sparse["quantization_mode"])
+        cell = np.asarray(sparse["cell_size"], np.float64).reshape(3)
+        self.sdf_box_lower = np.asarray(sparse["min_extents"], np.float64).reshape(3).astype(np.float32)
+        self.sdf_box_upper = np.asarray(sparse["max_extents"], np.float64).reshape(3).astype(np.float32)
+        self.inv_sdf_dx = np.array([1.0 / float(c) for c in cell], np.float32)
+        self.voxel_size = cell.astype(np.float32)
+        self.voxel_radius = np.float32(0.5 * np.linalg.norm(cell))
+        self.subgrids_min_sdf_value = np.float32(sparse["subgrids_min_sdf_value"])
+        self.subgrids_sdf_value_range = np.float32(sparse["subgrids_sdf_value_range"])
+        if device_arrays is not None:
+            self.coarse, self.slots, self.texels, self.texel_bytes = device_arrays
+        else:
+            data = np.ascontiguousarray(sparse["subgrid_data"])
+            self.texel_bytes = int(data.dtype.itemsize)
+            if data.dtype not in (np.float32, np.uint16, np.uint8):
+                raise ValueError(f"subgrid_data must be float32, uint16 or uint8, got {data.dtype}")
+            self.coarse = torch.from_numpy(np.ascontiguousarray(sparse["coarse_sdf"], np.float32)).to(device)
+            slots = np.ascontiguousarray(sparse["subgrid_start_slots"], np.uint32).view(np.int32)
+            self.slots = torch.from_numpy(slots.copy()).to(device)
+            self.texels = torch.from_numpy(data.reshape(-1).view(np.uint8).copy()).to(device)
+
+    def sparse_data(self) -> dict:
+        """The cooked arrays as the reference's ``build_sparse_sdf_from_mesh`` dictionary (host numpy)."""
+        if self._sparse.get("subgrid_data") is None:
+            ts = self.subgrid_tex_size
+            dt = {4: np.float32, 2: np.uint16, 1: np.uint8}[self.texel_bytes]
+            sp = dict(self._sparse)
+            sp["coarse_sdf"] = self.coarse.cpu().numpy()
+            sp["subgrid_start_slots"] = self.slots.cpu().numpy().view(np.uint32)
+            sp["subgrid_data"] = self.texels.cpu().numpy().view(dt).reshape(ts, ts, ts)
+            self._sparse = sp
+        return self._sparse
 
     @property
-    def shape(self):
-        return tuple(int(c) + 1 for c in self.cells[::-1])
+    def params(self) -> dict:
+        """Per-mesh table entries of the narrow phase (box, inverse cell size, quantisation, dimensions)."""
+        w, h, d = self.coarse_dims
+        return {
+            "box": np.concatenate([self.sdf_box_lower, self.sdf_box_upper, self.inv_sdf_dx]).astype(np.float32),
+            "quant": np.array([self.subgrids_min_sdf_value, self.subgrids_sdf_value_range], np.float32),
+            "dims": np.array([w, h, d, self.subgrid_tex_size, self.texel_bytes], np.int32),
+        }
 
 
 class Mesh:
@@ -52,6 +170,7 @@ class Mesh:
         self.is_solid = is_solid
         self._edges = None
         self.sdf: SDF | None = None
+        self._is_watertight = None
         self.has_inertia = False
         if compute_inertia:
             self.mass, self.com, self.inertia, self.volume = self.mass_properties(1.0)
@@ -143,70 +262,59 @@ class Mesh:
         t = self.indices.reshape(-1, 3)
         return float(np.einsum("ij,ij->i", v[t[:, 0]], np.cross(v[t[:, 1]], v[t[:, 2]])).sum() / 6.0)
 
-    def sdf_grid_geometry(self, max_resolution: int = 64, margin: float = 0.05, scale=(1.0, 1.0, 1.0)):
-        """(lower, voxel size, cells) of the grid: the AABB of the (scaled) vertices padded by ``margin``,
-        ``max_resolution`` cells along its longest axis and the same voxel size along the others."""
-        lo, hi = self.aabb(scale)
-        lo = lo - margin
-        hi = hi + margin
-        ext = hi - lo
-        voxel = float(ext.max()) / float(max_resolution)
-        cells = np.maximum(1, np.ceil(ext / voxel - 1e-9)).astype(np.int32)
-        return lo.astype(np.float32), np.float32(voxel), cells
-
-    def sdf_resolution(self, max_resolution: int | None = None, target_voxel_size: float | None = None,
-                       margin: float = 0.05, scale=(1.0, 1.0, 1.0)) -> int:
-        """Effective resolution along the longest padded axis: ``target_voxel_size`` takes precedence
-        (ceil(extent / size) rounded up to a multiple of 8, at least 8, sdf_texture.py:1448-1456), else
-        ``max_resolution`` (64 when both are None)."""
-        if target_voxel_size is not None:
-            if not target_voxel_size > 0.0:
-                raise ValueError("target_voxel_size must be > 0")
-            lo, hi = self.aabb(scale)
-            ext = float(((hi + margin) - (lo - margin)).max())
-            res = int(np.ceil(ext / float(target_voxel_size)))
-            return max(8, ((res + 7) // 8) * 8)
-        res = 64 if max_resolution is None else int(max_resolution)
-        if res <= 0:
-            raise ValueError("max_resolution must be > 0")
-        if res % 8:
-            raise ValueError("max_resolution must be divisible by 8")
-        return res
+    @property
+    def is_watertight(self) -> bool:
+        """Every geometric edge shared by exactly two triangles (types.py:868-907: vertices identified by
+        their coordinates rounded at 1e-7)."""
+        if self._is_watertight is None:
+            tris = self.indices.reshape(-1, 3)
+            if len(tris) == 0 or len(self.vertices) == 0:
+                self._is_watertight = False
+                return False
+            q = np.round(self.vertices * np.float32(1e7)).astype(np.int64)
+            _, canon = np.unique(q, axis=0, return_inverse=True)
+            c = canon.reshape(-1)[tris]
+            pairs = np.empty((3 * len(tris), 2), np.int64)
+            for k, (a, b) in enumerate(((0, 1), (1, 2), (0, 2))):
+                pairs[k::3, 0] = np.minimum(c[:, a], c[:, b])
+                pairs[k::3, 1] = np.maximum(c[:, a], c[:, b])
+            _, counts = np.unique(pairs, axis=0, return_counts=True)
+            self._is_watertight = bool(np.all(counts == 2))
+        return self._is_watertight
 
     def build_sdf(self, *, device=None, narrow_band_range: tuple[float, float] | None = None,
                   target_voxel_size: float | None = None, max_resolution: int | None = None,
                   margin: float | None = None, shape_margin: float = 0.0,
                   scale: tuple[float, float, float] | None = None, texture_format: str = "uint16",
                   cache_dir=None) -> SDF:
-        """Cook and attach the dense signed distance grid on the GPU (``nmc_sdf_build``).
+        """Cook and attach the reference's sparse texture SDF on the GPU.
 
         Mirrors ``Mesh.build_sdf`` (types.py:719-797) and ``SDF.create_from_mesh`` (sdf_utils.py:346-545):
 
-        * ``max_resolution`` (default 64, divisible by 8) cells along the longest axis of the AABB padded by
-          ``margin`` (default 0.05 m), or the resolution ``target_voxel_size`` implies (it takes precedence);
+        * ``max_resolution`` (default 64) cells along the longest axis of the AABB padded by ``margin``
+          (default 0.05 m), or the resolution ``target_voxel_size`` implies (it takes precedence);
+        * ``narrow_band_range`` (default (-0.1, 0.1) m) selects the subgrids that keep their samples;
+        * ``texture_format`` ``"uint16"`` (default), ``"float32"`` or ``"uint8"`` stores them;
         * ``scale`` is multiplied into the vertices before the cook and the SDF is marked ``scale_baked``, so
           the narrow phase samples it with unit scale;
         * inside is decided by the generalized winding number against +0.5, or -0.5 when the mesh's signed
-          volume is negative (inward winding);
-        * ``cache_dir`` keeps cooked grids on disk as ``{hash}.sdf.npz`` (``_sdf_cache``): a hit loads the
-          grid instead of cooking, a miss cooks and publishes it;
-        * ``shape_margin``, ``narrow_band_range`` and ``texture_format`` are validated and recorded.  They
-          select the reference's sparse, quantized texture storage (and a sample-time offset its contact
-          path does not read, sdf_utils.py:236-256); every vertex of the dense grid holds its FP32 distance.
+          volume is negative (inward winding).  The reference resolves ``sign_method="auto"`` to ray parity for
+          watertight meshes, which classifies every point of a watertight mesh the same way; the resolved
+          name enters the cache key as there;
+        * ``shape_margin`` is recorded (the reference applies it at sample time, outside the contact path);
+        * ``cache_dir`` keeps cooked textures on disk as ``{hash}.sdf.npz`` in the reference's format and
+          key (``_sdf_cache``): a hit loads the arrays instead of cooking, a miss cooks and publishes them.
 
         Raises ``RuntimeError`` when the mesh already has an SDF or ``device`` is not a GPU.
         """
         from . import _sdf_cache
-        from ._native import check, load, ptr, stream_handle
 
         if self.sdf is not None:
             raise RuntimeError("Mesh already has an SDF. Call clear_sdf() before rebuilding.")
-        valid_formats = ("float32", "uint16", "uint8")
-        if texture_format not in valid_formats:
-            raise ValueError(f"Unknown texture_format {texture_format!r}. Expected one of {list(valid_formats)}.")
+        if texture_format not in QUANTIZATION_MODES:
+            raise ValueError(f"Unknown texture_format {texture_format!r}. "
+                             f"Expected one of {list(QUANTIZATION_MODES)}.")
         band = (-0.1, 0.1) if narrow_band_range is None else tuple(float(x) for x in narrow_band_range)
-        if len(band) != 2 or not band[0] < band[1]:
-            raise ValueError("narrow_band_range must be (inner, outer) with inner < outer")
         margin = 0.05 if margin is None else float(margin)
         dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
         if dev.type != "cuda":
@@ -215,48 +323,112 @@ class Mesh:
             raise ValueError("Mesh.build_sdf needs at least one triangle")
         bake = scale is not None
         sc = (1.0, 1.0, 1.0) if scale is None else tuple(float(x) for x in scale)
-        res = self.sdf_resolution(max_resolution, target_voxel_size, margin, sc)
-        lower, voxel, cells = self.sdf_grid_geometry(res, margin, sc)
-        nx, ny, nz = (int(c) + 1 for c in cells)
-        if nx * ny * nz >= 2**31:
-            raise ValueError(f"SDF grid of {nx} x {ny} x {nz} vertices is too large")
+        eff_res = 64 if max_resolution is None and target_voxel_size is None else max_resolution
         meta = {"scale_baked": bake, "shape_margin": shape_margin, "narrow_band_range": band,
                 "texture_format": texture_format}
-        verts_f32 = (self.vertices.astype(np.float64) * np.asarray(sc, np.float64)).astype(np.float32)
         key = None
         if cache_dir is not None:
-            key = _sdf_cache.hash_inputs(vertices=verts_f32, indices=self.indices, is_solid=self.is_solid,
-                                         max_resolution=res, margin=margin, scale=scale)
-            hit = _sdf_cache.try_load_sdf_data(cache_dir, key)
-            if hit is not None and bool(hit["scale_baked"]) == bake:
-                grid = torch.from_numpy(hit["grid"]).to(dev)
-                self.sdf = SDF(grid, hit["lower"], float(hit["voxel_size"]), hit["cells"], **meta)
+            key = _sdf_cache.hash_inputs(
+                vertices=np.asarray(self.vertices, np.float32) * np.array(sc, np.float32),
+                indices=np.asarray(self.indices, np.int32).reshape(-1), is_solid=bool(self.is_solid),
+                narrow_band_range=band, target_voxel_size=target_voxel_size, max_resolution=eff_res,
+                margin=margin, texture_format=texture_format,
+                sign_method_resolved="parity" if self.is_watertight else "winding", winding_threshold=0.5,
+                scale=scale)
+            hit = _sdf_cache.try_load_sparse_data(cache_dir, key)
+            if hit is not None:
+                self.sdf = SDF(hit, dev, **meta)
                 return self.sdf
+        verts = (self.vertices * np.array(sc)[None, :]).astype(np.float32)
+        geo = texture_sdf_geometry(verts, margin=margin, narrow_band_range=band,
+                                   max_resolution=64 if eff_res is None else eff_res,
+                                   target_voxel_size=target_voxel_size)
+        if geo is None:
+            raise ValueError("Mesh.build_sdf: the mesh has no extent")
         threshold = 0.5 if self.signed_volume(sc) >= 0.0 else -0.5
-        verts = torch.from_numpy(verts_f32).to(dev)
-        tris = torch.from_numpy(self.indices).to(dev)
-        grid = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
-        L = load()
-        lo = (ctypes.c_float * 3)(*[float(x) for x in lower])
-        ce = (ctypes.c_int * 3)(*[int(x) for x in cells])
-        check(L.nmc_sdf_build(ctypes.c_void_p(ptr(verts)), int(len(verts_f32)), ctypes.c_void_p(ptr(tris)),
-                              int(self.indices.size // 3), lo, ctypes.c_float(float(voxel)), ce,
-                              ctypes.c_float(threshold), ctypes.c_void_p(ptr(grid)),
-                              ctypes.c_void_p(stream_handle(dev))), "nmc_sdf_build")
-        self.sdf = SDF(grid, lower, voxel, cells, **meta)
+        self.sdf = cook_texture_sdf(verts, self.indices, geo, texture_format, threshold, dev, scale_baked=bake,
+                                    shape_margin=shape_margin, narrow_band_range=band)
         if key is not None:
-            _sdf_cache.write(cache_dir, key, {"grid": grid.cpu().numpy(), "lower": lower, "voxel_size": voxel,
-                                              "cells": cells, "scale_baked": bake})
+            _sdf_cache.write(cache_dir, key, self.sdf.sparse_data())
         return self.sdf
 
     def clear_sdf(self) -> None:
         """Detach the attached SDF (types.py:799-805)."""
         self.sdf = None
 
-    def attach_sdf(self, grid, lower, voxel_size, cells, scale_baked: bool = False):
-        """Attach a grid cooked elsewhere (for example by the CPU restatement in the tests)."""
-        self.sdf = SDF(grid, lower, voxel_size, cells, scale_baked=scale_baked)
+    def attach_sdf(self, sparse: dict, device="cpu", scale_baked: bool = False) -> SDF:
+        """Attach a texture SDF cooked elsewhere (a ``build_sparse_sdf_from_mesh`` dictionary, for example one
+        the CPU restatement in the tests produced)."""
+        self.sdf = SDF(sparse, device, scale_baked=scale_baked)
         return self.sdf
+
+
+def cook_texture_sdf(vertices, indices, geo: dict, texture_format: str, winding_threshold: float, device, *,
+                     scale_baked: bool = False, shape_margin: float = 0.0, narrow_band_range=(-0.1, 0.1)) -> SDF:
+    """The GPU cook of ``build_sparse_sdf_from_mesh`` (sdf_texture.py:959-1308) for float32 ``vertices`` and
+    the grid geometry ``geo`` of :func:`texture_sdf_geometry`: ``nmc_sdf_build`` (padded fine grid),
+    ``nmc_sdf_subgrids`` (coarse grid, occupancy, linearity, addresses), the count of required subgrids read
+    back to size the texture, ``nmc_sdf_pack`` (slots and texels)."""
+    from ._native import check, load, ptr, stream_handle
+
+    L = load()
+    dev = torch.device(device)
+    st = ctypes.c_void_p(stream_handle(dev))
+    w, h, d = geo["coarse_dims"]
+    nx, ny, nz = (SUBGRID_SIZE * n + 1 for n in (w, h, d))
+    if nx * ny * nz >= 2 ** 31:
+        raise ValueError(f"SDF grid of {nx} x {ny} x {nz} vertices is too large")
+    lower = np.asarray(geo["min_extents"], np.float32)
+    cell = np.asarray(geo["cell_size"], np.float64).astype(np.float32)
+    vt = torch.from_numpy(np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)).to(dev)
+    tr = torch.from_numpy(np.ascontiguousarray(indices, np.int32).reshape(-1)).to(dev)
+    fine = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
+    check(L.nmc_sdf_build(ctypes.c_void_p(ptr(vt)), int(vt.shape[0]), ctypes.c_void_p(ptr(tr)), int(tr.numel() // 3),
+                          (ctypes.c_float * 3)(*lower.tolist()), (ctypes.c_float * 3)(*cell.tolist()),
+                          (ctypes.c_int * 3)(nx, ny, nz), ctypes.c_float(winding_threshold),
+                          ctypes.c_void_p(ptr(fine)), st), "nmc_sdf_build")
+    n = w * h * d
+    coarse = torch.empty((d + 1, h + 1, w + 1), dtype=torch.float32, device=dev)
+    flags = torch.empty(n, dtype=torch.int32, device=dev)
+    address = torch.empty(n, dtype=torch.int32, device=dev)
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    dims = (ctypes.c_int * 3)(w, h, d)
+    lo, hi = geo["band"]
+    check(L.nmc_sdf_subgrids(ctypes.c_void_p(ptr(fine)), dims, ctypes.c_float(float(lo)), ctypes.c_float(float(hi)),
+                             ctypes.c_float(geo["linear_threshold"]), ctypes.c_void_p(ptr(coarse)),
+                             ctypes.c_void_p(ptr(flags)), ctypes.c_void_p(ptr(address)), ctypes.c_void_p(ptr(count)),
+                             st), "nmc_sdf_subgrids")
+    num_required = int(count.item())
+    qmode = QUANTIZATION_MODES[texture_format]
+    slots = torch.empty((w, h, d), dtype=torch.int32, device=dev)
+    if num_required == 0:
+        # no subgrid keeps samples: a 1-texel float32 texture, unit quantisation (sdf_texture.py:1174-1179)
+        blocks, tbytes, ts, qmin, qrange = 1, 4, 1, 0.0, 1.0
+        texels = torch.zeros(729 * 4, dtype=torch.uint8, device=dev)
+    else:
+        blocks = texture_blocks_per_axis(num_required)
+        ts = 9 * blocks
+        tbytes = qmode
+        quantized = qmode != 4
+        qmin, qrange = (geo["sdf_min"], geo["sdf_range"]) if quantized else (0.0, 1.0)
+        texels = torch.zeros(ts ** 3 * tbytes, dtype=torch.uint8, device=dev)
+    qinv = 1.0 / geo["sdf_range"]
+    check(L.nmc_sdf_pack(ctypes.c_void_p(ptr(fine)), dims, ctypes.c_void_p(ptr(flags)), ctypes.c_void_p(ptr(address)),
+                         int(blocks), int(tbytes), ctypes.c_float(qmin if tbytes != 4 else 0.0), ctypes.c_float(qinv),
+                         ctypes.c_void_p(ptr(slots)), ctypes.c_void_p(ptr(texels)), st), "nmc_sdf_pack")
+    if num_required == 0:
+        texels = texels[:4]
+    fl = flags.cpu().numpy()
+    sparse = {
+        "coarse_sdf": None, "subgrid_data": None, "subgrid_start_slots": None,
+        "coarse_dims": (w, h, d), "subgrid_tex_size": ts, "num_subgrids": num_required,
+        "min_extents": lower, "max_extents": geo["padded_max"], "cell_size": np.asarray(geo["cell_size"], np.float64),
+        "subgrid_size": SUBGRID_SIZE, "quantization_mode": qmode, "subgrids_min_sdf_value": qmin,
+        "subgrids_sdf_value_range": qrange,
+        "subgrid_required": ((fl >> 2) & 1).astype(np.int32), "subgrid_occupied": (fl & 1).astype(np.int32),
+    }
+    return SDF(sparse, dev, scale_baked=scale_baked, shape_margin=shape_margin, narrow_band_range=narrow_band_range,
+               texture_format=texture_format, device_arrays=(coarse, slots, texels, tbytes))
 
 
 def convex_hull_mesh(points) -> Mesh:
@@ -288,4 +460,4 @@ def random_convex_hull(rng: np.random.Generator, n_min: int = 64, n_max: int = 2
     return convex_hull_mesh(d * r)
 
 
-__all__ = ["Mesh", "SDF", "convex_hull_mesh", "random_convex_hull"]
+__all__ = ["Mesh", "SDF", "convex_hull_mesh", "cook_texture_sdf", "random_convex_hull", "texture_sdf_geometry"]

@@ -262,7 +262,8 @@ class Model:
         self._voxel_table(mesh_aabb)
 
     def _mesh_sdf_tables(self):
-        """Concatenate the meshes' SDF grids on the device (cooked on the GPU when missing)."""
+        """Concatenate the meshes' texture SDFs on the device (cooked on the GPU when missing): coarse grids,
+        slot tables and texel bytes, with per-mesh offsets and sampling parameters."""
         meshes = getattr(self, "meshes", [])
         M = len(meshes)
         if M and self.device.type == "cuda":
@@ -271,23 +272,41 @@ class Model:
                     mm.build_sdf(device=self.device)
         have = M > 0 and all(mm.sdf is not None for mm in meshes)
         if have:
-            grids = [mm.sdf.grid.reshape(-1).to(self.device) for mm in meshes]
-            self.mesh_sdf = torch.cat(grids)
-            starts = np.concatenate([[0], np.cumsum([g.numel() for g in grids])[:-1]]).astype(np.int64)
-            lower = np.stack([mm.sdf.lower for mm in meshes]).astype(_F32)
-            voxel = np.asarray([mm.sdf.voxel_size for mm in meshes], _F32)
-            cells = np.stack([mm.sdf.cells for mm in meshes]).astype(_I32)
-            flags = np.asarray([1 if mm.sdf.scale_baked else 0 for mm in meshes], _I32)
+            sdfs = [mm.sdf for mm in meshes]
+            coarse = [sd.coarse.reshape(-1).to(self.device) for sd in sdfs]
+            slots = [sd.slots.reshape(-1).to(self.device) for sd in sdfs]
+            texels = [sd.texels.reshape(-1).to(self.device) for sd in sdfs]
+            self.mesh_sdf_coarse = torch.cat(coarse)
+            self.mesh_sdf_slots = torch.cat(slots)
+            # texel offsets stay 4-byte aligned so float32 textures are read as whole words
+            pad = [(-t.numel()) % 4 for t in texels]
+            parts = []
+            for t, p in zip(texels, pad):
+                parts.append(t)
+                if p:
+                    parts.append(torch.zeros(p, dtype=torch.uint8, device=self.device))
+            self.mesh_sdf_texels = torch.cat(parts)
+            starts = np.zeros((M, 3), np.int64)
+            starts[:, 0] = np.concatenate([[0], np.cumsum([c.numel() for c in coarse])[:-1]])
+            starts[:, 1] = np.concatenate([[0], np.cumsum([c.numel() for c in slots])[:-1]])
+            starts[:, 2] = np.concatenate([[0], np.cumsum([t.numel() + p for t, p in zip(texels, pad)])[:-1]])
+            box = np.stack([sd.params["box"] for sd in sdfs]).astype(_F32)
+            quant = np.stack([sd.params["quant"] for sd in sdfs]).astype(_F32)
+            dims = np.stack([sd.params["dims"] for sd in sdfs]).astype(_I32)
+            flags = np.asarray([1 if sd.scale_baked else 0 for sd in sdfs], _I32)
         else:
-            self.mesh_sdf = torch.zeros(1, dtype=torch.float32, device=self.device)
-            starts = np.zeros(max(M, 1), np.int64)
-            lower = np.zeros((max(M, 1), 3), _F32)
-            voxel = np.ones(max(M, 1), _F32)
-            cells = np.ones((max(M, 1), 3), _I32)
-            flags = np.zeros(max(M, 1), _I32)
+            self.mesh_sdf_coarse = torch.zeros(1, dtype=torch.float32, device=self.device)
+            self.mesh_sdf_slots = torch.zeros(1, dtype=torch.int32, device=self.device)
+            self.mesh_sdf_texels = torch.zeros(4, dtype=torch.uint8, device=self.device)
+            k = max(M, 1)
+            starts = np.zeros((k, 3), np.int64)
+            box = np.zeros((k, 9), _F32)
+            quant = np.zeros((k, 2), _F32)
+            dims = np.ones((k, 5), _I32)
+            flags = np.zeros(k, _I32)
         self.mesh_has_sdf = have
-        for k, v in (("mesh_sdf_start", starts), ("mesh_sdf_lower", lower), ("mesh_sdf_voxel", voxel),
-                     ("mesh_sdf_cells", cells), ("mesh_sdf_flags", flags)):
+        for k, v in (("mesh_sdf_start", starts), ("mesh_sdf_box", box), ("mesh_sdf_quant", quant),
+                     ("mesh_sdf_dims", dims), ("mesh_sdf_flags", flags)):
             self.host[k] = v
             setattr(self, k, torch.from_numpy(np.ascontiguousarray(v)).to(self.device))
 

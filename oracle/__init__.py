@@ -42,8 +42,9 @@ class OracleModel(ctypes.Structure):
         ("joint_target_ke", _f32p), ("joint_target_kd", _f32p),
         ("shape_mesh", _i32p), ("shape_aabb_lower", _f32p), ("shape_aabb_upper", _f32p), ("mesh_count", ctypes.c_int),
         ("mesh_vertex", _f32p), ("mesh_vertex_start", _i32p), ("mesh_edge", _i32p), ("mesh_edge_start", _i32p),
-        ("mesh_sdf", _f32p), ("mesh_sdf_start", _i64p), ("mesh_sdf_lower", _f32p), ("mesh_sdf_voxel", _f32p),
-        ("mesh_sdf_cells", _i32p), ("mesh_sdf_flags", _i32p), ("mesh_reduce", ctypes.c_int),
+        ("mesh_sdf_coarse", _f32p), ("mesh_sdf_slots", ctypes.c_void_p), ("mesh_sdf_texels", ctypes.c_void_p),
+        ("mesh_sdf_start", _i64p), ("mesh_sdf_box", _f32p), ("mesh_sdf_quant", _f32p), ("mesh_sdf_dims", _i32p),
+        ("mesh_sdf_flags", _i32p), ("mesh_reduce", ctypes.c_int),
         ("shape_voxel_res", _i32p),
         ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _i32p), ("explicit_pair_count", ctypes.c_int),
     ]
@@ -154,15 +155,17 @@ class Oracle:
         m.mesh_count = len(getattr(model, "meshes", []))
         m.mesh_vertex, m.mesh_vertex_start = f32("mesh_vertex"), i32("mesh_vertex_start")
         m.mesh_edge, m.mesh_edge_start = i32("mesh_edge"), i32("mesh_edge_start")
-        sdf = np.ascontiguousarray(model.mesh_sdf.detach().cpu().numpy(), np.float32)
-        keep["mesh_sdf"] = sdf
-        m.mesh_sdf = _p(sdf, _f32p)
+        coarse = np.ascontiguousarray(model.mesh_sdf_coarse.detach().cpu().numpy(), np.float32)
+        slots = np.ascontiguousarray(model.mesh_sdf_slots.detach().cpu().numpy()).view(np.uint32)
+        texels = np.ascontiguousarray(model.mesh_sdf_texels.detach().cpu().numpy(), np.uint8)
         st = np.ascontiguousarray(h["mesh_sdf_start"], np.int64)
-        keep["mesh_sdf_start"] = st
+        keep.update(mesh_sdf_coarse=coarse, mesh_sdf_slots=slots, mesh_sdf_texels=texels, mesh_sdf_start=st)
+        m.mesh_sdf_coarse = _p(coarse, _f32p)
+        m.mesh_sdf_slots = slots.ctypes.data
+        m.mesh_sdf_texels = texels.ctypes.data
         m.mesh_sdf_start = _p(st, _i64p)
-        m.mesh_sdf_lower, m.mesh_sdf_voxel, m.mesh_sdf_cells = (f32("mesh_sdf_lower"), f32("mesh_sdf_voxel"),
-                                                                i32("mesh_sdf_cells"))
-        m.mesh_sdf_flags = i32("mesh_sdf_flags")
+        m.mesh_sdf_box, m.mesh_sdf_quant = f32("mesh_sdf_box"), f32("mesh_sdf_quant")
+        m.mesh_sdf_dims, m.mesh_sdf_flags = i32("mesh_sdf_dims"), i32("mesh_sdf_flags")
         # the reference's mesh contact reduction and its value packing (CollisionPipeline reduce_contacts /
         # deterministic)
         m.mesh_reduce = 0 if not reduce_contacts else (2 if deterministic else 1)
@@ -362,22 +365,106 @@ class Oracle:
         return qo, qdo, feat[:n], lam[:n], res, fb
 
 
-def sdf_build(vertices, triangles, lower, voxel: float, cells, precision: str = "f32",
-              winding_threshold: float = 0.5) -> np.ndarray:
-    """The restatement's dense SDF cook; returns the grid as (nz + 1, ny + 1, nx + 1)."""
+def sdf_fine_grid(vertices, triangles, lower, cell, n, precision: str = "f32",
+                  winding_threshold: float = 0.5) -> np.ndarray:
+    """The restatement's mesh query at the vertices of a fine grid (n[0..2] vertices per axis from ``lower``
+    with per-axis spacing ``cell``); returns (n[2], n[1], n[0])."""
     L = lib()
     v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
     t = np.ascontiguousarray(triangles, np.int32).reshape(-1)
-    lo = np.ascontiguousarray(lower, np.float32)
-    ce = np.ascontiguousarray(cells, np.int32)
-    shape = (int(ce[2]) + 1, int(ce[1]) + 1, int(ce[0]) + 1)
+    lo = np.ascontiguousarray(lower, np.float32).reshape(3)
+    ce = np.ascontiguousarray(cell, np.float32).reshape(3)
+    nn = np.ascontiguousarray(n, np.int32).reshape(3)
     dt, tp, fn = ((np.float32, _f32p, L.oracle_sdf_build_f32) if precision == "f32"
                   else (np.float64, _f64p, L.oracle_sdf_build_f64))
-    out = np.zeros(shape, dt)
+    out = np.zeros((int(nn[2]), int(nn[1]), int(nn[0])), dt)
     fn.restype = None
-    fn(_p(v, _f32p), ctypes.c_int(len(v)), _p(t, _i32p), ctypes.c_int(len(t) // 3), _p(lo, _f32p),
-       ctypes.c_float(float(voxel)), _p(ce, _i32p), ctypes.c_float(float(winding_threshold)), _p(out, tp))
+    fn(_p(v, _f32p), ctypes.c_int(len(v)), _p(t, _i32p), ctypes.c_int(len(t) // 3), _p(lo, _f32p), _p(ce, _f32p),
+       _p(nn, _i32p), ctypes.c_float(float(winding_threshold)), _p(out, tp))
     return out
+
+
+def texture_geometry(points, margin: float = 0.05, narrow_band_range=(-0.1, 0.1), max_resolution=None,
+                     target_voxel_size=None):
+    """Grid geometry of the reference's texture cook (create_texture_sdf_from_mesh, sdf_texture.py:1436-1480,
+    and build_sparse_sdf_from_mesh, :1017-1036, 1134-1136, 1152-1160, 1290): numpy arithmetic on float32
+    vertices, as there.  Returns None for a mesh without extent."""
+    pts = np.asarray(points, np.float32).reshape(-1, 3)
+    mesh_min, mesh_max = np.min(pts, axis=0), np.max(pts, axis=0)
+    min_ext, max_ext = mesh_min - float(margin), mesh_max + float(margin)
+    ext = max_ext - min_ext
+    mx = np.max(ext)
+    if mx < 1e-10:
+        return None
+    if target_voxel_size is not None:
+        r = int(np.ceil(mx / float(target_voxel_size)))
+        max_resolution = max(8, ((r + 7) // 8) * 8)
+    elif max_resolution is None:
+        max_resolution = 64
+    cs = mx / int(max_resolution)
+    dims = np.ceil(ext / cs).astype(int) + 1
+    cell = ext / (dims - 1)
+    nb = max(abs(narrow_band_range[0]), abs(narrow_band_range[1]))
+    w, h, d = ((int(k) - 1 + 7) // 8 for k in dims)
+    radius = float(np.linalg.norm(8 * 0.5 * cell))
+    lo, hi = -nb - radius, nb + radius
+    rng = hi - lo
+    if rng < 1e-10:
+        rng = 1.0
+    return {"min_extents": min_ext, "cell_size": cell, "dims": dims, "coarse_dims": (w, h, d),
+            "band": (np.float32(lo), np.float32(hi)),
+            "linear_threshold": float(1e-6 * np.linalg.norm(max_ext - min_ext)), "sdf_min": lo, "sdf_range": rng,
+            "padded_max": min_ext + np.array([w, h, d], dtype=float) * 8 * cell}
+
+
+def sdf_build_sparse(vertices, triangles, geo: dict, texture_format: str = "uint16", winding_threshold: float = 0.5,
+                     fine: np.ndarray | None = None) -> dict:
+    """The restatement's texture cook: the reference's ``build_sparse_sdf_from_mesh`` dictionary for float32
+    vertices and the grid geometry ``geo`` (``texture_geometry``).  ``fine`` replaces the mesh queries by a
+    given fine grid ((8d+1, 8h+1, 8w+1) values)."""
+    L = lib()
+    w, h, d = geo["coarse_dims"]
+    n = (8 * w + 1, 8 * h + 1, 8 * d + 1)
+    if fine is None:
+        fine = sdf_fine_grid(vertices, triangles, geo["min_extents"], np.asarray(geo["cell_size"]).astype(np.float32),
+                             n, winding_threshold=winding_threshold)
+    fine = np.ascontiguousarray(fine, np.float32)
+    assert fine.shape == (n[2], n[1], n[0])
+    dims = np.array([w, h, d], np.int32)
+    total = w * h * d
+    coarse = np.zeros((d + 1, h + 1, w + 1), np.float32)
+    occ, req, lin, addr = (np.zeros(total, np.int32) for _ in range(4))
+    lo, hi = geo["band"]
+    L.oracle_sdf_subgrids.restype = ctypes.c_int
+    nreq = L.oracle_sdf_subgrids(_p(fine, _f32p), _p(dims, _i32p), ctypes.c_float(float(lo)), ctypes.c_float(float(hi)),
+                                 ctypes.c_float(geo["linear_threshold"]), _p(coarse, _f32p), _p(occ, _i32p),
+                                 _p(req, _i32p), _p(lin, _i32p), _p(addr, _i32p))
+    qmode = {"float32": 4, "uint16": 2, "uint8": 1}[texture_format]
+    if nreq == 0:
+        blocks, fmt, ts, qmin, qrange = 1, 4, 9, 0.0, 1.0
+    else:
+        blocks = max(1, int(np.ceil(nreq ** (1.0 / 3.0))))
+        while blocks ** 3 < nreq:
+            blocks += 1
+        fmt, ts = qmode, 9 * blocks
+        qmin, qrange = (geo["sdf_min"], geo["sdf_range"]) if qmode != 4 else (0.0, 1.0)
+    dt = {4: np.float32, 2: np.uint16, 1: np.uint8}[fmt]
+    tex = np.zeros((ts, ts, ts), dt)
+    slots = np.zeros((w, h, d), np.uint32)
+    L.oracle_sdf_pack.restype = None
+    L.oracle_sdf_pack(_p(fine, _f32p), _p(dims, _i32p), _p(req, _i32p), _p(lin, _i32p), _p(addr, _i32p),
+                      ctypes.c_int(blocks), ctypes.c_int(fmt), ctypes.c_float(qmin if fmt != 4 else 0.0),
+                      ctypes.c_float(1.0 / geo["sdf_range"]), ctypes.c_void_p(slots.ctypes.data),
+                      ctypes.c_void_p(tex.ctypes.data))
+    if nreq == 0:
+        tex = np.zeros((1, 1, 1), np.float32)
+        ts = 1
+    return {"coarse_sdf": coarse, "subgrid_data": tex, "subgrid_start_slots": slots, "coarse_dims": (w, h, d),
+            "subgrid_tex_size": ts, "num_subgrids": int(nreq), "min_extents": geo["min_extents"],
+            "max_extents": geo["padded_max"], "cell_size": np.asarray(geo["cell_size"], np.float64),
+            "subgrid_size": 8, "quantization_mode": qmode, "subgrids_min_sdf_value": qmin,
+            "subgrids_sdf_value_range": qrange, "subgrid_required": req, "subgrid_occupied": occ,
+            "subgrid_linear": lin}
 
 
 def broad_phase(mode: str, lower, upper, gap, group, world, filter_pairs=None, index_map=None, slice_ends=None,

@@ -213,15 +213,27 @@ extern "C" void oracle_mlp_backward(const float* w1, const float* b1, const floa
 }
 
 extern "C" void oracle_sdf_build_f32(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                                     const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                                     const float lower[3], const float cell[3], const int n[3], float winding_threshold,
                                      float* out) {
-  sdf_cook<float>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, winding_threshold, out);
+  sdf_cook<float>(vertices, vertex_count, triangles, triangle_count, lower, cell, n, winding_threshold, out);
 }
 
 extern "C" void oracle_sdf_build_f64(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                                     const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                                     const float lower[3], const float cell[3], const int n[3], float winding_threshold,
                                      double* out) {
-  sdf_cook<double>(vertices, vertex_count, triangles, triangle_count, lower, voxel, cells, winding_threshold, out);
+  sdf_cook<double>(vertices, vertex_count, triangles, triangle_count, lower, cell, n, winding_threshold, out);
+}
+
+extern "C" int oracle_sdf_subgrids(const float* fine, const int dims[3], float band_lo, float band_hi, float lin_thr,
+                                   float* coarse, int* occupied, int* required, int* linear, int* address) {
+  return sdf_subgrids(fine, dims[0], dims[1], dims[2], band_lo, band_hi, lin_thr, coarse, occupied, required, linear,
+                      address);
+}
+
+extern "C" void oracle_sdf_pack(const float* fine, const int dims[3], const int* required, const int* linear,
+                                const int* address, int blocks, int fmt, float qmin, float qinv, uint32_t* slots,
+                                unsigned char* texels) {
+  sdf_pack(fine, dims[0], dims[1], dims[2], required, linear, address, blocks, fmt, qmin, qinv, slots, texels);
 }
 
 extern "C" void oracle_sdf_sample_f32(const OracleModel* m, int mesh, const float* p, int n, float* value,

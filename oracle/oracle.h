@@ -61,7 +61,9 @@ typedef struct OracleModel {
   const float* joint_target_ke;
   const float* joint_target_kd;
   /* triangle meshes: per-shape mesh index (-1 otherwise) and local AABB with the scale baked in;
-   * concatenated vertices / unique edges; one dense SDF grid per mesh ((cells+1)^3 vertices, x fastest) */
+   * concatenated vertices / unique edges; one texture SDF per mesh (coarse grid, slots, texels; per mesh the
+   * offsets into the three arrays, box lower / upper / inverse cell size, quantisation min / range and
+   * w, h, d, texture size, bytes per texel) */
   const int* shape_mesh;
   const float* shape_aabb_lower; /* S x 3 */
   const float* shape_aabb_upper; /* S x 3 */
@@ -70,12 +72,14 @@ typedef struct OracleModel {
   const int* mesh_vertex_start; /* M + 1 */
   const int* mesh_edge;         /* E x 2 */
   const int* mesh_edge_start;   /* M + 1 */
-  const float* mesh_sdf;
-  const int64_t* mesh_sdf_start;
-  const float* mesh_sdf_lower; /* M x 3 */
-  const float* mesh_sdf_voxel;
-  const int* mesh_sdf_cells; /* M x 3 */
-  const int* mesh_sdf_flags; /* M: bit 0 = scale baked into the grid */
+  const float* mesh_sdf_coarse;
+  const uint32_t* mesh_sdf_slots;
+  const unsigned char* mesh_sdf_texels;
+  const int64_t* mesh_sdf_start; /* M x 3 */
+  const float* mesh_sdf_box;     /* M x 9 */
+  const float* mesh_sdf_quant;   /* M x 2 */
+  const int* mesh_sdf_dims;      /* M x 5 */
+  const int* mesh_sdf_flags;     /* M: bit 0 = scale baked into the grid */
   /* mesh contact reduction (reduce_contacts): 0 off (every contact), 1 fast packing, 2 deterministic packing;
    * per-shape voxel grid of the reduction (S x 3), builder.py:9936-10065 */
   int mesh_reduce;
@@ -134,14 +138,21 @@ void oracle_xpbd_step_f64(const OracleModel* m, const OracleXpbdParams* p, const
                           const double* margin0, const double* margin1, double dt, double* body_q_out,
                           double* body_qd_out);
 
-/* Dense signed distance grid of a triangle mesh (distance to the nearest triangle, negative where the
- * winding number exceeds 0.5); out holds (cells[2]+1)(cells[1]+1)(cells[0]+1) values, x fastest. */
+/* Signed distance of a triangle mesh at the vertices of a fine grid (distance to the nearest triangle,
+ * negative where the winding number is above the threshold): vertex (i, j, k) at lower + (i, j, k) * cell,
+ * n[0..2] vertices per axis, x fastest. */
 void oracle_sdf_build_f32(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                          const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                          const float lower[3], const float cell[3], const int n[3], float winding_threshold,
                           float* out);
 void oracle_sdf_build_f64(const float* vertices, int vertex_count, const int* triangles, int triangle_count,
-                          const float lower[3], float voxel, const int cells[3], float winding_threshold,
+                          const float lower[3], const float cell[3], const int n[3], float winding_threshold,
                           double* out);
+/* The sparse texture derived from a fine grid of dims = (w, h, d) subgrids: coarse grid, occupied / required /
+ * linear flags and addresses (returns the required count); then slots and texels (fmt bytes per texel). */
+int oracle_sdf_subgrids(const float* fine, const int dims[3], float band_lo, float band_hi, float lin_thr,
+                        float* coarse, int* occupied, int* required, int* linear, int* address);
+void oracle_sdf_pack(const float* fine, const int dims[3], const int* required, const int* linear, const int* address,
+                     int blocks, int fmt, float qmin, float qinv, uint32_t* slots, unsigned char* texels);
 
 /* SDF value and gradient of mesh `mesh` of the model at local points p (n x 3). */
 void oracle_sdf_sample_f32(const OracleModel* m, int mesh, const float* p, int n, float* value, float* grad);

@@ -1,20 +1,23 @@
 // CPU oracle -- TEST INFRASTRUCTURE ONLY.  Triangle-mesh contact restatement.
 //
 // Follows:
-//   texture_sample_sdf / _grad (dense grid)   newton/_src/geometry/sdf_texture.py:776-955
+//   texture_sample_sdf / _grad, _read_cell_corners    newton/_src/geometry/sdf_texture.py:597-955
 //   scale_sdf_result_to_world                 newton/_src/geometry/sdf_contact.py:51-82
 //   get_edge_bounding_sphere, edge culling    newton/_src/geometry/sdf_contact.py:447-460, 1010-1045
 //   do_edge_sdf_collision (Brent + endpoints) newton/_src/geometry/sdf_contact.py:526-681
 //   mesh-mesh contact per edge                newton/_src/geometry/sdf_contact.py:870-1150
 //   mesh-plane contact per vertex             newton/_src/geometry/narrow_phase.py:1121-1190
-//   SDF cook (distance to the nearest triangle, winding-number sign)
-//                                             newton/_src/geometry/sdf_texture.py:959-1370 (dense grid)
+//   SDF cook (distance to the nearest triangle, winding-number sign; coarse grid, narrow-band occupancy,
+//   linearity demotion, texture addresses, quantised packing)
+//                                             newton/_src/geometry/sdf_texture.py:119-566, 959-1308
 //   reduce_contacts=False: every candidate the writer keeps   narrow_phase.py:1090-1190, sdf_contact.py:830-1065
 //   reduce_contacts=True: the normal-bin / voxel reduction    oracle_reduce.h (contact_reduction_global.py)
 // A pair's contacts are written in sub-key order.
 #pragma once
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <cstring>
 #include <vector>
 
 #include "oracle_collide.h"
@@ -22,53 +25,110 @@
 
 namespace oracle {
 
+// The reference's texture SDF (TextureSDFData, sdf_texture.py:48-87): coarse grid at the subgrid corners
+// ((d+1)(h+1)(w+1), x fastest), one slot per subgrid (w x h x d, z fastest), texture of 9^3-sample blocks
+// (ts^3, x fastest; fmt = bytes per texel: float32, unorm16, unorm8).
+constexpr uint32_t SLOT_EMPTY = 0xFFFFFFFFu;
+constexpr uint32_t SLOT_LINEAR = 0xFFFFFFFEu;
+
 template <class T> struct SdfG {
-  const float* v;
-  V3<T> lower, upper;
-  T inv_dx;
-  int cx, cy, cz;
+  const float* coarse;
+  const uint32_t* slots;
+  const unsigned char* texels;
+  V3<T> lower, upper, inv_dx;
+  T qmin, qrange;
+  int w, h, d, ts, fmt;
 };
 
 template <class T> SdfG<T> sdf_grid(const OracleModel* m, int mesh) {
   SdfG<T> g;
-  g.v = m->mesh_sdf + m->mesh_sdf_start[mesh];
-  g.lower = V3<T>{T(m->mesh_sdf_lower[3 * mesh]), T(m->mesh_sdf_lower[3 * mesh + 1]), T(m->mesh_sdf_lower[3 * mesh + 2])};
-  T vx = T(m->mesh_sdf_voxel[mesh]);
-  g.cx = m->mesh_sdf_cells[3 * mesh];
-  g.cy = m->mesh_sdf_cells[3 * mesh + 1];
-  g.cz = m->mesh_sdf_cells[3 * mesh + 2];
-  g.upper = V3<T>{g.lower.x + vx * T(g.cx), g.lower.y + vx * T(g.cy), g.lower.z + vx * T(g.cz)};
-  g.inv_dx = T(1) / vx;
+  const int64_t* st = m->mesh_sdf_start + 3 * mesh;
+  g.coarse = m->mesh_sdf_coarse + st[0];
+  g.slots = m->mesh_sdf_slots + st[1];
+  g.texels = m->mesh_sdf_texels + st[2];
+  const float* b = m->mesh_sdf_box + 9 * mesh;
+  g.lower = V3<T>{T(b[0]), T(b[1]), T(b[2])};
+  g.upper = V3<T>{T(b[3]), T(b[4]), T(b[5])};
+  g.inv_dx = V3<T>{T(b[6]), T(b[7]), T(b[8])};
+  g.qmin = T(m->mesh_sdf_quant[2 * mesh]);
+  g.qrange = T(m->mesh_sdf_quant[2 * mesh + 1]);
+  const int* dm = m->mesh_sdf_dims + 5 * mesh;
+  g.w = dm[0];
+  g.h = dm[1];
+  g.d = dm[2];
+  g.ts = dm[3];
+  g.fmt = dm[4];
   return g;
+}
+
+// wp.texture_sample with closest filtering at a texel centre: the stored float, or a unorm texel normalised
+// to [0, 1].
+template <class T> T sdf_texel(const SdfG<T>& g, long long i) {
+  if (g.fmt == 4) {
+    float v;
+    std::memcpy(&v, g.texels + 4 * i, 4);
+    return T(v);
+  }
+  if (g.fmt == 2) {
+    uint16_t v;
+    std::memcpy(&v, g.texels + 2 * i, 2);
+    return T(v) / T(65535);
+  }
+  return T(g.texels[i]) / T(255);
 }
 
 template <class T> struct SdfCellO {
   T c[8];
   T tx, ty, tz;
+  bool fine;
   T diff_mag;
   V3<T> diff;
 };
 
+// texture_sample_sdf / _read_cell_corners (sdf_texture.py:597-701, 776-894): clamp to the SDF box, fine cell
+// of the point, slot of its subgrid; a texture block gives the cell's 8 texels, a linear or empty slot the
+// subgrid's 8 coarse values with the weights of the coarse cell.
 template <class T> SdfCellO<T> sdf_cell(const SdfG<T>& g, V3<T> p) {
   SdfCellO<T> s;
   V3<T> cl{tclamp(p.x, g.lower.x, g.upper.x), tclamp(p.y, g.lower.y, g.upper.y), tclamp(p.z, g.lower.z, g.upper.z)};
   s.diff = p - cl;
   s.diff_mag = length(s.diff);
-  V3<T> f = (cl - g.lower) * g.inv_dx;
-  T fx = tclamp(f.x, T(0), T(g.cx));
-  T fy = tclamp(f.y, T(0), T(g.cy));
-  T fz = tclamp(f.z, T(0), T(g.cz));
-  int ix = std::max(0, std::min(int(std::floor(fx)), g.cx - 1));
-  int iy = std::max(0, std::min(int(std::floor(fy)), g.cy - 1));
-  int iz = std::max(0, std::min(int(std::floor(fz)), g.cz - 1));
+  V3<T> q = cl - g.lower;
+  V3<T> f{q.x * g.inv_dx.x, q.y * g.inv_dx.y, q.z * g.inv_dx.z};
+  const T fvx = T(g.w) * T(8), fvy = T(g.h) * T(8), fvz = T(g.d) * T(8);
+  T fx = tclamp(f.x, T(0), fvx), fy = tclamp(f.y, T(0), fvy), fz = tclamp(f.z, T(0), fvz);
+  int ix = std::max(0, std::min(int(std::floor(fx)), int(fvx) - 1));
+  int iy = std::max(0, std::min(int(std::floor(fy)), int(fvy) - 1));
+  int iz = std::max(0, std::min(int(std::floor(fz)), int(fvz) - 1));
   s.tx = fx - T(ix);
   s.ty = fy - T(iy);
   s.tz = fz - T(iz);
-  const long long nx = g.cx + 1, ny = g.cy + 1;
-  const float* b = g.v + ((long long)iz * ny + iy) * nx + ix;
-  const long long sy = nx, sz = nx * ny;
-  const long long o[8] = {0, 1, sy, sy + 1, sz, sz + 1, sz + sy, sz + sy + 1};
-  for (int k = 0; k < 8; ++k) s.c[k] = T(b[o[k]]);
+  const T f2c = T(1) / T(8);
+  int xb = std::max(0, std::min(int(T(ix) * f2c), g.w - 1));
+  int yb = std::max(0, std::min(int(T(iy) * f2c), g.h - 1));
+  int zb = std::max(0, std::min(int(T(iz) * f2c), g.d - 1));
+  uint32_t slot = g.slots[((long long)xb * g.h + yb) * g.d + zb];
+  if (slot >= SLOT_LINEAR) {
+    s.fine = false;
+    V3<T> cf{fx * f2c, fy * f2c, fz * f2c};
+    s.tx = cf.x - T(xb);
+    s.ty = cf.y - T(yb);
+    s.tz = cf.z - T(zb);
+    const long long sx = g.w + 1, sy = (long long)(g.w + 1) * (g.h + 1);
+    const long long o[8] = {0, 1, sx, sx + 1, sy, sy + 1, sy + sx, sy + sx + 1};
+    const float* b = g.coarse + zb * sy + yb * sx + xb;
+    for (int k = 0; k < 8; ++k) s.c[k] = T(b[o[k]]);
+  } else {
+    s.fine = true;
+    // tex_o + l + 0.5 is a texel centre: point sampling reads texel block * 9 + l
+    const long long ox = (long long)(slot & 0x3FFu) * 9 + (ix - xb * 8);
+    const long long oy = (long long)((slot >> 10) & 0x3FFu) * 9 + (iy - yb * 8);
+    const long long oz = (long long)((slot >> 20) & 0x3FFu) * 9 + (iz - zb * 8);
+    const long long ts = g.ts;
+    const long long o[8] = {0, 1, ts, ts + 1, ts * ts, ts * ts + 1, ts * ts + ts, ts * ts + ts + 1};
+    const long long i0 = (oz * ts + oy) * ts + ox;
+    for (int k = 0; k < 8; ++k) s.c[k] = sdf_texel(g, i0 + o[k]);
+  }
   return s;
 }
 
@@ -82,13 +142,20 @@ template <class T> T sdf_trilinear(const SdfCellO<T>& s) {
   return c0 + (c1 - c0) * s.tz;
 }
 
+// texture_sample_sdf (sdf_texture.py:776-894): a texture cell's blend is de-quantised after blending.
 template <class T> T sdf_sample(const SdfG<T>& g, V3<T> p) {
   SdfCellO<T> s = sdf_cell(g, p);
-  return sdf_trilinear(s) + s.diff_mag;
+  T v = sdf_trilinear(s);
+  if (s.fine) v = v * g.qrange + g.qmin;
+  return v + s.diff_mag;
 }
 
+// texture_sample_sdf_grad (sdf_texture.py:897-955): corners de-quantised before blending
+// (apply_subgrid_sdf_scale in _read_cell_corners), gradient scaled by the fine inverse cell size.
 template <class T> T sdf_sample_grad(const SdfG<T>& g, V3<T> p, V3<T>& grad) {
   SdfCellO<T> s = sdf_cell(g, p);
+  if (s.fine)
+    for (int k = 0; k < 8; ++k) s.c[k] = s.c[k] * g.qrange + g.qmin;
   T val = sdf_trilinear(s);
   T omtx = T(1) - s.tx, omty = T(1) - s.ty, omtz = T(1) - s.tz;
   const T* c = s.c;
@@ -98,7 +165,7 @@ template <class T> T sdf_sample_grad(const SdfG<T>& g, V3<T> p, V3<T>& grad) {
          s.tx * s.tz * (c[7] - c[5]);
   T gz = omtx * omty * (c[4] - c[0]) + s.tx * omty * (c[5] - c[1]) + omtx * s.ty * (c[6] - c[2]) +
          s.tx * s.ty * (c[7] - c[3]);
-  grad = V3<T>{gx * g.inv_dx, gy * g.inv_dx, gz * g.inv_dx};
+  grad = V3<T>{gx * g.inv_dx.x, gy * g.inv_dx.y, gz * g.inv_dx.z};
   if (s.diff_mag > T(0)) {
     val = val + s.diff_mag;
     grad = s.diff / s.diff_mag;
@@ -209,14 +276,18 @@ template <class T> T solid_angle(V3<T> a, V3<T> b, V3<T> c) {
   return T(2) * std::atan2(num, den);
 }
 
+// get_distance_to_mesh (sdf_utils.py:640-651) at every vertex of a fine grid: vertex (ix, iy, iz) at
+// lower + (ix cell[0], iy cell[1], iz cell[2]) (the position arithmetic of the cook kernels,
+// sdf_texture.py:398-402), nearest-triangle distance, sign of `w > threshold` for the generalized winding
+// number w, flipped back for a negative threshold.  n[0..2] vertices per axis, x fastest.
 template <class T>
-void sdf_cook(const float* vtx, int nv, const int* tri, int nt, const float lower[3], float voxel, const int cells[3],
+void sdf_cook(const float* vtx, int nv, const int* tri, int nt, const float lower[3], const float cell[3], const int n[3],
               float threshold, T* out) {
-  const long long nx = cells[0] + 1, ny = cells[1] + 1, nz = cells[2] + 1;
+  const long long nx = n[0], ny = n[1], nz = n[2];
   const T pi = T(3.14159265358979323846);
   for (long long i = 0; i < nx * ny * nz; ++i) {
     int ix = int(i % nx), iy = int((i / nx) % ny), iz = int(i / (nx * ny));
-    V3<T> p{T(lower[0]) + T(voxel) * T(ix), T(lower[1]) + T(voxel) * T(iy), T(lower[2]) + T(voxel) * T(iz)};
+    V3<T> p{T(lower[0]) + T(ix) * T(cell[0]), T(lower[1]) + T(iy) * T(cell[1]), T(lower[2]) + T(iz) * T(cell[2])};
     T best = T(3.0e38f), omega = T(0);
     for (int t = 0; t < nt; ++t) {
       int i0 = tri[3 * t], i1 = tri[3 * t + 1], i2 = tri[3 * t + 2];
@@ -229,11 +300,118 @@ void sdf_cook(const float* vtx, int nv, const int* tri, int nt, const float lowe
       best = dd < best ? dd : best;
       omega += solid_angle(a - p, b - p, c - p);
     }
-    // get_distance_to_mesh (sdf_utils.py:640-651): the sign of `w > threshold`, flipped back for a
-    // negative threshold
     T wn = omega / (T(4) * pi);
     const bool above = wn > T(threshold);
     out[i] = (threshold >= 0.0f ? above : !above) ? -best : best;
+  }
+}
+
+// build_sparse_sdf_from_mesh after its mesh queries (sdf_texture.py:1017-1160), kernel by kernel, on the fine
+// grid of (w, h, d) subgrids (8 w + 1 vertices along x, x fastest).  Every query point of the reference is a
+// fine vertex: subgrid corners (_build_coarse_sdf_from_mesh_kernel), centres (_check_subgrid_occupied_kernel,
+// (8 b + 4) cells) and samples.  Returns the number of required subgrids.
+inline int sdf_subgrids(const float* fine, int w, int h, int d, float band_lo, float band_hi, float lin_thr,
+                        float* coarse, int* occupied, int* required, int* linear, int* address) {
+  const long long nx = 8LL * w + 1, ny = 8LL * h + 1;
+  auto F = [&](long long x, long long y, long long z) { return fine[(z * ny + y) * nx + x]; };
+  const int bgx = w + 1, bgy = h + 1, bgz = d + 1;
+  // _build_coarse_sdf_from_mesh_kernel (:329-360)
+  for (int t = 0; t < bgx * bgy * bgz; ++t) {
+    int z = t / (bgx * bgy), r = t - z * bgx * bgy, y = r / bgx, x = r - y * bgx;
+    coarse[t] = F(8LL * x, 8LL * y, 8LL * z);
+  }
+  const int total = w * h * d;
+  // _check_subgrid_occupied_kernel (:198-224) with _is_in_narrow_band (:164-169): wp.sign(0) is +1
+  for (int t = 0; t < total; ++t) {
+    int z = t / (w * h), r = t - z * w * h, y = r / w, x = r - y * w;
+    float c = F(8LL * x + 4, 8LL * y + 4, 8LL * z + 4);
+    bool in = (c < 0.0f ? -1.0f : 1.0f) > 0.0f ? c < band_hi : c > band_lo;
+    required[t] = in ? 1 : 0;
+    occupied[t] = required[t];
+    linear[t] = 0;
+  }
+  if (lin_thr > 0.0f) {
+    // _accumulate_subgrid_linearity_error_kernel (:227-304) with _interp_coarse_sdf (:119-161)
+    std::vector<float> err(total, 0.0f);
+    const float inv_cpsg = 1.0f / float(8);
+    for (int t = 0; t < total; ++t) {
+      if (!required[t]) continue;
+      int bz = t / (w * h), r = t - bz * w * h, by = r / w, bx = r - by * w;
+      for (int k = 0; k < 729; ++k) {
+        int lz = k / 81, rr = k - lz * 81, ly = rr / 9, lx = rr - ly * 9;
+        float mesh_val = F(8LL * bx + lx, 8LL * by + ly, 8LL * bz + lz);
+        float cfx = float(bx) + float(lx) * inv_cpsg, cfy = float(by) + float(ly) * inv_cpsg;
+        float cfz = float(bz) + float(lz) * inv_cpsg;
+        int x0 = std::max(0, std::min(int(std::floor(cfx)), bgx - 2));
+        int y0 = std::max(0, std::min(int(std::floor(cfy)), bgy - 2));
+        int z0 = std::max(0, std::min(int(std::floor(cfz)), bgz - 2));
+        float tx = tclamp(cfx - float(x0), 0.0f, 1.0f), ty = tclamp(cfy - float(y0), 0.0f, 1.0f);
+        float tz = tclamp(cfz - float(z0), 0.0f, 1.0f);
+        auto B = [&](int x, int y, int z) { return coarse[(z * bgy + y) * bgx + x]; };
+        float v000 = B(x0, y0, z0), v100 = B(x0 + 1, y0, z0), v010 = B(x0, y0 + 1, z0), v110 = B(x0 + 1, y0 + 1, z0);
+        float v001 = B(x0, y0, z0 + 1), v101 = B(x0 + 1, y0, z0 + 1), v011 = B(x0, y0 + 1, z0 + 1);
+        float v111 = B(x0 + 1, y0 + 1, z0 + 1);
+        float c00 = v000 * (1.0f - tx) + v100 * tx;
+        float c10 = v010 * (1.0f - tx) + v110 * tx;
+        float c01 = v001 * (1.0f - tx) + v101 * tx;
+        float c11 = v011 * (1.0f - tx) + v111 * tx;
+        float c0 = c00 * (1.0f - ty) + c10 * ty;
+        float c1 = c01 * (1.0f - ty) + c11 * ty;
+        float coarse_val = c0 * (1.0f - tz) + c1 * tz;
+        err[t] = std::max(err[t], std::fabs(mesh_val - coarse_val));
+      }
+    }
+    // _apply_subgrid_linearity_kernel (:307-326)
+    for (int t = 0; t < total; ++t) {
+      if (!required[t]) continue;
+      if (err[t] < lin_thr) {
+        linear[t] = 1;
+        required[t] = 0;
+      }
+    }
+  }
+  // exclusive prefix sum of the required flags (:1163-1166)
+  int run = 0;
+  for (int t = 0; t < total; ++t) {
+    address[t] = run;
+    run += required[t];
+  }
+  return run;
+}
+
+// _populate_subgrid_texture_{float32,uint16,uint8}_kernel (:363-566) with _write_subgrid_slot (:172-195), then
+// the SLOT_LINEAR tags of demoted subgrids (:1275-1288).  slots: (w, h, d), z fastest; texels: ts^3, x fastest,
+// ts = 9 blocks.
+inline void sdf_pack(const float* fine, int w, int h, int d, const int* required, const int* linear,
+                     const int* address, int blocks, int fmt, float qmin, float qinv, uint32_t* slots,
+                     unsigned char* texels) {
+  const long long nx = 8LL * w + 1, ny = 8LL * h + 1;
+  const long long ts = 9LL * blocks;
+  for (long long t = 0; t < (long long)w * h * d; ++t) slots[t] = SLOT_EMPTY;
+  for (int t = 0; t < w * h * d; ++t) {
+    int bz = t / (w * h), r = t - bz * w * h, by = r / w, bx = r - by * w;
+    long long si = ((long long)bx * h + by) * d + bz;
+    if (linear[t]) slots[si] = SLOT_LINEAR;
+    if (!required[t]) continue;
+    int a = address[t];
+    int az = a / (blocks * blocks), ar = a - az * blocks * blocks, ay = ar / blocks, ax = ar - ay * blocks;
+    slots[si] = uint32_t(ax) | (uint32_t(ay) << 10) | (uint32_t(az) << 20);
+    for (int k = 0; k < 729; ++k) {
+      int lz = k / 81, rr = k - lz * 81, ly = rr / 9, lx = rr - ly * 9;
+      float v = fine[((8LL * bz + lz) * ny + 8LL * by + ly) * nx + 8LL * bx + lx];
+      long long ti = ((long long)(az * 9 + lz) * ts + (ay * 9 + ly)) * ts + (ax * 9 + lx);
+      if (fmt == 4) {
+        std::memcpy(texels + 4 * ti, &v, 4);
+      } else {
+        float vn = tclamp((v - qmin) * qinv, 0.0f, 1.0f);
+        if (fmt == 2) {
+          uint16_t q = uint16_t(vn * 65535.0f);
+          std::memcpy(texels + 2 * ti, &q, 2);
+        } else {
+          texels[ti] = (unsigned char)(vn * 255.0f);
+        }
+      }
+    }
   }
 }
 

@@ -49,17 +49,18 @@ def icosphere(radius=0.5, subdivisions=3) -> Mesh:
     return Mesh((np.array(v) * radius).astype(np.float32), np.array(f, np.int32).reshape(-1))
 
 
-def cook_cpu(mesh: Mesh, max_resolution: int = 16, margin: float = 0.05, scale=None):
-    """Attach an SDF cooked by the oracle (CPU tests have no GPU cook), with Mesh.build_sdf's geometry,
+def cook_cpu(mesh: Mesh, max_resolution: int = 16, margin: float = 0.05, scale=None, texture_format="float32",
+             narrow_band_range=(-0.1, 0.1)):
+    """Attach a texture SDF cooked by the oracle (CPU tests have no GPU cook), with Mesh.build_sdf's geometry,
     winding threshold and scale baking."""
     import oracle
 
     sc = (1.0, 1.0, 1.0) if scale is None else scale
-    lo, vx, cells = mesh.sdf_grid_geometry(max_resolution, margin, sc)
-    verts = (mesh.vertices.astype(np.float64) * np.asarray(sc, np.float64)).astype(np.float32)
+    verts = (mesh.vertices * np.array(sc)[None, :]).astype(np.float32)
+    geo = oracle.texture_geometry(verts, margin, narrow_band_range, max_resolution)
     thr = 0.5 if mesh.signed_volume(sc) >= 0.0 else -0.5
-    g = oracle.sdf_build(verts, mesh.indices, lo, vx, cells, winding_threshold=thr)
-    mesh.attach_sdf(torch.from_numpy(g), lo, vx, cells, scale_baked=scale is not None)
+    sparse = oracle.sdf_build_sparse(verts, mesh.indices, geo, texture_format, winding_threshold=thr)
+    mesh.attach_sdf(sparse, "cpu", scale_baked=scale is not None)
     return mesh
 
 

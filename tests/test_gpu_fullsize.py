@@ -132,7 +132,7 @@ def test_fullsize_sdf_parity(cuda_device, reduce):
     print(f"[full size sdf] 64-hull library, 64^3 SDFs, 1024 worlds, reduce_contacts={reduce}", flush=True)
     lib = scenes.mesh_library(64, seed=0)
     for mm in lib:
-        mm.build_sdf(max_resolution=64, device=cuda_device)
+        mm.build_sdf(max_resolution=64, device=cuda_device, texture_format="float32")
     model = scenes.mesh_worlds(1024, seed=0, device=cuda_device, library=lib)
     # unreduced, pairs keep every contact (about 7 000 a world once piled): a capacity beyond the estimate
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True, reduce_contacts=reduce,

@@ -1,7 +1,8 @@
 """GPU tests of the mesh path: SDF cook, mesh-mesh / mesh-plane contacts and the step over them.
 
-* The GPU cook (nmc_sdf_build) against the oracle's cook, value for value.
-* The cooked grid's accuracy against the exact FP64 distance to the mesh, with the reference's own
+* The GPU cook of the reference's texture SDF (nmc_sdf_build / nmc_sdf_subgrids / nmc_sdf_pack) against the
+  oracle's cook, array for array, in the three texel formats.
+* The cooked texture's accuracy against the exact FP64 distance to the mesh, with the reference's own
   tolerances for a 64-cell grid of a radius-0.5 sphere (tests/test_sdf_texture.py:1123-1228: distance
   mean 2e-4 / median 1.5e-4 / p95 7e-4 / max 2e-3 m, gradient angle mean 2 / median 1.5 / p95 5 deg
   within 0.05 m of the surface).
@@ -29,20 +30,45 @@ pytestmark = pytest.mark.gpu
 DT = 1.0 / 240.0
 
 
+def assert_texture_matches(got: dict, ref: dict, fmt: str):
+    """A GPU-cooked texture against the oracle's: geometry, flags, slots and texture sizes equal; values equal
+    in magnitude, with the winding-number sign free to differ only on the surface itself (|v| < 1e-6), which
+    moves a unorm texel by at most one step."""
+    for k in ("coarse_dims", "subgrid_tex_size", "num_subgrids", "subgrid_size", "quantization_mode"):
+        assert tuple(np.atleast_1d(got[k])) == tuple(np.atleast_1d(ref[k])), k
+    for k in ("min_extents", "max_extents", "cell_size"):
+        np.testing.assert_array_equal(np.asarray(got[k], np.float64), np.asarray(ref[k], np.float64), err_msg=k)
+    for k in ("subgrids_min_sdf_value", "subgrids_sdf_value_range"):
+        assert np.float32(got[k]) == np.float32(ref[k]), k
+    for k in ("subgrid_required", "subgrid_occupied", "subgrid_start_slots"):
+        np.testing.assert_array_equal(got[k], ref[k], err_msg=k)
+    gc, rc = got["coarse_sdf"], ref["coarse_sdf"]
+    np.testing.assert_array_equal(np.abs(gc), np.abs(rc))
+    assert np.all(np.abs(rc[np.sign(gc) != np.sign(rc)]) < 1e-6)
+    gt, rt = got["subgrid_data"], ref["subgrid_data"]
+    assert gt.dtype == rt.dtype and gt.shape == rt.shape
+    if fmt == "float32":
+        np.testing.assert_array_equal(np.abs(gt), np.abs(rt))
+        assert np.all(np.abs(rt[np.sign(gt) != np.sign(rt)]) < 1e-6)
+    else:
+        assert np.abs(gt.astype(np.int64) - rt.astype(np.int64)).max() <= 1
+        assert np.mean(gt != rt) < 1e-3
+
+
 def test_sdf_cook_matches_oracle(cuda_device):
     rng = np.random.Generator(np.random.PCG64(11))
     meshes = [box_mesh(0.3, 0.2, 0.1), random_convex_hull(rng), random_convex_hull(rng, 64, 96)]
     for k, m in enumerate(meshes):
-        print(f"[sdf cook parity] mesh {k}: {len(m.vertices)} vertices", flush=True)
-        sdf = m.build_sdf(max_resolution=24, device=cuda_device)
-        torch.cuda.synchronize()
-        got = sdf.grid.cpu().numpy()
-        ref = oracle.sdf_build(m.vertices, m.indices, sdf.lower, sdf.voxel_size, sdf.cells)
-        assert got.shape == ref.shape
-        np.testing.assert_array_equal(np.abs(got), np.abs(ref))
-        flip = np.sign(got) != np.sign(ref)
-        # the winding-number sign can only differ on the surface itself
-        assert np.all(np.abs(ref[flip]) < 1e-6), f"{int(flip.sum())} sign flips off the surface"
+        for fmt, res in (("uint16", 24), ("float32", 32), ("uint8", 16)):
+            print(f"[sdf cook parity] mesh {k}: {len(m.vertices)} vertices, {fmt}, {res} cells", flush=True)
+            m.clear_sdf()
+            sdf = m.build_sdf(max_resolution=res, device=cuda_device, texture_format=fmt)
+            torch.cuda.synchronize()
+            got = sdf.sparse_data()
+            geo = oracle.texture_geometry(m.vertices, 0.05, (-0.1, 0.1), res)
+            ref = oracle.sdf_build_sparse(m.vertices, m.indices, geo, fmt)
+            assert ref["num_subgrids"] > 0
+            assert_texture_matches(got, ref, fmt)
 
 
 def test_sdf_accuracy_vs_exact_distance(cuda_device):

@@ -156,56 +156,82 @@ def test_solid_angle_closed_forms():
         assert abs(om.sum() / (4 * math.pi) - wn) < 1e-9
 
 
+def _fine_points(geo, dtype=np.float64):
+    """Positions of the padded fine grid's vertices, (nz, ny, nx, 3), as the cook computes them in FP32."""
+    w, h, d = geo["coarse_dims"]
+    lo = np.asarray(geo["min_extents"], np.float32)
+    ce = np.asarray(geo["cell_size"], np.float64).astype(np.float32)
+    iz, iy, ix = np.meshgrid(np.arange(8 * d + 1), np.arange(8 * h + 1), np.arange(8 * w + 1), indexing="ij")
+    p = np.stack([lo[0] + ix.astype(np.float32) * ce[0], lo[1] + iy.astype(np.float32) * ce[1],
+                  lo[2] + iz.astype(np.float32) * ce[2]], -1)
+    return p.astype(dtype)
+
+
 def test_sdf_cook_box_exact():
+    """The mesh query at every vertex of the padded fine grid equals the closed-form box SDF (FP64 cook to
+    1e-12, FP32 to 2e-7), beyond the mesh's padded box too."""
     h = (0.3, 0.2, 0.1)
     m = box_mesh(*h)
-    lo, vx, cells = m.sdf_grid_geometry(16, 0.05)
-    assert cells.max() == 16
-    np.testing.assert_allclose(lo, [-0.35, -0.25, -0.15], atol=1e-6)
-    g = oracle.sdf_build(m.vertices, m.indices, lo, vx, cells)
-    g64 = oracle.sdf_build(m.vertices, m.indices, lo, vx, cells, precision="f64")
-    iz, iy, ix = np.meshgrid(*[np.arange(n) for n in g.shape], indexing="ij")
+    geo = oracle.texture_geometry(m.vertices, 0.05, (-0.1, 0.1), 16)
+    w, hh, d = geo["coarse_dims"]
+    n = (8 * w + 1, 8 * hh + 1, 8 * d + 1)
+    ce = np.asarray(geo["cell_size"]).astype(np.float32)
+    g = oracle.sdf_fine_grid(m.vertices, m.indices, geo["min_extents"], ce, n)
+    g64 = oracle.sdf_fine_grid(m.vertices, m.indices, geo["min_extents"], ce, n, precision="f64")
     hf = np.array(h, np.float32).astype(np.float64)  # the mesh's vertices are FP32
-    lo64, vx64 = lo.astype(np.float64), float(np.float32(vx))
-    p64 = np.stack([lo64[0] + vx64 * ix, lo64[1] + vx64 * iy, lo64[2] + vx64 * iz], -1)
+    p = _fine_points(geo)
+    lo64 = np.asarray(geo["min_extents"], np.float64)
+    iz, iy, ix = np.meshgrid(*[np.arange(k) for k in g.shape], indexing="ij")
+    p64 = np.stack([lo64[0] + ce[0] * ix, lo64[1] + ce[1] * iy, lo64[2] + ce[2] * iz], -1)
     np.testing.assert_allclose(g64, box_sdf(p64, hf), rtol=0, atol=1e-12)
-    p32 = np.stack([lo[0] + np.float32(vx) * ix.astype(np.float32), lo[1] + np.float32(vx) * iy.astype(np.float32),
-                    lo[2] + np.float32(vx) * iz.astype(np.float32)], -1).astype(np.float64)
-    np.testing.assert_allclose(g, box_sdf(p32, hf), rtol=0, atol=2e-7)
+    np.testing.assert_allclose(g, box_sdf(p, hf), rtol=0, atol=2e-7)
     assert (g < 0).sum() > 0 and (g > 0).sum() > 0
 
 
-# ---------------------------------------------------------------- sampling and search
-def _linear_grid_model(a, b0, cells=(8, 6, 5), voxel=0.1, lower=(-0.4, -0.3, -0.2)):
-    """A model with one box mesh whose SDF grid holds the linear field a . x + b0."""
+def _field_sdf(fine, geo, fmt="float32", linear_threshold=None):
+    """A model with one box mesh whose texture SDF is built from the given fine-grid field."""
+    g = dict(geo)
+    if linear_threshold is not None:
+        g["linear_threshold"] = linear_threshold
+    sparse = oracle.sdf_build_sparse(None, None, g, fmt, fine=fine)
     m = box_mesh(0.2, 0.2, 0.1)
-    cx, cy, cz = cells
-    iz, iy, ix = np.meshgrid(np.arange(cz + 1), np.arange(cy + 1), np.arange(cx + 1), indexing="ij")
-    x = np.stack([lower[0] + voxel * ix, lower[1] + voxel * iy, lower[2] + voxel * iz], -1)
-    g = (x @ np.asarray(a, np.float64) + b0).astype(np.float32)
-    m.attach_sdf(torch.from_numpy(g), np.asarray(lower, np.float32), voxel, np.asarray(cells, np.int32))
+    m.attach_sdf(sparse, "cpu")
     b = ModelBuilder()
-    body = b.add_body()
-    b.add_shape_mesh(body, mesh=m)
-    return b.finalize("cpu"), np.asarray(lower), np.asarray(lower) + voxel * np.asarray(cells)
+    b.add_shape_mesh(b.add_body(), mesh=m)
+    return b.finalize("cpu"), sparse
 
 
-def test_sdf_sample_linear_field_and_extrapolation():
-    a = np.array([0.3, -0.7, 0.5])
-    model, lo, hi = _linear_grid_model(a, 0.05)
+@pytest.mark.parametrize("linear_threshold", [None, 0.0])
+def test_sdf_sample_linear_field_and_extrapolation(linear_threshold):
+    """A linear field is reproduced exactly, value and gradient, whether every subgrid is demoted to the
+    coarse grid (the linearity test passes a linear field) or keeps its texture (threshold 0); outside the
+    SDF box the value gains the distance to the box and the gradient is the unit offset."""
+    a = np.array([0.03, -0.07, 0.05])
+    geo = oracle.texture_geometry(box_mesh(0.2, 0.2, 0.1).vertices, 0.05, (-1.0, 1.0), 16)
+    p = _fine_points(geo)
+    model, sparse = _field_sdf((p @ a + 0.005).astype(np.float32), geo, linear_threshold=linear_threshold)
+    if linear_threshold is None:
+        assert sparse["num_subgrids"] == 0 and sparse["subgrid_linear"].all()
+        assert sparse["subgrid_data"].shape == (1, 1, 1) and sparse["subgrid_data"].dtype == np.float32
+    else:
+        assert sparse["num_subgrids"] == len(sparse["subgrid_required"]) and not sparse["subgrid_linear"].any()
     o = oracle.Oracle(model)
+    lo = np.asarray(sparse["min_extents"], np.float64)
+    hi = np.asarray(sparse["max_extents"], np.float32).astype(np.float64)
     rng = np.random.Generator(np.random.PCG64(1))
-    p = rng.uniform(lo, hi, size=(500, 3))
-    v, g = o.sdf_sample(0, p)
-    np.testing.assert_allclose(v, p @ a + 0.05, atol=2e-6)
-    np.testing.assert_allclose(g, np.tile(a, (len(p), 1)), atol=2e-5)
-    # outside the grid box: value at the clamped point + distance to the box, gradient = unit offset
-    q = p.copy()
-    q[:, 0] = hi[0] + rng.uniform(0.1, 1.0, len(q))
-    v2, g2 = o.sdf_sample(0, q)
-    cl = np.clip(q, lo, hi)
-    np.testing.assert_allclose(v2, cl @ a + 0.05 + np.linalg.norm(q - cl, axis=1), atol=3e-6)
-    np.testing.assert_allclose(g2, (q - cl) / np.linalg.norm(q - cl, axis=1)[:, None], atol=1e-6)
+    q = rng.uniform(lo, hi, size=(500, 3))
+    v, gr = o.sdf_sample(0, q)
+    np.testing.assert_allclose(v, q @ a + 0.005, atol=2e-7)
+    inv = (1.0 / np.asarray(sparse["cell_size"])).astype(np.float32)
+    scale = 1.0 if linear_threshold is not None else 8.0  # coarse cells: gradient of coarse weights x fine 1/dx
+    np.testing.assert_allclose(gr, np.tile(a * scale, (len(q), 1)), rtol=2e-4, atol=1e-6)
+    assert inv.min() > 0
+    q2 = q.copy()
+    q2[:, 0] = hi[0] + rng.uniform(0.1, 1.0, len(q2))
+    v2, g2 = o.sdf_sample(0, q2)
+    cl = np.clip(q2, lo, hi)
+    np.testing.assert_allclose(v2, cl @ a + 0.005 + np.linalg.norm(q2 - cl, axis=1), atol=3e-6)
+    np.testing.assert_allclose(g2, (q2 - cl) / np.linalg.norm(q2 - cl, axis=1)[:, None], atol=1e-6)
 
 
 def test_sdf_gradient_matches_finite_differences():
@@ -213,21 +239,25 @@ def test_sdf_gradient_matches_finite_differences():
     b = ModelBuilder()
     b.add_shape_mesh(b.add_body(), mesh=m)
     o = oracle.Oracle(b.finalize("cpu"))
+    sp = m.sdf.sparse_data()
     rng = np.random.Generator(np.random.PCG64(2))
-    lo, hi = m.sdf.lower.astype(np.float64), m.sdf.upper.astype(np.float64)
-    vx = m.sdf.voxel_size
-    # points at cell centres +- a quarter voxel, so +-h stays inside one cell
-    cells = rng.integers(0, m.sdf.cells, size=(200, 3))
-    p = lo + (cells + 0.5 + rng.uniform(-0.25, 0.25, size=(200, 3))) * vx
+    lo = np.asarray(sp["min_extents"], np.float64)
+    ce = np.asarray(sp["cell_size"], np.float64)
+    w, h, d = sp["coarse_dims"]
+    # cell centres +- a quarter cell inside subgrids that keep their texture, so +-h stays inside one cell
+    req = np.flatnonzero(sp["subgrid_required"])
+    s = req[rng.integers(0, len(req), 200)]
+    sub = np.stack([s % w, (s // w) % h, s // (w * h)], -1)
+    cells = sub * 8 + rng.integers(0, 8, size=(200, 3))
+    p = lo + (cells + 0.5 + rng.uniform(-0.25, 0.25, size=(200, 3))) * ce
     _, g = o.sdf_sample(0, p)
-    h = 0.05 * vx
+    hstep = 0.05 * ce.min()
     fd = np.zeros_like(p)
     for k in range(3):
         e = np.zeros(3)
-        e[k] = h
-        fd[:, k] = (o.sdf_sample(0, p + e)[0].astype(np.float64) - o.sdf_sample(0, p - e)[0]) / (2 * h)
+        e[k] = hstep
+        fd[:, k] = (o.sdf_sample(0, p + e)[0].astype(np.float64) - o.sdf_sample(0, p - e)[0]) / (2 * hstep)
     np.testing.assert_allclose(g, fd, atol=2e-3 * np.abs(fd).max())
-    assert np.all(hi > lo)
 
 
 def test_edge_search_vs_brute_force():
@@ -338,20 +368,24 @@ def test_reduced_mesh_contacts_subset_of_unreduced():
 # ---------------------------------------------------------------- winding threshold and baked scale
 def test_inward_winding_threshold():
     """An inward-wound mesh has negative signed volume and cooks, with threshold -0.5
-    (sdf_utils.py:506-510, 640-651), to the outward mesh's grid; with +0.5 it would be outside everywhere."""
+    (sdf_utils.py:506-510, 640-651), to the outward mesh's field; with +0.5 it would be outside everywhere."""
     m = box_mesh(0.3, 0.2, 0.1)
     inv = Mesh(m.vertices, m.indices.reshape(-1, 3)[:, [0, 2, 1]].reshape(-1))
     np.testing.assert_allclose(m.signed_volume(), 0.6 * 0.4 * 0.2, rtol=1e-6)
     np.testing.assert_allclose(inv.signed_volume(), -0.6 * 0.4 * 0.2, rtol=1e-6)
-    lo, vx, cells = m.sdf_grid_geometry(16, 0.05)
-    g = oracle.sdf_build(m.vertices, m.indices, lo, vx, cells)
-    gi = oracle.sdf_build(inv.vertices, inv.indices, lo, vx, cells, winding_threshold=-0.5)
-    wrong = oracle.sdf_build(inv.vertices, inv.indices, lo, vx, cells, winding_threshold=0.5)
+    geo = oracle.texture_geometry(m.vertices, 0.05, (-0.1, 0.1), 16)
+    w, h, d = geo["coarse_dims"]
+    n, ce = (8 * w + 1, 8 * h + 1, 8 * d + 1), np.asarray(geo["cell_size"]).astype(np.float32)
+    g = oracle.sdf_fine_grid(m.vertices, m.indices, geo["min_extents"], ce, n)
+    gi = oracle.sdf_fine_grid(inv.vertices, inv.indices, geo["min_extents"], ce, n, winding_threshold=-0.5)
+    wrong = oracle.sdf_fine_grid(inv.vertices, inv.indices, geo["min_extents"], ce, n, winding_threshold=0.5)
     np.testing.assert_allclose(gi, g, rtol=0, atol=1e-7)
     assert np.array_equal(np.sign(gi[np.abs(g) > 1e-6]), np.sign(g[np.abs(g) > 1e-6]))
     assert (g < 0).sum() > 0 and np.all(wrong >= 0)
     cook_cpu(inv, 16)
-    np.testing.assert_allclose(inv.sdf.grid.numpy(), g, rtol=0, atol=1e-7)
+    cook_cpu(m, 16)
+    np.testing.assert_allclose(inv.sdf.sparse_data()["coarse_sdf"], m.sdf.sparse_data()["coarse_sdf"], atol=1e-7)
+    np.testing.assert_allclose(inv.sdf.sparse_data()["subgrid_data"], m.sdf.sparse_data()["subgrid_data"], atol=1e-7)
 
 
 def _stacked_boxes(baked: bool):
