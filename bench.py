@@ -159,8 +159,8 @@ def phase_times(pipe, solver, a, b, contacts, sdt, reps):
 
 def timed_form(args, model, pipe, solver, worlds, settle, warmup, steps, dev, tune=None):
     """Settle (untimed), warm up, then time `steps` graph replays of one step (4 substeps of collide + step).
-    `tune(state_a, state_b, contacts, sdt)` (optional) runs on the settled state and may change the step's
-    launch shape; the graph is captured again after it.  Returns (states, contacts, ms, N_c, t_collide_ms,
+    `tune(state_a, state_b, contacts, sdt)` (optional) runs on the settled state and may change the collide and
+    step kernels' launch shapes; the graph is captured again after it.  Returns (states, contacts, ms, N_c, t_collide_ms,
     t_xpbd_ms, tune result)."""
     import torch
 
@@ -281,8 +281,12 @@ def main():
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=args.iterations)
     sdt = (1.0 / 60.0) / args.substeps
-    states, contacts, ms, n_c, t_col_ms, t_xp_ms, _ = timed_form(args, model, pipe, solver, args.worlds,
-                                                                  args.settle, args.warmup, args.steps, dev)
+    from newton_custom_contact_amd import autotune
+
+    tune = lambda a, b, c, sdt: autotune.tune(pipe, solver, a, b, c, sdt)  # noqa: E731
+    states, contacts, ms, n_c, t_col_ms, t_xp_ms, tuned = timed_form(args, model, pipe, solver, args.worlds,
+                                                                      args.settle, args.warmup, args.steps, dev,
+                                                                      tune=tune)
     ms_per_step = ms / args.steps
     ws_per_s = args.worlds * args.steps / (ms / 1e3)
     nonfinite = int((~torch.isfinite(states[0].body_qd).all(dim=1)).sum())
@@ -356,6 +360,7 @@ def main():
             "fraction_of_step_roofline": round(ws_per_s / roof_ws, 5),
             "collide_ms_per_substep": round(t_col_ms, 4),
             "xpbd_ms_per_substep": round(t_xp_ms, 4),
+            "launch_shapes": tuned,
         },
         "roofline": roof,
         "cpu_baseline": cpu,
@@ -648,10 +653,10 @@ def components_form(args, model, state, sdt, dev):
 def uneven_form(args, dev, job):
     """Uneven worlds: ``--uneven-worlds`` worlds of log-uniform 8-256 primitives (spheres, boxes, capsules) plus
     a 6-link revolute capsule chain with +-90 degree limits each, on a ground plane; 4 substeps, 10 iterations,
-    the step kernel's launch shape autotuned on the settled state, hipGraph step, timed after
+    the collide and step kernels' launch shapes autotuned on the settled state, hipGraph step, timed after
     ``--uneven-settle`` steps.  The scene was built on the host by a child process started at the beginning."""
     from newton_custom_contact_amd import CollisionPipeline
-    from newton_custom_contact_amd.autotune import tune_block_lanes
+    from newton_custom_contact_amd import autotune
     from newton_custom_contact_amd.model import Model
     from newton_custom_contact_amd.solvers import SolverXPBD
 
@@ -673,7 +678,7 @@ def uneven_form(args, dev, job):
           file=sys.stderr, flush=True)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=args.iterations)
-    tune = lambda a, b, c, sdt: tune_block_lanes(solver, a, b, c, sdt)  # noqa: E731
+    tune = lambda a, b, c, sdt: autotune.tune(pipe, solver, a, b, c, sdt)  # noqa: E731
     states, contacts, ms, n_c, t_col, t_xp, tuned = timed_form(args, model, pipe, solver, W, args.uneven_settle,
                                                                 3, args.uneven_steps, dev, tune=tune)
     ws = W * args.uneven_steps / (ms / 1e3)
@@ -706,7 +711,7 @@ def uneven_form(args, dev, job):
         "xpbd_ms_per_substep": round(t_xp, 4),
         "roofline": roofline_block(t_col, t_xp, W, b_collide, b_step),
         "overflow": oc,
-        "step_kernel_lanes": tuned,
+        "launch_shapes": tuned,
     }
 
 
@@ -714,12 +719,12 @@ def sdf_form(args, dev):
     """The triangle-mesh workload: ``--sdf-worlds`` worlds x 8 convex-hull meshes (64-256 points on a sphere
     of radius 0.1-0.3 m, from a seeded library of 64 hulls, 64^3 FP32 SDF each) dropped on a ground plane,
     mesh-mesh and mesh-plane contacts with the reference's contact reduction (at most 240 per pair), 4 substeps,
-    10 iterations, hipGraph step; the step kernel's launch shape autotuned on the settled state.  Timed after
+    10 iterations, hipGraph step; the collide and step kernels' launch shapes autotuned on the settled state.  Timed after
     ``--sdf-settle`` untimed steps, when the hulls have landed and pile up."""
     import torch
 
     from newton_custom_contact_amd import CollisionPipeline, scenes
-    from newton_custom_contact_amd.autotune import tune_block_lanes
+    from newton_custom_contact_amd import autotune
     from newton_custom_contact_amd.solvers import SolverXPBD
 
     t0 = time.time()
@@ -733,7 +738,7 @@ def sdf_form(args, dev):
           f"(library + cook {t_cook:.1f}s, total {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=args.iterations)
-    tune = lambda a, b, c, sdt: tune_block_lanes(solver, a, b, c, sdt)  # noqa: E731
+    tune = lambda a, b, c, sdt: autotune.tune(pipe, solver, a, b, c, sdt)  # noqa: E731
     states, contacts, ms, n_c, t_col, t_xp, tuned = timed_form(args, model, pipe, solver, args.sdf_worlds,
                                                                 args.sdf_settle, 5, args.sdf_steps, dev, tune=tune)
     ws = args.sdf_worlds * args.sdf_steps / (ms / 1e3)
@@ -759,7 +764,7 @@ def sdf_form(args, dev):
         "xpbd_ms_per_substep": round(t_xp, 4),
         "roofline": roofline_block(t_col, t_xp, args.sdf_worlds, b_collide, b_step, xpbd_kernel=False),
         "overflow": oc,
-        "step_kernel_lanes": tuned,
+        "launch_shapes": tuned,
         "lowest_body_z_m": round(float(states[0].body_q[:, 2].min()), 4),
     }
 

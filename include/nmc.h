@@ -402,6 +402,18 @@ int nmc_broad_phase_explicit_noalloc(int shape_count, const float* lower, const 
                                      int max_candidate_pair, int* candidate_pair_count, int append,
                                      nmc_stream_t stream);
 
+/* Launch shape of nmc_collide's kernels, picked by the package's autotuner (no counterpart in the reference,
+ * whose Warp launches have fixed shapes): segment_waves = segments of at most 64 shapes per workgroup of the
+ * broad phase (1, 2 or 4, one wave each); narrow_max_blocks / mesh_max_blocks = grid caps of the grid-stride
+ * narrow-phase and mesh kernels (256 lanes a workgroup).  0 keeps the default (4, 2048, 2048).  Every shape
+ * gives bit-identical contacts; the setting holds for later nmc_collide calls on the model. */
+typedef struct NmcLaunchConfig {
+  int segment_waves;
+  int narrow_max_blocks;
+  int mesh_max_blocks;
+} NmcLaunchConfig;
+int nmc_model_set_launch(NmcModel* model, const NmcLaunchConfig* config);
+
 /* Copies the counters accumulated since model creation to host (synchronises the stream):
  * out[0] = broad-phase pairs dropped (per-segment pair buffer full), out[1] = mesh-pair candidate
  * contacts dropped (candidate buffer full; cannot happen for models nmc_model_create accepted),

@@ -51,6 +51,11 @@ class NmcModelDesc(ctypes.Structure):
     ]
 
 
+class NmcLaunchConfig(ctypes.Structure):
+    _fields_ = [("segment_waves", ctypes.c_int), ("narrow_max_blocks", ctypes.c_int),
+                ("mesh_max_blocks", ctypes.c_int)]
+
+
 class NmcContacts(ctypes.Structure):
     _fields_ = [
         ("capacity", ctypes.c_int), ("count", _vp), ("shape0", _vp), ("shape1", _vp), ("point0", _vp),
@@ -100,7 +105,7 @@ EXPORTS = ("nmc_model_create", "nmc_model_destroy", "nmc_collide", "nmc_xpbd_ste
            "nmc_matcher_create", "nmc_matcher_destroy",
            "nmc_matcher_reset", "nmc_contact_match", "nmc_xpbd_step_report", "nmc_contact_force",
            "nmc_broad_phase_create", "nmc_broad_phase_destroy", "nmc_broad_phase_launch", "nmc_broad_phase_explicit",
-           "nmc_broad_phase_explicit_noalloc",
+           "nmc_broad_phase_explicit_noalloc", "nmc_model_set_launch",
            "nmc_get_counters", "nmc_last_error")
 
 
@@ -148,6 +153,8 @@ def load() -> ctypes.CDLL:
                                     ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_int), ctypes.c_float, _vp,
                                     _vp]
         L.nmc_sdf_build.restype = ctypes.c_int
+        L.nmc_model_set_launch.argtypes = [_vp, ctypes.POINTER(NmcLaunchConfig)]
+        L.nmc_model_set_launch.restype = ctypes.c_int
         L.nmc_sdf_subgrids.argtypes = [_vp, ctypes.POINTER(ctypes.c_int), ctypes.c_float, ctypes.c_float,
                                        ctypes.c_float, _vp, _vp, _vp, _vp, _vp]
         L.nmc_sdf_subgrids.restype = ctypes.c_int

@@ -22,7 +22,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._native import NativeModel, NmcContactReport, check, load, ptr, stream_handle
+from ._native import NativeModel, NmcContactReport, NmcLaunchConfig, check, load, ptr, stream_handle
 from ._types import GeoType, ShapeFlags
 from .contacts import Contacts
 
@@ -176,6 +176,7 @@ class CollisionPipeline:
                                   None if explicit_pairs is None else explicit_pairs.to(model.device),
                                   reduce_contacts=bool(reduce_contacts), deterministic=self.deterministic)
         self._lib = load()
+        self._launch = {"segment_waves": 4, "narrow_max_blocks": 2048, "mesh_max_blocks": 2048}
         self.contact_matching = contact_matching
         self._matching_enabled = matching_enabled
         self._matching_sticky = contact_matching == "sticky"
@@ -199,6 +200,31 @@ class CollisionPipeline:
     @property
     def soft_contact_max(self) -> int:
         return self._soft_contact_max
+
+    @property
+    def launch_config(self) -> dict:
+        """Launch shape of the collide kernels (``autotune.tune_collide`` picks it): ``segment_waves`` (segments
+        of at most 64 shapes per broad-phase workgroup: 1, 2 or 4), ``narrow_max_blocks`` and
+        ``mesh_max_blocks`` (grid caps of the narrow-phase and mesh kernels).  Contacts are bit-identical for
+        every shape."""
+        return dict(self._launch)
+
+    def set_launch(self, segment_waves: int | None = None, narrow_max_blocks: int | None = None,
+                   mesh_max_blocks: int | None = None) -> None:
+        """Set the collide kernels' launch shape (``None`` keeps the current value); raises ``ValueError`` for
+        values the kernels do not take."""
+        cfg = dict(self._launch)
+        for k, v in (("segment_waves", segment_waves), ("narrow_max_blocks", narrow_max_blocks),
+                     ("mesh_max_blocks", mesh_max_blocks)):
+            if v is not None:
+                cfg[k] = int(v)
+        if cfg["segment_waves"] not in (1, 2, 4):
+            raise ValueError("segment_waves must be 1, 2 or 4")
+        if not (1 <= cfg["narrow_max_blocks"] <= 65535 and 1 <= cfg["mesh_max_blocks"] <= 65535):
+            raise ValueError("grid caps must lie in [1, 65535]")
+        check(self._lib.nmc_model_set_launch(self.native.handle, ctypes.byref(NmcLaunchConfig(**cfg))),
+              "nmc_model_set_launch")
+        self._launch = cfg
 
     def contacts(self) -> Contacts:
         """collide.py:858-885."""

@@ -34,7 +34,6 @@ static_assert(nmc::kNarrowBlock == nmc::CBLOCK, "the narrow kernel's clip buffer
 
 namespace nmc {
 
-constexpr int NARROW_MAX_BLOCKS = 2048;  // 8 workgroups per CU; the narrow kernel strides over the rest
 
 // Inclusive scan of one value per lane over a CBLOCK-lane workgroup (wave shuffles + 4 wave totals).
 __device__ __forceinline__ int block_inclusive_scan(int v, int* s_wave) {
@@ -495,8 +494,9 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_kernel(NmcModelDesc d,
                            smem, s_tmp);
 }
 
-// One wave per listed segment (segments of at most 64 shapes), four segments per workgroup: the sorts and
-// the sweep synchronise one wave instead of the workgroup.  lds_per_wave: the dynamic LDS of one wave.
+// One wave per listed segment (segments of at most 64 shapes), blockDim / 64 segments per workgroup (1, 2 or
+// 4, a tuned launch parameter): the enumeration synchronises one wave instead of the workgroup.
+// lds_per_wave: the dynamic LDS of one wave.
 __global__ void __launch_bounds__(CBLOCK) collide_segment_wave_kernel(NmcModelDesc d, const float* __restrict__ body_q,
                                                                        StageBuffers st,
                                                                        const long long* __restrict__ seg_pair_off,
@@ -509,7 +509,7 @@ __global__ void __launch_bounds__(CBLOCK) collide_segment_wave_kernel(NmcModelDe
                                                                        int n_list, int lds_per_wave) {
   extern __shared__ __align__(16) unsigned char smem[];
   const int wave = threadIdx.x >> 6;
-  const int idx = blockIdx.x * (CBLOCK / 64) + wave;
+  const int idx = blockIdx.x * static_cast<int>(blockDim.x >> 6) + wave;
   if (idx >= n_list) return;  // whole waves only; the kernel has no workgroup barrier
   segment_broad_phase_enum(WaveGroup{static_cast<int>(threadIdx.x & 63)}, d, body_q, st, seg_pair_off, seg_pair_cap,
                            seg_pair_count, seg_conv_count, seg_mesh_count, counters, max_n, seg_list[idx],
@@ -824,8 +824,8 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
   int segs = d.segment_count;
   if (segs <= 0) return hipSuccess;
   if (m->n_seg_small > 0) {
-    const int per = CBLOCK / 64;
-    hipLaunchKernelGGL(collide_segment_wave_kernel, dim3((m->n_seg_small + per - 1) / per), dim3(CBLOCK),
+    const int per = m->segment_waves;
+    hipLaunchKernelGGL(collide_segment_wave_kernel, dim3((m->n_seg_small + per - 1) / per), dim3(64 * per),
                        static_cast<size_t>(per) * m->seg_small_lds, stream, d, body_q, m->stage, m->seg_pair_off,
                        m->seg_pair_cap, m->seg_pair_count, m->seg_conv_count, m->seg_mesh_count, m->counters,
                        m->seg_small_max_n, m->seg_small, m->n_seg_small,
@@ -841,7 +841,8 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
                      mesh ? m->seg_mesh_off : nullptr, m->stage.route_count, segs);
   long long pairs = m->stage.pair_capacity;
   long long nblk = (pairs + CBLOCK - 1) / CBLOCK;
-  int narrow_blocks = static_cast<int>(nblk < NARROW_MAX_BLOCKS ? nblk : NARROW_MAX_BLOCKS);
+  int narrow_blocks = static_cast<int>(nblk < m->narrow_max_blocks ? nblk : m->narrow_max_blocks);
+  if (narrow_blocks < 1) narrow_blocks = 1;
   hipLaunchKernelGGL(narrow_kernel, dim3(narrow_blocks), dim3(CBLOCK), 0, stream, d, body_q, m->stage,
                      m->seg_pair_off, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, segs);
   if (mesh) {

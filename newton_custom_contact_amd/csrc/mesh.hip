@@ -36,7 +36,6 @@
 namespace nmc {
 
 constexpr int MESH_BLOCK = 256;
-constexpr int MESH_MAX_BLOCKS = 2048;
 // Slot tables of up to this many subgrids (12^3: up to 96 cells a side) are staged in LDS per mesh pair, so a
 // sample's indirection is an LDS read; larger textures are read from global memory.
 constexpr int MESH_SLOT_LDS = 2048;
@@ -568,7 +567,7 @@ size_t mesh_lds_bytes(int cand_cap, bool reduce) {
 hipError_t launch_mesh_narrow(NmcModelImpl* m, const float* body_q, hipStream_t stream) {
   if (m->stage.mesh_pair_capacity <= 0) return hipSuccess;
   const long long cap = m->stage.mesh_pair_capacity;
-  const int blocks = static_cast<int>(cap < MESH_MAX_BLOCKS ? cap : MESH_MAX_BLOCKS);
+  const int blocks = static_cast<int>(cap < m->mesh_max_blocks ? cap : m->mesh_max_blocks);
   if (m->red.mode != 0)
     hipLaunchKernelGGL(mesh_narrow_kernel<true>, dim3(blocks), dim3(MESH_BLOCK), m->mesh_lds, stream, m->d, body_q,
                        m->stage, m->seg_pair_off, m->seg_pair_count, m->seg_mesh_count, m->seg_mesh_off,

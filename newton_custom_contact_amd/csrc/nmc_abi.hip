@@ -523,6 +523,20 @@ extern "C" int nmc_contact_force(const float* contact_impulse, const int* count,
   return e == hipSuccess ? 0 : hip_fail(e, "nmc_contact_force launch");
 }
 
+extern "C" int nmc_model_set_launch(NmcModel* m, const NmcLaunchConfig* cfg) {
+  if (!m || !cfg) return fail("nmc_model_set_launch: null argument");
+  const int sw = cfg->segment_waves ? cfg->segment_waves : 4;
+  const int nb = cfg->narrow_max_blocks ? cfg->narrow_max_blocks : 2048;
+  const int mb = cfg->mesh_max_blocks ? cfg->mesh_max_blocks : 2048;
+  if (sw != 1 && sw != 2 && sw != 4) return fail("nmc_model_set_launch: segment_waves must be 1, 2 or 4");
+  if (nb < 1 || nb > 65535 || mb < 1 || mb > 65535)
+    return fail("nmc_model_set_launch: grid caps must lie in [1, 65535]");
+  m->segment_waves = sw;
+  m->narrow_max_blocks = nb;
+  m->mesh_max_blocks = mb;
+  return 0;
+}
+
 extern "C" int nmc_get_counters(NmcModel* m, int* out, nmc_stream_t stream) {
   if (!m || !out) return fail("nmc_get_counters: null argument");
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);

@@ -58,6 +58,11 @@ struct StageBuffers {
 struct NmcModelImpl {
   NmcModelDesc d;
   int device;
+  // collide launch shape (nmc_model_set_launch): segments (one wave each) per workgroup of the small-segment
+  // broad phase, grid caps of the grid-stride narrow and mesh kernels
+  int segment_waves = 4;
+  int narrow_max_blocks = 2048;
+  int mesh_max_blocks = 2048;
   // collide scratch
   StageBuffers stage;
   long long* seg_pair_off;  // [segments] first pair entry of each segment's region

@@ -89,3 +89,43 @@ def test_tiered_launch_bit_identical_uneven(cuda_device):
     res = tune_block_lanes(solver, s0, s1, c, DT, reps=2)
     print(res, flush=True)
     assert res["block_lanes"] in (0, 256)
+
+
+@pytest.mark.parametrize("name,make", MODELS, ids=[m[0] for m in MODELS])
+def test_collide_launch_shapes_bit_identical(cuda_device, name, make):
+    """Every collide launch shape the tuner tries writes the same contacts, bit for bit; the tuner keeps one
+    of its candidates and leaves it set on the pipeline."""
+    from newton_custom_contact_amd.autotune import collide_candidates, tune_collide
+
+    print(f"[collide launch] {name}", flush=True)
+    model = make(cuda_device)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    solver = SolverXPBD(model, iterations=10)
+    c = pipe.contacts()
+    s0, s1 = model.state(), model.state()
+    for _ in range(60):
+        pipe.collide(s0, c)
+        solver.step(s0, s1, None, c, DT)
+        s0, s1 = s1, s0
+    pipe.collide(s0, c)
+    ref = c.numpy()
+    assert ref["count"] > 0
+    cand = collide_candidates(pipe)
+    assert ("mesh_max_blocks" in cand) == (name == "mesh_4w")
+    for key, values in cand.items():
+        for v in values:
+            pipe.set_launch(**{key: v})
+            pipe.collide(s0, c)
+            got = c.numpy()
+            assert got["count"] == ref["count"], (key, v)
+            for k in ("shape0", "shape1", "point0", "point1", "normal", "sort_key", "margin0", "margin1"):
+                assert np.array_equal(got[k], ref[k]), (key, v, k)
+        pipe.set_launch(**{key: values[-1]})
+    with pytest.raises(ValueError):
+        pipe.set_launch(segment_waves=3)
+    with pytest.raises(ValueError):
+        pipe.set_launch(narrow_max_blocks=0)
+    res = tune_collide(pipe, s0, c, reps=2)
+    print(name, res, flush=True)
+    for key, values in cand.items():
+        assert res["launch"][key] in values and pipe.launch_config[key] == res["launch"][key]
