@@ -24,7 +24,8 @@ the box's CPU share.
 network, warm launch with ``--warm-iterations`` iterations and the per-world residual check) and
 reports its penetration residual beside the plain solver's at the same state.
 ``sdf`` times the triangle-mesh workload (1024 worlds x 8 convex-hull meshes with 64^3 SDFs, mesh-mesh and
-mesh-plane contacts, the reference's contact reduction) with the same step; ``uneven`` the uneven-worlds workload
+mesh-plane contacts, the reference's contact reduction) with the same step, ``sdf_unique_hulls`` the same with a
+seeded hull of its own for every body (8192 hulls and SDFs); ``uneven`` the uneven-worlds workload
 (2048 worlds of 8-256 primitives plus a 6-link revolute chain each; its scene is built on the host in a separate
 process while the other forms run); ``single_world`` the one-world 64-box case.
 """
@@ -65,6 +66,8 @@ def parse():
     p.add_argument("--residual-tol", type=float, default=5e-3,
                    help="per-world penetration residual above which the learned form falls back to a cold solve")
     p.add_argument("--no-sdf", action="store_true", help="skip the SDF (triangle-mesh) workload")
+    p.add_argument("--no-sdf-unique", action="store_true",
+                   help="skip the SDF workload's form with a unique hull per body")
     p.add_argument("--no-uneven", action="store_true",
                    help="skip the uneven-worlds workload (log-uniform 8-256 bodies per world + revolute chains)")
     p.add_argument("--no-single", action="store_true", help="skip the one-world 64-box case")
@@ -320,8 +323,11 @@ def main():
         single = single_world_form(args, dev)
 
     sdf = None
+    sdf_unique = None
     if not args.no_sdf:
         sdf = sdf_form(args, dev)
+        if not args.no_sdf_unique:
+            sdf_unique = sdf_form(args, dev, unique=True)
 
     uneven = None
     if uneven_job is not None:
@@ -368,6 +374,7 @@ def main():
         "learned": learned,
         "single_world": single,
         "sdf": sdf,
+        "sdf_unique_hulls": sdf_unique,
         "uneven": uneven,
         "components": components,
     }
@@ -715,12 +722,13 @@ def uneven_form(args, dev, job):
     }
 
 
-def sdf_form(args, dev):
+def sdf_form(args, dev, unique: bool = False):
     """The triangle-mesh workload: ``--sdf-worlds`` worlds x 8 convex-hull meshes (64-256 points on a sphere
-    of radius 0.1-0.3 m, from a seeded library of 64 hulls, 64^3 FP32 SDF each) dropped on a ground plane,
-    mesh-mesh and mesh-plane contacts with the reference's contact reduction (at most 240 per pair), 4 substeps,
-    10 iterations, hipGraph step; the collide and step kernels' launch shapes autotuned on the settled state.  Timed after
-    ``--sdf-settle`` untimed steps, when the hulls have landed and pile up."""
+    of radius 0.1-0.3 m, 64^3 FP32 texture SDF each) dropped on a ground plane, mesh-mesh and mesh-plane contacts
+    with the reference's contact reduction (at most 240 per pair), 4 substeps, 10 iterations, hipGraph step; the
+    collide and step kernels' launch shapes autotuned on the settled state.  The hulls come from a seeded
+    library of 64, or with ``unique`` every body has its own seeded hull (8192 hulls and SDFs for 1024 worlds).
+    Timed after ``--sdf-settle`` untimed steps, when the hulls have landed and pile up."""
     import torch
 
     from newton_custom_contact_amd import CollisionPipeline, scenes
@@ -728,14 +736,21 @@ def sdf_form(args, dev):
     from newton_custom_contact_amd.solvers import SolverXPBD
 
     t0 = time.time()
-    lib = scenes.mesh_library(64, seed=args.seed)
-    for mm in lib:
-        mm.build_sdf(max_resolution=64, device=dev, texture_format="float32")
+    if unique:
+        lib = scenes.mesh_library(args.sdf_worlds * 8, seed=args.seed)
+    else:
+        lib = scenes.mesh_library(64, seed=args.seed)
+    t_hull = time.time() - t0
+    from newton_custom_contact_amd.mesh import cook_texture_sdfs
+
+    cook_texture_sdfs(lib, dev, max_resolution=64, texture_format="float32")
     torch.cuda.synchronize()
-    t_cook = time.time() - t0
-    model = scenes.mesh_worlds(args.sdf_worlds, seed=args.seed, device=dev, library=lib)
-    print(f"[bench] sdf scene: {args.sdf_worlds} worlds, {model.body_count} mesh bodies, 64 hulls "
-          f"(library + cook {t_cook:.1f}s, total {time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
+    t_cook = time.time() - t0 - t_hull
+    model = scenes.mesh_worlds(args.sdf_worlds, seed=args.seed, device=dev, library=lib, unique=unique)
+    sdf_bytes = sum(m.sdf.coarse.numel() * 4 + m.sdf.slots.numel() * 4 + m.sdf.texels.numel() for m in lib)
+    print(f"[bench] sdf scene: {args.sdf_worlds} worlds, {model.body_count} mesh bodies, {len(lib)} hulls "
+          f"(hulls {t_hull:.1f}s, SDF cook {t_cook:.1f}s, {sdf_bytes / 2**30:.2f} GiB of SDF textures, total "
+          f"{time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=args.iterations)
     tune = lambda a, b, c, sdt: autotune.tune(pipe, solver, a, b, c, sdt)  # noqa: E731
@@ -748,14 +763,19 @@ def sdf_form(args, dev):
     b_ws = args.substeps * (b_collide + b_step)
     roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
     oc = pipe.overflow_counters()
-    print(f"[bench] sdf: {ws:.1f} world-steps/s, N_c={n_c:.1f}", file=sys.stderr, flush=True)
+    print(f"[bench] sdf{' unique hulls' if unique else ''}: {ws:.1f} world-steps/s, N_c={n_c:.1f}", file=sys.stderr,
+          flush=True)
+    hulls = f"{len(lib)} unique hulls, one per body" if unique else "64-hull library"
     return {
         "value": round(ws, 1),
         "unit": "world-steps/s",
         "ms_per_step": round(ms / args.sdf_steps, 4),
-        "workload": f"{args.sdf_worlds} worlds x 8 convex-hull meshes (64-hull library, 64^3 fp32 SDF) on a ground "
-                    f"plane, reference contact reduction, dt=1/60 s, {args.substeps} substeps, {args.iterations} "
-                    f"iterations, timed after {args.sdf_settle} untimed settle steps",
+        "workload": f"{args.sdf_worlds} worlds x 8 convex-hull meshes ({hulls}, 64^3 fp32 texture SDF, "
+                    f"{sdf_bytes / 2**30:.2f} GiB of SDFs) on a ground plane, reference contact reduction, dt=1/60 s, "
+                    f"{args.substeps} substeps, {args.iterations} iterations, timed after {args.sdf_settle} untimed "
+                    f"settle steps",
+        "hulls": len(lib),
+        "sdf_texture_bytes": int(sdf_bytes),
         "mean_contacts_per_world_substep": round(n_c, 2),
         "bytes_per_world_step": round(b_ws, 1),
         "hbm_roofline_world_steps_per_s": round(roof_ws, 1),

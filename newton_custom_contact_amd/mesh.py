@@ -104,12 +104,14 @@ class SDF:
     """
 
     def __init__(self, sparse: dict, device, *, scale_baked: bool = False, shape_margin: float = 0.0,
-                 narrow_band_range=(-0.1, 0.1), texture_format: str = "uint16", device_arrays=None):
+                 narrow_band_range=(-0.1, 0.1), texture_format: str = "uint16", device_arrays=None,
+                 device_flags=None):
         self.scale_baked = bool(scale_baked)
         self.shape_margin = float(shape_margin)
         self.narrow_band_range = (float(narrow_band_range[0]), float(narrow_band_range[1]))
         self.texture_format = texture_format
         self._sparse = sparse
+        self._flags = device_flags  # occupied | linear << 1 | required << 2 per subgrid, until sparse_data()
         w, h, d = (int(x) for x in sparse["coarse_dims"])
         self.coarse_dims = (w, h, d)
         self.subgrid_size = int(sparse["subgrid_size"])
@@ -147,6 +149,11 @@ class SDF:
             sp["coarse_sdf"] = self.coarse.cpu().numpy()
             sp["subgrid_start_slots"] = self.slots.cpu().numpy().view(np.uint32)
             sp["subgrid_data"] = self.texels.cpu().numpy().view(dt).reshape(ts, ts, ts)
+            if self._flags is not None:
+                fl = self._flags.cpu().numpy()
+                sp["subgrid_required"] = ((fl >> 2) & 1).astype(np.int32)
+                sp["subgrid_occupied"] = (fl & 1).astype(np.int32)
+                self._flags = None
             self._sparse = sp
         return self._sparse
 
@@ -363,42 +370,44 @@ class Mesh:
         return self.sdf
 
 
-def cook_texture_sdf(vertices, indices, geo: dict, texture_format: str, winding_threshold: float, device, *,
-                     scale_baked: bool = False, shape_margin: float = 0.0, narrow_band_range=(-0.1, 0.1)) -> SDF:
-    """The GPU cook of ``build_sparse_sdf_from_mesh`` (sdf_texture.py:959-1308) for float32 ``vertices`` and
-    the grid geometry ``geo`` of :func:`texture_sdf_geometry`: ``nmc_sdf_build`` (padded fine grid),
-    ``nmc_sdf_subgrids`` (coarse grid, occupancy, linearity, addresses), the count of required subgrids read
-    back to size the texture, ``nmc_sdf_pack`` (slots and texels)."""
-    from ._native import check, load, ptr, stream_handle
+def _cook_start(L, vertices, indices, geo: dict, winding_threshold: float, dev, st, count_slot: torch.Tensor) -> dict:
+    """Queue the first two cook calls of one mesh (no host synchronisation): the fine grid, then the coarse
+    grid, flags and addresses; the required count lands in ``count_slot`` (one device int)."""
+    from ._native import check, ptr
 
-    L = load()
-    dev = torch.device(device)
-    st = ctypes.c_void_p(stream_handle(dev))
     w, h, d = geo["coarse_dims"]
     nx, ny, nz = (SUBGRID_SIZE * n + 1 for n in (w, h, d))
     if nx * ny * nz >= 2 ** 31:
         raise ValueError(f"SDF grid of {nx} x {ny} x {nz} vertices is too large")
     lower = np.asarray(geo["min_extents"], np.float32)
     cell = np.asarray(geo["cell_size"], np.float64).astype(np.float32)
-    vt = torch.from_numpy(np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)).to(dev)
-    tr = torch.from_numpy(np.ascontiguousarray(indices, np.int32).reshape(-1)).to(dev)
+    vt = torch.from_numpy(np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)).to(dev, non_blocking=True)
+    tr = torch.from_numpy(np.ascontiguousarray(indices, np.int32).reshape(-1)).to(dev, non_blocking=True)
     fine = torch.empty((nz, ny, nx), dtype=torch.float32, device=dev)
     check(L.nmc_sdf_build(ctypes.c_void_p(ptr(vt)), int(vt.shape[0]), ctypes.c_void_p(ptr(tr)), int(tr.numel() // 3),
                           (ctypes.c_float * 3)(*lower.tolist()), (ctypes.c_float * 3)(*cell.tolist()),
                           (ctypes.c_int * 3)(nx, ny, nz), ctypes.c_float(winding_threshold),
                           ctypes.c_void_p(ptr(fine)), st), "nmc_sdf_build")
     n = w * h * d
-    coarse = torch.empty((d + 1, h + 1, w + 1), dtype=torch.float32, device=dev)
-    flags = torch.empty(n, dtype=torch.int32, device=dev)
-    address = torch.empty(n, dtype=torch.int32, device=dev)
-    count = torch.zeros(1, dtype=torch.int32, device=dev)
-    dims = (ctypes.c_int * 3)(w, h, d)
+    job = {"geo": geo, "lower": lower, "fine": fine, "vt": vt, "tr": tr,
+           "coarse": torch.empty((d + 1, h + 1, w + 1), dtype=torch.float32, device=dev),
+           "flags": torch.empty(n, dtype=torch.int32, device=dev),
+           "address": torch.empty(n, dtype=torch.int32, device=dev)}
     lo, hi = geo["band"]
-    check(L.nmc_sdf_subgrids(ctypes.c_void_p(ptr(fine)), dims, ctypes.c_float(float(lo)), ctypes.c_float(float(hi)),
-                             ctypes.c_float(geo["linear_threshold"]), ctypes.c_void_p(ptr(coarse)),
-                             ctypes.c_void_p(ptr(flags)), ctypes.c_void_p(ptr(address)), ctypes.c_void_p(ptr(count)),
-                             st), "nmc_sdf_subgrids")
-    num_required = int(count.item())
+    check(L.nmc_sdf_subgrids(ctypes.c_void_p(ptr(fine)), (ctypes.c_int * 3)(w, h, d), ctypes.c_float(float(lo)),
+                             ctypes.c_float(float(hi)), ctypes.c_float(geo["linear_threshold"]),
+                             ctypes.c_void_p(ptr(job["coarse"])), ctypes.c_void_p(ptr(job["flags"])),
+                             ctypes.c_void_p(ptr(job["address"])), ctypes.c_void_p(ptr(count_slot)), st),
+          "nmc_sdf_subgrids")
+    return job
+
+
+def _cook_finish(L, job: dict, num_required: int, texture_format: str, dev, st, **meta) -> SDF:
+    """Size the texture from the required count and queue the pack call (sdf_texture.py:1163-1288)."""
+    from ._native import check, ptr
+
+    geo = job["geo"]
+    w, h, d = geo["coarse_dims"]
     qmode = QUANTIZATION_MODES[texture_format]
     slots = torch.empty((w, h, d), dtype=torch.int32, device=dev)
     if num_required == 0:
@@ -409,26 +418,73 @@ def cook_texture_sdf(vertices, indices, geo: dict, texture_format: str, winding_
         blocks = texture_blocks_per_axis(num_required)
         ts = 9 * blocks
         tbytes = qmode
-        quantized = qmode != 4
-        qmin, qrange = (geo["sdf_min"], geo["sdf_range"]) if quantized else (0.0, 1.0)
+        qmin, qrange = (geo["sdf_min"], geo["sdf_range"]) if qmode != 4 else (0.0, 1.0)
         texels = torch.zeros(ts ** 3 * tbytes, dtype=torch.uint8, device=dev)
-    qinv = 1.0 / geo["sdf_range"]
-    check(L.nmc_sdf_pack(ctypes.c_void_p(ptr(fine)), dims, ctypes.c_void_p(ptr(flags)), ctypes.c_void_p(ptr(address)),
-                         int(blocks), int(tbytes), ctypes.c_float(qmin if tbytes != 4 else 0.0), ctypes.c_float(qinv),
-                         ctypes.c_void_p(ptr(slots)), ctypes.c_void_p(ptr(texels)), st), "nmc_sdf_pack")
+    check(L.nmc_sdf_pack(ctypes.c_void_p(ptr(job["fine"])), (ctypes.c_int * 3)(w, h, d),
+                         ctypes.c_void_p(ptr(job["flags"])), ctypes.c_void_p(ptr(job["address"])), int(blocks),
+                         int(tbytes), ctypes.c_float(qmin if tbytes != 4 else 0.0),
+                         ctypes.c_float(1.0 / geo["sdf_range"]), ctypes.c_void_p(ptr(slots)),
+                         ctypes.c_void_p(ptr(texels)), st), "nmc_sdf_pack")
     if num_required == 0:
         texels = texels[:4]
-    fl = flags.cpu().numpy()
     sparse = {
         "coarse_sdf": None, "subgrid_data": None, "subgrid_start_slots": None,
         "coarse_dims": (w, h, d), "subgrid_tex_size": ts, "num_subgrids": num_required,
-        "min_extents": lower, "max_extents": geo["padded_max"], "cell_size": np.asarray(geo["cell_size"], np.float64),
-        "subgrid_size": SUBGRID_SIZE, "quantization_mode": qmode, "subgrids_min_sdf_value": qmin,
-        "subgrids_sdf_value_range": qrange,
-        "subgrid_required": ((fl >> 2) & 1).astype(np.int32), "subgrid_occupied": (fl & 1).astype(np.int32),
+        "min_extents": job["lower"], "max_extents": geo["padded_max"],
+        "cell_size": np.asarray(geo["cell_size"], np.float64), "subgrid_size": SUBGRID_SIZE,
+        "quantization_mode": qmode, "subgrids_min_sdf_value": qmin, "subgrids_sdf_value_range": qrange,
+        "subgrid_required": None, "subgrid_occupied": None,
     }
-    return SDF(sparse, dev, scale_baked=scale_baked, shape_margin=shape_margin, narrow_band_range=narrow_band_range,
-               texture_format=texture_format, device_arrays=(coarse, slots, texels, tbytes))
+    return SDF(sparse, dev, texture_format=texture_format, device_arrays=(job["coarse"], slots, texels, tbytes),
+               device_flags=job["flags"], **meta)
+
+
+def cook_texture_sdf(vertices, indices, geo: dict, texture_format: str, winding_threshold: float, device, *,
+                     scale_baked: bool = False, shape_margin: float = 0.0, narrow_band_range=(-0.1, 0.1)) -> SDF:
+    """The GPU cook of ``build_sparse_sdf_from_mesh`` (sdf_texture.py:959-1308) for float32 ``vertices`` and
+    the grid geometry ``geo`` of :func:`texture_sdf_geometry`: ``nmc_sdf_build`` (padded fine grid),
+    ``nmc_sdf_subgrids`` (coarse grid, occupancy, linearity, addresses), the count of required subgrids read
+    back to size the texture, ``nmc_sdf_pack`` (slots and texels)."""
+    from ._native import load, stream_handle
+
+    L = load()
+    dev = torch.device(device)
+    st = ctypes.c_void_p(stream_handle(dev))
+    count = torch.zeros(1, dtype=torch.int32, device=dev)
+    job = _cook_start(L, vertices, indices, geo, winding_threshold, dev, st, count)
+    return _cook_finish(L, job, int(count.item()), texture_format, dev, st, scale_baked=scale_baked,
+                        shape_margin=shape_margin, narrow_band_range=narrow_band_range)
+
+
+def cook_texture_sdfs(meshes, device, *, max_resolution: int = 64, texture_format: str = "uint16",
+                      margin: float = 0.05, narrow_band_range=(-0.1, 0.1), chunk: int = 512) -> None:
+    """``Mesh.build_sdf`` for many meshes at once (no scale, no cache): the first two cook calls of up to
+    ``chunk`` meshes are queued back to back, their required counts read back in one transfer, then the pack
+    calls queued; the same arrays as cooking them one by one."""
+    from ._native import load, stream_handle
+
+    L = load()
+    dev = torch.device(device)
+    if dev.type != "cuda":
+        raise RuntimeError("cook_texture_sdfs runs on the GPU only")
+    st = ctypes.c_void_p(stream_handle(dev))
+    band = tuple(float(x) for x in narrow_band_range)
+    todo = [m for m in meshes if m.sdf is None]
+    for c0 in range(0, len(todo), chunk):
+        part = todo[c0:c0 + chunk]
+        counts = torch.zeros(len(part), dtype=torch.int32, device=dev)
+        jobs = []
+        for k, m in enumerate(part):
+            verts = np.ascontiguousarray(m.vertices, np.float32)
+            geo = texture_sdf_geometry(verts, margin=margin, narrow_band_range=band, max_resolution=max_resolution)
+            if geo is None:
+                raise ValueError("cook_texture_sdfs: a mesh has no extent")
+            thr = 0.5 if m.signed_volume() >= 0.0 else -0.5
+            jobs.append(_cook_start(L, verts, m.indices, geo, thr, dev, st, counts[k:k + 1]))
+        nreq = counts.cpu().numpy()
+        for m, job, n in zip(part, jobs, nreq):
+            m.sdf = _cook_finish(L, job, int(n), texture_format, dev, st, narrow_band_range=band)
+            job.clear()
 
 
 def convex_hull_mesh(points) -> Mesh:
@@ -443,10 +499,9 @@ def convex_hull_mesh(points) -> Mesh:
     verts = pts[used]
     tris = remap[hull.simplices]
     center = verts.mean(0)
-    for t in tris:  # orient every face outward
-        a, b, c = verts[t]
-        if np.dot(np.cross(b - a, c - a), a - center) < 0:
-            t[1], t[2] = t[2], t[1]
+    a, b, c = verts[tris[:, 0]], verts[tris[:, 1]], verts[tris[:, 2]]
+    inward = np.einsum("ij,ij->i", np.cross(b - a, c - a), a - center) < 0
+    tris[inward] = tris[inward][:, [0, 2, 1]]  # orient every face outward
     return Mesh(verts.astype(np.float32), tris.astype(np.int32).reshape(-1))
 
 
@@ -460,4 +515,5 @@ def random_convex_hull(rng: np.random.Generator, n_min: int = 64, n_max: int = 2
     return convex_hull_mesh(d * r)
 
 
-__all__ = ["Mesh", "SDF", "convex_hull_mesh", "cook_texture_sdf", "random_convex_hull", "texture_sdf_geometry"]
+__all__ = ["Mesh", "SDF", "convex_hull_mesh", "cook_texture_sdf", "cook_texture_sdfs", "random_convex_hull",
+           "texture_sdf_geometry"]

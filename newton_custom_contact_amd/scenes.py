@@ -286,20 +286,24 @@ def mesh_library(count: int = 64, seed: int = 0, n_min: int = 64, n_max: int = 2
 
 
 def mesh_worlds(world_count: int = 1024, seed: int = 0, mu: float = 0.5, device=None, bodies: int = 8,
-                library=None, library_size: int = 64, sdf_resolution: int = 64, texture_format: str = "float32"):
+                library=None, library_size: int = 64, sdf_resolution: int = 64, texture_format: str = "float32",
+                unique: bool = False):
     """``world_count`` worlds of ``bodies`` convex-hull mesh bodies above an infinite ground plane.
 
     Each body's mesh is drawn from ``library`` (default: ``mesh_library(library_size, seed)``); worlds
-    share the library's meshes and SDFs.  On a GPU device the SDFs are cooked at ``sdf_resolution``
-    cells along each mesh's longest axis, with ``texture_format`` texels (float32: the 64^3 FP32 voxel SDF of
-    the 1024-world hull workload), before the model is finalized.
+    share the library's meshes and SDFs.  ``unique=True`` gives every body a seeded hull of its own instead
+    (``mesh_library(world_count * bodies, seed)``, body i takes hull i).  On a GPU device the SDFs are cooked
+    at ``sdf_resolution`` cells along each mesh's longest axis, with ``texture_format`` texels (float32: the
+    64^3 FP32 voxel SDF of the 1024-world hull workload), before the model is finalized.
     """
+    if unique and library is None:
+        library = mesh_library(world_count * bodies, seed)
     lib = mesh_library(library_size, seed) if library is None else library
     dev = device if device is not None else "cpu"
     if str(dev).startswith("cuda"):
-        for mm in lib:
-            if mm.sdf is None:
-                mm.build_sdf(max_resolution=sdf_resolution, device=dev, texture_format=texture_format)
+        from .mesh import cook_texture_sdfs  # noqa: PLC0415
+
+        cook_texture_sdfs(lib, dev, max_resolution=sdf_resolution, texture_format=texture_format)
     rng = np.random.Generator(np.random.PCG64(seed + 1))
     b = ModelBuilder()
     cfg = _cfg(mu)
@@ -309,6 +313,8 @@ def mesh_worlds(world_count: int = 1024, seed: int = 0, mu: float = 0.5, device=
         b.begin_world()
         placer = _Placer(rng)
         pick = rng.integers(0, len(lib), size=bodies)
+        if unique:
+            pick = np.arange(_w * bodies, (_w + 1) * bodies)
         for k in pick:
             p = placer.place(radius[k])
             body = b.add_body(xform=transform(p, shoemake_quat(rng)))

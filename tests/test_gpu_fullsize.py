@@ -121,19 +121,20 @@ def test_fullsize_mixed_parity_and_layout(cuda_device):
                                                     qd_ref, q64, qd64, DT, "mixed"), flush=True)
 
 
-@pytest.mark.parametrize("reduce", [True, False], ids=["reduced", "unreduced"])
-def test_fullsize_sdf_parity(cuda_device, reduce):
-    """The SDF workload at the benchmark's size: 1024 worlds x 8 convex-hull meshes (64-hull library, 64^3
-    SDFs), advanced 240 substeps (60 frames) on the GPU until the hulls pile up, with the reference's contact
-    reduction (deterministic packing) and without it.  At that state the GPU's mesh-mesh and mesh-plane
-    contacts equal the oracle's (count, keys, pairs exactly; float fields within 1e-5 relative, bit-exactness
-    reported), and one step from that state agrees bit for bit (reference-order kernel) and to the full-size
-    step bar (world-frame kernel; see the module docstring)."""
-    print(f"[full size sdf] 64-hull library, 64^3 SDFs, 1024 worlds, reduce_contacts={reduce}", flush=True)
-    lib = scenes.mesh_library(64, seed=0)
-    for mm in lib:
-        mm.build_sdf(max_resolution=64, device=cuda_device, texture_format="float32")
-    model = scenes.mesh_worlds(1024, seed=0, device=cuda_device, library=lib)
+@pytest.mark.parametrize("reduce,unique", [(True, False), (False, False), (True, True)],
+                         ids=["reduced", "unreduced", "unique_hulls"])
+def test_fullsize_sdf_parity(cuda_device, reduce, unique):
+    """The SDF workload at the benchmark's size: 1024 worlds x 8 convex-hull meshes (64-hull library, or a
+    hull of its own for every body: 8192 hulls; 64^3 float32 texture SDFs), advanced 240 substeps (60 frames) on
+    the GPU until the hulls pile up, with the reference's contact reduction (deterministic packing) and without
+    it.  At that state the GPU's mesh-mesh and mesh-plane contacts equal the oracle's (count, keys, pairs
+    exactly; float fields within 1e-5 relative, bit-exactness reported), and one step from that state agrees bit
+    for bit (reference-order kernel) and to the full-size step bar (world-frame kernel; see the module
+    docstring)."""
+    print(f"[full size sdf] {'8192 unique hulls' if unique else '64-hull library'}, 64^3 SDFs, 1024 worlds, "
+          f"reduce_contacts={reduce}", flush=True)
+    lib = scenes.mesh_library(1024 * 8 if unique else 64, seed=0)
+    model = scenes.mesh_worlds(1024, seed=0, device=cuda_device, library=lib, unique=unique)
     # unreduced, pairs keep every contact (about 7 000 a world once piled): a capacity beyond the estimate
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True, reduce_contacts=reduce,
                              rigid_contact_max=None if reduce else 12_000_000)
