@@ -63,7 +63,7 @@ def parse():
     p.add_argument("--kernel-reps", type=int, default=20)
     p.add_argument("--no-learned", action="store_true", help="skip the learned warm-start form")
     p.add_argument("--warm-iterations", type=int, default=3)
-    p.add_argument("--residual-tol", type=float, default=5e-3,
+    p.add_argument("--residual-tol", type=float, default=None,
                    help="per-world penetration residual above which the learned form falls back to a cold solve")
     p.add_argument("--no-sdf", action="store_true", help="skip the SDF (triangle-mesh) workload")
     p.add_argument("--no-sdf-unique", action="store_true",
@@ -385,7 +385,8 @@ def iterations_form(args, model, state, dev):
     """Iterations to a penetration residual at the settled state.  Residual of one substep = mean over worlds
     of the world's largest remaining penetration max(0, -d) after the solve, on the same state and contacts for
     every iteration count: plain 1..iters_max iterations, learned warm start 1..10 warm iterations (no fallback,
-    the trained network)."""
+    the trained network), and the warm start fed the exact total multipliers of the plain 10-iteration step with
+    0..10 iterations after it (what a perfect network would reach)."""
     import torch
 
     from newton_custom_contact_amd import CollisionPipeline
@@ -418,6 +419,14 @@ def iterations_form(args, model, state, dev):
     for k in range(1, min(10, kmax) + 1):
         solver.step(a, tmp, None, c, sdt, warm_iterations=k, residual_tol=float("inf"), remember=False)
         learned.append(float(solver.buffers["residual"][:W].mean()))
+    # the method's ceiling: the warm start fed the exact total multipliers of the plain 10-iteration step
+    solver.log_step(a, tmp, None, c, sdt, iterations=args.iterations, remember=False)
+    exact = solver.buffers["lambda_sum"].clone()
+    ceiling = []
+    for k in range(0, min(10, kmax) + 1):
+        solver.step(a, tmp, None, c, sdt, warm_iterations=k, residual_tol=float("inf"), remember=False,
+                    multipliers=exact)
+        ceiling.append(float(solver.buffers["residual"][:W].mean()))
     torch.cuda.synchronize()
     p10 = plain[args.iterations - 1]
 
@@ -429,10 +438,13 @@ def iterations_form(args, model, state, dev):
 
     out = {
         "state": f"settled plain workload, {W} worlds, one substep of dt {sdt:.6f} s from the same state and contacts",
-        "residual_m_by_iterations": {"plain": plain, "learned_warm_start": learned},
+        "residual_m_by_iterations": {"plain": plain, "learned_warm_start": learned,
+                                     "exact_multiplier_warm_start_from_0": ceiling},
         "plain_10_iteration_residual_m": p10,
-        "iters_to_1e-4_m": {"plain": first_below(plain, 1e-4), "learned": first_below(learned, 1e-4)},
-        "iters_to_plain10_residual": {"plain": first_below(plain, p10), "learned": first_below(learned, p10)},
+        "iters_to_1e-4_m": {"plain": first_below(plain, 1e-4), "learned": first_below(learned, 1e-4),
+                            "exact_multipliers": first_below(ceiling[1:], 1e-4)},
+        "iters_to_plain10_residual": {"plain": first_below(plain, p10), "learned": first_below(learned, p10),
+                                      "exact_multipliers": first_below(ceiling[1:], p10)},
     }
     print(f"[bench] iterations: {json.dumps(out)}", file=sys.stderr, flush=True)
     return out
@@ -469,8 +481,9 @@ def learned_form(args, model, dev, plain_ws_per_s):
         return None
     mlp = WarmStartMLP.load(WEIGHTS_PATH, device=dev)
     pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    tol = args.residual_tol
     solver = SolverXPBDWarmStart(model, mlp, iterations=args.iterations, warm_iterations=args.warm_iterations,
-                                 residual_tol=args.residual_tol)
+                                 residual_tol=float("inf") if tol is None else tol)
     # two Contacts buffers alternate, so the previous substep's contacts stay readable for the matching
     cbufs = [pipe.contacts(), pipe.contacts()]
     states = [model.state(), model.state()]
@@ -494,19 +507,38 @@ def learned_form(args, model, dev, plain_ws_per_s):
             states[0].body_q.copy_(states[1].body_q)
             states[0].body_qd.copy_(states[1].body_qd)
 
-    side = torch.cuda.Stream()
-    side.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.stream(side):
-        step()
-    torch.cuda.current_stream().wait_stream(side)
-    torch.cuda.synchronize()
-    graph = None
-    if not args.no_graph:
-        graph = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(graph):
+    def capture():
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            step()
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        if args.no_graph:
+            return None
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
             step()
         torch.cuda.synchronize()
-    for _ in range(args.settle + args.warmup):
+        return g
+
+    graph = capture()
+    for _ in range(args.settle):
+        graph.replay() if graph is not None else step()
+    torch.cuda.synchronize()
+    tol_rule = "given (--residual-tol)"
+    if tol is None:
+        # the fallback tolerance tied to the plain solver: twice the mean over worlds of the plain 10-iteration
+        # residual at the settled state (the learned run's own state and contacts)
+        a, tmp = states[0], model.state()
+        pipe.collide(a, cbufs[0])
+        solver.log_step(a, tmp, None, cbufs[0], sdt, remember=False)
+        tol = 2.0 * float(solver.buffers["residual"][:W].mean())
+        solver.residual_tol = tol
+        tol_rule = "2 x the plain 10-iteration residual at the settled state"
+        del graph
+        graph = capture()
+    for _ in range(args.warmup):
         graph.replay() if graph is not None else step()
     torch.cuda.synchronize()
     res_acc.zero_()
@@ -548,7 +580,8 @@ def learned_form(args, model, dev, plain_ws_per_s):
         "speedup_vs_plain": round(ws / plain_ws_per_s, 3),
         "warm_iterations": args.warm_iterations,
         "fallback_iterations": args.iterations,
-        "residual_tol_m": args.residual_tol,
+        "residual_tol_m": tol,
+        "residual_tol_rule": tol_rule,
         "fallback_world_fraction": round(fb_frac, 4),
         "mean_world_residual_m": warm_res,
         "end_state_residual_m": {"learned": warm_here, f"plain_{args.warm_iterations}": plain_k,

@@ -284,7 +284,11 @@ class SolverXPBDWarmStart(SolverXPBD):
             raise ValueError("the warm-started solver needs contacts from CollisionPipeline(deterministic=True)")
 
     def step(self, state_in, state_out, control, contacts, dt: float, warm_iterations: int | None = None,
-             residual_tol: float | None = None, remember: bool = True) -> None:
+             residual_tol: float | None = None, remember: bool = True, multipliers: torch.Tensor | None = None) -> None:
+        """Prep launch, network, warm launch.  ``multipliers`` (one per contact slot, in the buffer's order)
+        replaces the network's prediction: with the total normal multipliers a plain step logged on the same
+        state and contacts (``log_step``, ``buffers['lambda_sum']``) it measures what the warm start can reach
+        with a perfect prediction."""
         if state_in.body_q is state_out.body_q:
             raise ValueError("state_in and state_out must be different State objects")
         self._check_contacts(contacts)
@@ -293,7 +297,10 @@ class SolverXPBDWarmStart(SolverXPBD):
         if not 0 <= k <= self.iterations:
             raise ValueError("warm_iterations must be in [0, iterations]")
         self._launch(state_in, None, control, contacts, dt, self._io(NMC_STEP_PREP, b, contacts))
-        self.mlp.predict(b["feat"], contacts.rigid_contact_count, contacts.rigid_contact_max, b["lambda0"])
+        if multipliers is None:
+            self.mlp.predict(b["feat"], contacts.rigid_contact_count, contacts.rigid_contact_max, b["lambda0"])
+        else:
+            b["lambda0"].copy_(multipliers[: b["lambda0"].numel()])
         self._launch(state_in, state_out, control, contacts, dt,
                      self._io(NMC_STEP_WARM, b, contacts, k, residual_tol))
         self._last_remembered = remember

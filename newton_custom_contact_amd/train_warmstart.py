@@ -3,7 +3,8 @@
     python -m newton_custom_contact_amd.train_warmstart --out newton_custom_contact_amd/data/warmstart_mlp.npz
 
 1. Log: plain XPBD steps (10 iterations) of seeded mixed-primitive scenes (the 4096-world workload's
-   generator, training seeds), recording every contact's 16 inputs and total normal multiplier.
+   generator, training seeds), recording every contact's 16 inputs and total normal multiplier (contacts
+   with a non-finite input or multiplier, from a world whose body diverged, are left out).
 2. Fit: Adam on the library's forward/backward kernels.
 3. Evaluate on held-out seeds, from the states of a plain trajectory: the per-world penetration
    residual (mean over worlds of the largest max(0, -d)) after k plain iterations (k = 1..10) and after
@@ -38,10 +39,18 @@ def log_contacts(scene_seed: int, worlds: int, steps: int, skip: int, every: int
         if i >= skip * 4 and (i % every) == 0:
             n = min(int(c.rigid_contact_count.item()), c.rigid_contact_max)
             b = solver.buffers
-            out.append((b["feat"][:n].clone(), b["lambda_sum"][:n].clone()))
-            n_rows += n
+            f, lam = b["feat"][:n], b["lambda_sum"][:n]
+            # contacts of a world whose thin box diverged (non-finite state) carry no training signal
+            ok = torch.isfinite(f).all(dim=1) & torch.isfinite(lam)
+            out.append((f[ok].clone(), lam[ok].clone()))
+            n_rows += int(ok.sum())
         s0, s1 = s1, s0
     return n_rows
+
+
+def _finite_mean(r: torch.Tensor) -> float:
+    """Mean of the per-world residuals over the worlds whose state is finite."""
+    return float(r[torch.isfinite(r)].mean())
 
 
 def evaluate(mlp, seeds, worlds: int, steps: int, every: int, dev: str) -> dict:
@@ -67,12 +76,12 @@ def evaluate(mlp, seeds, worlds: int, steps: int, every: int, dev: str) -> dict:
             if i % every == every - 1:
                 for k in plain_k:
                     solver.log_step(s0, tmp, None, c, sdt, iterations=k, remember=False)
-                    acc_p[k].append(float(solver.buffers["residual"][:W].mean()))
+                    acc_p[k].append(_finite_mean(solver.buffers["residual"][:W]))
                 for k in warm_k:
                     solver.step(s0, tmp, None, c, sdt, warm_iterations=k, residual_tol=1e30, remember=False)
-                    acc_w[k].append(float(solver.buffers["residual"][:W].mean()))
+                    acc_w[k].append(_finite_mean(solver.buffers["residual"][:W]))
                 solver.step(s0, tmp, None, c, sdt, remember=False)
-                acc_wd.append(float(solver.buffers["residual"][:W].mean()))
+                acc_wd.append(_finite_mean(solver.buffers["residual"][:W]))
                 fallback.append(float(solver.buffers["fallback"][:W].float().mean()))
             # the trajectory itself: the plain 10-iteration step (which also keeps the matching history)
             solver.log_step(s0, s1, None, c, sdt)
@@ -96,15 +105,15 @@ def main():
     ap.add_argument("--worlds", type=int, default=1024)
     ap.add_argument("--train-seeds", type=int, nargs="+", default=[101, 102, 103, 104, 105, 106, 107, 108])
     ap.add_argument("--eval-seeds", type=int, nargs="+", default=[201, 202])
-    ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--skip", type=int, default=2)
-    ap.add_argument("--every", type=int, default=3)
-    ap.add_argument("--epochs", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=120)
+    ap.add_argument("--skip", type=int, default=4)
+    ap.add_argument("--every", type=int, default=8)
+    ap.add_argument("--epochs", type=int, default=8)
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--eval-worlds", type=int, default=512)
-    ap.add_argument("--eval-steps", type=int, default=30)
-    ap.add_argument("--eval-every", type=int, default=10)
+    ap.add_argument("--eval-steps", type=int, default=120)
+    ap.add_argument("--eval-every", type=int, default=40)
     ap.add_argument("--eval-only", default=None, help="weights file to evaluate instead of training")
     a = ap.parse_args()
     dev = "cuda:0"
