@@ -177,8 +177,13 @@ typedef struct NmcXpbdParams {
   int has_contacts; /* 0: step(..., contacts=None, ...) */
   int block_lanes;  /* launch shape: 0 = automatic (one 64-lane wave a world up to 64 bodies; models with
                      * larger worlds launch by size tier, worlds above 64 bodies on 512 lanes), 64 (worlds
-                     * of up to 64 bodies) or 256 (every world in one launch: 256 lanes up to 64 bodies a
-                     * world, else 512); the result does not depend on it (an autotuner picks per model) */
+                     * of up to 64 bodies), 256 (every world in one launch: 256 lanes up to 64 bodies a
+                     * world, else 512) or 512 (models with worlds above 64 bodies, plain step: one launch of
+                     * 512 lanes over CSR bins -- runs of consecutive worlds packed up to the largest world's
+                     * body count, largest bins first; other modes launch as 0).  The reference-order
+                     * arithmetic gives the same result for every shape; the world-frame arithmetic's per-body
+                     * sums follow the lanes a body gets, so bins (one lane a body) round like the 512-lane
+                     * worlds (an autotuner picks per model) */
   int reference_arithmetic; /* 0 (default): contact rows and body updates in world-frame arithmetic (rotation
                              * matrices, world inverse inertia, FMA, hardware reciprocal), within rounding of
                              * the reference; 1: the reference's operation order, bit-identical to a CPU

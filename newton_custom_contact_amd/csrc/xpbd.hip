@@ -436,6 +436,8 @@ struct XpbdArgs {
   int joint_lds;  // joints per world with LDS delta slots (0 when no world has a non-FREE joint)
   int lds_bodies;  // bodies per world the launch's LDS layout holds
   const int* world_list;  // the launch's worlds (null: workgroup w runs world w)
+  const int* bins;        // (first world, end world) per workgroup: a run of consecutive worlds solved as one
+                          // (their bodies, joints and contacts lie back to back), or null
   unsigned* keys_global;  // ordering buffer of worlds whose entries exceed key_cap (2 per contact slot)
   long long keys_global_cap;
   int* counters;
@@ -795,12 +797,15 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   __shared__ int s_fallback;
   const NmcModelDesc& d = a.d;
   const NmcLearnIO& lrn = a.lrn;
-  // world index; w == world_count owns the global bodies
-  const int w = a.world_list ? a.world_list[blockIdx.x] : static_cast<int>(blockIdx.x);
+  // world index; w == world_count owns the global bodies.  A bin of consecutive worlds w .. we - 1 is solved
+  // as one: no body, joint or contact of one of its worlds refers to another, so the Jacobi iterations over
+  // the bin are those of its worlds side by side.
+  const int w = a.bins ? a.bins[2 * blockIdx.x] : (a.world_list ? a.world_list[blockIdx.x] : static_cast<int>(blockIdx.x));
+  const int we = a.bins ? a.bins[2 * blockIdx.x + 1] : w + 1;
   const int b0 = d.world_body_first[w];
-  const int nb = d.world_body_count[w];
+  const int nb = d.world_body_first[we - 1] + d.world_body_count[we - 1] - b0;
   const int j0 = d.world_joint_first[w];
-  const int nj = d.world_joint_count[w];
+  const int nj = d.world_joint_first[we - 1] + d.world_joint_count[we - 1] - j0;
   const int mb = a.lds_bodies > 0 ? a.lds_bodies : 1;
   const int mj = a.joint_lds;
   // CR contacts are held in registers for the whole step; each iteration solves CH of them at a time (one
@@ -843,9 +848,11 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   int* jactive = reinterpret_cast<int*>(p); p += sizeof(int) * mj;
   unsigned* keys = reinterpret_cast<unsigned*>(p);
 
-  const bool has_c = a.p.has_contacts != 0 && w < d.segment_count;
+  const bool has_c = a.p.has_contacts != 0 && we - 1 < d.segment_count;
   const int cstart = has_c ? a.c.world_start[w] : 0;
-  int nc = has_c ? a.c.world_count[w] : 0;
+  int nc = 0;
+  if (has_c)
+    for (int v = w; v < we; ++v) nc += a.c.world_count[v];  // the canonical layout puts worlds back to back
   // World-frame kernels, one body per lane: the per-body (contact, side) lists come from two bit masks per
   // body over the world's contacts (the body is shape 0's / shape 1's), built by LDS atomic OR (order-free)
   // in the contribution slots (free until the iterations), then written out in ascending order by the body's
@@ -1614,19 +1621,29 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   if (lrn) a.lrn = *lrn; else a.lrn = NmcLearnIO{};
   if (rep) a.rep = *rep; else a.rep = NmcXpbdReport{};
   // launches: one per size tier of worlds (automatic shape: <= 64 bodies on one wave, larger on 512 lanes),
-  // or all worlds in one launch of the requested shape
-  struct Launch { XpbdShape sh; int worlds, bodies; const int* list; };
+  // or all worlds in one launch of the requested shape, or (block_lanes 512, plain step of a model with worlds
+  // above 64 bodies) one launch over the CSR bins, largest first
+  a.bins = nullptr;
+  struct Launch { XpbdShape sh; int worlds, bodies; const int* list; const int* bins; int joints; };
   Launch ls[3];
   int nl = 0;
-  if (params->block_lanes == 0 && m->xpbd_tiered) {
+  const bool plain = rep == nullptr && a.lrn.mode == NMC_STEP_PLAIN;
+  if (params->block_lanes == 512 && m->xpbd_bin_count > 0 && plain) {
+    const int jl = m->xpbd_joint_lds > 0 ? std::max(m->xpbd_bin_max_joints, 1) : 0;
+    const XpbdShape sh = xpbd_shape(m->xpbd_bin_max_bodies, 0);
+    if (xpbd_lds_bytes_shape(sh, m->xpbd_bin_max_bodies, jl, m->xpbd_contact_cap, !params->reference_arithmetic,
+                             false) <= 160 * 1024)
+      ls[nl++] = Launch{sh, m->xpbd_bin_count, m->xpbd_bin_max_bodies, nullptr, m->xpbd_bins, jl};
+  }
+  if (nl == 0 && params->block_lanes != 64 && params->block_lanes != 256 && m->xpbd_tiered) {
     for (int t = 2; t >= 0; --t)  // largest worlds first
       if (m->xpbd_tier_count[t] > 0)
         ls[nl++] = Launch{xpbd_shape(m->xpbd_tier_max[t], 0), m->xpbd_tier_count[t], m->xpbd_tier_max[t],
-                          m->xpbd_tier_worlds[t]};
-  } else {
-    ls[nl++] = Launch{xpbd_shape(m->d.max_world_bodies, params->block_lanes), m->d.world_count + 1,
-                      m->d.max_world_bodies, nullptr};
+                          m->xpbd_tier_worlds[t], nullptr, m->xpbd_joint_lds};
   }
+  if (nl == 0)
+    ls[nl++] = Launch{xpbd_shape(m->d.max_world_bodies, params->block_lanes == 512 ? 0 : params->block_lanes),
+                      m->d.world_count + 1, m->d.max_world_bodies, nullptr, nullptr, m->xpbd_joint_lds};
   // several tiers: launch them concurrently (tier i > 0 on a side stream forked from the caller's stream and
   // joined back into it), so that each size class's launch shares the chip with the others
   const bool fork = nl > 1;
@@ -1640,7 +1657,9 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
     hipStream_t s = i == 0 ? stream : m->tier_stream[i - 1];
     a.lds_bodies = L.bodies;
     a.world_list = L.list;
-    const size_t lds = xpbd_lds_bytes_shape(L.sh, L.bodies, m->xpbd_joint_lds, m->xpbd_contact_cap,
+    a.bins = L.bins;
+    a.joint_lds = L.joints;
+    const size_t lds = xpbd_lds_bytes_shape(L.sh, L.bodies, L.joints, m->xpbd_contact_cap,
                                             !params->reference_arithmetic, rep != nullptr);
     if (rep) {
       launch_mode<XM_REPORT>(L.sh, L.worlds, lds, s, a);

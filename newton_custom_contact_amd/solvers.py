@@ -49,8 +49,8 @@ class SolverXPBD(SolverBase):
                  rigid_contact_con_weighting: bool = True, angular_damping: float = 0.0,
                  enable_restitution: bool = False, block_lanes: int = 0, reference_arithmetic: bool = False):
         super().__init__(model)
-        if block_lanes not in (0, 64, 256):
-            raise ValueError("block_lanes must be 0 (automatic), 64 or 256")
+        if block_lanes not in (0, 64, 256, 512):
+            raise ValueError("block_lanes must be 0 (automatic), 64, 256 or 512 (CSR bins, worlds above 64 bodies)")
         self.block_lanes = int(block_lanes)  # lanes per world of the step kernel (see autotune.tune_block_lanes)
         # False: world-frame contact arithmetic (within rounding of the reference, the fast path); True: the
         # reference's operation order, bit-identical to the CPU restatement

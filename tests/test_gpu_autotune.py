@@ -1,4 +1,5 @@
-"""The step kernel's launch shapes give bit-identical results, and the autotuner picks one of them."""
+"""The step and collide kernels' launch shapes give the same results (bit for bit, or within the step bar where noted),
+and the autotuner picks one of them."""
 
 from __future__ import annotations
 
@@ -60,12 +61,17 @@ def test_block_lanes_bit_identical(cuda_device, name, make):
 
 
 def test_tiered_launch_bit_identical_uneven(cuda_device):
-    """Worlds of 8-262 bodies: launching by size tier (automatic) equals running every world on 256 lanes."""
-    print("[block lanes] uneven tiers", flush=True)
+    """Worlds of 8-262 bodies: launching by size tier (automatic) equals running every world on 256 lanes, bit
+    for bit; the one launch over CSR bins of consecutive worlds equals both bit for bit with the reference-order
+    arithmetic, and within the step bar with the world-frame arithmetic (its packed small worlds sum per body
+    on one lane instead of a tree of lanes)."""
+    from parity_helpers import assert_step_close
+
+    print("[block lanes] uneven tiers and bins", flush=True)
     model = scenes.uneven_worlds(12, seed=8, device=cuda_device)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=10)
-    assert candidate_block_lanes(solver) == [0, 256]
+    assert candidate_block_lanes(solver) == [0, 256, 512]
     counts = model.host["world_body_count"]
     assert counts.max() > 64 and (counts[:-1] <= 64).any()
     c = pipe.contacts()
@@ -75,20 +81,26 @@ def test_tiered_launch_bit_identical_uneven(cuda_device):
         solver.step(s0, s1, None, c, DT)
         s0, s1 = s1, s0
     pipe.collide(s0, c)
-    outs = {}
-    for lanes in (0, 256):
-        solver.block_lanes = lanes
-        a, b = model.state(), model.state()
-        a.body_q.copy_(s0.body_q)
-        a.body_qd.copy_(s0.body_qd)
-        solver.step(a, b, None, c, DT)
-        torch.cuda.synchronize()
-        outs[lanes] = (b.body_q.cpu().numpy(), b.body_qd.cpu().numpy())
     assert int(c.rigid_contact_count.item()) > 0
-    assert np.array_equal(outs[0][0], outs[256][0]) and np.array_equal(outs[0][1], outs[256][1])
+    for ref_order in (False, True):
+        sv = SolverXPBD(model, iterations=10, reference_arithmetic=ref_order)
+        outs = {}
+        for lanes in (0, 256, 512):
+            sv.block_lanes = lanes
+            a, b = model.state(), model.state()
+            a.body_q.copy_(s0.body_q)
+            a.body_qd.copy_(s0.body_qd)
+            sv.step(a, b, None, c, DT)
+            torch.cuda.synchronize()
+            outs[lanes] = (b.body_q.cpu().numpy(), b.body_qd.cpu().numpy())
+        assert np.array_equal(outs[0][0], outs[256][0]) and np.array_equal(outs[0][1], outs[256][1])
+        if ref_order:
+            assert np.array_equal(outs[0][0], outs[512][0]) and np.array_equal(outs[0][1], outs[512][1])
+        else:
+            assert_step_close(outs[512][0], outs[512][1], outs[0][0], outs[0][1], DT, "bins vs tiers")
     res = tune_block_lanes(solver, s0, s1, c, DT, reps=2)
     print(res, flush=True)
-    assert res["block_lanes"] in (0, 256)
+    assert res["block_lanes"] in (0, 256, 512)
 
 
 @pytest.mark.parametrize("name,make", MODELS, ids=[m[0] for m in MODELS])
