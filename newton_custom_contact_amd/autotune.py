@@ -8,8 +8,9 @@ Step kernel (``SolverXPBD.block_lanes``): one workgroup per world.  Two shapes e
 bodies: one 64-lane wave holding 128 contacts in registers (best when a world has a few hundred contacts at
 most), and 256 lanes holding 256 (more lanes per world for contact-heavy worlds, e.g. triangle meshes with up
 to 64 contacts per pair).  Models with larger worlds can launch by size tier (automatic, 0), run every world
-in one launch of the large-world shape (256: 512 lanes a world), or run one launch over CSR bins (512: runs of
-consecutive worlds packed up to the largest world's body count, largest bins first).  The reference-order
+in one launch of the large-world shape (256: 512 lanes a world); any multi-world model can run one launch over
+CSR bins (512: runs of consecutive worlds packed up to max(largest world, 64) bodies, largest bins first; 512
+lanes a bin, or one wave when every world fits one).  The reference-order
 arithmetic is bit-identical across all of them; the world-frame arithmetic rounds its per-body sums by the
 lanes a body gets, which the bins change for the small worlds they pack.
 
@@ -46,8 +47,9 @@ def _time(fn, device, reps: int) -> float:
 def candidate_block_lanes(solver) -> list[int]:
     """Launch shapes the solver's model admits: 64 lanes needs worlds of at most 64 bodies; 0 (automatic)
     launches worlds by size tier (<= 64 bodies on one wave, larger worlds on 512 lanes), 256 runs every
-    world in one launch of the large-world shape, 512 runs one launch over CSR bins of consecutive worlds."""
-    return [64, 256] if solver.native.desc.max_world_bodies <= 64 else [0, 256, 512]
+    world in one launch of the large-world shape, 512 runs one launch over CSR bins of consecutive worlds
+    (packed up to max(largest world, 64) bodies: one wave per bin when every world fits one)."""
+    return [64, 256, 512] if solver.native.desc.max_world_bodies <= 64 else [0, 256, 512]
 
 
 def tune_block_lanes(solver, state_in, state_out, contacts, dt: float, reps: int = 5) -> dict:

@@ -45,7 +45,7 @@ static hipError_t dmalloc(T** p, size_t count) {
 
 static bool valid_lanes(const NmcModelImpl* m, const NmcXpbdParams* p) {
   return p->block_lanes == 0 || p->block_lanes == 256 || (p->block_lanes == 64 && m->d.max_world_bodies <= 64) ||
-         (p->block_lanes == 512 && m->d.max_world_bodies > 64);
+         p->block_lanes == 512;
 }
 
 static int next_pow2(int n) {
@@ -236,45 +236,6 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
     int used = 0;
     for (int t = 0; t < 3; ++t) used += tier[t].empty() ? 0 : 1;
     m->xpbd_tiered = used > 1;
-    // CSR bins: consecutive worlds whose body and joint ranges follow each other, packed greedily while the
-    // bin's bodies stay within the largest world's (so one LDS layout holds every bin); largest bins first
-    {
-      std::vector<int> bf(W1), jf(W1), jc(W1);
-      if ((e = hipMemcpy(bf.data(), d.world_body_first, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess ||
-          (e = hipMemcpy(jf.data(), d.world_joint_first, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess ||
-          (e = hipMemcpy(jc.data(), d.world_joint_count, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess) {
-        nmc_model_destroy(m);
-        return hip_fail(e, "nmc_model_create: world ranges");
-      }
-      const int cap = d.max_world_bodies;
-      std::vector<std::array<int, 4>> bins;  // first, end, bodies, joints
-      for (int w = 0; w < W1;) {
-        int e1 = w + 1, bodies = nb[w], joints = jc[w];
-        while (e1 < W1 && bodies + nb[e1] <= cap && bf[e1] == bf[e1 - 1] + nb[e1 - 1] &&
-               jf[e1] == jf[e1 - 1] + jc[e1 - 1]) {
-          bodies += nb[e1];
-          joints += jc[e1];
-          ++e1;
-        }
-        bins.push_back({w, e1, bodies, joints});
-        w = e1;
-      }
-      std::stable_sort(bins.begin(), bins.end(),
-                       [](const std::array<int, 4>& x, const std::array<int, 4>& y) { return x[2] > y[2]; });
-      std::vector<int> flat;
-      for (const auto& b : bins) {
-        flat.push_back(b[0]);
-        flat.push_back(b[1]);
-        m->xpbd_bin_max_bodies = std::max(m->xpbd_bin_max_bodies, std::max(b[2], 1));
-        m->xpbd_bin_max_joints = std::max(m->xpbd_bin_max_joints, b[3]);
-      }
-      m->xpbd_bin_count = static_cast<int>(bins.size());
-      if ((e = dmalloc(&m->xpbd_bins, flat.size())) != hipSuccess ||
-          (e = hipMemcpy(m->xpbd_bins, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice)) != hipSuccess) {
-        nmc_model_destroy(m);
-        return hip_fail(e, "nmc_model_create: world bins");
-      }
-    }
     if (m->xpbd_tiered) {
       // side streams (default priority) and fork / join events for the concurrent tier launches
       e = hipEventCreateWithFlags(&m->tier_fork, hipEventDisableTiming);
@@ -296,6 +257,50 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
           delete m;
           return hip_fail(e, "nmc_model_create: world tiers");
         }
+      }
+    }
+  }
+  // CSR bins: consecutive worlds whose body and joint ranges follow each other, packed greedily while the
+  // bin's bodies stay within max(largest world, 64) (so one LDS layout holds every bin); largest bins first.
+  // Kept only when packing merges worlds.
+  if (d.world_count >= 1) {
+    const int W1 = d.world_count + 1;
+    std::vector<int> nb(W1), bf(W1), jf(W1), jc(W1);
+    if ((e = hipMemcpy(nb.data(), d.world_body_count, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(bf.data(), d.world_body_first, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(jf.data(), d.world_joint_first, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess ||
+        (e = hipMemcpy(jc.data(), d.world_joint_count, sizeof(int) * W1, hipMemcpyDeviceToHost)) != hipSuccess) {
+      nmc_model_destroy(m);
+      return hip_fail(e, "nmc_model_create: world ranges");
+    }
+    const int cap = std::max(d.max_world_bodies, 64);
+    std::vector<std::array<int, 4>> bins;  // first, end, bodies, joints
+    for (int w = 0; w < W1;) {
+      int e1 = w + 1, bodies = nb[w], joints = jc[w];
+      while (e1 < W1 && bodies + nb[e1] <= cap && bf[e1] == bf[e1 - 1] + nb[e1 - 1] &&
+             jf[e1] == jf[e1 - 1] + jc[e1 - 1]) {
+        bodies += nb[e1];
+        joints += jc[e1];
+        ++e1;
+      }
+      bins.push_back({w, e1, bodies, joints});
+      w = e1;
+    }
+    if (static_cast<int>(bins.size()) < W1) {
+      std::stable_sort(bins.begin(), bins.end(),
+                       [](const std::array<int, 4>& x, const std::array<int, 4>& y) { return x[2] > y[2]; });
+      std::vector<int> flat;
+      for (const auto& bn : bins) {
+        flat.push_back(bn[0]);
+        flat.push_back(bn[1]);
+        m->xpbd_bin_max_bodies = std::max(m->xpbd_bin_max_bodies, std::max(bn[2], 1));
+        m->xpbd_bin_max_joints = std::max(m->xpbd_bin_max_joints, bn[3]);
+      }
+      m->xpbd_bin_count = static_cast<int>(bins.size());
+      if ((e = dmalloc(&m->xpbd_bins, flat.size())) != hipSuccess ||
+          (e = hipMemcpy(m->xpbd_bins, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+        nmc_model_destroy(m);
+        return hip_fail(e, "nmc_model_create: world bins");
       }
     }
   }
@@ -449,7 +454,7 @@ extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q
   if (q_in == q_out || qd_in == qd_out) return fail("nmc_xpbd_step: state_in and state_out must not alias");
   if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
     return fail("nmc_xpbd_step: contacts capacity exceeds the rigid_contact_max the model was created with");
-  if (!valid_lanes(m, p)) return fail("nmc_xpbd_step: block_lanes must be 0, 256, 64 (worlds of up to 64 bodies) or 512 (worlds above 64 bodies)");
+  if (!valid_lanes(m, p)) return fail("nmc_xpbd_step: block_lanes must be 0, 256, 64 (worlds of up to 64 bodies) or 512 (CSR bins)");
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
                              qd_out, nullptr, nullptr, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step launch");
@@ -468,7 +473,7 @@ extern "C" int nmc_xpbd_step_learn(NmcModel* m, const NmcXpbdParams* p, const fl
   if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
     return fail("nmc_xpbd_step_learn: contacts capacity exceeds the rigid_contact_max the model was created with");
   if (!valid_lanes(m, p))
-    return fail("nmc_xpbd_step_learn: block_lanes must be 0, 256, 64 (worlds of up to 64 bodies) or 512 (worlds above 64 bodies)");
+    return fail("nmc_xpbd_step_learn: block_lanes must be 0, 256, 64 (worlds of up to 64 bodies) or 512 (CSR bins)");
   const int mode = lrn->mode;
   if (mode < NMC_STEP_PLAIN || mode > NMC_STEP_WARM) return fail("nmc_xpbd_step_learn: unknown mode");
   if (mode != NMC_STEP_PREP) {
@@ -504,7 +509,7 @@ extern "C" int nmc_xpbd_step_report(NmcModel* m, const NmcXpbdParams* p, const f
   if (p->has_contacts && 2ll * c->capacity > m->xpbd_keys_cap)
     return fail("nmc_xpbd_step_report: contacts capacity exceeds the rigid_contact_max the model was created with");
   if (!valid_lanes(m, p))
-    return fail("nmc_xpbd_step_report: block_lanes must be 0, 256, 64 (worlds of up to 64 bodies) or 512 (worlds above 64 bodies)");
+    return fail("nmc_xpbd_step_report: block_lanes must be 0, 256, 64 (worlds of up to 64 bodies) or 512 (CSR bins)");
   if (rep->contact_impulse && !rep->contact_scratch)
     return fail("nmc_xpbd_step_report: contact_impulse needs contact_scratch");
   if (rep->body_parent_f && m->d.joint_count > 0 && !rep->joint_impulse)

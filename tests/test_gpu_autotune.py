@@ -31,11 +31,13 @@ MODELS = [
 
 @pytest.mark.parametrize("name,make", MODELS, ids=[m[0] for m in MODELS])
 def test_block_lanes_bit_identical(cuda_device, name, make):
+    from parity_helpers import assert_step_close
+
     print(f"[block lanes] {name}", flush=True)
     model = make(cuda_device)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=10)
-    assert candidate_block_lanes(solver) == [64, 256]
+    assert candidate_block_lanes(solver) == [64, 256, 512]
     c = pipe.contacts()
     s0, s1 = model.state(), model.state()
     # settle a little so that worlds carry contacts
@@ -44,20 +46,28 @@ def test_block_lanes_bit_identical(cuda_device, name, make):
         solver.step(s0, s1, None, c, DT)
         s0, s1 = s1, s0
     pipe.collide(s0, c)
-    outs = {}
-    for lanes in (64, 256):
-        solver.block_lanes = lanes
-        a, b = model.state(), model.state()
-        a.body_q.copy_(s0.body_q)
-        a.body_qd.copy_(s0.body_qd)
-        solver.step(a, b, None, c, DT)
-        torch.cuda.synchronize()
-        outs[lanes] = (b.body_q.cpu().numpy(), b.body_qd.cpu().numpy())
     assert int(c.rigid_contact_count.item()) > 0
-    assert np.array_equal(outs[64][0], outs[256][0]) and np.array_equal(outs[64][1], outs[256][1])
+    for ref_order in (False, True):
+        sv = SolverXPBD(model, iterations=10, reference_arithmetic=ref_order)
+        outs = {}
+        for lanes in (64, 256, 512):
+            sv.block_lanes = lanes
+            a, b = model.state(), model.state()
+            a.body_q.copy_(s0.body_q)
+            a.body_qd.copy_(s0.body_qd)
+            sv.step(a, b, None, c, DT)
+            torch.cuda.synchronize()
+            outs[lanes] = (b.body_q.cpu().numpy(), b.body_qd.cpu().numpy())
+        assert np.array_equal(outs[64][0], outs[256][0]) and np.array_equal(outs[64][1], outs[256][1])
+        # CSR bins (worlds packed into one 64-body wave): the same with the reference order, within the step bar
+        # with the world-frame arithmetic (one lane a body instead of a tree of lanes)
+        if ref_order:
+            assert np.array_equal(outs[64][0], outs[512][0]) and np.array_equal(outs[64][1], outs[512][1])
+        else:
+            assert_step_close(outs[512][0], outs[512][1], outs[64][0], outs[64][1], DT, "bins vs worlds")
     res = tune_block_lanes(solver, s0, s1, c, DT, reps=2)
     print(name, res, flush=True)
-    assert res["block_lanes"] in (64, 256) and solver.block_lanes == res["block_lanes"]
+    assert res["block_lanes"] in (64, 256, 512) and solver.block_lanes == res["block_lanes"]
 
 
 def test_tiered_launch_bit_identical_uneven(cuda_device):
