@@ -163,8 +163,9 @@ def phase_times(pipe, solver, a, b, contacts, sdt, reps):
 def timed_form(args, model, pipe, solver, worlds, settle, warmup, steps, dev, tune=None):
     """Settle (untimed), warm up, then time `steps` graph replays of one step (4 substeps of collide + step).
     `tune(state_a, state_b, contacts, sdt)` (optional) runs on the settled state and may change the collide and
-    step kernels' launch shapes; the graph is captured again after it.  Returns (states, contacts, ms, N_c, t_collide_ms,
-    t_xpbd_ms, tune result)."""
+    step kernels' launch shapes; the graph is captured again after it.  N_c is counted over as many steps after
+    the timed ones (a second graph with the counter), so the timed graph holds the step alone.  Returns (states,
+    contacts, ms, N_c, t_collide_ms, t_xpbd_ms, tune result)."""
     import torch
 
     contacts = pipe.contacts()
@@ -172,42 +173,43 @@ def timed_form(args, model, pipe, solver, worlds, settle, warmup, steps, dev, tu
     sdt = (1.0 / 60.0) / args.substeps
     nc_acc = torch.zeros(1, dtype=torch.int64, device=dev)
 
-    def step():
+    def step(count=False):
         for s in range(args.substeps):
             a, b = states[s % 2], states[(s + 1) % 2]
             pipe.collide(a, contacts)
             solver.step(a, b, None, contacts, sdt)
-            nc_acc.add_(contacts.rigid_contact_count)
+            if count:
+                nc_acc.add_(contacts.rigid_contact_count)
         if args.substeps % 2:
             states[0].body_q.copy_(states[1].body_q)
             states[0].body_qd.copy_(states[1].body_qd)
 
-    def capture():
+    def capture(count=False):
         torch.cuda.synchronize()
         if args.no_graph:
             return None
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            step()
+            step(count)
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
-            step()
+            step(count)
         torch.cuda.synchronize()
         return g
 
     graph = capture()
 
-    def run(n):
+    def run(n, g, count=False):
         for _ in range(n):
-            if graph is not None:
-                graph.replay()
+            if g is not None:
+                g.replay()
             else:
-                step()
+                step(count)
 
-    run(settle)
+    run(settle, graph)
     tuned = None
     if tune is not None:
         torch.cuda.synchronize()
@@ -215,18 +217,24 @@ def timed_form(args, model, pipe, solver, worlds, settle, warmup, steps, dev, tu
         tuned = tune(states[0], states[1], contacts, sdt)
         del graph
         graph = capture()
-    run(warmup)
+    run(warmup, graph)
     torch.cuda.synchronize()
-    nc_acc.zero_()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     e0.record()
-    run(steps)
+    run(steps, graph)
     e1.record()
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1)
+    # contacts per world and substep over as many further steps (the bench's own accounting stays out of the
+    # timed graph)
+    counting = capture(count=True)
+    nc_acc.zero_()
+    run(steps, counting, count=True)
+    torch.cuda.synchronize()
     n_c = float(nc_acc.item()) / (steps * args.substeps * worlds)
+    del counting
     t_col, t_xp = phase_times(pipe, solver, states[0], states[1], contacts, sdt, args.kernel_reps)
     del graph
     return states, contacts, ms, n_c, t_col, t_xp, tuned

@@ -163,6 +163,9 @@ typedef struct NmcContacts {
    * world -1 segment last), by sort key inside a segment; a body's contacts are in the same relative order
    * in both. */
   int* order;
+  /* [1] or NULL: Contacts.contact_generation, bumped once per nmc_collide (wrapping from INT32_MAX to 0 as
+   * compute_shape_aabbs_and_prepare_kernel does, collide.py:181-191) */
+  int* generation;
 } NmcContacts;
 
 typedef struct NmcXpbdParams {

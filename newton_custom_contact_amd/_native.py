@@ -61,6 +61,7 @@ class NmcContacts(ctypes.Structure):
         ("capacity", ctypes.c_int), ("count", _vp), ("shape0", _vp), ("shape1", _vp), ("point0", _vp),
         ("point1", _vp), ("offset0", _vp), ("offset1", _vp), ("normal", _vp), ("margin0", _vp),
         ("margin1", _vp), ("sort_key", _vp), ("world_start", _vp), ("world_count", _vp), ("order", _vp),
+        ("generation", _vp),
     ]
 
 

@@ -257,7 +257,6 @@ class CollisionPipeline:
                 raise ValueError("CollisionPipeline has contact_report enabled but the Contacts buffer was created "
                                  "without contact_report=True. Use pipeline.contacts() to create a compatible buffer.")
             self._contact_matcher.match(q, contacts)
-        contacts.contact_generation += 1
 
     def overflow_counters(self) -> dict:
         """Overflow counters of every collide so far (verify_narrow_phase_buffers, narrow_phase.py:1334)."""

@@ -90,6 +90,7 @@ class Contacts:
             v.sort_key = ptr(self.rigid_contact_sort_key)
             v.world_start, v.world_count = ptr(self.world_contact_start), ptr(self.world_contact_count)
             v.order = ptr(self.rigid_contact_order)
+            v.generation = ptr(self.contact_generation)
             self._view = v
         return self._view
 

@@ -20,7 +20,7 @@
 //   narrow_kernel           <<<<=2048, 256>>>     narrow phase -> contact slots, per-pair counts
 //   mesh_narrow_kernel      <<<<=2048, 256>>>     mesh pairs, one workgroup per pair (models with meshes)
 //   contact_count_kernel    <<<segments, 64>>>    contacts per segment
-//   contact_scan_kernel     <<<1, 1024>>>         Contacts.world_start / world_count / count
+//   contact_scan_kernel     <<<1, 1024>>>         Contacts.world_start / world_count / count, generation bump
 //   contact_compact_kernel  <<<segments, 256>>>   slots -> Contacts arrays
 #include <hip/hip_runtime.h>
 
@@ -717,6 +717,10 @@ __global__ void __launch_bounds__(SCAN_BLOCK) contact_scan_kernel(const int* __r
   if (threadIdx.x == 0) {
     c.count[0] = total;
     if (total > c.capacity) atomicAdd(&counters[2], total - c.capacity);
+    if (c.generation) {
+      const int g = c.generation[0];
+      c.generation[0] = g == 2147483647 ? 0 : g + 1;
+    }
   }
 }
 
