@@ -39,6 +39,8 @@ constexpr int MESH_BLOCK = 256;
 // Slot tables of up to this many subgrids (12^3: up to 96 cells a side) are staged in LDS per mesh pair, so a
 // sample's indirection is an LDS read; larger textures are read from global memory.
 constexpr int MESH_SLOT_LDS = 2048;
+// Edges culled per round of the mesh-mesh pass (the survivors' list in LDS).
+constexpr int MESH_EDGE_LIST = 1024;
 
 __device__ __forceinline__ unsigned mesh_sortable(float f) {
   if (f == 0.0f) f = 0.0f;
@@ -265,17 +267,23 @@ __device__ __forceinline__ EdgeSide edge_side(const NmcModelDesc& d, const Shape
   return e;
 }
 
-// Edge e of one side (sdf_contact.py:1010-1150): culling, Brent search, gradient normal.  Returns false
-// when the edge yields no contact; else the contact's world point, its normal (from shape0 to shape1)
-// and distance.
-__device__ bool edge_contact(const EdgeSide& E, int mode, int e, f3& center, f3& normal, float& dist) {
+// The edge's end points in the SDF side's unscaled frame.
+__device__ __forceinline__ void edge_points(const EdgeSide& E, int e, f3& ea, f3& eb) {
   const int ia = E.edge[2 * e], ib = E.edge[2 * e + 1];
   f3 pa = load3(E.vtx + 3 * ia), pb = load3(E.vtx + 3 * ib);
   pa = f3{pa.x * E.tscale.x, pa.y * E.tscale.y, pa.z * E.tscale.z};
   pb = f3{pb.x * E.tscale.x, pb.y * E.tscale.y, pb.z * E.tscale.z};
-  f3 ea = xpoint(E.X_ts, pa), eb = xpoint(E.X_ts, pb);
+  ea = xpoint(E.X_ts, pa);
+  eb = xpoint(E.X_ts, pb);
   ea = f3{ea.x * E.inv_s.x, ea.y * E.inv_s.y, ea.z * E.inv_s.z};
   eb = f3{eb.x * E.inv_s.x, eb.y * E.inv_s.y, eb.z * E.inv_s.z};
+}
+
+// The culling of edge e (sdf_contact.py:1010-1045): its bounding sphere against the SDF box, then the SDF at
+// its midpoint.  True when the edge goes on to the search; msdf is the midpoint value the search starts from.
+__device__ __forceinline__ bool edge_cull(const EdgeSide& E, int e, float& msdf) {
+  f3 ea, eb;
+  edge_points(E, e, ea, eb);
   const f3 mid = (ea + eb) * 0.5f;
   const float rad = len(eb - ea) * 0.5f;
   const float cull = rad + E.threshold_u;
@@ -283,8 +291,17 @@ __device__ bool edge_contact(const EdgeSide& E, int mode, int e, f3& center, f3&
   const f3 cl{clampf(mid.x, G.lower.x, G.upper.x), clampf(mid.y, G.lower.y, G.upper.y),
               clampf(mid.z, G.lower.z, G.upper.z)};
   if (len_sq(mid - cl) > cull * cull) return false;
-  const float msdf = sdf_sample(G, mid);
-  if (!(msdf <= cull)) return false;
+  msdf = sdf_sample(G, mid);
+  return msdf <= cull;
+}
+
+// The rest of edge e once it passed the culling (sdf_contact.py:1045-1150): Brent search from the midpoint
+// value, threshold, gradient normal.  Returns false when the edge yields no contact; else the contact's world
+// point, its normal (from shape0 to shape1) and distance.
+__device__ bool edge_finish(const EdgeSide& E, int mode, int e, float msdf, f3& center, f3& normal, float& dist) {
+  f3 ea, eb;
+  edge_points(E, e, ea, eb);
+  const SdfGrid& G = E.G;
   f3 pu;
   const float du0 = edge_sdf_search(G, ea, eb, msdf, pu);
   if (!(du0 * E.min_s < E.threshold)) return false;
@@ -307,6 +324,14 @@ __device__ bool edge_contact(const EdgeSide& E, int mode, int e, f3& center, f3&
   }
   normal = mode == 0 ? -dw : dw;
   return true;
+}
+
+// Edge e of one side (sdf_contact.py:1010-1150): culling, Brent search, gradient normal.
+__device__ __forceinline__ bool edge_contact(const EdgeSide& E, int mode, int e, f3& center, f3& normal,
+                                             float& dist) {
+  float msdf;
+  if (!edge_cull(E, e, msdf)) return false;
+  return edge_finish(E, mode, e, msdf, center, normal, dist);
 }
 
 // Vertex v of a mesh against an infinite plane (narrow_phase.py:1150-1190).
@@ -394,6 +419,9 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
   __shared__ int s_n;
   __shared__ int s_w[MESH_BLOCK / 64];
   __shared__ unsigned s_slot[2][MESH_SLOT_LDS];  // [mode]: slots of the SDF that side samples
+  __shared__ int s_nlist;
+  __shared__ int s_list[MESH_EDGE_LIST];           // edges of one side that passed the culling
+  __shared__ float s_lmsdf[MESH_EDGE_LIST];        // and their midpoint SDF values
   // reduced: the 240 slots, then 256 winner sub keys; unreduced: cand_cap candidate sub keys
   unsigned long long* slots = reinterpret_cast<unsigned long long*>(smem);
   unsigned* sel = reinterpret_cast<unsigned*>(smem + (rp.mode ? sizeof(unsigned long long) * 256 : 0));
@@ -482,19 +510,44 @@ __global__ void __launch_bounds__(MESH_BLOCK) mesh_narrow_kernel(NmcModelDesc d,
         else push(v, center, normal, dist);
       }
     } else {
+      // Edges that pass the culling are listed in LDS first and searched from the list, so that the lanes
+      // of a wave run the (long) Brent search together instead of waiting on the few of them that do.
       for (int mode = 0; mode < 2; ++mode) {
         const int tm = mode == 0 ? ma : mb;
         const EdgeSide E = side(mode);
         RedFrame F;
         if (reduce) F = red_frame(d, mode == 0 ? sa : sb, E.XT, true, (E.XT.p + E.XS.p) * 0.5f);
         const int ne = d.mesh_edge_start[tm + 1] - d.mesh_edge_start[tm];
-        for (int e = threadIdx.x; e < ne; e += blockDim.x) {
-          f3 center, normal;
-          float dist;
-          if (!edge_contact(E, mode, e, center, normal, dist)) continue;
-          const int sub = (e << 2) | (mode << 1);
-          if (reduce) red_candidate(slots, rp, F, center, normal, dist, sub);
-          else push(sub, center, normal, dist);
+        for (int base = 0; base < ne; base += MESH_EDGE_LIST) {
+          const int end = ne < base + MESH_EDGE_LIST ? ne : base + MESH_EDGE_LIST;
+          if (threadIdx.x == 0) s_nlist = 0;
+          __syncthreads();
+          for (int e = base + threadIdx.x; e < end; e += blockDim.x) {
+            float msdf;
+            const bool pass = edge_cull(E, e, msdf);
+            const unsigned long long bal = __ballot(pass);
+            const int lane = threadIdx.x & 63;
+            int at = 0;
+            if (lane == 0 && bal) at = atomicAdd(&s_nlist, __popcll(bal));
+            at = __shfl(at, 0, 64);
+            if (pass) {
+              const int k = at + __popcll(bal & ((1ull << lane) - 1ull));
+              s_list[k] = e;
+              s_lmsdf[k] = msdf;
+            }
+          }
+          __syncthreads();
+          const int nl = s_nlist;
+          for (int k = threadIdx.x; k < nl; k += blockDim.x) {
+            const int e = s_list[k];
+            f3 center, normal;
+            float dist;
+            if (!edge_finish(E, mode, e, s_lmsdf[k], center, normal, dist)) continue;
+            const int sub = (e << 2) | (mode << 1);
+            if (reduce) red_candidate(slots, rp, F, center, normal, dist, sub);
+            else push(sub, center, normal, dist);
+          }
+          __syncthreads();
         }
       }
     }
