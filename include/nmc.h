@@ -138,6 +138,8 @@ typedef struct NmcModelDesc {
   int broad_phase_mode;
   const int* explicit_pairs;
   int explicit_pair_count;
+  /* S: shape_material_restitution (model.py), read by the restitution pass only; null = 0 for every shape */
+  const float* shape_material_restitution;
 } NmcModelDesc;
 
 /* Device-resident contact buffers (layout of newton.Contacts rigid part). */
@@ -191,6 +193,11 @@ typedef struct NmcXpbdParams {
                              * matrices, world inverse inertia, FMA, hardware reciprocal), within rounding of
                              * the reference; 1: the reference's operation order, bit-identical to a CPU
                              * sweep compiled without contraction.  Same method, same iteration order. */
+  int enable_restitution;   /* SolverXPBD(enable_restitution=True) (solver_xpbd.py:95-109): after the iterations,
+                             * apply_rigid_restitution (kernels.py:2472-2615) on the start-of-step poses and
+                             * velocities and the solved velocities, per-body deltas summed in contact order,
+                             * then added to body_qd_out (kernels.py:882-887).  Needs contacts; in
+                             * nmc_xpbd_step_learn every mode but NMC_STEP_PREP applies it. */
 } NmcXpbdParams;
 
 /* Learned warm start of the XPBD contact solve (no counterpart in the reference, whose contact solve
@@ -236,8 +243,9 @@ void nmc_model_destroy(NmcModel* model);
  * order (segment-major, ascending by the 64-bit sort key inside a segment). */
 int nmc_collide(NmcModel* model, const float* body_q, const NmcContacts* contacts, nmc_stream_t stream);
 
-/* One XPBD step: integrate, `iterations` x (contact solve + apply, joint solve + apply).  Fails when
- * contacts->capacity exceeds the model's rigid_contact_max or when state in/out arrays alias. */
+/* One XPBD step: integrate, `iterations` x (contact solve + apply, joint solve + apply), then the restitution
+ * pass when params->enable_restitution is set (a second launch).  Fails when contacts->capacity exceeds the
+ * model's rigid_contact_max or when state in/out arrays alias. */
 int nmc_xpbd_step(NmcModel* model, const NmcXpbdParams* params, const float* body_q_in, const float* body_qd_in,
                   const float* body_f_in, const float* joint_f, const float* joint_target_pos,
                   const float* joint_target_vel, const NmcContacts* contacts, float dt, float* body_q_out,

@@ -18,7 +18,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libnmc.so")
-SOURCES = ["collide.hip", "broad.hip", "mesh.hip", "match.hip", "xpbd.hip", "mlp.hip", "nmc_abi.hip", "noalloc.hip"]
+SOURCES = ["collide.hip", "broad.hip", "mesh.hip", "match.hip", "xpbd.hip", "restitution.hip", "mlp.hip", "nmc_abi.hip", "noalloc.hip"]
 HEADERS = ["nmc_math.h", "nmc_internal.h", "narrow.h", "sdf.h", "scan.h", "broad_common.h", "reduce.h", "xpbd_fast.h"]
 
 

@@ -48,6 +48,7 @@ class NmcModelDesc(ctypes.Structure):
         ("mesh_reduce", ctypes.c_int),
         ("shape_voxel_resolution", _vp),
         ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _vp), ("explicit_pair_count", ctypes.c_int),
+        ("shape_material_restitution", _vp),
     ]
 
 
@@ -72,6 +73,7 @@ class NmcXpbdParams(ctypes.Structure):
         ("joint_linear_relaxation", ctypes.c_float), ("joint_angular_relaxation", ctypes.c_float),
         ("joint_linear_compliance", ctypes.c_float), ("joint_angular_compliance", ctypes.c_float),
         ("has_contacts", ctypes.c_int), ("block_lanes", ctypes.c_int), ("reference_arithmetic", ctypes.c_int),
+        ("enable_restitution", ctypes.c_int),
     ]
 
 
@@ -311,6 +313,7 @@ class NativeModel:
         d.shape_group, d.shape_world = ptr(m.shape_collision_group), ptr(m.shape_world)
         d.shape_mu, d.shape_mu_torsional = ptr(m.shape_material_mu), ptr(m.shape_material_mu_torsional)
         d.shape_mu_rolling = ptr(m.shape_material_mu_rolling)
+        d.shape_material_restitution = ptr(m.shape_material_restitution)
         d.segment_count = len(seg_start) - 1
         d.segment_start, d.segment_shape = ptr(m.segment_start), ptr(m.segment_shape)
         d.max_segment_shapes = max_seg

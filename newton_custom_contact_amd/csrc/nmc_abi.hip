@@ -22,6 +22,8 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
                        const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
                        const NmcContacts* c, float dt, float* q_out, float* qd_out, const NmcLearnIO* lrn,
                        const NmcXpbdReport* rep, hipStream_t stream);
+hipError_t launch_restitution(NmcModelImpl* m, const float* q_prev, const float* qd_prev, const NmcContacts* c,
+                              float dt, float* qd, hipStream_t stream);
 
 static thread_local std::string g_error;
 void set_error(const std::string& msg) { g_error = msg; }
@@ -443,6 +445,12 @@ extern "C" int nmc_collide(NmcModel* m, const float* body_q, const NmcContacts* 
   return 0;
 }
 
+// The restitution pass runs when asked for, with contacts and bodies (solver_xpbd.py:719-764: only with a
+// contacts object).
+static bool restitution_on(const NmcModel* m, const NmcXpbdParams* p) {
+  return p->enable_restitution && p->has_contacts && m->d.body_count > 0;
+}
+
 extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q_in, const float* qd_in,
                              const float* f_in, const float* joint_f, const float* tpos, const float* tvel,
                              const NmcContacts* c, float dt, float* q_out, float* qd_out, nmc_stream_t stream) {
@@ -458,6 +466,10 @@ extern "C" int nmc_xpbd_step(NmcModel* m, const NmcXpbdParams* p, const float* q
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
                              qd_out, nullptr, nullptr, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step launch");
+  if (restitution_on(m, p)) {
+    e = launch_restitution(m, q_in, qd_in, c, dt, qd_out, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step restitution launch");
+  }
   return 0;
 }
 
@@ -493,6 +505,10 @@ extern "C" int nmc_xpbd_step_learn(NmcModel* m, const NmcXpbdParams* p, const fl
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
                              qd_out, lrn, nullptr, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step_learn launch");
+  if (mode != NMC_STEP_PREP && restitution_on(m, p)) {
+    e = launch_restitution(m, q_in, qd_in, c, dt, qd_out, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step_learn restitution launch");
+  }
   return 0;
 }
 
@@ -519,6 +535,10 @@ extern "C" int nmc_xpbd_step_report(NmcModel* m, const NmcXpbdParams* p, const f
   hipError_t e = launch_xpbd(m, p, q_in, qd_in, f_in, joint_f, tpos, tvel, p->has_contacts ? c : nullptr, dt, q_out,
                              qd_out, nullptr, &r, reinterpret_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step_report launch");
+  if (restitution_on(m, p)) {
+    e = launch_restitution(m, q_in, qd_in, c, dt, qd_out, reinterpret_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(e, "nmc_xpbd_step_report restitution launch");
+  }
   return 0;
 }
 

@@ -10,6 +10,9 @@ Differences in observable behaviour, both deliberate:
   (solver_xpbd.py:218-230) so its contents after ``step`` are unspecified there.
 * per-body sums over contacts and joints run in canonical (contact / joint index) order instead of
   float atomics, so results do not depend on thread scheduling.
+
+``enable_restitution=True`` adds the rigid restitution pass after the iterations (solver_xpbd.py:719-764,
+apply_rigid_restitution kernels.py:2472-2615): a second launch that changes only ``state_out.body_qd``.
 """
 
 from __future__ import annotations
@@ -55,8 +58,6 @@ class SolverXPBD(SolverBase):
         # False: world-frame contact arithmetic (within rounding of the reference, the fast path); True: the
         # reference's operation order, bit-identical to the CPU restatement
         self.reference_arithmetic = bool(reference_arithmetic)
-        if enable_restitution:
-            raise NotImplementedError("restitution is not part of this build")
         self.iterations = int(iterations)
         self.soft_body_relaxation = soft_body_relaxation
         self.soft_contact_relaxation = soft_contact_relaxation
@@ -90,7 +91,8 @@ class SolverXPBD(SolverBase):
         return NmcXpbdParams(self.iterations, self.rigid_contact_relaxation, int(self.rigid_contact_con_weighting),
                              self.angular_damping, self.joint_linear_relaxation, self.joint_angular_relaxation,
                              self.joint_linear_compliance, self.joint_angular_compliance, int(has_contacts),
-                             int(getattr(self, "block_lanes", 0)), int(getattr(self, "reference_arithmetic", False)))
+                             int(getattr(self, "block_lanes", 0)), int(getattr(self, "reference_arithmetic", False)),
+                             int(bool(getattr(self, "enable_restitution", False))))
 
     def step(self, state_in, state_out, control, contacts, dt: float) -> None:
         """solver_xpbd.py:259-764."""

@@ -47,6 +47,7 @@ class OracleModel(ctypes.Structure):
         ("mesh_sdf_flags", _i32p), ("mesh_reduce", ctypes.c_int),
         ("shape_voxel_res", _i32p),
         ("broad_phase_mode", ctypes.c_int), ("explicit_pairs", _i32p), ("explicit_pair_count", ctypes.c_int),
+        ("shape_restitution", _f32p),
     ]
 
 
@@ -56,6 +57,7 @@ class OracleXpbdParams(ctypes.Structure):
         ("rigid_contact_con_weighting", ctypes.c_int), ("angular_damping", ctypes.c_float),
         ("joint_linear_relaxation", ctypes.c_float), ("joint_angular_relaxation", ctypes.c_float),
         ("joint_linear_compliance", ctypes.c_float), ("joint_angular_compliance", ctypes.c_float),
+        ("enable_restitution", ctypes.c_int),
     ]
 
 
@@ -137,6 +139,7 @@ class Oracle:
         m.shape_group, m.shape_world = i32("shape_collision_group"), i32("shape_world")
         m.shape_mu, m.shape_mu_torsional = f32("shape_material_mu"), f32("shape_material_mu_torsional")
         m.shape_mu_rolling = f32("shape_material_mu_rolling")
+        m.shape_restitution = f32("shape_material_restitution")
         m.segment_count = len(h["segment_start"]) - 1
         m.segment_start, m.segment_shape = i32("segment_start"), i32("segment_shape")
         m.filter_count = len(h["shape_filter_pairs"])
@@ -253,15 +256,16 @@ class Oracle:
              angular_damping: float = 0.0, joint_linear_relaxation: float = 0.7,
              joint_angular_relaxation: float = 0.4, joint_linear_compliance: float = 0.0,
              joint_angular_compliance: float = 0.0, joint_f=None, joint_target_pos=None, joint_target_vel=None,
-             precision: str = "f32", report: bool = False):
+             precision: str = "f32", report: bool = False, enable_restitution: bool = False):
         """One SolverXPBD.step; returns (body_q_out, body_qd_out), plus (contact_force (n, 6), body_parent_f
-        (B, 6)) with ``report`` (FP32 only)."""
+        (B, 6)) with ``report`` (FP32 only).  ``enable_restitution``: the rigid restitution pass after the
+        iterations (solver_xpbd.py:680-764)."""
         h = self.model.host
         dtp = np.float32 if precision == "f32" else np.float64
         tp = _f32p if precision == "f32" else _f64p
         p = OracleXpbdParams(iterations, rigid_contact_relaxation, int(rigid_contact_con_weighting), angular_damping,
                              joint_linear_relaxation, joint_angular_relaxation, joint_linear_compliance,
-                             joint_angular_compliance)
+                             joint_angular_compliance, int(bool(enable_restitution)))
         q = np.ascontiguousarray(body_q, dtype=dtp)
         qd = np.ascontiguousarray(body_qd, dtype=dtp)
         f = np.ascontiguousarray(body_f, dtype=dtp)

@@ -87,6 +87,7 @@ typedef struct OracleModel {
   int broad_phase_mode;       /* 0 sap, 1 nxn (and explicit with the model's pairs), 2 explicit list */
   const int* explicit_pairs;  /* sorted (min, max) */
   int explicit_pair_count;
+  const float* shape_restitution; /* S: shape_material_restitution (null: 0 for every shape) */
 } OracleModel;
 
 typedef struct OracleXpbdParams {
@@ -98,6 +99,7 @@ typedef struct OracleXpbdParams {
   float joint_angular_relaxation;
   float joint_linear_compliance;
   float joint_angular_compliance;
+  int enable_restitution; /* SolverXPBD(enable_restitution=True): apply_rigid_restitution after the iterations */
 } OracleXpbdParams;
 
 /* Broad phase only: writes canonical (min,max) pairs in discovery order, returns the pair count

@@ -11,6 +11,8 @@
 //   solve_body_joints (+ positional/angular)    newton/_src/solvers/xpbd/kernels.py:1441-1943, 1982-2060
 //   update_joint_axis_limits / weighted target  newton/_src/solvers/xpbd/kernels.py:1005-1030
 //   copy_kinematic_body_state_kernel            newton/_src/solvers/xpbd/kernels.py:18-31
+//   apply_rigid_restitution                     newton/_src/solvers/xpbd/kernels.py:2472-2615
+//   apply_body_delta_velocities                 newton/_src/solvers/xpbd/kernels.py:882-887
 // Per-body accumulations (deltas, contact counts, joint forces) are summed in the order contacts and
 // joints are given; that order is the canonical one the HIP kernels reduce in.
 #pragma once
@@ -628,6 +630,87 @@ void iterate(const XModel<T>& X, const OracleModel* m, const OracleXpbdParams* p
   }
 }
 
+// apply_rigid_restitution (kernels.py:2472-2615) and apply_body_delta_velocities (kernels.py:882-887), run
+// after the iterations when the solver is created with enable_restitution (solver_xpbd.py:719-764).  The
+// contact points are taken on the start-of-step poses `prev` (point + offset, so on the surface), the
+// approach velocity from the start-of-step velocities plus one step of gravity, the separation velocity from
+// the solved ones in `state`; the velocity change along the normal that turns the separation into -e x the
+// approach is split over both bodies by their effective inverse masses.  Per-body deltas are summed in
+// contact order (the reference adds them atomically) and added to every body's velocity (+0 for bodies
+// without an active contact); kinematic bodies, whose deltas are zero, are restored by the copy that follows.
+template <class T>
+void restitution(const XModel<T>& X, const OracleModel* m, const std::vector<Body<T>>& prev,
+                 std::vector<Body<T>>& state, const std::vector<ContactIn<T>>& contacts, T dt) {
+  const int B = m->body_count;
+  const V3<T> zero{T(0), T(0), T(0)};
+  std::vector<Spatial<T>> deltas(B, Spatial<T>{zero, zero});
+  M3<T> mzero;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) mzero.m[i][j] = T(0);
+  auto rest = [&](int s) { return m->shape_restitution ? T(m->shape_restitution[s]) : T(0); };
+  for (const ContactIn<T>& c : contacts) {
+    const int sa = c.shape0, sb = c.shape1;
+    if (sa == sb) continue;
+    int ba = -1, bb = -1, nz = 0;
+    T e = T(0);
+    if (sa >= 0) { nz += 1; e += rest(sa); ba = m->shape_body[sa]; }
+    if (sb >= 0) { nz += 1; e += rest(sb); bb = m->shape_body[sb]; }
+    if (nz > 0) e /= T(nz);
+    if (ba == bb) continue;
+    Xf<T> Xa = xf_identity<T>(), Xb = xf_identity<T>();
+    V3<T> com_a = zero, com_b = zero;
+    T mia = T(0), mib = T(0);
+    M3<T> Ia = mzero, Ib = mzero;
+    if (ba >= 0) { Xa = prev[ba].q; mia = X.inv_m_eff[ba]; Ia = X.inv_I_eff[ba]; com_a = X.com[ba]; }
+    if (bb >= 0) { Xb = prev[bb].q; mib = X.inv_m_eff[bb]; Ib = X.inv_I_eff[bb]; com_b = X.com[bb]; }
+    const V3<T> bx_a = xf_point(Xa, c.p0 + c.o0);
+    const V3<T> bx_b = xf_point(Xb, c.p1 + c.o1);
+    const T thickness = c.m0 + c.m1;
+    const V3<T> n = c.n;
+    const T d = dot(n, bx_b - bx_a) - thickness;
+    if (d >= T(0)) continue;
+    const V3<T> r_a = bx_a - xf_point(Xa, com_a);
+    const V3<T> r_b = bx_b - xf_point(Xb, com_b);
+    V3<T> va = zero, vb = zero, va_new = zero, vb_new = zero, rxn_a = zero, rxn_b = zero;
+    T inv_mass = T(0);
+    auto grav = [&](int b) {
+      const int wi = m->body_world[b];
+      const int gi = wi > 0 ? wi : 0;
+      return V3<T>{T(m->gravity[3 * gi]), T(m->gravity[3 * gi + 1]), T(m->gravity[3 * gi + 2])};
+    };
+    if (ba >= 0) {
+      va = (cross(prev[ba].w, r_a) + prev[ba].v) + grav(ba) * dt;
+      va_new = cross(state[ba].w, r_a) + state[ba].v;
+      rxn_a = qrot_inv(Xa.q, cross(r_a, n));
+      inv_mass = inv_mass + (mia + dot(rxn_a, mv(Ia, rxn_a)));
+    }
+    if (bb >= 0) {
+      vb = (cross(prev[bb].w, r_b) + prev[bb].v) + grav(bb) * dt;
+      vb_new = cross(state[bb].w, r_b) + state[bb].v;
+      rxn_b = qrot_inv(Xb.q, cross(r_b, n));
+      inv_mass = inv_mass + (mib + dot(rxn_b, mv(Ib, rxn_b)));
+    }
+    if (inv_mass == T(0)) continue;
+    const T rel_old = dot(n, vb - va);
+    const T rel_new = dot(n, vb_new - va_new);
+    if (rel_old >= T(0)) continue;
+    const T dv = (-rel_new - e * rel_old) / inv_mass;
+    if (ba >= 0) {
+      const T dva = -dv;
+      deltas[ba].lin = deltas[ba].lin + (n * mia) * dva;
+      deltas[ba].ang = deltas[ba].ang + qrot(Xa.q, mv(Ia, rxn_a) * dva);
+    }
+    if (bb >= 0) {
+      deltas[bb].lin = deltas[bb].lin + (n * mib) * dv;
+      deltas[bb].ang = deltas[bb].ang + qrot(Xb.q, mv(Ib, rxn_b) * dv);
+    }
+  }
+  for (int b = 0; b < B; ++b) {
+    state[b].v = state[b].v + deltas[b].lin;
+    state[b].w = state[b].w + deltas[b].ang;
+  }
+}
+
 // contact_force (per contact, lin + ang) and body_parent_f (per body): update_contacts and the reported
 // parent-joint wrench (solver_xpbd.py:632-655, 767-821; kernels.py:2364-2435), when given.
 template <class T>
@@ -656,6 +739,7 @@ void xpbd_step(const OracleModel* m, const OracleXpbdParams* p, std::vector<Body
       body_parent_f[id_c] = Spatial<T>{jimp[j].lin * inv_dt, jimp[j].ang * inv_dt};
     }
   }
+  if (p->enable_restitution && has_contacts && m->body_count) restitution(X, m, in, state, contacts, dt);
   for (int b = 0; b < m->body_count; ++b)
     if (m->body_flags[b] & KINEMATIC) state[b] = in[b];
 }

@@ -1,6 +1,7 @@
 """Graph capture of the substep: collide + XPBD step (plain; deterministic with contact matching and
 reported forces; worlds of uneven size, whose step launches its size tiers concurrently on side streams forked
-from and joined into the capturing stream) captured once with torch.cuda.CUDAGraph and replayed; every replay,
+from and joined into the capturing stream; the same with per-shape restitution, whose pass follows the joined
+tiers) captured once with torch.cuda.CUDAGraph and replayed; every replay,
 with unrelated eager work and a synchronisation between replays, equals the eager substep bit for bit.  The
 library records no memset nodes (scratch is cleared by its own kernels), so replays after the first are
 checked too.
@@ -20,15 +21,18 @@ pytestmark = pytest.mark.gpu
 DT = 1.0 / 240.0
 
 
-def _setup(dev, report, uneven=False):
+def _setup(dev, report, uneven=False, restitution=False):
     model = scenes.uneven_worlds(12, seed=8, device=dev) if uneven else scenes.mixed_worlds(16, seed=5, device=dev)
+    if restitution:
+        e = np.random.default_rng(4).uniform(0.0, 0.9, model.shape_count).astype(np.float32)
+        model.shape_material_restitution.copy_(torch.from_numpy(e))
     if report:
         model.request_contact_attributes("force")
         model.request_state_attributes("body_parent_f")
         pipe = CollisionPipeline(model, broad_phase="sap", contact_matching="latest", contact_report=True)
     else:
         pipe = CollisionPipeline(model, broad_phase="sap")
-    solver = SolverXPBD(model, iterations=10)
+    solver = SolverXPBD(model, iterations=10, enable_restitution=restitution)
     return model, pipe, solver
 
 
@@ -39,13 +43,14 @@ def _outputs(c, s, report):
     return out
 
 
-@pytest.mark.parametrize("report,uneven", [(False, False), (True, False), (False, True)],
-                         ids=["plain", "matching_and_forces", "uneven_tiers"])
-def test_graph_replays_equal_eager(cuda_device, report, uneven):
-    print(f"[graph] report={report} uneven={uneven}", flush=True)
+@pytest.mark.parametrize("report,uneven,restitution",
+                         [(False, False, False), (True, False, False), (False, True, False), (False, True, True)],
+                         ids=["plain", "matching_and_forces", "uneven_tiers", "uneven_tiers_restitution"])
+def test_graph_replays_equal_eager(cuda_device, report, uneven, restitution):
+    print(f"[graph] report={report} uneven={uneven} restitution={restitution}", flush=True)
     steps = 12
     # eager reference
-    model, pipe, solver = _setup(cuda_device, report, uneven)
+    model, pipe, solver = _setup(cuda_device, report, uneven, restitution)
     if uneven:
         counts = model.host["world_body_count"][:-1]
         assert counts.max() > 64 and (counts <= 64).any(), "the scene should launch its step in several size tiers"
@@ -63,7 +68,7 @@ def test_graph_replays_equal_eager(cuda_device, report, uneven):
         torch.cuda.synchronize()
         ref.append([t.cpu().numpy() for t in _outputs(c, b, report)])
     # the same substep captured and replayed (fresh objects, same initial state)
-    model, pipe, solver = _setup(cuda_device, report, uneven)
+    model, pipe, solver = _setup(cuda_device, report, uneven, restitution)
     c = pipe.contacts()
     a, b = model.state(), model.state()
 

@@ -1,0 +1,187 @@
+"""Restitution pass on the GPU (SolverXPBD(enable_restitution=True)) through the C-ABI, against the oracle.
+
+* One step from identical inputs with per-shape restitution drawn at random: the reference-order kernels
+  (step + restitution pass) equal the oracle bit for bit; the world-frame step followed by the pass agrees within
+  the step bar (1e-5 relative; 1e-5 m / 1e-5 m per substep floors).  Scenes: 8 worlds of 32 primitives (one wave
+  a world), two worlds of 280-400 bodies (the pass walks its chunks once per group of 256 bodies), and worlds
+  with a kinematic platform that moving bodies strike (zero effective inverse mass: the platform's velocity
+  passes through unchanged).
+* The reporting step (contact forces requested) applies the same pass: the same state bit for bit.
+* The reference's known answer (test_physics_validation.py:644-722): a sphere dropped from 1 m rebounds to
+  e^2 m within 1 % for e = 0.5 and 0.8, the ratio of the two within 1 % of 2.56; both arithmetic modes.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from newton_custom_contact_amd import CollisionPipeline, scenes
+from newton_custom_contact_amd._types import transform
+from newton_custom_contact_amd.builder import ModelBuilder
+from newton_custom_contact_amd.solvers import SolverXPBD
+from oracle import Oracle
+from parity_helpers import assert_step_close, bit_equal, max_abs_diff
+
+pytestmark = pytest.mark.gpu
+
+DT = 1.0 / 240.0
+
+
+def _set_restitution(model, seed):
+    e = np.random.default_rng(seed).uniform(0.0, 0.9, model.shape_count).astype(np.float32)
+    model.host["shape_material_restitution"] = e
+    model.shape_material_restitution.copy_(torch.from_numpy(e))
+    return model
+
+
+def _platform_worlds(dev, worlds: int = 6, seed: int = 11):
+    """Per world: a kinematic box platform sliding sideways at 0.2 m above the ground and 6 spheres / boxes
+    thrown at it and at the ground."""
+    rng = np.random.default_rng(seed)
+    b = ModelBuilder()
+    cfg = ModelBuilder.ShapeConfig(mu=0.4, restitution=0.5)
+    b.add_ground_plane(cfg=cfg)
+    vel = []
+    for w in range(worlds):
+        b.begin_world()
+        plat = b.add_body(xform=transform((0.0, 0.0, 0.2), (0.0, 0.0, 0.0, 1.0)), is_kinematic=True)
+        b.add_shape_box(plat, hx=0.6, hy=0.6, hz=0.05, cfg=cfg)
+        vel.append((0.3, -0.1, 0.0, 0.0, 0.0, 0.2))
+        for k in range(6):
+            p = (rng.uniform(-0.9, 0.9), rng.uniform(-0.9, 0.9), rng.uniform(0.4, 0.9))
+            body = b.add_body(xform=transform(p, (0.0, 0.0, 0.0, 1.0)))
+            if k % 2:
+                b.add_shape_sphere(body, radius=0.05 + 0.03 * rng.random(), cfg=cfg)
+            else:
+                b.add_shape_box(body, hx=0.06, hy=0.05, hz=0.04, cfg=cfg)
+            vel.append((rng.normal(0, 0.3), rng.normal(0, 0.3), -rng.uniform(1.0, 3.0), *rng.normal(0, 2.0, 3)))
+        b.end_world()
+    model = b.finalize(dev)
+    qd = np.asarray(vel, np.float32)
+    model.host["body_qd"] = qd
+    model.body_qd.copy_(torch.from_numpy(qd))
+    return model
+
+
+SCENES = [
+    ("mixed_8w", lambda dev: _set_restitution(scenes.mixed_worlds(8, seed=3, device=dev), 1)),
+    ("uneven_2w_large", lambda dev: _set_restitution(scenes.uneven_worlds(2, seed=4, lo=280, hi=400, device=dev), 2)),
+    ("kinematic_platform_6w", _platform_worlds),
+]
+
+
+def _oracle_states(oracle, model, n_substeps, every):
+    h = model.host
+    q, qd = h["body_q"].copy(), h["body_qd"].copy()
+    f = np.zeros_like(qd)
+    out = [(q.copy(), qd.copy())]
+    for i in range(n_substeps):
+        c = oracle.collide(q)
+        q, qd = oracle.step(q, qd, f, c, DT, iterations=10, enable_restitution=True)
+        if (i + 1) % every == 0:
+            out.append((q.copy(), qd.copy()))
+    return out
+
+
+@pytest.mark.parametrize("name,make", SCENES, ids=[s[0] for s in SCENES])
+def test_restitution_step_matches_oracle(cuda_device, name, make):
+    print(f"[restitution parity] {name}", flush=True)
+    model = make(cuda_device)
+    oracle = Oracle(model)
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    kin = (model.host["body_flags"] & 2) != 0
+    err = 0.0
+    active = 0
+    for q, qd in _oracle_states(oracle, model, n_substeps=120, every=30):
+        ref_c = oracle.collide(q)
+        state = model.state()
+        state.body_q.copy_(torch.from_numpy(np.ascontiguousarray(q, np.float32)))
+        state.body_qd.copy_(torch.from_numpy(np.ascontiguousarray(qd, np.float32)))
+        c = pipe.contacts()
+        pipe.collide(state, c)
+        q_ref, qd_ref = oracle.step(q, qd, np.zeros_like(qd), ref_c, DT, iterations=10, enable_restitution=True)
+        _, qd_plain = oracle.step(q, qd, np.zeros_like(qd), ref_c, DT, iterations=10)
+        active += int(np.any(qd_plain != qd_ref))
+        for ref_arith in (True, False):
+            solver = SolverXPBD(model, iterations=10, reference_arithmetic=ref_arith, enable_restitution=True)
+            out = model.state()
+            solver.step(state, out, None, c, DT)
+            torch.cuda.synchronize()
+            qg, qdg = out.body_q.cpu().numpy(), out.body_qd.cpu().numpy()
+            if ref_arith:
+                assert bit_equal(qg, q_ref) and bit_equal(qdg, qd_ref), "reference-order step + pass differs"
+            else:
+                assert_step_close(qg, qdg, q_ref, qd_ref, DT, name)
+                err = max(err, max_abs_diff(qdg, qd_ref))
+            if kin.any():
+                np.testing.assert_array_equal(qdg[kin], qd[kin])
+    print(name, f"states where the pass changed a velocity: {active}; world-frame largest |dqd| {err:.3g}",
+          flush=True)
+    assert active > 0
+
+
+def test_reporting_step_applies_the_pass(cuda_device):
+    print("[restitution] reporting step", flush=True)
+    model = _set_restitution(scenes.mixed_worlds(8, seed=3, device=cuda_device), 1)
+    model.request_contact_attributes("force")
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    solver = SolverXPBD(model, iterations=10, reference_arithmetic=True, enable_restitution=True)
+    s0, s1 = model.state(), model.state()
+    c = pipe.contacts()
+    assert c.force is not None
+    for _ in range(60):
+        pipe.collide(s0, c)
+        solver.step(s0, s1, None, c, DT)
+        s0, s1 = s1, s0
+    pipe.collide(s0, c)
+    oracle = Oracle(model)
+    q, qd = s0.body_q.cpu().numpy(), s0.body_qd.cpu().numpy()
+    ref_c = oracle.collide(q)
+    q_ref, qd_ref = oracle.step(q, qd, np.zeros_like(qd), ref_c, DT, iterations=10, enable_restitution=True)
+    solver.step(s0, s1, None, c, DT)
+    torch.cuda.synchronize()
+    assert bit_equal(s1.body_q.cpu().numpy(), q_ref) and bit_equal(s1.body_qd.cpu().numpy(), qd_ref)
+
+
+def _bounce(dev, e: float, reference_arithmetic: bool):
+    h_drop, radius, dt = 1.0, 0.05, 1e-3
+    cfg = ModelBuilder.ShapeConfig(mu=0.0, restitution=e, ke=1e4, kd=100.0, kf=0.0, margin=0.001, gap=0.0)
+    b = ModelBuilder(up_axis="Y", gravity=-10.0)
+    b.add_ground_plane(cfg=cfg)
+    body = b.add_body(xform=transform((0.0, radius + h_drop, 0.0), (0.0, 0.0, 0.0, 1.0)))
+    b.add_shape_sphere(body, radius=radius, cfg=cfg)
+    model = b.finalize(dev)
+    pipe = CollisionPipeline(model)
+    solver = SolverXPBD(model, iterations=10, angular_damping=0.0, enable_restitution=True,
+                        reference_arithmetic=reference_arithmetic)
+    c = pipe.contacts()
+    s0, s1 = model.state(), model.state()
+    steps = int(3.0 * math.sqrt(2.0 * h_drop / 10.0) / dt)
+    ys = torch.empty(steps, device=dev)
+    for i in range(steps):
+        s0.clear_forces()
+        pipe.collide(s0, c)
+        solver.step(s0, s1, None, c, dt)
+        s0, s1 = s1, s0
+        ys[i] = s0.body_q[0, 1]
+    y = ys.cpu().numpy().astype(np.float64)
+    impact = next(i for i in range(1, steps - 1) if y[i] < y[i - 1] and y[i] <= y[i + 1])
+    return float(y[impact:].max() - radius), float(y.min())
+
+
+@pytest.mark.parametrize("reference_arithmetic", [False, True], ids=["world_frame", "reference_order"])
+def test_bounce_height_known_answer_gpu(cuda_device, reference_arithmetic):
+    print(f"[restitution] bounce, reference_arithmetic={reference_arithmetic}", flush=True)
+    heights = {}
+    for e in (0.5, 0.8):
+        h, lowest = _bounce(cuda_device, e, reference_arithmetic)
+        assert lowest > -0.01
+        assert abs(h - e * e) <= 0.01 * e * e, (e, h)
+        heights[e] = h
+    ratio = heights[0.8] / heights[0.5]
+    print(heights, ratio, flush=True)
+    assert abs(ratio - 2.56) <= 0.01 * 2.56
