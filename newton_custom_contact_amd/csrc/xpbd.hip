@@ -120,16 +120,19 @@ __device__ __forceinline__ float angular_correction(float err, float derr, qf qa
   return dl;
 }
 
-// Per-world LDS views of body data (local body index).
+// Per-world views of body data (local body index).  The reference-order kernels copy the constant body data
+// into LDS (inverse mass and inertia already effective); the world-frame kernels, whose LDS holds the per-body
+// frames instead, read it from the model arrays (L2-resident) and zero the kinematic bodies' on the way.
 struct WorldLds {
   float* q;      // 7
   float* qd;     // 6 (slot nb: zero, the static side of a contact)
   float* com;    // 3
-  float* inv_m;  // 1 (effective)
-  float* inv_I;  // 9 (effective)
+  float* inv_m;  // 1 (effective when `eff`)
+  float* inv_I;  // 9 (effective when `eff`)
   float* I;      // 9
   int* flags;    // (slot nb: 0)
   float4* F;     // world-frame arithmetic: fastm::FRAME_F4 per body, slot nb static
+  bool eff;      // inv_m / inv_I hold effective values (LDS copies)
 };
 
 __device__ __forceinline__ xform lds_q(const WorldLds& L, int b) {
@@ -143,6 +146,21 @@ __device__ __forceinline__ m33 lds_m33(const float* a) {
 #pragma unroll
     for (int j = 0; j < 3; ++j) r.m[i][j] = a[3 * i + j];
   return r;
+}
+// Effective inverse mass and inertia of local body b (solver.py:160-175: zero for kinematic bodies).
+__device__ __forceinline__ float body_inv_m(const WorldLds& L, int b) {
+  return (!L.eff && (L.flags[b] & BODY_KINEMATIC)) ? 0.0f : L.inv_m[b];
+}
+__device__ __forceinline__ m33 body_inv_I(const WorldLds& L, int b) {
+  if (!L.eff && (L.flags[b] & BODY_KINEMATIC)) {
+    m33 z;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) z.m[i][j] = 0.0f;
+    return z;
+  }
+  return lds_m33(L.inv_I + 9 * b);
 }
 
 // solve_body_contact_positions (kernels.py:2062-2299) for one contact; bodies are local indices or -1.
@@ -162,10 +180,10 @@ __device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f
   m33 Ia = Z, Ib = Z;
   f3 com_a = zero, com_b = zero, om_a = zero, om_b = zero;
   if (la >= 0) {
-    com_a = load3(L.com + 3 * la); mia = L.inv_m[la]; Ia = lds_m33(L.inv_I + 9 * la); om_a = load3(L.qd + 6 * la + 3);
+    com_a = load3(L.com + 3 * la); mia = body_inv_m(L, la); Ia = body_inv_I(L, la); om_a = load3(L.qd + 6 * la + 3);
   }
   if (lb >= 0) {
-    com_b = load3(L.com + 3 * lb); mib = L.inv_m[lb]; Ib = lds_m33(L.inv_I + 9 * lb); om_b = load3(L.qd + 6 * lb + 3);
+    com_b = load3(L.com + 3 * lb); mib = body_inv_m(L, lb); Ib = body_inv_I(L, lb); om_b = load3(L.qd + 6 * lb + 3);
   }
   // world centres of mass (the reference transforms them again for the friction rows; same values)
   const f3 wcom_a = xpoint(Xa, com_a), wcom_b = xpoint(Xb, com_b);
@@ -291,16 +309,16 @@ __device__ bool joint_solve(const NmcModelDesc& d, const WorldLds& L, int j, int
     pose_p = lds_q(L, lp);
     X_wp = xmul(pose_p, X_wp);
     com_p = load3(L.com + 3 * lp);
-    m_inv_p = L.inv_m[lp];
-    I_inv_p = lds_m33(L.inv_I + 9 * lp);
+    m_inv_p = body_inv_m(L, lp);
+    I_inv_p = body_inv_I(L, lp);
     vel_p = load3(L.qd + 6 * lp);
     omega_p = load3(L.qd + 6 * lp + 3);
   }
   xform pose_c = lds_q(L, lc);
   xform X_wc = xmul(pose_c, X_cj);
   f3 com_c = load3(L.com + 3 * lc);
-  float m_inv_c = L.inv_m[lc];
-  m33 I_inv_c = lds_m33(L.inv_I + 9 * lc);
+  float m_inv_c = body_inv_m(L, lc);
+  m33 I_inv_c = body_inv_I(L, lc);
   f3 vel_c = load3(L.qd + 6 * lc), omega_c = load3(L.qd + 6 * lc + 3);
   if (m_inv_p == 0.0f && m_inv_c == 0.0f) return false;
   f3 ldp = zero, adp = zero, ldc = zero, adc = zero;
@@ -445,10 +463,11 @@ struct XpbdArgs {
   NmcXpbdReport rep;  // XM_REPORT: contact impulses, joint impulses, body_parent_f
 };
 
-// Floats of the contribution region: 12 per slot of a solve chunk, at least the 6 per body the body forces
-// take there before the iterations.
-__host__ __device__ inline int xpbd_contrib_floats(int chunk, int max_bodies) {
-  const int a = 12 * chunk, b = 6 * (max_bodies > 0 ? max_bodies : 1);
+// Floats of the contribution region: per slot of a solve chunk 12 (da, db), or 10 in the split world-frame
+// kernels ((da.lin, active) as one float4, then da.ang and db.ang), at least the 6 per body the body forces take
+// there before the iterations.
+__host__ __device__ inline int xpbd_contrib_floats(int chunk, int max_bodies, bool split = false) {
+  const int a = (split ? 10 : 12) * chunk, b = 6 * (max_bodies > 0 ? max_bodies : 1);
   return a > b ? a : b;
 }
 
@@ -522,7 +541,7 @@ __device__ __forceinline__ void frame_from_state(const WorldLds& L, int lb) {
   r1.w = X.p.y;
   r2.w = X.p.z;
   const f3 c = fpoint(r0, r1, r2, load3(L.com + 3 * lb));
-  frame_store(L.F + FRAME_F4 * lb, r0, r1, r2, c, lds_m33(L.inv_I + 9 * lb), L.inv_m[lb]);
+  frame_store(L.F + FRAME_F4 * lb, r0, r1, r2, c, body_inv_I(L, lb), body_inv_m(L, lb));
 }
 
 // solve_body_contact_positions (kernels.py:2062-2299) for one contact in world-frame arithmetic: the normal
@@ -626,7 +645,7 @@ __device__ __forceinline__ void apply_fast(const WorldLds& L, int lb, const Spat
   const float inv_m = cm.w;
   if (inv_m == 0.0f) return;
   const float4 r0 = F[0], r1 = F[1], r2 = F[2];
-  const m33 J = lds_m33(L.inv_I + 9 * lb), I = lds_m33(L.I + 9 * lb);
+  const m33 J = body_inv_I(L, lb), I = lds_m33(L.I + 9 * lb);
   const float weight = (has_weights && inv_weight > 0.0f) ? rcp(inv_weight) : 1.0f;
   const f3 dp = delta.lin * (inv_m * weight);
   const f3 dq = delta.ang * weight;
@@ -715,10 +734,10 @@ __device__ void contact_features(const WorldLds& L, const CReg& r, int cnt_a, in
     f3 r_b = bx_b - xpoint(Xb, com_b);
     f3 ang_a = -cross(r_a, r.n);
     f3 ang_b = cross(r_b, r.n);
-    float mia = la >= 0 ? L.inv_m[la] : 0.0f;
-    float mib = lb >= 0 ? L.inv_m[lb] : 0.0f;
-    m33 Ia = la >= 0 ? lds_m33(L.inv_I + 9 * la) : Z;
-    m33 Ib = lb >= 0 ? lds_m33(L.inv_I + 9 * lb) : Z;
+    float mia = la >= 0 ? body_inv_m(L, la) : 0.0f;
+    float mib = lb >= 0 ? body_inv_m(L, lb) : 0.0f;
+    m33 Ia = la >= 0 ? body_inv_I(L, la) : Z;
+    m33 Ib = lb >= 0 ? body_inv_I(L, lb) : Z;
     float w = 0.0f;
     w += len_sq(-r.n) * mia;
     w += len_sq(r.n) * mib;
@@ -789,8 +808,13 @@ __device__ __forceinline__ bool warm_contrib(const WorldLds& L, const CReg& r, f
 // FAST: contact rows and body updates in world-frame arithmetic (xpbd_fast.h); otherwise the reference's
 // operation order, bit-identical to a sequential CPU sweep compiled without contraction.  ART: the model has
 // joints other than FREE / DISTANCE (without them the joint solve is not compiled in, which frees registers).
+// The plain one-wave world-frame kernel without the joint solve is held to 128 VGPRs: with its LDS below 10 KiB a
+// world (32 bodies), 16 worlds are then resident per CU (4 waves per SIMD) instead of 12, so 4096 worlds run in
+// one round on 256 CUs.
 template <int BS, int CPL, int BPL, int MODE, bool FAST, bool ART>
-__global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
+__global__ void __launch_bounds__(BS)
+__attribute__((amdgpu_waves_per_eu((BS == 64 && FAST && !ART && MODE == XM_PLAIN) ? 4 : 1)))
+xpbd_world_kernel(XpbdArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_articulated;
   __shared__ float s_red[BS / 64];
@@ -822,14 +846,20 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   L.F = reinterpret_cast<float4*>(p); p += sizeof(float4) * fastm::FRAME_F4 * (mb + 1);
   L.q = reinterpret_cast<float*>(p); p += sizeof(float) * 7 * mb;
   L.qd = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * (mb + 1);
-  L.com = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * mb;
-  L.inv_m = reinterpret_cast<float*>(p); p += sizeof(float) * mb;
-  L.inv_I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
   if (FAST) {
-    // the body-frame inertia (apply only) is read from the model arrays
+    // constant body data from the model arrays (the per-body frames carry what the contact rows need); the
+    // smaller LDS footprint keeps 16 one-wave worlds resident per CU
+    L.com = const_cast<float*>(d.body_com) + 3ll * b0;
+    L.inv_m = const_cast<float*>(d.body_inv_mass) + b0;
+    L.inv_I = const_cast<float*>(d.body_inv_inertia) + 9ll * b0;
     L.I = const_cast<float*>(d.body_inertia) + 9ll * b0;
+    L.eff = false;
   } else {
+    L.com = reinterpret_cast<float*>(p); p += sizeof(float) * 3 * mb;
+    L.inv_m = reinterpret_cast<float*>(p); p += sizeof(float) * mb;
+    L.inv_I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
     L.I = reinterpret_cast<float*>(p); p += sizeof(float) * 9 * mb;
+    L.eff = true;
   }
   L.flags = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
   int* seg = reinterpret_cast<int*>(p); p += sizeof(int) * (mb + 1);
@@ -838,7 +868,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   p = smem + ((p - smem + 15) & ~static_cast<ptrdiff_t>(15));  // 16-byte aligned contribution slots
   // contribution slots; before the iterations the body forces use their last 6 mb floats (the report mode,
   // which uses them during the iterations, has them apart)
-  const int cfl = xpbd_contrib_floats(CH, mb);
+  const int cfl = xpbd_contrib_floats(CH, mb, SPLIT_SHAPE);
   float* contrib = reinterpret_cast<float*>(p); p += sizeof(float) * cfl;
   float* fl = contrib + (cfl - 6 * mb);
   if (MODE == XM_REPORT) { fl = reinterpret_cast<float*>(p); p += sizeof(float) * 6 * mb; }
@@ -890,9 +920,11 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
     int b = b0 + lb;
     bool kin = (d.body_flags[b] & BODY_KINEMATIC) != 0;
     L.flags[lb] = d.body_flags[b];
-    L.inv_m[lb] = kin ? 0.0f : d.body_inv_mass[b];
-    for (int k = 0; k < 9; ++k) L.inv_I[9 * lb + k] = kin ? 0.0f : d.body_inv_inertia[9 * b + k];
-    for (int k = 0; k < 3; ++k) L.com[3 * lb + k] = d.body_com[3 * b + k];
+    if (!FAST) {
+      L.inv_m[lb] = kin ? 0.0f : d.body_inv_mass[b];
+      for (int k = 0; k < 9; ++k) L.inv_I[9 * lb + k] = kin ? 0.0f : d.body_inv_inertia[9 * b + k];
+      for (int k = 0; k < 3; ++k) L.com[3 * lb + k] = d.body_com[3 * b + k];
+    }
     for (int k = 0; k < 7; ++k) L.q[7 * lb + k] = q_src[7 * b + k];
     for (int k = 0; k < 6; ++k) L.qd[6 * lb + k] = qd_src[6 * b + k];
     cur[lb] = 0;
@@ -1178,13 +1210,14 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
   // World-frame kernels, one body per lane: LPB lanes walk each body's list (entries k, k + LPB, ... in lane k
   // of the body's group) and the partial sums combine by a fixed shuffle tree, so small worlds keep their
   // lanes busy.  LPB depends on the world's body count only, so every launch shape sums in the same order
-  // (deterministic, not the sequential order).  Contribution slots then hold (da.lin, active) | (da.ang, 0) |
-  // (db.ang, 0) as float4, db.lin = -da.lin, zeros when inactive.
+  // (deterministic, not the sequential order).  Contribution slots then hold (da.lin, active) as a float4 and
+  // da.ang, db.ang in a second array, db.lin = -da.lin, zeros when inactive.
   constexpr bool SPLIT = SPLIT_SHAPE;
   const int lpb = SPLIT ? (nb <= 4 ? 16 : nb <= 8 ? 8 : nb <= 16 ? 4 : nb <= 32 ? 2 : 1) : 1;
   const int sub = SPLIT ? (tid & (lpb - 1)) : 0;
   const int walk_body = SPLIT ? tid / lpb : tid;
-  float4* const contrib4 = reinterpret_cast<float4*>(contrib);
+  float4* const contrib4 = reinterpret_cast<float4*>(contrib);  // split: (da.lin, active) per slot
+  float* const contrib_ang = contrib + 4 * CH;                     // split: da.ang, db.ang per slot
 
   // pass 0: the step (XM_WARM: warm-start pass first, warm_iterations iterations); pass 1 (XM_WARM
   // only, when the world's residual exceeds the tolerance): cold solve from the integrated state
@@ -1266,9 +1299,10 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
               }
               if (SPLIT) {
                 // every slot written (zeros when inactive): the walk needs no separate flag
-                contrib4[3 * slot] = on ? make_float4(da.lin.x, da.lin.y, da.lin.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                contrib4[3 * slot + 1] = on ? make_float4(da.ang.x, da.ang.y, da.ang.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-                contrib4[3 * slot + 2] = on ? make_float4(db.ang.x, db.ang.y, db.ang.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                const f3 z{0.0f, 0.0f, 0.0f};
+                contrib4[slot] = on ? make_float4(da.lin.x, da.lin.y, da.lin.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                store3(contrib_ang + 6 * slot, on ? da.ang : z);
+                store3(contrib_ang + 6 * slot + 3, on ? db.ang : z);
               }
               if (on) {
                 act = 1;
@@ -1297,8 +1331,8 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
               const unsigned cc = ent >> 1;
               if (cc >= cend) break;
               const int slot = static_cast<int>(cc) - cb;
-              const float4 u = contrib4[3 * slot];
-              const float4 v = contrib4[3 * slot + 1 + (ent & 1u)];
+              const float4 u = contrib4[slot];
+              const f3 v = load3(contrib_ang + 6 * slot + 3 * (ent & 1u));
               const float sgn = (ent & 1u) ? -1.0f : 1.0f;
               acc[0].lin = fastm::axpy(sgn, f3{u.x, u.y, u.z}, acc[0].lin);
               acc[0].ang = acc[0].ang + f3{v.x, v.y, v.z};
@@ -1374,7 +1408,7 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
               apply_fast(L, lb, acc[r], weighted, cnt[r], dt);
             } else {
               BodyState o = apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb),
-                                         L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc[r], weighted, cnt[r], dt);
+                                         body_inv_m(L, lb), body_inv_I(L, lb), acc[r], weighted, cnt[r], dt);
               lds_store_state(L, lb, o);
             }
           }
@@ -1432,8 +1466,8 @@ __global__ void __launch_bounds__(BS) xpbd_world_kernel(XpbdArgs a) {
             if (any) apply_fast(L, lb, acc, false, 0.0f, dt);
           } else {
             BodyState o = any ? apply_deltas(lds_state(L, lb), load3(L.com + 3 * lb), lds_m33(L.I + 9 * lb),
-                                             L.inv_m[lb], lds_m33(L.inv_I + 9 * lb), acc, false, 0.0f, dt)
-                              : apply_zero_delta(lds_state(L, lb), load3(L.com + 3 * lb), L.inv_m[lb]);
+                                             body_inv_m(L, lb), body_inv_I(L, lb), acc, false, 0.0f, dt)
+                              : apply_zero_delta(lds_state(L, lb), load3(L.com + 3 * lb), body_inv_m(L, lb));
             lds_store_state(L, lb, o);
           }
         }
@@ -1546,11 +1580,11 @@ size_t xpbd_lds_bytes_shape(const XpbdShape& sh, int max_bodies, int max_joints,
   const bool split = fast && sh.bpl == 1;
   const int ch = split ? sh.bs : sh.bs * sh.cpl;
   size_t b = sizeof(float4) * 6 * (mb + 1);                           // frames (+ static slot)
-  b += sizeof(float) * (7 * mb + 6 * (mb + 1) + 3 * mb + mb + 9 * mb);  // q, qd (+ static), com, inv_m, inv_I
-  if (!fast) b += sizeof(float) * 9 * mb;                               // I
+  b += sizeof(float) * (7 * mb + 6 * (mb + 1));                         // q, qd (+ static)
+  if (!fast) b += sizeof(float) * (3 * mb + mb + 9 * mb + 9 * mb);      // com, inv_m, inv_I, I
   b += sizeof(int) * 3 * (mb + 1) + sizeof(int) * sh.bs;              // flags, seg, cur, scan
   b = (b + 15) & ~size_t(15);
-  b += sizeof(float) * (size_t)xpbd_contrib_floats(ch, static_cast<int>(mb));
+  b += sizeof(float) * (size_t)xpbd_contrib_floats(ch, static_cast<int>(mb), split);
   if (report) b += sizeof(float) * 6 * mb;
   if (!split) b += sizeof(int) * ch;                                    // active flags
   b += sizeof(float) * 12 * mj + sizeof(int) * mj;                      // joint slots
