@@ -164,7 +164,7 @@ __device__ __forceinline__ m33 body_inv_I(const WorldLds& L, int b) {
 }
 
 // solve_body_contact_positions (kernels.py:2062-2299) for one contact; bodies are local indices or -1.
-__device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f3 o0, f3 o1, f3 n, float m0, float m1,
+__device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f3 o0, f3 o1, f3 n, float thickness,
                               float mu, float mut, float mur, float relax, float dt, Spatial& da, Spatial& db,
                               float& lam_out) {
   const m33 Z = {{{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}}};
@@ -172,7 +172,6 @@ __device__ bool contact_solve(const WorldLds& L, int la, int lb, f3 p0, f3 p1, f
   xform Xb = lb >= 0 ? lds_q(L, lb) : xident();
   f3 bx_a = xpoint(Xa, p0);
   f3 bx_b = xpoint(Xb, p1);
-  float thickness = m0 + m1;
   float d = dot(n, bx_b - bx_a) - thickness;
   if (d >= 0.0f) return false;
   f3 zero{0.0f, 0.0f, 0.0f};
@@ -476,11 +475,15 @@ __host__ __device__ inline int xpbd_contrib_floats(int chunk, int max_bodies, bo
 enum { XM_PLAIN = 0, XM_LOG = 1, XM_PREP = 2, XM_WARM = 3, XM_REPORT = 4 };
 
 // Per-contact data that is constant for the whole step (solve_body_contact_positions inputs).
+// The body indices and the validity flag share one register: (la + 1) | (lb + 1) << 11 | valid << 22 (local
+// indices below 2047; a world holds at most 512 bodies).
 struct CReg {
   f3 p0, p1, o0, o1, n;
-  float m0, m1, mu, mut, mur;
-  int la, lb;  // local body indices, -1 = static
-  int valid;
+  float thick, mu, mut, mur;  // thick = margin0 + margin1
+  int ids;
+  __device__ __forceinline__ int la() const { return (ids & 0x7FF) - 1; }  // local body index, -1 = static
+  __device__ __forceinline__ int lb() const { return ((ids >> 11) & 0x7FF) - 1; }
+  __device__ __forceinline__ bool valid() const { return ((ids >> 22) & 1) != 0; }
 };
 
 // Contact at canonical position j (the world's range in key order); its buffer position is cpos(a.c, j).  A
@@ -490,20 +493,18 @@ __device__ __forceinline__ CReg load_contact(const XpbdArgs& a, int j, int b0, i
   CReg r;
   const int gc = cpos(a.c, j);
   if (gc < 0) {
-    r.valid = 0;
-    r.la = r.lb = -1;
+    r.ids = 0;
     r.p0 = r.p1 = r.o0 = r.o1 = r.n = f3{0.0f, 0.0f, 0.0f};
-    r.m0 = r.m1 = r.mu = r.mut = r.mur = 0.0f;
+    r.thick = r.mu = r.mut = r.mur = 0.0f;
     return r;
   }
   int s0 = a.c.shape0[gc], s1 = a.c.shape1[gc];
   int ba = s0 >= 0 ? d.shape_body[s0] : -1;
   int bb = s1 >= 0 ? d.shape_body[s1] : -1;
-  r.valid = (s0 != s1 && ba != bb) ? 1 : 0;
-  r.la = ba >= 0 ? ba - b0 : -1;
-  r.lb = bb >= 0 ? bb - b0 : -1;
-  if (r.la >= nb) r.la = -1;  // bodies outside this world never appear in its segment
-  if (r.lb >= nb) r.lb = -1;
+  // bodies outside this world never appear in its segment
+  const int la = (ba >= b0 && ba - b0 < nb) ? ba - b0 : -1;
+  const int lb = (bb >= b0 && bb - b0 < nb) ? bb - b0 : -1;
+  r.ids = (la + 1) | ((lb + 1) << 11) | ((s0 != s1 && ba != bb) ? (1 << 22) : 0);
   int nz = 0;
   float mu = 0.0f, mut = 0.0f, mur = 0.0f;
   if (s0 >= 0) { nz += 1; mu += d.shape_mu[s0]; mut += d.shape_mu_torsional[s0]; mur += d.shape_mu_rolling[s0]; }
@@ -515,8 +516,7 @@ __device__ __forceinline__ CReg load_contact(const XpbdArgs& a, int j, int b0, i
   r.o0 = load3(a.c.offset0 + 3 * gc);
   r.o1 = load3(a.c.offset1 + 3 * gc);
   r.n = load3(a.c.normal + 3 * gc);
-  r.m0 = a.c.margin0[gc];
-  r.m1 = a.c.margin1[gc];
+  r.thick = a.c.margin0[gc] + a.c.margin1[gc];
   return r;
 }
 
@@ -551,14 +551,14 @@ __device__ __forceinline__ void frame_from_state(const WorldLds& L, int lb) {
 __device__ __forceinline__ bool contact_solve_fast(const WorldLds& L, const CReg& r, int nb, float relax, float dt,
                                                    Spatial& da, Spatial& db, float& lam_out) {
   using namespace fastm;
-  const int ia = r.la >= 0 ? r.la : nb, ib = r.lb >= 0 ? r.lb : nb;
+  const int ia = r.la() >= 0 ? r.la() : nb, ib = r.lb() >= 0 ? r.lb() : nb;
   const float4* A = L.F + FRAME_F4 * ia;
   const float4* B = L.F + FRAME_F4 * ib;
   const float4 a0 = A[0], a1 = A[1], a2 = A[2], b0 = B[0], b1 = B[1], b2 = B[2];
   f3 bx_a = fpoint(a0, a1, a2, r.p0);
   f3 bx_b = fpoint(b0, b1, b2, r.p1);
   const f3 n = r.n;
-  const float d = dot3(n, bx_b - bx_a) - (r.m0 + r.m1);
+  const float d = dot3(n, bx_b - bx_a) - r.thick;
   if (d >= 0.0f) return false;
   const float4 ca = A[3], cb = B[3], wa0 = A[4], wa1 = A[5], wb0 = B[4], wb1 = B[5];
   const f3 ca3 = xyz(ca), cb3 = xyz(cb);
@@ -720,14 +720,14 @@ __device__ void contact_features(const WorldLds& L, const CReg& r, int cnt_a, in
   float f[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) f[k] = 0.0f;
-  if (r.valid) {
+  if (r.valid()) {
     const f3 zero{0.0f, 0.0f, 0.0f};
-    const int la = r.la, lb = r.lb;
+    const int la = r.la(), lb = r.lb();
     xform Xa = la >= 0 ? lds_q(L, la) : xident();
     xform Xb = lb >= 0 ? lds_q(L, lb) : xident();
     f3 bx_a = xpoint(Xa, r.p0);
     f3 bx_b = xpoint(Xb, r.p1);
-    float dd = dot(r.n, bx_b - bx_a) - (r.m0 + r.m1);
+    float dd = dot(r.n, bx_b - bx_a) - r.thick;
     f3 com_a = la >= 0 ? load3(L.com + 3 * la) : zero;
     f3 com_b = lb >= 0 ? load3(L.com + 3 * lb) : zero;
     f3 r_a = bx_a - xpoint(Xa, com_a);
@@ -776,23 +776,23 @@ __device__ void contact_features(const WorldLds& L, const CReg& r, int cnt_a, in
 
 // Penetration of one contact at the current state: max(0, -d) (0 for invalid contacts).
 __device__ __forceinline__ float contact_penetration(const WorldLds& L, const CReg& r) {
-  if (!r.valid) return 0.0f;
-  xform Xa = r.la >= 0 ? lds_q(L, r.la) : xident();
-  xform Xb = r.lb >= 0 ? lds_q(L, r.lb) : xident();
-  float dd = dot(r.n, xpoint(Xb, r.p1) - xpoint(Xa, r.p0)) - (r.m0 + r.m1);
+  if (!r.valid()) return 0.0f;
+  xform Xa = r.la() >= 0 ? lds_q(L, r.la()) : xident();
+  xform Xb = r.lb() >= 0 ? lds_q(L, r.lb()) : xident();
+  float dd = dot(r.n, xpoint(Xb, r.p1) - xpoint(Xa, r.p0)) - r.thick;
   return -dd > 0.0f ? -dd : 0.0f;
 }
 
 // Warm-start deltas of one contact: its predicted normal multiplier applied along the normal.
 __device__ __forceinline__ bool warm_contrib(const WorldLds& L, const CReg& r, float lam, Spatial& da, Spatial& db) {
-  if (!(r.valid && lam > 0.0f)) return false;
+  if (!(r.valid() && lam > 0.0f)) return false;
   const f3 zero{0.0f, 0.0f, 0.0f};
-  xform Xa = r.la >= 0 ? lds_q(L, r.la) : xident();
-  xform Xb = r.lb >= 0 ? lds_q(L, r.lb) : xident();
+  xform Xa = r.la() >= 0 ? lds_q(L, r.la()) : xident();
+  xform Xb = r.lb() >= 0 ? lds_q(L, r.lb()) : xident();
   f3 bx_a = xpoint(Xa, r.p0);
   f3 bx_b = xpoint(Xb, r.p1);
-  f3 com_a = r.la >= 0 ? load3(L.com + 3 * r.la) : zero;
-  f3 com_b = r.lb >= 0 ? load3(L.com + 3 * r.lb) : zero;
+  f3 com_a = r.la() >= 0 ? load3(L.com + 3 * r.la()) : zero;
+  f3 com_b = r.lb() >= 0 ? load3(L.com + 3 * r.lb()) : zero;
   f3 ang_a = -cross(bx_a - xpoint(Xa, com_a), r.n);
   f3 ang_b = cross(bx_b - xpoint(Xb, com_b), r.n);
   da = Spatial{(-r.n) * lam, ang_a * lam};
@@ -945,7 +945,7 @@ xpbd_world_kernel(XpbdArgs a) {
 #pragma unroll
   for (int jj = 0; jj < CPL; ++jj) {
     int c = jj * BS + tid;
-    creg[jj].valid = 0;
+    creg[jj].ids = 0;
     if (c < nc) creg[jj] = load_contact(a, cstart + c, b0, nb);
   }
   __syncthreads();
@@ -1015,14 +1015,14 @@ xpbd_world_kernel(XpbdArgs a) {
     } else {
       r = load_contact(a, cstart + c, b0, nb);
     }
-    if (r.valid) {
+    if (r.valid()) {
       if (masks) {
         const unsigned bit = 1u << (c & 31);
-        if (r.la >= 0) atomicOr(&cmask[(2 * r.la) * mask_words + (c >> 5)], bit);
-        if (r.lb >= 0) atomicOr(&cmask[(2 * r.lb + 1) * mask_words + (c >> 5)], bit);
+        if (r.la() >= 0) atomicOr(&cmask[(2 * r.la()) * mask_words + (c >> 5)], bit);
+        if (r.lb() >= 0) atomicOr(&cmask[(2 * r.lb() + 1) * mask_words + (c >> 5)], bit);
       } else {
-        if (r.la >= 0) atomicAdd(&cur[r.la], 1);
-        if (r.lb >= 0) atomicAdd(&cur[r.lb], 1);
+        if (r.la() >= 0) atomicAdd(&cur[r.la()], 1);
+        if (r.lb() >= 0) atomicAdd(&cur[r.lb()], 1);
       }
     }
   }
@@ -1094,9 +1094,9 @@ xpbd_world_kernel(XpbdArgs a) {
         } else {
           r = load_contact(a, cstart + c, b0, nb);
         }
-        if (r.valid) {
-          la = r.la;
-          lbb = r.lb;
+        if (r.valid()) {
+          la = r.la();
+          lbb = r.lb();
         }
       }
       bool pa = la >= 0, pb = lbb >= 0;
@@ -1128,9 +1128,9 @@ xpbd_world_kernel(XpbdArgs a) {
       } else {
         r = load_contact(a, cstart + c, b0, nb);
       }
-      if (r.valid) {
-        if (r.la >= 0) keys[atomicAdd(&cur[r.la], 1)] = static_cast<unsigned>(2 * c);
-        if (r.lb >= 0) keys[atomicAdd(&cur[r.lb], 1)] = static_cast<unsigned>(2 * c + 1);
+      if (r.valid()) {
+        if (r.la() >= 0) keys[atomicAdd(&cur[r.la()], 1)] = static_cast<unsigned>(2 * c);
+        if (r.lb() >= 0) keys[atomicAdd(&cur[r.lb()], 1)] = static_cast<unsigned>(2 * c + 1);
       }
     }
     __syncthreads();
@@ -1176,8 +1176,8 @@ xpbd_world_kernel(XpbdArgs a) {
           if (km < key) lo = mid + 1; else hi = mid - 1;
         }
       }
-      int cnt_a = r.la >= 0 ? seg[r.la + 1] - seg[r.la] : 0;
-      int cnt_b = r.lb >= 0 ? seg[r.lb + 1] - seg[r.lb] : 0;
+      int cnt_a = r.la() >= 0 ? seg[r.la() + 1] - seg[r.la()] : 0;
+      int cnt_b = r.lb() >= 0 ? seg[r.lb() + 1] - seg[r.lb()] : 0;
       contact_features(L, r, cnt_a, cnt_b, lam_prev, dt, lrn.feat + 16ll * gc);
     }
   }
@@ -1283,9 +1283,9 @@ xpbd_world_kernel(XpbdArgs a) {
                 on = pc >= 0 && warm_contrib(L, r, lrn.lambda0[pc], da, db);
               } else {
                 if (FAST)
-                  on = r.valid && contact_solve_fast(L, r, nb, relax, dt, da, db, lam);
+                  on = r.valid() && contact_solve_fast(L, r, nb, relax, dt, da, db, lam);
                 else
-                  on = r.valid && contact_solve(L, r.la, r.lb, r.p0, r.p1, r.o0, r.o1, r.n, r.m0, r.m1, r.mu,
+                  on = r.valid() && contact_solve(L, r.la(), r.lb(), r.p0, r.p1, r.o0, r.o1, r.n, r.thick, r.mu,
                                                 r.mut, r.mur, relax, dt, da, db, lam);
               }
               if (MODE == XM_REPORT && a.rep.contact_impulse) {
@@ -1294,8 +1294,8 @@ xpbd_world_kernel(XpbdArgs a) {
                 const bool hit = on;
                 sc[0] = hit ? da.lin.x : 0.0f; sc[1] = hit ? da.lin.y : 0.0f; sc[2] = hit ? da.lin.z : 0.0f;
                 sc[3] = hit ? da.ang.x : 0.0f; sc[4] = hit ? da.ang.y : 0.0f; sc[5] = hit ? da.ang.z : 0.0f;
-                sc[6] = __int_as_float(hit ? r.la : -2);
-                sc[7] = __int_as_float(r.lb);
+                sc[6] = __int_as_float(hit ? r.la() : -2);
+                sc[7] = __int_as_float(r.lb());
               }
               if (SPLIT) {
                 // every slot written (zeros when inactive): the walk needs no separate flag
