@@ -810,10 +810,11 @@ __device__ __forceinline__ bool warm_contrib(const WorldLds& L, const CReg& r, f
 // joints other than FREE / DISTANCE (without them the joint solve is not compiled in, which frees registers).
 // The plain one-wave world-frame kernel without the joint solve is held to 128 VGPRs: with its LDS below 10 KiB a
 // world (32 bodies), 16 worlds are then resident per CU (4 waves per SIMD) instead of 12, so 4096 worlds run in
-// one round on 256 CUs.
+// one round on 256 CUs.  The warm-started one (more state per lane) is held to 168 VGPRs, 12 worlds per CU
+// instead of 8.
 template <int BS, int CPL, int BPL, int MODE, bool FAST, bool ART>
 __global__ void __launch_bounds__(BS)
-__attribute__((amdgpu_waves_per_eu((BS == 64 && FAST && !ART && MODE == XM_PLAIN) ? 4 : 1)))
+__attribute__((amdgpu_waves_per_eu((BS == 64 && FAST && !ART && MODE == XM_PLAIN) ? 4 : ((BS == 64 && FAST && !ART && MODE == XM_WARM) ? 3 : 1))))
 xpbd_world_kernel(XpbdArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ int s_articulated;
