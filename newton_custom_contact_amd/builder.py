@@ -92,6 +92,28 @@ def compute_inertia_shape(geo_type: int, scale, density: float):
         ia = mc * (0.25 * r * r + (1.0 / 12.0) * h * h) + ms * (0.4 * r * r + 0.375 * r * h + 0.25 * h * h)
         ib = (mc * 0.5 + ms * 0.4) * r * r
         return m, np.zeros(3), np.diag([ia, ia, ib])
+    if geo_type == GeoType.CYLINDER:
+        # inertia.py:115-138 (axis +Z)
+        r = float(scale[0])
+        h = 2.0 * float(scale[1])
+        m = density * math.pi * r * r * h
+        ia = 1 / 12 * m * (3 * r * r + h * h)
+        ib = 1 / 2 * m * r * r
+        return m, np.zeros(3), np.diag([ia, ia, ib])
+    if geo_type == GeoType.CONE:
+        # inertia.py:141-169 (apex at +Z, base at -half_height; COM a quarter height above the base)
+        r = float(scale[0])
+        h = 2.0 * float(scale[1])
+        m = density * math.pi * r * r * h / 3.0
+        ia = 3 / 20 * m * r * r + 3 / 80 * m * h * h
+        ib = 3 / 10 * m * r * r
+        return m, np.array([0.0, 0.0, -h / 4.0]), np.diag([ia, ia, ib])
+    if geo_type == GeoType.ELLIPSOID:
+        # inertia.py:172-201 (semi-axes rx, ry, rz)
+        rx, ry, rz = (float(v) for v in scale)
+        m = density * ((4.0 / 3.0) * math.pi * rx * ry * rz)
+        return m, np.zeros(3), np.diag([(1.0 / 5.0) * m * (ry * ry + rz * rz), (1.0 / 5.0) * m * (rx * rx + rz * rz),
+                                        (1.0 / 5.0) * m * (rx * rx + ry * ry)])
     if geo_type == GeoType.PLANE:
         return 0.0, np.zeros(3), np.zeros((3, 3))
     raise NotImplementedError(f"inertia of GeoType {geo_type} is not part of the contact-step subset")
@@ -649,6 +671,24 @@ class ModelBuilder:
                           label=None) -> int:
         """Capsule along local +Z; ``half_height`` is half the cylinder length (builder.py:5892)."""
         return self.add_shape(body=body, type=GeoType.CAPSULE, xform=xform, cfg=cfg,
+                              scale=(radius, half_height, 0.0), label=label)
+
+    def add_shape_ellipsoid(self, body: int, xform=None, rx: float = 1.0, ry: float = 0.75, rz: float = 0.5, cfg=None,
+                            label=None) -> int:
+        """Ellipsoid with semi-axes rx, ry, rz along its local axes (builder.py:5747)."""
+        return self.add_shape(body=body, type=GeoType.ELLIPSOID, xform=xform, cfg=cfg, scale=(rx, ry, rz),
+                              label=label)
+
+    def add_shape_cylinder(self, body: int, xform=None, radius: float = 1.0, half_height: float = 0.5, cfg=None,
+                           label=None) -> int:
+        """Cylinder along local +Z, centred at its origin (builder.py:5945)."""
+        return self.add_shape(body=body, type=GeoType.CYLINDER, xform=xform, cfg=cfg,
+                              scale=(radius, half_height, 0.0), label=label)
+
+    def add_shape_cone(self, body: int, xform=None, radius: float = 1.0, half_height: float = 0.5, cfg=None,
+                       label=None) -> int:
+        """Cone along local +Z: base disc at -half_height, apex at +half_height (builder.py:5998)."""
+        return self.add_shape(body=body, type=GeoType.CONE, xform=xform, cfg=cfg,
                               scale=(radius, half_height, 0.0), label=label)
 
     # ------------------------------------------------------------------ finalize

@@ -26,7 +26,8 @@ from ._native import NativeModel, NmcContactReport, NmcLaunchConfig, check, load
 from ._types import GeoType, ShapeFlags
 from .contacts import Contacts
 
-_SUPPORTED = {int(GeoType.PLANE), int(GeoType.SPHERE), int(GeoType.CAPSULE), int(GeoType.BOX), int(GeoType.MESH)}
+_SUPPORTED = {int(GeoType.PLANE), int(GeoType.SPHERE), int(GeoType.CAPSULE), int(GeoType.ELLIPSOID),
+              int(GeoType.CYLINDER), int(GeoType.BOX), int(GeoType.MESH), int(GeoType.CONE)}
 
 
 def estimate_rigid_contact_max(model) -> int:

@@ -72,6 +72,43 @@ __device__ __forceinline__ f3 support(const GShape& g, f3 d) {
     float sy = d.y >= 0.0f ? 1.0f : -1.0f;
     return {sx * g.scale.x, sy * g.scale.y, 0.0f};
   }
+  if (g.type == GEO_ELLIPSOID) {
+    // (a^2 dx, b^2 dy, c^2 dz) / |(a dx, b dy, c dz)| (support_function.py:218-239)
+    const float a = g.scale.x, b = g.scale.y, c = g.scale.z;
+    if (len_sq(d) > 1.0e-12f) {
+      const float adx = a * d.x, bdy = b * d.y, cdz = c * d.z;
+      const float den_sq = adx * adx + bdy * bdy + cdz * cdz;
+      if (den_sq > 1.0e-12f) {
+        const float den = sqrtf(den_sq);
+        return {(a * a) * d.x / den, (b * b) * d.y / den, (c * c) * d.z / den};
+      }
+    }
+    return {a, 0.0f, 0.0f};
+  }
+  if (g.type == GEO_CYLINDER) {
+    // rim point in the direction's XY projection, on the cap the direction's z picks (support_function.py:240-260)
+    const float r = g.scale.x, hh = g.scale.y;
+    const f3 dxy{d.x, d.y, 0.0f};
+    f3 lat{r, 0.0f, 0.0f};
+    if (len_sq(dxy) > 1.0e-12f) {
+      const f3 nxy = normalize(dxy);
+      lat = f3{nxy.x * r, nxy.y * r, 0.0f};
+    }
+    if (d.z > 0.0f) return {lat.x, lat.y, hh};
+    if (d.z < 0.0f) return {lat.x, lat.y, -hh};
+    return lat;
+  }
+  if (g.type == GEO_CONE) {
+    // apex (0, 0, hh) when dz >= r / (2 hh) |d_xy|, else the base rim at z = -hh (support_function.py:262-285)
+    const float r = g.scale.x, hh = g.scale.y;
+    const f3 dxy{d.x, d.y, 0.0f};
+    const float lxy = len(dxy);
+    const float k = hh > 1.0e-12f ? r / (2.0f * hh) : 0.0f;
+    if (lxy <= 1.0e-12f) return d.z >= 0.0f ? f3{0.0f, 0.0f, hh} : f3{r, 0.0f, -hh};
+    if (d.z >= k * lxy) return {0.0f, 0.0f, hh};
+    const f3 nxy = dxy / lxy;
+    return {nxy.x * r, nxy.y * r, -hh};
+  }
   return {0.0f, 0.0f, 0.0f};
 }
 
@@ -308,6 +345,108 @@ __device__ __forceinline__ void plane_box(f3 n, f3 pp, f3 bp, const m33& R, f3 s
   }
   dist[0] = d0; dist[1] = d1; dist[2] = d2; dist[3] = d3;
   pos[0] = q0; pos[1] = q1; pos[2] = q2; pos[3] = q3;
+}
+
+// collide_plane_ellipsoid (collision_primitive.py:334-363): the ellipsoid's support point against the plane's
+// normal, through the unit sphere it is the image of.
+__device__ __forceinline__ void plane_ellipsoid(f3 n, f3 pp, f3 ep, const m33& R, f3 size, float& dist, f3& pos) {
+  const f3 t = mtv(R, n);
+  const f3 ss = -normalize(f3{t.x * size.x, t.y * size.y, t.z * size.z});
+  const f3 p = ep + mv(R, f3{ss.x * size.x, ss.y * size.y, ss.z * size.z});
+  dist = dot(n, p - pp);
+  pos = p - (n * dist) * 0.5f;
+}
+
+// collide_sphere_cylinder (collision_primitive.py:443-513): side, cap or rim, by where the centre projects.
+__device__ void sphere_cylinder(f3 sp, float sr, f3 cp, f3 axis, float cr, float chh, float& dist, f3& pos, f3& n) {
+  const f3 vec = sp - cp;
+  const float x = dot(vec, axis);
+  const f3 a_proj = axis * x;
+  const f3 p_proj = vec - a_proj;
+  const float pps = dot(p_proj, p_proj);
+  bool side = fabsf(x) < chh;
+  bool cap = pps < cr * cr;
+  if (side && cap) {
+    const float dcap = chh - fabsf(x);
+    const float drad = cr - sqrtf(pps);
+    if (dcap < drad) side = false;
+    else cap = false;
+  }
+  if (side) {
+    sphere_sphere(sp, sr, cp + a_proj, cr, dist, pos, n);
+  } else if (cap) {
+    const f3 pc = x > 0.0f ? cp + axis * chh : cp - axis * chh;
+    const f3 pn = x > 0.0f ? axis : -axis;
+    plane_sphere(pn, pc, sp, sr, dist, pos);
+    n = -pn;
+  } else {
+    const float l = sqrtf(pps);
+    const float inv = 1.0f / (l != 0.0f ? l : 1.0e-15f);  // safe_div (newton/_src/math/__init__.py:282)
+    const f3 rim = p_proj * (cr * inv);
+    const f3 cap_off = axis * ((x < 0.0f ? -1.0f : 1.0f) * chh);
+    sphere_sphere(sp, sr, (cp + cap_off) + rim, 0.0f, dist, pos, n);
+  }
+}
+
+// collide_plane_cylinder (collision_primitive.py:516-669): the deepest rim point, then (axis within 22.5 degrees
+// of the plane normal) a tripod on the near cap in a world-anchored orientation, or (rolling) the two side
+// generators through the deepest radial direction and the near-cap rim point facing the plane; extra points
+// closer than 1 % of max(radius, half height) to the deepest one are dropped.  Kept points live in named
+// registers (no runtime-indexed array).
+__device__ void plane_cylinder(f3 n, f3 pp, f3 cp, f3 axis_in, float cr, float chh, float dist[4], f3 pos[4]) {
+  f3 axis = axis_in;
+  float dna = dot(n, axis);
+  if (dna > 0.0f) {
+    axis = -axis;
+    dna = -dna;
+  }
+  const f3 capc = cp + axis * chh;
+  f3 pal = -n + axis * dna;
+  const float pal2 = dot(pal, pal);
+  const bool has_align = pal2 > 1.0e-10f;
+  if (has_align) pal = pal * (1.0f / sqrtf(pal2));
+  f3 ref{1.0f, 0.0f, 0.0f};
+  if (fabsf(dot(axis, ref)) > 0.9f) ref = f3{0.0f, 1.0f, 0.0f};
+  const f3 pfix = normalize(ref - axis * dot(axis, ref));
+  const float abs_dot = -dna;
+  const bool flat = abs_dot >= (float)0.92387953251128674;  // cos(22.5 degrees)
+  const f3 dperp = has_align ? pal : pfix;
+  const f3 dpt = capc + dperp * cr;
+  const float dd = dot(dpt - pp, n);
+  const f3 dpos = dpt - n * (dd * 0.5f);
+  const f3 zero{0.0f, 0.0f, 0.0f};
+  float d1 = MAXVAL, d2 = MAXVAL, d3 = MAXVAL;
+  f3 q1 = zero, q2 = zero, q3 = zero;
+  int nc = 1;
+  const float mt = 0.01f * fmax2(cr, chh);
+  const float mt2 = mt * mt;
+  auto add = [&](f3 pt) {
+    const float d = dot(pt - pp, n);
+    const f3 q = pt - n * (d * 0.5f);
+    if (nc < 4 && len_sq(q - dpos) > mt2) {
+      if (nc == 1) { d1 = d; q1 = q; }
+      else if (nc == 2) { d2 = d; q2 = q; }
+      else { d3 = d; q3 = q; }
+      nc += 1;
+    }
+  };
+  if (flat) {
+    const f3 u = pfix * cr, v = cross(axis, pfix) * cr;
+    const float c120 = -0.5f, s120 = 0.8660254f;
+    add(capc + u);
+    add((capc + c120 * u) + s120 * v);
+    add((capc + c120 * u) - s120 * v);
+  } else {
+    const f3 pr = has_align ? pal : pfix;
+    const f3 u = pr * cr, v = cross(axis, pr) * cr;
+    add((cp + axis * chh) + u);
+    add((cp - axis * chh) + u);
+    const f3 pv = capc + v, nv = capc - v;
+    const float dpv = dot(pv - pp, n), dnv = dot(nv - pp, n);
+    add(dpv <= dnv ? pv : nv);
+  }
+  dist[0] = dd; dist[1] = d1; dist[2] = d2; dist[3] = d3;
+  pos[0] = dpos; pos[1] = q1; pos[2] = q2; pos[3] = q3;
 }
 
 __device__ void sphere_box(f3 sp, float r, f3 bp, const m33& R, f3 size, float& dist, f3& pos_out, f3& normal) {
@@ -1017,6 +1156,43 @@ struct PairCtx {
   CData tmpl;
 };
 
+// post_process_axial_on_discrete_contact (collision_core.py:172-275): the Minkowski adjustment, then, for a
+// cylinder or cone against a flat-faced shape (box, plane or the infinite plane's box proxy) in a rolling pose,
+// the point projected onto the plane through the axial shape's centre spanned by its axis and the normal.
+// Rolling: a cylinder's axis within 2 degrees of perpendicular to the normal; a cone's axis at its half angle
+// atan2(r, 2 hh) from the normal plane to within 2 degrees, tested through sin(half angle -/+ 2 degrees)
+// = sin a cos 2 -/+ cos a sin 2 with sin a = r / L, cos a = 2 hh / L, L = |(r, 2 hh)| (the same thresholds,
+// without a trigonometric call).
+__device__ __forceinline__ CData post_process(CData c, const PairCtx& P) {
+  const f3 normal = c.normal;
+  c = post_process(c, P.ga.type, P.gb.type);
+  const int ta = P.ga.type, tb = P.gb.type;
+  const bool da = ta == GEO_BOX || ta == GEO_PLANE, db = tb == GEO_BOX || tb == GEO_PLANE;
+  const bool xa = ta == GEO_CYLINDER || ta == GEO_CONE, xb = tb == GEO_CYLINDER || tb == GEO_CONE;
+  if (!((da && xb) || (db && xa))) return c;
+  const bool b_axial = da && xb;
+  const f3 axis = rot(b_axial ? P.ob_world : P.oa, f3{0.0f, 0.0f, 1.0f});
+  const GShape& g = b_axial ? P.gb : P.ga;
+  const f3 spos = b_axial ? P.pb_world : P.pa_world;
+  const f3 an = b_axial ? normal : -normal;
+  const float and_ = fabsf(dot(axis, an));
+  const float SIN2 = (float)0.034899496702500969, COS2 = (float)0.99939082701909576;
+  bool rolling;
+  if (g.type == GEO_CONE) {
+    const float hh2 = 2.0f * g.scale.y;
+    const float l = sqrtf(g.scale.x * g.scale.x + hh2 * hh2);
+    const float sa = g.scale.x / l, ca = hh2 / l;
+    rolling = and_ >= sa * COS2 - ca * SIN2 && and_ <= sa * COS2 + ca * SIN2;
+  } else {
+    rolling = and_ <= SIN2;
+  }
+  if (rolling) {
+    const f3 ppn = normalize(cross(axis, an));
+    c.center = c.center - ppn * dot(c.center - spos, ppn);  // project_point_onto_plane (collision_core.py:51-65)
+  }
+  return c;
+}
+
 // Directions of the five tilted support probes (cos and sin of k 2 pi / 5), in constant memory: the probe
 // loop index is wave-uniform, so the loads are scalar.
 static __constant__ float c_probe_cos[5] = {1.0f, (float)0.30901699437494745, (float)-0.8090169943749473,
@@ -1097,7 +1273,7 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
           cd.normal = normal_world;
           cd.dist = sd;
           cd.sub_key = (P.tmpl.sub_key << 3) | i;
-          cd = post_process(cd, P.ga.type, P.gb.type);
+          cd = post_process(cd, P);
           emit(cd, w, true);
         }
       } else {
@@ -1116,7 +1292,7 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
     cd.normal = normal_world;
     cd.dist = dsd;
     cd.sub_key = (P.tmpl.sub_key << 3) | count_out;
-    cd = post_process(cd, P.ga.type, P.gb.type);
+    cd = post_process(cd, P);
     emit(cd, w, true);
   }
 }
@@ -1152,18 +1328,22 @@ __device__ void solve_convex(const PairCtx& P, float threshold, bool skip_multi,
     cd.normal = nws;
     cd.dist = sd;
     cd.sub_key = P.tmpl.sub_key << 3;
-    cd = post_process(cd, P.ga.type, P.gb.type);
+    cd = post_process(cd, P);
     emit(cd, w, true);
     return;
   }
   manifold(P, rel_ob, rel_pb, point_a, point_b, normal, w);
 }
 
-// Pairs that go through MPR/GJK + the manifold (narrow_phase.py:136-605 routes every other supported
-// type pair to an analytic collider).
-__device__ __forceinline__ bool convex_route(int ta, int tb) {
-  return tb == GEO_BOX && (ta == GEO_BOX || ta == GEO_CAPSULE);
+// Type pairs with an analytic collider (narrow_phase.py:327-513); every other primitive pair goes through MPR/GJK
+// and the manifold (narrow_phase.py:595-600), as does an analytic pair whose collider reports no point.
+__device__ __forceinline__ bool analytic_pair(int ta, int tb) {
+  if (ta == GEO_PLANE)
+    return tb == GEO_SPHERE || tb == GEO_ELLIPSOID || tb == GEO_BOX || tb == GEO_CAPSULE || tb == GEO_CYLINDER;
+  if (ta == GEO_SPHERE) return tb == GEO_SPHERE || tb == GEO_CAPSULE || tb == GEO_CYLINDER || tb == GEO_BOX;
+  return ta == GEO_CAPSULE && tb == GEO_CAPSULE;
 }
+__device__ __forceinline__ bool convex_route(int ta, int tb) { return !analytic_pair(ta, tb); }
 
 // Route class of a pair: 0 analytic, 1 MPR/GJK, 2 mesh (mesh-mesh or mesh-infinite plane,
 // narrow_phase.py:290-320; other mesh pairs are refused when the model is created).
@@ -1191,6 +1371,10 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     f3 pn = rot(qa, ez);
     plane_sphere(pn, pos_a, pos_b, scale_b.x, cd[0], cp[0]);
     cn = pn;
+  } else if (plane_a && tb == GEO_ELLIPSOID) {
+    f3 pn = rot(qa, ez);
+    plane_ellipsoid(pn, pos_a, pos_b, qmat(qb), scale_b, cd[0], cp[0]);
+    cn = pn;
   } else if (plane_a && tb == GEO_BOX) {
     f3 pn = rot(qa, ez);
     m33 R = qmat(qb);
@@ -1205,6 +1389,10 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     plane_sphere(pn, pos_a, pos_b + seg, scale_b.x, cd[0], cp[0]);
     plane_sphere(pn, pos_a, pos_b - seg, scale_b.x, cd[1], cp[1]);
     cn = pn;
+  } else if (plane_a && tb == GEO_CYLINDER) {
+    f3 pn = rot(qa, ez);
+    plane_cylinder(pn, pos_a, pos_b, rot(qb, ez), scale_b.x, scale_b.y, cd, cp);
+    cn = pn;
   } else if (ta == GEO_SPHERE && tb == GEO_CAPSULE) {
     f3 axis = rot(qb, ez);
     f3 seg = axis * scale_b.y;
@@ -1216,6 +1404,8 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     f3 p2[2];
     capsule_capsule(pos_a, ax1, scale_a.x, scale_a.y, pos_b, ax2, scale_b.x, scale_b.y, d2, p2, cn);
     cd[0] = d2[0]; cp[0] = p2[0]; cd[1] = d2[1]; cp[1] = p2[1];
+  } else if (ta == GEO_SPHERE && tb == GEO_CYLINDER) {
+    sphere_cylinder(pos_a, scale_a.x, pos_b, rot(qb, ez), scale_b.x, scale_b.y, cd[0], cp[0], cn);
   } else if (ta == GEO_SPHERE && tb == GEO_BOX) {
     m33 R = qmat(qb);
     sphere_box(pos_a, scale_a.x, pos_b, R, scale_b, cd[0], cp[0], cn);
@@ -1311,7 +1501,8 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
   t.gap_sum = gap_sum;
   t.sub_key = 0;
   float threshold = gap_sum + rea + reb + A.margin + B.margin;
-  bool skip_multi = P.ga.type == GEO_SPHERE || P.gb.type == GEO_SPHERE;
+  bool skip_multi = P.ga.type == GEO_SPHERE || P.gb.type == GEO_SPHERE || P.ga.type == GEO_ELLIPSOID ||
+                    P.gb.type == GEO_ELLIPSOID;
   solve_convex(P, threshold, skip_multi, w);
 }
 

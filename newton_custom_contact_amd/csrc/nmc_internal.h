@@ -9,7 +9,8 @@
 namespace nmc {
 
 // GeoType values (newton/_src/geometry/types.py:38-78)
-enum GeoTypeId { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_BOX = 7, GEO_MESH = 8 };
+enum GeoTypeId { GEO_PLANE = 1, GEO_SPHERE = 3, GEO_CAPSULE = 4, GEO_ELLIPSOID = 5, GEO_CYLINDER = 6, GEO_BOX = 7,
+                 GEO_MESH = 8, GEO_CONE = 9 };
 // ShapeFlags.COLLIDE_SHAPES (newton/_src/geometry/flags.py:25), BodyFlags.KINEMATIC (sim/enums.py:21)
 enum { SHAPE_COLLIDE_SHAPES = 2, BODY_KINEMATIC = 2 };
 // JointType (newton/_src/sim/enums.py:29-56)

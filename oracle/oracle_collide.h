@@ -27,7 +27,7 @@
 
 namespace oracle {
 
-enum { PLANE = 1, SPHERE = 3, CAPSULE = 4, BOX = 7, MESH = 8 };
+enum { PLANE = 1, SPHERE = 3, CAPSULE = 4, ELLIPSOID = 5, CYLINDER = 6, BOX = 7, MESH = 8, CONE = 9 };
 enum { COLLIDE_SHAPES = 2 };
 static const double MAXVAL = 1.0e10;
 
@@ -82,6 +82,40 @@ template <class T> V3<T> support(const Shape<T>& g, V3<T> d) {
     T sx = d.x >= T(0) ? T(1) : T(-1);
     T sy = d.y >= T(0) ? T(1) : T(-1);
     return {sx * g.scale.x, sy * g.scale.y, T(0)};
+  }
+  if (g.type == ELLIPSOID) {  // support_function.py:218-239
+    const T a = g.scale.x, b = g.scale.y, c = g.scale.z;
+    if (length_sq(d) > eps) {
+      const T adx = a * d.x, bdy = b * d.y, cdz = c * d.z;
+      const T den_sq = adx * adx + bdy * bdy + cdz * cdz;
+      if (den_sq > eps) {
+        const T den = tsqrt(den_sq);
+        return {(a * a) * d.x / den, (b * b) * d.y / den, (c * c) * d.z / den};
+      }
+    }
+    return {a, T(0), T(0)};
+  }
+  if (g.type == CYLINDER) {  // support_function.py:240-260
+    const T r = g.scale.x, hh = g.scale.y;
+    const V3<T> dxy{d.x, d.y, T(0)};
+    V3<T> lat{r, T(0), T(0)};
+    if (length_sq(dxy) > eps) {
+      const V3<T> nxy = normalize(dxy);
+      lat = V3<T>{nxy.x * r, nxy.y * r, T(0)};
+    }
+    if (d.z > T(0)) return {lat.x, lat.y, hh};
+    if (d.z < T(0)) return {lat.x, lat.y, -hh};
+    return lat;
+  }
+  if (g.type == CONE) {  // support_function.py:262-285
+    const T r = g.scale.x, hh = g.scale.y;
+    const V3<T> dxy{d.x, d.y, T(0)};
+    const T lxy = length(dxy);
+    const T k = hh > eps ? r / (T(2) * hh) : T(0);
+    if (lxy <= eps) return d.z >= T(0) ? V3<T>{T(0), T(0), hh} : V3<T>{r, T(0), -hh};
+    if (d.z >= k * lxy) return {T(0), T(0), hh};
+    const V3<T> nxy = dxy / lxy;
+    return {nxy.x * r, nxy.y * r, -hh};
   }
   return {T(0), T(0), T(0)};
 }
@@ -330,6 +364,108 @@ void plane_box(V3<T> n, V3<T> pp, V3<T> bp, const M3<T>& R, V3<T> size, T margin
       if (dist[2] > dist[worst]) worst = 2;
       if (dist[3] > dist[worst]) worst = 3;
     }
+  }
+}
+
+// collide_plane_ellipsoid (collision_primitive.py:334-363)
+template <class T>
+void plane_ellipsoid(V3<T> n, V3<T> pp, V3<T> ep, const M3<T>& R, V3<T> size, T& dist, V3<T>& pos) {
+  const V3<T> t = mtv(R, n);
+  const V3<T> ss = -normalize(V3<T>{t.x * size.x, t.y * size.y, t.z * size.z});
+  const V3<T> p = ep + mv(R, V3<T>{ss.x * size.x, ss.y * size.y, ss.z * size.z});
+  dist = dot(n, p - pp);
+  pos = p - (n * dist) * T(0.5);
+}
+
+// collide_sphere_cylinder (collision_primitive.py:443-513)
+template <class T>
+void sphere_cylinder(V3<T> sp, T sr, V3<T> cp, V3<T> axis, T cr, T chh, T& dist, V3<T>& pos, V3<T>& n) {
+  const V3<T> vec = sp - cp;
+  const T x = dot(vec, axis);
+  const V3<T> a_proj = axis * x;
+  const V3<T> p_proj = vec - a_proj;
+  const T pps = dot(p_proj, p_proj);
+  bool side = tabs(x) < chh;
+  bool cap = pps < cr * cr;
+  if (side && cap) {
+    const T dcap = chh - tabs(x);
+    const T drad = cr - tsqrt(pps);
+    if (dcap < drad) side = false;
+    else cap = false;
+  }
+  if (side) {
+    sphere_sphere(sp, sr, cp + a_proj, cr, dist, pos, n);
+  } else if (cap) {
+    const V3<T> pc = x > T(0) ? cp + axis * chh : cp - axis * chh;
+    const V3<T> pn = x > T(0) ? axis : -axis;
+    plane_sphere(pn, pc, sp, sr, dist, pos);
+    n = -pn;
+  } else {
+    const T l = tsqrt(pps);
+    const T inv = T(1) / (l != T(0) ? l : T(1.0e-15));  // safe_div (newton/_src/math/__init__.py:282)
+    const V3<T> rim = p_proj * (cr * inv);
+    const V3<T> cap_off = axis * ((x < T(0) ? T(-1) : T(1)) * chh);
+    sphere_sphere(sp, sr, (cp + cap_off) + rim, T(0), dist, pos, n);
+  }
+}
+
+// collide_plane_cylinder (collision_primitive.py:516-669): deepest rim point, then the flat-mode tripod or the
+// rolling-mode generators and facing cap-rim point, each kept when farther than 1 % of max(r, hh) from the deepest.
+template <class T>
+void plane_cylinder(V3<T> n, V3<T> pp, V3<T> cp, V3<T> axis_in, T cr, T chh, T dist[4], V3<T> pos[4]) {
+  V3<T> axis = axis_in;
+  T dna = dot(n, axis);
+  if (dna > T(0)) {
+    axis = -axis;
+    dna = -dna;
+  }
+  const V3<T> capc = cp + axis * chh;
+  V3<T> pal = -n + axis * dna;
+  const T pal2 = dot(pal, pal);
+  const bool has_align = pal2 > T(1.0e-10);
+  if (has_align) pal = pal * (T(1) / tsqrt(pal2));
+  V3<T> ref{T(1), T(0), T(0)};
+  if (tabs(dot(axis, ref)) > T(0.9)) ref = V3<T>{T(0), T(1), T(0)};
+  const V3<T> pfix = normalize(ref - axis * dot(axis, ref));
+  const T abs_dot = -dna;
+  const bool flat = abs_dot >= T(0.92387953251128674);  // cos(22.5 degrees)
+  const V3<T> dperp = has_align ? pal : pfix;
+  const V3<T> dpt = capc + dperp * cr;
+  const T dd = dot(dpt - pp, n);
+  const V3<T> dpos = dpt - n * (dd * T(0.5));
+  const V3<T> zero{T(0), T(0), T(0)};
+  for (int k = 0; k < 4; ++k) {
+    dist[k] = T(MAXVAL);
+    pos[k] = zero;
+  }
+  dist[0] = dd;
+  pos[0] = dpos;
+  int nc = 1;
+  const T mt = T(0.01) * tmax(cr, chh);
+  const T mt2 = mt * mt;
+  auto add = [&](V3<T> pt) {
+    const T d = dot(pt - pp, n);
+    const V3<T> q = pt - n * (d * T(0.5));
+    if (nc < 4 && length_sq(q - dpos) > mt2) {
+      dist[nc] = d;
+      pos[nc] = q;
+      nc += 1;
+    }
+  };
+  if (flat) {
+    const V3<T> u = pfix * cr, v = cross(axis, pfix) * cr;
+    const T c120 = T(-0.5), s120 = T(0.8660254f);
+    add(capc + u);
+    add((capc + c120 * u) + s120 * v);
+    add((capc + c120 * u) - s120 * v);
+  } else {
+    const V3<T> pr = has_align ? pal : pfix;
+    const V3<T> u = pr * cr, v = cross(axis, pr) * cr;
+    add((cp + axis * chh) + u);
+    add((cp - axis * chh) + u);
+    const V3<T> pv = capc + v, nv = capc - v;
+    const T dpv = dot(pv - pp, n), dnv = dot(nv - pp, n);
+    add(dpv <= dnv ? pv : nv);
   }
 }
 
@@ -1012,6 +1148,39 @@ template <class T> struct PairCtx {
   ContactData<T> tmpl;
 };
 
+// post_process_axial_on_discrete_contact (collision_core.py:172-275): Minkowski adjustment, then the rolling
+// projection for a cylinder or cone against a box / plane.  The cone's thresholds sin(atan2(r, 2 hh) -/+ 2 deg)
+// are evaluated as sin a cos 2 -/+ cos a sin 2 with sin a = r / L, cos a = 2 hh / L, L = |(r, 2 hh)|.
+template <class T> ContactData<T> post_process(ContactData<T> c, const PairCtx<T>& P) {
+  const V3<T> normal = c.normal;
+  c = post_process(c, P.ga.type, P.gb.type);
+  const int ta = P.ga.type, tb = P.gb.type;
+  const bool da = ta == BOX || ta == PLANE, db = tb == BOX || tb == PLANE;
+  const bool xa = ta == CYLINDER || ta == CONE, xb = tb == CYLINDER || tb == CONE;
+  if (!((da && xb) || (db && xa))) return c;
+  const bool b_axial = da && xb;
+  const V3<T> axis = qrot(b_axial ? P.ob_world : P.oa, V3<T>{T(0), T(0), T(1)});
+  const Shape<T>& g = b_axial ? P.gb : P.ga;
+  const V3<T> spos = b_axial ? P.pb_world : P.pa_world;
+  const V3<T> an = b_axial ? normal : -normal;
+  const T and_ = tabs(dot(axis, an));
+  const T SIN2 = T(0.034899496702500969), COS2 = T(0.99939082701909576);
+  bool rolling;
+  if (g.type == CONE) {
+    const T hh2 = T(2) * g.scale.y;
+    const T l = tsqrt(g.scale.x * g.scale.x + hh2 * hh2);
+    const T sa = g.scale.x / l, ca = hh2 / l;
+    rolling = and_ >= sa * COS2 - ca * SIN2 && and_ <= sa * COS2 + ca * SIN2;
+  } else {
+    rolling = and_ <= SIN2;
+  }
+  if (rolling) {
+    const V3<T> ppn = normalize(cross(axis, an));
+    c.center = c.center - ppn * dot(c.center - spos, ppn);  // project_point_onto_plane (collision_core.py:51-65)
+  }
+  return c;
+}
+
 template <class T> void orthonormal_basis(V3<T> n, V3<T>& b1, V3<T>& b2) {
   if (n.z < T(0)) {
     T a = T(1) / (T(1) - n.z);
@@ -1103,7 +1272,7 @@ template <class T> int build_manifold(const PairCtx<T>& P, Q<T> rel_ob, V3<T> re
           cd.normal = normal_world;
           cd.dist = sd;
           cd.sub_key = (P.tmpl.sub_key << 3) | i;
-          cd = post_process(cd, P.ga.type, P.gb.type);
+          cd = post_process(cd, P);
           write_contact(cd, w, true);
         }
       } else {
@@ -1122,7 +1291,7 @@ template <class T> int build_manifold(const PairCtx<T>& P, Q<T> rel_ob, V3<T> re
     cd.normal = normal_world;
     cd.dist = dsd;
     cd.sub_key = (P.tmpl.sub_key << 3) | count_out;
-    cd = post_process(cd, P.ga.type, P.gb.type);
+    cd = post_process(cd, P);
     write_contact(cd, w, true);
     count_out += 1;
   }
@@ -1161,7 +1330,7 @@ template <class T> void solve_convex(const PairCtx<T>& P, T threshold, bool skip
     cd.normal = nws;
     cd.dist = sd;
     cd.sub_key = P.tmpl.sub_key << 3;
-    cd = post_process(cd, P.ga.type, P.gb.type);
+    cd = post_process(cd, P);
     write_contact(cd, w, true);
     return;
   }
@@ -1201,6 +1370,10 @@ template <class T> void narrow_pair(const OracleModel* m, const std::vector<Shap
     V3<T> pn = qrot(qa, ez);
     plane_sphere(pn, pos_a, pos_b, scale_b.x, cd[0], cp[0]);
     cn = pn;
+  } else if (plane_a && tb == ELLIPSOID) {
+    V3<T> pn = qrot(qa, ez);
+    plane_ellipsoid(pn, pos_a, pos_b, quat_to_matrix(qb), scale_b, cd[0], cp[0]);
+    cn = pn;
   } else if (plane_a && tb == BOX) {
     V3<T> pn = qrot(qa, ez);
     M3<T> R = quat_to_matrix(qb);
@@ -1215,6 +1388,10 @@ template <class T> void narrow_pair(const OracleModel* m, const std::vector<Shap
     plane_sphere(pn, pos_a, pos_b + seg, scale_b.x, cd[0], cp[0]);
     plane_sphere(pn, pos_a, pos_b - seg, scale_b.x, cd[1], cp[1]);
     cn = pn;
+  } else if (plane_a && tb == CYLINDER) {
+    V3<T> pn = qrot(qa, ez);
+    plane_cylinder(pn, pos_a, pos_b, qrot(qb, ez), scale_b.x, scale_b.y, cd, cp);
+    cn = pn;
   } else if (ta == SPHERE && tb == CAPSULE) {
     V3<T> axis = qrot(qb, ez);
     V3<T> seg = axis * scale_b.y;
@@ -1226,6 +1403,8 @@ template <class T> void narrow_pair(const OracleModel* m, const std::vector<Shap
     V3<T> p2[2];
     capsule_capsule(pos_a, ax1, scale_a.x, scale_a.y, pos_b, ax2, scale_b.x, scale_b.y, d2, p2, cn);
     cd[0] = d2[0]; cp[0] = p2[0]; cd[1] = d2[1]; cp[1] = p2[1];
+  } else if (ta == SPHERE && tb == CYLINDER) {
+    sphere_cylinder(pos_a, scale_a.x, pos_b, qrot(qb, ez), scale_b.x, scale_b.y, cd[0], cp[0], cn);
   } else if (ta == SPHERE && tb == BOX) {
     M3<T> R = quat_to_matrix(qb);
     sphere_box(pos_a, scale_a.x, pos_b, R, scale_b, cd[0], cp[0], cn);
@@ -1319,7 +1498,7 @@ template <class T> void narrow_pair(const OracleModel* m, const std::vector<Shap
   t.gap_sum = gap_sum;
   t.sub_key = 0;
   T threshold = gap_sum + rea + reb + A.margin + B.margin;
-  bool skip_multi = P.ga.type == SPHERE || P.gb.type == SPHERE;
+  bool skip_multi = P.ga.type == SPHERE || P.gb.type == SPHERE || P.ga.type == ELLIPSOID || P.gb.type == ELLIPSOID;
   solve_convex(P, threshold, skip_multi, w);
 }
 
