@@ -26,7 +26,7 @@ class Contacts:
     def __init__(self, rigid_contact_max: int, soft_contact_max: int = 0, requires_grad: bool = False,
                  device=None, segment_count: int = 1, deterministic: bool = False,
                  requested_attributes: set[str] | None = None, contact_matching: bool = False,
-                 contact_report: bool = False):
+                 contact_report: bool = False, per_contact_shape_properties: bool = False):
         dev = torch.device(device) if device is not None else torch.device("cuda:0")
         C = int(rigid_contact_max)
         self.device = dev
@@ -50,6 +50,15 @@ class Contacts:
         self.rigid_contact_margin0 = torch.zeros((C,), **f)
         self.rigid_contact_margin1 = torch.zeros((C,), **f)
         self.rigid_contact_tids = torch.zeros((C,), **i)
+        # per-contact stiffness / damping / friction (contacts.py:210-224): only the hydroelastic narrow phase
+        # writes non-zero values; every collider of this package leaves them at 0, as the reference's do
+        self.per_contact_shape_properties = bool(per_contact_shape_properties)
+        if self.per_contact_shape_properties:
+            self.rigid_contact_stiffness = torch.zeros((C,), **f)
+            self.rigid_contact_damping = torch.zeros((C,), **f)
+            self.rigid_contact_friction = torch.zeros((C,), **f)
+        else:
+            self.rigid_contact_stiffness = self.rigid_contact_damping = self.rigid_contact_friction = None
         self.rigid_contact_sort_key = torch.zeros((C,), device=dev, dtype=torch.int64) if deterministic else None
         # contact matching (contacts.py:226-270): match index per contact (>= 0 matched previous-frame
         # index, -1 MATCH_NOT_FOUND, -2 MATCH_BROKEN) and the optional new / broken report
@@ -98,6 +107,10 @@ class Contacts:
         """contacts.py: reset the counters (buffers are overwritten by the next collide)."""
         self.contact_counters.zero_()
         self.world_contact_count.zero_()
+        if self.per_contact_shape_properties:
+            self.rigid_contact_stiffness.zero_()
+            self.rigid_contact_damping.zero_()
+            self.rigid_contact_friction.zero_()
         if bump_generation:
             self.contact_generation += 1
 

@@ -131,3 +131,19 @@ def test_builder_rejects_cross_world_joint():
     with pytest.raises(ValueError):
         b.add_joint_revolute(a, c)
     b.end_world()
+
+
+def test_per_contact_shape_properties_arrays():
+    """contacts.py:210-224 / :338-341: the optional stiffness, damping and friction arrays exist only when asked
+    for, and clear() zeroes them."""
+    from newton_custom_contact_amd.contacts import Contacts
+
+    c = Contacts(16, device="cpu", per_contact_shape_properties=True)
+    for a in (c.rigid_contact_stiffness, c.rigid_contact_damping, c.rigid_contact_friction):
+        assert a is not None and a.shape == (16,)
+        a.fill_(3.0)
+    c.clear()
+    assert float(c.rigid_contact_stiffness.abs().sum() + c.rigid_contact_damping.abs().sum()
+                 + c.rigid_contact_friction.abs().sum()) == 0.0
+    d = Contacts(16, device="cpu")
+    assert d.rigid_contact_stiffness is None and d.rigid_contact_damping is None and d.rigid_contact_friction is None
