@@ -183,9 +183,10 @@ typedef struct NmcXpbdParams {
   int block_lanes;  /* launch shape: 0 = automatic (one 64-lane wave a world up to 64 bodies; models with
                      * larger worlds launch by size tier, worlds above 64 bodies on 512 lanes), 64 (worlds
                      * of up to 64 bodies), 256 (every world in one launch: 256 lanes up to 64 bodies a
-                     * world, else 512) or 512 (models with worlds above 64 bodies, plain step: one launch of
-                     * 512 lanes over CSR bins -- runs of consecutive worlds packed up to the largest world's
-                     * body count, largest bins first; other modes launch as 0).  The reference-order
+                     * world, else 512) or 512 (plain step: one launch over CSR bins -- runs of consecutive
+                     * worlds packed up to max(largest world, 64) bodies, or 256 when a world exceeds 256 (those
+                     * worlds then run in a 512-lane launch beside the bins); bins of up to 64 bodies on one wave,
+                     * larger ones on 256 lanes, largest first; other modes launch as 0).  The reference-order
                      * arithmetic gives the same result for every shape; the world-frame arithmetic's per-body
                      * sums follow the lanes a body gets, so bins (one lane a body) round like the 512-lane
                      * worlds (an autotuner picks per model) */

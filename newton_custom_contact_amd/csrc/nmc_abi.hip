@@ -264,7 +264,8 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   }
   // CSR bins: consecutive worlds whose body and joint ranges follow each other, packed greedily while the
   // bin's bodies stay within max(largest world, 64) (so one LDS layout holds every bin); largest bins first.
-  // Kept only when packing merges worlds.
+  // With worlds above 256 bodies the bins stop at 256 (one 256-lane workgroup each, two resident per CU) and the
+  // larger worlds run in a 512-lane launch of their own beside them.  Kept only when packing merges worlds.
   if (d.world_count >= 1) {
     const int W1 = d.world_count + 1;
     std::vector<int> nb(W1), bf(W1), jf(W1), jc(W1);
@@ -275,9 +276,16 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       nmc_model_destroy(m);
       return hip_fail(e, "nmc_model_create: world ranges");
     }
-    const int cap = std::max(d.max_world_bodies, 64);
+    const int cap = d.max_world_bodies <= 256 ? std::max(d.max_world_bodies, 64) : 256;
     std::vector<std::array<int, 4>> bins;  // first, end, bodies, joints
+    std::vector<int> big;
     for (int w = 0; w < W1;) {
+      if (nb[w] > cap) {
+        big.push_back(w);
+        m->xpbd_big_max = std::max(m->xpbd_big_max, nb[w]);
+        ++w;
+        continue;
+      }
       int e1 = w + 1, bodies = nb[w], joints = jc[w];
       while (e1 < W1 && bodies + nb[e1] <= cap && bf[e1] == bf[e1 - 1] + nb[e1 - 1] &&
              jf[e1] == jf[e1 - 1] + jc[e1 - 1]) {
@@ -288,7 +296,8 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
       bins.push_back({w, e1, bodies, joints});
       w = e1;
     }
-    if (static_cast<int>(bins.size()) < W1) {
+    // bins beside a launch of the large worlds need the side stream of the tiered launch
+    if (static_cast<int>(bins.size() + big.size()) < W1 && !bins.empty() && (big.empty() || m->xpbd_tiered)) {
       std::stable_sort(bins.begin(), bins.end(),
                        [](const std::array<int, 4>& x, const std::array<int, 4>& y) { return x[2] > y[2]; });
       std::vector<int> flat;
@@ -299,8 +308,12 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
         m->xpbd_bin_max_joints = std::max(m->xpbd_bin_max_joints, bn[3]);
       }
       m->xpbd_bin_count = static_cast<int>(bins.size());
+      m->xpbd_big_count = static_cast<int>(big.size());
       if ((e = dmalloc(&m->xpbd_bins, flat.size())) != hipSuccess ||
-          (e = hipMemcpy(m->xpbd_bins, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice)) != hipSuccess) {
+          (e = hipMemcpy(m->xpbd_bins, flat.data(), sizeof(int) * flat.size(), hipMemcpyHostToDevice)) != hipSuccess ||
+          (!big.empty() &&
+           ((e = dmalloc(&m->xpbd_big, big.size())) != hipSuccess ||
+            (e = hipMemcpy(m->xpbd_big, big.data(), sizeof(int) * big.size(), hipMemcpyHostToDevice)) != hipSuccess))) {
         nmc_model_destroy(m);
         return hip_fail(e, "nmc_model_create: world bins");
       }
@@ -415,7 +428,7 @@ extern "C" void nmc_model_destroy(NmcModel* m) {
                   s.shape_wc, m->seg_pair_off, m->seg_pair_cap,
                   m->seg_pair_count, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, m->seg_contact_count,
                   m->counters, m->xpbd_keys, m->xpbd_tier_worlds[0], m->xpbd_tier_worlds[1], m->xpbd_tier_worlds[2],
-                  m->xpbd_bins,
+                  m->xpbd_bins, m->xpbd_big,
                   m->ord_gid, m->ord_unit_order, m->ord_delta0, m->ord_M, m->ord_gstart, m->ord_L, m->ord_H,
                   m->ord_unit_off, m->ord_base};
   for (void* p : ptrs)

@@ -114,13 +114,18 @@ struct NmcModelImpl {
   // caller's stream through events (graph-capturable); created with the model when it is tiered
   hipStream_t tier_stream[2];
   hipEvent_t tier_fork, tier_join[2];
-  // CSR bins for the one-launch step of models with worlds above 64 bodies (block_lanes 512): runs of
-  // consecutive worlds with adjacent body and joint ranges, packed while their bodies fit the largest world's;
+  // CSR bins for the binned step (block_lanes 512): runs of consecutive worlds with adjacent body and joint
+  // ranges, packed while their bodies fit max(largest world, 64) or 256 when a world exceeds 256 bodies;
   // xpbd_bins holds (first world, end world) per bin, largest bins first (the launch order)
   int* xpbd_bins;
   int xpbd_bin_count;
   int xpbd_bin_max_bodies;
   int xpbd_bin_max_joints;
+  // worlds above 256 bodies stay out of the bins (which then launch on 256 lanes) and run in a launch of their
+  // own beside them: their indices, count and largest body count
+  int* xpbd_big;
+  int xpbd_big_count;
+  int xpbd_big_max;
   int xpbd_joint_lds;    // joints per world with LDS delta slots: max_world_joints when some enabled joint is
                          // not FREE / DISTANCE (the only joints the joint pass solves), else 0
   unsigned* xpbd_keys;   // [xpbd_keys_cap] ordering entries of denser worlds, at 2 x their contact offset

@@ -1656,8 +1656,8 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   if (lrn) a.lrn = *lrn; else a.lrn = NmcLearnIO{};
   if (rep) a.rep = *rep; else a.rep = NmcXpbdReport{};
   // launches: one per size tier of worlds (automatic shape: <= 64 bodies on one wave, larger on 512 lanes),
-  // or all worlds in one launch of the requested shape, or (block_lanes 512, plain step of a model with worlds
-  // above 64 bodies) one launch over the CSR bins, largest first
+  // or all worlds in one launch of the requested shape, or (block_lanes 512, plain step) one launch over the
+  // CSR bins, largest first, beside a launch of the worlds above 256 bodies when the model has any
   a.bins = nullptr;
   struct Launch { XpbdShape sh; int worlds, bodies; const int* list; const int* bins; int joints; };
   Launch ls[3];
@@ -1665,10 +1665,16 @@ hipError_t launch_xpbd(NmcModelImpl* m, const NmcXpbdParams* params, const float
   const bool plain = rep == nullptr && a.lrn.mode == NMC_STEP_PLAIN;
   if (params->block_lanes == 512 && m->xpbd_bin_count > 0 && plain) {
     const int jl = m->xpbd_joint_lds > 0 ? std::max(m->xpbd_bin_max_joints, 1) : 0;
-    const XpbdShape sh = xpbd_shape(m->xpbd_bin_max_bodies, 0);
+    // bins of up to 64 bodies on one wave, larger ones (at most 256 bodies) on 256 lanes, one body a lane
+    const XpbdShape sh = m->xpbd_bin_max_bodies <= 64 ? xpbd_shape(m->xpbd_bin_max_bodies, 0) : XpbdShape{256, 1, 1};
     if (xpbd_lds_bytes_shape(sh, m->xpbd_bin_max_bodies, jl, m->xpbd_contact_cap, !params->reference_arithmetic,
-                             false) <= 160 * 1024)
+                             false) <= 160 * 1024) {
+      // the worlds above 256 bodies first (the longest), the bins beside them on the side stream
+      if (m->xpbd_big_count > 0)
+        ls[nl++] = Launch{xpbd_shape(m->xpbd_big_max, 0), m->xpbd_big_count, m->xpbd_big_max, m->xpbd_big, nullptr,
+                          m->xpbd_joint_lds};
       ls[nl++] = Launch{sh, m->xpbd_bin_count, m->xpbd_bin_max_bodies, nullptr, m->xpbd_bins, jl};
+    }
   }
   if (nl == 0 && params->block_lanes != 64 && params->block_lanes != 256 && m->xpbd_tiered) {
     for (int t = 2; t >= 0; --t)  // largest worlds first

@@ -70,20 +70,28 @@ def test_block_lanes_bit_identical(cuda_device, name, make):
     assert res["block_lanes"] in (64, 256, 512) and solver.block_lanes == res["block_lanes"]
 
 
-def test_tiered_launch_bit_identical_uneven(cuda_device):
-    """Worlds of 8-262 bodies: launching by size tier (automatic) equals running every world on 256 lanes, bit
-    for bit; the one launch over CSR bins of consecutive worlds equals both bit for bit with the reference-order
-    arithmetic, and within the step bar with the world-frame arithmetic (its packed small worlds sum per body
-    on one lane instead of a tree of lanes)."""
+@pytest.mark.parametrize("big", [False, True], ids=["bins_256", "bins_and_large_worlds"])
+def test_tiered_launch_bit_identical_uneven(cuda_device, big):
+    """Worlds of 8-262 bodies (or, `big`, 60-306 with some above 256): launching by size tier (automatic) equals
+    running every world on 256 lanes, bit for bit; the binned launch (CSR bins of consecutive worlds on 256 lanes,
+    beside a 512-lane launch of the worlds above 256 bodies) equals both bit for bit with the reference-order
+    arithmetic, and within the step bar with the world-frame arithmetic (its packed worlds sum per body on one
+    lane instead of a tree of lanes)."""
     from parity_helpers import assert_step_close
 
-    print("[block lanes] uneven tiers and bins", flush=True)
-    model = scenes.uneven_worlds(12, seed=8, device=cuda_device)
+    print(f"[block lanes] uneven tiers and bins, big={big}", flush=True)
+    if big:
+        model = scenes.uneven_worlds(10, seed=5, lo=60, hi=300, device=cuda_device)
+    else:
+        model = scenes.uneven_worlds(12, seed=8, device=cuda_device)
     pipe = CollisionPipeline(model, broad_phase="sap")
     solver = SolverXPBD(model, iterations=10)
     assert candidate_block_lanes(solver) == [0, 256, 512]
     counts = model.host["world_body_count"]
-    assert counts.max() > 64 and (counts[:-1] <= 64).any()
+    if big:
+        assert (counts[:-1] > 256).any() and ((counts[:-1] > 64) & (counts[:-1] <= 256)).any()
+    else:
+        assert counts.max() > 64 and (counts[:-1] <= 64).any()
     c = pipe.contacts()
     s0, s1 = model.state(), model.state()
     for _ in range(60):
