@@ -549,6 +549,8 @@ __global__ void __launch_bounds__(SCAN_BLOCK) route_scan_kernel(const int* __res
 // Narrow phase over every staged pair, one pair per lane: MPR/GJK pairs first (the long ones), then
 // the analytic pairs, so that a wave's lanes mostly run the same collider.  Lane i finds its segment
 // by binary search over the route offsets.
+// ALL: the model holds ellipsoids, cylinders or cones (their colliders and support maps compiled in)
+template <bool ALL>
 __global__ void __launch_bounds__(CBLOCK) narrow_kernel(NmcModelDesc d, const float* __restrict__ body_q,
                                                         StageBuffers st, const long long* __restrict__ seg_pair_off,
                                                         const int* __restrict__ seg_conv_count,
@@ -572,7 +574,7 @@ __global__ void __launch_bounds__(CBLOCK) narrow_kernel(NmcModelDesc d, const fl
     ShapeW A = load_wc(st.shape_wc + 20ll * sa), B = load_wc(st.shape_wc + 20ll * sb);
     float gap_sum = d.shape_gap[sa] + d.shape_gap[sb];
     WriteCtx w{d.shape_gap, st, 5ll * g, 0};
-    narrow_pair(sa, sb, d.shape_type[sa], d.shape_type[sb], A, B, gap_sum, w);
+    narrow_pair<ALL>(sa, sb, d.shape_type[sa], d.shape_type[sb], A, B, gap_sum, w);
     st.mask[g] = static_cast<unsigned short>(w.n);
   }
 }
@@ -847,8 +849,12 @@ hipError_t launch_collide(NmcModelImpl* m, const float* body_q, const NmcContact
   long long nblk = (pairs + CBLOCK - 1) / CBLOCK;
   int narrow_blocks = static_cast<int>(nblk < m->narrow_max_blocks ? nblk : m->narrow_max_blocks);
   if (narrow_blocks < 1) narrow_blocks = 1;
-  hipLaunchKernelGGL(narrow_kernel, dim3(narrow_blocks), dim3(CBLOCK), 0, stream, d, body_q, m->stage,
-                     m->seg_pair_off, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, segs);
+  if (m->all_types)
+    hipLaunchKernelGGL(narrow_kernel<true>, dim3(narrow_blocks), dim3(CBLOCK), 0, stream, d, body_q, m->stage,
+                       m->seg_pair_off, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, segs);
+  else
+    hipLaunchKernelGGL(narrow_kernel<false>, dim3(narrow_blocks), dim3(CBLOCK), 0, stream, d, body_q, m->stage,
+                       m->seg_pair_off, m->seg_conv_count, m->seg_conv_off, m->seg_ana_off, segs);
   if (mesh) {
     hipError_t e = launch_mesh_narrow(m, body_q, stream);
     if (e != hipSuccess) return e;

@@ -49,6 +49,9 @@ struct WriteCtx {
   int n;  // records written so far
 };
 
+// ALL: the ellipsoid, cylinder and cone maps are compiled in (kernels of models without those shapes leave them
+// out, as the reference's lean support does, collide.py:743-759: fewer registers in the MPR/GJK loops)
+template <bool ALL = true>
 __device__ __forceinline__ f3 support(const GShape& g, f3 d) {
   if (g.type == GEO_BOX) {
     float thr = 1.0e-10f * len(d);
@@ -72,7 +75,7 @@ __device__ __forceinline__ f3 support(const GShape& g, f3 d) {
     float sy = d.y >= 0.0f ? 1.0f : -1.0f;
     return {sx * g.scale.x, sy * g.scale.y, 0.0f};
   }
-  if (g.type == GEO_ELLIPSOID) {
+  if (ALL && g.type == GEO_ELLIPSOID) {
     // (a^2 dx, b^2 dy, c^2 dz) / |(a dx, b dy, c dz)| (support_function.py:218-239)
     const float a = g.scale.x, b = g.scale.y, c = g.scale.z;
     if (len_sq(d) > 1.0e-12f) {
@@ -85,7 +88,7 @@ __device__ __forceinline__ f3 support(const GShape& g, f3 d) {
     }
     return {a, 0.0f, 0.0f};
   }
-  if (g.type == GEO_CYLINDER) {
+  if (ALL && g.type == GEO_CYLINDER) {
     // rim point in the direction's XY projection, on the cap the direction's z picks (support_function.py:240-260)
     const float r = g.scale.x, hh = g.scale.y;
     const f3 dxy{d.x, d.y, 0.0f};
@@ -98,7 +101,7 @@ __device__ __forceinline__ f3 support(const GShape& g, f3 d) {
     if (d.z < 0.0f) return {lat.x, lat.y, -hh};
     return lat;
   }
-  if (g.type == GEO_CONE) {
+  if (ALL && g.type == GEO_CONE) {
     // apex (0, 0, hh) when dz >= r / (2 hh) |d_xy|, else the base rim at z = -hh (support_function.py:262-285)
     const float r = g.scale.x, hh = g.scale.y;
     const f3 dxy{d.x, d.y, 0.0f};
@@ -358,7 +361,7 @@ __device__ __forceinline__ void plane_ellipsoid(f3 n, f3 pp, f3 ep, const m33& R
 }
 
 // collide_sphere_cylinder (collision_primitive.py:443-513): side, cap or rim, by where the centre projects.
-__device__ void sphere_cylinder(f3 sp, float sr, f3 cp, f3 axis, float cr, float chh, float& dist, f3& pos, f3& n) {
+__device__ __forceinline__ void sphere_cylinder(f3 sp, float sr, f3 cp, f3 axis, float cr, float chh, float& dist, f3& pos, f3& n) {
   const f3 vec = sp - cp;
   const float x = dot(vec, axis);
   const f3 a_proj = axis * x;
@@ -393,7 +396,7 @@ __device__ void sphere_cylinder(f3 sp, float sr, f3 cp, f3 axis, float cr, float
 // generators through the deepest radial direction and the near-cap rim point facing the plane; extra points
 // closer than 1 % of max(radius, half height) to the deepest one are dropped.  Kept points live in named
 // registers (no runtime-indexed array).
-__device__ void plane_cylinder(f3 n, f3 pp, f3 cp, f3 axis_in, float cr, float chh, float dist[4], f3 pos[4]) {
+__device__ __forceinline__ void plane_cylinder(f3 n, f3 pp, f3 cp, f3 axis_in, float cr, float chh, float dist[4], f3 pos[4]) {
   f3 axis = axis_in;
   float dna = dot(n, axis);
   if (dna > 0.0f) {
@@ -480,12 +483,13 @@ __device__ void sphere_box(f3 sp, float r, f3 bp, const m33& R, f3 size, float& 
 // ------------------------------------------------------------------------- MPR
 struct Vert { f3 B, A; };  // A = BtoA
 
+template <bool ALL>
 __device__ __forceinline__ Vert mink(const GShape& ga, const GShape& gb, f3 dir, qf ob, f3 pb, float extend) {
   Vert v;
-  f3 pa = support(ga, dir);
+  f3 pa = support<ALL>(ga, dir);
   f3 nd = -dir;
   f3 tmp = rot_inv(ob, nd);
-  f3 sb = rot(ob, support(gb, tmp));
+  f3 sb = rot(ob, support<ALL>(gb, tmp));
   v.B = sb + pb;
   if (extend != 0.0f) {
     f3 d = normalize(dir) * extend * 0.5f;
@@ -511,6 +515,7 @@ struct CvxRes {
   float value;
 };
 
+template <bool ALL>
 __device__ CvxRes mpr_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, float extend) {
   const int MAX_ITER = 30;
   const float CE = 1e-5f;
@@ -526,14 +531,14 @@ __device__ CvxRes mpr_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
     for (int ax = 0; ax < 3; ++ax) {
       f3 probe = zero;
       put(probe, ax, 1.0f);
-      Vert sv = mink(ga, gb, probe, ob, pb, extend);
+      Vert sv = mink<ALL>(ga, gb, probe, ob, pb, extend);
       float d = dot(sv.A, probe);
       if (d > best_dot) { best_dot = d; best_dir = probe; }
     }
     v0.A = best_dir * 1e-05f;
   }
   normal = -v0.A;
-  Vert v1 = mink(ga, gb, normal, ob, pb, extend);
+  Vert v1 = mink<ALL>(ga, gb, normal, ob, pb, extend);
   point_a = v1.B + v1.A;
   point_b = v1.B;
   if (dot(v1.A, normal) <= 0.0f) return {false, point_a, point_b, normal, pen};
@@ -544,7 +549,7 @@ __device__ CvxRes mpr_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
     pen = dot(v1.A, normal);
     return {true, point_a, point_b, normal, pen};
   }
-  Vert v2 = mink(ga, gb, normal, ob, pb, extend);
+  Vert v2 = mink<ALL>(ga, gb, normal, ob, pb, extend);
   if (dot(v2.A, normal) <= 0.0f) return {false, point_a, point_b, normal, pen};
   f3 t1 = v1.A - v0.A;
   f3 t2 = v2.A - v0.A;
@@ -562,7 +567,7 @@ __device__ CvxRes mpr_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
   while (true) {
     if (phase1 > MAX_ITER) return {false, point_a, point_b, normal, pen};
     phase1 += 1;
-    v3 = mink(ga, gb, normal, ob, pb, extend);
+    v3 = mink<ALL>(ga, gb, normal, ob, pb, extend);
     if (dot(v3.A, normal) <= 0.0f) return {false, point_a, point_b, normal, pen};
     t1 = cross(v1.A, v3.A);
     if (dot(t1, v0.A) < 0.0f) {
@@ -593,7 +598,7 @@ __device__ CvxRes mpr_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
       float d = dot(normal, v1.A);
       hit = d >= 0.0f;
     }
-    Vert v4 = mink(ga, gb, normal, ob, pb, extend);
+    Vert v4 = mink<ALL>(ga, gb, normal, ob, pb, extend);
     f3 t3 = v4.A - v3.A;
     float delta = dot(t3, normal);
     pen = dot(v4.A, normal);
@@ -788,6 +793,7 @@ __device__ __forceinline__ void smp_closest(const Simplex& s, f3& pa, f3& pb) {
   }
 }
 
+template <bool ALL>
 __device__ CvxRes gjk_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, float extend) {
   const int MAX_ITER = 30;
   const float CE = 1e-4f;
@@ -813,7 +819,7 @@ __device__ CvxRes gjk_core(const GShape& ga, const GShape& gb, qf ob, f3 pb, flo
     }
     f3 search = -v;
     last_search = search;
-    Vert w = mink(ga, gb, search, ob, pb, extend);
+    Vert w = mink<ALL>(ga, gb, search, ob, pb, extend);
     f3 wv = w.A;
     float delta_dist = dot(v, v - wv);
     if (delta_dist < CE * sqrtf(dist_sq)) break;
@@ -1163,9 +1169,11 @@ struct PairCtx {
 // atan2(r, 2 hh) from the normal plane to within 2 degrees, tested through sin(half angle -/+ 2 degrees)
 // = sin a cos 2 -/+ cos a sin 2 with sin a = r / L, cos a = 2 hh / L, L = |(r, 2 hh)| (the same thresholds,
 // without a trigonometric call).
+template <bool ALL>
 __device__ __forceinline__ CData post_process(CData c, const PairCtx& P) {
   const f3 normal = c.normal;
   c = post_process(c, P.ga.type, P.gb.type);
+  if (!ALL) return c;
   const int ta = P.ga.type, tb = P.gb.type;
   const bool da = ta == GEO_BOX || ta == GEO_PLANE, db = tb == GEO_BOX || tb == GEO_PLANE;
   const bool xa = ta == GEO_CYLINDER || ta == GEO_CONE, xb = tb == GEO_CYLINDER || tb == GEO_CONE;
@@ -1200,6 +1208,7 @@ static __constant__ float c_probe_cos[5] = {1.0f, (float)0.30901699437494745, (f
 static __constant__ float c_probe_sin[5] = {0.0f, (float)0.9510565162951535, (float)0.5877852522924732,
                                             (float)-0.587785252292473, (float)-0.9510565162951536};
 
+template <bool ALL>
 __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b, f3 normal, WriteCtx& w) {
   const float EPS = 0.00001f;
   const float SIN_TILT = (float)0.03489949670250097;
@@ -1222,14 +1231,14 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
     float ct = COS_TILT;
     float cs = c * SIN_TILT, ss = s * SIN_TILT;
     f3 dir_a = normal * ct + cs * ta + ss * tb;
-    f3 pt_a = support(P.ga, dir_a);
+    f3 pt_a = support<ALL>(P.ga, dir_a);
     f3 pr_a = pt_a - center;
     f2 a2{dot(ta, pr_a), dot(tb, pr_a)};
     bool added;
     a_count = add_nodup(buf + 5, a_count, a2, EPS, added);
     if (added) track(tra, pt_a, a_count - 1);
     f3 ldb = lnb * ct + cs * ltab + ss * ltbb;
-    f3 pbl = support(P.gb, ldb);
+    f3 pbl = support<ALL>(P.gb, ldb);
     f3 pt_b = rot(rel_ob, pbl) + rel_pb;
     f3 pr_b = pt_b - center;
     f2 b2{dot(ta, pr_b), dot(tb, pr_b)};
@@ -1273,7 +1282,7 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
           cd.normal = normal_world;
           cd.dist = sd;
           cd.sub_key = (P.tmpl.sub_key << 3) | i;
-          cd = post_process(cd, P);
+          cd = post_process<ALL>(cd, P);
           emit(cd, w, true);
         }
       } else {
@@ -1292,18 +1301,19 @@ __device__ void manifold(const PairCtx& P, qf rel_ob, f3 rel_pb, f3 p_a, f3 p_b,
     cd.normal = normal_world;
     cd.dist = dsd;
     cd.sub_key = (P.tmpl.sub_key << 3) | count_out;
-    cd = post_process(cd, P);
+    cd = post_process<ALL>(cd, P);
     emit(cd, w, true);
   }
 }
 
+template <bool ALL>
 __device__ void solve_convex(const PairCtx& P, float threshold, bool skip_multi, WriteCtx& w) {
   qf rel_ob = qmul(qinv(P.oa), P.ob_world);
   f3 rel_pb = rot_inv(P.oa, P.pb_world - P.pa_world);
   float margin_sum = P.tmpl.m_a + P.tmpl.m_b;
   const float eps = 1.0e-4f;
   float enlarge = margin_sum <= 0.0f ? eps : (margin_sum < eps ? 2.0f * eps : 0.0f);
-  CvxRes r = mpr_core(P.ga, P.gb, rel_ob, rel_pb, enlarge);
+  CvxRes r = mpr_core<ALL>(P.ga, P.gb, rel_ob, rel_pb, enlarge);
   f3 point_a, point_b, normal;
   float sd;
   if (r.flag) {
@@ -1313,7 +1323,7 @@ __device__ void solve_convex(const PairCtx& P, float threshold, bool skip_multi,
     point_b = r.pb + r.n * half;
     normal = r.n;
   } else {
-    CvxRes g = gjk_core(P.ga, P.gb, rel_ob, rel_pb, 0.0f);
+    CvxRes g = gjk_core<ALL>(P.ga, P.gb, rel_ob, rel_pb, 0.0f);
     point_a = g.pa;
     point_b = g.pb;
     normal = g.n;
@@ -1328,11 +1338,11 @@ __device__ void solve_convex(const PairCtx& P, float threshold, bool skip_multi,
     cd.normal = nws;
     cd.dist = sd;
     cd.sub_key = P.tmpl.sub_key << 3;
-    cd = post_process(cd, P);
+    cd = post_process<ALL>(cd, P);
     emit(cd, w, true);
     return;
   }
-  manifold(P, rel_ob, rel_pb, point_a, point_b, normal, w);
+  manifold<ALL>(P, rel_ob, rel_pb, point_a, point_b, normal, w);
 }
 
 // Type pairs with an analytic collider (narrow_phase.py:327-513); every other primitive pair goes through MPR/GJK
@@ -1353,6 +1363,7 @@ __device__ __forceinline__ int route_class(int ta, int tb) {
 }
 
 // One broad-phase pair (sa, sb already ordered by GeoType) -> up to 5 records at w.slot0.
+template <bool ALL>
 __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, const ShapeW& B, float gap_sum,
                             WriteCtx& w) {
   w.n = 0;
@@ -1371,7 +1382,7 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     f3 pn = rot(qa, ez);
     plane_sphere(pn, pos_a, pos_b, scale_b.x, cd[0], cp[0]);
     cn = pn;
-  } else if (plane_a && tb == GEO_ELLIPSOID) {
+  } else if (ALL && plane_a && tb == GEO_ELLIPSOID) {
     f3 pn = rot(qa, ez);
     plane_ellipsoid(pn, pos_a, pos_b, qmat(qb), scale_b, cd[0], cp[0]);
     cn = pn;
@@ -1389,7 +1400,7 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     plane_sphere(pn, pos_a, pos_b + seg, scale_b.x, cd[0], cp[0]);
     plane_sphere(pn, pos_a, pos_b - seg, scale_b.x, cd[1], cp[1]);
     cn = pn;
-  } else if (plane_a && tb == GEO_CYLINDER) {
+  } else if (ALL && plane_a && tb == GEO_CYLINDER) {
     f3 pn = rot(qa, ez);
     plane_cylinder(pn, pos_a, pos_b, rot(qb, ez), scale_b.x, scale_b.y, cd, cp);
     cn = pn;
@@ -1404,7 +1415,7 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
     f3 p2[2];
     capsule_capsule(pos_a, ax1, scale_a.x, scale_a.y, pos_b, ax2, scale_b.x, scale_b.y, d2, p2, cn);
     cd[0] = d2[0]; cp[0] = p2[0]; cd[1] = d2[1]; cp[1] = p2[1];
-  } else if (ta == GEO_SPHERE && tb == GEO_CYLINDER) {
+  } else if (ALL && ta == GEO_SPHERE && tb == GEO_CYLINDER) {
     sphere_cylinder(pos_a, scale_a.x, pos_b, rot(qb, ez), scale_b.x, scale_b.y, cd[0], cp[0], cn);
   } else if (ta == GEO_SPHERE && tb == GEO_BOX) {
     m33 R = qmat(qb);
@@ -1503,7 +1514,7 @@ __device__ void narrow_pair(int sa, int sb, int ta, int tb, const ShapeW& A, con
   float threshold = gap_sum + rea + reb + A.margin + B.margin;
   bool skip_multi = P.ga.type == GEO_SPHERE || P.gb.type == GEO_SPHERE || P.ga.type == GEO_ELLIPSOID ||
                     P.gb.type == GEO_ELLIPSOID;
-  solve_convex(P, threshold, skip_multi, w);
+  solve_convex<ALL>(P, threshold, skip_multi, w);
 }
 
 }  // namespace nmc

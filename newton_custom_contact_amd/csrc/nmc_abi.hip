@@ -183,6 +183,15 @@ extern "C" int nmc_model_create(const NmcModelDesc* desc, NmcModel** out) {
   // mesh_k contacts per pair, few bodies per world) size the LDS buffer for their worst world,
   // up to 8192 entries, so that each body's contact list is walked out of LDS.
   m->xpbd_contact_cap = 512;
+  if (d.shape_count > 0) {
+    // ellipsoids, cylinders or cones present: the narrow phase compiled with their colliders and support maps
+    std::vector<int> types(d.shape_count);
+    if ((e = hipMemcpy(types.data(), d.shape_type, sizeof(int) * d.shape_count, hipMemcpyDeviceToHost)) != hipSuccess) {
+      delete m;
+      return hip_fail(e, "nmc_model_create: shape types");
+    }
+    for (int t : types) m->all_types = m->all_types || t == GEO_ELLIPSOID || t == GEO_CYLINDER || t == GEO_CONE;
+  }
   if (d.mesh_count > 0 && segs > 1) {
     std::vector<int> starts(segs + 1), seg_shape, stype(d.shape_count > 0 ? d.shape_count : 1);
     if ((e = hipMemcpy(starts.data(), d.segment_start, sizeof(int) * (segs + 1), hipMemcpyDeviceToHost)) != hipSuccess) {

@@ -62,6 +62,7 @@ struct NmcModelImpl {
   // collide launch shape (nmc_model_set_launch): segments (one wave each) per workgroup of the small-segment
   // broad phase, grid caps of the grid-stride narrow and mesh kernels
   int segment_waves = 4;
+  bool all_types = false;  // the model holds ellipsoids, cylinders or cones (narrow phase with their colliders)
   int narrow_max_blocks = 2048;
   int mesh_max_blocks = 2048;
   // collide scratch
