@@ -311,7 +311,7 @@ def main():
     roof_ws = HBM_PEAK_GBS * 1e9 / b_ws
     roof = roofline_block(t_col_ms, t_xp_ms, args.worlds, b_collide, b_step,
                           traffic=lambda k: measured_traffic(args, k),
-                          traffic_at="rocprofv3 FETCH_SIZE + WRITE_SIZE passes at the settled state (N_c 123.4), "
+                          traffic_at="rocprofv3 FETCH_SIZE + WRITE_SIZE passes at the settled state (N_c 120.8), "
                                      "profiles/mixed4096_hbm_traffic.md")
 
     iterations = None
