@@ -4,7 +4,7 @@
   multipliers, per-world residual and the state are bit-identical to the restatement; the state equals
   the plain step;
 * warm step from a given lambda0: state, multipliers, residual and fallback flags bit-identical;
-  lambda0 = 0 with warm_iterations = iterations is the plain step;
+  lambda0 = 0 with warm_iterations = iterations is the plain step, also with the restitution pass on;
 * network: the MFMA forward within 1e-2 of the output scale of the restatement (BF16 operands, FP32
   accumulation in a different order), gradients within 2e-2 relative (Frobenius) of the restatement;
 * the committed trained network: warm start + k iterations leaves less penetration than k plain
@@ -221,3 +221,31 @@ def test_trained_network_gains_over_plain(cuda_device):
     # the warm start is worth at least one more plain iteration at every k
     for k in (1, 2, 3, 4):
         assert ev["warm_residual"][k] <= ev["plain_residual"][k + 1], k
+
+
+@pytest.mark.parametrize("reference_arithmetic", [True, False], ids=["reference_order", "world_frame"])
+def test_warm_step_applies_restitution(cuda_device, reference_arithmetic):
+    """The warm launch runs the restitution pass on the start-of-step state: with lambda0 = 0 and
+    warm_iterations = iterations it equals the plain step with the pass, bit for bit, and the pass changes
+    the velocities of this state (so the identity does not hold by the pass doing nothing)."""
+    print(f"[learned] warm step with restitution, reference_arithmetic={reference_arithmetic}", flush=True)
+    model, o, q, qd = _settled(cuda_device, seed=5)
+    e = np.random.default_rng(2).uniform(0.2, 0.9, model.shape_count).astype(np.float32)
+    model.host["shape_material_restitution"] = e
+    model.shape_material_restitution.copy_(torch.from_numpy(e))
+    # a downward kick, so contacts approach at the start of the step and the pass has work to do
+    qd = qd.copy()
+    qd[:, 2] -= 0.5
+    pipe = CollisionPipeline(model, broad_phase="sap", deterministic=True)
+    c = pipe.contacts()
+    s_in, s_out, s_plain, s_norest = _state(model, q, qd), model.state(), model.state(), model.state()
+    pipe.collide(s_in, c)
+    kw = dict(iterations=10, reference_arithmetic=reference_arithmetic)
+    SolverXPBD(model, enable_restitution=True, **kw).step(s_in, s_plain, None, c, DT)
+    SolverXPBD(model, **kw).step(s_in, s_norest, None, c, DT)
+    warm = SolverXPBDWarmStart(model, WarmStartMLP(device=cuda_device), enable_restitution=True, **kw)
+    zeros = np.zeros(c.rigid_contact_max, np.float32)
+    _warm(warm, s_in, s_out, c, zeros, 10, 1e30)
+    assert not np.array_equal(s_plain.body_qd.cpu().numpy(), s_norest.body_qd.cpu().numpy())
+    assert np.array_equal(s_out.body_q.cpu().numpy(), s_plain.body_q.cpu().numpy())
+    assert np.array_equal(s_out.body_qd.cpu().numpy(), s_plain.body_qd.cpu().numpy())
