@@ -5,7 +5,7 @@
   the step bar (1e-5 relative; 1e-5 m / 1e-5 m per substep floors).  Scenes: 8 worlds of 32 primitives (one wave
   a world), two worlds of 280-400 bodies (the pass walks its chunks once per group of 256 bodies), and worlds
   with a kinematic platform that moving bodies strike (zero effective inverse mass: the platform's velocity
-  passes through unchanged).
+  passes through unchanged), and worlds of cylinders, cones, ellipsoids, spheres, boxes and capsules.
 * The reporting step (contact forces requested) applies the same pass: the same state bit for bit.
 * The reference's known answer (test_physics_validation.py:644-722): a sphere dropped from 1 m rebounds to
   e^2 m within 1 % for e = 0.5 and 0.8, the ratio of the two within 1 % of 2.56; both arithmetic modes.
@@ -25,6 +25,7 @@ from newton_custom_contact_amd.builder import ModelBuilder
 from newton_custom_contact_amd.solvers import SolverXPBD
 from oracle import Oracle
 from parity_helpers import assert_step_close, bit_equal, max_abs_diff
+from test_gpu_shapes import _shape_worlds
 
 pytestmark = pytest.mark.gpu
 
@@ -71,6 +72,7 @@ SCENES = [
     ("mixed_8w", lambda dev: _set_restitution(scenes.mixed_worlds(8, seed=3, device=dev), 1)),
     ("uneven_2w_large", lambda dev: _set_restitution(scenes.uneven_worlds(2, seed=4, lo=280, hi=400, device=dev), 2)),
     ("kinematic_platform_6w", _platform_worlds),
+    ("shapes_6w", lambda dev: _set_restitution(_shape_worlds(dev), 3)),
 ]
 
 
